@@ -1,0 +1,166 @@
+// Internal declarations of libepsx: the context, error reporting, device buffers and the
+// kernels shared between translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "epsx.h"
+
+namespace epsx {
+
+// A device buffer owned by the context (freed with it or on reallocation).
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+// Device-resident scGD0 problem (see epsx_scgd0_set_problem).
+struct Scgd0Dev {
+    bool ready = false;
+    int64_t nm = 0, ni = 0, nf = 0, nq = 0, nki = 0, nkf = 0, nw = 0, nwa = 0, n_mult = 0;
+    double W = 0, nstates_base = 0;
+    // grid of ωs: relative integer offsets (from offsets[0]) and the node values
+    double w_origin = 0, dw_unit = 0, dw_min = 0;
+    int64_t span = 0;                 // offsets[n-1] - offsets[0]
+    std::vector<double> ws_host, wsa_host;
+    bool einstein = false;            // ωph[ν, :] constant for every ν
+    bool star_ok = false;             // e_f == e_{f_to_i[f]} bitwise
+    std::vector<double> wnu_host;     // [nm] mode frequency when einstein
+    int64_t nstar = 0;                // number of outer states that have inner states
+    // device arrays
+    DevBuf g2p;        // [nm, nf (star-sorted), ni]  FP64
+    DevBuf g2p32;      // same, FP32 (built on demand)
+    DevBuf wph, iq_kk, iks_i, iks_f;  // phonon table and q map
+    DevBuf ef_s, wf_s, fi_s, fsrc_s;  // per sorted inner state: ε, w, i_f (0-based), source f
+    DevBuf e_i, w_i;
+    DevBuf ws, off, rgap, ltab;       // output grid: nodes, rel. offsets (int32), 1/gap, cell table
+    DevBuf wsa, Wa, jsp, tsp;         // spectral grid: nodes, trapezoid weights, table from ωs
+    DevBuf KK;                        // [nw, nw]
+    DevBuf mult_ptr, mult_idx;
+    DevBuf star_ptr;                  // [ni + 1] sorted-position ranges of each star
+    DevBuf Gt;                        // star-collapsed coupling (built on demand)
+    int64_t ltab_len = 0;
+    bool gt_ready = false;
+};
+
+}  // namespace epsx
+
+struct epsx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    epsx::Scgd0Dev s;
+    std::vector<epsx::DevBuf *> owned;
+    double t_class[4] = {0, 0, 0, 0};
+    int64_t n_class[4] = {0, 0, 0, 0};
+};
+
+namespace epsx {
+
+int set_error(epsx_ctx *ctx, int code, const char *fmt, ...);
+int hip_fail(epsx_ctx *ctx, hipError_t e, const char *what, const char *file, int line);
+int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes);   // context-owned, freed at close
+void dev_free(DevBuf &b);
+int tmp_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes);   // caller-owned (see Tmp)
+
+// A scoped device temporary.
+struct Tmp {
+    DevBuf b;
+    Tmp() = default;
+    Tmp(const Tmp &) = delete;
+    Tmp &operator=(const Tmp &) = delete;
+    ~Tmp() { dev_free(b); }
+    template <class T> T *ptr() { return static_cast<T *>(b.p); }
+};
+int tmp_upload(epsx_ctx *ctx, Tmp &t, const void *host, size_t bytes);
+
+#define EPSX_HIP(ctx, call)                                                    \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) return epsx::hip_fail((ctx), e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define EPSX_TRY(call)              \
+    do {                            \
+        int r_ = (call);            \
+        if (r_ != EPSX_OK) return r_; \
+    } while (0)
+
+template <class T>
+inline T *dptr(DevBuf &b) { return static_cast<T *>(b.p); }
+
+// Upload a host array into a (re)allocated device buffer.
+int upload(epsx_ctx *ctx, DevBuf &b, const void *host, size_t bytes);
+
+// Host-side grid maps (epsx_grids.cpp), shared by the C ABI and the loop set-up.
+int64_t cell_search_table(int64_t n, const double *ws, double dw_min, double *x0,
+                          double *inv_h, int32_t *jtab, int64_t cap);
+void linear_interpolation_table(int64_t n, const double *ws, int64_t m, const double *xs,
+                                int32_t *j, double *t);
+void trapezoid_weights(int64_t n, const double *ws, double *W);
+void grid_nodes(int64_t n, double origin, double unit, const int64_t *offsets, double *ws);
+
+// Kernel launchers (epsx_kk.hip)
+int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K);
+// C[m, ncol] = A[m, k] * B[k, ncol] (+ C if beta != 0), FP64, column-major, on ctx->stream
+int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A, int64_t lda,
+             const double *B, int64_t ldb, double *C, int64_t ldc, bool accumulate);
+
+// Grid data the Fan-Migdal kernels read (device pointers).
+struct FmGrid {
+    int64_t nw = 0, span = 0, ltab_len = 0;
+    double dw_unit = 0;
+    const double *ws = nullptr;     // nodes
+    const int32_t *off = nullptr;   // relative offsets
+    const double *rgap = nullptr;   // 1 / (off[k+1] - off[k])
+    const int32_t *ltab = nullptr;  // cell of each relative offset in [0, span]
+};
+
+// Inputs of one direct Fan-Migdal launch (device pointers).
+struct FmDirectArgs {
+    int64_t nm, ni, nfs;            // modes, outer states, sorted inner states
+    const double *g2p;              // [nm, nfs, ni] coupling, inner states star-sorted
+    const float *g2p32;             // FP32 copy (precision 1) or null
+    const double *ef, *wf;          // [nfs] per sorted inner state
+    const int32_t *fi;              // [nfs] 0-based column of Sin (IBZ representative)
+    const int32_t *fsrc;            // [nfs] 0-based inner state (for the q gather)
+    // phonon frequency of (ν, i, f): wq_pair[ν + nm (i + ni f)] if non-null, else
+    // wph[ν + nm (iq_kk[(iks_i[i]-1) + nki (iks_f[f]-1)] - 1)]
+    const double *wq_pair;
+    const double *wph;
+    const int32_t *iq_kk, *iks_i, *iks_f;
+    int64_t nki;
+    const double *S;                // complex [nw, ncol, nT] unclamped self-energy
+    int64_t ncol;
+    double w_acoustic;
+};
+
+// The temperatures of one launch: slot z of the launch runs temperature index tidx[z]; T, mu
+// and eta_min are indexed by the temperature index.
+struct FmTemps {
+    int nT;                         // launched slots
+    const double *T;                // device [nT_all]
+    const double *mu;               // device [nT_all]
+    const double *eta_min;          // device [nT_all] (clamp); may be null (no clamp)
+    const int32_t *tidx;            // device [nT] temperature index of each slot
+};
+
+// One direct Fan-Migdal pass over the launched temperatures: ImS[:, :, tidx[z]] (+)= the sum.
+int fm_direct_launch_sp(epsx_ctx *ctx, const FmGrid &g, const FmDirectArgs &a, const FmTemps &tp,
+                        const int64_t *d_star_ptr, int64_t f_tile, int precision, double *d_part,
+                        int64_t part_cap, double *d_ImS, bool accumulate);
+size_t fm_part_elems(int64_t nfs, int64_t ni, int64_t nw, int64_t nT, int64_t f_tile);
+int fm_grid_host(int64_t nw, const int64_t *offsets, std::vector<int32_t> &off,
+                 std::vector<double> &rgap, std::vector<int32_t> &ltab, int64_t &span);
+void star_order(int64_t nf, int64_t ncol, const int32_t *f_to_i, std::vector<int32_t> &fsrc,
+                std::vector<int64_t> &star_ptr);
+
+}  // namespace epsx

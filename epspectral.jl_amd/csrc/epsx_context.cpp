@@ -1,0 +1,127 @@
+// Context, errors and device buffers of libepsx.
+#include <cstdarg>
+#include <cstring>
+
+#include "epsx_common.h"
+
+namespace epsx {
+
+int set_error(epsx_ctx *ctx, int code, const char *fmt, ...)
+{
+    if (!ctx) return code;
+    char buf[2048];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+    return code;
+}
+
+int hip_fail(epsx_ctx *ctx, hipError_t e, const char *what, const char *file, int line)
+{
+    int code = (e == hipErrorOutOfMemory) ? EPSX_ERR_OOM : EPSX_ERR_DEVICE;
+    return set_error(ctx, code, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e),
+                     what, file, line);
+}
+
+void dev_free(DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes)
+{
+    if (b.p && b.bytes >= bytes) return EPSX_OK;
+    dev_free(b);
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return set_error(ctx, EPSX_ERR_OOM, "device allocation of %zu bytes failed: %s", bytes,
+                         hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    bool seen = false;
+    for (auto *q : ctx->owned) seen |= (q == &b);
+    if (!seen) ctx->owned.push_back(&b);
+    return EPSX_OK;
+}
+
+int tmp_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes)
+{
+    dev_free(b);
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return set_error(ctx, EPSX_ERR_OOM, "device allocation of %zu bytes failed: %s", bytes,
+                         hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return EPSX_OK;
+}
+
+int tmp_upload(epsx_ctx *ctx, Tmp &t, const void *host, size_t bytes)
+{
+    EPSX_TRY(tmp_alloc(ctx, t.b, bytes));
+    if (bytes) EPSX_HIP(ctx, hipMemcpyAsync(t.b.p, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return EPSX_OK;
+}
+
+int upload(epsx_ctx *ctx, DevBuf &b, const void *host, size_t bytes)
+{
+    EPSX_TRY(dev_alloc(ctx, b, bytes));
+    if (bytes) EPSX_HIP(ctx, hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return EPSX_OK;
+}
+
+}  // namespace epsx
+
+extern "C" {
+
+int epsx_abi_version(void) { return EPSX_ABI_VERSION; }
+
+int epsx_open(int device, epsx_ctx **out)
+{
+    if (!out) return EPSX_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return EPSX_ERR_DEVICE;
+    if (device < 0 || device >= n) return EPSX_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return EPSX_ERR_DEVICE;
+    auto *ctx = new epsx_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return EPSX_ERR_DEVICE;
+    }
+    *out = ctx;
+    return EPSX_OK;
+}
+
+void epsx_close(epsx_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto *b : ctx->owned) epsx::dev_free(*b);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *epsx_last_error(const epsx_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int epsx_last_timings(const epsx_ctx *ctx, double *seconds4, int64_t *launches4)
+{
+    if (!ctx) return EPSX_ERR_ARG;
+    for (int k = 0; k < 4; k++) {
+        if (seconds4) seconds4[k] = ctx->t_class[k];
+        if (launches4) launches4[k] = ctx->n_class[k];
+    }
+    return EPSX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,675 @@
+// The scGD0 iteration on the device (src/scgd0.jl:189-346): set-up of the resident problem and
+// the loop of Fan-Migdal sum, Kramers-Kronig, multiplet average, spectral DOS, μ solve and the
+// convergence test.  Everything stays on the device; per iteration the host reads back only
+// err, μ, min ∫A and a status word per temperature.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "epsx_common.h"
+
+namespace epsx {
+namespace {
+
+constexpr double PI_D = 3.141592653589793;
+constexpr double SQRT_EPS = 1.4901161193847656e-08;
+
+__device__ inline double occ_fermion_d(double e, double T)
+{
+    if (T > SQRT_EPS) return 1.0 / (exp(e / T) + 1.0);
+    const double s = e > 0 ? 1.0 : (e < 0 ? -1.0 : 0.0);
+    return (1.0 - s) / 2.0;
+}
+
+// g2p[ν, p, i] = g2[ν, i, fsrc[p]]: the inner states in star order, contiguous per outer state.
+__global__ void permute_g2_kernel(int64_t nm, int64_t ni, int64_t nf, const double *__restrict__ g2,
+                                  const int32_t *__restrict__ fsrc, double *__restrict__ g2p)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nm * nf * ni) return;
+    const int64_t nu = q % nm, p = (q / nm) % nf, i = q / (nm * nf);
+    g2p[q] = g2[nu + nm * (i + ni * (int64_t)fsrc[p])];
+}
+
+__global__ void fill_sigma_kernel(int64_t n, double2 *S, double eta_init)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < n) S[q] = make_double2(0.0, -eta_init);
+}
+
+// Σ_new[:, :, t] = complex(ReΣ, ImΣ[:, :, t])  (src/scgd0.jl:291)
+__global__ void combine_kernel(int64_t n, const double *__restrict__ Re, const double *__restrict__ Im,
+                               double2 *__restrict__ Snew)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < n) Snew[q] = make_double2(Re[q], Im[q]);
+}
+
+// Multiplet average (src/scgd0.jl:294-300): members summed in order, divided by the count,
+// written to every member (so members are bitwise equal).
+__global__ void multiplet_kernel(int64_t nw, int64_t ni, int64_t n_mult, const int64_t *__restrict__ ptr,
+                                 const int32_t *__restrict__ idx, const int32_t *__restrict__ tidx,
+                                 double2 *__restrict__ Snew)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nw * n_mult) return;
+    const int64_t w = q % nw, m = q / nw;
+    double2 *St = Snew + nw * ni * (int64_t)tidx[blockIdx.y];
+    const int64_t a = ptr[m], b = ptr[m + 1];
+    double sr = 0, si = 0;
+    for (int64_t k = a; k < b; k++) {
+        const double2 v = St[w + nw * (int64_t)(idx[k] - 1)];
+        sr += v.x;
+        si += v.y;
+    }
+    const double cnt = (double)(b - a);
+    sr /= cnt;
+    si /= cnt;
+    for (int64_t k = a; k < b; k++) St[w + nw * (int64_t)(idx[k] - 1)] = make_double2(sr, si);
+}
+
+// A[a, i, z] = -Im(inv(ωa - ε_i - Σ(ωa)))/π with Σ gathered by the spectral table
+// (src/scgd0.jl:304-305); the complex inverse is Julia's scaled (Smith) form.
+__global__ void dos_A_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *__restrict__ wsa,
+                             const double *__restrict__ e_i, const int32_t *__restrict__ jsp,
+                             const double *__restrict__ tsp, const int32_t *__restrict__ tidx,
+                             const double2 *__restrict__ Snew, double *__restrict__ A)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nwa * ni) return;
+    const int z = blockIdx.y;
+    const int64_t a = q % nwa, i = q / nwa;
+    const double2 *St = Snew + nw * ni * (int64_t)tidx[z] + nw * i;
+    const int64_t j = jsp[a] - 1;
+    const double t = tsp[a];
+    const double2 lo = St[j], hi = St[j + 1];
+    const double sr = (1 - t) * lo.x + t * hi.x, si = (1 - t) * lo.y + t * hi.y;
+    const double c = wsa[a] - e_i[i] - sr, d = -si;
+    double im;
+    if (fabs(d) <= fabs(c)) {
+        const double r = d / c, zz = fma(d, r, c);
+        im = -r / zz;
+    } else {
+        const double r = c / d, zz = fma(-c, r, -d);
+        im = 1.0 / zz;
+    }
+    A[q + nwa * ni * (int64_t)z] = -im / PI_D;
+}
+
+// D[a, z] = (Σ_i A[a, i] w_i) Wω[a]  (src/scgd0.jl:306-307), i ascending
+__global__ void dos_D_kernel(int64_t ni, int64_t nwa, const double *__restrict__ A,
+                             const double *__restrict__ w_i, const double *__restrict__ Wa,
+                             double *__restrict__ D)
+{
+    const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (a >= nwa) return;
+    const double *Az = A + nwa * ni * (int64_t)z;
+    double acc = 0;
+    for (int64_t i = 0; i < ni; i++) acc = fma(Az[a + nwa * i], w_i[i], acc);
+    D[a + nwa * (int64_t)z] = acc * Wa[a];
+}
+
+template <int NT>
+__device__ inline double block_sum(double v, double *red)
+{
+    const int tid = threadIdx.x;
+    red[tid] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// ∫A_i = Σ_a A[a, i] Wω[a] for one outer state per block (src/scgd0.jl:308)
+__global__ __launch_bounds__(256) void dos_int_kernel(int64_t ni, int64_t nwa, const double *__restrict__ A,
+                                                      const double *__restrict__ Wa, double *__restrict__ integ)
+{
+    __shared__ double red[256];
+    const int64_t i = blockIdx.x;
+    const int z = blockIdx.y;
+    const double *Ai = A + nwa * ni * (int64_t)z + nwa * i;
+    double acc = 0;
+    for (int64_t a = threadIdx.x; a < nwa; a += 256) acc = fma(Ai[a], Wa[a], acc);
+    const double s = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) integ[i + ni * (int64_t)z] = s;
+}
+
+__global__ __launch_bounds__(256) void min_kernel(int64_t ni, const double *__restrict__ integ,
+                                                  const int32_t *__restrict__ tidx, double *__restrict__ minA)
+{
+    __shared__ double red[256];
+    const int z = blockIdx.x;
+    double m = INFINITY;
+    for (int64_t i = threadIdx.x; i < ni; i += 256) {
+        const double v = integ[i + ni * (int64_t)z];
+        m = (v < m || v != v) ? v : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            if (o < red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) minA[tidx[z]] = red[0];
+}
+
+// Ordered-bit midpoint of a bracket (Float64 bisection as in Roots.jl), 0 across a sign change.
+__device__ inline double bits_middle(double a, double b)
+{
+    if ((a < 0) != (b < 0) && a != 0 && b != 0) return 0.0;
+    if (a == 0 || b == 0) {
+        const double o = a == 0 ? b : a;
+        const unsigned long long uo = (unsigned long long)__double_as_longlong(fabs(o));
+        const double r = __longlong_as_double((long long)(uo / 2));
+        return o < 0 ? -r : r;
+    }
+    const unsigned long long ua = (unsigned long long)__double_as_longlong(fabs(a));
+    const unsigned long long ub = (unsigned long long)__double_as_longlong(fabs(b));
+    const unsigned long long m = ua / 2 + ub / 2 + (ua & ub & 1ULL);
+    const double r = __longlong_as_double((long long)m);
+    return a < 0 ? -r : r;
+}
+
+struct MuArgs {
+    int64_t nwa;
+    const double *wsa, *D;   // D [nwa, nz]
+    const int32_t *tidx;
+    const double *T, *nlist; // per temperature index
+    double nelec, spin, W, nsb;
+    int is_semi;
+    double *mu;              // per temperature index
+    int32_t *status;         // per temperature index
+};
+
+// The μ root of n(μ) = n_target on the bracket extrema(ωs_spectral) ± 0.1 (src/scgd0.jl:310-324,
+// :361-366), one block per temperature: every n(μ) is a fixed-order block reduction, and the
+// bracket is bisected on the ordered bit pattern until two adjacent floats hold the sign change.
+__global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
+{
+    __shared__ double red[1024];
+    __shared__ double br[4];  // a, b, fa, fb
+    __shared__ int done;
+    const int z = blockIdx.x;
+    const int tcol = m.tidx[z];
+    const double T = m.T[tcol];
+    const double n_target = m.nlist[tcol] + m.nelec;
+    const int holes = m.is_semi && m.nlist[tcol] < 0;
+    const double *D = m.D + m.nwa * (int64_t)z;
+    auto excess = [&](double mu) -> double {
+        double acc = 0;
+        for (int64_t a = threadIdx.x; a < m.nwa; a += 1024) {
+            const double x = holes ? mu - m.wsa[a] : m.wsa[a] - mu;
+            acc = fma(D[a], occ_fermion_d(x, T), acc);
+        }
+        double s = block_sum<1024>(acc, red);
+        if (holes) s = m.W - s;
+        return (s + m.nsb) * m.spin - n_target;
+    };
+    const double a0 = m.wsa[0] - 0.1, b0 = m.wsa[m.nwa - 1] + 0.1;
+    const double fa0 = excess(a0), fb0 = excess(b0);
+    if (threadIdx.x == 0) {
+        br[0] = a0; br[1] = b0; br[2] = fa0; br[3] = fb0;
+        done = 0;
+        if (fa0 * fb0 > 0) { m.status[tcol] = 2; done = 1; }
+        else {
+            m.status[tcol] = 0;
+            if (fa0 == 0) { m.mu[tcol] = a0; done = 1; }
+            else if (fb0 == 0) { m.mu[tcol] = b0; done = 1; }
+        }
+    }
+    __syncthreads();
+    if (done) return;
+    for (int it = 0; it < 200; it++) {
+        const double a = br[0], b = br[1];
+        const double mid = bits_middle(a, b);
+        if (mid == a || mid == b) break;
+        const double fm = excess(mid);
+        if (threadIdx.x == 0) {
+            if (fm == 0) { br[0] = br[1] = mid; br[2] = br[3] = 0; }
+            else if ((fm > 0) == (br[2] > 0)) { br[0] = mid; br[2] = fm; }
+            else { br[1] = mid; br[3] = fm; }
+        }
+        __syncthreads();
+        if (br[0] == br[1]) break;
+    }
+    if (threadIdx.x == 0) m.mu[tcol] = fabs(br[2]) <= fabs(br[3]) ? br[0] : br[1];
+}
+
+// err = max |Σ_new - Σ| over the slice, then Σ ← Σ_new (src/scgd0.jl:326-328): per-block maxima,
+// reduced by err_final_kernel.
+__global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t *__restrict__ tidx,
+                                                       const double2 *__restrict__ Snew, double2 *__restrict__ S,
+                                                       double *__restrict__ bmax)
+{
+    __shared__ double red[256];
+    const int z = blockIdx.y;
+    const int64_t base = n * (int64_t)tidx[z];
+    double m = 0;
+    for (int64_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) {
+        const double2 a = Snew[base + q], b = S[base + q];
+        const double e = hypot(a.x - b.x, a.y - b.y);
+        m = (e > m || e != e) ? e : m;
+        S[base + q] = a;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bmax[blockIdx.x + gridDim.x * z] = red[0];
+}
+
+__global__ void err_final_kernel(int nb, const double *__restrict__ bmax, const int32_t *__restrict__ tidx,
+                                 double *__restrict__ err)
+{
+    const int z = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    double m = 0;
+    for (int b = 0; b < nb; b++) {
+        const double o = bmax[b + nb * z];
+        m = (o > m || o != o) ? o : m;
+    }
+    err[tidx[z]] = m;
+}
+
+inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+}  // namespace epsx
+
+using namespace epsx;
+
+extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P)
+{
+    if (!ctx) return EPSX_ERR_ARG;
+    if (!P || P->struct_size != (int64_t)sizeof(epsx_scgd0_problem))
+        return set_error(ctx, EPSX_ERR_ARG, "epsx_scgd0_problem: struct size mismatch (ABI %d)",
+                         EPSX_ABI_VERSION);
+    Scgd0Dev &s = ctx->s;
+    s.ready = false;
+    const int64_t nm = P->nm, ni = P->ni, nf = P->nf, nw = P->nw, nwa = P->nwa;
+    if (nm < 1 || ni < 1 || nf < 1 || nw < 2 || nwa < 2 || P->nq < 1 || P->nki < 1 || P->nkf < 1)
+        return set_error(ctx, EPSX_ERR_ARG, "scgd0 problem: empty dimension (nm=%lld ni=%lld nf=%lld nw=%lld nwa=%lld)",
+                         (long long)nm, (long long)ni, (long long)nf, (long long)nw, (long long)nwa);
+    if (!P->g2 || !P->wph || !P->iq_kk || !P->iks_i || !P->iks_f || !P->e_i || !P->w_i || !P->e_f ||
+        !P->w_f || !P->f_to_i || !P->w_offsets || !P->wa_offsets)
+        return set_error(ctx, EPSX_ERR_ARG, "scgd0 problem: null array");
+    if (!(std::isfinite(P->w_origin) && std::isfinite(P->dw_unit) && P->dw_unit > 0) ||
+        !(std::isfinite(P->wa_origin) && std::isfinite(P->dwa_unit) && P->dwa_unit > 0))
+        return set_error(ctx, EPSX_ERR_ARG, "ω_origin must be finite and dω_unit finite and positive");
+    int64_t gmin = INT64_MAX;
+    for (int64_t k = 0; k + 1 < nw; k++) {
+        const int64_t gp = P->w_offsets[k + 1] - P->w_offsets[k];
+        if (gp <= 0) return set_error(ctx, EPSX_ERR_ARG, "offsets must be strictly increasing, got offsets[%lld] >= offsets[%lld]",
+                                      (long long)k + 1, (long long)k + 2);
+        gmin = gp < gmin ? gp : gmin;
+    }
+    for (int64_t k = 0; k + 1 < nwa; k++)
+        if (P->wa_offsets[k + 1] <= P->wa_offsets[k])
+            return set_error(ctx, EPSX_ERR_ARG, "ωs_spectral offsets must be strictly increasing");
+    for (int64_t f = 0; f < nf; f++) {
+        if (P->f_to_i[f] < 1 || P->f_to_i[f] > ni)
+            return set_error(ctx, EPSX_ERR_ARG, "f_to_i[%lld] = %d is outside 1:%lld", (long long)f + 1, P->f_to_i[f], (long long)ni);
+        if (P->iks_f[f] < 1 || P->iks_f[f] > P->nkf)
+            return set_error(ctx, EPSX_ERR_ARG, "iks_f[%lld] = %d is outside 1:%lld", (long long)f + 1, P->iks_f[f], (long long)P->nkf);
+    }
+    for (int64_t i = 0; i < ni; i++)
+        if (P->iks_i[i] < 1 || P->iks_i[i] > P->nki)
+            return set_error(ctx, EPSX_ERR_ARG, "iks_i[%lld] = %d is outside 1:%lld", (long long)i + 1, P->iks_i[i], (long long)P->nki);
+    for (int64_t q = 0; q < P->nki * P->nkf; q++)
+        if (P->iq_kk[q] < 1 || P->iq_kk[q] > P->nq)
+            return set_error(ctx, EPSX_ERR_ARG, "iq_kk holds %d, outside 1:%lld", P->iq_kk[q], (long long)P->nq);
+    if (P->n_mult < 0 || (P->n_mult > 0 && (!P->mult_ptr || !P->mult_idx)))
+        return set_error(ctx, EPSX_ERR_ARG, "multiplets: bad CSR");
+    for (int64_t m = 0; m < P->n_mult; m++) {
+        if (P->mult_ptr[m + 1] < P->mult_ptr[m]) return set_error(ctx, EPSX_ERR_ARG, "multiplets: bad CSR");
+        for (int64_t k = P->mult_ptr[m]; k < P->mult_ptr[m + 1]; k++)
+            if (P->mult_idx[k] < 1 || P->mult_idx[k] > ni)
+                return set_error(ctx, EPSX_ERR_ARG, "multiplet member %d is outside 1:%lld", P->mult_idx[k], (long long)ni);
+    }
+    EPSX_HIP(ctx, hipSetDevice(ctx->device));
+
+    s.nm = nm; s.ni = ni; s.nf = nf; s.nq = P->nq; s.nki = P->nki; s.nkf = P->nkf;
+    s.nw = nw; s.nwa = nwa; s.n_mult = P->n_mult;
+    s.W = P->W; s.nstates_base = P->nstates_base;
+    s.w_origin = P->w_origin; s.dw_unit = P->dw_unit;
+    s.dw_min = P->dw_unit * (double)gmin;
+    s.ws_host.resize(nw);
+    s.wsa_host.resize(nwa);
+    grid_nodes(nw, P->w_origin, P->dw_unit, P->w_offsets, s.ws_host.data());
+    grid_nodes(nwa, P->wa_origin, P->dwa_unit, P->wa_offsets, s.wsa_host.data());
+
+    // Fan-Migdal grid data
+    std::vector<int32_t> off, ltab;
+    std::vector<double> rgap;
+    if (fm_grid_host(nw, P->w_offsets, off, rgap, ltab, s.span))
+        return set_error(ctx, EPSX_ERR_ARG, "ωs spans more than 2^30 steps of dω_unit");
+    s.ltab_len = (int64_t)ltab.size();
+    EPSX_TRY(upload(ctx, s.ws, s.ws_host.data(), sizeof(double) * nw));
+    EPSX_TRY(upload(ctx, s.off, off.data(), sizeof(int32_t) * nw));
+    EPSX_TRY(upload(ctx, s.rgap, rgap.data(), sizeof(double) * nw));
+    EPSX_TRY(upload(ctx, s.ltab, ltab.data(), sizeof(int32_t) * ltab.size()));
+    // spectral grid, its table from ωs and its trapezoid weights (src/scgd0.jl:163, :181-182)
+    std::vector<int32_t> jsp(nwa);
+    std::vector<double> tsp(nwa), Wa(nwa);
+    linear_interpolation_table(nw, s.ws_host.data(), nwa, s.wsa_host.data(), jsp.data(), tsp.data());
+    trapezoid_weights(nwa, s.wsa_host.data(), Wa.data());
+    EPSX_TRY(upload(ctx, s.wsa, s.wsa_host.data(), sizeof(double) * nwa));
+    EPSX_TRY(upload(ctx, s.Wa, Wa.data(), sizeof(double) * nwa));
+    EPSX_TRY(upload(ctx, s.jsp, jsp.data(), sizeof(int32_t) * nwa));
+    EPSX_TRY(upload(ctx, s.tsp, tsp.data(), sizeof(double) * nwa));
+    // KK matrix (src/scgd0.jl:183)
+    EPSX_TRY(dev_alloc(ctx, s.KK, sizeof(double) * nw * nw));
+    EPSX_TRY(kk_build(ctx, nw, dptr<double>(s.ws), dptr<double>(s.KK)));
+
+    // star order of the inner states
+    std::vector<int32_t> fsrc;
+    std::vector<int64_t> sptr;
+    star_order(nf, ni, P->f_to_i, fsrc, sptr);
+    std::vector<double> efs(nf), wfs(nf);
+    std::vector<int32_t> fis(nf);
+    s.star_ok = true;
+    for (int64_t p = 0; p < nf; p++) {
+        const int64_t f = fsrc[p];
+        efs[p] = P->e_f[f];
+        wfs[p] = P->w_f[f];
+        fis[p] = P->f_to_i[f] - 1;
+        if (!(P->e_f[f] == P->e_i[fis[p]])) s.star_ok = false;
+    }
+    s.nstar = 0;
+    for (int64_t i = 0; i < ni; i++) s.nstar += sptr[i + 1] > sptr[i];
+    EPSX_TRY(upload(ctx, s.ef_s, efs.data(), sizeof(double) * nf));
+    EPSX_TRY(upload(ctx, s.wf_s, wfs.data(), sizeof(double) * nf));
+    EPSX_TRY(upload(ctx, s.fi_s, fis.data(), sizeof(int32_t) * nf));
+    EPSX_TRY(upload(ctx, s.fsrc_s, fsrc.data(), sizeof(int32_t) * nf));
+    EPSX_TRY(upload(ctx, s.star_ptr, sptr.data(), sizeof(int64_t) * (ni + 1)));
+    // Einstein modes: ωph[ν, :] constant
+    s.einstein = true;
+    s.wnu_host.assign(nm, 0.0);
+    for (int64_t nu = 0; nu < nm; nu++) {
+        s.wnu_host[nu] = P->wph[nu];
+        for (int64_t q = 1; q < P->nq; q++)
+            if (!(P->wph[nu + nm * q] == P->wph[nu])) s.einstein = false;
+    }
+    EPSX_TRY(upload(ctx, s.wph, P->wph, sizeof(double) * nm * P->nq));
+    EPSX_TRY(upload(ctx, s.iq_kk, P->iq_kk, sizeof(int32_t) * P->nki * P->nkf));
+    EPSX_TRY(upload(ctx, s.iks_i, P->iks_i, sizeof(int32_t) * ni));
+    EPSX_TRY(upload(ctx, s.iks_f, P->iks_f, sizeof(int32_t) * nf));
+    EPSX_TRY(upload(ctx, s.e_i, P->e_i, sizeof(double) * ni));
+    EPSX_TRY(upload(ctx, s.w_i, P->w_i, sizeof(double) * ni));
+    if (P->n_mult > 0) {
+        EPSX_TRY(upload(ctx, s.mult_ptr, P->mult_ptr, sizeof(int64_t) * (P->n_mult + 1)));
+        EPSX_TRY(upload(ctx, s.mult_idx, P->mult_idx, sizeof(int32_t) * P->mult_ptr[P->n_mult]));
+    }
+    // g2: raw upload, then the star-ordered layout [ν, p, i] on the device
+    {
+        Tmp raw;
+        const size_t n = (size_t)nm * ni * nf;
+        EPSX_TRY(tmp_upload(ctx, raw, P->g2, sizeof(double) * n));
+        EPSX_TRY(dev_alloc(ctx, s.g2p, sizeof(double) * n));
+        hipLaunchKernelGGL(permute_g2_kernel, dim3(nblk((int64_t)n)), dim3(256), 0, ctx->stream, nm, ni,
+                           nf, raw.ptr<double>(), dptr<int32_t>(s.fsrc_s), dptr<double>(s.g2p));
+        EPSX_HIP(ctx, hipGetLastError());
+        EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    s.gt_ready = false;
+    s.ready = true;
+    return EPSX_OK;
+}
+
+namespace {
+
+struct RunBufs {
+    Tmp S, Snew, ImS, ReS, A, D, integ, part, bmax;
+    Tmp T, mu, eta, nlist, err, minA, status, tidx;
+};
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd0_outputs *out)
+{
+    if (!ctx) return EPSX_ERR_ARG;
+    Scgd0Dev &s = ctx->s;
+    if (!s.ready) return set_error(ctx, EPSX_ERR_STATE, "epsx_scgd0_run before epsx_scgd0_set_problem");
+    if (!par || par->struct_size != (int64_t)sizeof(epsx_scgd0_params) || !out ||
+        out->struct_size != (int64_t)sizeof(epsx_scgd0_outputs))
+        return set_error(ctx, EPSX_ERR_ARG, "epsx_scgd0_run: struct size mismatch (ABI %d)", EPSX_ABI_VERSION);
+    const int64_t nT = par->nT, maxiter = par->maxiter;
+    if (nT < 1 || !par->Tlist || !par->nlist || !par->mu_start || !par->eta_min)
+        return set_error(ctx, EPSX_ERR_ARG, "scgd0 run: nT must be positive with Tlist, nlist, μ_start and η_min given");
+    if (maxiter < 1) return set_error(ctx, EPSX_ERR_ARG, "maxiter must be at least 1, got %lld", (long long)maxiter);
+    for (int64_t t = 0; t < nT; t++)
+        if (!(par->Tlist[t] >= 0)) return set_error(ctx, EPSX_ERR_ARG, "Temperature must be positive");
+    if (!out->Sigma || !out->mu || !out->converged || !out->hist_err || !out->hist_mu || !out->hist_minA ||
+        !out->wall || !out->iter_frozen)
+        return set_error(ctx, EPSX_ERR_ARG, "scgd0 run: null output array");
+    if (par->fm_method < EPSX_FM_DIRECT || par->fm_method > EPSX_FM_AUTO)
+        return set_error(ctx, EPSX_ERR_ARG, "unknown Fan-Migdal method %d", par->fm_method);
+    if (par->precision != EPSX_PREC_FP64 && par->precision != EPSX_PREC_FP32)
+        return set_error(ctx, EPSX_ERR_ARG, "precision %d is not available for this method", par->precision);
+    EPSX_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t nw = s.nw, ni = s.ni, nwa = s.nwa, nf = s.nf;
+    const size_t nS = (size_t)nw * ni * nT;
+    const hipStream_t st = ctx->stream;
+
+    RunBufs B;
+    EPSX_TRY(tmp_alloc(ctx, B.S.b, 16 * nS));
+    EPSX_TRY(tmp_alloc(ctx, B.Snew.b, 16 * nS));
+    EPSX_TRY(tmp_alloc(ctx, B.ImS.b, 8 * nS));
+    EPSX_TRY(tmp_alloc(ctx, B.ReS.b, 8 * (size_t)nw * ni));
+    EPSX_TRY(tmp_alloc(ctx, B.A.b, 8 * (size_t)nwa * ni * nT));
+    EPSX_TRY(tmp_alloc(ctx, B.D.b, 8 * (size_t)nwa * nT));
+    EPSX_TRY(tmp_alloc(ctx, B.integ.b, 8 * (size_t)ni * nT));
+    size_t npart = 0;  // the largest partial buffer any launch of this run needs
+    for (int64_t a = 1; a <= (par->batched ? nT : 1); a++) {
+        const size_t e = fm_part_elems(nf, ni, nw, a, par->f_tile);
+        npart = e > npart ? e : npart;
+    }
+    EPSX_TRY(tmp_alloc(ctx, B.part.b, 8 * npart));
+    const int nbmax = 512;
+    EPSX_TRY(tmp_alloc(ctx, B.bmax.b, 8 * (size_t)nbmax * nT));
+    EPSX_TRY(tmp_upload(ctx, B.T, par->Tlist, 8 * nT));
+    EPSX_TRY(tmp_upload(ctx, B.mu, par->mu_start, 8 * nT));
+    EPSX_TRY(tmp_upload(ctx, B.eta, par->eta_min, 8 * nT));
+    EPSX_TRY(tmp_upload(ctx, B.nlist, par->nlist, 8 * nT));
+    EPSX_TRY(tmp_alloc(ctx, B.err.b, 8 * nT));
+    EPSX_TRY(tmp_alloc(ctx, B.minA.b, 8 * nT));
+    EPSX_TRY(tmp_alloc(ctx, B.status.b, 4 * nT));
+    EPSX_TRY(tmp_alloc(ctx, B.tidx.b, 4 * nT));
+    EPSX_HIP(ctx, hipMemsetAsync(B.status.b.p, 0, 4 * nT, st));
+
+    if (par->Sigma_init) {
+        EPSX_HIP(ctx, hipMemcpyAsync(B.S.b.p, par->Sigma_init, 16 * nS, hipMemcpyHostToDevice, st));
+    } else {
+        hipLaunchKernelGGL(fill_sigma_kernel, dim3(nblk((int64_t)nS)), dim3(256), 0, st, (int64_t)nS,
+                           B.S.ptr<double2>(), par->eta_init);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
+    EPSX_HIP(ctx, hipMemcpyAsync(B.Snew.b.p, B.S.b.p, 16 * nS, hipMemcpyDeviceToDevice, st));
+
+    FmGrid g;
+    g.nw = nw; g.span = s.span; g.ltab_len = s.ltab_len; g.dw_unit = s.dw_unit;
+    g.ws = dptr<double>(s.ws); g.off = dptr<int32_t>(s.off); g.rgap = dptr<double>(s.rgap);
+    g.ltab = dptr<int32_t>(s.ltab);
+    FmDirectArgs fa{};
+    fa.nm = s.nm; fa.ni = ni; fa.nfs = nf;
+    fa.g2p = dptr<double>(s.g2p);
+    fa.g2p32 = nullptr;
+    fa.ef = dptr<double>(s.ef_s); fa.wf = dptr<double>(s.wf_s);
+    fa.fi = dptr<int32_t>(s.fi_s); fa.fsrc = dptr<int32_t>(s.fsrc_s);
+    fa.wq_pair = nullptr;
+    fa.wph = dptr<double>(s.wph); fa.iq_kk = dptr<int32_t>(s.iq_kk);
+    fa.iks_i = dptr<int32_t>(s.iks_i); fa.iks_f = dptr<int32_t>(s.iks_f); fa.nki = s.nki;
+    fa.S = B.S.ptr<double>(); fa.ncol = ni; fa.w_acoustic = par->w_acoustic;
+
+    std::vector<int32_t> active(nT, 1), act_idx;
+    std::vector<double> h_err(nT), h_mu(nT), h_minA(nT);
+    std::vector<int32_t> h_status(nT);
+    std::vector<double> mu_host(par->mu_start, par->mu_start + nT);
+    for (int64_t q = 0; q < nT * maxiter; q++) out->hist_err[q] = out->hist_mu[q] = out->hist_minA[q] = NAN;
+    for (int64_t t = 0; t < nT; t++) out->iter_frozen[t] = 0;
+    for (int k = 0; k < 4; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
+    hipEvent_t ev[8];
+    for (auto &e : ev) EPSX_HIP(ctx, hipEventCreate(&e));
+    struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int k = 0; k < 8; k++) (void)hipEventDestroy(e[k]); } } eg{ev};
+
+    out->niter = 0;
+    out->fm_method_used = EPSX_FM_DIRECT;
+    out->n_low_precision = 0;
+    int rc = EPSX_OK;
+    for (int64_t iter = 1; iter <= maxiter; iter++) {
+        const double t0 = now_s();
+        act_idx.clear();
+        for (int64_t t = 0; t < nT; t++) if (active[t]) act_idx.push_back((int32_t)t);
+        const int nA = (int)act_idx.size();
+        EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, act_idx.data(), 4 * nA, hipMemcpyHostToDevice, st));
+        const int32_t *d_tidx = B.tidx.ptr<int32_t>();
+        // 1-3. clamp + Fan-Migdal (the clamp is applied when the column is staged)
+        EPSX_HIP(ctx, hipEventRecord(ev[0], st));
+        FmTemps tp;
+        tp.T = B.T.ptr<double>(); tp.mu = B.mu.ptr<double>(); tp.eta_min = B.eta.ptr<double>();
+        if (par->batched) {
+            tp.nT = nA;
+            tp.tidx = d_tidx;
+            EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile, par->precision,
+                                         B.part.ptr<double>(), (int64_t)npart, B.ImS.ptr<double>(), false));
+        } else {
+            for (int z = 0; z < nA; z++) {
+                tp.nT = 1;
+                tp.tidx = d_tidx + z;
+                EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile,
+                                             par->precision, B.part.ptr<double>(), (int64_t)npart,
+                                             B.ImS.ptr<double>(), false));
+            }
+        }
+        EPSX_HIP(ctx, hipEventRecord(ev[1], st));
+        // 4. Re Σ by Kramers-Kronig
+        for (int z = 0; z < nA; z++) {
+            const int64_t t = act_idx[z];
+            EPSX_TRY(dgemm_nn(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>() + nw * ni * t, nw,
+                              B.ReS.ptr<double>(), nw, false));
+            hipLaunchKernelGGL(combine_kernel, dim3(nblk(nw * ni)), dim3(256), 0, st, nw * ni,
+                               B.ReS.ptr<double>(), B.ImS.ptr<double>() + nw * ni * t,
+                               B.Snew.ptr<double2>() + nw * ni * t);
+            EPSX_HIP(ctx, hipGetLastError());
+        }
+        EPSX_HIP(ctx, hipEventRecord(ev[2], st));
+        // 5. multiplets
+        if (s.n_mult > 0) {
+            hipLaunchKernelGGL(multiplet_kernel, dim3(nblk(nw * s.n_mult), nA), dim3(256), 0, st, nw, ni, s.n_mult,
+                               dptr<int64_t>(s.mult_ptr), dptr<int32_t>(s.mult_idx), d_tidx, B.Snew.ptr<double2>());
+            EPSX_HIP(ctx, hipGetLastError());
+        }
+        // 6. spectral DOS and min ∫A
+        hipLaunchKernelGGL(dos_A_kernel, dim3(nblk(nwa * ni), nA), dim3(256), 0, st, nw, ni, nwa,
+                           dptr<double>(s.wsa), dptr<double>(s.e_i), dptr<int32_t>(s.jsp), dptr<double>(s.tsp),
+                           d_tidx, B.Snew.ptr<double2>(), B.A.ptr<double>());
+        EPSX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(dos_D_kernel, dim3(nblk(nwa), nA), dim3(256), 0, st, ni, nwa, B.A.ptr<double>(),
+                           dptr<double>(s.w_i), dptr<double>(s.Wa), B.D.ptr<double>());
+        EPSX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(dos_int_kernel, dim3((unsigned)ni, nA), dim3(256), 0, st, ni, nwa, B.A.ptr<double>(),
+                           dptr<double>(s.Wa), B.integ.ptr<double>());
+        EPSX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
+                           B.minA.ptr<double>());
+        EPSX_HIP(ctx, hipGetLastError());
+        // 7. μ
+        if (!par->fix_mu) {
+            MuArgs m;
+            m.nwa = nwa; m.wsa = dptr<double>(s.wsa); m.D = B.D.ptr<double>(); m.tidx = d_tidx;
+            m.T = B.T.ptr<double>(); m.nlist = B.nlist.ptr<double>();
+            m.nelec = par->nelec; m.spin = par->spin_degeneracy; m.W = s.W; m.nsb = s.nstates_base;
+            m.is_semi = par->is_semiconductor; m.mu = B.mu.ptr<double>(); m.status = B.status.ptr<int32_t>();
+            hipLaunchKernelGGL(mu_kernel, dim3(nA), dim3(1024), 0, st, m);
+            EPSX_HIP(ctx, hipGetLastError());
+        }
+        EPSX_HIP(ctx, hipEventRecord(ev[3], st));
+        // 8-9. err and Σ ← Σ_new
+        int nb = (int)((nw * ni + 255) / 256);
+        nb = nb > nbmax ? nbmax : nb;
+        hipLaunchKernelGGL(err_copy_kernel, dim3(nb, nA), dim3(256), 0, st, nw * ni, d_tidx,
+                           B.Snew.ptr<double2>(), B.S.ptr<double2>(), B.bmax.ptr<double>());
+        EPSX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(err_final_kernel, dim3(nA), dim3(64), 0, st, nb, B.bmax.ptr<double>(), d_tidx,
+                           B.err.ptr<double>());
+        EPSX_HIP(ctx, hipGetLastError());
+        EPSX_HIP(ctx, hipEventRecord(ev[4], st));
+        EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+        EPSX_HIP(ctx, hipMemcpyAsync(h_mu.data(), B.mu.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+        EPSX_HIP(ctx, hipMemcpyAsync(h_minA.data(), B.minA.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+        EPSX_HIP(ctx, hipMemcpyAsync(h_status.data(), B.status.b.p, 4 * nT, hipMemcpyDeviceToHost, st));
+        EPSX_HIP(ctx, hipStreamSynchronize(st));
+        float ms[4];
+        for (int k = 0; k < 4; k++) {
+            EPSX_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+            ctx->t_class[k == 0 ? 0 : (k == 1 ? 1 : (k == 2 ? 2 : 3))] += ms[k] * 1e-3;
+        }
+        ctx->n_class[0] += par->batched ? 1 : nA;
+        ctx->n_class[1] += nA;
+        for (int z = 0; z < nA; z++) {
+            const int64_t t = act_idx[z];
+            if (!par->fix_mu && h_status[t] == 2) {
+                const double n_target = par->nlist[t] + par->nelec;
+                rc = set_error(ctx, EPSX_ERR_NO_MU_ROOT,
+                               "no chemical potential in (%.17g, %.17g) gives %.17g electrons per cell at "
+                               "T = %.17g: the window holds too few states on one side of the gap, or the "
+                               "spectral weight it loses outside ωs_spectral moves the count past the target. "
+                               "Pass occ.μlist and fix_μ = true, or widen the window or ωs.",
+                               s.wsa_host[0] - 0.1, s.wsa_host[nwa - 1] + 0.1, n_target, par->Tlist[t]);
+                break;
+            }
+            mu_host[t] = h_mu[t];
+            out->hist_minA[t + nT * (iter - 1)] = h_minA[t];
+            out->hist_err[t + nT * (iter - 1)] = h_err[t];
+            out->hist_mu[t + nT * (iter - 1)] = h_mu[t];
+            if (h_err[t] < par->tol) {
+                active[t] = 0;
+                out->iter_frozen[t] = iter;
+            }
+        }
+        if (rc != EPSX_OK) break;
+        out->wall[iter - 1] = now_s() - t0;
+        out->niter = iter;
+        if (par->verbosity > 0) {
+            fprintf(stderr, "scGD0 iteration %lld: wall %.3f s\n", (long long)iter, out->wall[iter - 1]);
+            for (int z = 0; z < nA; z++)
+                fprintf(stderr, "  T=%g err=%.6g mu=%.12g\n", par->Tlist[act_idx[z]], h_err[act_idx[z]], h_mu[act_idx[z]]);
+        }
+        bool any = false;
+        for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
+        if (!any) break;
+    }
+    if (rc != EPSX_OK) return rc;
+    EPSX_HIP(ctx, hipMemcpyAsync(out->Sigma, B.S.b.p, 16 * nS, hipMemcpyDeviceToHost, st));
+    EPSX_HIP(ctx, hipStreamSynchronize(st));
+    for (int64_t t = 0; t < nT; t++) {
+        out->mu[t] = mu_host[t];
+        out->converged[t] = !active[t];
+    }
+    if (par->maxiter > 1) {
+        bool any = false;
+        for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
+        if (any) set_error(ctx, EPSX_OK, "scGD0 not converged after maxiter = %lld iterations", (long long)maxiter);
+    }
+    return EPSX_OK;
+}

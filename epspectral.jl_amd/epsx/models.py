@@ -1,0 +1,166 @@
+"""Synthetic scGD0 setups: the inputs that `_setup_scgd0` would extract, built directly.
+
+* `holstein_1d_setup`: the 1D Holstein chain of test/holstein_1d_reference.jl:62-77 (outer
+  states on the k -> -k wedge, one dispersionless mode, constant coupling).
+* `t2g_setup`: a cubic three-band t2g tight-binding model on a Γ-centred nk³ grid reduced by
+  the 48 operations of Oh, with Einstein modes and seeded log-normal couplings.
+
+Energies at symmetry-related k are computed from one canonical k, so ε_f equals ε of its IBZ
+representative bitwise and the degenerate multiplets are exact.
+"""
+import math
+
+import numpy as np
+
+from .grids import FrequencyGrid, graded_frequency_grid, uniform_spectral_grid
+from .scgd0 import Scgd0Setup, build_multiplets, ELECTRON_DEGEN_CUTOFF
+
+KB_EV = 8.617333262e-5   # Boltzmann constant in eV/K (CODATA 2018)
+
+
+def fermi(x, T):
+    with np.errstate(over="ignore"):
+        return 1.0 / (np.exp(x / T) + 1.0)
+
+
+def bare_mu(es, ws, T, n_target, spin_degeneracy=2.0, nstates_base=0.0):
+    """The μ at which spin * (Σ w f(ε - μ) + nstates_base) = n_target (the bare-band μ of the
+    window states), by bisection to adjacent floats."""
+    es = np.asarray(es, dtype=np.float64)
+    ws = np.asarray(ws, dtype=np.float64)
+
+    def g(mu):
+        return spin_degeneracy * (np.sum(ws * fermi(es - mu, T)) + nstates_base) - n_target
+
+    a, b = es.min() - 1.0, es.max() + 1.0
+    fa = g(a)
+    for _ in range(200):
+        m = 0.5 * (a + b)
+        if m == a or m == b:
+            break
+        fm = g(m)
+        if (fm > 0) == (fa > 0):
+            a, fa = m, fm
+        else:
+            b = m
+    return a
+
+
+def holstein_energies(nk, t):
+    """ε_j = -2t cos(2πj/nk), evaluated at min(j, nk - j) so that k and -k agree bitwise."""
+    j = np.arange(nk)
+    jj = np.minimum(j, nk - j)
+    return -2.0 * t * np.cos(2.0 * np.pi * jj / nk)
+
+
+def holstein_1d_setup(nk, t, ω0, g2, ωs, μ_start, ωs_spectral=None):
+    """The hand-built Holstein input of test/holstein_1d_reference.jl:62-77."""
+    ni = nk // 2 + 1
+    εs = holstein_energies(nk, t)
+    ji = np.arange(ni)
+    jf = np.arange(nk)
+    iq_kk = (np.mod(jf[None, :] - ji[:, None], nk) + 1).astype(np.int32)
+    return Scgd0Setup(
+        g2=np.full((1, ni, nk), float(g2), order="F"),
+        ωph=np.full((1, nk), float(ω0), order="F"),
+        iq_kk=np.asfortranarray(iq_kk),
+        iks_i=np.arange(1, ni + 1, dtype=np.int32),
+        iks_f=np.arange(1, nk + 1, dtype=np.int32),
+        ε_i=εs[:ni].copy(),
+        w_i=np.array([(1 if (j == 0 or 2 * j == nk) else 2) / nk for j in range(ni)]),
+        ε_f=εs.copy(),
+        w_f=np.full(nk, 1.0 / nk),
+        f_to_i=np.array([min(j, nk - j) + 1 for j in range(nk)], dtype=np.int32),
+        W=1.0, nstates_base=0.0, multiplets=[],
+        ωs=ωs, ωs_spectral=ωs_spectral if ωs_spectral is not None else uniform_spectral_grid(ωs),
+        μ_start=np.atleast_1d(np.asarray(μ_start, dtype=np.float64)),
+    )
+
+
+def t2g_bands(cx, cy, cz, t, tp):
+    """The three t2g bands (xy, yz, zx) at cos k = (cx, cy, cz), sorted ascending."""
+    exy = -2 * t * (cx + cy) - 2 * tp * cz
+    eyz = -2 * t * (cy + cz) - 2 * tp * cx
+    ezx = -2 * t * (cz + cx) - 2 * tp * cy
+    return np.sort(np.stack([exy, eyz, ezx], axis=-1), axis=-1)
+
+
+def oh_ibz(nk):
+    """Γ-centred nk³ grid reduced by Oh (sign flips and permutations of the axes).
+
+    Returns (canon index of each full-grid k [nk³], list of canonical triples (a<=b<=c, each in
+    0..nk//2), star sizes)."""
+    n = np.arange(nk)
+    fold = np.minimum(n, nk - n)
+    k1, k2, k3 = np.meshgrid(fold, fold, fold, indexing="ij")
+    trip = np.sort(np.stack([k1.ravel(order="F"), k2.ravel(order="F"), k3.ravel(order="F")], axis=1), axis=1)
+    h = nk // 2 + 1
+    key = trip[:, 0] * h * h + trip[:, 1] * h + trip[:, 2]
+    uniq, inv, counts = np.unique(key, return_inverse=True, return_counts=True)
+    canon = np.stack([uniq // (h * h), (uniq // h) % h, uniq % h], axis=1)
+    return inv, canon, counts
+
+
+def t2g_setup(nk=24, t=0.26, tp=0.03, nmodes=9, wmin=0.010, wmax=0.080, g2_median=2e-3,
+              g2_sigma_ln=1.0, window=0.8, nelec=1.0, Tlist_K=(100.0, 300.0), nlist=0.0, seed=1234,
+              grid_windows=((0.3, 1e-3), (1.2, 4e-3), (3.0, 16e-3)), spectral_step=None,
+              degen_cutoff=ELECTRON_DEGEN_CUTOFF, make_g2=True):
+    """The synthetic cubic t2g model of the benchmark, in eV.
+
+    * bands ε_xy = -2t(cos kx + cos ky) - 2t' cos kz and cyclic, sorted per k;
+    * outer states: Oh-irreducible k x bands within ±`window` of the bare μ at the first
+      temperature; inner states: the full grid in the same window;
+    * `nmodes` Einstein modes ω_ν ~ U(wmin, wmax); g2[ν,i,f] = |g|²/(2ω_ν) with
+      |g|² ~ LogNormal(ln g2_median, g2_sigma_ln), i.i.d., seeded;
+    * ωs graded around the bare μ: 1 meV to ±0.3 eV, 4 meV to ±1.2 eV, 16 meV to ±3 eV;
+      ωs_spectral uniform at ωs.dω_min over the span of ωs (the run_scgd0 default).
+    """
+    rng = np.random.default_rng(seed)
+    inv, canon, counts = oh_ibz(nk)
+    nkf = nk ** 3
+    ctab = np.cos(2 * np.pi * np.arange(nk // 2 + 1) / nk)
+    e_ibz = t2g_bands(ctab[canon[:, 0]], ctab[canon[:, 1]], ctab[canon[:, 2]], t, tp)  # (nkibz, 3)
+    e_full = e_ibz[inv]                                                                 # (nkf, 3)
+    Tlist = np.asarray(Tlist_K, dtype=np.float64) * KB_EV
+    mu_all = bare_mu(e_full.ravel(), np.full(e_full.size, 1.0 / nkf), Tlist[0], nelec)
+    lo, hi = mu_all - window, mu_all + window
+    nstates_base = float(np.sum(e_full < lo)) / nkf
+    # outer states: (IBZ k, band) in the window, ordered by k then band
+    sel_i = [(k, b) for k in range(len(canon)) for b in range(3) if lo <= e_ibz[k, b] <= hi]
+    index_i = {kb: n for n, kb in enumerate(sel_i)}
+    ε_i = np.array([e_ibz[k, b] for k, b in sel_i])
+    w_i = np.array([counts[k] / nkf for k, b in sel_i])
+    iks_i = np.array([k + 1 for k, b in sel_i], dtype=np.int32)
+    # inner states: (full k, band) in the window
+    sel_f = [(k, b) for k in range(nkf) for b in range(3) if lo <= e_full[k, b] <= hi]
+    ε_f = np.array([e_full[k, b] for k, b in sel_f])
+    f_to_i = np.array([index_i[(inv[k], b)] + 1 for k, b in sel_f], dtype=np.int32)
+    ni, nf = len(sel_i), len(sel_f)
+    ω_ν = np.sort(rng.uniform(wmin, wmax, size=nmodes))
+    if make_g2:
+        g2 = np.empty((nmodes, ni, nf), order="F")
+        chunk = max(1, (1 << 25) // (nmodes * ni))
+        for f0 in range(0, nf, chunk):
+            f1 = min(nf, f0 + chunk)
+            absg2 = rng.lognormal(mean=math.log(g2_median), sigma=g2_sigma_ln, size=(nmodes, ni, f1 - f0))
+            g2[:, :, f0:f1] = absg2 / (2.0 * ω_ν[:, None, None])
+    else:
+        g2 = np.empty((nmodes, ni, 0), order="F")
+    μ_start = np.array([bare_mu(ε_i, w_i, T, nelec + nl, 2.0, nstates_base)
+                        for T, nl in zip(Tlist, np.broadcast_to(nlist, Tlist.shape))])
+    wins = [(mu_all - w, mu_all + w, dw) for (w, dw) in grid_windows]
+    ωs = graded_frequency_grid(wins)
+    ωs_spectral = (uniform_spectral_grid(ωs) if spectral_step is None else
+                   FrequencyGrid(ωs[0], spectral_step, np.arange(int(round((ωs[-1] - ωs[0]) / spectral_step)) + 1)))
+    setup = Scgd0Setup(
+        g2=g2, ωph=np.asfortranarray(ω_ν.reshape(nmodes, 1)),
+        iq_kk=np.ones((1, 1), dtype=np.int32, order="F"),
+        iks_i=np.ones(ni, dtype=np.int32), iks_f=np.ones(nf, dtype=np.int32),
+        ε_i=ε_i, w_i=w_i, ε_f=ε_f, w_f=np.full(nf, 1.0 / nkf), f_to_i=f_to_i,
+        W=float(w_i.sum()), nstates_base=nstates_base,
+        multiplets=build_multiplets(iks_i, ε_i, degen_cutoff),
+        ωs=ωs, ωs_spectral=ωs_spectral, μ_start=μ_start)
+    meta = dict(nk=nk, t=t, tp=tp, ω_ν=ω_ν, Tlist=Tlist, Tlist_K=np.asarray(Tlist_K, dtype=float),
+                mu_bare=mu_all, window=(lo, hi), ni=ni, nf=nf, seed=seed, canon=canon, inv=inv,
+                counts=counts, sel_i=sel_i, sel_f=sel_f)
+    return setup, meta

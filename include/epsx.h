@@ -1,0 +1,183 @@
+/*
+ * epsx.h -- the C ABI of libepsx, the MI355X (gfx950) implementation of EPSpectral.jl's
+ * scGD0 hot path.
+ *
+ * Every entry point takes plain pointers and sizes.  Arrays are column-major with the first
+ * index fastest (Julia's layout), complex numbers are interleaved (re, im) doubles
+ * (ComplexF64), and integer indices into state, q or grid arrays are 1-based, exactly as a
+ * Julia `ccall` hands them over.  Host pointers are read during the call only; results go
+ * into caller-allocated host arrays.
+ *
+ * Return codes: EPSX_OK, or one of the EPSX_ERR_* codes below.  After an error,
+ * epsx_last_error(ctx) holds the message the reference throws or warns with (for a
+ * Julia wrapper: EPSX_ERR_ARG and EPSX_ERR_NO_MU_ROOT become ArgumentError).
+ *
+ * Threading: one HIP stream per context; calls on one context are serialised by the
+ * caller; separate contexts may be used from separate threads.
+ *
+ * The reference interface each entry replaces is cited as file:line of the EPSpectral.jl
+ * repository.
+ */
+#ifndef EPSX_H
+#define EPSX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EPSX_ABI_VERSION 1
+
+#define EPSX_OK 0
+#define EPSX_ERR_ARG 1          /* invalid argument: ArgumentError in the reference */
+#define EPSX_ERR_NO_MU_ROOT 2   /* no μ in the bracket: src/scgd0.jl:317-322 */
+#define EPSX_ERR_DEVICE 3       /* a HIP runtime error */
+#define EPSX_ERR_OOM 4          /* device allocation failed */
+#define EPSX_ERR_STATE 5        /* call order: e.g. run before set_problem */
+
+typedef struct epsx_ctx epsx_ctx;
+
+/* ---- context ------------------------------------------------------------------------ */
+/* The device handle that replaces the `backend` keyword threaded through run_scgd0 /
+ * run_scgd0_ladder (src/scgd0.jl:77; src/ladder.jl:67). */
+int epsx_abi_version(void);
+int epsx_open(int device, epsx_ctx **ctx);
+void epsx_close(epsx_ctx *ctx);
+const char *epsx_last_error(const epsx_ctx *ctx);
+/* Wall-clock seconds of the last run's kernels, by kernel class (FM, KK, DOS+μ, other),
+ * measured with HIP events on the context's stream. */
+int epsx_last_timings(const epsx_ctx *ctx, double *seconds4, int64_t *launches4);
+
+/* ---- grid maps (host, bit-exact with the reference) ---------------------------------- */
+/* cell_search_table, src/frequency_grids.jl:184-196.  ws: the nodes, dw_min = ωs.dω_min.
+ * jtab may be NULL to query *len.  Errors as the reference (table over 2^24 entries, two
+ * nodes in one bucket). */
+int epsx_cell_search_table(epsx_ctx *ctx, int64_t n, const double *ws, double dw_min,
+                           int32_t *jtab, int64_t cap, int64_t *len, double *x0,
+                           double *inv_h);
+/* linear_interpolation_table, src/frequency_grids.jl:65-84 (j 1-based). */
+int epsx_linear_interpolation_table(epsx_ctx *ctx, int64_t n, const double *ws, int64_t m,
+                                    const double *xs, int32_t *j, double *t);
+/* trapezoid_weights, src/frequency_grids.jl:53-54. */
+int epsx_trapezoid_weights(epsx_ctx *ctx, int64_t n, const double *ws, double *W);
+/* vertex_shift_tables, src/ladder_kernel.jl:158-176: klo (1-based), wlo, whi of size
+ * [nw, nshift, 2] from the integer offsets of the grid. */
+int epsx_vertex_shift_tables(epsx_ctx *ctx, int64_t nw, const int64_t *offsets,
+                             int64_t nshift, int32_t *klo, double *wlo, double *whi);
+
+/* ---- device operators on host buffers (the reference's unit-level functions) -------- */
+/* kramers_kronig_matrix, src/frequency_grids.jl:30-44, built on the device. K: [n, n]. */
+int epsx_kramers_kronig_matrix(epsx_ctx *ctx, int64_t n, const double *ws, double *K);
+/* kramers_kronig(ωs, zs), src/frequency_grids.jl:227: ys = K * zs, zs [n, ncol] real,
+ * the product on the device. */
+int epsx_kramers_kronig(epsx_ctx *ctx, int64_t n, const double *ws, int64_t ncol,
+                        const double *zs, double *ys);
+
+/* add_fan_migdal_imsigma!, src/scgd0_kernel.jl:59-72, with the per-pair inputs of the
+ * reference's [ν, i, f] arrays (the (νf, i) fusion of src/scgd0.jl:264-279 is a layout
+ * change the kernel does itself):
+ *   ImS[ω, i] += Σ_{f, ν} _fan_migdal_imsigma_term(ωs[ω], g2[ν,i,f], wq[ν,i,f], e_f[f],
+ *                                                 f_to_i[f], Sin, ...)
+ * The grid is a FrequencyGrid (origin, unit, integer offsets).  Sin: complex [nw, ncol],
+ * the clamped input self-energy; f_to_i in 1..ncol.  precision: 0 FP64, 1 FP32 (FP32
+ * terms, compensated FP32 sum over f). */
+int epsx_add_fan_migdal_imsigma(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nf,
+                                const double *g2, const double *wq, const double *e_f,
+                                const int32_t *f_to_i, const double *w_f, int64_t nw,
+                                double w_origin, double dw_unit, const int64_t *offsets,
+                                int64_t ncol, const double *Sin, double mu, double T,
+                                double w_acoustic, int32_t precision, double *ImS);
+
+/* ---- scGD0 ---------------------------------------------------------------------------- */
+/* The `setup` of _setup_scgd0 (src/scgd0.jl:173-183; built by hand at
+ * test/holstein_1d_reference.jl:62-77), as plain arrays.  The library derives the grid
+ * maps (cell and spectral tables, trapezoid weights, KK matrix) from the two grids itself.
+ * g2 is copied to the device once, in the layout the Fan-Migdal kernel streams, and stays
+ * resident. */
+typedef struct {
+    int64_t struct_size;      /* sizeof(epsx_scgd0_problem) */
+    int64_t nm, ni, nf;       /* phonon modes, outer (IBZ) states, inner (full BZ) states */
+    int64_t nq, nki, nkf;     /* phonon q points; rows and columns of iq_kk */
+    const double *g2;         /* [nm, ni, nf] stored coupling */
+    const double *wph;        /* [nm, nq] phonon frequencies */
+    const int32_t *iq_kk;     /* [nki, nkf] 1-based q index of the pair (k_i, k_f) */
+    const int32_t *iks_i;     /* [ni] 1-based row of iq_kk */
+    const int32_t *iks_f;     /* [nf] 1-based column of iq_kk */
+    const double *e_i, *w_i;  /* [ni] energies and weights of the outer states */
+    const double *e_f, *w_f;  /* [nf] of the inner states */
+    const int32_t *f_to_i;    /* [nf] 1-based IBZ representative */
+    double W, nstates_base;   /* Σ w_i and the states below the window */
+    int64_t nw;               /* ωs: ω_origin + offsets * dω_unit */
+    double w_origin, dw_unit;
+    const int64_t *w_offsets;
+    int64_t nwa;              /* ωs_spectral */
+    double wa_origin, dwa_unit;
+    const int64_t *wa_offsets;
+    int64_t n_mult;           /* multiplets (src/scgd0.jl:146-160) as CSR: */
+    const int64_t *mult_ptr;  /* [n_mult + 1], 0-based */
+    const int32_t *mult_idx;  /* 1-based outer states */
+} epsx_scgd0_problem;
+
+/* Fan-Migdal methods.  DIRECT is the reference's sum term by term (any phonon dispersion).
+ * The factorised methods are exact rewrites behind a precondition the library checks:
+ * EINSTEIN needs ωph[ν, :] constant per mode; STAR also needs e_f == e_{f_to_i[f]}
+ * bitwise.  AUTO picks STAR, then EINSTEIN, then DIRECT. */
+#define EPSX_FM_DIRECT 0
+#define EPSX_FM_EINSTEIN 1
+#define EPSX_FM_STAR 2
+#define EPSX_FM_AUTO 3
+
+/* Precision of the Fan-Migdal sum: FP64, or FP32 terms and couplings with a compensated
+ * FP32 sum over inner states, or BF16 couplings with FP32 accumulation (factorised methods
+ * only).  A non-FP64 run ends with FP64 sweeps until tol is met in FP64. */
+#define EPSX_PREC_FP64 0
+#define EPSX_PREC_FP32 1
+#define EPSX_PREC_BF16 2
+
+typedef struct {
+    int64_t struct_size;      /* sizeof(epsx_scgd0_params) */
+    int64_t nT;
+    const double *Tlist;      /* [nT] temperatures (energy units) */
+    const double *nlist;      /* [nT] doping per cell */
+    const double *mu_start;   /* [nT] starting μ (occ.μlist or the bare μ) */
+    const double *eta_min;    /* [nT] clamp of Im Σ */
+    double nelec, spin_degeneracy;
+    int32_t is_semiconductor; /* occ.type === :Semiconductor */
+    int32_t fix_mu;
+    double eta_init, tol, w_acoustic;
+    int64_t maxiter;
+    int64_t f_tile;           /* inner states per Fan-Migdal block (0: automatic); like
+                                 gpu_tile it only reorders the sum over f */
+    int32_t fm_method;        /* EPSX_FM_* */
+    int32_t precision;        /* EPSX_PREC_* */
+    int32_t batched;          /* 1: all active temperatures per launch */
+    int32_t verbosity;
+    const double *Sigma_init; /* NULL (start at -iη_init) or complex [nw, ni, nT] */
+} epsx_scgd0_params;
+
+typedef struct {
+    int64_t struct_size;      /* sizeof(epsx_scgd0_outputs) */
+    double *Sigma;            /* complex [nw, ni, nT] */
+    double *mu;               /* [nT] */
+    int32_t *converged;       /* [nT] */
+    double *hist_err;         /* [nT, maxiter], NaN after a temperature froze */
+    double *hist_mu;          /* [nT, maxiter] */
+    double *hist_minA;        /* [nT, maxiter] min_i ∫A_i */
+    double *wall;             /* [maxiter] seconds per iteration */
+    int64_t *iter_frozen;     /* [nT], 0 if not converged */
+    int64_t niter;            /* out: iterations run */
+    int32_t fm_method_used;   /* out: EPSX_FM_* actually run */
+    int32_t n_low_precision;  /* out: iterations run below FP64 */
+} epsx_scgd0_outputs;
+
+/* Upload the problem (replaces the previous one). */
+int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *prob);
+/* _loop_scgd0, src/scgd0.jl:189-346, on the uploaded problem.  Returns EPSX_ERR_NO_MU_ROOT
+ * with the reference's message when the μ bracket holds no root; a run that reaches maxiter
+ * unconverged returns EPSX_OK with converged[T] = 0 (the reference warns). */
+int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd0_outputs *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes bindings of the CPU oracle (oracle/_build/libepsx_oracle.so,
+built from epsx_oracle.c by oracle/Makefile).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module;
+the product (epspectral.jl_amd/) never does.  Functions take plain numpy arrays in the
+reference's column-major index order, or a setup object with the attributes of the reference's
+`setup` NamedTuple (g2, ωph, iq_kk, iks_i, iks_f, ε_i, w_i, ε_f, w_f, f_to_i, W, nstates_base,
+multiplets, ωs, ωs_spectral, μ_start), as the tests build it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libepsx_oracle.so")
+
+P, I64, I32, D = C.c_void_p, C.c_int64, C.c_int32, C.c_double
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        sig = {
+            "orc_occ_fermion": (D, [D, D]),
+            "orc_occ_boson": (D, [D, D]),
+            "orc_grid_from_nodes": (C.c_int, [I64, P, D, P, P, P, P, P, P]),
+            "orc_graded_grid": (I64, [I64, P, P, P, P, I64]),
+            "orc_cell_search_table": (I64, [I64, P, D, P, P, P, I64]),
+            "orc_interp_linear_grid": (None, [P, I64, D, I64, P, P, D, D, P]),
+            "orc_linear_interpolation_table": (None, [I64, P, I64, P, P, P]),
+            "orc_apply_interpolation_table": (None, [I64, I64, I64, P, P, P, P]),
+            "orc_trapezoid_weights": (None, [I64, P, P]),
+            "orc_kramers_kronig_matrix": (None, [I64, P, P]),
+            "orc_fm_term": (D, [D, D, D, D, I64, P, I64, P, P, D, D, D, D, D, D]),
+            "orc_add_fan_migdal": (None, [I64, I64, I64, I64, I64, P, P, P, P, P, I64, P, P, D, D,
+                                          I64, P, D, D, D, P, C.c_int]),
+            "orc_loop_scgd0": (C.c_int, [P, P, P]),
+            "orc_ncarrier_spectral": (D, [D, D, I64, P, P, D, C.c_int, D, D]),
+            "orc_mu_bisection": (D, [D, D, D, I64, P, P, D, C.c_int, D, D, D, P]),
+            "orc_build_multiplets": (I64, [I64, P, P, D, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.asfortranarray(a, dtype=np.int32)
+
+
+def _i64(a):
+    return np.asfortranarray(a, dtype=np.int64)
+
+
+def occ_fermion(e, T):
+    return lib().orc_occ_fermion(float(e), float(T))
+
+
+def occ_boson(e, T):
+    return lib().orc_occ_boson(float(e), float(T))
+
+
+class OracleError(Exception):
+    pass
+
+
+def grid_from_nodes(ws, atol=1e-8):
+    """(origin, unit, offsets) or raises OracleError(code, bad_j, dev, best_unit)."""
+    ws = _f64(ws)
+    n = len(ws)
+    o, u, dev, bu = np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1)
+    j = np.zeros(1, dtype=np.int64)
+    off = np.zeros(n, dtype=np.int64)
+    rc = lib().orc_grid_from_nodes(n, _p(ws), atol, _p(o), _p(u), _p(off), _p(dev), _p(j), _p(bu))
+    if rc:
+        raise OracleError(rc, int(j[0]), float(dev[0]), float(bu[0]))
+    return float(o[0]), float(u[0]), off
+
+
+def graded_grid(windows):
+    w = _f64(np.asarray(windows, dtype=np.float64).ravel())
+    cap = 1 << 22
+    off = np.zeros(cap, dtype=np.int64)
+    o, u = np.zeros(1), np.zeros(1)
+    n = lib().orc_graded_grid(len(windows), _p(w), _p(o), _p(u), _p(off), cap)
+    if n < 0:
+        raise OracleError(int(n))
+    return float(o[0]), float(u[0]), off[:n].copy()
+
+
+def cell_search_table(ws, dw_min):
+    ws = _f64(ws)
+    x0, ih = np.zeros(1), np.zeros(1)
+    n = lib().orc_cell_search_table(len(ws), _p(ws), dw_min, _p(x0), _p(ih), None, 0)
+    if n < 0:
+        raise OracleError(int(n))
+    jtab = np.zeros(n, dtype=np.int32)
+    lib().orc_cell_search_table(len(ws), _p(ws), dw_min, _p(x0), _p(ih), _p(jtab), n)
+    return jtab, float(x0[0]), float(ih[0])
+
+
+def interp_linear_grid(ys, col, x, ws, table):
+    ys = np.asfortranarray(ys, dtype=np.complex128)
+    ws = _f64(ws)
+    jtab, x0, ih = table
+    out = np.zeros(2)
+    lib().orc_interp_linear_grid(_p(ys), col, float(x), len(ws), _p(ws), _p(jtab), x0, ih, _p(out))
+    return complex(out[0], out[1])
+
+
+def linear_interpolation_table(ws, xs):
+    ws, xs = _f64(ws), _f64(xs)
+    j = np.zeros(len(xs), dtype=np.int32)
+    t = np.zeros(len(xs))
+    lib().orc_linear_interpolation_table(len(ws), _p(ws), len(xs), _p(xs), _p(j), _p(t))
+    return j, t
+
+
+def apply_interpolation_table(ys, j, t):
+    ys = np.asfortranarray(ys, dtype=np.complex128)
+    ys2 = ys.reshape(ys.shape[0], -1, order="F")
+    Y = np.zeros((len(j), ys2.shape[1]), dtype=np.complex128, order="F")
+    lib().orc_apply_interpolation_table(len(j), ys2.shape[0], ys2.shape[1], _p(ys2), _p(_i32(j)),
+                                        _p(_f64(t)), _p(Y))
+    return Y
+
+
+def trapezoid_weights(ws):
+    ws = _f64(ws)
+    W = np.zeros(len(ws))
+    lib().orc_trapezoid_weights(len(ws), _p(ws), _p(W))
+    return W
+
+
+def kramers_kronig_matrix(ws):
+    ws = _f64(ws)
+    K = np.zeros((len(ws), len(ws)), order="F")
+    lib().orc_kramers_kronig_matrix(len(ws), _p(ws), _p(K))
+    return K
+
+
+def add_fan_migdal(ImS, g2, wq_fi, e_f, f_to_i, w_f, ws, table, Sin, mu, T, w_acoustic, nthreads=8,
+                   f0=0, f1=None):
+    """ImS[ω, i] += Σ_{f in [f0, f1), ν} term, with wq_fi the pair table [ν, f, i]."""
+    nm, ni, nf = g2.shape
+    f1 = nf if f1 is None else f1
+    ImS = np.asfortranarray(ImS, dtype=np.float64)
+    jtab, x0, ih = table
+    lib().orc_add_fan_migdal(nm, ni, nf, f0, f1, _p(_f64(g2)), _p(_f64(wq_fi)), _p(_f64(e_f)),
+                             _p(_i32(f_to_i)), _p(_f64(w_f)), len(ws), _p(_f64(ws)), _p(jtab), x0, ih,
+                             Sin.shape[1], _p(np.asfortranarray(Sin, dtype=np.complex128)), mu, T,
+                             w_acoustic, _p(ImS), nthreads)
+    return ImS
+
+
+def ncarrier_spectral(mu, T, wsa, D, W, count_holes, nstates_base, spin_degeneracy):
+    return lib().orc_ncarrier_spectral(mu, T, len(wsa), _p(_f64(wsa)), _p(_f64(D)), W, int(count_holes),
+                                       nstates_base, spin_degeneracy)
+
+
+def build_multiplets(iks, es, cutoff):
+    ni = len(es)
+    ptr = np.zeros(ni + 1, dtype=np.int64)
+    idx = np.zeros(ni, dtype=np.int64)
+    nm = lib().orc_build_multiplets(ni, _p(_i32(iks)), _p(_f64(es)), cutoff, _p(ptr), _p(idx))
+    return [list(idx[ptr[m]:ptr[m + 1]]) for m in range(nm)]
+
+
+class _Setup(C.Structure):
+    _fields_ = [("nm", I64), ("ni", I64), ("nf", I64), ("nw", I64), ("nwa", I64), ("nq", I64),
+                ("nki", I64), ("nkf", I64),
+                ("g2", P), ("wph", P), ("iq_kk", P), ("iks_i", P), ("iks_f", P),
+                ("e_i", P), ("w_i", P), ("e_f", P), ("w_f", P), ("f_to_i", P),
+                ("W", D), ("nstates_base", D), ("ws", P), ("dw_min", D), ("wsa", P),
+                ("n_mult", I64), ("mult_ptr", P), ("mult_idx", P)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("nT", I64), ("Tlist", P), ("nlist", P), ("mu_start", P), ("eta_min", P),
+                ("nelec", D), ("spin_degeneracy", D), ("is_semiconductor", C.c_int),
+                ("eta_init", D), ("tol", D), ("w_acoustic", D), ("maxiter", I64), ("nf_chunk", I64),
+                ("fix_mu", C.c_int), ("Sigma_init", P), ("nthreads", C.c_int)]
+
+
+class _Out(C.Structure):
+    _fields_ = [("Sigma", P), ("mu", P), ("converged", P), ("hist_err", P), ("hist_mu", P),
+                ("hist_minA", P), ("wall", P), ("iter_frozen", P), ("niter", I64), ("status", C.c_int),
+                ("err_T", I64), ("err_n_target", D), ("err_bracket", D * 2)]
+
+
+def loop_scgd0(setup, Tlist, nlist, nelec, *, η_min, η_init, maxiter, tol, ω_acoustic,
+               spin_degeneracy=2.0, semiconductor=False, fix_μ=False, μ_start=None, Σ_init=None,
+               gpu_tile=None, nthreads=8):
+    """`_loop_scgd0` (src/scgd0.jl:189-346) on the CPU.  Returns dict(Σ, μ, converged, history)
+    or raises OracleError(2, T index) when no μ root is bracketed."""
+    s = setup
+    nm, ni, nf = s.g2.shape
+    keep = dict(g2=_f64(s.g2), wph=_f64(s.ωph), iq=_i32(s.iq_kk), iki=_i32(s.iks_i), ikf=_i32(s.iks_f),
+                ei=_f64(s.ε_i), wi=_f64(s.w_i), ef=_f64(s.ε_f), wf=_f64(s.w_f), fi=_i32(s.f_to_i),
+                ws=_f64(np.asarray(s.ωs)), wsa=_f64(np.asarray(s.ωs_spectral)))
+    mptr, midx = [0], []
+    for m in s.multiplets:
+        midx.extend(m)
+        mptr.append(len(midx))
+    keep["mptr"] = _i64(mptr)
+    keep["midx"] = _i64(midx if midx else [0])
+    S = _Setup(nm, ni, nf, len(keep["ws"]), len(keep["wsa"]), keep["wph"].shape[1], keep["iq"].shape[0],
+               keep["iq"].shape[1], _p(keep["g2"]), _p(keep["wph"]), _p(keep["iq"]), _p(keep["iki"]),
+               _p(keep["ikf"]), _p(keep["ei"]), _p(keep["wi"]), _p(keep["ef"]), _p(keep["wf"]),
+               _p(keep["fi"]), float(s.W), float(s.nstates_base), _p(keep["ws"]), float(s.ωs.dω_min),
+               _p(keep["wsa"]), len(s.multiplets), _p(keep["mptr"]), _p(keep["midx"]))
+    Tl = _f64(np.atleast_1d(Tlist))
+    nT = len(Tl)
+    nl = _f64(np.broadcast_to(np.asarray(nlist, dtype=float), (nT,)))
+    mu0 = _f64(np.broadcast_to(np.asarray(s.μ_start if μ_start is None else μ_start, dtype=float), (nT,)))
+    eta = _f64(np.broadcast_to(np.asarray(η_min, dtype=float), (nT,)))
+    Si = None if Σ_init is None else np.asfortranarray(Σ_init, dtype=np.complex128)
+    Pm = _Params(nT, _p(Tl), _p(nl), _p(mu0), _p(eta), float(nelec), float(spin_degeneracy),
+                 int(semiconductor), float(η_init), float(tol), float(ω_acoustic), int(maxiter),
+                 0 if gpu_tile is None else int(gpu_tile), int(fix_μ), _p(Si), int(nthreads))
+    nw = len(keep["ws"])
+    Σ = np.zeros((nw, ni, nT), dtype=np.complex128, order="F")
+    μ = np.zeros(nT)
+    conv = np.zeros(nT, dtype=np.int32)
+    he, hm, ha = (np.zeros((nT, maxiter), order="F") for _ in range(3))
+    wall = np.zeros(maxiter)
+    fr = np.zeros(nT, dtype=np.int64)
+    O = _Out(_p(Σ), _p(μ), _p(conv), _p(he), _p(hm), _p(ha), _p(wall), _p(fr), 0, 0, 0, 0.0, (D * 2)(0, 0))
+    rc = lib().orc_loop_scgd0(C.byref(S), C.byref(Pm), C.byref(O))
+    if rc:
+        raise OracleError(rc, int(O.err_T))
+    n = int(O.niter)
+    return dict(Σ=Σ, μ=μ, converged=conv.astype(bool),
+                history=dict(err=he[:, :n].copy(), μ=hm[:, :n].copy(), min_spectral_weight=ha[:, :n].copy(),
+                             wall=wall[:n].copy(), iter_frozen=fr.copy()))

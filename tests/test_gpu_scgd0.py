@@ -1,0 +1,249 @@
+"""GPU parity of the scGD0 path: libepsx (through its C ABI) against the CPU oracle.
+
+Tolerances: 1e-13 relative for single kernels in FP64 (the device's exp/log/division differ from
+glibc's by an ulp), 1e-10 relative for whole scGD0 runs in FP64 (the reference's own GPU gate,
+test/test_scgd0_gpu.jl:36-37), bitwise for run-to-run repeats and for integer tables.
+"""
+import math
+import warnings
+
+import numpy as np
+import pytest
+
+import epsx
+import oracle
+from epsx.grids import FrequencyGrid, graded_frequency_grid
+from epsx.models import holstein_1d_setup, t2g_setup, bare_mu
+from ref_numpy import pseudorandom, pseudorandom_complex, naive_fan_migdal, holstein_bare_mu
+
+pytestmark = pytest.mark.gpu
+
+SRVO3_NODES = np.concatenate([np.arange(-0.35, -0.11 + 1e-9, 0.02), np.arange(-0.099, 0.100 + 1e-9, 0.001),
+                              np.arange(0.12, 0.34 + 1e-9, 0.02)])
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b))
+
+
+def test_kk_matrix_and_apply(ctx):
+    g = FrequencyGrid.from_nodes(SRVO3_NODES)
+    K = epsx.kramers_kronig_matrix(g, ctx)
+    K_ref = oracle.kramers_kronig_matrix(np.asarray(g))
+    assert rel(K, K_ref) <= 1e-13
+    z = pseudorandom(len(g), 5, seed=1)
+    y = epsx.kramers_kronig(g, z, ctx)
+    assert rel(y, K_ref @ z) <= 1e-13
+    # a non-square shape through the MFMA GEMM edges
+    g2 = FrequencyGrid(0.0, 0.01, np.arange(77))
+    z2 = pseudorandom(77, 3, seed=2)
+    assert rel(epsx.kramers_kronig(g2, z2, ctx), oracle.kramers_kronig_matrix(np.asarray(g2)) @ z2) <= 1e-13
+
+
+def kernel_case():
+    """The data of test/test_scgd0_kernel.jl:6-31."""
+    nm, ni, nf = 3, 13, 17
+    ωs = FrequencyGrid.from_nodes(np.concatenate([np.linspace(-1.0, -0.6, 21), np.linspace(-0.5, 0.5, 11),
+                                                  np.linspace(0.6, 1.0, 21)]))
+    nω = len(ωs)
+    Sin = 0.1 * pseudorandom(nω, ni, seed=7) + 1j * (-0.05 - 0.05 * np.abs(pseudorandom(nω, ni, seed=8)))
+    g2 = np.abs(pseudorandom(nm, ni, nf, seed=9))
+    wq = 0.05 + 0.2 * np.abs(pseudorandom(nm, ni, nf, seed=10))
+    w_ac = 1e-3
+    wq[1, 2, :] = 0.0
+    g2[1, 2, :] = np.inf
+    e_f = 0.8 * pseudorandom(nf, seed=11)
+    f_to_i = np.array([(5 * f) % ni + 1 for f in range(1, nf + 1)], dtype=np.int32)
+    w_f = np.full(nf, 1.0 / nf)
+    return dict(ωs=ωs, Sin=Sin, g2=g2, wq=wq, w_ac=w_ac, e_f=e_f, f_to_i=f_to_i, w_f=w_f, mu=0.07, T=0.03)
+
+
+def run_fm_device(ctx, c, T=None, precision=0):
+    from epsx import _lib
+    nm, ni, nf = c["g2"].shape
+    ws = c["ωs"]
+    ImS = np.full((len(ws), ni), 0.5, order="F")
+    g2 = np.asfortranarray(c["g2"])
+    wq = np.asfortranarray(c["wq"])
+    off = np.ascontiguousarray(ws.offsets)
+    Sin = np.asfortranarray(c["Sin"], dtype=np.complex128)
+    ctx.check(_lib.lib().epsx_add_fan_migdal_imsigma(
+        ctx.handle, nm, ni, nf, _lib.ptr(g2), _lib.ptr(wq), _lib.ptr(np.ascontiguousarray(c["e_f"])),
+        _lib.ptr(c["f_to_i"]), _lib.ptr(np.ascontiguousarray(c["w_f"])), len(ws), ws.ω_origin, ws.dω_unit,
+        _lib.ptr(off), Sin.shape[1], _lib.ptr(Sin), c["mu"], c["T"] if T is None else T, c["w_ac"],
+        precision, _lib.ptr(ImS)))
+    return ImS
+
+
+def run_fm_oracle(c, T=None):
+    nm, ni, nf = c["g2"].shape
+    ws = np.asarray(c["ωs"])
+    table = oracle.cell_search_table(ws, c["ωs"].dω_min)
+    wq_fi = np.asfortranarray(np.transpose(c["wq"], (0, 2, 1)))
+    ImS = np.full((len(ws), ni), 0.5, order="F")
+    return oracle.add_fan_migdal(ImS, c["g2"], wq_fi, c["e_f"], c["f_to_i"], c["w_f"], ws, table, c["Sin"],
+                                 c["mu"], c["T"] if T is None else T, c["w_ac"])
+
+
+def test_fan_migdal_kernel_fp64(ctx):
+    c = kernel_case()
+    ImS = run_fm_device(ctx, c)
+    ref = run_fm_oracle(c)
+    naive = naive_fan_migdal(np.full_like(ref, 0.5), np.asarray(c["ωs"]), c["Sin"], c["g2"], c["wq"], c["e_f"],
+                             c["f_to_i"], c["w_f"], c["mu"], c["T"], c["w_ac"])
+    assert np.all(np.isfinite(ImS))
+    assert rel(ImS, ref) <= 1e-13
+    assert rel(ImS, naive) <= 1e-13
+    # run to run: bitwise
+    assert np.array_equal(run_fm_device(ctx, c), ImS)
+
+
+def test_fan_migdal_kernel_step_temperature(ctx):
+    # T = 0: the step occupations of src/utils.jl:18-21 and :29-36
+    c = kernel_case()
+    ImS = run_fm_device(ctx, c, T=0.0)
+    ref = run_fm_oracle(c, T=0.0)
+    assert np.all(np.isfinite(ImS))
+    assert rel(ImS, ref) <= 1e-13
+
+
+def test_fan_migdal_kernel_fp32(ctx):
+    c = kernel_case()
+    ImS = run_fm_device(ctx, c, precision=1)
+    ref = run_fm_oracle(c)
+    # FP32 terms with a compensated sum over f; stated tolerance 1e-5 relative
+    assert rel(ImS - 0.5, ref - 0.5) <= 1e-5
+
+
+def holstein(nk=64, nodes=None, T=(0.1,), nlist=0.0, μ_start=None):
+    ωs = FrequencyGrid.from_nodes(np.arange(-200, 201) * 0.02 if nodes is None else nodes)
+    if μ_start is None:
+        μ_start = [holstein_bare_mu(nk=nk, t=1.0, T=Ti, n_target=1 + nlist, spin_degeneracy=2) for Ti in T]
+    return holstein_1d_setup(nk, 1.0, 0.2, 0.2, ωs, μ_start)
+
+
+def occ(T, nlist=0.0, typ="Metal", μlist=None):
+    return epsx.Occupation(Tlist=np.asarray(T, dtype=float), nlist=nlist, nelec=1.0, spin_degeneracy=2.0,
+                           type=typ, μlist=μlist)
+
+
+def run_both(s, o, **kw):
+    kw.setdefault("η_min", 0.0)
+    kw.setdefault("ω_acoustic", 1e-6)
+    dev_kw = {k: v for k, v in kw.items() if k not in ("semiconductor",)}
+    r = epsx.loop_scgd0(s, o, **dev_kw)
+    okw = {k: v for k, v in kw.items() if k not in ("method", "precision", "batched")}
+    ref = oracle.loop_scgd0(s, o.Tlist, o.nlist, o.nelec, semiconductor=(o.type == "Semiconductor"),
+                            spin_degeneracy=o.spin_degeneracy, **okw)
+    return r, ref
+
+
+@pytest.mark.parametrize("nlist", [0.0, -0.3])
+def test_loop_holstein_self_consistent(ctx, nlist):
+    s = holstein(nlist=nlist)
+    r, ref = run_both(s, occ([0.1], nlist), η_init=0.05, maxiter=30, tol=1e-6)
+    assert r["converged"].all() and ref["converged"].all()
+    assert r["history"]["err"].shape == ref["history"]["err"].shape
+    assert rel(r["Σ"], ref["Σ"]) <= 1e-10
+    assert abs(r["μ"][0] - ref["μ"][0]) <= 1e-10 * max(1.0, abs(ref["μ"][0]))
+    assert np.allclose(r["history"]["min_spectral_weight"], ref["history"]["min_spectral_weight"], rtol=1e-10)
+    # k-independence of the Holstein Σ (test/test_scgd0_holstein.jl:39)
+    assert np.max(np.abs(r["Σ"] - r["Σ"][:, :1, :])) < 1e-12
+
+
+def test_loop_two_temperatures_and_batching(ctx):
+    s = holstein(T=(0.1, 0.2))
+    o = occ([0.1, 0.2])
+    r, ref = run_both(s, o, η_init=0.05, maxiter=30, tol=1e-6)
+    assert rel(r["Σ"], ref["Σ"]) <= 1e-10
+    assert np.array_equal(r["history"]["iter_frozen"], ref["history"]["iter_frozen"])
+    rb = epsx.loop_scgd0(s, o, η_min=0.0, η_init=0.05, maxiter=30, tol=1e-6, batched=True)
+    assert rel(rb["Σ"], r["Σ"]) <= 1e-12
+    assert np.array_equal(rb["history"]["iter_frozen"], r["history"]["iter_frozen"])
+    # each temperature alone (test/test_scgd0_holstein.jl:77-85)
+    for iT, T in enumerate((0.1, 0.2)):
+        s1 = holstein(T=(T,))
+        r1 = epsx.loop_scgd0(s1, occ([T]), η_min=0.0, η_init=0.05, maxiter=30, tol=1e-6)
+        assert rel(r["Σ"][:, :, iT], r1["Σ"][:, :, 0]) <= 1e-12
+        assert abs(r["μ"][iT] - r1["μ"][0]) < 1e-12
+
+
+def test_loop_one_shot_fixed_mu_and_tiling(ctx):
+    s = holstein(μ_start=[0.3], nlist=-0.3)
+    o = occ([0.1], -0.3, μlist=[0.3])
+    r, ref = run_both(s, o, η_init=0.05, maxiter=1, tol=1e-6, fix_μ=True)
+    assert rel(r["Σ"], ref["Σ"]) <= 1e-12
+    assert r["μ"][0] == 0.3 and not r["converged"][0]
+    rt = epsx.loop_scgd0(s, o, η_min=0.0, η_init=0.05, maxiter=1, tol=1e-6, fix_μ=True, gpu_tile=7)
+    assert rel(rt["Σ"], r["Σ"]) <= 1e-12
+
+
+def test_loop_restart_converges_in_one(ctx):
+    s = holstein()
+    o = occ([0.1])
+    r = epsx.loop_scgd0(s, o, η_min=0.0, η_init=0.05, maxiter=30, tol=1e-6)
+    r2 = epsx.loop_scgd0(s, o, η_min=0.0, η_init=0.05, maxiter=30, tol=1e-6, Σ_init=r["Σ"], μ_start=r["μ"])
+    assert r2["history"]["err"].shape[1] == 1 and r2["converged"].all()
+    assert np.max(np.abs(r2["Σ"] - r["Σ"])) < 1e-6
+    assert abs(r2["μ"][0] - r["μ"][0]) < 1e-6
+
+
+def test_loop_maxiter_no_root_and_multiplets(ctx):
+    s = holstein()
+    with pytest.warns(UserWarning, match="not converged"):
+        r = epsx.loop_scgd0(s, occ([0.1]), η_min=0.0, η_init=0.05, maxiter=2, tol=1e-6)
+    assert not r["converged"][0] and r["history"]["iter_frozen"][0] == 0 and r["history"]["err"].shape[1] == 2
+    with pytest.raises(epsx.ArgumentError, match="no chemical potential"):
+        epsx.loop_scgd0(s, occ([0.1], 1.5), η_min=0.0, η_init=0.05, maxiter=20, tol=1e-6)
+    # multiplet average (test/test_scgd0_loop.jl:25-37)
+    import dataclasses
+    nk = 64
+    g2v = np.array([[0.2 * (1 + 0.1 * i) for _ in range(nk)] for i in range(1, nk // 2 + 2)]).reshape(1, nk // 2 + 1, nk)
+    sv = dataclasses.replace(s, g2=np.asfortranarray(g2v))
+    kw = dict(η_min=0.0, η_init=0.05, maxiter=1, tol=1e-6, fix_μ=True)
+    o = occ([0.1], μlist=[0.0])
+    ra = epsx.loop_scgd0(sv, o, **kw)
+    rb = epsx.loop_scgd0(dataclasses.replace(sv, multiplets=[[2, 3]]), o, **kw)
+    assert np.linalg.norm(ra["Σ"][:, 1, 0] - ra["Σ"][:, 2, 0]) > 1e-3 * np.linalg.norm(ra["Σ"][:, 1, 0])
+    assert np.array_equal(rb["Σ"][:, 1, 0], rb["Σ"][:, 2, 0])
+    assert rel(rb["Σ"][:, 1, 0], (ra["Σ"][:, 1, 0] + ra["Σ"][:, 2, 0]) / 2) <= 1e-14
+    keep = [0] + list(range(3, nk // 2 + 1))
+    assert np.array_equal(rb["Σ"][:, keep, :], ra["Σ"][:, keep, :])
+    ref = oracle.loop_scgd0(dataclasses.replace(sv, multiplets=[[2, 3]]), [0.1], [0.0], 1.0, η_min=0.0, η_init=0.05,
+                            maxiter=1, tol=1e-6, ω_acoustic=1e-6, fix_μ=True, μ_start=[0.0])
+    assert rel(rb["Σ"], ref["Σ"]) <= 1e-12
+
+
+def test_loop_run_to_run_bitwise(ctx):
+    s = holstein(T=(0.1, 0.2))
+    o = occ([0.1, 0.2])
+    a = epsx.loop_scgd0(s, o, η_min=0.0, η_init=0.05, maxiter=4, tol=0.0)
+    b = epsx.loop_scgd0(s, o, η_min=0.0, η_init=0.05, maxiter=4, tol=0.0)
+    assert np.array_equal(a["Σ"], b["Σ"]) and np.array_equal(a["μ"], b["μ"])
+
+
+def test_loop_holstein_c1(ctx):
+    """C1: nk = 128, ωs = -4:0.005:4 (1601 nodes), η_init = 0.01, T = 0.1 and 0.3, tol 1e-8."""
+    T = (0.1, 0.3)
+    s = holstein(nk=128, nodes=np.arange(-800, 801) * 0.005, T=T)
+    r, ref = run_both(s, occ(list(T)), η_init=0.01, maxiter=100, tol=1e-8)
+    assert r["converged"].all() and ref["converged"].all()
+    assert np.array_equal(r["history"]["iter_frozen"], ref["history"]["iter_frozen"])
+    assert rel(r["Σ"], ref["Σ"]) <= 1e-10
+    assert np.all(np.abs(r["μ"] - ref["μ"]) <= 1e-10 * np.maximum(1.0, np.abs(ref["μ"])))
+
+
+def test_loop_t2g_small_vs_oracle(ctx):
+    """The synthetic t2g model at nk = 8 (multiplets, 9 Einstein modes, graded grid) against the
+    oracle for 3 iterations with the μ update."""
+    s, meta = t2g_setup(nk=8, Tlist_K=(100.0, 300.0), grid_windows=((0.3, 4e-3), (1.2, 16e-3), (3.0, 64e-3)))
+    assert len(s.multiplets) > 0
+    o = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r, ref = run_both(s, o, η_init=5e-2, maxiter=3, tol=0.0, ω_acoustic=1e-6)
+    assert rel(r["Σ"], ref["Σ"]) <= 1e-10
+    assert np.all(np.abs(r["μ"] - ref["μ"]) <= 1e-10 * np.abs(ref["μ"]))
+    for m in s.multiplets:
+        for k in m[1:]:
+            assert np.array_equal(r["Σ"][:, k - 1, :], r["Σ"][:, m[0] - 1, :])
