@@ -18,8 +18,7 @@ from ref_numpy import pseudorandom, pseudorandom_complex, naive_fan_migdal, hols
 
 pytestmark = pytest.mark.gpu
 
-SRVO3_NODES = np.concatenate([np.arange(-0.35, -0.11 + 1e-9, 0.02), np.arange(-0.099, 0.100 + 1e-9, 0.001),
-                              np.arange(0.12, 0.34 + 1e-9, 0.02)])
+SRVO3_NODES = np.concatenate([np.arange(-350, -109, 20), np.arange(-99, 101), np.arange(120, 341, 20)]) / 1000.0  # Julia's correctly rounded ranges
 
 
 def rel(a, b):
