@@ -23,6 +23,7 @@ SOURCES = [
     "epsx_grids.cpp",
     "epsx_kk.hip",
     "epsx_fm.hip",
+    "epsx_fm_star.hip",
     "epsx_scgd0.hip",
 ]
 

@@ -46,9 +46,12 @@ struct Scgd0Dev {
     DevBuf KK;                        // [nw, nw]
     DevBuf mult_ptr, mult_idx;
     DevBuf star_ptr;                  // [ni + 1] sorted-position ranges of each star
-    DevBuf Gt;                        // star-collapsed coupling (built on demand)
+    DevBuf Gt;                        // star-collapsed coupling G̃ [(ν, i'), i] (built on demand)
+    DevBuf chunks, skip, wnu;         // star chunks of the reduction, acoustic flags, ω_ν
+    int64_t nchunks = 0;
     int64_t ltab_len = 0;
     bool gt_ready = false;
+    double gt_wac = 0;                // ω_acoustic G̃ was built with
 };
 
 }  // namespace epsx
@@ -59,8 +62,10 @@ struct epsx_ctx {
     std::string err;
     epsx::Scgd0Dev s;
     std::vector<epsx::DevBuf *> owned;
-    double t_class[4] = {0, 0, 0, 0};
-    int64_t n_class[4] = {0, 0, 0, 0};
+    // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
+    // 3 convergence + copies, 4 one-time coupling reduction (star method), 5-7 reserved
+    double t_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t n_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace epsx {
@@ -157,6 +162,14 @@ struct FmTemps {
 int fm_direct_launch_sp(epsx_ctx *ctx, const FmGrid &g, const FmDirectArgs &a, const FmTemps &tp,
                         const int64_t *d_star_ptr, int64_t f_tile, int precision, double *d_part,
                         int64_t part_cap, double *d_ImS, bool accumulate);
+// factorised (star) Fan-Migdal method, epsx_fm_star.hip
+void star_chunks(int64_t nm, const std::vector<int64_t> &star_ptr, std::vector<int32_t> &chunk);
+int star_reduce_launch(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nfs, const double *g2p, const double *wf,
+                       const int64_t *d_star_ptr, const int32_t *d_chunks, int64_t nchunks, const uint8_t *d_skip,
+                       double *Gt);
+int h_build_launch(epsx_ctx *ctx, const FmGrid &g, int64_t nm, int64_t ni, const double *wnu, const double *e_i,
+                   const double *S, const double *T, const double *mu, const double *eta, const int32_t *tidx,
+                   int nslots, double w_acoustic, double *hout);
 size_t fm_part_elems(int64_t nfs, int64_t ni, int64_t nw, int64_t nT, int64_t f_tile);
 int fm_grid_host(int64_t nw, const int64_t *offsets, std::vector<int32_t> &off,
                  std::vector<double> &rgap, std::vector<int32_t> &ltab, int64_t &span);

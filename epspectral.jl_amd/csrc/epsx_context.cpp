@@ -114,12 +114,12 @@ void epsx_close(epsx_ctx *ctx)
 
 const char *epsx_last_error(const epsx_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
 
-int epsx_last_timings(const epsx_ctx *ctx, double *seconds4, int64_t *launches4)
+int epsx_last_timings(const epsx_ctx *ctx, double *seconds8, int64_t *launches8)
 {
     if (!ctx) return EPSX_ERR_ARG;
-    for (int k = 0; k < 4; k++) {
-        if (seconds4) seconds4[k] = ctx->t_class[k];
-        if (launches4) launches4[k] = ctx->n_class[k];
+    for (int k = 0; k < 8; k++) {
+        if (seconds8) seconds8[k] = ctx->t_class[k];
+        if (launches8) launches8[k] = ctx->n_class[k];
     }
     return EPSX_OK;
 }

@@ -406,6 +406,13 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
             if (!(P->wph[nu + nm * q] == P->wph[nu])) s.einstein = false;
     }
     EPSX_TRY(upload(ctx, s.wph, P->wph, sizeof(double) * nm * P->nq));
+    EPSX_TRY(upload(ctx, s.wnu, s.wnu_host.data(), sizeof(double) * nm));
+    {
+        std::vector<int32_t> ch;
+        star_chunks(nm, sptr, ch);
+        s.nchunks = (int64_t)ch.size() - 1;
+        EPSX_TRY(upload(ctx, s.chunks, ch.data(), sizeof(int32_t) * ch.size()));
+    }
     EPSX_TRY(upload(ctx, s.iq_kk, P->iq_kk, sizeof(int32_t) * P->nki * P->nkf));
     EPSX_TRY(upload(ctx, s.iks_i, P->iks_i, sizeof(int32_t) * ni));
     EPSX_TRY(upload(ctx, s.iks_f, P->iks_f, sizeof(int32_t) * nf));
@@ -466,6 +473,15 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         return set_error(ctx, EPSX_ERR_ARG, "unknown Fan-Migdal method %d", par->fm_method);
     if (par->precision != EPSX_PREC_FP64 && par->precision != EPSX_PREC_FP32)
         return set_error(ctx, EPSX_ERR_ARG, "precision %d is not available for this method", par->precision);
+    // The Fan-Migdal method: the factorised forms only behind their preconditions.
+    int method = par->fm_method;
+    if (method == EPSX_FM_AUTO) method = (s.einstein && s.star_ok) ? EPSX_FM_STAR : EPSX_FM_DIRECT;
+    if (method == EPSX_FM_EINSTEIN) method = EPSX_FM_DIRECT;  // served by the direct kernel
+    if (method == EPSX_FM_STAR && !(s.einstein && s.star_ok))
+        return set_error(ctx, EPSX_ERR_ARG, "the star Fan-Migdal method needs dispersionless modes (ωph[ν, :] "
+                         "constant) and ε_f == ε_i of the representative for every inner state");
+    if (method == EPSX_FM_STAR && par->precision != EPSX_PREC_FP64)
+        return set_error(ctx, EPSX_ERR_ARG, "the star Fan-Migdal method runs in FP64 only");
     EPSX_HIP(ctx, hipSetDevice(ctx->device));
     const int64_t nw = s.nw, ni = s.ni, nwa = s.nwa, nf = s.nf;
     const size_t nS = (size_t)nw * ni * nT;
@@ -485,6 +501,34 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         npart = e > npart ? e : npart;
     }
     EPSX_TRY(tmp_alloc(ctx, B.part.b, 8 * npart));
+    Tmp hbuf;
+    if (method == EPSX_FM_STAR) {
+        EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
+        if (!s.gt_ready || s.gt_wac != par->w_acoustic) {
+            // G̃[(ν, i'), i] = Σ_{f in star(i')} w_f g2[ν, i, f], acoustic modes zeroed
+            std::vector<uint8_t> skip(s.nm);
+            for (int64_t nu = 0; nu < s.nm; nu++) skip[nu] = s.wnu_host[nu] < par->w_acoustic;
+            EPSX_TRY(upload(ctx, s.skip, skip.data(), s.nm));
+            EPSX_TRY(dev_alloc(ctx, s.Gt, 8 * (size_t)s.nm * ni * ni));
+            hipEvent_t e0, e1;
+            EPSX_HIP(ctx, hipEventCreate(&e0));
+            EPSX_HIP(ctx, hipEventCreate(&e1));
+            EPSX_HIP(ctx, hipEventRecord(e0, st));
+            EPSX_TRY(star_reduce_launch(ctx, s.nm, ni, nf, dptr<double>(s.g2p), dptr<double>(s.wf_s),
+                                        dptr<int64_t>(s.star_ptr), dptr<int32_t>(s.chunks), s.nchunks,
+                                        dptr<uint8_t>(s.skip), dptr<double>(s.Gt)));
+            EPSX_HIP(ctx, hipEventRecord(e1, st));
+            EPSX_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0;
+            EPSX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            s.gt_ready = true;
+            s.gt_wac = par->w_acoustic;
+            ctx->t_class[4] = ms * 1e-3;
+            ctx->n_class[4] = 1;
+        }
+    }
     const int nbmax = 512;
     EPSX_TRY(tmp_alloc(ctx, B.bmax.b, 8 * (size_t)nbmax * nT));
     EPSX_TRY(tmp_upload(ctx, B.T, par->Tlist, 8 * nT));
@@ -528,12 +572,13 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     for (int64_t q = 0; q < nT * maxiter; q++) out->hist_err[q] = out->hist_mu[q] = out->hist_minA[q] = NAN;
     for (int64_t t = 0; t < nT; t++) out->iter_frozen[t] = 0;
     for (int k = 0; k < 4; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
+    for (int k = 5; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
     hipEvent_t ev[8];
     for (auto &e : ev) EPSX_HIP(ctx, hipEventCreate(&e));
     struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int k = 0; k < 8; k++) (void)hipEventDestroy(e[k]); } } eg{ev};
 
     out->niter = 0;
-    out->fm_method_used = EPSX_FM_DIRECT;
+    out->fm_method_used = method;
     out->n_low_precision = 0;
     int rc = EPSX_OK;
     for (int64_t iter = 1; iter <= maxiter; iter++) {
@@ -547,7 +592,17 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         EPSX_HIP(ctx, hipEventRecord(ev[0], st));
         FmTemps tp;
         tp.T = B.T.ptr<double>(); tp.mu = B.mu.ptr<double>(); tp.eta_min = B.eta.ptr<double>();
-        if (par->batched) {
+        if (method == EPSX_FM_STAR) {
+            // h for every active temperature in one launch, then one GEMM per temperature
+            EPSX_TRY(h_build_launch(ctx, g, s.nm, ni, dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(),
+                                    B.T.ptr<double>(), B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA,
+                                    par->w_acoustic, hbuf.ptr<double>()));
+            for (int z = 0; z < nA; z++) {
+                const int64_t t = act_idx[z];
+                EPSX_TRY(dgemm_nn(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>() + nw * s.nm * ni * z, nw,
+                                  dptr<double>(s.Gt), s.nm * ni, B.ImS.ptr<double>() + nw * ni * t, nw, false));
+            }
+        } else if (par->batched) {
             tp.nT = nA;
             tp.tidx = d_tidx;
             EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile, par->precision,
@@ -624,7 +679,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             EPSX_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
             ctx->t_class[k == 0 ? 0 : (k == 1 ? 1 : (k == 2 ? 2 : 3))] += ms[k] * 1e-3;
         }
-        ctx->n_class[0] += par->batched ? 1 : nA;
+        ctx->n_class[0] += (par->batched || method == EPSX_FM_STAR) ? 1 : nA;
         ctx->n_class[1] += nA;
         for (int z = 0; z < nA; z++) {
             const int64_t t = act_idx[z];

@@ -170,11 +170,11 @@ class Context:
         raise DeviceError(f"libepsx error {rc}: {msg}")
 
     def timings(self):
-        s = np.zeros(4)
-        n = np.zeros(4, dtype=np.int64)
+        s = np.zeros(8)
+        n = np.zeros(8, dtype=np.int64)
         lib().epsx_last_timings(self.handle, ptr(s), ptr(n))
         return {"fan_migdal": s[0], "kramers_kronig": s[1], "dos_mu": s[2], "other": s[3],
-                "fan_migdal_launches": int(n[0])}
+                "star_reduce": s[4], "fan_migdal_launches": int(n[0]), "star_reduce_launches": int(n[4])}
 
 
 _default = None

@@ -164,3 +164,47 @@ def t2g_setup(nk=24, t=0.26, tp=0.03, nmodes=9, wmin=0.010, wmax=0.080, g2_media
                 mu_bare=mu_all, window=(lo, hi), ni=ni, nf=nf, seed=seed, canon=canon, inv=inv,
                 counts=counts, sel_i=sel_i, sel_f=sel_f)
     return setup, meta
+
+
+def holstein_ladder_grid():
+    """The ladder fixture grid of test/ladder_fixtures.jl:23: 0.02 cells inside ±1, 0.04 outside."""
+    nodes = np.concatenate([np.arange(-100, -24) * 0.04, np.arange(-49, 50) * 0.02, np.arange(25, 101) * 0.04])
+    return FrequencyGrid.from_nodes(np.round(nodes, 12))
+
+
+def holstein_ladder_setup(nk=16, ω0=0.173, t=1.0, λ=0.5, symmetry=False, ωs=None, μ_start=(0.0, 0.0)):
+    """The 1D Holstein ladder case of test/ladder_fixtures.jl:22-31 as (scGD0 setup, R, v).
+
+    symmetry=False: outer states are all nk states (f_to_i the identity, R = 1).  symmetry=True:
+    the k -> -k wedge, with R_f = -1 for the partner -k of a representative (the velocity is
+    odd under k -> -k).  g2 = 2 λ ω0 t (one dimension); v_x = 2t sin k."""
+    from .scgd0 import Scgd0Setup
+    ωs = holstein_ladder_grid() if ωs is None else ωs
+    g2 = 2.0 * λ * ω0 * t
+    if symmetry:
+        s = holstein_1d_setup(nk, t, ω0, g2, ωs, list(μ_start))
+        ni = nk // 2 + 1
+        R = np.zeros((3, 3, nk), order="F")
+        for f in range(nk):
+            sgn = 1.0 if f <= nk // 2 else -1.0
+            for a in range(3):
+                R[a, a, f] = sgn
+        js = np.arange(ni)
+    else:
+        εs = holstein_energies(nk, t)
+        j = np.arange(nk)
+        iq_kk = (np.mod(j[None, :] - j[:, None], nk) + 1).astype(np.int32)
+        s = Scgd0Setup(
+            g2=np.full((1, nk, nk), g2, order="F"), ωph=np.full((1, nk), ω0, order="F"),
+            iq_kk=np.asfortranarray(iq_kk), iks_i=np.arange(1, nk + 1, dtype=np.int32),
+            iks_f=np.arange(1, nk + 1, dtype=np.int32), ε_i=εs.copy(), w_i=np.full(nk, 1.0 / nk), ε_f=εs.copy(),
+            w_f=np.full(nk, 1.0 / nk), f_to_i=np.arange(1, nk + 1, dtype=np.int32), W=1.0, nstates_base=0.0,
+            multiplets=[], ωs=ωs, ωs_spectral=uniform_spectral_grid(ωs),
+            μ_start=np.atleast_1d(np.asarray(μ_start, dtype=np.float64)))
+        R = np.zeros((3, 3, nk), order="F")
+        for a in range(3):
+            R[a, a, :] = 1.0
+        js = np.arange(nk)
+    v = np.zeros((3, len(js)), order="F")
+    v[0] = 2.0 * t * np.sin(2.0 * np.pi * js / nk)
+    return s, R, v

@@ -44,9 +44,11 @@ int epsx_abi_version(void);
 int epsx_open(int device, epsx_ctx **ctx);
 void epsx_close(epsx_ctx *ctx);
 const char *epsx_last_error(const epsx_ctx *ctx);
-/* Wall-clock seconds of the last run's kernels, by kernel class (FM, KK, DOS+μ, other),
- * measured with HIP events on the context's stream. */
-int epsx_last_timings(const epsx_ctx *ctx, double *seconds4, int64_t *launches4);
+/* Seconds of the last run's kernels by class, measured with HIP events on the context's
+ * stream: [0] Fan-Migdal sum, [1] Kramers-Kronig, [2] spectral DOS and μ, [3] convergence and
+ * copies, [4] the one-time coupling reduction of the star method, [5..7] reserved; and the
+ * number of launches of each class.  Both arrays hold 8 entries. */
+int epsx_last_timings(const epsx_ctx *ctx, double *seconds8, int64_t *launches8);
 
 /* ---- grid maps (host, bit-exact with the reference) ---------------------------------- */
 /* cell_search_table, src/frequency_grids.jl:184-196.  ws: the nodes, dw_min = ωs.dω_min.

@@ -256,3 +256,139 @@ def loop_scgd0(setup, Tlist, nlist, nelec, *, η_min, η_init, maxiter, tol, ω_
     return dict(Σ=Σ, μ=μ, converged=conv.astype(bool),
                 history=dict(err=he[:, :n].copy(), μ=hm[:, :n].copy(), min_spectral_weight=ha[:, :n].copy(),
                              wall=wall[:n].copy(), iter_frozen=fr.copy()))
+
+
+# ------------------------------------------------------------------------------ ladder
+def _ladder_sigs():
+    L = lib()
+    if getattr(L, "_ladder_ready", False):
+        return L
+    sig = {
+        "orc_contour_green": (None, [D, D, D, D, D, D, C.c_int, P]),
+        "orc_contour_greens": (None, [I64, I64, P, P, P, P, D, D, D, P, P]),
+        "orc_attach_greens": (None, [I64, I64, P, P, P, P]),
+        "orc_attach_greens_bare": (None, [I64, I64, P, P, P, P]),
+        "orc_conductivity_from_vertex": (None, [I64, I64, P, P, P, P, D, D, P]),
+        "orc_vertex_shift_tables": (None, [I64, P, I64, P, P, P]),
+        "orc_apply_ladder_kernel": (C.c_int, [P, P, P, P, D, D, I64, P, C.c_int]),
+        "orc_ladder_solve": (C.c_int, [P, P, P, P, P, P, D, D, D, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    L._ladder_ready = True
+    return L
+
+
+def _c128(a):
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def contour_green(w, e, S, mu, T, c12):
+    out = np.zeros(2)
+    _ladder_sigs().orc_contour_green(w, e, S.real, S.imag, mu, T, c12, _p(out))
+    return complex(out[0], out[1])
+
+
+def contour_greens(ws, e_i, Sd_m, Sd_p, Omega, mu, T):
+    ws = _f64(ws)
+    nw, ni = len(ws), len(e_i)
+    Gm = np.zeros((nw, 4, ni), dtype=np.complex128, order="F")
+    Gp = np.zeros_like(Gm)
+    _ladder_sigs().orc_contour_greens(nw, ni, _p(ws), _p(_f64(e_i)), _p(_c128(Sd_m)), _p(_c128(Sd_p)), Omega, mu, T,
+                                      _p(Gm), _p(Gp))
+    return Gm, Gp
+
+
+def attach_greens(Gm, X, Gp):
+    nw, _, ni = Gm.shape
+    J = np.zeros((nw, 4, 3, 2, ni), dtype=np.complex128, order="F")
+    _ladder_sigs().orc_attach_greens(nw, ni, _p(_c128(Gm)), _p(_c128(X)), _p(_c128(Gp)), _p(J))
+    return J
+
+
+def attach_greens_bare(Gm, v, Gp):
+    nw, _, ni = Gm.shape
+    J = np.zeros((nw, 4, 3, 2, ni), dtype=np.complex128, order="F")
+    _ladder_sigs().orc_attach_greens_bare(nw, ni, _p(_c128(Gm)), _p(_f64(v)), _p(_c128(Gp)), _p(J))
+    return J
+
+
+def conductivity_from_vertex(J, v, w_i, Ww, Omega, T):
+    nw = J.shape[0]
+    ni = J.shape[4]
+    s = np.zeros((3, 3), order="F")
+    _ladder_sigs().orc_conductivity_from_vertex(nw, ni, _p(_c128(J)), _p(_f64(v)), _p(_f64(w_i)), _p(_f64(Ww)),
+                                                Omega, T, _p(s))
+    return s
+
+
+def vertex_shift_tables(offsets, nshift):
+    off = _i64(offsets)
+    nw = len(off)
+    klo = np.zeros((nw, nshift, 2), dtype=np.int32, order="F")
+    wlo = np.zeros((nw, nshift, 2), order="F")
+    whi = np.zeros((nw, nshift, 2), order="F")
+    _ladder_sigs().orc_vertex_shift_tables(nw, _p(off), nshift, _p(klo), _p(wlo), _p(whi))
+    return klo, wlo, whi
+
+
+class _LSetup(C.Structure):
+    _fields_ = [("nm", I64), ("ni", I64), ("nf", I64), ("nw", I64), ("nq", I64), ("nki", I64), ("nkf", I64),
+                ("nshift", I64), ("g2", P), ("wph", P), ("iq_kk", P), ("iks_i", P), ("iks_f", P), ("f_to_i", P),
+                ("w_f", P), ("R", P), ("ws", P), ("offsets", P), ("dw_unit", D), ("KK", P)]
+
+
+class _LParams(C.Structure):
+    _fields_ = [("solver", C.c_int), ("rtol", D), ("maxiter", I64), ("warm", C.c_int), ("gpu_tile", I64),
+                ("w_acoustic", D), ("nthreads", C.c_int)]
+
+
+class _LOut(C.Structure):
+    _fields_ = [("converged", I32), ("niter", I64), ("n_norms", I64), ("norms", P), ("sigma_iter", P),
+                ("sigma_bubble", D * 9), ("sigma_ladder", D * 9)]
+
+
+def _lsetup(ld):
+    """ld: an object with the fields of the reference's ladder setup (g2, ωph, iq_kk, iks_i, iks_f,
+    f_to_i, w_f, R, ωs (FrequencyGrid), nshift, KK)."""
+    nm, ni, nf = ld.g2.shape
+    ws = np.asarray(ld.ωs)
+    keep = dict(g2=_f64(ld.g2), wph=_f64(ld.ωph), iq=_i32(ld.iq_kk), iki=_i32(ld.iks_i), ikf=_i32(ld.iks_f),
+                fi=_i32(ld.f_to_i), wf=_f64(ld.w_f), R=_f64(ld.R), ws=_f64(ws), off=_i64(ld.ωs.offsets),
+                KK=_f64(ld.KK if getattr(ld, "KK", None) is not None else kramers_kronig_matrix(ws)))
+    S = _LSetup(nm, ni, nf, len(ws), keep["wph"].shape[1], keep["iq"].shape[0], keep["iq"].shape[1], int(ld.nshift),
+                _p(keep["g2"]), _p(keep["wph"]), _p(keep["iq"]), _p(keep["iki"]), _p(keep["ikf"]), _p(keep["fi"]),
+                _p(keep["wf"]), _p(keep["R"]), _p(keep["ws"]), _p(keep["off"]), float(ld.ωs.dω_unit), _p(keep["KK"]))
+    return S, keep
+
+
+def apply_ladder_kernel(ld, J, Gm, Gp, T, w_acoustic, gpu_tile=None, nthreads=8):
+    S, keep = _lsetup(ld)
+    LJ = np.zeros(J.shape, dtype=np.complex128, order="F")
+    _ladder_sigs().orc_apply_ladder_kernel(C.byref(S), _p(_c128(J)), _p(_c128(Gm)), _p(_c128(Gp)), T, w_acoustic,
+                                           0 if gpu_tile is None else int(gpu_tile), _p(LJ), nthreads)
+    return LJ
+
+
+def ladder_solve(ld, Sigma_T, mu, T, Omega, *, solver="gmres", rtol=1e-2, maxiter=30, J=None, gpu_tile=None,
+                 w_acoustic=1e-6, nthreads=8):
+    """One (T, Ω) of `_loop_ladder` (src/ladder.jl:327-371).  J (optional) is the warm start;
+    returns dict(J, σ_bubble, σ_ladder, converged, niter, norms, σ_iter)."""
+    S, keep = _lsetup(ld)
+    nw, ni = len(ld.ωs), ld.g2.shape[1]
+    warm = J is not None
+    Jw = _c128(J).copy(order="F") if warm else np.zeros((nw, 4, 3, 2, ni), dtype=np.complex128, order="F")
+    norms = np.zeros(maxiter + 2)
+    sit = np.zeros(9 * maxiter)
+    O = _LOut(0, 0, 0, _p(norms), _p(sit), (D * 9)(), (D * 9)())
+    Pm = _LParams(0 if solver == "gmres" else 1, float(rtol), int(maxiter), int(warm),
+                  0 if gpu_tile is None else int(gpu_tile), float(w_acoustic), int(nthreads))
+    Ww = trapezoid_weights(np.asarray(ld.ωs))
+    _ladder_sigs().orc_ladder_solve(C.byref(S), _p(_c128(Sigma_T)), _p(_f64(ld.ε_i)), _p(_f64(ld.w_i)), _p(_f64(ld.v)),
+                                    _p(_f64(Ww)), float(Omega), float(mu), float(T), C.byref(Pm), _p(Jw), C.byref(O))
+    return dict(J=Jw, σ_bubble=np.array(O.sigma_bubble[:]).reshape(3, 3, order="F"),
+                σ_ladder=np.array(O.sigma_ladder[:]).reshape(3, 3, order="F"), converged=bool(O.converged),
+                niter=int(O.niter), norms=norms[:O.n_norms].copy(),
+                σ_iter=[sit[9 * k:9 * k + 9].reshape(3, 3, order="F").copy() for k in range(O.niter)] if solver != "gmres" else [])

@@ -150,3 +150,83 @@ def n_old(μ, T, wsa, Ssp, ε_i, w_i, nstates_base, spin_degeneracy, count_holes
         else:
             nocc += so * w_i[i]
     return (nocc + nbase + nstates_base) * spin_degeneracy
+
+
+def hat_value(y, ws, x):
+    """Linear interpolation of y on ws, zero outside (test/test_ladder_kernel.jl:6-11); vector x."""
+    ws = np.asarray(ws)
+    x = np.asarray(x, dtype=float)
+    out = np.zeros(x.shape, dtype=np.result_type(y, np.float64))
+    inside = (x >= ws[0]) & (x <= ws[-1])
+    k = np.minimum(np.searchsorted(ws, x[inside], side="right") - 1, len(ws) - 2)
+    t = (x[inside] - ws[k]) / (ws[k + 1] - ws[k])
+    out[inside] = (1 - t) * y[k] + t * y[k + 1]
+    return out
+
+
+def naive_apply_ladder(g2, wph, iq_kk, iks_i, iks_f, w_f, f_to_i, R, ws, KK, J, Gm, Gp, T, w_acoustic):
+    """L(J) = -Gm (s ⊙ K[J]) Gp by loops, with the channel sums of apply_phonon_exchange_kernel!
+    (test/test_ladder_kernel.jl:14-54): the shifted vertex by exact hat interpolation at
+    ωs[j] ± ωq, no shift grid."""
+    nm, ni, nf = g2.shape
+    ws = np.asarray(ws)
+    nw = len(ws)
+    F = np.zeros((nw, 4, 3, 2, nf), dtype=complex)
+    for f in range(nf):
+        for c0 in range(2):
+            for a in range(3):
+                for c12 in range(4):
+                    F[:, c12, a, c0, f] = sum(R[a, b, f] * J[:, c12, b, c0, f_to_i[f] - 1] for b in range(3))
+    Up, Um, Vp, Vm = (np.zeros((nw, 4, 3, 2, ni), dtype=complex) for _ in range(4))
+    for i in range(ni):
+        for f in range(nf):
+            q = iq_kk[iks_i[i] - 1, iks_f[f] - 1] - 1
+            for nu in range(nm):
+                om = wph[nu, q]
+                if om < w_acoustic:
+                    continue
+                w = w_f[f] * g2[nu, i, f]
+                n = 1 / math.expm1(om / T)
+                for c0 in range(2):
+                    for a in range(3):
+                        for c12 in range(4):
+                            y = F[:, c12, a, c0, f]
+                            up = w * hat_value(y, ws, ws + om)
+                            um = w * hat_value(y, ws, ws - om)
+                            Up[:, c12, a, c0, i] += up
+                            Um[:, c12, a, c0, i] += um
+                            Vp[:, c12, a, c0, i] += n * up
+                            Vm[:, c12, a, c0, i] += n * um
+    KY = (KK @ ((Up - Um) / 2).reshape(nw, -1, order="F")).reshape(Up.shape, order="F")
+    LJ = np.zeros((nw, 4, 3, 2, ni), dtype=complex)
+    s = np.array([[1, -1], [-1, 1]])
+    for i in range(ni):
+        for c0 in range(2):
+            for a in range(3):
+                k11 = -1j * KY[:, 0, a, c0, i] - (Up[:, 0, a, c0, i] + Um[:, 0, a, c0, i]) / 2 - (Vp[:, 0, a, c0, i] + Vm[:, 0, a, c0, i])
+                k22 = 1j * KY[:, 3, a, c0, i] - (Up[:, 3, a, c0, i] + Um[:, 3, a, c0, i]) / 2 - (Vp[:, 3, a, c0, i] + Vm[:, 3, a, c0, i])
+                k21 = -(Vp[:, 1, a, c0, i] + Um[:, 1, a, c0, i] + Vm[:, 1, a, c0, i])
+                k12 = -(Up[:, 2, a, c0, i] + Vp[:, 2, a, c0, i] + Vm[:, 2, a, c0, i])
+                for j in range(nw):
+                    Kab = s * np.array([[k11[j], k12[j]], [k21[j], k22[j]]])
+                    gm = Gm[j, :, i].reshape(2, 2, order="F")
+                    gp = Gp[j, :, i].reshape(2, 2, order="F")
+                    LJ[j, :, a, c0, i] = -(gm @ Kab @ gp).reshape(4, order="F")
+    return LJ
+
+
+def spectral_bubble(ws, Σ, e_i, w_i, v, mu, T, Ω):
+    """σ_ab = π Σ_i w_i v_a v_b Σ_ε Wω A_i(ε-Ω/2) A_i(ε+Ω/2) (f(ε-Ω/2-μ) - f(ε+Ω/2-μ))/Ω, with
+    f(1-f)/T at Ω = 0 (test/test_ladder_bubble.jl:10-26); Σ [nω, ni] read by numpy.interp."""
+    ws = np.asarray(ws)
+    Ww = trapezoid(ws)
+    σ = np.zeros((3, 3))
+    for i in range(len(e_i)):
+        A = lambda x: -np.imag(1 / (x - e_i[i] - interp_flat(x, ws, Σ[:, i]))) / math.pi
+        if Ω == 0:
+            fac = fermi(ws - mu, T) * (1 - fermi(ws - mu, T)) / T
+        else:
+            fac = (fermi(ws - Ω / 2 - mu, T) - fermi(ws + Ω / 2 - mu, T)) / Ω
+        win = np.sum(Ww * A(ws - Ω / 2) * A(ws + Ω / 2) * fac)
+        σ += math.pi * w_i[i] * win * np.outer(v[:, i], v[:, i])
+    return σ

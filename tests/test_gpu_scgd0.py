@@ -246,3 +246,32 @@ def test_loop_t2g_small_vs_oracle(ctx):
     for m in s.multiplets:
         for k in m[1:]:
             assert np.array_equal(r["Σ"][:, k - 1, :], r["Σ"][:, m[0] - 1, :])
+
+
+def test_star_method_matches_direct_and_oracle(ctx):
+    """The factorised Fan-Migdal method (Einstein modes, ε_f == ε_{i_f}) against the direct kernel
+    and the oracle on the t2g model at nk = 8 (multiplets, 9 modes, graded grid)."""
+    s, meta = t2g_setup(nk=8, Tlist_K=(100.0, 300.0), grid_windows=((0.3, 4e-3), (1.2, 16e-3), (3.0, 64e-3)))
+    o = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    kw = dict(η_min=0.0, η_init=5e-2, maxiter=3, tol=0.0, ω_acoustic=1e-6)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dev = epsx.Scgd0Device(s, ctx)
+        rd = dev.loop(o, method="direct", **kw)
+        rs = dev.loop(o, method="star", **kw)
+        ra = dev.loop(o, method="auto", batched=True, **kw)
+        ref = oracle.loop_scgd0(s, o.Tlist, o.nlist, 1.0, **kw)
+    assert rs["info"]["fm_method"] == 2 and ra["info"]["fm_method"] == 2 and rd["info"]["fm_method"] == 0
+    assert rel(rs["Σ"], rd["Σ"]) <= 1e-12
+    assert rel(rs["Σ"], ref["Σ"]) <= 1e-10
+    assert np.all(np.abs(rs["μ"] - ref["μ"]) <= 1e-10 * np.abs(ref["μ"]))
+    assert np.array_equal(ra["Σ"], rs["Σ"])
+    # the precondition is checked: a dispersive mode refuses the star method
+    import dataclasses
+    s2 = dataclasses.replace(s, ωph=np.asfortranarray(np.hstack([s.ωph, s.ωph * 1.01])),
+                             iq_kk=np.asfortranarray(np.array([[1, 2]], dtype=np.int32)),
+                             iks_f=np.asfortranarray(1 + (np.arange(len(s.ε_f)) % 2).astype(np.int32)))
+    with pytest.raises(epsx.ArgumentError, match="star"):
+        epsx.loop_scgd0(s2, o, method="star", ctx=ctx, **kw)
+    r2 = epsx.loop_scgd0(s2, o, method="auto", ctx=ctx, **kw)
+    assert r2["info"]["fm_method"] == 0
