@@ -109,13 +109,18 @@ def test_shift_regrouping_and_tables():
 
 
 def disordered_case():
-    """A case with a nonzero vertex correction: the Holstein chain without symmetry, two modes (one
-    below the shift step, one spanning several shifts) and a coupling that varies with (i, f)."""
+    """A case with a large vertex correction: the Holstein chain without symmetry, two modes (one
+    spanning several shift steps, one below the step) and a forward-scattering coupling that
+    peaks at small |k_f - k_i|, with a pseudorandom modulation."""
     s, R, v = holstein_ladder_setup(symmetry=False)
     nk = 16
-    s.ωph = np.asfortranarray(np.vstack([np.full(nk, 0.173), np.full(nk, 0.011)]))
-    s.g2 = np.asfortranarray(0.1 + 0.1 * np.abs(pseudorandom(2, nk, nk, seed=2.7)))
+    s.ωph = np.asfortranarray(np.vstack([np.full(nk, 0.173), np.full(nk, 0.015)]))
+    j = np.arange(nk)
+    d = np.minimum(np.abs(j[:, None] - j[None, :]), nk - np.abs(j[:, None] - j[None, :]))
+    fw = 0.1 * np.exp(-(d / 1.5) ** 2)
+    s.g2 = np.asfortranarray(np.stack([0.02 + fw, 0.01 + 0.3 * fw]) * (1 + 0.2 * pseudorandom(2, nk, nk, seed=2.7)))
     r = oracle.loop_scgd0(s, [0.1, 0.2], 0.0, 1.0, η_min=0.0, η_init=0.05, tol=1e-6, maxiter=100, ω_acoustic=1e-6)
+    assert r["converged"].all()
     return s, setup_ladder(s, R, v), r
 
 
@@ -162,7 +167,7 @@ def test_solvers_satisfy_the_equation():
     """test/test_ladder_pb.jl:11-22, :48-57 on the disordered case: a converged GMRES solve
     satisfies J = J0 + L(J) to 10 rtol, the fixed point agrees, the correction is not small."""
     s, L, r = disordered_case()
-    T, Ω = 0.1, 0.05
+    T, Ω = 0.1, 0.3
     g = oracle.ladder_solve(L, r["Σ"][:, :, 0], r["μ"][0], T, Ω, rtol=1e-8, maxiter=60)
     assert g["converged"]
     Gm, Gp = propagators(L, r, 0, Ω, T)
