@@ -25,6 +25,7 @@ SOURCES = [
     "epsx_fm.hip",
     "epsx_fm_star.hip",
     "epsx_scgd0.hip",
+    "epsx_ladder.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

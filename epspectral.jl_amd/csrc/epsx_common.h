@@ -30,7 +30,7 @@ struct Scgd0Dev {
     // grid of ωs: relative integer offsets (from offsets[0]) and the node values
     double w_origin = 0, dw_unit = 0, dw_min = 0;
     int64_t span = 0;                 // offsets[n-1] - offsets[0]
-    std::vector<double> ws_host, wsa_host;
+    std::vector<double> ws_host, wsa_host, w_i_host;
     bool einstein = false;            // ωph[ν, :] constant for every ν
     bool star_ok = false;             // e_f == e_{f_to_i[f]} bitwise
     std::vector<double> wnu_host;     // [nm] mode frequency when einstein
@@ -54,6 +54,20 @@ struct Scgd0Dev {
     double gt_wac = 0;                // ω_acoustic G̃ was built with
 };
 
+// Device-resident ladder problem on top of the scGD0 problem (epsx_ladder_set_problem).
+struct LadderDev {
+    bool ready = false;
+    int64_t ns = 0;                   // shift tables of step dω_unit
+    double h = 0;
+    std::vector<double> v_host;       // [3, ni]
+    DevBuf Rs, v, Ww, klo, wlo, whi;  // rotations in star order, velocities, weights, shift tables
+    DevBuf Bt;                        // B̃ of the current temperature
+    double bt_T = -1, bt_wac = -1;
+    int32_t bt_prec = -1;
+    // work buffers
+    DevBuf Gm, Gp, Jm, W, Cp, Cm, Y, KY, S, jm, tm, jp, tp, Yc, red, Bt32, Jm32;
+};
+
 }  // namespace epsx
 
 struct epsx_ctx {
@@ -61,6 +75,7 @@ struct epsx_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     epsx::Scgd0Dev s;
+    epsx::LadderDev l;
     std::vector<epsx::DevBuf *> owned;
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 one-time coupling reduction (star method), 5-7 reserved

@@ -1,0 +1,801 @@
+// The ladder (vertex-corrected Kubo conductivity) on the device: src/ladder_kernel.jl and the
+// application / solve loop of src/ladder.jl:245-379.
+//
+// The shift sums of apply_ladder_kernel! are reorganised exactly.  The reference forms, for each
+// sign s, C_s = X_s B̃ with X_s[(j, q), (m, b, ip)] = (S_{s,m} J_ip^b)[j, q] the shifted vertex
+// (q = (c12, c0, re/im)).  The shift S_{s,m} acts on the frequency index only (two taps,
+// wlo/whi at klo), so with W_m = J B̃_m, where J[(j', q), (b, ip)] is the vertex itself,
+//
+//   C_s[(j, q), col] = Σ_m wlo[j,m,s] W_m[(klo[j,m,s], q), col] + whi[j,m,s] W_m[(klo+1, q), col].
+//
+// W_m does not depend on the sign, so both signs share one GEMM of
+// (16 nω) x (3 ni) x (6 ni nshift) -- half the reference's flops, no gather in the operand --
+// followed by a two-tap shift-add.  B̃ depends only on T and is built once per temperature.
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "epsx_common.h"
+
+namespace epsx {
+namespace {
+
+constexpr double SQRT_EPS_L = 1.4901161193847656e-08;
+
+__device__ inline double occ_fermion_l(double e, double T)
+{
+    if (T > SQRT_EPS_L) return 1.0 / (exp(e / T) + 1.0);
+    const double s = e > 0 ? 1.0 : (e < 0 ? -1.0 : 0.0);
+    return (1.0 - s) / 2.0;
+}
+
+__device__ inline double occ_boson_l(double e, double T)
+{
+    if (T > SQRT_EPS_L) return e == 0.0 ? -0.5 : 1.0 / expm1(e / T);
+    return e > 0 ? 0.0 : (e == 0 ? -0.5 : -1.0);
+}
+
+// Julia's inv(::ComplexF64) (Smith's algorithm) for finite inputs
+__device__ inline double2 jinv(double c, double d)
+{
+    if (fabs(d) <= fabs(c)) {
+        const double r = d / c, z = fma(d, r, c);
+        return make_double2(1.0 / z, -r / z);
+    }
+    const double r = c / d, z = fma(-c, r, -d);
+    return make_double2(-r / z, 1.0 / z);
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+// Gm[ω, c12, i] = G(ωs[ω] - Ω/2), Gp = G(ωs[ω] + Ω/2) with Σ read at ωs ∓ Ω/2 through the host
+// tables (src/ladder.jl:322-334; src/ladder_kernel.jl:27-57).
+__global__ void greens_kernel(int64_t nw, int64_t ni, const double *__restrict__ ws, const double *__restrict__ e_i,
+                              const double2 *__restrict__ S, const int32_t *__restrict__ jm, const double *__restrict__ tm,
+                              const int32_t *__restrict__ jp, const double *__restrict__ tp, double Omega, double mu,
+                              double T, double2 *__restrict__ Gm, double2 *__restrict__ Gp)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nw * ni) return;
+    const int64_t w = q % nw, i = q / nw;
+    const double2 *col = S + nw * i;
+    for (int side = 0; side < 2; side++) {
+        const int64_t j = (side == 0 ? jm[w] : jp[w]) - 1;
+        const double t = side == 0 ? tm[w] : tp[w];
+        const double sr = (1 - t) * col[j].x + t * col[j + 1].x, si = (1 - t) * col[j].y + t * col[j + 1].y;
+        const double x = side == 0 ? ws[w] - Omega / 2 : ws[w] + Omega / 2;
+        const double2 GR = jinv(x - e_i[i] - sr, -si);
+        const double2 GA = make_double2(GR.x, -GR.y);
+        const double k = 1 - 2 * occ_fermion_l(x - mu, T);
+        const double2 GK = make_double2((GR.x - GA.x) * k, (GR.y - GA.y) * k);
+        double2 *G = side == 0 ? Gm : Gp;
+        G[w + nw * (0 + 4 * i)] = make_double2((GR.x + GA.x + GK.x) / 2, (GR.y + GA.y + GK.y) / 2);
+        G[w + nw * (1 + 4 * i)] = make_double2((GR.x - GA.x + GK.x) / 2, (GR.y - GA.y + GK.y) / 2);
+        G[w + nw * (2 + 4 * i)] = make_double2((-GR.x + GA.x + GK.x) / 2, (-GR.y + GA.y + GK.y) / 2);
+        G[w + nw * (3 + 4 * i)] = make_double2((-GR.x - GA.x + GK.x) / 2, (-GR.y - GA.y + GK.y) / 2);
+    }
+}
+
+#define JIX(w, c12, cart, c0, i) ((w) + nw * ((c12) + 4 * ((cart) + 3 * ((c0) + 2 * (int64_t)(i)))))
+#define GIX(w, c, i) ((w) + nw * ((c) + 4 * (int64_t)(i)))
+
+// J0^{c1c2,c0} = Gm^{c1 c0} v Gp^{c0 c2}  (src/ladder_kernel.jl:70-73)
+__global__ void bare_vertex_kernel(int64_t nw, int64_t ni, const double2 *__restrict__ Gm, const double *__restrict__ v,
+                                   const double2 *__restrict__ Gp, double2 *__restrict__ J0)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nw * 24 * ni) return;
+    const int64_t w = q % nw;
+    int64_t r = q / nw;
+    const int c12 = r % 4; r /= 4;
+    const int cart = r % 3; r /= 3;
+    const int c0 = r % 2;
+    const int64_t i = r / 2;
+    const int c1 = c12 % 2, c2 = c12 / 2;
+    const double2 a = Gm[GIX(w, c1 + 2 * c0, i)], b = Gp[GIX(w, c0 + 2 * c2, i)];
+    const double vv = v[cart + 3 * i];
+    const double2 av = make_double2(a.x * vv, a.y * vv);
+    J0[q] = cmul(av, b);
+}
+
+// Y[b, i] = Σ_ω Wω[ω] (J^{22,c0=1} + J^{11,c0=2})(ω)  (src/ladder_kernel.jl:119), one block per (b, i)
+__global__ __launch_bounds__(256) void cond_Y_kernel(int64_t nw, int64_t ni, const double2 *__restrict__ J,
+                                                     const double *__restrict__ Ww, double2 *__restrict__ Y)
+{
+    __shared__ double rr[256], ri[256];
+    const int b = blockIdx.x % 3;
+    const int64_t i = blockIdx.x / 3;
+    double sr = 0, si = 0;
+    for (int64_t w = threadIdx.x; w < nw; w += 256) {
+        const double2 x = J[JIX(w, 3, b, 0, i)], y = J[JIX(w, 0, b, 1, i)];
+        sr = fma(Ww[w], x.x + y.x, sr);
+        si = fma(Ww[w], x.y + y.y, si);
+    }
+    rr[threadIdx.x] = sr;
+    ri[threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            rr[threadIdx.x] += rr[threadIdx.x + s];
+            ri[threadIdx.x] += ri[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Y[b + 3 * i] = make_double2(rr[0], ri[0]);
+}
+
+struct BtArgs {
+    int64_t nm, ni, nfs, ns;
+    const double *g2p;        // [nm, nfs, ni]
+    const double *wf;         // [nfs]
+    const double *Rs;         // [3, 3, nfs] rotations in star order
+    const int64_t *star_ptr;  // [ni + 1]
+    const int32_t *fsrc;      // [nfs]
+    const double *wph;
+    const int32_t *iq_kk, *iks_i, *iks_f;
+    int64_t nki;
+    double h, T, w_acoustic;
+    double *Bt;               // [(b, ip), (a, i, t), m]: (b + 3 ip) + 3 ni ((a + 3 (i + ni t)) + 6 ni m)
+};
+
+// B̃[m, b, ip, a, i, t] = Σ_{f in star(ip)} R_f[a, b] w_f Σ_ν g2[ν, i, f] tent(ωq/h - m) (n_B(ωq) for
+// t = 2), modes below ω_acoustic skipped (src/ladder_kernel.jl:195-264).  One thread per
+// (m, ip, i) with its 18 (a, b, t) sums in registers.
+__global__ __launch_bounds__(256) void bt_build_kernel(BtArgs a)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t ni = a.ni, ns = a.ns;
+    if (q >= ns * ni * ni) return;
+    const int64_t m = q % ns, ip = (q / ns) % ni, i = q / (ns * ni);
+    double acc[2][9];
+    for (int t = 0; t < 2; t++)
+        for (int k = 0; k < 9; k++) acc[t][k] = 0.0;
+    for (int64_t p = a.star_ptr[ip]; p < a.star_ptr[ip + 1]; p++) {
+        const int64_t fs = a.fsrc[p];
+        const int64_t qi = a.iq_kk[(a.iks_i[i] - 1) + a.nki * (int64_t)(a.iks_f[fs] - 1)] - 1;
+        double s1 = 0, s2 = 0;
+        for (int64_t nu = 0; nu < a.nm; nu++) {
+            const double om = a.wph[nu + a.nm * qi];
+            if (om < a.w_acoustic) continue;
+            const double x = om / a.h;
+            const double tent = 1.0 - fabs(x - (double)m);
+            if (!(tent > 0)) continue;
+            const double c = a.g2p[nu + a.nm * (p + a.nfs * i)] * tent;
+            s1 += c;
+            s2 += c * occ_boson_l(om, a.T);
+        }
+        if (s1 == 0 && s2 == 0) continue;
+        const double w = a.wf[p];
+        const double *R = a.Rs + 9 * p;
+        for (int k = 0; k < 9; k++) {
+            acc[0][k] = fma(R[k] * w, s1, acc[0][k]);
+            acc[1][k] = fma(R[k] * w, s2, acc[1][k]);
+        }
+    }
+    // R[k] with k = a_ + 3 b_
+    for (int t = 0; t < 2; t++)
+        for (int b = 0; b < 3; b++)
+            for (int aa = 0; aa < 3; aa++) {
+                const int64_t col = aa + 3 * (i + ni * t);
+                a.Bt[(b + 3 * ip) + 3 * ni * (col + 6 * ni * m)] = acc[t][aa + 3 * b];
+            }
+}
+
+// Jm[(j, c12, c0, ri), (b, ip)] = Re / Im of J[j, c12, b, c0, ip]: the vertex as the A operand.
+__global__ void jperm_kernel(int64_t nw, int64_t ni, const double2 *__restrict__ J, double *__restrict__ Jm)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nrow = 16 * nw;
+    if (q >= nrow * 3 * ni) return;
+    const int64_t r = q % nrow, k = q / nrow;
+    const int64_t j = r % nw, c12 = (r / nw) % 4, c0 = (r / (4 * nw)) % 2, ri = r / (8 * nw);
+    const int64_t b = k % 3, ip = k / 3;
+    const double2 z = J[JIX(j, c12, b, c0, ip)];
+    Jm[q] = ri == 0 ? z.x : z.y;
+}
+
+// C_s[(j, q), col] = Σ_m wlo W_m[(klo - 1, q), col] + whi W_m[(klo, q), col], both signs
+// (klo 1-based; W[(r, col), m] with r = j' + nω q).
+__global__ void shift_add_kernel(int64_t nw, int64_t ncol, int64_t ns, const double *__restrict__ W,
+                                 const int32_t *__restrict__ klo, const double *__restrict__ wlo,
+                                 const double *__restrict__ whi, double *__restrict__ Cp, double *__restrict__ Cm)
+{
+    const int64_t nrow = 16 * nw;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nrow * ncol) return;
+    const int64_t r = q % nrow, col = q / nrow;
+    const int64_t j = r % nw, qb = r - j;
+    double sp = 0, sm = 0;
+    for (int64_t m = 0; m < ns; m++) {
+        const double *Wm = W + nrow * (col + ncol * m);
+        const int64_t t0 = j + nw * m, t1 = j + nw * (m + ns);
+        const double lo0 = wlo[t0], hi0 = whi[t0], lo1 = wlo[t1], hi1 = whi[t1];
+        if (lo0 != 0 || hi0 != 0) {
+            const int64_t k = klo[t0] - 1;
+            sp += lo0 * Wm[qb + k] + hi0 * Wm[qb + k + 1];
+        }
+        if (lo1 != 0 || hi1 != 0) {
+            const int64_t k = klo[t1] - 1;
+            sm += lo1 * Wm[qb + k] + hi1 * Wm[qb + k + 1];
+        }
+    }
+    Cp[q] = sp;
+    Cm[q] = sm;
+}
+
+// Y = (U+ - U-)/2 for the KK product: Y[(j, q), (a, i)] from the type-1 columns
+__global__ void exch_Y_kernel(int64_t n, const double *__restrict__ Cp, const double *__restrict__ Cm, double *__restrict__ Y)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < n) Y[q] = (Cp[q] - Cm[q]) / 2;
+}
+
+// The channels K^{ab} with the contour signs (src/ladder_kernel.jl:296-307), then
+// LJ = -Gm K Gp as 2x2 contour matrices (src/ladder.jl:294-295).  One thread per (ω, a, c0, i).
+__global__ void exch_attach_kernel(int64_t nw, int64_t ni, const double *__restrict__ Cp, const double *__restrict__ Cm,
+                                   const double *__restrict__ KY, const double2 *__restrict__ Gm,
+                                   const double2 *__restrict__ Gp, double2 *__restrict__ LJ)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nw * 3 * 2 * ni) return;
+    const int64_t w = q % nw;
+    int64_t r = q / nw;
+    const int a = r % 3; r /= 3;
+    const int c0 = r % 2;
+    const int64_t i = r / 2;
+    const int64_t nrow = 16 * nw;
+    // C[(j, c12, c0, ri), (a, i, t)]
+    auto C = [&](const double *X, int c12, int ri, int t) -> double {
+        return X[(w + nw * (c12 + 4 * (c0 + 2 * ri))) + nrow * (a + 3 * (i + ni * t))];
+    };
+    auto Ky = [&](int c12, int ri) -> double { return KY[(w + nw * (c12 + 4 * (c0 + 2 * ri))) + nrow * (a + 3 * i)]; };
+    double2 K[4];
+    for (int c12 = 0; c12 < 4; c12++) {
+        const double2 Up = make_double2(C(Cp, c12, 0, 0), C(Cp, c12, 1, 0));
+        const double2 Um = make_double2(C(Cm, c12, 0, 0), C(Cm, c12, 1, 0));
+        const double2 Vp = make_double2(C(Cp, c12, 0, 1), C(Cp, c12, 1, 1));
+        const double2 Vm = make_double2(C(Cm, c12, 0, 1), C(Cm, c12, 1, 1));
+        if (c12 == 0 || c12 == 3) {
+            const double2 ky = make_double2(Ky(c12, 0), Ky(c12, 1));
+            // (c12 == 1 ? -i : +i) * ky
+            const double2 iky = c12 == 0 ? make_double2(ky.y, -ky.x) : make_double2(-ky.y, ky.x);
+            K[c12] = make_double2(iky.x - (Up.x + Um.x) / 2 - (Vp.x + Vm.x), iky.y - (Up.y + Um.y) / 2 - (Vp.y + Vm.y));
+        } else if (c12 == 1) {
+            K[c12] = make_double2(Vp.x + Um.x + Vm.x, Vp.y + Um.y + Vm.y);
+        } else {
+            K[c12] = make_double2(Up.x + Vp.x + Vm.x, Up.y + Vp.y + Vm.y);
+        }
+    }
+    // J^{c1c2} = Σ_ab Gm^{c1 a} K^{ab} Gp^{b c2}, K^{ab} at c12 = a + 2b
+    for (int c12 = 0; c12 < 4; c12++) {
+        const int c1 = c12 % 2, c2 = c12 / 2;
+        const double2 g1 = Gm[GIX(w, c1, i)], g2 = Gm[GIX(w, c1 + 2, i)];
+        const double2 p1 = Gp[GIX(w, 0 + 2 * c2, i)], p2 = Gp[GIX(w, 1 + 2 * c2, i)];
+        double2 s = cmul(cmul(g1, K[0]), p1);
+        s = cadd(s, cmul(cmul(g2, K[1]), p1));
+        s = cadd(s, cmul(cmul(g1, K[2]), p2));
+        s = cadd(s, cmul(cmul(g2, K[3]), p2));
+        LJ[JIX(w, c12, a, c0, i)] = make_double2(-s.x, -s.y);
+    }
+}
+
+inline unsigned nb(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+}  // namespace epsx
+
+// ------------------------------------------------------------------ GMRES vector kernels
+namespace epsx {
+namespace {
+
+constexpr int RED_BLOCKS = 512;
+
+// part[b] = Σ over this block's strided elements of conj(x) y (fixed order)
+__global__ __launch_bounds__(256) void cdot_partial_kernel(int64_t n, const double2 *__restrict__ x,
+                                                           const double2 *__restrict__ y, double2 *__restrict__ part)
+{
+    __shared__ double rr[256], ri[256];
+    double sr = 0, si = 0;
+    for (int64_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) {
+        const double2 a = x[q], b = y[q];
+        sr = fma(a.x, b.x, fma(a.y, b.y, sr));
+        si = fma(a.x, b.y, fma(-a.y, b.x, si));
+    }
+    rr[threadIdx.x] = sr;
+    ri[threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            rr[threadIdx.x] += rr[threadIdx.x + s];
+            ri[threadIdx.x] += ri[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rr[0], ri[0]);
+}
+
+__global__ __launch_bounds__(256) void cdot_final_kernel(int nb, const double2 *__restrict__ part, double2 *__restrict__ out)
+{
+    __shared__ double rr[256], ri[256];
+    double sr = 0, si = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        sr += part[b].x;
+        si += part[b].y;
+    }
+    rr[threadIdx.x] = sr;
+    ri[threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            rr[threadIdx.x] += rr[threadIdx.x + s];
+            ri[threadIdx.x] += ri[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = make_double2(rr[0], ri[0]);
+}
+
+// y -= c x, c read from device memory (the Arnoldi coefficient just computed)
+__global__ void caxpy_dev_kernel(int64_t n, const double2 *__restrict__ c, const double2 *__restrict__ x,
+                                 double2 *__restrict__ y)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double2 a = *c, b = x[q];
+    y[q] = make_double2(y[q].x - (a.x * b.x - a.y * b.y), y[q].y - (a.x * b.y + a.y * b.x));
+}
+
+__global__ void caxpy_kernel(int64_t n, double ar, double ai, const double2 *__restrict__ x, double2 *__restrict__ y)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double2 b = x[q];
+    y[q] = make_double2(y[q].x + (ar * b.x - ai * b.y), y[q].y + (ar * b.y + ai * b.x));
+}
+
+__global__ void cscale_kernel(int64_t n, double s, const double2 *__restrict__ x, double2 *__restrict__ y)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < n) y[q] = make_double2(s * x[q].x, s * x[q].y);
+}
+
+// y = a + sa * b + sc * c  (elementwise complex)
+__global__ void combine3_kernel(int64_t n, const double2 *__restrict__ a, double sb, const double2 *__restrict__ b,
+                                double sc, const double2 *__restrict__ c, double2 *__restrict__ y)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    y[q] = make_double2(a[q].x + sb * b[q].x + sc * c[q].x, a[q].y + sb * b[q].y + sc * c[q].y);
+}
+
+// C_s (+)= shifted W for the shifts [m0, m0 + mb) of one chunk
+__global__ void shift_add_chunk_kernel(int64_t nw, int64_t ncol, int64_t ns, int64_t m0, int64_t mb,
+                                       const double *__restrict__ W, const int32_t *__restrict__ klo,
+                                       const double *__restrict__ wlo, const double *__restrict__ whi,
+                                       double *__restrict__ Cp, double *__restrict__ Cm, int accumulate)
+{
+    const int64_t nrow = 16 * nw;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nrow * ncol) return;
+    const int64_t r = q % nrow, col = q / nrow;
+    const int64_t j = r % nw, qb = r - j;
+    double sp = accumulate ? Cp[q] : 0.0, sm = accumulate ? Cm[q] : 0.0;
+    for (int64_t mm = 0; mm < mb; mm++) {
+        const int64_t m = m0 + mm;
+        const double *Wm = W + nrow * (col + ncol * mm);
+        const int64_t t0 = j + nw * m, t1 = j + nw * (m + ns);
+        const double lo0 = wlo[t0], hi0 = whi[t0], lo1 = wlo[t1], hi1 = whi[t1];
+        if (lo0 != 0 || hi0 != 0) {
+            const int64_t k = klo[t0] - 1;
+            sp += lo0 * Wm[qb + k] + hi0 * Wm[qb + k + 1];
+        }
+        if (lo1 != 0 || hi1 != 0) {
+            const int64_t k = klo[t1] - 1;
+            sm += lo1 * Wm[qb + k] + hi1 * Wm[qb + k + 1];
+        }
+    }
+    Cp[q] = sp;
+    Cm[q] = sm;
+}
+
+}  // namespace
+
+static int ldot(epsx_ctx *ctx, int64_t n, const double2 *x, const double2 *y, double2 *d_out)
+{
+    LadderDev &l = ctx->l;
+    hipLaunchKernelGGL(cdot_partial_kernel, dim3(RED_BLOCKS), dim3(256), 0, ctx->stream, n, x, y, dptr<double2>(l.red));
+    hipLaunchKernelGGL(cdot_final_kernel, dim3(1), dim3(256), 0, ctx->stream, RED_BLOCKS, dptr<double2>(l.red), d_out);
+    EPSX_HIP(ctx, hipGetLastError());
+    return EPSX_OK;
+}
+
+// B̃ for temperature T (once per T and ω_acoustic)
+static int ensure_Bt(epsx_ctx *ctx, double T, double w_ac)
+{
+    Scgd0Dev &s = ctx->s;
+    LadderDev &l = ctx->l;
+    if (l.bt_T == T && l.bt_wac == w_ac) return EPSX_OK;
+    const int64_t ni = s.ni;
+    EPSX_TRY(dev_alloc(ctx, l.Bt, 8 * (size_t)(3 * ni) * (6 * ni) * l.ns));
+    BtArgs a;
+    a.nm = s.nm; a.ni = ni; a.nfs = s.nf; a.ns = l.ns;
+    a.g2p = dptr<double>(s.g2p); a.wf = dptr<double>(s.wf_s); a.Rs = dptr<double>(l.Rs);
+    a.star_ptr = dptr<int64_t>(s.star_ptr); a.fsrc = dptr<int32_t>(s.fsrc_s);
+    a.wph = dptr<double>(s.wph); a.iq_kk = dptr<int32_t>(s.iq_kk); a.iks_i = dptr<int32_t>(s.iks_i);
+    a.iks_f = dptr<int32_t>(s.iks_f); a.nki = s.nki;
+    a.h = l.h; a.T = T; a.w_acoustic = w_ac; a.Bt = dptr<double>(l.Bt);
+    hipLaunchKernelGGL(bt_build_kernel, dim3(nb(l.ns * ni * ni)), dim3(256), 0, ctx->stream, a);
+    EPSX_HIP(ctx, hipGetLastError());
+    l.bt_T = T;
+    l.bt_wac = w_ac;
+    return EPSX_OK;
+}
+
+// LJ = L(J) on device vectors at the temperature whose B̃ is loaded and the propagators in l.Gm/l.Gp
+static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ)
+{
+    Scgd0Dev &s = ctx->s;
+    LadderDev &l = ctx->l;
+    const int64_t nw = s.nw, ni = s.ni, nrow = 16 * nw, ncol = 6 * ni, K = 3 * ni;
+    const hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(jperm_kernel, dim3(nb(nrow * K)), dim3(256), 0, st, nw, ni, J, dptr<double>(l.Jm));
+    EPSX_HIP(ctx, hipGetLastError());
+    // W for chunks of shifts, each followed by its shift-add
+    const size_t per_m = 8 * (size_t)nrow * ncol;
+    int64_t mb = (int64_t)((size_t(8) << 30) / per_m);
+    mb = mb < 1 ? 1 : (mb > l.ns ? l.ns : mb);
+    EPSX_TRY(dev_alloc(ctx, l.W, per_m * mb));
+    for (int64_t m0 = 0; m0 < l.ns; m0 += mb) {
+        const int64_t mc = m0 + mb <= l.ns ? mb : l.ns - m0;
+        EPSX_TRY(dgemm_nn(ctx, nrow, ncol * mc, K, dptr<double>(l.Jm), nrow, dptr<double>(l.Bt) + K * ncol * m0, K,
+                          dptr<double>(l.W), nrow, false));
+        hipLaunchKernelGGL(shift_add_chunk_kernel, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns, m0, mc,
+                           dptr<double>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
+                           dptr<double>(l.Cp), dptr<double>(l.Cm), m0 > 0 ? 1 : 0);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
+    // KY = KK (U+ - U-)/2, then the channels and the attached propagators
+    const int64_t ny = nrow * 3 * ni;
+    hipLaunchKernelGGL(exch_Y_kernel, dim3(nb(ny)), dim3(256), 0, st, ny, dptr<double>(l.Cp), dptr<double>(l.Cm),
+                       dptr<double>(l.Y));
+    EPSX_HIP(ctx, hipGetLastError());
+    EPSX_TRY(dgemm_nn(ctx, nw, ny / nw, nw, dptr<double>(s.KK), nw, dptr<double>(l.Y), nw, dptr<double>(l.KY), nw, false));
+    hipLaunchKernelGGL(exch_attach_kernel, dim3(nb(nw * 6 * ni)), dim3(256), 0, st, nw, ni, dptr<double>(l.Cp),
+                       dptr<double>(l.Cm), dptr<double>(l.KY), dptr<double2>(l.Gm), dptr<double2>(l.Gp), LJ);
+    EPSX_HIP(ctx, hipGetLastError());
+    return EPSX_OK;
+}
+
+static int ladder_work(epsx_ctx *ctx)
+{
+    Scgd0Dev &s = ctx->s;
+    LadderDev &l = ctx->l;
+    const int64_t nw = s.nw, ni = s.ni, nrow = 16 * nw;
+    EPSX_TRY(dev_alloc(ctx, l.Gm, 16 * (size_t)nw * 4 * ni));
+    EPSX_TRY(dev_alloc(ctx, l.Gp, 16 * (size_t)nw * 4 * ni));
+    EPSX_TRY(dev_alloc(ctx, l.Jm, 8 * (size_t)nrow * 3 * ni));
+    EPSX_TRY(dev_alloc(ctx, l.Cp, 8 * (size_t)nrow * 6 * ni));
+    EPSX_TRY(dev_alloc(ctx, l.Cm, 8 * (size_t)nrow * 6 * ni));
+    EPSX_TRY(dev_alloc(ctx, l.Y, 8 * (size_t)nrow * 3 * ni));
+    EPSX_TRY(dev_alloc(ctx, l.KY, 8 * (size_t)nrow * 3 * ni));
+    EPSX_TRY(dev_alloc(ctx, l.red, 16 * (size_t)RED_BLOCKS));
+    EPSX_TRY(dev_alloc(ctx, l.Yc, 16 * (size_t)3 * ni));
+    return EPSX_OK;
+}
+
+// σ_ab = Re[F/(2πi) Σ_i v_a w_i Y[b, i]] (src/ladder_kernel.jl:116-122)
+static void sigma_from_Y(const std::vector<double2> &Y, const std::vector<double> &v, const std::vector<double> &w,
+                         int64_t ni, double Omega, double T, double *sigma)
+{
+    using cd = std::complex<double>;
+    cd F;
+    if (Omega == 0) F = cd(0, 1) / (4 * T);
+    else {
+        const double nB = T > 1.4901161193847656e-08 ? 1.0 / std::expm1(Omega / T) : (Omega > 0 ? 0.0 : -1.0);
+        F = -1.0 / (cd(0, 2) * Omega * (1 + 2 * nB));
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            cd S = 0;
+            for (int64_t i = 0; i < ni; i++) S += v[a + 3 * i] * w[i] * cd(Y[b + 3 * i].x, Y[b + 3 * i].y);
+            sigma[a + 3 * b] = (F / (2 * 3.141592653589793 * cd(0, 1)) * S).real();
+        }
+}
+
+static int sigma_of(epsx_ctx *ctx, const double2 *J, double Omega, double T, double *sigma)
+{
+    Scgd0Dev &s = ctx->s;
+    LadderDev &l = ctx->l;
+    hipLaunchKernelGGL(cond_Y_kernel, dim3((unsigned)(3 * s.ni)), dim3(256), 0, ctx->stream, s.nw, s.ni, J,
+                       dptr<double>(l.Ww), dptr<double2>(l.Yc));
+    EPSX_HIP(ctx, hipGetLastError());
+    std::vector<double2> Y(3 * s.ni);
+    EPSX_HIP(ctx, hipMemcpyAsync(Y.data(), l.Yc.p, 16 * 3 * s.ni, hipMemcpyDeviceToHost, ctx->stream));
+    EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    sigma_from_Y(Y, l.v_host, s.w_i_host, s.ni, Omega, T, sigma);
+    return EPSX_OK;
+}
+
+}  // namespace epsx
+
+using namespace epsx;
+
+extern "C" int epsx_ladder_set_problem(epsx_ctx *ctx, const epsx_ladder_problem *P)
+{
+    if (!ctx) return EPSX_ERR_ARG;
+    if (!P || P->struct_size != (int64_t)sizeof(epsx_ladder_problem))
+        return set_error(ctx, EPSX_ERR_ARG, "epsx_ladder_problem: struct size mismatch (ABI %d)", EPSX_ABI_VERSION);
+    Scgd0Dev &s = ctx->s;
+    LadderDev &l = ctx->l;
+    if (!s.ready) return set_error(ctx, EPSX_ERR_STATE, "epsx_ladder_set_problem before epsx_scgd0_set_problem");
+    if (!P->R || !P->v) return set_error(ctx, EPSX_ERR_ARG, "ladder problem: null R or v");
+    EPSX_HIP(ctx, hipSetDevice(ctx->device));
+    l.ready = false;
+    const int64_t nf = s.nf, ni = s.ni, nw = s.nw;
+    // rotations in star order
+    std::vector<int32_t> fsrc(nf);
+    EPSX_HIP(ctx, hipMemcpy(fsrc.data(), s.fsrc_s.p, 4 * nf, hipMemcpyDeviceToHost));
+    std::vector<double> Rs(9 * (size_t)nf);
+    for (int64_t p = 0; p < nf; p++)
+        for (int k = 0; k < 9; k++) Rs[9 * p + k] = P->R[9 * (int64_t)fsrc[p] + k];
+    EPSX_TRY(upload(ctx, l.Rs, Rs.data(), 8 * Rs.size()));
+    l.v_host.assign(P->v, P->v + 3 * ni);
+    EPSX_TRY(upload(ctx, l.v, P->v, 8 * 3 * ni));
+    std::vector<double> Ww(nw);
+    trapezoid_weights(nw, s.ws_host.data(), Ww.data());
+    EPSX_TRY(upload(ctx, l.Ww, Ww.data(), 8 * nw));
+    // shift tables on the grid's step (src/ladder.jl:151-152)
+    double wmax = 0;
+    std::vector<double> wph(s.nm * s.nq);
+    EPSX_HIP(ctx, hipMemcpy(wph.data(), s.wph.p, 8 * wph.size(), hipMemcpyDeviceToHost));
+    for (double x : wph) wmax = x > wmax ? x : wmax;
+    l.h = s.dw_unit;
+    l.ns = (int64_t)std::ceil(wmax / l.h) + 2;
+    std::vector<int64_t> offs(nw);
+    for (int64_t j = 0; j < nw; j++) offs[j] = (int64_t)std::llround((s.ws_host[j] - s.ws_host[0]) / s.dw_unit);
+    {
+        // the grid's integer offsets, recovered from the relative offsets kept on the device
+        std::vector<int32_t> off(nw);
+        EPSX_HIP(ctx, hipMemcpy(off.data(), s.off.p, 4 * nw, hipMemcpyDeviceToHost));
+        for (int64_t j = 0; j < nw; j++) offs[j] = off[j];
+    }
+    std::vector<int32_t> klo(nw * l.ns * 2);
+    std::vector<double> wlo(nw * l.ns * 2), whi(nw * l.ns * 2);
+    EPSX_TRY(epsx_vertex_shift_tables(ctx, nw, offs.data(), l.ns, klo.data(), wlo.data(), whi.data()));
+    EPSX_TRY(upload(ctx, l.klo, klo.data(), 4 * klo.size()));
+    EPSX_TRY(upload(ctx, l.wlo, wlo.data(), 8 * wlo.size()));
+    EPSX_TRY(upload(ctx, l.whi, whi.data(), 8 * whi.size()));
+    EPSX_TRY(ladder_work(ctx));
+    l.bt_T = -1;
+    l.ready = true;
+    EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return EPSX_OK;
+}
+
+extern "C" int epsx_apply_ladder_kernel(epsx_ctx *ctx, const double *J, const double *Gm, const double *Gp, double T,
+                                        double w_acoustic, int32_t precision, double *LJ)
+{
+    if (!ctx) return EPSX_ERR_ARG;
+    LadderDev &l = ctx->l;
+    Scgd0Dev &s = ctx->s;
+    if (!l.ready) return set_error(ctx, EPSX_ERR_STATE, "epsx_apply_ladder_kernel before epsx_ladder_set_problem");
+    if (!J || !Gm || !Gp || !LJ) return set_error(ctx, EPSX_ERR_ARG, "apply_ladder_kernel!: null array");
+    if (!(w_acoustic > 0)) return set_error(ctx, EPSX_ERR_ARG, "ω_acoustic must be positive, got %g", w_acoustic);
+    if (precision != EPSX_PREC_FP64) return set_error(ctx, EPSX_ERR_ARG, "ladder precision %d is not available", precision);
+    EPSX_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t n = s.nw * 24 * s.ni, ng = s.nw * 4 * s.ni;
+    Tmp dJ, dLJ;
+    EPSX_TRY(tmp_upload(ctx, dJ, J, 16 * n));
+    EPSX_TRY(tmp_alloc(ctx, dLJ.b, 16 * n));
+    EPSX_HIP(ctx, hipMemcpyAsync(l.Gm.p, Gm, 16 * ng, hipMemcpyHostToDevice, ctx->stream));
+    EPSX_HIP(ctx, hipMemcpyAsync(l.Gp.p, Gp, 16 * ng, hipMemcpyHostToDevice, ctx->stream));
+    EPSX_TRY(ensure_Bt(ctx, T, w_acoustic));
+    EPSX_TRY(ladder_apply_dev(ctx, dJ.ptr<double2>(), dLJ.ptr<double2>()));
+    EPSX_HIP(ctx, hipMemcpyAsync(LJ, dLJ.b.p, 16 * n, hipMemcpyDeviceToHost, ctx->stream));
+    EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return EPSX_OK;
+}
+
+namespace {
+
+// Krylov.jl's sym_givens for complex a and real b >= 0 (c real, s complex)
+void sym_givens(std::complex<double> a, double b, double &c, std::complex<double> &s, std::complex<double> &rho)
+{
+    const double aa = std::abs(a);
+    if (aa == 0) { c = 0; s = 1; rho = b; return; }
+    if (b == 0) { c = 1; s = 0; rho = a; return; }
+    const double nrm = std::hypot(aa, b);
+    c = aa / nrm;
+    s = (a / aa) * (b / nrm);
+    rho = (a / aa) * nrm;
+}
+
+}  // namespace
+
+extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, epsx_ladder_outputs *out)
+{
+    using cd = std::complex<double>;
+    if (!ctx) return EPSX_ERR_ARG;
+    LadderDev &l = ctx->l;
+    Scgd0Dev &s = ctx->s;
+    if (!l.ready) return set_error(ctx, EPSX_ERR_STATE, "epsx_ladder_run before epsx_ladder_set_problem");
+    if (!par || par->struct_size != (int64_t)sizeof(epsx_ladder_params) || !out ||
+        out->struct_size != (int64_t)sizeof(epsx_ladder_outputs))
+        return set_error(ctx, EPSX_ERR_ARG, "epsx_ladder_run: struct size mismatch (ABI %d)", EPSX_ABI_VERSION);
+    if (par->solver != EPSX_SOLVER_GMRES && par->solver != EPSX_SOLVER_FIXED_POINT)
+        return set_error(ctx, EPSX_ERR_ARG, "solver must be :gmres or :fixed_point, got %d", par->solver);
+    if (par->maxiter < 1) return set_error(ctx, EPSX_ERR_ARG, "maxiter must be at least 1, got %lld", (long long)par->maxiter);
+    if (par->nOmega < 1) return set_error(ctx, EPSX_ERR_ARG, "Ωs must hold at least one probe frequency");
+    if (!(par->w_acoustic > 0))
+        return set_error(ctx, EPSX_ERR_ARG, "ω_acoustic must be positive, got %g: the shift-grid coupling build has "
+                         "no row for a negative phonon frequency", par->w_acoustic);
+    if (par->nT < 1 || !par->Tlist || !par->mu || !par->Sigma || !par->Omegas)
+        return set_error(ctx, EPSX_ERR_ARG, "ladder run: null input");
+    if (par->precision != EPSX_PREC_FP64) return set_error(ctx, EPSX_ERR_ARG, "ladder precision %d is not available", par->precision);
+    if (!out->sigma_bubble || !out->sigma_ladder || !out->converged || !out->niter || !out->norms || !out->sigma_iter)
+        return set_error(ctx, EPSX_ERR_ARG, "ladder run: null output");
+    EPSX_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t nw = s.nw, ni = s.ni, n = nw * 24 * ni, nT = par->nT, nO = par->nOmega, it_max = par->maxiter;
+    const hipStream_t st = ctx->stream;
+    EPSX_TRY(dev_alloc(ctx, l.S, 16 * (size_t)nw * ni));
+    EPSX_TRY(dev_alloc(ctx, l.jm, 4 * nw));
+    EPSX_TRY(dev_alloc(ctx, l.tm, 8 * nw));
+    EPSX_TRY(dev_alloc(ctx, l.jp, 4 * nw));
+    EPSX_TRY(dev_alloc(ctx, l.tp, 8 * nw));
+    const int gm = par->solver == EPSX_SOLVER_GMRES;
+    Tmp J0, J, LJ, R, V, hcol;
+    EPSX_TRY(tmp_alloc(ctx, J0.b, 16 * n));
+    EPSX_TRY(tmp_alloc(ctx, J.b, 16 * n));
+    EPSX_TRY(tmp_alloc(ctx, LJ.b, 16 * n));
+    EPSX_TRY(tmp_alloc(ctx, R.b, 16 * n));
+    if (gm) EPSX_TRY(tmp_alloc(ctx, V.b, 16 * (size_t)n * (it_max + 1)));
+    EPSX_TRY(tmp_alloc(ctx, hcol.b, 16 * (size_t)(it_max + 2)));
+    const double btol = std::pow(2.220446049250313e-16, 0.75);
+    for (int64_t q = 0; q < (it_max + 1) * nO * nT; q++) out->norms[q] = NAN;
+    for (int64_t q = 0; q < 9 * it_max * nO * nT; q++) out->sigma_iter[q] = NAN;
+    std::vector<int32_t> jv(nw);
+    std::vector<double> tv(nw), xs(nw);
+    double2 *dJ0 = J0.ptr<double2>(), *dJ = J.ptr<double2>(), *dLJ = LJ.ptr<double2>(), *dR = R.ptr<double2>();
+    double2 *dh = hcol.ptr<double2>();
+    auto launch_n = [&](int64_t m) { return dim3(nb(m)); };
+    for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
+    for (int64_t iT = 0; iT < nT; iT++) {
+        const double T = par->Tlist[iT], mu = par->mu[iT];
+        if (!(T >= 0)) return set_error(ctx, EPSX_ERR_ARG, "Temperature must be positive");
+        EPSX_TRY(ensure_Bt(ctx, T, par->w_acoustic));
+        EPSX_HIP(ctx, hipMemcpyAsync(l.S.p, par->Sigma + 2 * nw * ni * iT, 16 * nw * ni, hipMemcpyHostToDevice, st));
+        for (int64_t iO = 0; iO < nO; iO++) {
+            const double Om = par->Omegas[iO];
+            const int64_t slot = iO + nO * iT;
+            // Σ at ωs ∓ Ω/2 through host tables, then the propagators and the bare vertex
+            for (int64_t w = 0; w < nw; w++) xs[w] = s.ws_host[w] - Om / 2;
+            linear_interpolation_table(nw, s.ws_host.data(), nw, xs.data(), jv.data(), tv.data());
+            EPSX_HIP(ctx, hipMemcpyAsync(l.jm.p, jv.data(), 4 * nw, hipMemcpyHostToDevice, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(l.tm.p, tv.data(), 8 * nw, hipMemcpyHostToDevice, st));
+            for (int64_t w = 0; w < nw; w++) xs[w] = s.ws_host[w] + Om / 2;
+            linear_interpolation_table(nw, s.ws_host.data(), nw, xs.data(), jv.data(), tv.data());
+            EPSX_HIP(ctx, hipMemcpyAsync(l.jp.p, jv.data(), 4 * nw, hipMemcpyHostToDevice, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(l.tp.p, tv.data(), 8 * nw, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(greens_kernel, launch_n(nw * ni), dim3(256), 0, st, nw, ni, dptr<double>(s.ws),
+                               dptr<double>(s.e_i), dptr<double2>(l.S), dptr<int32_t>(l.jm), dptr<double>(l.tm),
+                               dptr<int32_t>(l.jp), dptr<double>(l.tp), Om, mu, T, dptr<double2>(l.Gm),
+                               dptr<double2>(l.Gp));
+            hipLaunchKernelGGL(bare_vertex_kernel, launch_n(n), dim3(256), 0, st, nw, ni, dptr<double2>(l.Gm),
+                               dptr<double>(l.v), dptr<double2>(l.Gp), dJ0);
+            EPSX_HIP(ctx, hipGetLastError());
+            EPSX_TRY(sigma_of(ctx, dJ0, Om, T, out->sigma_bubble + 9 * slot));
+            int64_t niter = 0;
+            int conv = 0;
+            if (gm) {
+                // (I - L) J = J0 by GMRES (Krylov.jl: MGS Arnoldi, Givens QR, no restart), warm-started
+                // from the previous Ω at this T, stopping at rtol ||J0||.
+                double2 nrm;
+                EPSX_TRY(ldot(ctx, n, dJ0, dJ0, dh));
+                EPSX_HIP(ctx, hipMemcpyAsync(&nrm, dh, 16, hipMemcpyDeviceToHost, st));
+                EPSX_HIP(ctx, hipStreamSynchronize(st));
+                const double atol = par->rtol * std::sqrt(nrm.x);
+                const bool warm = iO > 0;
+                if (warm) {
+                    EPSX_TRY(ladder_apply_dev(ctx, dJ, dLJ));
+                    // r = b - (x - Lx)
+                    hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, dJ0, -1.0, dJ, 1.0, dLJ, dR);
+                } else {
+                    EPSX_HIP(ctx, hipMemcpyAsync(dR, dJ0, 16 * n, hipMemcpyDeviceToDevice, st));
+                    EPSX_HIP(ctx, hipMemsetAsync(dJ, 0, 16 * n, st));
+                }
+                EPSX_TRY(ldot(ctx, n, dR, dR, dh));
+                EPSX_HIP(ctx, hipMemcpyAsync(&nrm, dh, 16, hipMemcpyDeviceToHost, st));
+                EPSX_HIP(ctx, hipStreamSynchronize(st));
+                const double beta = std::sqrt(nrm.x);
+                double *norms = out->norms + (it_max + 1) * slot;
+                int64_t hn = 0;
+                norms[hn++] = beta;
+                bool solved = beta <= atol;
+                if (beta > 0 && !solved) {
+                    std::vector<cd> Rm((it_max + 1) * (it_max + 1)), sn(it_max + 1), z(it_max + 2), hc(it_max + 2);
+                    std::vector<double> cs(it_max + 1);
+                    double2 *Vb = V.ptr<double2>();
+                    hipLaunchKernelGGL(cscale_kernel, launch_n(n), dim3(256), 0, st, n, 1.0 / beta, dR, Vb);
+                    z[0] = beta;
+                    bool breakdown = false;
+                    const int64_t cap = it_max + 1;
+                    while (!(solved || niter >= it_max || breakdown)) {
+                        const int64_t j = niter++;
+                        double2 *Vj = Vb + n * j, *q = dR;
+                        EPSX_TRY(ladder_apply_dev(ctx, Vj, dLJ));
+                        hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, Vj, -1.0, dLJ, 0.0, dLJ, q);
+                        for (int64_t i = 0; i <= j; i++) {
+                            EPSX_TRY(ldot(ctx, n, Vb + n * i, q, dh + i));
+                            hipLaunchKernelGGL(caxpy_dev_kernel, launch_n(n), dim3(256), 0, st, n, dh + i, Vb + n * i, q);
+                        }
+                        EPSX_TRY(ldot(ctx, n, q, q, dh + j + 1));
+                        std::vector<double2> hv(j + 2);
+                        EPSX_HIP(ctx, hipMemcpyAsync(hv.data(), dh, 16 * (j + 2), hipMemcpyDeviceToHost, st));
+                        EPSX_HIP(ctx, hipStreamSynchronize(st));
+                        for (int64_t i = 0; i <= j; i++) Rm[i + cap * j] = cd(hv[i].x, hv[i].y);
+                        const double Hbis = std::sqrt(hv[j + 1].x);
+                        for (int64_t i = 0; i < j; i++) {
+                            const cd a = Rm[i + cap * j], b = Rm[i + 1 + cap * j];
+                            Rm[i + cap * j] = cs[i] * a + sn[i] * b;
+                            Rm[i + 1 + cap * j] = std::conj(sn[i]) * a - cs[i] * b;
+                        }
+                        cd rho;
+                        sym_givens(Rm[j + cap * j], Hbis, cs[j], sn[j], rho);
+                        Rm[j + cap * j] = rho;
+                        z[j + 1] = std::conj(sn[j]) * z[j];
+                        z[j] = cs[j] * z[j];
+                        const double rn = std::abs(z[j + 1]);
+                        norms[hn++] = rn;
+                        breakdown = Hbis <= btol;
+                        solved = rn <= atol;
+                        if (!(solved || niter >= it_max || breakdown))
+                            hipLaunchKernelGGL(cscale_kernel, launch_n(n), dim3(256), 0, st, n, 1.0 / Hbis, q, Vb + n * (j + 1));
+                    }
+                    // y = R \ z; x += V y
+                    std::vector<cd> y(niter);
+                    for (int64_t i = niter - 1; i >= 0; i--) {
+                        cd sacc = z[i];
+                        for (int64_t k = i + 1; k < niter; k++) sacc -= Rm[i + cap * k] * y[k];
+                        y[i] = sacc / Rm[i + cap * i];
+                    }
+                    for (int64_t i = 0; i < niter; i++)
+                        hipLaunchKernelGGL(caxpy_kernel, launch_n(n), dim3(256), 0, st, n, y[i].real(), y[i].imag(), Vb + n * i, dJ);
+                    EPSX_HIP(ctx, hipGetLastError());
+                }
+                conv = solved;
+            } else {
+                // J ← J0 + L(J) from J0 until σ changes by less than rtol (src/ladder.jl:354-369)
+                EPSX_HIP(ctx, hipMemcpyAsync(dJ, dJ0, 16 * n, hipMemcpyDeviceToDevice, st));
+                double *sit = out->sigma_iter + 9 * it_max * slot;
+                for (int64_t it = 1; it <= it_max; it++) {
+                    EPSX_TRY(ladder_apply_dev(ctx, dJ, dLJ));
+                    hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, dJ0, 1.0, dLJ, 0.0, dLJ, dJ);
+                    EPSX_TRY(sigma_of(ctx, dJ, Om, T, sit + 9 * (it - 1)));
+                    niter = it;
+                    if (it > 1) {
+                        double d = 0, nn = 0;
+                        for (int k = 0; k < 9; k++) {
+                            const double a = sit[9 * (it - 1) + k], b = sit[9 * (it - 2) + k];
+                            d += (a - b) * (a - b);
+                            nn += a * a;
+                        }
+                        if (std::sqrt(d) < par->rtol * std::sqrt(nn)) { conv = 1; break; }
+                    }
+                }
+            }
+            out->converged[slot] = conv;
+            out->niter[slot] = niter;
+            EPSX_TRY(sigma_of(ctx, dJ, Om, T, out->sigma_ladder + 9 * slot));
+            if (par->store_vertex && out->J) {
+                EPSX_HIP(ctx, hipMemcpyAsync(out->J + 2 * n * slot, dJ, 16 * n, hipMemcpyDeviceToHost, st));
+                EPSX_HIP(ctx, hipStreamSynchronize(st));
+            }
+            if (par->verbosity > 0)
+                fprintf(stderr, "ladder T = %g, Ω = %g: niter %lld converged %d\n", T, Om, (long long)niter, conv);
+        }
+    }
+    EPSX_HIP(ctx, hipStreamSynchronize(st));
+    return EPSX_OK;
+}

@@ -418,6 +418,9 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     EPSX_TRY(upload(ctx, s.iks_f, P->iks_f, sizeof(int32_t) * nf));
     EPSX_TRY(upload(ctx, s.e_i, P->e_i, sizeof(double) * ni));
     EPSX_TRY(upload(ctx, s.w_i, P->w_i, sizeof(double) * ni));
+    s.w_i_host.assign(P->w_i, P->w_i + ni);
+    ctx->l.ready = false;  // a ladder problem belongs to the scGD0 problem it was set on
+    ctx->l.bt_T = -1;
     if (P->n_mult > 0) {
         EPSX_TRY(upload(ctx, s.mult_ptr, P->mult_ptr, sizeof(int64_t) * (P->n_mult + 1)));
         EPSX_TRY(upload(ctx, s.mult_idx, P->mult_idx, sizeof(int32_t) * P->mult_ptr[P->n_mult]));

@@ -48,6 +48,9 @@ EXPORTS = [
      [P, I64, I64, I64, P, P, P, P, P, I64, D, D, P, I64, P, D, D, D, I32, P]),
     ("epsx_scgd0_set_problem", C.c_int, [P, P]),
     ("epsx_scgd0_run", C.c_int, [P, P, P]),
+    ("epsx_ladder_set_problem", C.c_int, [P, P]),
+    ("epsx_ladder_run", C.c_int, [P, P, P]),
+    ("epsx_apply_ladder_kernel", C.c_int, [P, P, P, P, D, D, I32, P]),
 ]
 
 
@@ -103,6 +106,26 @@ class Scgd0Outputs(C.Structure):
         ("hist_err", P), ("hist_mu", P), ("hist_minA", P), ("wall", P), ("iter_frozen", P),
         ("niter", I64),
         ("fm_method_used", I32), ("n_low_precision", I32),
+    ]
+
+
+class LadderProblem(C.Structure):
+    _fields_ = [("struct_size", I64), ("R", P), ("v", P)]
+
+
+class LadderParams(C.Structure):
+    _fields_ = [
+        ("struct_size", I64), ("nT", I64), ("nOmega", I64),
+        ("Tlist", P), ("mu", P), ("Sigma", P), ("Omegas", P),
+        ("solver", I32), ("precision", I32), ("rtol", D), ("maxiter", I64), ("w_acoustic", D),
+        ("store_vertex", I32), ("verbosity", I32),
+    ]
+
+
+class LadderOutputs(C.Structure):
+    _fields_ = [
+        ("struct_size", I64), ("sigma_bubble", P), ("sigma_ladder", P), ("converged", P), ("niter", P),
+        ("norms", P), ("sigma_iter", P), ("J", P),
     ]
 
 

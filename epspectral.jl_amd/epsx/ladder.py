@@ -59,3 +59,123 @@ def setup_ladder(setup: Scgd0Setup, R=None, v=None):
     klo, wlo, whi = vertex_shift_tables(ωs, nshift)
     return LadderSetup(scgd0=setup, R=R, v=v, inner_states_by_star=by_star, inner_star_ptr=ptr, nshift=nshift,
                        dω_unit=float(h), ωs_weights=trapezoid_weights(ωs), klo=klo, wlo=wlo, whi=whi)
+
+
+@dataclasses.dataclass
+class SCGD0LadderResult:
+    """`SCGD0LadderResult` (src/ladder.jl:27-38)."""
+    Ωs: np.ndarray
+    Tlist: np.ndarray
+    μ: np.ndarray
+    σ_bubble: np.ndarray        # (3, 3, nΩ, nT), symmetrised, with spin_degeneracy / volume
+    σ_ladder: np.ndarray
+    converged: np.ndarray       # (nΩ, nT)
+    niter: np.ndarray
+    history: list               # [iΩ][iT] -> dict(norms, σ_iter)
+    dω_unit: float
+    J: object = None
+
+
+class LadderDevice:
+    """A ladder problem on the device, on top of an uploaded scGD0 setup (`Scgd0Device`)."""
+
+    def __init__(self, dev, ladder: LadderSetup):
+        import ctypes
+        from . import _lib
+        self.dev, self.ctx, self.ladder = dev, dev.ctx, ladder
+        R = np.asfortranarray(ladder.R, dtype=np.float64)
+        v = np.asfortranarray(ladder.v, dtype=np.float64)
+        P = _lib.LadderProblem(ctypes.sizeof(_lib.LadderProblem), _lib.ptr(R), _lib.ptr(v))
+        self.ctx.check(_lib.lib().epsx_ladder_set_problem(self.ctx.handle, ctypes.addressof(P)))
+
+    def apply(self, J, Gm, Gp, T, ω_acoustic, precision="fp64"):
+        """`apply_ladder_kernel!` (src/ladder.jl:245-296) on host arrays."""
+        from . import _lib
+        J = np.asfortranarray(J, dtype=np.complex128)
+        Gm = np.asfortranarray(Gm, dtype=np.complex128)
+        Gp = np.asfortranarray(Gp, dtype=np.complex128)
+        LJ = np.empty_like(J, order="F")
+        self.ctx.check(_lib.lib().epsx_apply_ladder_kernel(
+            self.ctx.handle, _lib.ptr(J), _lib.ptr(Gm), _lib.ptr(Gp), float(T), float(ω_acoustic),
+            _lib.PRECISIONS[precision], _lib.ptr(LJ)))
+        return LJ
+
+    def loop(self, Σ, μ, Tlist, Ωs, *, solver="gmres", rtol=1e-2, maxiter=30, store_vertex=False,
+             ω_acoustic=None, precision="fp64", verbosity=0):
+        """`_loop_ladder` (src/ladder.jl:303-379): unsymmetrised σ without spin / volume."""
+        import ctypes
+        import warnings
+        from . import _lib
+        from .scgd0 import OMEGA_ACOUSTIC
+        ω_acoustic = OMEGA_ACOUSTIC if ω_acoustic is None else ω_acoustic
+        if solver not in ("gmres", "fixed_point"):
+            raise ArgumentError(f"solver must be :gmres or :fixed_point, got {solver}")
+        Σ = np.asfortranarray(Σ, dtype=np.complex128)
+        Tl = np.ascontiguousarray(np.atleast_1d(Tlist), dtype=np.float64)
+        mu = np.ascontiguousarray(np.atleast_1d(μ), dtype=np.float64)
+        Om = np.ascontiguousarray(np.atleast_1d(Ωs), dtype=np.float64)
+        nT, nO = len(Tl), len(Om)
+        nw, ni = Σ.shape[0], Σ.shape[1]
+        sb = np.empty((3, 3, nO, nT), order="F")
+        sl = np.empty((3, 3, nO, nT), order="F")
+        conv = np.empty((nO, nT), dtype=np.int32, order="F")
+        nit = np.empty((nO, nT), dtype=np.int64, order="F")
+        norms = np.empty((maxiter + 1, nO, nT), order="F")
+        sit = np.empty((3, 3, maxiter, nO, nT), order="F")
+        J = np.empty((nw, 4, 3, 2, ni, nO, nT), dtype=np.complex128, order="F") if store_vertex else None
+        par = _lib.LadderParams(ctypes.sizeof(_lib.LadderParams), nT, nO, _lib.ptr(Tl), _lib.ptr(mu), _lib.ptr(Σ),
+                                _lib.ptr(Om), 0 if solver == "gmres" else 1, _lib.PRECISIONS[precision], float(rtol),
+                                int(maxiter), float(ω_acoustic), int(store_vertex), int(verbosity))
+        out = _lib.LadderOutputs(ctypes.sizeof(_lib.LadderOutputs), _lib.ptr(sb), _lib.ptr(sl), _lib.ptr(conv),
+                                 _lib.ptr(nit), _lib.ptr(norms), _lib.ptr(sit), _lib.ptr(J))
+        self.ctx.check(_lib.lib().epsx_ladder_run(self.ctx.handle, ctypes.addressof(par), ctypes.addressof(out)))
+        converged = conv.astype(bool)
+        history = [[dict(norms=norms[:nit[iO, iT] + 1, iO, iT].copy() if solver == "gmres" else np.zeros(0),
+                         σ_iter=[sit[:, :, k, iO, iT].copy() for k in range(nit[iO, iT])] if solver != "gmres" else [])
+                    for iT in range(nT)] for iO in range(nO)]
+        if not converged.all():
+            bad = [(Om[a], Tl[b]) for a in range(nO) for b in range(nT) if not converged[a, b]]
+            warnings.warn(f"ladder not converged (rtol = {rtol}, maxiter = {maxiter}) at (Ω, T) = {bad}")
+        return dict(σ_bubble=sb, σ_ladder=sl, converged=converged, niter=nit, history=history, J=J)
+
+
+def symmetrize(σ, ops=None):
+    """The 3x3 tensor averaged over the Cartesian point-group matrices `ops` (identity if None)."""
+    if ops is None:
+        return σ.copy()
+    return sum(g @ σ @ g.T for g in ops) / len(ops)
+
+
+def run_scgd0_ladder(dev, ladder: LadderSetup, scgd0, occ, *, Ωs=(0.0,), solver="gmres", rtol=1e-2, maxiter=30,
+                     store_vertex=False, ω_acoustic=None, symmetry_ops=None, precision="fp64", verbosity=0):
+    """`run_scgd0_ladder` (src/ladder.jl:65-105) on an uploaded scGD0 setup and its result: the
+    argument checks, the loop and σ symmetrised with spin_degeneracy / volume."""
+    from .scgd0 import OMEGA_ACOUSTIC
+    ω_acoustic = OMEGA_ACOUSTIC if ω_acoustic is None else ω_acoustic
+    if solver not in ("gmres", "fixed_point"):
+        raise ArgumentError(f"solver must be :gmres or :fixed_point, got {solver}")
+    if maxiter < 1:
+        raise ArgumentError(f"maxiter must be at least 1, got {maxiter}")
+    if len(Ωs) == 0:
+        raise ArgumentError("Ωs must hold at least one probe frequency")
+    if not ω_acoustic > 0:
+        raise ArgumentError(f"ω_acoustic must be positive, got {ω_acoustic}: the shift-grid coupling build has no "
+                            "row for a negative phonon frequency")
+    if not (len(occ.Tlist) == len(scgd0.Tlist) and np.allclose(occ.Tlist, scgd0.Tlist)):
+        raise ArgumentError(f"occ.Tlist = {list(occ.Tlist)} must equal the temperatures of scgd0, {list(scgd0.Tlist)}")
+    ld = LadderDevice(dev, ladder)
+    r = ld.loop(scgd0.Σ, scgd0.μ, scgd0.Tlist, np.asarray(Ωs, dtype=float), solver=solver, rtol=rtol,
+                maxiter=maxiter, store_vertex=store_vertex, ω_acoustic=ω_acoustic, precision=precision,
+                verbosity=verbosity)
+    fac = occ.spin_degeneracy / occ.volume
+
+    def sym(σ):
+        out = np.empty_like(σ)
+        for iT in range(σ.shape[3]):
+            for iO in range(σ.shape[2]):
+                out[:, :, iO, iT] = fac * symmetrize(σ[:, :, iO, iT], symmetry_ops)
+        return out
+
+    return SCGD0LadderResult(np.asarray(Ωs, dtype=float), np.asarray(scgd0.Tlist), np.asarray(scgd0.μ),
+                             sym(r["σ_bubble"]), sym(r["σ_ladder"]), r["converged"], r["niter"], r["history"],
+                             ladder.dω_unit, r["J"])

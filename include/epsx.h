@@ -179,6 +179,59 @@ int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *prob);
  * unconverged returns EPSX_OK with converged[T] = 0 (the reference warns). */
 int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd0_outputs *out);
 
+/* ---- ladder ---------------------------------------------------------------------------- */
+/* What _setup_ladder adds to the scGD0 setup (src/ladder.jl:112-158): the rotation R[:, :, f] of
+ * each inner state's vertex and the band velocities v[cart, i].  The shift tables (step
+ * dω_unit of ωs, nshift = ceil(max ωph / dω_unit) + 2), the inner-state stars and the trapezoid
+ * weights are derived by the library.  Needs epsx_scgd0_set_problem first (same states, grid
+ * and coupling). */
+typedef struct {
+    int64_t struct_size;      /* sizeof(epsx_ladder_problem) */
+    const double *R;          /* [3, 3, nf] */
+    const double *v;          /* [3, ni] */
+} epsx_ladder_problem;
+
+#define EPSX_SOLVER_GMRES 0
+#define EPSX_SOLVER_FIXED_POINT 1
+
+typedef struct {
+    int64_t struct_size;      /* sizeof(epsx_ladder_params) */
+    int64_t nT, nOmega;
+    const double *Tlist;      /* [nT] */
+    const double *mu;         /* [nT] */
+    const double *Sigma;      /* complex [nw, ni, nT]: the scGD0 self-energy */
+    const double *Omegas;     /* [nOmega], >= 0 */
+    int32_t solver;           /* EPSX_SOLVER_* */
+    int32_t precision;        /* GEMM operands: EPSX_PREC_FP64, _FP32 or _BF16 (FP32 sums) */
+    double rtol;
+    int64_t maxiter;
+    double w_acoustic;        /* > 0 */
+    int32_t store_vertex;
+    int32_t verbosity;
+} epsx_ladder_params;
+
+typedef struct {
+    int64_t struct_size;      /* sizeof(epsx_ladder_outputs) */
+    double *sigma_bubble;     /* [3, 3, nOmega, nT], unsymmetrised, without spin/volume */
+    double *sigma_ladder;     /* [3, 3, nOmega, nT] */
+    int32_t *converged;       /* [nOmega, nT] */
+    int64_t *niter;           /* [nOmega, nT] */
+    double *norms;            /* [maxiter + 1, nOmega, nT] GMRES residual norms, NaN-padded */
+    double *sigma_iter;       /* [3, 3, maxiter, nOmega, nT] fixed-point iterates, NaN-padded */
+    double *J;                /* NULL, or complex [nw, 4, 3, 2, ni, nOmega, nT] */
+} epsx_ladder_outputs;
+
+int epsx_ladder_set_problem(epsx_ctx *ctx, const epsx_ladder_problem *prob);
+/* _loop_ladder, src/ladder.jl:303-379: per (T, Ω) the propagators, the bare vertex and
+ * σ_bubble, then (I - L) J = J0 by GMRES (warm-started from the previous Ω, stopping at
+ * rtol ||J0||) or the fixed point, and σ_ladder.  An unconverged solve returns EPSX_OK with
+ * converged = 0 (the reference warns). */
+int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, epsx_ladder_outputs *out);
+/* apply_ladder_kernel!, src/ladder.jl:245-296, on host buffers: LJ = L(J) at temperature T for
+ * the propagators Gm, Gp (complex [nw, 4, ni]); J, LJ complex [nw, 4, 3, 2, ni]. */
+int epsx_apply_ladder_kernel(epsx_ctx *ctx, const double *J, const double *Gm, const double *Gp,
+                             double T, double w_acoustic, int32_t precision, double *LJ);
+
 #ifdef __cplusplus
 }
 #endif

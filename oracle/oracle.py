@@ -63,6 +63,13 @@ def _p(a):
     return None if a is None else a.ctypes.data
 
 
+def _c(fn, *args):
+    """Call a foreign function with numpy arrays passed as data pointers.  The arrays stay
+    referenced by `args` until the call returns (a pointer taken from a temporary copy would
+    dangle once the copy is freed)."""
+    return fn(*[a.ctypes.data if isinstance(a, np.ndarray) else a for a in args])
+
+
 def _f64(a):
     return np.asfortranarray(a, dtype=np.float64)
 
@@ -76,11 +83,11 @@ def _i64(a):
 
 
 def occ_fermion(e, T):
-    return lib().orc_occ_fermion(float(e), float(T))
+    return _c(lib().orc_occ_fermion, float(e), float(T))
 
 
 def occ_boson(e, T):
-    return lib().orc_occ_boson(float(e), float(T))
+    return _c(lib().orc_occ_boson, float(e), float(T))
 
 
 class OracleError(Exception):
@@ -94,7 +101,7 @@ def grid_from_nodes(ws, atol=1e-8):
     o, u, dev, bu = np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1)
     j = np.zeros(1, dtype=np.int64)
     off = np.zeros(n, dtype=np.int64)
-    rc = lib().orc_grid_from_nodes(n, _p(ws), atol, _p(o), _p(u), _p(off), _p(dev), _p(j), _p(bu))
+    rc = _c(lib().orc_grid_from_nodes, n, (ws), atol, (o), (u), (off), (dev), (j), (bu))
     if rc:
         raise OracleError(rc, int(j[0]), float(dev[0]), float(bu[0]))
     return float(o[0]), float(u[0]), off
@@ -105,7 +112,7 @@ def graded_grid(windows):
     cap = 1 << 22
     off = np.zeros(cap, dtype=np.int64)
     o, u = np.zeros(1), np.zeros(1)
-    n = lib().orc_graded_grid(len(windows), _p(w), _p(o), _p(u), _p(off), cap)
+    n = _c(lib().orc_graded_grid, len(windows), (w), (o), (u), (off), cap)
     if n < 0:
         raise OracleError(int(n))
     return float(o[0]), float(u[0]), off[:n].copy()
@@ -114,11 +121,11 @@ def graded_grid(windows):
 def cell_search_table(ws, dw_min):
     ws = _f64(ws)
     x0, ih = np.zeros(1), np.zeros(1)
-    n = lib().orc_cell_search_table(len(ws), _p(ws), dw_min, _p(x0), _p(ih), None, 0)
+    n = _c(lib().orc_cell_search_table, len(ws), (ws), dw_min, (x0), (ih), None, 0)
     if n < 0:
         raise OracleError(int(n))
     jtab = np.zeros(n, dtype=np.int32)
-    lib().orc_cell_search_table(len(ws), _p(ws), dw_min, _p(x0), _p(ih), _p(jtab), n)
+    _c(lib().orc_cell_search_table, len(ws), (ws), dw_min, (x0), (ih), (jtab), n)
     return jtab, float(x0[0]), float(ih[0])
 
 
@@ -127,7 +134,7 @@ def interp_linear_grid(ys, col, x, ws, table):
     ws = _f64(ws)
     jtab, x0, ih = table
     out = np.zeros(2)
-    lib().orc_interp_linear_grid(_p(ys), col, float(x), len(ws), _p(ws), _p(jtab), x0, ih, _p(out))
+    _c(lib().orc_interp_linear_grid, (ys), col, float(x), len(ws), (ws), (jtab), x0, ih, (out))
     return complex(out[0], out[1])
 
 
@@ -135,7 +142,7 @@ def linear_interpolation_table(ws, xs):
     ws, xs = _f64(ws), _f64(xs)
     j = np.zeros(len(xs), dtype=np.int32)
     t = np.zeros(len(xs))
-    lib().orc_linear_interpolation_table(len(ws), _p(ws), len(xs), _p(xs), _p(j), _p(t))
+    _c(lib().orc_linear_interpolation_table, len(ws), (ws), len(xs), (xs), (j), (t))
     return j, t
 
 
@@ -143,22 +150,22 @@ def apply_interpolation_table(ys, j, t):
     ys = np.asfortranarray(ys, dtype=np.complex128)
     ys2 = ys.reshape(ys.shape[0], -1, order="F")
     Y = np.zeros((len(j), ys2.shape[1]), dtype=np.complex128, order="F")
-    lib().orc_apply_interpolation_table(len(j), ys2.shape[0], ys2.shape[1], _p(ys2), _p(_i32(j)),
-                                        _p(_f64(t)), _p(Y))
+    _c(lib().orc_apply_interpolation_table, len(j), ys2.shape[0], ys2.shape[1], (ys2), (_i32(j)),
+                                        (_f64(t)), (Y))
     return Y
 
 
 def trapezoid_weights(ws):
     ws = _f64(ws)
     W = np.zeros(len(ws))
-    lib().orc_trapezoid_weights(len(ws), _p(ws), _p(W))
+    _c(lib().orc_trapezoid_weights, len(ws), (ws), (W))
     return W
 
 
 def kramers_kronig_matrix(ws):
     ws = _f64(ws)
     K = np.zeros((len(ws), len(ws)), order="F")
-    lib().orc_kramers_kronig_matrix(len(ws), _p(ws), _p(K))
+    _c(lib().orc_kramers_kronig_matrix, len(ws), (ws), (K))
     return K
 
 
@@ -169,15 +176,15 @@ def add_fan_migdal(ImS, g2, wq_fi, e_f, f_to_i, w_f, ws, table, Sin, mu, T, w_ac
     f1 = nf if f1 is None else f1
     ImS = np.asfortranarray(ImS, dtype=np.float64)
     jtab, x0, ih = table
-    lib().orc_add_fan_migdal(nm, ni, nf, f0, f1, _p(_f64(g2)), _p(_f64(wq_fi)), _p(_f64(e_f)),
-                             _p(_i32(f_to_i)), _p(_f64(w_f)), len(ws), _p(_f64(ws)), _p(jtab), x0, ih,
-                             Sin.shape[1], _p(np.asfortranarray(Sin, dtype=np.complex128)), mu, T,
-                             w_acoustic, _p(ImS), nthreads)
+    _c(lib().orc_add_fan_migdal, nm, ni, nf, f0, f1, (_f64(g2)), (_f64(wq_fi)), (_f64(e_f)),
+                             (_i32(f_to_i)), (_f64(w_f)), len(ws), (_f64(ws)), (jtab), x0, ih,
+                             Sin.shape[1], (np.asfortranarray(Sin, dtype=np.complex128)), mu, T,
+                             w_acoustic, (ImS), nthreads)
     return ImS
 
 
 def ncarrier_spectral(mu, T, wsa, D, W, count_holes, nstates_base, spin_degeneracy):
-    return lib().orc_ncarrier_spectral(mu, T, len(wsa), _p(_f64(wsa)), _p(_f64(D)), W, int(count_holes),
+    return _c(lib().orc_ncarrier_spectral, mu, T, len(wsa), (_f64(wsa)), (_f64(D)), W, int(count_holes),
                                        nstates_base, spin_degeneracy)
 
 
@@ -185,7 +192,7 @@ def build_multiplets(iks, es, cutoff):
     ni = len(es)
     ptr = np.zeros(ni + 1, dtype=np.int64)
     idx = np.zeros(ni, dtype=np.int64)
-    nm = lib().orc_build_multiplets(ni, _p(_i32(iks)), _p(_f64(es)), cutoff, _p(ptr), _p(idx))
+    nm = _c(lib().orc_build_multiplets, ni, (_i32(iks)), (_f64(es)), cutoff, (ptr), (idx))
     return [list(idx[ptr[m]:ptr[m + 1]]) for m in range(nm)]
 
 
@@ -249,7 +256,7 @@ def loop_scgd0(setup, Tlist, nlist, nelec, *, η_min, η_init, maxiter, tol, ω_
     wall = np.zeros(maxiter)
     fr = np.zeros(nT, dtype=np.int64)
     O = _Out(_p(Σ), _p(μ), _p(conv), _p(he), _p(hm), _p(ha), _p(wall), _p(fr), 0, 0, 0, 0.0, (D * 2)(0, 0))
-    rc = lib().orc_loop_scgd0(C.byref(S), C.byref(Pm), C.byref(O))
+    rc = _c(lib().orc_loop_scgd0, C.byref(S), C.byref(Pm), C.byref(O))
     if rc:
         raise OracleError(rc, int(O.err_T))
     n = int(O.niter)
@@ -287,7 +294,7 @@ def _c128(a):
 
 def contour_green(w, e, S, mu, T, c12):
     out = np.zeros(2)
-    _ladder_sigs().orc_contour_green(w, e, S.real, S.imag, mu, T, c12, _p(out))
+    _c(_ladder_sigs().orc_contour_green, w, e, S.real, S.imag, mu, T, c12, (out))
     return complex(out[0], out[1])
 
 
@@ -296,22 +303,22 @@ def contour_greens(ws, e_i, Sd_m, Sd_p, Omega, mu, T):
     nw, ni = len(ws), len(e_i)
     Gm = np.zeros((nw, 4, ni), dtype=np.complex128, order="F")
     Gp = np.zeros_like(Gm)
-    _ladder_sigs().orc_contour_greens(nw, ni, _p(ws), _p(_f64(e_i)), _p(_c128(Sd_m)), _p(_c128(Sd_p)), Omega, mu, T,
-                                      _p(Gm), _p(Gp))
+    _c(_ladder_sigs().orc_contour_greens, nw, ni, (ws), (_f64(e_i)), (_c128(Sd_m)), (_c128(Sd_p)), Omega, mu, T,
+                                      (Gm), (Gp))
     return Gm, Gp
 
 
 def attach_greens(Gm, X, Gp):
     nw, _, ni = Gm.shape
     J = np.zeros((nw, 4, 3, 2, ni), dtype=np.complex128, order="F")
-    _ladder_sigs().orc_attach_greens(nw, ni, _p(_c128(Gm)), _p(_c128(X)), _p(_c128(Gp)), _p(J))
+    _c(_ladder_sigs().orc_attach_greens, nw, ni, (_c128(Gm)), (_c128(X)), (_c128(Gp)), (J))
     return J
 
 
 def attach_greens_bare(Gm, v, Gp):
     nw, _, ni = Gm.shape
     J = np.zeros((nw, 4, 3, 2, ni), dtype=np.complex128, order="F")
-    _ladder_sigs().orc_attach_greens_bare(nw, ni, _p(_c128(Gm)), _p(_f64(v)), _p(_c128(Gp)), _p(J))
+    _c(_ladder_sigs().orc_attach_greens_bare, nw, ni, (_c128(Gm)), (_f64(v)), (_c128(Gp)), (J))
     return J
 
 
@@ -319,8 +326,8 @@ def conductivity_from_vertex(J, v, w_i, Ww, Omega, T):
     nw = J.shape[0]
     ni = J.shape[4]
     s = np.zeros((3, 3), order="F")
-    _ladder_sigs().orc_conductivity_from_vertex(nw, ni, _p(_c128(J)), _p(_f64(v)), _p(_f64(w_i)), _p(_f64(Ww)),
-                                                Omega, T, _p(s))
+    _c(_ladder_sigs().orc_conductivity_from_vertex, nw, ni, (_c128(J)), (_f64(v)), (_f64(w_i)), (_f64(Ww)),
+                                                Omega, T, (s))
     return s
 
 
@@ -330,7 +337,7 @@ def vertex_shift_tables(offsets, nshift):
     klo = np.zeros((nw, nshift, 2), dtype=np.int32, order="F")
     wlo = np.zeros((nw, nshift, 2), order="F")
     whi = np.zeros((nw, nshift, 2), order="F")
-    _ladder_sigs().orc_vertex_shift_tables(nw, _p(off), nshift, _p(klo), _p(wlo), _p(whi))
+    _c(_ladder_sigs().orc_vertex_shift_tables, nw, (off), nshift, (klo), (wlo), (whi))
     return klo, wlo, whi
 
 
@@ -367,8 +374,8 @@ def _lsetup(ld):
 def apply_ladder_kernel(ld, J, Gm, Gp, T, w_acoustic, gpu_tile=None, nthreads=8):
     S, keep = _lsetup(ld)
     LJ = np.zeros(J.shape, dtype=np.complex128, order="F")
-    _ladder_sigs().orc_apply_ladder_kernel(C.byref(S), _p(_c128(J)), _p(_c128(Gm)), _p(_c128(Gp)), T, w_acoustic,
-                                           0 if gpu_tile is None else int(gpu_tile), _p(LJ), nthreads)
+    _c(_ladder_sigs().orc_apply_ladder_kernel, C.byref(S), (_c128(J)), (_c128(Gm)), (_c128(Gp)), T, w_acoustic,
+                                           0 if gpu_tile is None else int(gpu_tile), (LJ), nthreads)
     return LJ
 
 
@@ -386,8 +393,8 @@ def ladder_solve(ld, Sigma_T, mu, T, Omega, *, solver="gmres", rtol=1e-2, maxite
     Pm = _LParams(0 if solver == "gmres" else 1, float(rtol), int(maxiter), int(warm),
                   0 if gpu_tile is None else int(gpu_tile), float(w_acoustic), int(nthreads))
     Ww = trapezoid_weights(np.asarray(ld.ωs))
-    _ladder_sigs().orc_ladder_solve(C.byref(S), _p(_c128(Sigma_T)), _p(_f64(ld.ε_i)), _p(_f64(ld.w_i)), _p(_f64(ld.v)),
-                                    _p(_f64(Ww)), float(Omega), float(mu), float(T), C.byref(Pm), _p(Jw), C.byref(O))
+    _c(_ladder_sigs().orc_ladder_solve, C.byref(S), (_c128(Sigma_T)), (_f64(ld.ε_i)), (_f64(ld.w_i)), (_f64(ld.v)),
+                                    (_f64(Ww)), float(Omega), float(mu), float(T), C.byref(Pm), (Jw), C.byref(O))
     return dict(J=Jw, σ_bubble=np.array(O.sigma_bubble[:]).reshape(3, 3, order="F"),
                 σ_ladder=np.array(O.sigma_ladder[:]).reshape(3, 3, order="F"), converged=bool(O.converged),
                 niter=int(O.niter), norms=norms[:O.n_norms].copy(),

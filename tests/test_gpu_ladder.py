@@ -1,0 +1,110 @@
+"""GPU parity of the ladder: libepsx (through its C ABI) against the CPU oracle.
+
+Tolerances: one kernel application 1e-12 relative (the device regroups the shift sums, see
+epsx_ladder.hip, so the sum order differs); σ and J of full solves 1e-10 relative (the
+reference's CPU-vs-GPU gate, test/test_ladder_gpu.jl:46-48) with equal iteration counts;
+bitwise run to run.
+"""
+import warnings
+
+import numpy as np
+import pytest
+
+import epsx
+import oracle
+from epsx.ladder import LadderDevice, setup_ladder
+from test_oracle_ladder import disordered_case, holstein_case, propagators
+from ref_numpy import pseudorandom_complex
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b))
+
+
+def device(ctx, s, L):
+    dev = epsx.Scgd0Device(s, ctx)
+    return LadderDevice(dev, L)
+
+
+@pytest.mark.parametrize("case", ["holstein", "forward"])
+def test_apply_ladder_kernel(ctx, case):
+    s, L, r = holstein_case(False) if case == "holstein" else disordered_case()
+    ld = device(ctx, s, L)
+    nw, ni = len(s.ωs), s.g2.shape[1]
+    J = pseudorandom_complex(nw, 4, 3, 2, ni, seed=1.3)
+    J[0] = 0
+    J[-1] = 0
+    for iT, T in enumerate((0.1, 0.2)):
+        Gm, Gp = propagators(L, r, iT, 0.2, T)
+        ref = oracle.apply_ladder_kernel(L, J, Gm, Gp, T, 1e-6)
+        LJ = ld.apply(J, Gm, Gp, T, 1e-6)
+        assert rel(LJ, ref) <= 1e-12
+        assert np.array_equal(ld.apply(J, Gm, Gp, T, 1e-6), LJ)
+
+
+def oracle_loop(L, r, Ωs, T_idx, **kw):
+    out = []
+    for iT in T_idx:
+        T = (0.1, 0.2)[iT]
+        J = None
+        for Ω in Ωs:
+            o = oracle.ladder_solve(L, r["Σ"][:, :, iT], r["μ"][iT], T, Ω, J=J, **kw)
+            J = o["J"]
+            out.append(o)
+    return out
+
+
+def test_gmres_loop_against_oracle(ctx):
+    s, L, r = disordered_case()
+    ld = device(ctx, s, L)
+    Ωs = [0.0, 0.1, 0.3]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dv = ld.loop(r["Σ"], r["μ"], [0.1, 0.2], Ωs, rtol=1e-8, maxiter=60, store_vertex=True)
+    ref = oracle_loop(L, r, Ωs, (0, 1), rtol=1e-8, maxiter=60)
+    k = 0
+    for iT in range(2):
+        for iO in range(len(Ωs)):
+            o = ref[k]
+            k += 1
+            assert dv["niter"][iO, iT] == o["niter"] and dv["converged"][iO, iT] == o["converged"]
+            assert rel(dv["σ_bubble"][:, :, iO, iT], o["σ_bubble"]) <= 1e-12
+            assert rel(dv["σ_ladder"][:, :, iO, iT], o["σ_ladder"]) <= 1e-10
+            assert rel(dv["J"][..., iO, iT], o["J"]) <= 1e-10
+            h = dv["history"][iO][iT]["norms"]
+            assert len(h) == len(o["norms"]) and np.allclose(h, o["norms"], rtol=1e-6, atol=1e-12 * o["norms"][0])
+    assert dv["converged"].all()
+    # the correction is not negligible, so the agreement is not a trivial one
+    assert np.linalg.norm(dv["σ_ladder"] - dv["σ_bubble"]) > 1e-2 * np.linalg.norm(dv["σ_bubble"])
+    again = ld.loop(r["Σ"], r["μ"], [0.1, 0.2], Ωs, rtol=1e-8, maxiter=60, store_vertex=True)
+    assert np.array_equal(again["J"], dv["J"]) and np.array_equal(again["σ_ladder"], dv["σ_ladder"])
+
+
+def test_fixed_point_against_oracle(ctx):
+    s, L, r = disordered_case()
+    ld = device(ctx, s, L)
+    dv = ld.loop(r["Σ"], r["μ"], [0.1], [0.3], solver="fixed_point", rtol=1e-6, maxiter=200)
+    o = oracle.ladder_solve(L, r["Σ"][:, :, 0], r["μ"][0], 0.1, 0.3, solver="fixed_point", rtol=1e-6, maxiter=200)
+    assert dv["niter"][0, 0] == o["niter"] and dv["converged"][0, 0] and o["converged"]
+    assert rel(dv["σ_ladder"][:, :, 0, 0], o["σ_ladder"]) <= 1e-10
+
+
+def test_holstein_ladder_equals_bubble(ctx):
+    """test/test_ladder_holstein.jl:23-25: with the k -> -k symmetry the current correction
+    vanishes, σ_ladder = σ_bubble, and run_scgd0_ladder symmetrises with spin / volume."""
+    s, L, r = holstein_case(True)
+    dev = epsx.Scgd0Device(s, ctx)
+    res = epsx.SCGD0Result(s.ωs, s.ωs_spectral, r["Σ"], r["μ"], np.array([0.1, 0.2]), np.zeros(2),
+                           r["converged"], r["history"])
+    occ = epsx.Occupation(Tlist=[0.1, 0.2], nlist=0.0, nelec=1.0, spin_degeneracy=2.0, volume=1.0)
+    out = epsx.run_scgd0_ladder(dev, L, res, occ, Ωs=[0.0, 0.2])
+    assert out.converged.all()
+    assert rel(out.σ_ladder, out.σ_bubble) <= 1e-12
+    ref = oracle.ladder_solve(L, r["Σ"][:, :, 1], r["μ"][1], 0.2, 0.2, rtol=1e-2, maxiter=30)
+    assert rel(out.σ_bubble[:, :, 1, 1], 2.0 * ref["σ_bubble"]) <= 1e-12
+    with pytest.raises(epsx.ArgumentError, match="ω_acoustic"):
+        epsx.run_scgd0_ladder(dev, L, res, occ, ω_acoustic=-1.0)
+    with pytest.raises(epsx.ArgumentError, match="solver"):
+        epsx.run_scgd0_ladder(dev, L, res, occ, solver="newton")

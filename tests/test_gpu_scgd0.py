@@ -66,9 +66,12 @@ def run_fm_device(ctx, c, T=None, precision=0):
     wq = np.asfortranarray(c["wq"])
     off = np.ascontiguousarray(ws.offsets)
     Sin = np.asfortranarray(c["Sin"], dtype=np.complex128)
+    e_f = np.ascontiguousarray(c["e_f"], dtype=np.float64)
+    w_f = np.ascontiguousarray(c["w_f"], dtype=np.float64)
+    f_to_i = np.ascontiguousarray(c["f_to_i"], dtype=np.int32)
     ctx.check(_lib.lib().epsx_add_fan_migdal_imsigma(
-        ctx.handle, nm, ni, nf, _lib.ptr(g2), _lib.ptr(wq), _lib.ptr(np.ascontiguousarray(c["e_f"])),
-        _lib.ptr(c["f_to_i"]), _lib.ptr(np.ascontiguousarray(c["w_f"])), len(ws), ws.ω_origin, ws.dω_unit,
+        ctx.handle, nm, ni, nf, _lib.ptr(g2), _lib.ptr(wq), _lib.ptr(e_f),
+        _lib.ptr(f_to_i), _lib.ptr(w_f), len(ws), ws.ω_origin, ws.dω_unit,
         _lib.ptr(off), Sin.shape[1], _lib.ptr(Sin), c["mu"], c["T"] if T is None else T, c["w_ac"],
         precision, _lib.ptr(ImS)))
     return ImS
