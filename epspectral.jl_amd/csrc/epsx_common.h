@@ -132,7 +132,8 @@ void grid_nodes(int64_t n, double origin, double unit, const int64_t *offsets, d
 int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K);
 // C[m, ncol] = A[m, k] * B[k, ncol] (+ C if beta != 0), FP64, column-major, on ctx->stream
 int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A, int64_t lda,
-             const double *B, int64_t ldb, double *C, int64_t ldc, bool accumulate);
+             const double *B, int64_t ldb, double *C, int64_t ldc, bool accumulate,
+             const int32_t *gate = nullptr);   // gate: device flag, 0 = skip the product
 
 // Grid data the Fan-Migdal kernels read (device pointers).
 struct FmGrid {
@@ -171,6 +172,7 @@ struct FmTemps {
     const double *mu;               // device [nT_all]
     const double *eta_min;          // device [nT_all] (clamp); may be null (no clamp)
     const int32_t *tidx;            // device [nT] temperature index of each slot
+    const int32_t *act = nullptr;   // device [nT_all] convergence mask (null: all active)
 };
 
 // One direct Fan-Migdal pass over the launched temperatures: ImS[:, :, tidx[z]] (+)= the sum.
@@ -184,7 +186,7 @@ int star_reduce_launch(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nfs, const
                        double *Gt);
 int h_build_launch(epsx_ctx *ctx, const FmGrid &g, int64_t nm, int64_t ni, const double *wnu, const double *e_i,
                    const double *S, const double *T, const double *mu, const double *eta, const int32_t *tidx,
-                   int nslots, double w_acoustic, double *hout);
+                   int nslots, double w_acoustic, double *hout, const int32_t *act = nullptr);
 size_t fm_part_elems(int64_t nfs, int64_t ni, int64_t nw, int64_t nT, int64_t f_tile);
 int fm_grid_host(int64_t nw, const int64_t *offsets, std::vector<int32_t> &off,
                  std::vector<double> &rgap, std::vector<int32_t> &ltab, int64_t &span);

@@ -131,6 +131,7 @@ __global__ __launch_bounds__(FM_THREADS) void fm_direct_kernel(FmGrid g, FmDirec
     const int64_t i = bx / S;
     const int tz = blockIdx.y;
     const int64_t tcol = tp.tidx[tz];
+    if (tp.act && !tp.act[tcol]) return;
     const double T = tp.T[tcol], mu = tp.mu[tcol];
     const double eta = tp.eta_min ? tp.eta_min[tcol] : -INFINITY;
     const bool step = !(T > 1.4901161193847656e-08);
@@ -295,11 +296,12 @@ __global__ __launch_bounds__(FM_THREADS) void fm_direct_kernel(FmGrid g, FmDirec
 // ImS[ω, i, tidx[tz]] (+)= Σ_s part[tz, s, i, ω], s ascending
 __global__ void fm_reduce_kernel(int64_t nw, int64_t ni, int64_t S, const int32_t *tidx,
                                  const double *__restrict__ part, double *__restrict__ ImS,
-                                 int accumulate)
+                                 int accumulate, const int32_t *act)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int tz = blockIdx.y;
     if (q >= nw * ni) return;
+    if (act && !act[tidx[tz]]) return;
     double acc = 0.0;
     for (int64_t s = 0; s < S; s++) acc += part[((int64_t)tz * S + s) * nw * ni + q];
     double *o = ImS + nw * ni * (int64_t)tidx[tz] + q;
@@ -312,14 +314,20 @@ int launch_R(epsx_ctx *ctx, const FmGrid &g, const FmDirectArgs &a, const FmTemp
              size_t lds)
 {
     dim3 grid((unsigned)(a.ni * S * nwt), (unsigned)tp.nT);
+    // The LDS limit is raised outside graph capture; a captured launch repeats an eager one.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    EPSX_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
+    const bool set_attr = cap == hipStreamCaptureStatusNone;
     if (precision == EPSX_PREC_FP32) {
         auto k = fm_direct_kernel<R, float>;
-        EPSX_HIP(ctx, hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (set_attr)
+            EPSX_HIP(ctx, hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k, grid, dim3(FM_THREADS), lds, ctx->stream, g, a, tp, S, seg_len, nwt,
                            star_ptr, part);
     } else {
         auto k = fm_direct_kernel<R, double>;
-        EPSX_HIP(ctx, hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (set_attr)
+            EPSX_HIP(ctx, hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k, grid, dim3(FM_THREADS), lds, ctx->stream, g, a, tp, S, seg_len, nwt,
                            star_ptr, part);
     }
@@ -390,7 +398,7 @@ int fm_direct_launch_sp(epsx_ctx *ctx, const FmGrid &g, const FmDirectArgs &a, c
     if (r != EPSX_OK) return r;
     dim3 grid((unsigned)((g.nw * a.ni + 255) / 256), (unsigned)tp.nT);
     hipLaunchKernelGGL(fm_reduce_kernel, grid, dim3(256), 0, ctx->stream, g.nw, a.ni, S, tp.tidx,
-                       d_part, d_ImS, accumulate ? 1 : 0);
+                       d_part, d_ImS, accumulate ? 1 : 0, tp.act);
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
 }

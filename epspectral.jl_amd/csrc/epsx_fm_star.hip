@@ -86,6 +86,7 @@ struct HArgs {
     const int32_t *tidx;      // slot -> temperature index
     double w_acoustic;
     double *hout;             // [nw, nm, ni] per slot
+    const int32_t *act;       // convergence mask per temperature index, or null
 };
 
 __device__ inline double2 lookup(const double2 *col, int64_t nw, int64_t span, const int32_t *off,
@@ -114,6 +115,7 @@ __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
     if (q >= total) return;
     const int64_t w = q % nw, nu = (q / nw) % a.nm, ip = q / (nw * a.nm);
     const int tcol = a.tidx[z];
+    if (a.act && !a.act[tcol]) return;
     double *out = a.hout + total * (int64_t)z;
     const double wnu = a.wnu[nu];
     if (wnu < a.w_acoustic) { out[q] = 0.0; return; }
@@ -183,9 +185,10 @@ int star_reduce_launch(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nfs, const
 
 int h_build_launch(epsx_ctx *ctx, const FmGrid &g, int64_t nm, int64_t ni, const double *wnu, const double *e_i,
                    const double *S, const double *T, const double *mu, const double *eta, const int32_t *tidx,
-                   int nslots, double w_acoustic, double *hout)
+                   int nslots, double w_acoustic, double *hout, const int32_t *act)
 {
     HArgs a;
+    a.act = act;
     a.nw = g.nw; a.nm = nm; a.ni = ni; a.span = g.span;
     a.ws = g.ws; a.off = g.off; a.rgap = g.rgap; a.ltab = g.ltab; a.h = g.dw_unit;
     a.wnu = wnu; a.e_i = e_i; a.S = S; a.T = T; a.mu = mu; a.eta = eta; a.tidx = tidx;

@@ -40,8 +40,9 @@ __global__ __launch_bounds__(256) void dgemm_nn_kernel(int64_t m, int64_t ncol, 
                                                        const double *__restrict__ A, int64_t lda,
                                                        const double *__restrict__ B, int64_t ldb,
                                                        double *__restrict__ C, int64_t ldc,
-                                                       int accumulate)
+                                                       int accumulate, const int32_t *__restrict__ gate)
 {
+    if (gate && *gate == 0) return;  // a masked (converged) instance
     __shared__ double As[BK][BM + 1];
     __shared__ double Bs[BK][BN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -81,14 +82,15 @@ __global__ __launch_bounds__(256) void dgemm_nn_kernel(int64_t m, int64_t ncol, 
         }
         __syncthreads();
     }
-    // D[i][j] of a 16x16 tile: lane = j + 16 * (i / 4), element i % 4
+    // D[i][j] of a 16x16 f64 tile: element v of lane l holds i = 4 v + l / 16, j = l % 16
+    // (one row per register group, four input blocks)
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int64_t gi = row0 + wm * 32 + a * 16 + (lane >> 4) * 4 + v;
+                const int64_t gi = row0 + wm * 32 + a * 16 + 4 * v + (lane >> 4);
                 const int64_t gj = col0 + wn * 32 + b * 16 + (lane & 15);
                 if (gi < m && gj < ncol) {
                     double *c = C + gi + ldc * gj;
@@ -108,12 +110,12 @@ int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K)
 }
 
 int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A, int64_t lda,
-             const double *B, int64_t ldb, double *C, int64_t ldc, bool accumulate)
+             const double *B, int64_t ldb, double *C, int64_t ldc, bool accumulate, const int32_t *gate)
 {
     if (m <= 0 || ncol <= 0) return EPSX_OK;
     dim3 grid((unsigned)((m + BM - 1) / BM), (unsigned)((ncol + BN - 1) / BN));
     hipLaunchKernelGGL(dgemm_nn_kernel, grid, dim3(256), 0, ctx->stream, m, ncol, k, A, lda, B,
-                       ldb, C, ldc, accumulate ? 1 : 0);
+                       ldb, C, ldc, accumulate ? 1 : 0, gate);
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
 }

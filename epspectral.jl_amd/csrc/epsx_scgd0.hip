@@ -40,20 +40,25 @@ __global__ void fill_sigma_kernel(int64_t n, double2 *S, double eta_init)
 
 // Σ_new[:, :, t] = complex(ReΣ, ImΣ[:, :, t])  (src/scgd0.jl:291)
 __global__ void combine_kernel(int64_t n, const double *__restrict__ Re, const double *__restrict__ Im,
-                               double2 *__restrict__ Snew)
+                               double2 *__restrict__ Snew, const int32_t *__restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q < n) Snew[q] = make_double2(Re[q], Im[q]);
 }
+
+// the device iteration counter of the graph-replayed loop
+__global__ void iter_inc_kernel(int32_t *it) { if (threadIdx.x == 0 && blockIdx.x == 0) *it += 1; }
 
 // Multiplet average (src/scgd0.jl:294-300): members summed in order, divided by the count,
 // written to every member (so members are bitwise equal).
 __global__ void multiplet_kernel(int64_t nw, int64_t ni, int64_t n_mult, const int64_t *__restrict__ ptr,
                                  const int32_t *__restrict__ idx, const int32_t *__restrict__ tidx,
-                                 double2 *__restrict__ Snew)
+                                 double2 *__restrict__ Snew, const int32_t *__restrict__ act)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nw * n_mult) return;
+    if (act && !act[tidx[blockIdx.y]]) return;
     const int64_t w = q % nw, m = q / nw;
     double2 *St = Snew + nw * ni * (int64_t)tidx[blockIdx.y];
     const int64_t a = ptr[m], b = ptr[m + 1];
@@ -74,11 +79,12 @@ __global__ void multiplet_kernel(int64_t nw, int64_t ni, int64_t n_mult, const i
 __global__ void dos_A_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *__restrict__ wsa,
                              const double *__restrict__ e_i, const int32_t *__restrict__ jsp,
                              const double *__restrict__ tsp, const int32_t *__restrict__ tidx,
-                             const double2 *__restrict__ Snew, double *__restrict__ A)
+                             const double2 *__restrict__ Snew, double *__restrict__ A, const int32_t *__restrict__ act)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= nwa * ni) return;
     const int z = blockIdx.y;
+    if (act && !act[tidx[z]]) return;
     const int64_t a = q % nwa, i = q / nwa;
     const double2 *St = Snew + nw * ni * (int64_t)tidx[z] + nw * i;
     const int64_t j = jsp[a] - 1;
@@ -100,11 +106,12 @@ __global__ void dos_A_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *
 // D[a, z] = (Σ_i A[a, i] w_i) Wω[a]  (src/scgd0.jl:306-307), i ascending
 __global__ void dos_D_kernel(int64_t ni, int64_t nwa, const double *__restrict__ A,
                              const double *__restrict__ w_i, const double *__restrict__ Wa,
-                             double *__restrict__ D)
+                             double *__restrict__ D, const int32_t *__restrict__ tidx, const int32_t *__restrict__ act)
 {
     const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
     if (a >= nwa) return;
+    if (act && !act[tidx[z]]) return;
     const double *Az = A + nwa * ni * (int64_t)z;
     double acc = 0;
     for (int64_t i = 0; i < ni; i++) acc = fma(Az[a + nwa * i], w_i[i], acc);
@@ -129,11 +136,13 @@ __device__ inline double block_sum(double v, double *red)
 
 // ∫A_i = Σ_a A[a, i] Wω[a] for one outer state per block (src/scgd0.jl:308)
 __global__ __launch_bounds__(256) void dos_int_kernel(int64_t ni, int64_t nwa, const double *__restrict__ A,
-                                                      const double *__restrict__ Wa, double *__restrict__ integ)
+                                                      const double *__restrict__ Wa, double *__restrict__ integ,
+                                                      const int32_t *__restrict__ tidx, const int32_t *__restrict__ act)
 {
     __shared__ double red[256];
     const int64_t i = blockIdx.x;
     const int z = blockIdx.y;
+    if (act && !act[tidx[z]]) return;
     const double *Ai = A + nwa * ni * (int64_t)z + nwa * i;
     double acc = 0;
     for (int64_t a = threadIdx.x; a < nwa; a += 256) acc = fma(Ai[a], Wa[a], acc);
@@ -142,10 +151,12 @@ __global__ __launch_bounds__(256) void dos_int_kernel(int64_t ni, int64_t nwa, c
 }
 
 __global__ __launch_bounds__(256) void min_kernel(int64_t ni, const double *__restrict__ integ,
-                                                  const int32_t *__restrict__ tidx, double *__restrict__ minA)
+                                                  const int32_t *__restrict__ tidx, double *__restrict__ minA,
+                                                  const int32_t *__restrict__ act)
 {
     __shared__ double red[256];
     const int z = blockIdx.x;
+    if (act && !act[tidx[z]]) return;
     double m = INFINITY;
     for (int64_t i = threadIdx.x; i < ni; i += 256) {
         const double v = integ[i + ni * (int64_t)z];
@@ -189,6 +200,7 @@ struct MuArgs {
     int is_semi;
     double *mu;              // per temperature index
     int32_t *status;         // per temperature index
+    const int32_t *act;      // convergence mask or null
 };
 
 // The μ root of n(μ) = n_target on the bracket extrema(ωs_spectral) ± 0.1 (src/scgd0.jl:310-324,
@@ -201,6 +213,7 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
     __shared__ int done;
     const int z = blockIdx.x;
     const int tcol = m.tidx[z];
+    if (m.act && !m.act[tcol]) return;
     const double T = m.T[tcol];
     const double n_target = m.nlist[tcol] + m.nelec;
     const int holes = m.is_semi && m.nlist[tcol] < 0;
@@ -249,10 +262,11 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
 // reduced by err_final_kernel.
 __global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t *__restrict__ tidx,
                                                        const double2 *__restrict__ Snew, double2 *__restrict__ S,
-                                                       double *__restrict__ bmax)
+                                                       double *__restrict__ bmax, const int32_t *__restrict__ act)
 {
     __shared__ double red[256];
     const int z = blockIdx.y;
+    if (act && !act[tidx[z]]) return;
     const int64_t base = n * (int64_t)tidx[z];
     double m = 0;
     for (int64_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) {
@@ -273,17 +287,43 @@ __global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t 
     if (threadIdx.x == 0) bmax[blockIdx.x + gridDim.x * z] = red[0];
 }
 
+// Device-side bookkeeping of the graph-replayed loop: the history row of this iteration, the
+// freeze of a converged temperature (src/scgd0.jl:326-333).
+struct GraphBook {
+    int32_t *act;            // [nT] convergence mask
+    int64_t *frozen;         // [nT] iteration at which T froze
+    const int32_t *it;       // 0-based iteration counter
+    double *hist_err, *hist_mu, *hist_minA;  // [nT, maxiter]
+    const double *mu, *minA;
+    int64_t nT, maxiter;
+    double tol;
+};
+
 __global__ void err_final_kernel(int nb, const double *__restrict__ bmax, const int32_t *__restrict__ tidx,
-                                 double *__restrict__ err)
+                                 double *__restrict__ err, GraphBook g)
 {
     const int z = blockIdx.x;
     if (threadIdx.x != 0) return;
+    const int t = tidx[z];
+    if (g.act && !g.act[t]) return;
     double m = 0;
     for (int b = 0; b < nb; b++) {
         const double o = bmax[b + nb * z];
         m = (o > m || o != o) ? o : m;
     }
-    err[tidx[z]] = m;
+    err[t] = m;
+    if (g.act) {
+        const int it = *g.it;
+        if (it < g.maxiter) {
+            g.hist_err[t + g.nT * it] = m;
+            g.hist_mu[t + g.nT * it] = g.mu[t];
+            g.hist_minA[t + g.nT * it] = g.minA[t];
+        }
+        if (m < g.tol) {
+            g.act[t] = 0;
+            g.frozen[t] = it + 1;
+        }
+    }
 }
 
 inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
@@ -494,7 +534,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     EPSX_TRY(tmp_alloc(ctx, B.S.b, 16 * nS));
     EPSX_TRY(tmp_alloc(ctx, B.Snew.b, 16 * nS));
     EPSX_TRY(tmp_alloc(ctx, B.ImS.b, 8 * nS));
-    EPSX_TRY(tmp_alloc(ctx, B.ReS.b, 8 * (size_t)nw * ni));
+    EPSX_TRY(tmp_alloc(ctx, B.ReS.b, 8 * (size_t)nw * ni * nT));
     EPSX_TRY(tmp_alloc(ctx, B.A.b, 8 * (size_t)nwa * ni * nT));
     EPSX_TRY(tmp_alloc(ctx, B.D.b, 8 * (size_t)nwa * nT));
     EPSX_TRY(tmp_alloc(ctx, B.integ.b, 8 * (size_t)ni * nT));
@@ -568,10 +608,6 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     fa.iks_i = dptr<int32_t>(s.iks_i); fa.iks_f = dptr<int32_t>(s.iks_f); fa.nki = s.nki;
     fa.S = B.S.ptr<double>(); fa.ncol = ni; fa.w_acoustic = par->w_acoustic;
 
-    std::vector<int32_t> active(nT, 1), act_idx;
-    std::vector<double> h_err(nT), h_mu(nT), h_minA(nT);
-    std::vector<int32_t> h_status(nT);
-    std::vector<double> mu_host(par->mu_start, par->mu_start + nT);
     for (int64_t q = 0; q < nT * maxiter; q++) out->hist_err[q] = out->hist_mu[q] = out->hist_minA[q] = NAN;
     for (int64_t t = 0; t < nT; t++) out->iter_frozen[t] = 0;
     for (int k = 0; k < 4; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
@@ -579,31 +615,31 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     hipEvent_t ev[8];
     for (auto &e : ev) EPSX_HIP(ctx, hipEventCreate(&e));
     struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int k = 0; k < 8; k++) (void)hipEventDestroy(e[k]); } } eg{ev};
-
     out->niter = 0;
     out->fm_method_used = method;
     out->n_low_precision = 0;
-    int rc = EPSX_OK;
-    for (int64_t iter = 1; iter <= maxiter; iter++) {
-        const double t0 = now_s();
-        act_idx.clear();
-        for (int64_t t = 0; t < nT; t++) if (active[t]) act_idx.push_back((int32_t)t);
-        const int nA = (int)act_idx.size();
-        EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, act_idx.data(), 4 * nA, hipMemcpyHostToDevice, st));
-        const int32_t *d_tidx = B.tidx.ptr<int32_t>();
-        // 1-3. clamp + Fan-Migdal (the clamp is applied when the column is staged)
-        EPSX_HIP(ctx, hipEventRecord(ev[0], st));
+    int nb_err = (int)((nw * ni + 255) / 256);
+    nb_err = nb_err > nbmax ? nbmax : nb_err;
+
+    // One iteration of src/scgd0.jl:252-333 for the nA temperature slots in d_tidx.  With a
+    // device mask (graph mode) every kernel skips converged temperatures, and the convergence
+    // test writes the history and freezes temperatures on the device.
+    auto enqueue = [&](int nA, const int32_t *d_tidx, const std::vector<int32_t> &h_tidx, const int32_t *d_act,
+                       const GraphBook &book, bool events) -> int {
+        if (events) EPSX_HIP(ctx, hipEventRecord(ev[0], st));
         FmTemps tp;
         tp.T = B.T.ptr<double>(); tp.mu = B.mu.ptr<double>(); tp.eta_min = B.eta.ptr<double>();
+        tp.act = d_act;
+        // 1-3. clamp + Fan-Migdal (the clamp is applied where Σin is read)
         if (method == EPSX_FM_STAR) {
-            // h for every active temperature in one launch, then one GEMM per temperature
             EPSX_TRY(h_build_launch(ctx, g, s.nm, ni, dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(),
                                     B.T.ptr<double>(), B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA,
-                                    par->w_acoustic, hbuf.ptr<double>()));
+                                    par->w_acoustic, hbuf.ptr<double>(), d_act));
             for (int z = 0; z < nA; z++) {
-                const int64_t t = act_idx[z];
+                const int64_t t = h_tidx[z];
                 EPSX_TRY(dgemm_nn(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>() + nw * s.nm * ni * z, nw,
-                                  dptr<double>(s.Gt), s.nm * ni, B.ImS.ptr<double>() + nw * ni * t, nw, false));
+                                  dptr<double>(s.Gt), s.nm * ni, B.ImS.ptr<double>() + nw * ni * t, nw, false,
+                                  d_act ? d_act + t : nullptr));
             }
         } else if (par->batched) {
             tp.nT = nA;
@@ -619,37 +655,39 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                                              B.ImS.ptr<double>(), false));
             }
         }
-        EPSX_HIP(ctx, hipEventRecord(ev[1], st));
+        if (events) EPSX_HIP(ctx, hipEventRecord(ev[1], st));
         // 4. Re Σ by Kramers-Kronig
         for (int z = 0; z < nA; z++) {
-            const int64_t t = act_idx[z];
+            const int64_t t = h_tidx[z];
+            const int32_t *gate = d_act ? d_act + t : nullptr;
             EPSX_TRY(dgemm_nn(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>() + nw * ni * t, nw,
-                              B.ReS.ptr<double>(), nw, false));
+                              B.ReS.ptr<double>() + nw * ni * z, nw, false, gate));
             hipLaunchKernelGGL(combine_kernel, dim3(nblk(nw * ni)), dim3(256), 0, st, nw * ni,
-                               B.ReS.ptr<double>(), B.ImS.ptr<double>() + nw * ni * t,
-                               B.Snew.ptr<double2>() + nw * ni * t);
+                               B.ReS.ptr<double>() + nw * ni * z, B.ImS.ptr<double>() + nw * ni * t,
+                               B.Snew.ptr<double2>() + nw * ni * t, gate);
             EPSX_HIP(ctx, hipGetLastError());
         }
-        EPSX_HIP(ctx, hipEventRecord(ev[2], st));
+        if (events) EPSX_HIP(ctx, hipEventRecord(ev[2], st));
         // 5. multiplets
         if (s.n_mult > 0) {
             hipLaunchKernelGGL(multiplet_kernel, dim3(nblk(nw * s.n_mult), nA), dim3(256), 0, st, nw, ni, s.n_mult,
-                               dptr<int64_t>(s.mult_ptr), dptr<int32_t>(s.mult_idx), d_tidx, B.Snew.ptr<double2>());
+                               dptr<int64_t>(s.mult_ptr), dptr<int32_t>(s.mult_idx), d_tidx, B.Snew.ptr<double2>(),
+                               d_act);
             EPSX_HIP(ctx, hipGetLastError());
         }
         // 6. spectral DOS and min ∫A
         hipLaunchKernelGGL(dos_A_kernel, dim3(nblk(nwa * ni), nA), dim3(256), 0, st, nw, ni, nwa,
                            dptr<double>(s.wsa), dptr<double>(s.e_i), dptr<int32_t>(s.jsp), dptr<double>(s.tsp),
-                           d_tidx, B.Snew.ptr<double2>(), B.A.ptr<double>());
+                           d_tidx, B.Snew.ptr<double2>(), B.A.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(dos_D_kernel, dim3(nblk(nwa), nA), dim3(256), 0, st, ni, nwa, B.A.ptr<double>(),
-                           dptr<double>(s.w_i), dptr<double>(s.Wa), B.D.ptr<double>());
+                           dptr<double>(s.w_i), dptr<double>(s.Wa), B.D.ptr<double>(), d_tidx, d_act);
         EPSX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(dos_int_kernel, dim3((unsigned)ni, nA), dim3(256), 0, st, ni, nwa, B.A.ptr<double>(),
-                           dptr<double>(s.Wa), B.integ.ptr<double>());
+                           dptr<double>(s.Wa), B.integ.ptr<double>(), d_tidx, d_act);
         EPSX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
-                           B.minA.ptr<double>());
+                           B.minA.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
         // 7. μ
         if (!par->fix_mu) {
@@ -658,64 +696,159 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             m.T = B.T.ptr<double>(); m.nlist = B.nlist.ptr<double>();
             m.nelec = par->nelec; m.spin = par->spin_degeneracy; m.W = s.W; m.nsb = s.nstates_base;
             m.is_semi = par->is_semiconductor; m.mu = B.mu.ptr<double>(); m.status = B.status.ptr<int32_t>();
+            m.act = d_act;
             hipLaunchKernelGGL(mu_kernel, dim3(nA), dim3(1024), 0, st, m);
             EPSX_HIP(ctx, hipGetLastError());
         }
-        EPSX_HIP(ctx, hipEventRecord(ev[3], st));
+        if (events) EPSX_HIP(ctx, hipEventRecord(ev[3], st));
         // 8-9. err and Σ ← Σ_new
-        int nb = (int)((nw * ni + 255) / 256);
-        nb = nb > nbmax ? nbmax : nb;
-        hipLaunchKernelGGL(err_copy_kernel, dim3(nb, nA), dim3(256), 0, st, nw * ni, d_tidx,
-                           B.Snew.ptr<double2>(), B.S.ptr<double2>(), B.bmax.ptr<double>());
+        hipLaunchKernelGGL(err_copy_kernel, dim3(nb_err, nA), dim3(256), 0, st, nw * ni, d_tidx,
+                           B.Snew.ptr<double2>(), B.S.ptr<double2>(), B.bmax.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(err_final_kernel, dim3(nA), dim3(64), 0, st, nb, B.bmax.ptr<double>(), d_tidx,
-                           B.err.ptr<double>());
+        hipLaunchKernelGGL(err_final_kernel, dim3(nA), dim3(64), 0, st, nb_err, B.bmax.ptr<double>(), d_tidx,
+                           B.err.ptr<double>(), book);
         EPSX_HIP(ctx, hipGetLastError());
-        EPSX_HIP(ctx, hipEventRecord(ev[4], st));
-        EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
-        EPSX_HIP(ctx, hipMemcpyAsync(h_mu.data(), B.mu.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
-        EPSX_HIP(ctx, hipMemcpyAsync(h_minA.data(), B.minA.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
-        EPSX_HIP(ctx, hipMemcpyAsync(h_status.data(), B.status.b.p, 4 * nT, hipMemcpyDeviceToHost, st));
-        EPSX_HIP(ctx, hipStreamSynchronize(st));
+        if (d_act) {
+            hipLaunchKernelGGL(iter_inc_kernel, dim3(1), dim3(64), 0, st, const_cast<int32_t *>(book.it));
+            EPSX_HIP(ctx, hipGetLastError());
+        }
+        if (events) EPSX_HIP(ctx, hipEventRecord(ev[4], st));
+        return EPSX_OK;
+    };
+    auto add_event_times = [&](int nA) -> int {
         float ms[4];
         for (int k = 0; k < 4; k++) {
             EPSX_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
-            ctx->t_class[k == 0 ? 0 : (k == 1 ? 1 : (k == 2 ? 2 : 3))] += ms[k] * 1e-3;
+            ctx->t_class[k] += ms[k] * 1e-3;
         }
         ctx->n_class[0] += (par->batched || method == EPSX_FM_STAR) ? 1 : nA;
         ctx->n_class[1] += nA;
-        for (int z = 0; z < nA; z++) {
-            const int64_t t = act_idx[z];
-            if (!par->fix_mu && h_status[t] == 2) {
-                const double n_target = par->nlist[t] + par->nelec;
-                rc = set_error(ctx, EPSX_ERR_NO_MU_ROOT,
-                               "no chemical potential in (%.17g, %.17g) gives %.17g electrons per cell at "
-                               "T = %.17g: the window holds too few states on one side of the gap, or the "
-                               "spectral weight it loses outside ωs_spectral moves the count past the target. "
-                               "Pass occ.μlist and fix_μ = true, or widen the window or ωs.",
-                               s.wsa_host[0] - 0.1, s.wsa_host[nwa - 1] + 0.1, n_target, par->Tlist[t]);
-                break;
+        ctx->n_class[2] += 1;
+        ctx->n_class[3] += 1;
+        return EPSX_OK;
+    };
+    auto no_root = [&](int64_t t) {
+        const double n_target = par->nlist[t] + par->nelec;
+        return set_error(ctx, EPSX_ERR_NO_MU_ROOT,
+                         "no chemical potential in (%.17g, %.17g) gives %.17g electrons per cell at "
+                         "T = %.17g: the window holds too few states on one side of the gap, or the "
+                         "spectral weight it loses outside ωs_spectral moves the count past the target. "
+                         "Pass occ.μlist and fix_μ = true, or widen the window or ωs.",
+                         s.wsa_host[0] - 0.1, s.wsa_host[nwa - 1] + 0.1, n_target, par->Tlist[t]);
+    };
+
+    std::vector<int32_t> active(nT, 1), act_idx;
+    std::vector<double> h_err(nT), h_mu(nT), h_minA(nT);
+    std::vector<int32_t> h_status(nT);
+    std::vector<double> mu_host(par->mu_start, par->mu_start + nT);
+    int rc = EPSX_OK;
+    if (!par->batched) {
+        // The reference's loop: the active temperatures one after another, decisions on the host.
+        GraphBook none{};
+        for (int64_t iter = 1; iter <= maxiter; iter++) {
+            const double t0 = now_s();
+            act_idx.clear();
+            for (int64_t t = 0; t < nT; t++) if (active[t]) act_idx.push_back((int32_t)t);
+            const int nA = (int)act_idx.size();
+            EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, act_idx.data(), 4 * nA, hipMemcpyHostToDevice, st));
+            EPSX_TRY(enqueue(nA, B.tidx.ptr<int32_t>(), act_idx, nullptr, none, true));
+            EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(h_mu.data(), B.mu.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(h_minA.data(), B.minA.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(h_status.data(), B.status.b.p, 4 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipStreamSynchronize(st));
+            EPSX_TRY(add_event_times(nA));
+            for (int z = 0; z < nA; z++) {
+                const int64_t t = act_idx[z];
+                if (!par->fix_mu && h_status[t] == 2) { rc = no_root(t); break; }
+                mu_host[t] = h_mu[t];
+                out->hist_minA[t + nT * (iter - 1)] = h_minA[t];
+                out->hist_err[t + nT * (iter - 1)] = h_err[t];
+                out->hist_mu[t + nT * (iter - 1)] = h_mu[t];
+                if (h_err[t] < par->tol) {
+                    active[t] = 0;
+                    out->iter_frozen[t] = iter;
+                }
             }
-            mu_host[t] = h_mu[t];
-            out->hist_minA[t + nT * (iter - 1)] = h_minA[t];
-            out->hist_err[t + nT * (iter - 1)] = h_err[t];
-            out->hist_mu[t + nT * (iter - 1)] = h_mu[t];
-            if (h_err[t] < par->tol) {
-                active[t] = 0;
-                out->iter_frozen[t] = iter;
+            if (rc != EPSX_OK) break;
+            out->wall[iter - 1] = now_s() - t0;
+            out->niter = iter;
+            if (par->verbosity > 0) {
+                fprintf(stderr, "scGD0 iteration %lld: wall %.3f s\n", (long long)iter, out->wall[iter - 1]);
+                for (int z = 0; z < nA; z++)
+                    fprintf(stderr, "  T=%g err=%.6g mu=%.12g\n", par->Tlist[act_idx[z]], h_err[act_idx[z]], h_mu[act_idx[z]]);
             }
+            bool any = false;
+            for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
+            if (!any) break;
         }
-        if (rc != EPSX_OK) break;
-        out->wall[iter - 1] = now_s() - t0;
-        out->niter = iter;
-        if (par->verbosity > 0) {
-            fprintf(stderr, "scGD0 iteration %lld: wall %.3f s\n", (long long)iter, out->wall[iter - 1]);
-            for (int z = 0; z < nA; z++)
-                fprintf(stderr, "  T=%g err=%.6g mu=%.12g\n", par->Tlist[act_idx[z]], h_err[act_idx[z]], h_mu[act_idx[z]]);
+    } else {
+        // Batched: every temperature in each launch, converged ones masked on the device.  The
+        // first iteration runs eagerly with timing events; the iteration is then captured in a
+        // hipGraph and replayed, the host reading back only the mask and the status words.
+        std::vector<int32_t> all(nT);
+        for (int64_t t = 0; t < nT; t++) all[t] = (int32_t)t;
+        Tmp d_act, d_frozen, d_it, d_he, d_hm, d_ha;
+        EPSX_TRY(tmp_upload(ctx, d_act, active.data(), 4 * nT));
+        std::vector<int64_t> zeros64(nT, 0);
+        EPSX_TRY(tmp_upload(ctx, d_frozen, zeros64.data(), 8 * nT));
+        const int32_t zero = 0;
+        EPSX_TRY(tmp_upload(ctx, d_it, &zero, 4));
+        EPSX_TRY(tmp_upload(ctx, d_he, out->hist_err, 8 * nT * maxiter));
+        EPSX_TRY(tmp_upload(ctx, d_hm, out->hist_mu, 8 * nT * maxiter));
+        EPSX_TRY(tmp_upload(ctx, d_ha, out->hist_minA, 8 * nT * maxiter));
+        EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, all.data(), 4 * nT, hipMemcpyHostToDevice, st));
+        GraphBook book;
+        book.act = d_act.ptr<int32_t>(); book.frozen = d_frozen.ptr<int64_t>(); book.it = d_it.ptr<int32_t>();
+        book.hist_err = d_he.ptr<double>(); book.hist_mu = d_hm.ptr<double>(); book.hist_minA = d_ha.ptr<double>();
+        book.mu = B.mu.ptr<double>(); book.minA = B.minA.ptr<double>();
+        book.nT = nT; book.maxiter = maxiter; book.tol = par->tol;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        struct GraphGuard {
+            hipGraph_t *g; hipGraphExec_t *e;
+            ~GraphGuard() { if (*e) (void)hipGraphExecDestroy(*e); if (*g) (void)hipGraphDestroy(*g); }
+        } gg{&graph, &exec};
+        for (int64_t iter = 1; iter <= maxiter; iter++) {
+            const double t0 = now_s();
+            if (iter == 1) {
+                EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, true));
+            } else {
+                if (!exec) {
+                    EPSX_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, false);
+                    hipError_t ec = hipStreamEndCapture(st, &graph);
+                    if (r != EPSX_OK) return r;
+                    if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture", __FILE__, __LINE__);
+                    EPSX_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                }
+                EPSX_HIP(ctx, hipGraphLaunch(exec, st));
+            }
+            EPSX_HIP(ctx, hipMemcpyAsync(active.data(), book.act, 4 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(h_status.data(), B.status.b.p, 4 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipStreamSynchronize(st));
+            if (iter == 1) EPSX_TRY(add_event_times((int)nT));
+            for (int64_t t = 0; t < nT; t++)
+                if (!par->fix_mu && h_status[t] == 2) { rc = no_root(t); break; }
+            if (rc != EPSX_OK) break;
+            out->wall[iter - 1] = now_s() - t0;
+            out->niter = iter;
+            bool any = false;
+            for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
+            if (par->verbosity > 0) fprintf(stderr, "scGD0 iteration %lld: wall %.4f s\n", (long long)iter, out->wall[iter - 1]);
+            if (!any) break;
         }
-        bool any = false;
-        for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
-        if (!any) break;
+        if (rc == EPSX_OK) {
+            EPSX_HIP(ctx, hipMemcpyAsync(out->hist_err, book.hist_err, 8 * nT * maxiter, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(out->hist_mu, book.hist_mu, 8 * nT * maxiter, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(out->hist_minA, book.hist_minA, 8 * nT * maxiter, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(out->iter_frozen, book.frozen, 8 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(mu_host.data(), B.mu.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipStreamSynchronize(st));
+            // history beyond the last iteration is NaN, as in the reference
+            for (int64_t q = nT * out->niter; q < nT * maxiter; q++)
+                out->hist_err[q] = out->hist_mu[q] = out->hist_minA[q] = NAN;
+        }
     }
     if (rc != EPSX_OK) return rc;
     EPSX_HIP(ctx, hipMemcpyAsync(out->Sigma, B.S.b.p, 16 * nS, hipMemcpyDeviceToHost, st));
@@ -723,11 +856,6 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     for (int64_t t = 0; t < nT; t++) {
         out->mu[t] = mu_host[t];
         out->converged[t] = !active[t];
-    }
-    if (par->maxiter > 1) {
-        bool any = false;
-        for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
-        if (any) set_error(ctx, EPSX_OK, "scGD0 not converged after maxiter = %lld iterations", (long long)maxiter);
     }
     return EPSX_OK;
 }
