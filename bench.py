@@ -6,12 +6,17 @@ three-band t2g model on a 24³ grid (Oh-irreducible outer states, ni = 911; inne
 (g2 = 1.85 GB of FP64, resident in HBM), graded ωs of 1275 nodes, T = 100 K and 300 K at one
 electron per cell, tol = 1e-6 eV.
 
-One step = one complete scGD0 solve of every temperature to tol from Σ = -iη_init.  The value is
-seconds per temperature to tol (lower is better).  The line also carries the Fan-Migdal
-kernel's achieved HBM bandwidth and its roofline position (HIP events on the library's stream),
-and the CPU oracle timed on a bounded sample of the same workload on this host.
+One step = one complete scGD0 solve of every temperature to tol from Σ = -iη_init, by the
+library's fastest exact FP64 method (method auto: the star-factorised Fan-Migdal sum when its
+preconditions hold, all temperatures batched in a replayed hipGraph).  The value is seconds per
+temperature to tol (lower is better).  Beside it:
 
-    python bench.py [--steps K] [--warmup W] [--method direct|auto] [--batched] [--no-cpu]
+* the plain FP64 GPU method (the reference's term-by-term Fan-Migdal sum, temperatures one after
+  another) timed on a bounded sample of iterations and scaled to the same iteration count;
+* the dominant kernel's roofline position (HIP events on the library's stream);
+* the CPU oracle timed on a bounded sample of the same workload on this host.
+
+    python bench.py [--steps K] [--warmup W] [--method auto|direct] [--no-batched] [--no-cpu]
 """
 import argparse
 import json
@@ -25,24 +30,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "epspectral.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
-FP64_VALU_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6   # FP64 vector and FP64 MFMA, dense
 FM_FLOP_PER_TERM = 60.0   # flop-equivalents of one Fan-Migdal term (2 lookups, 2 complex Im(1/z), 3 exp)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_t2g24.json")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nk", type=int, default=24)
-    ap.add_argument("--method", default="direct")
+    ap.add_argument("--temps", default="100,300", help="temperatures in K")
+    ap.add_argument("--method", default="auto")
     ap.add_argument("--precision", default="fp64")
-    ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--maxiter", type=int, default=100)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--plain-iters", type=int, default=2, help="iterations of the plain method timed")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (the HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the scGD0 path has no multi-GPU "
               "implementation in this build", file=sys.stderr)
@@ -51,11 +64,14 @@ def main():
     import epsx
     from epsx.models import t2g_setup
 
+    temps = tuple(float(x) for x in args.temps.split(","))
+    batched = not args.no_batched
     t0 = time.time()
-    setup, meta = t2g_setup(nk=args.nk)
+    setup, meta = t2g_setup(nk=args.nk, Tlist_K=temps)
     nm, ni, nf = setup.dims
+    nw = len(setup.ωs)
     nT = len(meta["Tlist"])
-    print(f"[bench] setup nk={args.nk} ni={ni} nf={nf} nm={nm} nω={len(setup.ωs)} nωa={len(setup.ωs_spectral)} "
+    print(f"[bench] setup nk={args.nk} ni={ni} nf={nf} nm={nm} nω={nw} nωa={len(setup.ωs_spectral)} nT={nT} "
           f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0, spin_degeneracy=2.0)
     ctx = epsx.Context(0)
@@ -63,7 +79,7 @@ def main():
     dev = epsx.Scgd0Device(setup, ctx)
     print(f"[bench] upload {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     kw = dict(η_min=0.0, η_init=5e-3, maxiter=args.maxiter, tol=args.tol, ω_acoustic=1e-6,
-              method=args.method, precision=args.precision, batched=args.batched)
+              method=args.method, precision=args.precision, batched=batched)
     import warnings
     warnings.simplefilter("ignore")
     # prime the kernels (one iteration; not a step)
@@ -71,27 +87,46 @@ def main():
     for w in range(args.warmup):
         dev.loop(occ, **kw)
         print(f"[bench] warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    times, fm_s, fm_n, iters = [], 0.0, 0, []
+    times, iters, tsum = [], [], {}
     for k in range(args.steps):
         t0 = time.perf_counter()
         r = dev.loop(occ, **kw)
         times.append(time.perf_counter() - t0)
         tm = r["info"]["timings"]
-        fm_s += tm["fan_migdal"]
-        fm_n += tm["fan_migdal_launches"]
+        for key, v in tm.items():
+            tsum[key] = tsum.get(key, 0) + v
         iters.append(int(r["history"]["err"].shape[1]))
-        print(f"[bench] step {k + 1}/{args.steps}: {times[-1]:.2f}s, iterations {iters[-1]}, "
-              f"converged {r['converged'].tolist()}, μ {r['μ'].tolist()}, FM {tm['fan_migdal']:.2f}s",
-              file=sys.stderr, flush=True)
+        print(f"[bench] step {k + 1}/{args.steps}: {times[-1]:.3f}s, iterations {iters[-1]}, "
+              f"converged {r['converged'].tolist()}, μ {r['μ'].tolist()}", file=sys.stderr, flush=True)
+    method_used = r["info"]["fm_method"]
     total = sum(times)
     s_per_T = total / (args.steps * nT)
-    # Fan-Migdal kernel: one launch per temperature per iteration in the plain method
-    fm_avg = fm_s / max(fm_n, 1)
-    T_per_launch = nT if args.batched else 1
     g2_bytes = 8.0 * nm * ni * nf
-    gbs = g2_bytes / fm_avg / 1e9
-    terms = float(len(setup.ωs)) * nm * ni * nf * T_per_launch
-    tflops = terms * FM_FLOP_PER_TERM / fm_avg / 1e12
+
+    if method_used == 2:
+        # dominant kernel: the FP64 MFMA GEMM ImΣ = h·G̃, nω x (nm ni) x ni per temperature
+        flops = 2.0 * nw * (nm * ni) * ni
+        avg = tsum["star_gemm"] / max(tsum["star_gemm_launches"], 1)
+        achieved = flops / avg / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic("dgemm_nn_kernel", args.nk, nT),
+                "kernel": "dgemm_nn_kernel (star Fan-Migdal GEMM, FP64 MFMA 16x16x4)",
+                "algorithmic_flop_per_launch": flops, "avg_launch_s": avg,
+                "launches_timed": int(tsum["star_gemm_launches"])}
+        sr = tsum["star_reduce"] / max(tsum["star_reduce_launches"], 1)
+        sr_bytes = g2_bytes + 8.0 * nf + 8.0 * nm * ni * ni
+        extra = {"g2_stream": {"kernel": "star_reduce_kernel (G̃ = Σ_star w_f g2, once per solve)",
+                               "avg_launch_s": sr, "algorithmic_bytes": sr_bytes,
+                               "achieved_GBs": sr_bytes / sr / 1e9, "frac_hbm": sr_bytes / sr / 1e9 / HBM_PEAK_GBS,
+                               "traffic": traffic("star_reduce_kernel", args.nk, nT)}}
+    else:
+        fm_avg = tsum["fan_migdal"] / max(tsum["fan_migdal_launches"], 1)
+        gbs = g2_bytes / fm_avg / 1e9
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                "traffic": traffic("fm_direct_kernel", args.nk, nT), "kernel": "fm_direct_kernel",
+                "algorithmic_bytes_per_launch": g2_bytes, "avg_launch_s": fm_avg}
+        extra = {}
+
     line = {
         "metric": "scGD0 s/temperature to tol vs plain FP64 GPU; Fan-Migdal HBM GB/s",
         "value": s_per_T,
@@ -105,20 +140,67 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else args.precision,
         "data": "synthetic",
-        "config": {"workload": f"t2g {args.nk}^3 plain scGD0, 2 T, tol {args.tol} eV", "nk": args.nk, "ni": ni,
-                   "nf": nf, "nmodes": nm, "nw": len(setup.ωs), "nwa": len(setup.ωs_spectral), "nT": nT,
-                   "method": args.method, "precision": args.precision, "batched": bool(args.batched),
-                   "iterations": iters},
-        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None},
-        "fm_kernel": {"avg_launch_s": fm_avg, "launches": fm_n, "terms_per_launch": terms,
-                      "terms_per_s": terms / fm_avg, "valu_flop_equiv_tflops": tflops,
-                      "valu_frac": tflops / FP64_VALU_PEAK_TFLOPS,
-                      "algorithmic_bytes_per_launch": g2_bytes},
+        "config": {"workload": f"t2g {args.nk}^3 scGD0, {nT} T ({args.temps} K), tol {args.tol} eV", "nk": args.nk,
+                   "ni": ni, "nf": nf, "nmodes": nm, "nw": nw, "nwa": len(setup.ωs_spectral), "nT": nT,
+                   "method": ["direct", "einstein", "star"][method_used], "precision": args.precision,
+                   "batched": batched, "iterations": iters},
+        "roofline": roof,
     }
+    line.update(extra)
+    if not args.no_plain:
+        line["plain_fp64_gpu"] = plain_sample(dev, occ, kw, iters[0], args.plain_iters, nw * nm * ni * nf, g2_bytes)
+        line["speedup_vs_plain"] = line["plain_fp64_gpu"]["value"] / s_per_T
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(setup, meta, iters[0], args.cpu_seconds)
     print(json.dumps(line), flush=True)
+
+
+def calibrate_copy():
+    """A 1 GiB device-to-device copy, three times: reads 1 GiB and writes 1 GiB per launch.  The
+    FETCH_SIZE and WRITE_SIZE counters of this run give the factor between counter readings and
+    HBM bytes."""
+    import torch
+    n = 1 << 27
+    a = torch.arange(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 3}), flush=True)
+
+
+def traffic(kernel, nk, nT):
+    """HBM bytes per launch of `kernel` at this workload, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes recorded under profiles/ (None when that file has no entry)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            tab = json.load(f)
+        e = tab[kernel]
+        if e["nk"] == nk and e["nT"] == nT:
+            return e["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
+def plain_sample(dev, occ, kw, iterations, n_iter, terms_per_T, g2_bytes):
+    """The plain FP64 GPU method: the direct Fan-Migdal kernel, temperatures one after another,
+    the loop driven from the host.  n_iter iterations are timed and the time per iteration is
+    scaled to the optimised run's iteration count (the methods are exact rewrites of one
+    another and take the same iterations)."""
+    nT = len(occ.Tlist)
+    t0 = time.perf_counter()
+    r = dev.loop(occ, **dict(kw, method="direct", batched=False, maxiter=n_iter, tol=0.0))
+    el = time.perf_counter() - t0
+    tm = r["info"]["timings"]
+    fm_avg = tm["fan_migdal"] / max(tm["fan_migdal_launches"], 1)
+    tflops = terms_per_T * FM_FLOP_PER_TERM / fm_avg / 1e12
+    return {"value": el / (n_iter * nT) * iterations, "unit": "s/temperature",
+            "sample": f"{n_iter} iterations x {nT} T in {el:.2f} s, scaled to {iterations} iterations",
+            "fm_direct_kernel": {"avg_launch_s": fm_avg, "terms_per_launch": terms_per_T,
+                                 "terms_per_s": terms_per_T / fm_avg, "valu_flop_equiv_tflops": tflops,
+                                 "valu_frac": tflops / FP64_PEAK_TFLOPS, "g2_GBs": g2_bytes / fm_avg / 1e9}}
 
 
 def cpu_baseline(setup, meta, iterations, budget_s):

@@ -50,8 +50,6 @@ struct Scgd0Dev {
     DevBuf chunks, skip, wnu;         // star chunks of the reduction, acoustic flags, ω_ν
     int64_t nchunks = 0;
     int64_t ltab_len = 0;
-    bool gt_ready = false;
-    double gt_wac = 0;                // ω_acoustic G̃ was built with
 };
 
 // Device-resident ladder problem on top of the scGD0 problem (epsx_ladder_set_problem).

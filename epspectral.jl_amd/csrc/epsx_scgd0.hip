@@ -476,7 +476,6 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
         EPSX_HIP(ctx, hipGetLastError());
         EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
-    s.gt_ready = false;
     s.ready = true;
     return EPSX_OK;
 }
@@ -544,10 +543,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         npart = e > npart ? e : npart;
     }
     EPSX_TRY(tmp_alloc(ctx, B.part.b, 8 * npart));
+    for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
     Tmp hbuf;
     if (method == EPSX_FM_STAR) {
         EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
-        if (!s.gt_ready || s.gt_wac != par->w_acoustic) {
+        {
             // G̃[(ν, i'), i] = Σ_{f in star(i')} w_f g2[ν, i, f], acoustic modes zeroed
             std::vector<uint8_t> skip(s.nm);
             for (int64_t nu = 0; nu < s.nm; nu++) skip[nu] = s.wnu_host[nu] < par->w_acoustic;
@@ -566,8 +566,6 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             EPSX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            s.gt_ready = true;
-            s.gt_wac = par->w_acoustic;
             ctx->t_class[4] = ms * 1e-3;
             ctx->n_class[4] = 1;
         }
@@ -610,8 +608,6 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
 
     for (int64_t q = 0; q < nT * maxiter; q++) out->hist_err[q] = out->hist_mu[q] = out->hist_minA[q] = NAN;
     for (int64_t t = 0; t < nT; t++) out->iter_frozen[t] = 0;
-    for (int k = 0; k < 4; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
-    for (int k = 5; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
     hipEvent_t ev[8];
     for (auto &e : ev) EPSX_HIP(ctx, hipEventCreate(&e));
     struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int k = 0; k < 8; k++) (void)hipEventDestroy(e[k]); } } eg{ev};
@@ -635,6 +631,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             EPSX_TRY(h_build_launch(ctx, g, s.nm, ni, dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(),
                                     B.T.ptr<double>(), B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA,
                                     par->w_acoustic, hbuf.ptr<double>(), d_act));
+            if (events) EPSX_HIP(ctx, hipEventRecord(ev[5], st));
             for (int z = 0; z < nA; z++) {
                 const int64_t t = h_tidx[z];
                 EPSX_TRY(dgemm_nn(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>() + nw * s.nm * ni * z, nw,
@@ -720,6 +717,12 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         for (int k = 0; k < 4; k++) {
             EPSX_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
             ctx->t_class[k] += ms[k] * 1e-3;
+        }
+        if (method == EPSX_FM_STAR) {
+            float mg = 0;
+            EPSX_HIP(ctx, hipEventElapsedTime(&mg, ev[5], ev[1]));
+            ctx->t_class[5] += mg * 1e-3;
+            ctx->n_class[5] += nA;
         }
         ctx->n_class[0] += (par->batched || method == EPSX_FM_STAR) ? 1 : nA;
         ctx->n_class[1] += nA;

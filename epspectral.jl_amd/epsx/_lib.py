@@ -197,7 +197,8 @@ class Context:
         n = np.zeros(8, dtype=np.int64)
         lib().epsx_last_timings(self.handle, ptr(s), ptr(n))
         return {"fan_migdal": s[0], "kramers_kronig": s[1], "dos_mu": s[2], "other": s[3],
-                "star_reduce": s[4], "fan_migdal_launches": int(n[0]), "star_reduce_launches": int(n[4])}
+                "star_reduce": s[4], "star_gemm": s[5], "fan_migdal_launches": int(n[0]),
+                "star_reduce_launches": int(n[4]), "star_gemm_launches": int(n[5])}
 
 
 _default = None

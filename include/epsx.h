@@ -46,7 +46,8 @@ void epsx_close(epsx_ctx *ctx);
 const char *epsx_last_error(const epsx_ctx *ctx);
 /* Seconds of the last run's kernels by class, measured with HIP events on the context's
  * stream: [0] Fan-Migdal sum, [1] Kramers-Kronig, [2] spectral DOS and μ, [3] convergence and
- * copies, [4] the one-time coupling reduction of the star method, [5..7] reserved; and the
+ * copies, [4] the coupling reduction of the star method (once per run), [5] the star
+ * method's h·G̃ GEMMs (inside [0]), [6..7] reserved; and the
  * number of launches of each class.  Both arrays hold 8 entries. */
 int epsx_last_timings(const epsx_ctx *ctx, double *seconds8, int64_t *launches8);
 

@@ -75,7 +75,10 @@ def test_gmres_loop_against_oracle(ctx):
             assert rel(dv["J"][..., iO, iT], o["J"]) <= 1e-10
             h = dv["history"][iO][iT]["norms"]
             assert len(h) == len(o["norms"]) and np.allclose(h, o["norms"], rtol=1e-6, atol=1e-12 * o["norms"][0])
-    assert dv["converged"].all()
+    # GMRES without restart reaches 1e-8 within 60 iterations only at the largest Ω here; the
+    # others stop at maxiter, as the oracle's do
+    assert dv["converged"][-1].all()
+    assert np.all(dv["niter"][~dv["converged"]] == 60)
     # the correction is not negligible, so the agreement is not a trivial one
     assert np.linalg.norm(dv["σ_ladder"] - dv["σ_bubble"]) > 1e-2 * np.linalg.norm(dv["σ_bubble"])
     again = ld.loop(r["Σ"], r["μ"], [0.1, 0.2], Ωs, rtol=1e-8, maxiter=60, store_vertex=True)
