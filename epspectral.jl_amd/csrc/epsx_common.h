@@ -75,8 +75,9 @@ struct epsx_ctx {
     epsx::Scgd0Dev s;
     epsx::LadderDev l;
     std::vector<epsx::DevBuf *> owned;
+    epsx::DevBuf gemm_ws;   // split-k partial products
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
-    // 3 convergence + copies, 4 one-time coupling reduction (star method), 5-7 reserved
+    // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
     double t_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t n_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -128,7 +129,23 @@ void grid_nodes(int64_t n, double origin, double unit, const int64_t *offsets, d
 
 // Kernel launchers (epsx_kk.hip)
 int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K);
-// C[m, ncol] = A[m, k] * B[k, ncol] (+ C if beta != 0), FP64, column-major, on ctx->stream
+
+// A batch of GEMM instances (epsx_gemm.hip).  Instance z uses A + sA a, B + sB b, C + sC c,
+// where a, b, c are z, or idx[z] for the operands flagged in imap (bit 0 A, bit 1 B, bit 2 C).
+// gate (device, indexed by idx[z] or z): 0 skips the instance.  Z/Zim: write
+// Z = complex(result, Zim) at C's positions instead of C.
+struct GemmBatch {
+    int64_t n_inst = 1;
+    int64_t sA = 0, sB = 0, sC = 0;
+    const int32_t *idx = nullptr;
+    int imap = 0;
+    const int32_t *gate = nullptr;
+    double2 *Z = nullptr;
+    const double *Zim = nullptr;
+};
+// C[m, n] = A[m, k] * B[k, n] (+ C if accumulate), FP64, column-major, on ctx->stream
+int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *B,
+          int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt);
 int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A, int64_t lda,
              const double *B, int64_t ldb, double *C, int64_t ldc, bool accumulate,
              const int32_t *gate = nullptr);   // gate: device flag, 0 = skip the product

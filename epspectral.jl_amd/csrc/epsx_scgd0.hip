@@ -38,15 +38,6 @@ __global__ void fill_sigma_kernel(int64_t n, double2 *S, double eta_init)
     if (q < n) S[q] = make_double2(0.0, -eta_init);
 }
 
-// Σ_new[:, :, t] = complex(ReΣ, ImΣ[:, :, t])  (src/scgd0.jl:291)
-__global__ void combine_kernel(int64_t n, const double *__restrict__ Re, const double *__restrict__ Im,
-                               double2 *__restrict__ Snew, const int32_t *__restrict__ gate)
-{
-    if (gate && *gate == 0) return;
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q < n) Snew[q] = make_double2(Re[q], Im[q]);
-}
-
 // the device iteration counter of the graph-replayed loop
 __global__ void iter_inc_kernel(int32_t *it) { if (threadIdx.x == 0 && blockIdx.x == 0) *it += 1; }
 
@@ -483,7 +474,7 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
 namespace {
 
 struct RunBufs {
-    Tmp S, Snew, ImS, ReS, A, D, integ, part, bmax;
+    Tmp S, Snew, ImS, A, D, integ, part, bmax;
     Tmp T, mu, eta, nlist, err, minA, status, tidx;
 };
 
@@ -533,7 +524,6 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     EPSX_TRY(tmp_alloc(ctx, B.S.b, 16 * nS));
     EPSX_TRY(tmp_alloc(ctx, B.Snew.b, 16 * nS));
     EPSX_TRY(tmp_alloc(ctx, B.ImS.b, 8 * nS));
-    EPSX_TRY(tmp_alloc(ctx, B.ReS.b, 8 * (size_t)nw * ni * nT));
     EPSX_TRY(tmp_alloc(ctx, B.A.b, 8 * (size_t)nwa * ni * nT));
     EPSX_TRY(tmp_alloc(ctx, B.D.b, 8 * (size_t)nwa * nT));
     EPSX_TRY(tmp_alloc(ctx, B.integ.b, 8 * (size_t)ni * nT));
@@ -632,12 +622,16 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                                     B.T.ptr<double>(), B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA,
                                     par->w_acoustic, hbuf.ptr<double>(), d_act));
             if (events) EPSX_HIP(ctx, hipEventRecord(ev[5], st));
-            for (int z = 0; z < nA; z++) {
-                const int64_t t = h_tidx[z];
-                EPSX_TRY(dgemm_nn(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>() + nw * s.nm * ni * z, nw,
-                                  dptr<double>(s.Gt), s.nm * ni, B.ImS.ptr<double>() + nw * ni * t, nw, false,
-                                  d_act ? d_act + t : nullptr));
-            }
+            // ImΣ[:, :, t] = h_z G̃ for every slot z in one batched launch (G̃ shared)
+            GemmBatch bt;
+            bt.n_inst = nA;
+            bt.sA = nw * s.nm * ni;
+            bt.sC = nw * ni;
+            bt.idx = d_tidx;
+            bt.imap = 4;
+            bt.gate = d_act;
+            EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
+                           B.ImS.ptr<double>(), nw, false, bt));
         } else if (par->batched) {
             tp.nT = nA;
             tp.tidx = d_tidx;
@@ -653,16 +647,17 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             }
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[1], st));
-        // 4. Re Σ by Kramers-Kronig
-        for (int z = 0; z < nA; z++) {
-            const int64_t t = h_tidx[z];
-            const int32_t *gate = d_act ? d_act + t : nullptr;
-            EPSX_TRY(dgemm_nn(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>() + nw * ni * t, nw,
-                              B.ReS.ptr<double>() + nw * ni * z, nw, false, gate));
-            hipLaunchKernelGGL(combine_kernel, dim3(nblk(nw * ni)), dim3(256), 0, st, nw * ni,
-                               B.ReS.ptr<double>() + nw * ni * z, B.ImS.ptr<double>() + nw * ni * t,
-                               B.Snew.ptr<double2>() + nw * ni * t, gate);
-            EPSX_HIP(ctx, hipGetLastError());
+        // 4. Re Σ by Kramers-Kronig, Σ_new = complex(KK ImΣ, ImΣ) in the GEMM's epilogue
+        {
+            GemmBatch bt;
+            bt.n_inst = nA;
+            bt.sB = bt.sC = nw * ni;
+            bt.idx = d_tidx;
+            bt.imap = 2 | 4;
+            bt.gate = d_act;
+            bt.Z = B.Snew.ptr<double2>();
+            bt.Zim = B.ImS.ptr<double>();
+            EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[2], st));
         // 5. multiplets
