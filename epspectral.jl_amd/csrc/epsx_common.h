@@ -48,6 +48,7 @@ struct Scgd0Dev {
     DevBuf star_ptr;                  // [ni + 1] sorted-position ranges of each star
     DevBuf Gt;                        // star-collapsed coupling G̃ [(ν, i'), i] (built on demand)
     DevBuf chunks, skip, wnu;         // star chunks of the reduction, acoustic flags, ω_ν
+    DevBuf hk, ht;                    // cells and weights of ω ± ω_ν (Einstein modes)
     int64_t nchunks = 0;
     int64_t ltab_len = 0;
 };
@@ -199,9 +200,13 @@ void star_chunks(int64_t nm, const std::vector<int64_t> &star_ptr, std::vector<i
 int star_reduce_launch(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nfs, const double *g2p, const double *wf,
                        const int64_t *d_star_ptr, const int32_t *d_chunks, int64_t nchunks, const uint8_t *d_skip,
                        double *Gt);
-int h_build_launch(epsx_ctx *ctx, const FmGrid &g, int64_t nm, int64_t ni, const double *wnu, const double *e_i,
-                   const double *S, const double *T, const double *mu, const double *eta, const int32_t *tidx,
-                   int nslots, double w_acoustic, double *hout, const int32_t *act = nullptr);
+void star_lookup_host(int64_t nw, int64_t nm, const double *wnu, double h, const std::vector<int32_t> &off,
+                      const std::vector<int32_t> &ltab, const std::vector<double> &rgap, int64_t span,
+                      std::vector<int32_t> &hk, std::vector<double> &ht);
+int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const double *ws, const int32_t *hk,
+                   const double *ht, const double *wnu, const double *e_i, const double *S, const double *T,
+                   const double *mu, const double *eta, const int32_t *tidx, int nslots, double w_acoustic,
+                   double *occ, double *hout, const int32_t *act);
 size_t fm_part_elems(int64_t nfs, int64_t ni, int64_t nw, int64_t nT, int64_t f_tile);
 int fm_grid_host(int64_t nw, const int64_t *offsets, std::vector<int32_t> &off,
                  std::vector<double> &rgap, std::vector<int32_t> &ltab, int64_t &span);

@@ -24,7 +24,8 @@ constexpr int SR_THREADS = 256;
 constexpr int SR_CAP = 4096;   // doubles of g2 staged per block (whole stars)
 
 // One block per (chunk of whole stars, outer state i): the chunk's slab of g2p[:, p, i] is read
-// coalesced into LDS (times w_f), then one thread per (ν, i') sums its star in ascending p.
+// coalesced into LDS (times w_f), then one thread per (ν, i') sums its star in ascending p.  The
+// (ν, p) of a thread's elements advance incrementally: no integer division in the streaming loop.
 __global__ __launch_bounds__(SR_THREADS) void star_reduce_kernel(
     int64_t nm, int64_t ni, int64_t nfs, const double *__restrict__ g2p, const double *__restrict__ wf,
     const int64_t *__restrict__ star_ptr, const int32_t *__restrict__ chunk_star, const uint8_t *__restrict__ skip_mode,
@@ -34,19 +35,25 @@ __global__ __launch_bounds__(SR_THREADS) void star_reduce_kernel(
     const int64_t c = blockIdx.x, i = blockIdx.y;
     const int64_t s0 = chunk_star[c], s1 = chunk_star[c + 1];
     const int64_t p0 = star_ptr[s0], p1 = star_ptr[s1];
-    const int64_t n = (p1 - p0) * nm;
+    const int n = (int)((p1 - p0) * nm);
+    const int nm32 = (int)nm;
     const double *src = g2p + nm * (p0 + nfs * i);
-    const bool staged = n <= SR_CAP;
+    const bool staged = (p1 - p0) * nm <= SR_CAP;
     if (staged) {
-        for (int64_t e = threadIdx.x; e < n; e += SR_THREADS) {
-            const int64_t nu = e % nm;
-            buf[e] = skip_mode[nu] ? 0.0 : wf[p0 + e / nm] * src[e];
+        const int step_p = SR_THREADS / nm32, step_nu = SR_THREADS % nm32;
+        int nu = (int)threadIdx.x % nm32, pl = (int)threadIdx.x / nm32;
+        for (int e = threadIdx.x; e < n; e += SR_THREADS) {
+            buf[e] = skip_mode[nu] ? 0.0 : wf[p0 + pl] * src[e];
+            nu += step_nu;
+            pl += step_p;
+            if (nu >= nm32) { nu -= nm32; pl++; }
         }
         __syncthreads();
     }
-    const int64_t nout = (s1 - s0) * nm;
-    for (int64_t o = threadIdx.x; o < nout; o += SR_THREADS) {
-        const int64_t nu = o % nm, ip = s0 + o / nm;
+    const int nout = (int)((s1 - s0) * nm);
+    for (int o = threadIdx.x; o < nout; o += SR_THREADS) {
+        const int nu = o % nm32;
+        const int64_t ip = s0 + o / nm32;
         double acc = 0.0;
         if (!skip_mode[nu]) {
             const int64_t a = star_ptr[ip], b = star_ptr[ip + 1];
@@ -59,99 +66,82 @@ __global__ __launch_bounds__(SR_THREADS) void star_reduce_kernel(
     }
 }
 
-// e^y = M 2^k (Cody-Waite), as in the direct kernel
-__device__ inline void split_exp(double y, double &M, int &k)
-{
-    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
-    const double log2e = 1.44269504088896338700e+00;
-    if (!(y == y)) { M = y; k = 0; return; }
-    double yk = rint(y * log2e);
-    yk = yk > 1.0e6 ? 1.0e6 : (yk < -1.0e6 ? -1.0e6 : yk);
-    const double r = (y - yk * ln2_hi) - yk * ln2_lo;
-    M = exp(r);
-    k = (int)yk;
-}
-
 struct HArgs {
-    int64_t nw, nm, ni, span;
+    int64_t nw, nm, ni;
     const double *ws;
-    const int32_t *off;
-    const double *rgap;
-    const int32_t *ltab;
-    double h;                 // dω_unit
+    const int32_t *hk;        // [2, nw, nm] interpolation cell of ω ± ω_ν
+    const double *ht;         // [2, nw, nm] its weight
     const double *wnu;        // [nm]
     const double *e_i;        // [ni]
     const double *S;          // complex [nw, ni, nT_all]
     const double *T, *mu, *eta;  // per temperature index
     const int32_t *tidx;      // slot -> temperature index
     double w_acoustic;
+    double *occ;              // [2, nw, nm] per slot: n_ν + f(ω+ω_ν-μ), n_ν + 1 - f(ω-ω_ν-μ)
     double *hout;             // [nw, nm, ni] per slot
     const int32_t *act;       // convergence mask per temperature index, or null
 };
 
-__device__ inline double2 lookup(const double2 *col, int64_t nw, int64_t span, const int32_t *off,
-                                 const double *rgap, const int32_t *ltab, int pos, double frac, double eta)
+// The occupation factors of the two terms for every (ω, ν) of a slot (src/utils.jl:14-38):
+// n = 1/expm1(ω_ν/T), f(x) = 1/(e^{x/T} + 1) and 1 - f(x) = 1/(e^{-x/T} + 1); step functions for
+// T ≤ sqrt(eps).
+__global__ __launch_bounds__(256) void h_occ_kernel(HArgs a)
 {
-    double2 v;
-    if (pos < 0) v = col[0];
-    else if (pos >= span) v = col[nw - 1];
-    else {
-        const int k = ltab[pos];
-        const double t = ((double)(pos - off[k]) + frac) * rgap[k];
-        const double2 lo = col[k], hi = col[k + 1];
-        const double lim = lo.y < -eta ? lo.y : -eta, him = hi.y < -eta ? hi.y : -eta;
-        return make_double2((1 - t) * lo.x + t * hi.x, (1 - t) * lim + t * him);
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (q >= a.nw * a.nm) return;
+    const int tcol = a.tidx[z];
+    if (a.act && !a.act[tcol]) return;
+    const int64_t w = q % a.nw, nu = q / a.nw;
+    const double wnu = a.wnu[nu], x0 = a.ws[w];
+    const double T = a.T[tcol], mu = a.mu[tcol];
+    double fa = 0, fe = 0;
+    if (!(wnu < a.w_acoustic)) {
+        const double ya = x0 + wnu - mu, ye = x0 - wnu - mu;
+        if (T > 1.4901161193847656e-08) {
+            const double nq = 1.0 / expm1(wnu / T);
+            fa = nq + 1.0 / (1.0 + exp(ya / T));
+            fe = nq + 1.0 / (1.0 + exp(-ye / T));
+        } else {
+            const double nq = wnu > 0 ? 0.0 : (wnu == 0 ? -0.5 : -1.0);
+            fa = nq + (ya > 0 ? 0.0 : (ya < 0 ? 1.0 : 0.5));
+            fe = nq + (ye > 0 ? 1.0 : (ye < 0 ? 0.0 : 0.5));
+        }
     }
-    return make_double2(v.x, v.y < -eta ? v.y : -eta);
+    double *o = a.occ + 2 * (q + a.nw * a.nm * (int64_t)z);
+    o[0] = fa;
+    o[1] = fe;
 }
 
-// h[ω, ν, i', slot]: one thread per (ω, ν, i'); Σin is the clamped Σ, read at ω ± ω_ν through
-// the integer offsets of the grid.
+// h[ω, ν, i', slot] = Im G_{i'}(ω+ω_ν) occ_a + Im G_{i'}(ω-ω_ν) occ_e: one thread per (ω, ν, i').
+// Σin is the clamped Σ, read at ω ± ω_ν through the precomputed cells of the integer grid.
 __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
     const int64_t nw = a.nw, total = nw * a.nm * a.ni;
     if (q >= total) return;
-    const int64_t w = q % nw, nu = (q / nw) % a.nm, ip = q / (nw * a.nm);
     const int tcol = a.tidx[z];
     if (a.act && !a.act[tcol]) return;
+    const int64_t wn = q % (nw * a.nm), ip = q / (nw * a.nm);
+    const int64_t w = wn % nw, nu = wn / nw;
     double *out = a.hout + total * (int64_t)z;
     const double wnu = a.wnu[nu];
     if (wnu < a.w_acoustic) { out[q] = 0.0; return; }
-    const double T = a.T[tcol], mu = a.mu[tcol], eta = a.eta ? a.eta[tcol] : -INFINITY;
-    const bool step = !(T > 1.4901161193847656e-08);
+    const double eta = a.eta ? a.eta[tcol] : -INFINITY;
     const double2 *col = reinterpret_cast<const double2 *>(a.S) + nw * a.ni * (int64_t)tcol + nw * ip;
-    const double x0 = a.ws[w];
-    const int o = a.off[w];
-    const double xa = wnu / a.h, xe = -wnu / a.h;
-    const double fla = floor(xa), fle = floor(xe);
-    const double2 Sa = lookup(col, nw, a.span, a.off, a.rgap, a.ltab, o + (int)fla, xa - fla, eta);
-    const double2 Se = lookup(col, nw, a.span, a.off, a.rgap, a.ltab, o + (int)fle, xe - fle, eta);
-    const double ei = a.e_i[ip];
-    const double zar = x0 + wnu - ei - Sa.x, zai = -Sa.y;
-    const double zer = x0 - wnu - ei - Se.x, zei = -Se.y;
-    double nq, Ea, Ee;
-    if (!step) {
-        nq = 1.0 / expm1(wnu / T);
-        double Mw, Mq;
-        int kw, kq;
-        split_exp((x0 - mu) / T, Mw, kw);
-        split_exp(wnu / T, Mq, kq);
-        int ka = kw + kq, ke = kw - kq;
-        ka = ka > 900 ? 900 : (ka < -900 ? -900 : ka);
-        ke = ke > 900 ? 900 : (ke < -900 ? -900 : ke);
-        Ea = ldexp(Mw * Mq, ka);
-        Ee = ldexp(Mw / Mq, ke);
-    } else {
-        nq = wnu > 0 ? 0.0 : (wnu == 0 ? -0.5 : -1.0);
-        const double ya = x0 + wnu - mu, ye = x0 - wnu - mu;
-        Ea = ya > 0 ? 1e270 : (ya < 0 ? 0.0 : 1.0);
-        Ee = ye > 0 ? 1e270 : (ye < 0 ? 0.0 : 1.0);
-    }
-    const double ta = -zai * (nq * (1 + Ea) + 1) / ((zar * zar + zai * zai) * (1 + Ea));
-    const double te = -zei * (nq * (1 + Ee) + Ee) / ((zer * zer + zei * zei) * (1 + Ee));
-    out[q] = ta + te;
+    const int ka = a.hk[2 * wn], ke = a.hk[2 * wn + 1];
+    const double ta = a.ht[2 * wn], te = a.ht[2 * wn + 1];
+    const double2 la = col[ka], ha = col[ka + 1], le = col[ke], he = col[ke + 1];
+    const double sar = (1 - ta) * la.x + ta * ha.x;
+    const double sai = (1 - ta) * fmin(la.y, -eta) + ta * fmin(ha.y, -eta);
+    const double ser = (1 - te) * le.x + te * he.x;
+    const double sei = (1 - te) * fmin(le.y, -eta) + te * fmin(he.y, -eta);
+    const double x0 = a.ws[w], ei = a.e_i[ip];
+    const double zar = x0 + wnu - ei - sar, zai = -sai;
+    const double zer = x0 - wnu - ei - ser, zei = -sei;
+    const double2 o = reinterpret_cast<const double2 *>(a.occ)[wn + nw * a.nm * (int64_t)z];
+    out[q] = -zai / (zar * zar + zai * zai) * o.x - zei / (zer * zer + zei * zei) * o.y;
 }
 
 }  // namespace
@@ -183,19 +173,50 @@ int star_reduce_launch(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nfs, const
     return EPSX_OK;
 }
 
-int h_build_launch(epsx_ctx *ctx, const FmGrid &g, int64_t nm, int64_t ni, const double *wnu, const double *e_i,
-                   const double *S, const double *T, const double *mu, const double *eta, const int32_t *tidx,
-                   int nslots, double w_acoustic, double *hout, const int32_t *act)
+// Cell and weight of ω_w ± ω_ν on the integer grid (off: offsets from the first node, ltab: cell of
+// every integer position, rgap: 1 / cell width): flat extrapolation outside, as a cell with t = 0
+// (below) or t = 1 (above).
+void star_lookup_host(int64_t nw, int64_t nm, const double *wnu, double h, const std::vector<int32_t> &off,
+                      const std::vector<int32_t> &ltab, const std::vector<double> &rgap, int64_t span,
+                      std::vector<int32_t> &hk, std::vector<double> &ht)
+{
+    hk.assign(2 * nw * nm, 0);
+    ht.assign(2 * nw * nm, 0.0);
+    for (int64_t nu = 0; nu < nm; nu++)
+        for (int64_t w = 0; w < nw; w++)
+            for (int sgn = 0; sgn < 2; sgn++) {
+                const double x = (sgn == 0 ? wnu[nu] : -wnu[nu]) / h;
+                const double fl = std::floor(x);
+                const int64_t pos = off[w] + (int64_t)fl;
+                const double frac = x - fl;
+                int32_t k;
+                double t;
+                if (pos < 0) { k = 0; t = 0.0; }
+                else if (pos >= span) { k = (int32_t)(nw - 2); t = 1.0; }
+                else {
+                    k = ltab[pos];
+                    t = ((double)(pos - off[k]) + frac) * rgap[k];
+                }
+                hk[2 * (w + nw * nu) + sgn] = k;
+                ht[2 * (w + nw * nu) + sgn] = t;
+            }
+}
+
+int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const double *ws, const int32_t *hk,
+                   const double *ht, const double *wnu, const double *e_i, const double *S, const double *T,
+                   const double *mu, const double *eta, const int32_t *tidx, int nslots, double w_acoustic,
+                   double *occ, double *hout, const int32_t *act)
 {
     HArgs a;
-    a.act = act;
-    a.nw = g.nw; a.nm = nm; a.ni = ni; a.span = g.span;
-    a.ws = g.ws; a.off = g.off; a.rgap = g.rgap; a.ltab = g.ltab; a.h = g.dw_unit;
+    a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht;
     a.wnu = wnu; a.e_i = e_i; a.S = S; a.T = T; a.mu = mu; a.eta = eta; a.tidx = tidx;
-    a.w_acoustic = w_acoustic; a.hout = hout;
-    const int64_t total = g.nw * nm * ni;
-    dim3 grid((unsigned)((total + 255) / 256), (unsigned)nslots);
-    hipLaunchKernelGGL(h_build_kernel, grid, dim3(256), 0, ctx->stream, a);
+    a.w_acoustic = w_acoustic; a.occ = occ; a.hout = hout; a.act = act;
+    hipLaunchKernelGGL(h_occ_kernel, dim3((unsigned)((nw * nm + 255) / 256), (unsigned)nslots), dim3(256), 0,
+                       ctx->stream, a);
+    EPSX_HIP(ctx, hipGetLastError());
+    const int64_t total = nw * nm * ni;
+    hipLaunchKernelGGL(h_build_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)nslots), dim3(256), 0,
+                       ctx->stream, a);
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
 }

@@ -94,19 +94,30 @@ __global__ void dos_A_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *
     A[q + nwa * ni * (int64_t)z] = -im / PI_D;
 }
 
-// D[a, z] = (Σ_i A[a, i] w_i) Wω[a]  (src/scgd0.jl:306-307), i ascending
-__global__ void dos_D_kernel(int64_t ni, int64_t nwa, const double *__restrict__ A,
-                             const double *__restrict__ w_i, const double *__restrict__ Wa,
-                             double *__restrict__ D, const int32_t *__restrict__ tidx, const int32_t *__restrict__ act)
+// D[a, z] = (Σ_i A[a, i] w_i) Wω[a]  (src/scgd0.jl:306-307): a block per 64 spectral points, its
+// 16 waves over interleaved outer states, then the 16 partial sums in wave order.
+__global__ __launch_bounds__(1024) void dos_D_kernel(int64_t ni, int64_t nwa, const double *__restrict__ A,
+                                                     const double *__restrict__ w_i, const double *__restrict__ Wa,
+                                                     double *__restrict__ D, const int32_t *__restrict__ tidx,
+                                                     const int32_t *__restrict__ act)
 {
-    const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    __shared__ double part[16][64];
     const int z = blockIdx.y;
-    if (a >= nwa) return;
     if (act && !act[tidx[z]]) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t a = blockIdx.x * 64 + lane;
     const double *Az = A + nwa * ni * (int64_t)z;
     double acc = 0;
-    for (int64_t i = 0; i < ni; i++) acc = fma(Az[a + nwa * i], w_i[i], acc);
-    D[a + nwa * (int64_t)z] = acc * Wa[a];
+    if (a < nwa)
+        for (int64_t i = wave; i < ni; i += 16) acc = fma(Az[a + nwa * i], w_i[i], acc);
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && a < nwa) {
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < 16; w++) s += part[w][lane];
+        D[a + nwa * (int64_t)z] = s * Wa[a];
+    }
 }
 
 template <int NT>
@@ -194,59 +205,68 @@ struct MuArgs {
     const int32_t *act;      // convergence mask or null
 };
 
+// Sum over a wave by an xor butterfly: every lane adds the same pairs in the same order, so all
+// lanes hold the bitwise same total.
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 // The μ root of n(μ) = n_target on the bracket extrema(ωs_spectral) ± 0.1 (src/scgd0.jl:310-324,
-// :361-366), one block per temperature: every n(μ) is a fixed-order block reduction, and the
-// bracket is bisected on the ordered bit pattern until two adjacent floats hold the sign change.
+// :361-366), one block of 16 waves per temperature.  Each n(μ) is a fixed-order sum (per thread,
+// per wave, then the 16 wave totals in order), which every thread ends with, so every thread
+// takes the same bisection step without a broadcast.  The bracket is bisected on the ordered bit
+// pattern until two adjacent floats hold the sign change.
 __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
 {
-    __shared__ double red[1024];
-    __shared__ double br[4];  // a, b, fa, fb
-    __shared__ int done;
+    __shared__ double part[2][16];
     const int z = blockIdx.x;
     const int tcol = m.tidx[z];
     if (m.act && !m.act[tcol]) return;
+    const int tid = threadIdx.x, wave = tid >> 6;
     const double T = m.T[tcol];
     const double n_target = m.nlist[tcol] + m.nelec;
     const int holes = m.is_semi && m.nlist[tcol] < 0;
     const double *D = m.D + m.nwa * (int64_t)z;
+    int buf = 0;
     auto excess = [&](double mu) -> double {
         double acc = 0;
-        for (int64_t a = threadIdx.x; a < m.nwa; a += 1024) {
+        for (int64_t a = tid; a < m.nwa; a += 1024) {
             const double x = holes ? mu - m.wsa[a] : m.wsa[a] - mu;
             acc = fma(D[a], occ_fermion_d(x, T), acc);
         }
-        double s = block_sum<1024>(acc, red);
+        acc = wave_sum(acc);
+        if ((tid & 63) == 0) part[buf][wave] = acc;
+        __syncthreads();
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < 16; w++) s += part[buf][w];
+        buf ^= 1;
         if (holes) s = m.W - s;
         return (s + m.nsb) * m.spin - n_target;
     };
-    const double a0 = m.wsa[0] - 0.1, b0 = m.wsa[m.nwa - 1] + 0.1;
-    const double fa0 = excess(a0), fb0 = excess(b0);
-    if (threadIdx.x == 0) {
-        br[0] = a0; br[1] = b0; br[2] = fa0; br[3] = fb0;
-        done = 0;
-        if (fa0 * fb0 > 0) { m.status[tcol] = 2; done = 1; }
-        else {
-            m.status[tcol] = 0;
-            if (fa0 == 0) { m.mu[tcol] = a0; done = 1; }
-            else if (fb0 == 0) { m.mu[tcol] = b0; done = 1; }
-        }
+    double a = m.wsa[0] - 0.1, b = m.wsa[m.nwa - 1] + 0.1;
+    double fa = excess(a), fb = excess(b);
+    if (fa * fb > 0) {
+        if (tid == 0) m.status[tcol] = 2;
+        return;
     }
-    __syncthreads();
-    if (done) return;
+    if (tid == 0) m.status[tcol] = 0;
+    if (fa == 0 || fb == 0) {
+        if (tid == 0) m.mu[tcol] = fa == 0 ? a : b;
+        return;
+    }
     for (int it = 0; it < 200; it++) {
-        const double a = br[0], b = br[1];
         const double mid = bits_middle(a, b);
         if (mid == a || mid == b) break;
         const double fm = excess(mid);
-        if (threadIdx.x == 0) {
-            if (fm == 0) { br[0] = br[1] = mid; br[2] = br[3] = 0; }
-            else if ((fm > 0) == (br[2] > 0)) { br[0] = mid; br[2] = fm; }
-            else { br[1] = mid; br[3] = fm; }
-        }
-        __syncthreads();
-        if (br[0] == br[1]) break;
+        if (fm == 0) { a = b = mid; fa = fb = 0; break; }
+        if ((fm > 0) == (fa > 0)) { a = mid; fa = fm; }
+        else { b = mid; fb = fm; }
     }
-    if (threadIdx.x == 0) m.mu[tcol] = fabs(br[2]) <= fabs(br[3]) ? br[0] : br[1];
+    if (tid == 0) m.mu[tcol] = fabs(fa) <= fabs(fb) ? a : b;
 }
 
 // err = max |Σ_new - Σ| over the slice, then Σ ← Σ_new (src/scgd0.jl:326-328): per-block maxima,
@@ -290,18 +310,24 @@ struct GraphBook {
     double tol;
 };
 
-__global__ void err_final_kernel(int nb, const double *__restrict__ bmax, const int32_t *__restrict__ tidx,
-                                 double *__restrict__ err, GraphBook g)
+__global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__restrict__ bmax,
+                                                      const int32_t *__restrict__ tidx, double *__restrict__ err,
+                                                      GraphBook g)
 {
     const int z = blockIdx.x;
-    if (threadIdx.x != 0) return;
     const int t = tidx[z];
     if (g.act && !g.act[t]) return;
     double m = 0;
-    for (int b = 0; b < nb; b++) {
+    for (int b = threadIdx.x; b < nb; b += 64) {
         const double o = bmax[b + nb * z];
         m = (o > m || o != o) ? o : m;
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v = __shfl_xor(m, o);
+        m = (v > m || v != v) ? v : m;
+    }
+    if (threadIdx.x != 0) return;
     err[t] = m;
     if (g.act) {
         const int it = *g.it;
@@ -438,6 +464,13 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     }
     EPSX_TRY(upload(ctx, s.wph, P->wph, sizeof(double) * nm * P->nq));
     EPSX_TRY(upload(ctx, s.wnu, s.wnu_host.data(), sizeof(double) * nm));
+    if (s.einstein) {
+        std::vector<int32_t> hk;
+        std::vector<double> ht;
+        star_lookup_host(nw, nm, s.wnu_host.data(), P->dw_unit, off, ltab, rgap, s.span, hk, ht);
+        EPSX_TRY(upload(ctx, s.hk, hk.data(), sizeof(int32_t) * hk.size()));
+        EPSX_TRY(upload(ctx, s.ht, ht.data(), sizeof(double) * ht.size()));
+    }
     {
         std::vector<int32_t> ch;
         star_chunks(nm, sptr, ch);
@@ -534,9 +567,10 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     }
     EPSX_TRY(tmp_alloc(ctx, B.part.b, 8 * npart));
     for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
-    Tmp hbuf;
+    Tmp hbuf, hocc;
     if (method == EPSX_FM_STAR) {
         EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
+        EPSX_TRY(tmp_alloc(ctx, hocc.b, 16 * (size_t)nw * s.nm * nT));
         {
             // G̃[(ν, i'), i] = Σ_{f in star(i')} w_f g2[ν, i, f], acoustic modes zeroed
             std::vector<uint8_t> skip(s.nm);
@@ -618,9 +652,10 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         tp.act = d_act;
         // 1-3. clamp + Fan-Migdal (the clamp is applied where Σin is read)
         if (method == EPSX_FM_STAR) {
-            EPSX_TRY(h_build_launch(ctx, g, s.nm, ni, dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(),
-                                    B.T.ptr<double>(), B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA,
-                                    par->w_acoustic, hbuf.ptr<double>(), d_act));
+            EPSX_TRY(h_build_launch(ctx, nw, s.nm, ni, dptr<double>(s.ws), dptr<int32_t>(s.hk), dptr<double>(s.ht),
+                                    dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(), B.T.ptr<double>(),
+                                    B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA, par->w_acoustic,
+                                    hocc.ptr<double>(), hbuf.ptr<double>(), d_act));
             if (events) EPSX_HIP(ctx, hipEventRecord(ev[5], st));
             // ImΣ[:, :, t] = h_z G̃ for every slot z in one batched launch (G̃ shared)
             GemmBatch bt;
@@ -672,7 +707,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                            dptr<double>(s.wsa), dptr<double>(s.e_i), dptr<int32_t>(s.jsp), dptr<double>(s.tsp),
                            d_tidx, B.Snew.ptr<double2>(), B.A.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(dos_D_kernel, dim3(nblk(nwa), nA), dim3(256), 0, st, ni, nwa, B.A.ptr<double>(),
+        hipLaunchKernelGGL(dos_D_kernel, dim3(nblk(nwa, 64), nA), dim3(1024), 0, st, ni, nwa, B.A.ptr<double>(),
                            dptr<double>(s.w_i), dptr<double>(s.Wa), B.D.ptr<double>(), d_tidx, d_act);
         EPSX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(dos_int_kernel, dim3((unsigned)ni, nA), dim3(256), 0, st, ni, nwa, B.A.ptr<double>(),
