@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "epsx_common.h"
+#include "epsx_device.h"
 
 namespace epsx {
 namespace {
@@ -214,11 +215,15 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
+constexpr int MU_REG = 8;  // spectral points per thread held in registers (nωa ≤ 8192)
+
 // The μ root of n(μ) = n_target on the bracket extrema(ωs_spectral) ± 0.1 (src/scgd0.jl:310-324,
 // :361-366), one block of 16 waves per temperature.  Each n(μ) is a fixed-order sum (per thread,
 // per wave, then the 16 wave totals in order), which every thread ends with, so every thread
 // takes the same bisection step without a broadcast.  The bracket is bisected on the ordered bit
-// pattern until two adjacent floats hold the sign change.
+// pattern until two adjacent floats hold the sign change.  For T > sqrt(eps) each thread keeps
+// its D_a and e^{±ωa/T} (as mantissa and binary exponent) in registers, so a step costs one
+// exponential per thread: e^{(ωa-μ)/T} = e^{ωa/T} e^{-μ/T}.
 __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
 {
     __shared__ double part[2][16];
@@ -230,12 +235,38 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
     const double n_target = m.nlist[tcol] + m.nelec;
     const int holes = m.is_semi && m.nlist[tcol] < 0;
     const double *D = m.D + m.nwa * (int64_t)z;
+    const bool regs = T > SQRT_EPS && m.nwa <= 1024 * MU_REG;
+    double Dv[MU_REG], Mv[MU_REG];
+    int kv[MU_REG];
+    if (regs) {
+#pragma unroll
+        for (int r = 0; r < MU_REG; r++) {
+            const int64_t a = tid + 1024 * r;
+            Dv[r] = 0.0; Mv[r] = 1.0; kv[r] = 0;
+            if (a < m.nwa) {
+                Dv[r] = D[a];
+                split_exp((holes ? -m.wsa[a] : m.wsa[a]) / T, Mv[r], kv[r]);
+            }
+        }
+    }
     int buf = 0;
     auto excess = [&](double mu) -> double {
         double acc = 0;
-        for (int64_t a = tid; a < m.nwa; a += 1024) {
-            const double x = holes ? mu - m.wsa[a] : m.wsa[a] - mu;
-            acc = fma(D[a], occ_fermion_d(x, T), acc);
+        if (regs) {
+            double Mm;
+            int km;
+            split_exp((holes ? mu : -mu) / T, Mm, km);
+#pragma unroll
+            for (int r = 0; r < MU_REG; r++) {
+                int k = kv[r] + km;
+                k = k > 900 ? 900 : (k < -900 ? -900 : k);
+                acc = fma(Dv[r], 1.0 / (ldexp(Mv[r] * Mm, k) + 1.0), acc);
+            }
+        } else {
+            for (int64_t a = tid; a < m.nwa; a += 1024) {
+                const double x = holes ? mu - m.wsa[a] : m.wsa[a] - mu;
+                acc = fma(D[a], occ_fermion_d(x, T), acc);
+            }
         }
         acc = wave_sum(acc);
         if ((tid & 63) == 0) part[buf][wave] = acc;
