@@ -194,6 +194,16 @@ __device__ inline double bits_middle(double a, double b)
     return a < 0 ? -r : r;
 }
 
+// Number of floats between a < b (as a double): the distance of their ordered bit patterns.
+__device__ inline double ordered_gap(double a, double b)
+{
+    auto key = [](double x) -> double {
+        const double m = (double)(unsigned long long)__double_as_longlong(fabs(x));
+        return x < 0 ? -m : m;
+    };
+    return key(b) - key(a);
+}
+
 struct MuArgs {
     int64_t nwa;
     const double *wsa, *D;   // D [nwa, nz]
@@ -289,13 +299,31 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
         if (tid == 0) m.mu[tcol] = fa == 0 ? a : b;
         return;
     }
-    for (int it = 0; it < 200; it++) {
-        const double mid = bits_middle(a, b);
-        if (mid == a || mid == b) break;
-        const double fm = excess(mid);
-        if (fm == 0) { a = b = mid; fa = fb = 0; break; }
-        if ((fm > 0) == (fa > 0)) { a = mid; fa = fm; }
-        else { b = mid; fb = fm; }
+    // Illinois steps (regula falsi with the retained end's value halved) shrink the bracket while
+    // it is wider than 2^12 floats, then ordered-bit bisection closes it to two adjacent floats.
+    // n(μ) is a monotone fixed-order sum, so the adjacent pair holding the sign change, and thus
+    // the root returned, is the one plain bisection reaches.
+    double ga = fa, gb = fb;  // Illinois-scaled end values
+    int side = 0;
+    for (int it = 0; it < 300; it++) {
+        const double bis = bits_middle(a, b);
+        if (bis == a || bis == b) break;
+        double x = bis;
+        if (it < 40 && ordered_gap(a, b) > 4096.0) {
+            const double r = (a * gb - b * ga) / (gb - ga);
+            if (r > a && r < b) x = r;
+        }
+        const double fm = excess(x);
+        if (fm == 0) { a = b = x; fa = fb = 0; break; }
+        if ((fm > 0) == (fa > 0)) {
+            a = x; fa = ga = fm;
+            if (side == -1) gb *= 0.5;
+            side = -1;
+        } else {
+            b = x; fb = gb = fm;
+            if (side == 1) ga *= 0.5;
+            side = 1;
+        }
     }
     if (tid == 0) m.mu[tcol] = fabs(fa) <= fabs(fb) ? a : b;
 }
