@@ -9,7 +9,7 @@
 //     into registers before the current tile's MFMAs, so HBM/L2 latency hides behind compute;
 //   * each wave owns a 32x32 sub-tile as 2x2 accumulators of 16x16;
 //   * split-k partials go to a workspace and are summed in ascending split order by a second
-//     kernel, so results are bitwise reproducible and independent of the batch size;
+//     kernel, so results are bitwise reproducible;
 //   * workgroups are enumerated XCD-major (dispatch sends workgroup b to XCD b mod 8), so the
 //     tiles one XCD runs are neighbours that share B panels in that XCD's L2.
 #include "epsx_common.h"
@@ -167,11 +167,19 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     p.mt = (int)((m + GM - 1) / GM);
     p.nt = (int)((n + GN - 1) / GN);
     p.ktiles = (int)((k + GK - 1) / GK);
-    // Split k until one instance alone gives ~4 workgroups per CU, keeping >= 8 k-tiles per split.
-    // The split depends on the shape only, so batched and one-by-one products agree bitwise.
+    // Split k to fill the machine: 256 CUs hold 4 workgroups each (the LDS and register use above),
+    // so a launch runs in rounds of 1024 units; pick the split with the least rounds x k-tiles per
+    // unit (+2 for the prologue and epilogue), keeping >= 4 k-tiles per split.  The choice depends
+    // on the shape and the batch size only, so results are bitwise reproducible.
     const int64_t tiles = (int64_t)p.mt * p.nt;
+    constexpr int64_t SLOTS = 256 * 4;
     int S = 1;
-    while (tiles * S < 1024 && p.ktiles / (2 * S) >= 8) S *= 2;
+    double best = 0;
+    for (int c = 1; c <= 16 && (c == 1 || p.ktiles / c >= 4); c++) {
+        const int64_t rounds = (tiles * c * bt.n_inst + SLOTS - 1) / SLOTS;
+        const double cost = (double)rounds * ((double)p.ktiles / c + 2.0);
+        if (c == 1 || cost < best) { best = cost; S = c; }
+    }
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     EPSX_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
     if (S > 1) {
