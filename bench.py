@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", default="scgd0", choices=("scgd0", "ladder"),
+                    help="scgd0: the headline solve; ladder: the Kubo ladder on the 16³ model")
+    ap.add_argument("--omegas", type=int, default=4, help="ladder: probe frequencies 0, 5, ... meV timed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (the HBM counter calibration)")
     args = ap.parse_args()
@@ -60,6 +63,8 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the scGD0 path has no multi-GPU "
               "implementation in this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ladder":
+        return ladder_bench(args)
 
     import epsx
     from epsx.models import t2g_setup
@@ -152,6 +157,65 @@ def main():
         line["speedup_vs_plain"] = line["plain_fp64_gpu"]["value"] / s_per_T
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(setup, meta, iters[0], args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+def ladder_bench(args):
+    """The Kubo ladder on the converged scGD0 result of the t2g model at 16³ (ni = 331,
+    nf = 8448, nshift = 82): scGD0 at 100, 200, 300, 400 K to 1e-6 eV by the star method, then
+    GMRES at rtol 1e-2 for the first `--omegas` probe frequencies of 0:5:200 meV at 100 K.  The
+    value is seconds per (Ω, T) solve; the roofline line is the shift-table GEMM W = J·B̃
+    (FP64 MFMA), 2·(16 nω)·(3 ni)·(6 ni·nshift) flop per application."""
+    import warnings
+    import epsx
+    from epsx.ladder import LadderDevice, setup_ladder
+    from epsx.models import t2g_ladder_setup
+
+    nk = 16 if args.nk == 24 else args.nk
+    temps = (100.0, 200.0, 300.0, 400.0)
+    t0 = time.time()
+    setup, R, v, meta = t2g_ladder_setup(nk=nk, Tlist_K=temps)
+    nm, ni, nf = setup.dims
+    nw = len(setup.ωs)
+    print(f"[bench] ladder setup nk={nk} ni={ni} nf={nf} nω={nw} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    ctx = epsx.Context(0)
+    dev = epsx.Scgd0Device(setup, ctx)
+    occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0, spin_degeneracy=2.0)
+    warnings.simplefilter("ignore")
+    t0 = time.perf_counter()
+    r = dev.loop(occ, η_min=0.0, η_init=5e-3, maxiter=200, tol=1e-6, ω_acoustic=1e-6, method="auto", batched=True)
+    t_scgd0 = time.perf_counter() - t0
+    print(f"[bench] scGD0 4 T: {t_scgd0:.2f}s, converged {r['converged'].tolist()}", file=sys.stderr, flush=True)
+    L = setup_ladder(setup, R, v)
+    ld = LadderDevice(dev, L)
+    Ωs = 0.005 * np.arange(args.omegas)
+    # prime: B̃ of the first temperature and one solve (not timed)
+    ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs[:1], rtol=1e-2, maxiter=30)
+    t0 = time.perf_counter()
+    out = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs, rtol=1e-2, maxiter=30)
+    el = time.perf_counter() - t0
+    tm = ctx.ladder_timings()
+    ns = L.nshift
+    flops = 2.0 * (16 * nw) * (3 * ni) * (6 * ni * ns)
+    gemm_per_app = tm["shift_gemm"] / max(tm["applications"], 1)
+    achieved = flops / gemm_per_app / 1e12
+    line = {
+        "metric": "ladder s per (Ω, T) solve at rtol 1e-2; shift-table GEMM TFLOP/s",
+        "value": el / len(Ωs), "unit": "s/(Ω,T)", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * el, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"t2g {nk}^3 ladder, GMRES rtol 1e-2, Ω = 0..{5 * (len(Ωs) - 1)} meV at 100 K",
+                   "nk": nk, "ni": ni, "nf": nf, "nw": nw, "nshift": ns, "niter": out["niter"][:, 0].tolist(),
+                   "scgd0_s_4T": t_scgd0, "applications": tm["applications"],
+                   "extrapolated_s_41x4": el / len(Ωs) * 41 * 4},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "dgemm_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)",
+                     "algorithmic_flop_per_application": flops, "gemm_s_per_application": gemm_per_app,
+                     "apply_s": tm["apply"] / max(tm["applications"], 1)},
+        "sigma_ladder_xx": out["σ_ladder"][0, 0, :, 0].tolist(),
+        "sigma_bubble_xx": out["σ_bubble"][0, 0, :, 0].tolist(),
+    }
     print(json.dumps(line), flush=True)
 
 

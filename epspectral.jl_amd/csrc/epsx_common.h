@@ -131,6 +131,33 @@ void grid_nodes(int64_t n, double origin, double unit, const int64_t *offsets, d
 // Kernel launchers (epsx_kk.hip)
 int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K);
 
+// Kernel time of a stretch of the stream added to ctx->t_class[k] (HIP events; stop() waits for
+// the end event).  Not for use under graph capture.
+struct ClassTimer {
+    epsx_ctx *ctx;
+    int k;
+    hipEvent_t a = nullptr, b = nullptr;
+    ClassTimer(epsx_ctx *c, int cls) : ctx(c), k(cls)
+    {
+        if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) (void)hipEventRecord(a, c->stream);
+    }
+    int stop(int64_t launches)
+    {
+        float ms = 0;
+        EPSX_HIP(ctx, hipEventRecord(b, ctx->stream));
+        EPSX_HIP(ctx, hipEventSynchronize(b));
+        EPSX_HIP(ctx, hipEventElapsedTime(&ms, a, b));
+        ctx->t_class[k] += ms * 1e-3;
+        ctx->n_class[k] += launches;
+        return EPSX_OK;
+    }
+    ~ClassTimer()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 // A batch of GEMM instances (epsx_gemm.hip).  Instance z uses A + sA a, B + sB b, C + sC c,
 // where a, b, c are z, or idx[z] for the operands flagged in imap (bit 0 A, bit 1 B, bit 2 C).
 // gate (device, indexed by idx[z] or z): 0 skips the instance.  Z/Zim: write

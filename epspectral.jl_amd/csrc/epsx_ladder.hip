@@ -419,6 +419,7 @@ static int ensure_Bt(epsx_ctx *ctx, double T, double w_ac)
     if (l.bt_T == T && l.bt_wac == w_ac) return EPSX_OK;
     const int64_t ni = s.ni;
     EPSX_TRY(dev_alloc(ctx, l.Bt, 8 * (size_t)(3 * ni) * (6 * ni) * l.ns));
+    ClassTimer tm(ctx, 2);
     BtArgs a;
     a.nm = s.nm; a.ni = ni; a.nfs = s.nf; a.ns = l.ns;
     a.g2p = dptr<double>(s.g2p); a.wf = dptr<double>(s.wf_s); a.Rs = dptr<double>(l.Rs);
@@ -430,7 +431,7 @@ static int ensure_Bt(epsx_ctx *ctx, double T, double w_ac)
     EPSX_HIP(ctx, hipGetLastError());
     l.bt_T = T;
     l.bt_wac = w_ac;
-    return EPSX_OK;
+    return tm.stop(1);
 }
 
 // LJ = L(J) on device vectors at the temperature whose B̃ is loaded and the propagators in l.Gm/l.Gp
@@ -440,6 +441,7 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ)
     LadderDev &l = ctx->l;
     const int64_t nw = s.nw, ni = s.ni, nrow = 16 * nw, ncol = 6 * ni, K = 3 * ni;
     const hipStream_t st = ctx->stream;
+    ClassTimer t_apply(ctx, 1);
     hipLaunchKernelGGL(jperm_kernel, dim3(nb(nrow * K)), dim3(256), 0, st, nw, ni, J, dptr<double>(l.Jm));
     EPSX_HIP(ctx, hipGetLastError());
     // W for chunks of shifts, each followed by its shift-add
@@ -449,8 +451,10 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ)
     EPSX_TRY(dev_alloc(ctx, l.W, per_m * mb));
     for (int64_t m0 = 0; m0 < l.ns; m0 += mb) {
         const int64_t mc = m0 + mb <= l.ns ? mb : l.ns - m0;
+        ClassTimer t_gemm(ctx, 0);
         EPSX_TRY(dgemm_nn(ctx, nrow, ncol * mc, K, dptr<double>(l.Jm), nrow, dptr<double>(l.Bt) + K * ncol * m0, K,
                           dptr<double>(l.W), nrow, false));
+        EPSX_TRY(t_gemm.stop(1));
         hipLaunchKernelGGL(shift_add_chunk_kernel, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns, m0, mc,
                            dptr<double>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
                            dptr<double>(l.Cp), dptr<double>(l.Cm), m0 > 0 ? 1 : 0);
@@ -465,7 +469,7 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ)
     hipLaunchKernelGGL(exch_attach_kernel, dim3(nb(nw * 6 * ni)), dim3(256), 0, st, nw, ni, dptr<double>(l.Cp),
                        dptr<double>(l.Cm), dptr<double>(l.KY), dptr<double2>(l.Gm), dptr<double2>(l.Gp), LJ);
     EPSX_HIP(ctx, hipGetLastError());
-    return EPSX_OK;
+    return t_apply.stop(1);
 }
 
 static int ladder_work(epsx_ctx *ctx)
@@ -585,6 +589,7 @@ extern "C" int epsx_apply_ladder_kernel(epsx_ctx *ctx, const double *J, const do
     if (!(w_acoustic > 0)) return set_error(ctx, EPSX_ERR_ARG, "ω_acoustic must be positive, got %g", w_acoustic);
     if (precision != EPSX_PREC_FP64) return set_error(ctx, EPSX_ERR_ARG, "ladder precision %d is not available", precision);
     EPSX_HIP(ctx, hipSetDevice(ctx->device));
+    for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
     const int64_t n = s.nw * 24 * s.ni, ng = s.nw * 4 * s.ni;
     Tmp dJ, dLJ;
     EPSX_TRY(tmp_upload(ctx, dJ, J, 16 * n));

@@ -192,6 +192,14 @@ class Context:
             raise ArgumentError(msg)
         raise DeviceError(f"libepsx error {rc}: {msg}")
 
+    def ladder_timings(self):
+        """Kernel seconds and launch counts of the last ladder run (classes of epsx_last_timings)."""
+        s = np.zeros(8)
+        n = np.zeros(8, dtype=np.int64)
+        lib().epsx_last_timings(self.handle, ptr(s), ptr(n))
+        return {"shift_gemm": s[0], "shift_gemm_launches": int(n[0]), "apply": s[1], "applications": int(n[1]),
+                "bt_build": s[2], "bt_builds": int(n[2])}
+
     def timings(self):
         s = np.zeros(8)
         n = np.zeros(8, dtype=np.int64)

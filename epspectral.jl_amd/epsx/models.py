@@ -208,3 +208,66 @@ def holstein_ladder_setup(nk=16, ω0=0.173, t=1.0, λ=0.5, symmetry=False, ωs=N
     v = np.zeros((3, len(js)), order="F")
     v[0] = 2.0 * t * np.sin(2.0 * np.pi * js / nk)
     return s, R, v
+
+
+def oh_operations():
+    """The 48 signed permutation matrices of Oh acting on Cartesian k (and on vectors)."""
+    import itertools
+    ops = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            S = np.zeros((3, 3))
+            for a in range(3):
+                S[a, perm[a]] = signs[a]
+            ops.append(S)
+    return np.array(ops)
+
+
+def t2g_velocities(kvec, t, tp):
+    """Band energies and velocities ∇ε (eV times the lattice constant) of the sorted t2g bands at
+    the reduced k points kvec [n, 3] (radians).  Returns (ε [n, 3], v [n, 3 bands, 3 cart])."""
+    s = np.sin(kvec)
+    c = np.cos(kvec)
+    cx, cy, cz = c[:, 0], c[:, 1], c[:, 2]
+    e = np.stack([-2 * t * (cx + cy) - 2 * tp * cz,     # xy
+                  -2 * t * (cy + cz) - 2 * tp * cx,     # yz
+                  -2 * t * (cz + cx) - 2 * tp * cy],    # zx
+                 axis=1)
+    coef = np.array([[t, t, tp], [tp, t, t], [t, tp, t]])  # orbital x cart hopping of ∂/∂k
+    v = 2.0 * coef[None, :, :] * s[:, None, :]
+    order = np.argsort(e, axis=1, kind="stable")
+    idx = np.arange(len(kvec))[:, None]
+    return e[idx, order], v[idx, order]
+
+
+def t2g_ladder_setup(nk=16, t=0.26, tp=0.03, **kw):
+    """The t2g model of `t2g_setup` with what the ladder adds (src/ladder.jl:112-158):
+
+    * v[:, i]: the band velocity of outer state i at its IBZ representative;
+    * R[:, :, f]: the averaged vector unfolding of the representative to inner state f, the mean
+      of the Oh operations S with S k_rep = k_f (mod the grid), so that v_f = R_f v_{i_f} where
+      the band is non-degenerate (a vector field odd under k -> -k, as inversion is in Oh).
+
+    Returns (scGD0 setup, R, v, meta)."""
+    setup, meta = t2g_setup(nk=nk, t=t, tp=tp, **kw)
+    canon, sel_i, sel_f = meta["canon"], meta["sel_i"], meta["sel_f"]
+    inv = meta["inv"]
+    two_pi_n = 2.0 * np.pi / nk
+    # outer-state velocities at the canonical k of each representative
+    rep_k = np.array([canon[k] for k, b in sel_i], dtype=np.float64) * two_pi_n
+    _, v_rep = t2g_velocities(rep_k, t, tp)
+    bands_i = np.array([b for k, b in sel_i])
+    v = np.asfortranarray(v_rep[np.arange(len(sel_i)), bands_i].T)      # (3, ni)
+    # inner states: full-grid triples (n1 fastest, as the grid is enumerated)
+    kf = np.array([k for k, b in sel_f])
+    nvec = np.stack([kf % nk, (kf // nk) % nk, kf // (nk * nk)], axis=1)
+    rep = canon[inv[kf]]                                                   # (nf, 3) canonical triple
+    ops = oh_operations()                                                  # (48, 3, 3)
+    mapped = np.einsum("sab,fb->sfa", ops, rep.astype(np.float64))
+    hit = np.all(np.mod(np.rint(mapped).astype(np.int64), nk) == nvec[None, :, :], axis=2)   # (48, nf)
+    count = hit.sum(axis=0)
+    if not np.all(count > 0):
+        raise RuntimeError("an inner state has no Oh operation from its representative")
+    R = np.einsum("sf,sab->abf", hit.astype(np.float64), ops) / count[None, None, :]
+    meta = dict(meta, v_rep=v_rep, bands_i=bands_i, kf=kf, nvec=nvec)
+    return setup, np.asfortranarray(R), v, meta

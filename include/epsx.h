@@ -45,10 +45,12 @@ int epsx_open(int device, epsx_ctx **ctx);
 void epsx_close(epsx_ctx *ctx);
 const char *epsx_last_error(const epsx_ctx *ctx);
 /* Seconds of the last run's kernels by class, measured with HIP events on the context's
- * stream: [0] Fan-Migdal sum, [1] Kramers-Kronig, [2] spectral DOS and μ, [3] convergence and
- * copies, [4] the coupling reduction of the star method (once per run), [5] the star
- * method's h·G̃ GEMMs (inside [0]), [6..7] reserved; and the
- * number of launches of each class.  Both arrays hold 8 entries. */
+ * stream, and the number of launches of each class.  Both arrays hold 8 entries.
+ * After epsx_scgd0_run: [0] Fan-Migdal sum, [1] Kramers-Kronig, [2] spectral DOS and μ,
+ * [3] convergence and copies, [4] the coupling reduction of the star method (once per run),
+ * [5] the star method's h·G̃ GEMMs (inside [0]), [6..7] unused.
+ * After epsx_ladder_run / epsx_apply_ladder_kernel: [0] the shift-table GEMMs W = J·B̃,
+ * [1] whole kernel applications (count = applications), [2] B̃ builds, [3..7] unused. */
 int epsx_last_timings(const epsx_ctx *ctx, double *seconds8, int64_t *launches8);
 
 /* ---- grid maps (host, bit-exact with the reference) ---------------------------------- */
