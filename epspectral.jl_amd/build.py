@@ -21,7 +21,7 @@ OBJ = os.path.join(HERE, "_build")
 SOURCES = [
     "epsx_context.cpp",
     "epsx_grids.cpp",
-    "epsx_kk.hip", "epsx_gemm.hip",
+    "epsx_kk.hip", "epsx_gemm.hip", "epsx_net.hip",
     "epsx_fm.hip",
     "epsx_fm_star.hip",
     "epsx_scgd0.hip",

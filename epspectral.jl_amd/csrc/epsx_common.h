@@ -67,6 +67,18 @@ struct LadderDev {
     DevBuf Gm, Gp, Jm, W, Cp, Cm, Y, KY, S, jm, tm, jp, tp, Yc, red, Bt32, Jm32;
 };
 
+constexpr int NET_MAX_LAYERS = 16;
+
+// The warm-start network: configuration, FP32 master weights, Adam moments and activations.
+struct NetDev {
+    bool ready = false;
+    int layers = 0, kernel = 0, c_in = 0, c_hidden = 0, c_out = 0;
+    int64_t length = 0, nparam = 0, cap_nb = 0, step = 0;
+    DevBuf params, grad, m, v, Wt, loss, bsum, part;
+    DevBuf X[NET_MAX_LAYERS + 1], Z[NET_MAX_LAYERS];
+    DevBuf Y, dA, dB, T, idx;
+};
+
 }  // namespace epsx
 
 struct epsx_ctx {
@@ -75,6 +87,7 @@ struct epsx_ctx {
     std::string err;
     epsx::Scgd0Dev s;
     epsx::LadderDev l;
+    epsx::NetDev net;
     std::vector<epsx::DevBuf *> owned;
     epsx::DevBuf gemm_ws;   // split-k partial products
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,

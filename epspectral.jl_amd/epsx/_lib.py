@@ -51,6 +51,13 @@ EXPORTS = [
     ("epsx_ladder_set_problem", C.c_int, [P, P]),
     ("epsx_ladder_run", C.c_int, [P, P, P]),
     ("epsx_apply_ladder_kernel", C.c_int, [P, P, P, P, D, D, I32, P]),
+    ("epsx_net_create", C.c_int, [P, P, C.c_uint64]),
+    ("epsx_net_param_count", C.c_int, [P, P]),
+    ("epsx_net_get_params", C.c_int, [P, P]),
+    ("epsx_net_set_params", C.c_int, [P, P]),
+    ("epsx_net_forward", C.c_int, [P, P, I64, I32, P]),
+    ("epsx_net_gradient", C.c_int, [P, P, P, I64, I32, P, P]),
+    ("epsx_net_train", C.c_int, [P, P, P, I64, P, P]),
 ]
 
 
@@ -69,6 +76,16 @@ def lib():
             raise ImportError("libepsx ABI version mismatch")
         _lib = L
     return _lib
+
+
+class NetConfig(C.Structure):
+    _fields_ = [("struct_size", I64), ("n_layers", I32), ("kernel", I32), ("c_in", I32), ("c_hidden", I32),
+                ("c_out", I32), ("pad", I32), ("length", I64)]
+
+
+class NetTrainParams(C.Structure):
+    _fields_ = [("struct_size", I64), ("epochs", I64), ("batch", I64), ("lr", D), ("beta1", D), ("beta2", D),
+                ("eps", D), ("weight_decay", D), ("precision", I32), ("verbosity", I32), ("seed", C.c_uint64)]
 
 
 class Scgd0Problem(C.Structure):

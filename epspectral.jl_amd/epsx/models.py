@@ -104,7 +104,7 @@ def oh_ibz(nk):
 def t2g_setup(nk=24, t=0.26, tp=0.03, nmodes=9, wmin=0.010, wmax=0.080, g2_median=2e-3,
               g2_sigma_ln=1.0, window=0.8, nelec=1.0, Tlist_K=(100.0, 300.0), nlist=0.0, seed=1234,
               grid_windows=((0.3, 1e-3), (1.2, 4e-3), (3.0, 16e-3)), spectral_step=None,
-              degen_cutoff=ELECTRON_DEGEN_CUTOFF, make_g2=True):
+              degen_cutoff=ELECTRON_DEGEN_CUTOFF, make_g2=True, uniform_grid=None):
     """The synthetic cubic t2g model of the benchmark, in eV.
 
     * bands ε_xy = -2t(cos kx + cos ky) - 2t' cos kz and cyclic, sorted per k;
@@ -112,7 +112,8 @@ def t2g_setup(nk=24, t=0.26, tp=0.03, nmodes=9, wmin=0.010, wmax=0.080, g2_media
       temperature; inner states: the full grid in the same window;
     * `nmodes` Einstein modes ω_ν ~ U(wmin, wmax); g2[ν,i,f] = |g|²/(2ω_ν) with
       |g|² ~ LogNormal(ln g2_median, g2_sigma_ln), i.i.d., seeded;
-    * ωs graded around the bare μ: 1 meV to ±0.3 eV, 4 meV to ±1.2 eV, 16 meV to ±3 eV;
+    * ωs graded around the bare μ: 1 meV to ±0.3 eV, 4 meV to ±1.2 eV, 16 meV to ±3 eV, or, with
+      `uniform_grid = (L, half_width)`, L uniform nodes over bare μ ± half_width;
       ωs_spectral uniform at ωs.dω_min over the span of ωs (the run_scgd0 default).
     """
     rng = np.random.default_rng(seed)
@@ -148,8 +149,12 @@ def t2g_setup(nk=24, t=0.26, tp=0.03, nmodes=9, wmin=0.010, wmax=0.080, g2_media
         g2 = np.empty((nmodes, ni, 0), order="F")
     μ_start = np.array([bare_mu(ε_i, w_i, T, nelec + nl, 2.0, nstates_base)
                         for T, nl in zip(Tlist, np.broadcast_to(nlist, Tlist.shape))])
-    wins = [(mu_all - w, mu_all + w, dw) for (w, dw) in grid_windows]
-    ωs = graded_frequency_grid(wins)
+    if uniform_grid is None:
+        wins = [(mu_all - w, mu_all + w, dw) for (w, dw) in grid_windows]
+        ωs = graded_frequency_grid(wins)
+    else:
+        L, hw = uniform_grid
+        ωs = FrequencyGrid(mu_all - hw, 2.0 * hw / (L - 1), np.arange(L, dtype=np.int64))
     ωs_spectral = (uniform_spectral_grid(ωs) if spectral_step is None else
                    FrequencyGrid(ωs[0], spectral_step, np.arange(int(round((ωs[-1] - ωs[0]) / spectral_step)) + 1)))
     setup = Scgd0Setup(

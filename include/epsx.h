@@ -235,6 +235,48 @@ int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, epsx_ladder_ou
 int epsx_apply_ladder_kernel(epsx_ctx *ctx, const double *J, const double *Gm, const double *Gp,
                              double T, double w_acoustic, int32_t precision, double *LJ);
 
+/* ---- learned Σ_init warm start ------------------------------------------------------ */
+/* No counterpart in the reference: its hook is the Σ_init argument of _loop_scgd0
+ * (src/scgd0.jl:66-67, :204-212), which these entry points fill.  A 1D convolutional network
+ * over the frequency grid of one outer state: n_layers 'same'-padded convolutions of odd width
+ * `kernel`, c_in -> c_hidden -> ... -> c_out channels, SiLU between layers.  Samples are
+ * float [n, channels, length] (C order).  Parameters: per layer W [cout, cin, kernel] then
+ * b [cout], FP32.  precision: EPSX_PREC_BF16 (operands rounded to BF16, FP32 accumulation)
+ * or EPSX_PREC_FP32. */
+typedef struct {
+    int64_t struct_size;      /* sizeof(epsx_net_config) */
+    int32_t n_layers;         /* 2..16 */
+    int32_t kernel;           /* odd */
+    int32_t c_in, c_hidden, c_out;
+    int32_t pad;
+    int64_t length;           /* grid points per sample */
+} epsx_net_config;
+
+typedef struct {
+    int64_t struct_size;      /* sizeof(epsx_net_train_params) */
+    int64_t epochs, batch;
+    double lr, beta1, beta2, eps, weight_decay;   /* Adam with decoupled weight decay */
+    int32_t precision;
+    int32_t verbosity;
+    uint64_t seed;            /* shuffling */
+} epsx_net_train_params;
+
+/* A new network with seeded Kaiming-uniform weights (last layer x 0.1) and zero biases. */
+int epsx_net_create(epsx_ctx *ctx, const epsx_net_config *cfg, uint64_t seed);
+int epsx_net_param_count(epsx_ctx *ctx, int64_t *count);
+int epsx_net_get_params(epsx_ctx *ctx, float *params);
+/* Sets the weights and clears the optimiser state. */
+int epsx_net_set_params(epsx_ctx *ctx, const float *params);
+/* y [n, c_out, length] = net(x [n, c_in, length]). */
+int epsx_net_forward(epsx_ctx *ctx, const float *x, int64_t n, int32_t precision, float *y);
+/* Mean squared error against target [n, c_out, length] and its gradient (param_count floats). */
+int epsx_net_gradient(epsx_ctx *ctx, const float *x, const float *target, int64_t n, int32_t precision,
+                      double *loss, float *grad);
+/* Adam over shuffled mini-batches of the (device-resident) training set; epoch_loss[epochs]
+ * receives the mean batch loss of each epoch. */
+int epsx_net_train(epsx_ctx *ctx, const float *x, const float *target, int64_t n,
+                   const epsx_net_train_params *params, double *epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,110 @@
+"""GPU parity of the warm-start network (libepsx through its C ABI) against the float64 NumPy
+oracle (oracle/net_oracle.py), and the warm start's gain on scGD0.
+
+Tolerances (relative, in the 2-norm over all outputs / all gradient entries): FP32 MFMA path
+1e-5 (forward) and 1e-4 (gradient); BF16 operands with FP32 accumulation 3e-2 and 6e-2.
+"""
+import warnings
+
+import numpy as np
+import pytest
+
+import epsx
+import net_oracle
+from epsx.learn import WarmStartNet, make_training_set, random_t2g_model, g0d0_and_converged, run_warm_started
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _data(cfg, n, L, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, cfg[2], L)).astype(np.float32)
+    t = rng.normal(scale=0.5, size=(n, cfg[4], L)).astype(np.float32)
+    return x, t
+
+
+@pytest.mark.parametrize("layers,kernel,hidden,L,n", [(3, 5, 16, 70, 5), (6, 9, 64, 1024, 3)])
+def test_forward_and_gradient_match_oracle(ctx, layers, kernel, hidden, L, n):
+    net = WarmStartNet(ctx, L, n_layers=layers, kernel=kernel, c_hidden=hidden, seed=11)
+    p = net.params
+    cfg = net.cfg
+    rng = np.random.default_rng(2)
+    # a trained-like net: larger last-layer weights than the initialisation
+    p = p + rng.normal(scale=0.02, size=p.shape).astype(np.float32)
+    net.params = p
+    x, t = _data(cfg, n, L, 4)
+    y_ref = net_oracle.forward(p.astype(np.float64), x, cfg)
+    y32 = net.forward(x, "fp32")
+    ybf = net.forward(x, "bf16")
+    assert rel(y32, y_ref) <= 1e-5
+    assert rel(ybf, y_ref) <= 3e-2
+    loss_ref, g_ref = net_oracle.loss_and_grad(p.astype(np.float64), x, t, cfg)
+    l32, g32 = net.gradient(x, t, "fp32")
+    lbf, gbf = net.gradient(x, t, "bf16")
+    assert abs(l32 - loss_ref) <= 1e-5 * loss_ref
+    assert rel(g32, g_ref) <= 1e-4
+    assert abs(lbf - loss_ref) <= 3e-2 * loss_ref
+    assert rel(gbf, g_ref) <= 6e-2
+    # bitwise reproducible
+    assert np.array_equal(net.gradient(x, t, "bf16")[1], gbf)
+
+
+def test_training_reduces_loss_and_is_reproducible(ctx):
+    L = 128
+    net = WarmStartNet(ctx, L, n_layers=4, kernel=7, c_hidden=32, seed=3)
+    p0 = net.params
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(96, 5, L)).astype(np.float32)
+    # a learnable target: a smoothed combination of the first two input channels
+    k = np.exp(-0.5 * (np.arange(-3, 4) / 1.5) ** 2)
+    k /= k.sum()
+    sm = np.apply_along_axis(lambda r: np.convolve(r, k, mode="same"), 2, x[:, :2, :])
+    t = (0.5 * sm[:, ::-1, :] - 0.2 * x[:, 2:4, :]).astype(np.float32)
+    l0, _ = net.gradient(x, t, "fp32")
+    losses = net.train(x, t, epochs=30, batch=16, lr=2e-3, precision="bf16", seed=1)
+    l1, _ = net.gradient(x, t, "fp32")
+    assert losses[-1] < 0.2 * losses[0] and l1 < 0.2 * l0
+    p1 = net.params
+    net.params = p0
+    net.train(x, t, epochs=30, batch=16, lr=2e-3, precision="bf16", seed=1)
+    assert np.array_equal(net.params, p1)
+
+
+def test_warm_start_cuts_iterations(ctx):
+    """Trained on 12 random models (nk = 8, 512 uniform nodes), the warm start converges held-out
+    models in fewer iterations than the plain start, to the same fixed point; a corrupted network
+    falls back to the plain start."""
+    L = 512
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        x, t = make_training_set(ctx, 12, nk=8, length=L, seed=100)
+        net = WarmStartNet(ctx, L, seed=0)
+        losses = net.train(x, t, epochs=60, batch=32, lr=1e-3, precision="bf16", seed=0)
+        assert losses[-1] < 0.5 * losses[0]
+        rng = np.random.default_rng(999)
+        gains = []
+        for _ in range(3):
+            setup, meta = random_t2g_model(rng, nk=8, length=L)
+            dev = epsx.Scgd0Device(setup, ctx)
+            occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+            _, rc = g0d0_and_converged(dev, occ)
+            n_plain = rc["history"]["err"].shape[1]
+            w = run_warm_started(dev, setup, occ, net)
+            assert w["converged"].all() and rc["converged"].all()
+            # the same fixed point to within the tolerance scale
+            assert np.max(np.abs(w["Σ"] - rc["Σ"])) < 1e-4
+            gains.append((n_plain, w["iterations"], bool(w["fallback"][0])))
+        assert any(wi < npl for npl, wi, fb in gains), gains
+        # a corrupted network: huge output bias -> the prediction is rejected
+        p = net.params
+        p[-2:] = 50.0
+        net.params = p
+        setup, meta = random_t2g_model(rng, nk=8, length=L)
+        dev = epsx.Scgd0Device(setup, ctx)
+        occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+        w = run_warm_started(dev, setup, occ, net)
+        assert w["fallback"].all() and w["converged"].all()
