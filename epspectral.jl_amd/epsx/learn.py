@@ -186,7 +186,8 @@ def run_warm_started(dev, setup, occ, net, *, η_init=5e-3, tol=1e-6, maxiter=30
     1. one iteration from -iη_init gives Σ_G0D0 and μ (counted);
     2. the prediction Σ_p; one iteration from Σ_p and one from Σ_G0D0 (both counted) give the
        residuals err_p and err_0: per temperature the start with the smaller residual is kept
-       (the plain start whenever the prediction does not cut the residual);
+       (the plain start whenever the prediction does not cut the residual, or when no μ
+       satisfies the electron count for it);
     3. the loop continues from the kept iterate to tol.
     Returns dict(Σ, μ, converged, iterations, fallback[T], wall)."""
     from .scgd0 import OMEGA_ACOUSTIC
@@ -200,9 +201,14 @@ def run_warm_started(dev, setup, occ, net, *, η_init=5e-3, tol=1e-6, maxiter=30
         Σp = np.empty_like(Σ0)
         for iT in range(nT):
             Σp[:, :, iT] = predict_sigma(net, Σ0[:, :, iT], setup.ωs, setup.ε_i, μ0[iT], occ.Tlist[iT], precision)
-        rp = dev.loop(occ, maxiter=1, tol=0.0, Σ_init=Σp, μ_start=μ0, **kw)
         rq = dev.loop(occ, maxiter=1, tol=0.0, Σ_init=Σ0, μ_start=μ0, **kw)
-        ep, eq = rp["history"]["err"][:, 0], rq["history"]["err"][:, 0]
+        eq = rq["history"]["err"][:, 0]
+        try:
+            rp = dev.loop(occ, maxiter=1, tol=0.0, Σ_init=Σp, μ_start=μ0, **kw)
+            ep = rp["history"]["err"][:, 0]
+        except ArgumentError:
+            # a prediction so far off that no μ holds the electron count: rejected
+            rp, ep = rq, np.full(nT, np.inf)
         keep = ep < eq
         start = np.where(keep[None, None, :], rp["Σ"], rq["Σ"])
         μs = np.where(keep, rp["μ"], rq["μ"])

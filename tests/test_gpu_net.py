@@ -98,6 +98,7 @@ def test_warm_start_cuts_iterations(ctx):
             # the same fixed point to within the tolerance scale
             assert np.max(np.abs(w["Σ"] - rc["Σ"])) < 1e-4
             gains.append((n_plain, w["iterations"], bool(w["fallback"][0])))
+        print("plain vs warm-started iterations (fallback):", gains)
         assert any(wi < npl for npl, wi, fb in gains), gains
         # a corrupted network: huge output bias -> the prediction is rejected
         p = net.params
