@@ -51,8 +51,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="scgd0", choices=("scgd0", "ladder"),
-                    help="scgd0: the headline solve; ladder: the Kubo ladder on the 16³ model")
+    ap.add_argument("--workload", default="scgd0", choices=("scgd0", "ladder", "warmstart"),
+                    help="scgd0: the headline solve; ladder: the Kubo ladder on the 16³ model; "
+                         "warmstart: train the Σ_init network and evaluate it on held-out models")
+    ap.add_argument("--models", type=int, default=64, help="warmstart: training models (12³)")
+    ap.add_argument("--heldout", type=int, default=4, help="warmstart: held-out models (24³)")
+    ap.add_argument("--epochs", type=int, default=40, help="warmstart: training epochs")
     ap.add_argument("--omegas", type=int, default=4, help="ladder: probe frequencies 0, 5, ... meV timed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (the HBM counter calibration)")
@@ -65,6 +69,8 @@ def main():
         sys.exit(2)
     if args.workload == "ladder":
         return ladder_bench(args)
+    if args.workload == "warmstart":
+        return warmstart_bench(args)
 
     import epsx
     from epsx.models import t2g_setup
@@ -215,6 +221,64 @@ def ladder_bench(args):
                      "apply_s": tm["apply"] / max(tm["applications"], 1)},
         "sigma_ladder_xx": out["σ_ladder"][0, 0, :, 0].tolist(),
         "sigma_bubble_xx": out["σ_bubble"][0, 0, :, 0].tolist(),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def warmstart_bench(args):
+    """The learned Σ_init warm start: a training set from scGD0 runs of `--models` seeded random
+    three-band models at 12³ on 1024 uniform nodes (G0D0 -> converged pairs), the 6-layer,
+    64-channel network trained in BF16 on the device, then `--heldout` held-out random models at
+    24³ run to 1e-6 eV from the plain start and from the warm start.  The value is the wall time
+    of the warm-started solve relative to the plain one (lower is better)."""
+    import warnings
+    import epsx
+    from epsx.learn import WarmStartNet, make_training_set, random_t2g_model, g0d0_and_converged, run_warm_started
+
+    ctx = epsx.Context(0)
+    L = 1024
+    warnings.simplefilter("ignore")
+    t0 = time.perf_counter()
+    x, t = make_training_set(ctx, args.models, nk=12, length=L, seed=2024, states_per_model=96)
+    t_data = time.perf_counter() - t0
+    print(f"[bench] training set: {len(x)} samples from {args.models} models ({t_data:.1f}s)", file=sys.stderr, flush=True)
+    net = WarmStartNet(ctx, L, seed=0)
+    t0 = time.perf_counter()
+    losses = net.train(x, t, epochs=args.epochs, batch=64, lr=1e-3, precision="bf16", seed=0)
+    t_train = time.perf_counter() - t0
+    print(f"[bench] trained {args.epochs} epochs in {t_train:.1f}s, loss {losses[0]:.3e} -> {losses[-1]:.3e}",
+          file=sys.stderr, flush=True)
+    rng = np.random.default_rng(777)
+    rows = []
+    for h in range(args.heldout):
+        setup, meta = random_t2g_model(rng, nk=args.nk, length=L)
+        dev = epsx.Scgd0Device(setup, ctx)
+        occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+        dev.loop(occ, η_min=0.0, η_init=5e-3, maxiter=1, tol=0.0, ω_acoustic=1e-6, method="auto")  # prime
+        t0 = time.perf_counter()
+        _, rc = g0d0_and_converged(dev, occ)
+        t_plain = time.perf_counter() - t0
+        w = run_warm_started(dev, setup, occ, net)
+        dΣ = float(np.max(np.abs(w["Σ"] - rc["Σ"])))
+        rows.append(dict(plain_iterations=int(rc["history"]["err"].shape[1]), warm_iterations=int(w["iterations"]),
+                         fallback=bool(w["fallback"][0]), plain_s=t_plain, warm_s=w["wall"], max_dSigma=dΣ,
+                         T_K=float(meta["Tlist_K"][0]), converged=bool(w["converged"].all() and rc["converged"].all())))
+        print(f"[bench] held-out {h + 1}/{args.heldout}: {rows[-1]}", file=sys.stderr, flush=True)
+    it_p = sum(r["plain_iterations"] for r in rows)
+    it_w = sum(r["warm_iterations"] for r in rows)
+    s_p = sum(r["plain_s"] for r in rows)
+    s_w = sum(r["warm_s"] for r in rows)
+    line = {
+        "metric": "warm-started / plain scGD0 wall time to tol on held-out models",
+        "value": s_w / s_p, "unit": "ratio", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1e3 * s_w,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 net, f64 scGD0",
+        "data": "synthetic",
+        "config": {"workload": f"warm start: {args.models} training models at 12^3, {args.heldout} held-out at "
+                               f"{args.nk}^3, {L} uniform nodes, tol 1e-6 eV",
+                   "samples": int(len(x)), "epochs": args.epochs, "params": net.nparam,
+                   "train_s": t_train, "data_s": t_data, "loss_first": float(losses[0]), "loss_last": float(losses[-1])},
+        "iterations_plain": it_p, "iterations_warm": it_w, "iterations_saved_frac": 1 - it_w / it_p,
+        "heldout": rows,
     }
     print(json.dumps(line), flush=True)
 

@@ -10,11 +10,13 @@ prediction with a fallback: the plain iteration always runs to the same tol, and
 is kept only if its first iteration leaves a smaller residual than the plain start's.
 
 Features per outer state i (channels over the L grid points):
-  0, 1: Re Σ_G0D0, Im Σ_G0D0 (divided by SIGMA_SCALE)
+  0, 1: asinh(Re Σ_G0D0 / SIGMA_SCALE), asinh(Im Σ_G0D0 / SIGMA_SCALE) (the G0D0 self-energy of
+        a narrow -iη_init start has spikes of tens of eV; asinh keeps them in range)
   2:    (ω - μ) / ENERGY_SCALE
   3:    (ε_i - μ) / ENERGY_SCALE (constant)
   4:    T / T_SCALE (constant)
-Targets: (Σ_converged - Σ_G0D0) / SIGMA_SCALE, real and imaginary parts.
+Targets: Σ_converged / SIGMA_SCALE, real and imaginary parts (the converged self-energy is
+smooth and of order 1 eV, unlike the correction from the spiky G0D0 one).
 """
 import ctypes
 import time
@@ -110,24 +112,24 @@ def features(Σ0, ωs, ε_i, μ, T):
     ws = np.asarray(ωs, dtype=np.float64)
     nw, ni = Σ0.shape
     x = np.empty((ni, N_FEATURES, nw), dtype=np.float32)
-    x[:, 0, :] = (Σ0.real / SIGMA_SCALE).T
-    x[:, 1, :] = (Σ0.imag / SIGMA_SCALE).T
+    x[:, 0, :] = np.arcsinh(Σ0.real / SIGMA_SCALE).T
+    x[:, 1, :] = np.arcsinh(Σ0.imag / SIGMA_SCALE).T
     x[:, 2, :] = ((ws - μ) / ENERGY_SCALE)[None, :]
     x[:, 3, :] = ((np.asarray(ε_i) - μ) / ENERGY_SCALE)[:, None]
     x[:, 4, :] = T / T_SCALE
     return x
 
 
-def targets(Σ0, Σc):
-    """Targets [ni, 2, nω]: the scaled correction from Σ0 to the converged Σc."""
-    d = (np.asarray(Σc) - np.asarray(Σ0)) / SIGMA_SCALE
+def targets(Σc):
+    """Targets [ni, 2, nω]: the converged Σc [nω, ni], scaled."""
+    d = np.asarray(Σc) / SIGMA_SCALE
     return np.ascontiguousarray(np.stack([d.real.T, d.imag.T], axis=1), dtype=np.float32)
 
 
 def predict_sigma(net, Σ0, ωs, ε_i, μ, T, precision="bf16"):
     """Σ [nω, ni] predicted from the G0D0 Σ0 [nω, ni] at one temperature."""
     y = net.forward(features(Σ0, ωs, ε_i, μ, T), precision)
-    return np.asarray(Σ0) + SIGMA_SCALE * (y[:, 0, :].T + 1j * y[:, 1, :].T)
+    return SIGMA_SCALE * (y[:, 0, :].T + 1j * y[:, 1, :].T)
 
 
 def random_t2g_model(rng, nk=12, length=1024, half_width=3.0):
@@ -167,7 +169,9 @@ def make_training_set(ctx, n_models, nk=12, length=1024, seed=0, states_per_mode
             continue
         Σ0, Σc = r1["Σ"][:, :, 0], rc["Σ"][:, :, 0]
         x = features(Σ0, setup.ωs, setup.ε_i, r1["μ"][0], meta["Tlist"][0])
-        t = targets(Σ0, Σc)
+        t = targets(Σc)
+        if not (np.isfinite(x).all() and np.isfinite(t).all()):
+            continue
         if states_per_model is not None and states_per_model < len(x):
             pick = rng.choice(len(x), size=states_per_model, replace=False)
             x, t = x[pick], t[pick]
