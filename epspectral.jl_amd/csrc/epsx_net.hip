@@ -513,10 +513,15 @@ int epsx_net_create(epsx_ctx *ctx, const epsx_net_config *cfg, uint64_t seed)
     EPSX_TRY(dev_alloc(ctx, n.grad, 4 * (size_t)n.nparam));
     EPSX_TRY(dev_alloc(ctx, n.m, 4 * (size_t)n.nparam));
     EPSX_TRY(dev_alloc(ctx, n.v, 4 * (size_t)n.nparam));
+    // Adam starts from zero moments at step 0
+    EPSX_HIP(ctx, hipMemsetAsync(n.m.p, 0, 4 * n.nparam, ctx->stream));
+    EPSX_HIP(ctx, hipMemsetAsync(n.v.p, 0, 4 * n.nparam, ctx->stream));
+    n.step = 0;
     int64_t wmax = 0;
     for (int l = 0; l < n.layers; l++) wmax = std::max(wmax, (int64_t)cout[l] * cin[l] * n.kernel);
     EPSX_TRY(dev_alloc(ctx, n.Wt, 4 * (size_t)wmax));
     EPSX_TRY(dev_alloc(ctx, n.loss, 8));
+    EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     n.ready = true;
     return EPSX_OK;
 }

@@ -65,13 +65,19 @@ def test_training_reduces_loss_and_is_reproducible(ctx):
     sm = np.apply_along_axis(lambda r: np.convolve(r, k, mode="same"), 2, x[:, :2, :])
     t = (0.5 * sm[:, ::-1, :] - 0.2 * x[:, 2:4, :]).astype(np.float32)
     l0, _ = net.gradient(x, t, "fp32")
+    # trained straight after creation: the optimiser state starts at zero
     losses = net.train(x, t, epochs=30, batch=16, lr=2e-3, precision="bf16", seed=1)
+    assert np.all(np.isfinite(losses))
     l1, _ = net.gradient(x, t, "fp32")
     assert losses[-1] < 0.2 * losses[0] and l1 < 0.2 * l0
     p1 = net.params
     net.params = p0
     net.train(x, t, epochs=30, batch=16, lr=2e-3, precision="bf16", seed=1)
     assert np.array_equal(net.params, p1)
+    # a fresh network of the same seed trains to the same weights as a reset one
+    fresh = WarmStartNet(ctx, L, n_layers=4, kernel=7, c_hidden=32, seed=3)
+    fresh.train(x, t, epochs=30, batch=16, lr=2e-3, precision="bf16", seed=1)
+    assert np.array_equal(fresh.params, p1)
 
 
 def test_warm_start_cuts_iterations(ctx):
