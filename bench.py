@@ -233,7 +233,7 @@ def warmstart_bench(args):
     of the warm-started solve relative to the plain one (lower is better)."""
     import warnings
     import epsx
-    from epsx.learn import WarmStartNet, make_training_set, random_t2g_model, g0d0_and_converged, run_warm_started
+    from epsx.learn import WarmStartNet, make_training_set, random_t2g_model, run_warm_started
 
     ctx = epsx.Context(0)
     L = 1024
@@ -254,9 +254,10 @@ def warmstart_bench(args):
         setup, meta = random_t2g_model(rng, nk=args.nk, length=L)
         dev = epsx.Scgd0Device(setup, ctx)
         occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
-        dev.loop(occ, η_min=0.0, η_init=5e-3, maxiter=1, tol=0.0, ω_acoustic=1e-6, method="auto")  # prime
+        kw = dict(η_min=0.0, η_init=5e-3, ω_acoustic=1e-6, method="auto", batched=True)
+        dev.loop(occ, maxiter=1, tol=0.0, **kw)  # prime
         t0 = time.perf_counter()
-        _, rc = g0d0_and_converged(dev, occ)
+        rc = dev.loop(occ, maxiter=300, tol=1e-6, **kw)
         t_plain = time.perf_counter() - t0
         w = run_warm_started(dev, setup, occ, net)
         dΣ = float(np.max(np.abs(w["Σ"] - rc["Σ"])))

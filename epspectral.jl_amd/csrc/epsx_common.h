@@ -47,6 +47,8 @@ struct Scgd0Dev {
     DevBuf mult_ptr, mult_idx;
     DevBuf star_ptr;                  // [ni + 1] sorted-position ranges of each star
     DevBuf Gt;                        // star-collapsed coupling G̃ [(ν, i'), i] (built on demand)
+    bool gt_ready = false;            // G̃ holds the current problem's reduction ...
+    double gt_wac = 0;                // ... with modes below this ω_acoustic skipped
     DevBuf chunks, skip, wnu;         // star chunks of the reduction, acoustic flags, ω_ν
     DevBuf hk, ht;                    // cells and weights of ω ± ω_ν (Einstein modes)
     int64_t nchunks = 0;

@@ -417,6 +417,7 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
                          EPSX_ABI_VERSION);
     Scgd0Dev &s = ctx->s;
     s.ready = false;
+    s.gt_ready = false;
     const int64_t nm = P->nm, ni = P->ni, nf = P->nf, nw = P->nw, nwa = P->nwa;
     if (nm < 1 || ni < 1 || nf < 1 || nw < 2 || nwa < 2 || P->nq < 1 || P->nki < 1 || P->nkf < 1)
         return set_error(ctx, EPSX_ERR_ARG, "scgd0 problem: empty dimension (nm=%lld ni=%lld nf=%lld nw=%lld nwa=%lld)",
@@ -630,7 +631,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     if (method == EPSX_FM_STAR) {
         EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
         EPSX_TRY(tmp_alloc(ctx, hocc.b, 16 * (size_t)nw * s.nm * nT));
-        {
+        if (!(par->reuse_star && s.gt_ready && s.gt_wac == par->w_acoustic)) {
             // G̃[(ν, i'), i] = Σ_{f in star(i')} w_f g2[ν, i, f], acoustic modes zeroed
             std::vector<uint8_t> skip(s.nm);
             for (int64_t nu = 0; nu < s.nm; nu++) skip[nu] = s.wnu_host[nu] < par->w_acoustic;
@@ -651,6 +652,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             (void)hipEventDestroy(e1);
             ctx->t_class[4] = ms * 1e-3;
             ctx->n_class[4] = 1;
+            s.gt_ready = true;
+            s.gt_wac = par->w_acoustic;
         }
     }
     const int nbmax = 512;

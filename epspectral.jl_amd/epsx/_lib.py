@@ -113,6 +113,7 @@ class Scgd0Params(C.Structure):
         ("maxiter", I64), ("f_tile", I64),
         ("fm_method", I32), ("precision", I32), ("batched", I32), ("verbosity", I32),
         ("Sigma_init", P),
+        ("reuse_star", I32), ("pad", I32),
     ]
 
 

@@ -169,7 +169,7 @@ class Scgd0Device:
 
     def loop(self, occ: Occupation, *, η_min, η_init, maxiter, tol, gpu_tile=None, Σ_init=None,
              fix_μ=False, ω_acoustic=OMEGA_ACOUSTIC, verbosity=0, method="direct",
-             precision="fp64", batched=False, μ_start=None):
+             precision="fp64", batched=False, μ_start=None, reuse_star=False):
         """`_loop_scgd0` (src/scgd0.jl:189-346) on the device."""
         s = self.setup
         nm, ni, nf = s.dims
@@ -198,6 +198,7 @@ class Scgd0Device:
         par.f_tile = 0 if gpu_tile is None else int(gpu_tile)
         par.fm_method = _lib.FM_METHODS[method]
         par.precision = _lib.PRECISIONS[precision]
+        par.reuse_star = int(bool(reuse_star))
         par.batched = int(bool(batched))
         par.verbosity = int(verbosity)
         par.Sigma_init = ptr(Σ_init)

@@ -54,6 +54,7 @@ struct Scgd0Params
     maxiter::Int64; f_tile::Int64
     fm_method::Int32; precision::Int32; batched::Int32; verbosity::Int32
     Sigma_init::Ptr{ComplexF64}
+    reuse_star::Int32; pad::Int32
 end
 
 mutable struct Scgd0Outputs
@@ -111,7 +112,7 @@ function loop_scgd0(b::MI355XBackend, setup, occ; η_min, η_init, maxiter, tol,
             occ.nelec, occ.spin_degeneracy, Int32(occ.type === :Semiconductor), Int32(fix_μ),
             η_init, tol, ω_acoustic, maxiter, gpu_tile === nothing ? 0 : gpu_tile,
             Int32(fm_method), Int32(precision), Int32(batched), Int32(verbosity),
-            Σi === nothing ? Ptr{ComplexF64}(C_NULL) : pointer(Σi)))
+            Σi === nothing ? Ptr{ComplexF64}(C_NULL) : pointer(Σi), Int32(0), Int32(0)))
         out = Scgd0Outputs(sizeof(Scgd0Outputs), pointer(Σ), pointer(μ), pointer(conv), pointer(herr),
             pointer(hμ), pointer(hA), pointer(wall), pointer(frozen), 0, 0, 0)
         check(b, ccall((:epsx_scgd0_run, libepsx), Cint,

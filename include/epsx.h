@@ -158,6 +158,9 @@ typedef struct {
     int32_t batched;          /* 1: all active temperatures per launch */
     int32_t verbosity;
     const double *Sigma_init; /* NULL (start at -iη_init) or complex [nw, ni, nT] */
+    int32_t reuse_star;       /* 1: keep the star method's G̃ from the previous run on this
+                                 problem when it was built with the same w_acoustic (0: rebuild) */
+    int32_t pad;
 } epsx_scgd0_params;
 
 typedef struct {

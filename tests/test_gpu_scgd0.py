@@ -265,6 +265,11 @@ def test_star_method_matches_direct_and_oracle(ctx):
         ra = dev.loop(o, method="auto", batched=True, **kw)
         ref = oracle.loop_scgd0(s, o.Tlist, o.nlist, 1.0, **kw)
     assert rs["info"]["fm_method"] == 2 and ra["info"]["fm_method"] == 2 and rd["info"]["fm_method"] == 0
+    # the coupling reduction kept from the previous run gives the same bits
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        rr = dev.loop(o, method="star", reuse_star=True, **kw)
+    assert np.array_equal(rr["Σ"], rs["Σ"]) and rr["info"]["timings"]["star_reduce_launches"] == 0
     assert rel(rs["Σ"], rd["Σ"]) <= 1e-12
     assert rel(rs["Σ"], ref["Σ"]) <= 1e-10
     assert np.all(np.abs(rs["μ"] - ref["μ"]) <= 1e-10 * np.abs(ref["μ"]))
