@@ -204,9 +204,11 @@ def run_warm_started(dev, setup, occ, net, *, η_init=5e-3, tol=1e-6, maxiter=30
         warnings.simplefilter("ignore")
         r1 = dev.loop(occ, maxiter=1, tol=0.0, **kw)
         Σ0, μ0 = r1["Σ"], r1["μ"]
+        t1 = time.perf_counter()
         Σp = np.empty_like(Σ0)
         for iT in range(nT):
             Σp[:, :, iT] = predict_sigma(net, Σ0[:, :, iT], setup.ωs, setup.ε_i, μ0[iT], occ.Tlist[iT], precision)
+        t2 = time.perf_counter()
         occ2 = dataclasses.replace(occ, Tlist=np.concatenate([occ.Tlist, occ.Tlist]),
                                    nlist=np.concatenate([occ.nlist, occ.nlist]),
                                    μlist=None if occ.μlist is None else np.concatenate([occ.μlist, occ.μlist]))
@@ -224,8 +226,10 @@ def run_warm_started(dev, setup, occ, net, *, η_init=5e-3, tol=1e-6, maxiter=30
         keep = ep < eq
         start = np.where(keep[None, None, :], Σ2[:, :, :nT], Σ2[:, :, nT:])
         μs = np.where(keep, μ2[:nT], μ2[nT:])
+        t3 = time.perf_counter()
         rf = dev.loop(occ, maxiter=maxiter, tol=tol, Σ_init=start, μ_start=μs, reuse_star=True, **kw)
+    t4 = time.perf_counter()
     n_final = rf["history"]["err"].shape[1]
     return dict(Σ=rf["Σ"], μ=rf["μ"], converged=rf["converged"], iterations=1 + 2 + n_final,
-                iterations_final=n_final, fallback=~keep, err_pred=ep, err_plain=eq,
-                wall=time.perf_counter() - t0)
+                iterations_final=n_final, fallback=~keep, err_pred=ep, err_plain=eq, wall=t4 - t0,
+                phases=dict(g0d0=t1 - t0, predict=t2 - t1, judge=t3 - t2, final=t4 - t3))
