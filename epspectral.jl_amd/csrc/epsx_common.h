@@ -3,7 +3,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
+#include <map>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -92,6 +94,7 @@ struct epsx_ctx {
     epsx::NetDev net;
     std::vector<epsx::DevBuf *> owned;
     epsx::DevBuf gemm_ws;   // split-k partial products
+    std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, split) -> tile shape
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
     double t_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};

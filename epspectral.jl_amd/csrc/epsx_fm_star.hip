@@ -23,15 +23,19 @@ namespace {
 constexpr int SR_THREADS = 256;
 constexpr int SR_CAP = 4096;   // doubles of g2 staged per block (whole stars)
 
-// One block per (chunk of whole stars, outer state i): the chunk's slab of g2p[:, p, i] is read
-// coalesced into LDS (times w_f), then one thread per (ν, i') sums its star in ascending p.  The
-// (ν, p) of a thread's elements advance incrementally: no integer division in the streaming loop.
+constexpr int SR_PER = SR_CAP / SR_THREADS;   // staged doubles per thread
+
+// One block per (chunk of whole stars, outer state i).  The chunk's slab of g2p[:, p, i]
+// (contiguous) is loaded with all of a thread's loads in flight at once (a fixed, unrolled
+// count) and staged in LDS times w_f, then one thread per (ν, i') sums its star in ascending p
+// from LDS.  A star too large for the stage is read from global memory.
 __global__ __launch_bounds__(SR_THREADS) void star_reduce_kernel(
     int64_t nm, int64_t ni, int64_t nfs, const double *__restrict__ g2p, const double *__restrict__ wf,
     const int64_t *__restrict__ star_ptr, const int32_t *__restrict__ chunk_star, const uint8_t *__restrict__ skip_mode,
     double *__restrict__ Gt)
 {
     __shared__ double buf[SR_CAP];
+    const int tid = threadIdx.x;
     const int64_t c = blockIdx.x, i = blockIdx.y;
     const int64_t s0 = chunk_star[c], s1 = chunk_star[c + 1];
     const int64_t p0 = star_ptr[s0], p1 = star_ptr[s1];
@@ -40,26 +44,40 @@ __global__ __launch_bounds__(SR_THREADS) void star_reduce_kernel(
     const double *src = g2p + nm * (p0 + nfs * i);
     const bool staged = (p1 - p0) * nm <= SR_CAP;
     if (staged) {
+        double v[SR_PER], w[SR_PER];
+        int nus[SR_PER];
+        // element e = tid + 256 q sits at mode e % nm of member e / nm, advanced without division
         const int step_p = SR_THREADS / nm32, step_nu = SR_THREADS % nm32;
-        int nu = (int)threadIdx.x % nm32, pl = (int)threadIdx.x / nm32;
-        for (int e = threadIdx.x; e < n; e += SR_THREADS) {
-            buf[e] = skip_mode[nu] ? 0.0 : wf[p0 + pl] * src[e];
+        int nu = tid % nm32, pl = tid / nm32;
+#pragma unroll
+        for (int q = 0; q < SR_PER; q++) {
+            const int e = tid + SR_THREADS * q;
+            v[q] = e < n ? src[e] : 0.0;
+            w[q] = e < n ? wf[p0 + pl] : 0.0;
+            nus[q] = nu;
             nu += step_nu;
             pl += step_p;
             if (nu >= nm32) { nu -= nm32; pl++; }
         }
+#pragma unroll
+        for (int q = 0; q < SR_PER; q++) {
+            const int e = tid + SR_THREADS * q;
+            if (e < n) buf[e] = skip_mode[nus[q]] ? 0.0 : w[q] * v[q];
+        }
         __syncthreads();
     }
     const int nout = (int)((s1 - s0) * nm);
-    for (int o = threadIdx.x; o < nout; o += SR_THREADS) {
+    for (int o = tid; o < nout; o += SR_THREADS) {
         const int nu = o % nm32;
         const int64_t ip = s0 + o / nm32;
         double acc = 0.0;
         if (!skip_mode[nu]) {
-            const int64_t a = star_ptr[ip], b = star_ptr[ip + 1];
-            for (int64_t p = a; p < b; p++) {
-                const int64_t e = (p - p0) * nm + nu;
-                acc += staged ? buf[e] : wf[p] * src[e];
+            const int a = (int)(star_ptr[ip] - p0), b = (int)(star_ptr[ip + 1] - p0);
+            if (staged) {
+#pragma unroll 4
+                for (int pl = a; pl < b; pl++) acc += buf[pl * nm32 + nu];
+            } else {
+                for (int pl = a; pl < b; pl++) acc += wf[p0 + pl] * src[(int64_t)pl * nm + nu];
             }
         }
         Gt[(nu + nm * ip) + nm * ni * i] = acc;

@@ -4,14 +4,20 @@
 //
 // Layout: column-major operands, C[m, n] (+)= A[m, k] B[k, n], for a batch of instances that may
 // share an operand (stride 0) and may be picked through an index array (the active temperatures).
-// One workgroup of 4 waves owns a 64x64 tile of one instance and one split of the k range:
-//   * operand tiles of 64x32 are staged through LDS; the next tile's global loads are issued
-//     into registers before the current tile's MFMAs, so HBM/L2 latency hides behind compute;
-//   * each wave owns a 32x32 sub-tile as 2x2 accumulators of 16x16;
+// One workgroup owns a BM x BN tile of one instance and one split of the k range:
+//   * operand tiles of BM x 32 and 32 x BN are staged through LDS; the next tile's global loads
+//     are issued into registers before the current tile's MFMAs, so HBM/L2 latency hides
+//     behind compute;
+//   * each wave owns a sub-tile as a grid of 16x16 accumulators; the tile shape (64x64 or
+//     128x64 with 32x32 per wave, 64x128 or 128x128 with 32x64 per wave) is picked per problem
+//     size by an autotuner that times each once;
 //   * split-k partials go to a workspace and are summed in ascending split order by a second
 //     kernel, so results are bitwise reproducible;
 //   * workgroups are enumerated XCD-major (dispatch sends workgroup b to XCD b mod 8), so the
 //     tiles one XCD runs are neighbours that share B panels in that XCD's L2.
+#include <array>
+#include <cstdlib>
+
 #include "epsx_common.h"
 
 namespace epsx {
@@ -19,7 +25,7 @@ namespace {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int GM = 64, GN = 64, GK = 32, GT = 256, NXCD = 8;
+constexpr int GK = 32, NXCD = 8;
 
 struct GemmArgs {
     int64_t m, n, k, lda, ldb, ldc, sA, sB, sC;
@@ -48,8 +54,15 @@ __device__ inline void store_out(const GemmArgs &p, int64_t zc, int64_t gi, int6
     }
 }
 
-__global__ __launch_bounds__(GT, 2) void dgemm_kernel(GemmArgs p)
+// A BM x BN tile of one instance and one split of k per workgroup of WMW x WNW waves; each wave
+// owns a (BM / WMW) x (BN / WNW) sub-tile as TM x TN accumulators of 16x16.  Every tile shape
+// runs the same k-tiles of 32 in the same order, so all shapes give the same bits.
+template <int BM, int BN, int WMW, int WNW, int MINW>
+__global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
 {
+    constexpr int T = 64 * WMW * WNW;
+    constexpr int TM = BM / WMW / 16, TN = BN / WNW / 16;
+    constexpr int QA = BM * GK / T, QB = GK * BN / T;
     const int64_t per_xcd = (int64_t)gridDim.x / NXCD;
     const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + blockIdx.x / NXCD;
     if (v >= p.units) return;
@@ -63,44 +76,54 @@ __global__ __launch_bounds__(GT, 2) void dgemm_kernel(GemmArgs p)
     const double *B = p.B + p.sB * opi(p, z, 2);
     const int kt0 = (int)((int64_t)p.ktiles * s / p.S), kt1 = (int)((int64_t)p.ktiles * (s + 1) / p.S);
 
-    __shared__ double As[GK][GM + 1];
-    __shared__ double Bs[GK][GN + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int64_t row0 = (int64_t)tm * GM, col0 = (int64_t)tn * GN;
+    __shared__ double As[GK][BM + 1];
+    __shared__ double Bs[GK][BN + 1];
+    const unsigned utid = threadIdx.x;
+    const int tid = (int)utid, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WMW, wn = wave / WMW;
+    const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
     const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
-    double ra[8], rb[8];
-    // A tile 64 x 32: row tid % 64, k = tid / 64 + 4 q; B tile 32 x 64: k = tid % 32, column tid / 32 + 8 q
+    double ra[QA], rb[QB];
+    // A tile BM x 32: row e % BM, k = e / BM; B tile 32 x BN: k = e % 32, column e / 32 (e = tid + T q)
     auto load = [&](int kt) {
         const int64_t k0 = (int64_t)kt * GK;
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int64_t gi = row0 + (tid & 63), gk = k0 + (tid >> 6) + 4 * q;
+        for (int q = 0; q < QA; q++) {
+            const unsigned e = utid + T * q;
+            const int64_t gi = row0 + e % BM, gk = k0 + e / BM;
             ra[q] = (gi < m && gk < k) ? A[gi + lda * gk] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int64_t gk = k0 + (tid & 31), gj = col0 + (tid >> 5) + 8 * q;
+        for (int q = 0; q < QB; q++) {
+            const unsigned e = utid + T * q;
+            const int64_t gk = k0 + e % GK, gj = col0 + e / GK;
             rb[q] = (gk < k && gj < n) ? B[gk + ldb * gj] : 0.0;
         }
     };
     auto stage = [&]() {
 #pragma unroll
-        for (int q = 0; q < 8; q++) As[(tid >> 6) + 4 * q][tid & 63] = ra[q];
+        for (int q = 0; q < QA; q++) {
+            const unsigned e = utid + T * q;
+            As[e / BM][e % BM] = ra[q];
+        }
 #pragma unroll
-        for (int q = 0; q < 8; q++) Bs[tid & 31][(tid >> 5) + 8 * q] = rb[q];
+        for (int q = 0; q < QB; q++) {
+            const unsigned e = utid + T * q;
+            Bs[e % GK][e / GK] = rb[q];
+        }
     };
-    double4_t acc[2][2];
+    double4_t acc[TM][TN];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < TM; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = (double4_t){0, 0, 0, 0};
+        for (int b = 0; b < TN; b++) acc[a][b] = (double4_t){0, 0, 0, 0};
 
     if (kt0 < kt1) {
         load(kt0);
         stage();
     }
     __syncthreads();
+    const int mb = wm * (BM / WMW), nbase = wn * (BN / WNW);
     for (int kt = kt0; kt < kt1; kt++) {
         const bool more = kt + 1 < kt1;
         if (more) load(kt + 1);
@@ -108,14 +131,16 @@ __global__ __launch_bounds__(GT, 2) void dgemm_kernel(GemmArgs p)
 #pragma unroll
         for (int ks = 0; ks < GK; ks += 4) {
             const int kk = ks + (lane >> 4);
-            const double a0 = As[kk][wm * 32 + (lane & 15)];
-            const double a1 = As[kk][wm * 32 + 16 + (lane & 15)];
-            const double b0 = Bs[kk][wn * 32 + (lane & 15)];
-            const double b1 = Bs[kk][wn * 32 + 16 + (lane & 15)];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            double af[TM], bf[TN];
+#pragma unroll
+            for (int a = 0; a < TM; a++) af[a] = As[kk][mb + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < TN; b++) bf[b] = Bs[kk][nbase + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < TM; a++)
+#pragma unroll
+                for (int b = 0; b < TN; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
         }
         __syncthreads();
         if (more) stage();
@@ -124,13 +149,13 @@ __global__ __launch_bounds__(GT, 2) void dgemm_kernel(GemmArgs p)
     // D[i][j] of a 16x16 f64 tile: element e of lane l holds i = 4 e + l / 16, j = l % 16
     const int64_t zc = opi(p, z, 4);
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < TM; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int b = 0; b < TN; b++)
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const int64_t gi = row0 + wm * 32 + a * 16 + 4 * e + (lane >> 4);
-                const int64_t gj = col0 + wn * 32 + b * 16 + (lane & 15);
+                const int64_t gi = row0 + mb + a * 16 + 4 * e + (lane >> 4);
+                const int64_t gj = col0 + nbase + b * 16 + (lane & 15);
                 if (gi < m && gj < n) {
                     if (p.S == 1) store_out(p, zc, gi, gj, acc[a][b][e]);
                     else p.part[gi + m * gj + m * n * (s + (int64_t)p.S * z)] = acc[a][b][e];
@@ -152,6 +177,30 @@ __global__ __launch_bounds__(256) void dgemm_splitk_reduce_kernel(GemmArgs p)
     store_out(p, opi(p, z, 4), q % p.m, q / p.m, acc);
 }
 
+// The tile shapes the autotuner chooses from: (BM, BN, waves along m, waves along n)
+struct GemmShape { int bm, bn, wmw, wnw; };
+constexpr GemmShape SHAPES[] = {{64, 64, 2, 2}, {128, 64, 4, 2}, {64, 128, 2, 2}, {128, 128, 4, 2}};
+constexpr int NSHAPES = 4;
+
+static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
+{
+    const GemmShape &g = SHAPES[c];
+    p.mt = (int)((p.m + g.bm - 1) / g.bm);
+    p.nt = (int)((p.n + g.bn - 1) / g.bn);
+    p.units = (int64_t)p.mt * p.nt * p.S * (p.units);   // p.units holds the instance count here
+    const int64_t grid = (p.units + NXCD - 1) / NXCD * NXCD;
+    const dim3 blk(64 * g.wmw * g.wnw);
+    switch (c) {
+    // the last parameter: waves per SIMD the register budget is cut for
+    case 0: hipLaunchKernelGGL((dgemm_kernel<64, 64, 2, 2, 3>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
+    case 1: hipLaunchKernelGGL((dgemm_kernel<128, 64, 4, 2, 4>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
+    case 2: hipLaunchKernelGGL((dgemm_kernel<64, 128, 2, 2, 2>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
+    default: hipLaunchKernelGGL((dgemm_kernel<128, 128, 4, 2, 2>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
+    }
+    EPSX_HIP(ctx, hipGetLastError());
+    return EPSX_OK;
+}
+
 }  // namespace
 
 int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *B,
@@ -164,14 +213,12 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     p.A = A; p.B = B; p.C = C; p.Z = bt.Z; p.Zim = bt.Zim;
     p.idx = bt.idx; p.gate = bt.gate; p.imap = bt.imap;
     p.accumulate = accumulate ? 1 : 0;
-    p.mt = (int)((m + GM - 1) / GM);
-    p.nt = (int)((n + GN - 1) / GN);
     p.ktiles = (int)((k + GK - 1) / GK);
-    // Split k to fill the machine: 256 CUs hold 4 workgroups each (the LDS and register use above),
-    // so a launch runs in rounds of 1024 units; pick the split with the least rounds x k-tiles per
-    // unit (+2 for the prologue and epilogue), keeping >= 4 k-tiles per split.  The choice depends
-    // on the shape and the batch size only, so results are bitwise reproducible.
-    const int64_t tiles = (int64_t)p.mt * p.nt;
+    // Split k to fill the machine, by the 64x64 tile count whatever shape runs (so the split,
+    // and with it every sum, does not depend on the tuned shape): 256 CUs hold 4 workgroups
+    // each, so a launch runs in rounds of 1024 units; pick the split with the least rounds x
+    // k-tiles per unit (+2 for the prologue and epilogue), keeping >= 4 k-tiles per split.
+    const int64_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
     constexpr int64_t SLOTS = 256 * 4;
     int S = 1;
     double best = 0;
@@ -182,21 +229,50 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     }
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     EPSX_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
     if (S > 1) {
         const size_t need = 8 * (size_t)m * n * S * bt.n_inst;
         if (ctx->gemm_ws.bytes < need) {
-            if (cap != hipStreamCaptureStatusNone)
-                return set_error(ctx, EPSX_ERR_STATE, "GEMM workspace must be sized before graph capture");
+            if (capturing) return set_error(ctx, EPSX_ERR_STATE, "GEMM workspace must be sized before graph capture");
             EPSX_TRY(dev_alloc(ctx, ctx->gemm_ws, need));
         }
         p.part = static_cast<double *>(ctx->gemm_ws.p);
     }
     p.S = S;
-    p.units = tiles * S * bt.n_inst;
-    const int64_t grid = (p.units + NXCD - 1) / NXCD * NXCD;
-    hipLaunchKernelGGL(dgemm_kernel, dim3((unsigned)grid), dim3(GT), 0, ctx->stream, p);
-    EPSX_HIP(ctx, hipGetLastError());
+    p.units = bt.n_inst;
+    // The tile shape: EPSX_GEMM_SHAPE forces one; otherwise the autotuner times every shape once
+    // on the first product of this size that overwrites its output (outside graph capture) and
+    // keeps the fastest for the context.  All shapes give the same bits.
+    int c = 0;
+    const char *force = getenv("EPSX_GEMM_SHAPE");
+    const std::array<int64_t, 5> key{m, n, k, bt.n_inst, S};
+    auto it = ctx->gemm_tune.find(key);
+    if (force && *force) {
+        c = atoi(force) % NSHAPES;
+    } else if (it != ctx->gemm_tune.end()) {
+        c = it->second;
+    } else if (!accumulate && !capturing) {
+        hipEvent_t e0, e1;
+        EPSX_HIP(ctx, hipEventCreate(&e0));
+        EPSX_HIP(ctx, hipEventCreate(&e1));
+        float bestms = 0;
+        for (int q = 0; q < NSHAPES; q++) {
+            EPSX_TRY(launch_shape(ctx, q, p));   // warm
+            EPSX_HIP(ctx, hipEventRecord(e0, ctx->stream));
+            EPSX_TRY(launch_shape(ctx, q, p));
+            EPSX_HIP(ctx, hipEventRecord(e1, ctx->stream));
+            EPSX_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0;
+            EPSX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            if (q == 0 || ms < bestms) { bestms = ms; c = q; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->gemm_tune[key] = c;
+    }
+    EPSX_TRY(launch_shape(ctx, c, p));
     if (S > 1) {
+        p.mt = p.nt = 0;
         hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
                            dim3(256), 0, ctx->stream, p);
         EPSX_HIP(ctx, hipGetLastError());
