@@ -9,8 +9,8 @@
 //     are issued into registers before the current tile's MFMAs, so HBM/L2 latency hides
 //     behind compute;
 //   * each wave owns a sub-tile as a grid of 16x16 accumulators; the tile shape (64x64 or
-//     128x64 with 32x32 per wave, 64x128 or 128x128 with 32x64 per wave) is picked per problem
-//     size by an autotuner that times each once;
+//     128x64 with 32x32 per wave, 64x128 or 128x128 with 32x64 per wave, 128x128 with 64x64
+//     per wave) is picked per problem size by an autotuner that times each once;
 //   * split-k partials go to a workspace and are summed in ascending split order by a second
 //     kernel, so results are bitwise reproducible;
 //   * workgroups are enumerated XCD-major (dispatch sends workgroup b to XCD b mod 8), so the
@@ -57,12 +57,12 @@ __device__ inline void store_out(const GemmArgs &p, int64_t zc, int64_t gi, int6
 // A BM x BN tile of one instance and one split of k per workgroup of WMW x WNW waves; each wave
 // owns a (BM / WMW) x (BN / WNW) sub-tile as TM x TN accumulators of 16x16.  Every tile shape
 // runs the same k-tiles of 32 in the same order, so all shapes give the same bits.
-template <int BM, int BN, int WMW, int WNW, int MINW>
+template <int BM, int BN, int WMW, int WNW, int MINW, int GKT = GK>
 __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
 {
     constexpr int T = 64 * WMW * WNW;
     constexpr int TM = BM / WMW / 16, TN = BN / WNW / 16;
-    constexpr int QA = BM * GK / T, QB = GK * BN / T;
+    constexpr int QA = BM * GKT / T, QB = GKT * BN / T;
     const int64_t per_xcd = (int64_t)gridDim.x / NXCD;
     const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + blockIdx.x / NXCD;
     if (v >= p.units) return;
@@ -74,19 +74,21 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
     if (p.gate && p.gate[p.idx ? p.idx[z] : z] == 0) return;  // a masked (converged) instance
     const double *A = p.A + p.sA * opi(p, z, 1);
     const double *B = p.B + p.sB * opi(p, z, 2);
-    const int kt0 = (int)((int64_t)p.ktiles * s / p.S), kt1 = (int)((int64_t)p.ktiles * (s + 1) / p.S);
+    // the split's k range in whole 32-wide tiles, whatever the staging width GKT
+    constexpr int R = GK / GKT;
+    const int kt0 = R * (int)((int64_t)p.ktiles * s / p.S), kt1 = R * (int)((int64_t)p.ktiles * (s + 1) / p.S);
 
-    __shared__ double As[GK][BM + 1];
-    __shared__ double Bs[GK][BN + 1];
+    __shared__ double As[GKT][BM + 1];
+    __shared__ double Bs[GKT][BN + 1];
     const unsigned utid = threadIdx.x;
     const int tid = (int)utid, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WMW, wn = wave / WMW;
     const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
     const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
     double ra[QA], rb[QB];
-    // A tile BM x 32: row e % BM, k = e / BM; B tile 32 x BN: k = e % 32, column e / 32 (e = tid + T q)
+    // A tile BM x GKT: row e % BM, k = e / BM; B tile GKT x BN: k = e % GKT, column e / GKT (e = tid + T q)
     auto load = [&](int kt) {
-        const int64_t k0 = (int64_t)kt * GK;
+        const int64_t k0 = (int64_t)kt * GKT;
 #pragma unroll
         for (int q = 0; q < QA; q++) {
             const unsigned e = utid + T * q;
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
 #pragma unroll
         for (int q = 0; q < QB; q++) {
             const unsigned e = utid + T * q;
-            const int64_t gk = k0 + e % GK, gj = col0 + e / GK;
+            const int64_t gk = k0 + e % GKT, gj = col0 + e / GKT;
             rb[q] = (gk < k && gj < n) ? B[gk + ldb * gj] : 0.0;
         }
     };
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
 #pragma unroll
         for (int q = 0; q < QB; q++) {
             const unsigned e = utid + T * q;
-            Bs[e % GK][e / GK] = rb[q];
+            Bs[e % GKT][e / GKT] = rb[q];
         }
     };
     double4_t acc[TM][TN];
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
         if (more) load(kt + 1);
         // A[i][k] of a 16x4 block sits in lane i + 16 k, B[k][j] in lane j + 16 k
 #pragma unroll
-        for (int ks = 0; ks < GK; ks += 4) {
+        for (int ks = 0; ks < GKT; ks += 4) {
             const int kk = ks + (lane >> 4);
             double af[TM], bf[TN];
 #pragma unroll
@@ -179,8 +181,8 @@ __global__ __launch_bounds__(256) void dgemm_splitk_reduce_kernel(GemmArgs p)
 
 // The tile shapes the autotuner chooses from: (BM, BN, waves along m, waves along n)
 struct GemmShape { int bm, bn, wmw, wnw; };
-constexpr GemmShape SHAPES[] = {{64, 64, 2, 2}, {128, 64, 4, 2}, {64, 128, 2, 2}, {128, 128, 4, 2}};
-constexpr int NSHAPES = 4;
+constexpr GemmShape SHAPES[] = {{64, 64, 2, 2}, {128, 64, 4, 2}, {64, 128, 2, 2}, {128, 128, 4, 2}, {128, 128, 2, 2}};
+constexpr int NSHAPES = 5;
 
 static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
 {
@@ -195,7 +197,8 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
     case 0: hipLaunchKernelGGL((dgemm_kernel<64, 64, 2, 2, 3>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
     case 1: hipLaunchKernelGGL((dgemm_kernel<128, 64, 4, 2, 4>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
     case 2: hipLaunchKernelGGL((dgemm_kernel<64, 128, 2, 2, 2>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
-    default: hipLaunchKernelGGL((dgemm_kernel<128, 128, 4, 2, 2>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
+    case 3: hipLaunchKernelGGL((dgemm_kernel<128, 128, 4, 2, 2>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
+    default: hipLaunchKernelGGL((dgemm_kernel<128, 128, 2, 2, 2, 16>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
     }
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
