@@ -46,6 +46,7 @@ struct Scgd0Dev {
     DevBuf ws, off, rgap, ltab;       // output grid: nodes, rel. offsets (int32), 1/gap, cell table
     DevBuf wsa, Wa, jsp, tsp;         // spectral grid: nodes, trapezoid weights, table from ωs
     DevBuf KK;                        // [nw, nw]
+    DevBuf KKo, KKd;                  // KK without its diagonal, and the diagonal
     DevBuf mult_ptr, mult_idx;
     DevBuf star_ptr;                  // [ni + 1] sorted-position ranges of each star
     DevBuf Gt;                        // star-collapsed coupling G̃ [(ν, i'), i] (built on demand)
@@ -148,6 +149,7 @@ void grid_nodes(int64_t n, double origin, double unit, const int64_t *offsets, d
 
 // Kernel launchers (epsx_kk.hip)
 int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K);
+int kk_split(epsx_ctx *ctx, int64_t n, const double *d_K, double *d_Ko, double *d_Kd);
 
 // Kernel time of a stretch of the stream added to ctx->t_class[k] (HIP events; stop() waits for
 // the end event).  Not for use under graph capture.
@@ -188,6 +190,8 @@ struct GemmBatch {
     const int32_t *gate = nullptr;
     double2 *Z = nullptr;
     const double *Zim = nullptr;
+    int precision = EPSX_PREC_FP64;   // FP32 / BF16: operands rounded when staged, FP32 accumulation
+    const double *diag = nullptr;     // with Z: add diag[i] Zim[i, j] in FP64
 };
 // C[m, n] = A[m, k] * B[k, n] (+ C if accumulate), FP64, column-major, on ctx->stream
 int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *B,

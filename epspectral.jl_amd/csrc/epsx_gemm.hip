@@ -17,6 +17,7 @@
 //     tiles one XCD runs are neighbours that share B panels in that XCD's L2.
 #include <array>
 #include <cstdlib>
+#include <type_traits>
 
 #include "epsx_common.h"
 
@@ -31,8 +32,9 @@ struct GemmArgs {
     int64_t m, n, k, lda, ldb, ldc, sA, sB, sC;
     const double *A, *B;
     double *C;
-    double2 *Z;           // complex epilogue: Z = complex(result, Zim), C unused
+    double2 *Z;           // complex epilogue: Z = complex(result + diag[i] Zim, Zim), C unused
     const double *Zim;
+    const double *diag;   // FP64 diagonal term of the complex epilogue, or null
     double *part;         // split-k partials [m, n] per (instance, split)
     const int32_t *idx;   // instance -> operand index for the operands flagged in imap
     const int32_t *gate;  // per operand index: 0 = skip
@@ -47,7 +49,8 @@ __device__ inline void store_out(const GemmArgs &p, int64_t zc, int64_t gi, int6
 {
     const int64_t o = gi + p.ldc * gj + p.sC * zc;
     if (p.Z) {
-        p.Z[o] = make_double2(v, p.Zim[o]);
+        const double im = p.Zim[o];
+        p.Z[o] = make_double2(p.diag ? v + p.diag[gi] * im : v, im);
     } else {
         double *c = p.C + o;
         *c = p.accumulate ? *c + v : v;
@@ -179,6 +182,143 @@ __global__ __launch_bounds__(256) void dgemm_splitk_reduce_kernel(GemmArgs p)
     store_out(p, opi(p, z, 4), q % p.m, q / p.m, acc);
 }
 
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short to_bf16(double x)
+{
+    const float f = (float)x;
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+
+// The same product with the FP64 operands rounded to FP32 (v_mfma_f32_16x16x4f32) or BF16
+// (v_mfma_f32_16x16x32_bf16) as they are staged to LDS, FP32 accumulation, FP64 output.  64x64
+// tiles of 4 waves (32x32 each); operands staged k-contiguous (As[m][k], Bs[n][k]) so a lane
+// reads its k-run of a row in one access.  Split bounds in 32-wide k tiles as the FP64 kernel.
+template <bool BF16>
+__global__ __launch_bounds__(256, 2) void lpgemm_kernel(GemmArgs p)
+{
+    constexpr int BM = 64, BN = 64;
+    const int64_t per_xcd = (int64_t)gridDim.x / NXCD;
+    const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + blockIdx.x / NXCD;
+    if (v >= p.units) return;
+    int64_t r = v;
+    const int tm = (int)(r % p.mt); r /= p.mt;
+    const int tn = (int)(r % p.nt); r /= p.nt;
+    const int s = (int)(r % p.S); r /= p.S;
+    const int64_t z = r;
+    if (p.gate && p.gate[p.idx ? p.idx[z] : z] == 0) return;
+    const double *A = p.A + p.sA * opi(p, z, 1);
+    const double *B = p.B + p.sB * opi(p, z, 2);
+    const int kt0 = (int)((int64_t)p.ktiles * s / p.S), kt1 = (int)((int64_t)p.ktiles * (s + 1) / p.S);
+    typedef typename std::conditional<BF16, unsigned short, float>::type st_t;
+    constexpr int PAD = BF16 ? 8 : 1;
+    __shared__ __align__(16) st_t As[BM][GK + PAD];
+    __shared__ __align__(16) st_t Bs[BN][GK + PAD];
+    const unsigned utid = threadIdx.x;
+    const int tid = (int)utid, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
+    const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
+    auto put = [](st_t &d, double x) {
+        if constexpr (BF16) d = to_bf16(x);
+        else d = (float)x;
+    };
+    double ra[8], rb[8];
+    auto load = [&](int kt) {
+        const int64_t k0 = (int64_t)kt * GK;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const unsigned e = utid + 256 * q;
+            const int64_t gi = row0 + e % BM, gk = k0 + e / BM;
+            ra[q] = (gi < m && gk < k) ? A[gi + lda * gk] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const unsigned e = utid + 256 * q;
+            const int64_t gk = k0 + e % GK, gj = col0 + e / GK;
+            rb[q] = (gk < k && gj < n) ? B[gk + ldb * gj] : 0.0;
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const unsigned e = utid + 256 * q;
+            put(As[e % BM][e / BM], ra[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const unsigned e = utid + 256 * q;
+            put(Bs[e / GK][e % GK], rb[q]);
+        }
+    };
+    f32x4_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    if (kt0 < kt1) {
+        load(kt0);
+        stage();
+    }
+    __syncthreads();
+    for (int kt = kt0; kt < kt1; kt++) {
+        const bool more = kt + 1 < kt1;
+        if (more) load(kt + 1);
+        if constexpr (BF16) {
+            // lane l holds A[l % 16][8 (l / 16) .. + 7] and B[8 (l / 16) .. + 7][l % 16]
+            const int kb = 8 * (lane >> 4);
+            bf16x8_t af[2], bf[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) __builtin_memcpy(&af[a], &As[wm * 32 + a * 16 + (lane & 15)][kb], 16);
+#pragma unroll
+            for (int b = 0; b < 2; b++) __builtin_memcpy(&bf[b], &Bs[wn * 32 + b * 16 + (lane & 15)][kb], 16);
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bf[b], acc[a][b], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < GK; ks += 4) {
+                const int kk = ks + (lane >> 4);
+                float af[2], bf[2];
+#pragma unroll
+                for (int a = 0; a < 2; a++) af[a] = As[wm * 32 + a * 16 + (lane & 15)][kk];
+#pragma unroll
+                for (int b = 0; b < 2; b++) bf[b] = Bs[wn * 32 + b * 16 + (lane & 15)][kk];
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        if (more) stage();
+        __syncthreads();
+    }
+    // C[i][j] of a 16x16 FP32 tile: element e of lane l holds i = 4 (l / 16) + e, j = l % 16
+    const int64_t zc = opi(p, z, 4);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t gi = row0 + wm * 32 + a * 16 + 4 * (lane >> 4) + e;
+                const int64_t gj = col0 + wn * 32 + b * 16 + (lane & 15);
+                if (gi < m && gj < n) {
+                    const double val = (double)acc[a][b][e];
+                    if (p.S == 1) store_out(p, zc, gi, gj, val);
+                    else p.part[gi + m * gj + m * n * (s + (int64_t)p.S * z)] = val;
+                }
+            }
+}
+
 // The tile shapes the autotuner chooses from: (BM, BN, waves along m, waves along n)
 struct GemmShape { int bm, bn, wmw, wnw; };
 constexpr GemmShape SHAPES[] = {{64, 64, 2, 2}, {128, 64, 4, 2}, {64, 128, 2, 2}, {128, 128, 4, 2}, {128, 128, 2, 2}};
@@ -213,7 +353,7 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     GemmArgs p{};
     p.m = m; p.n = n; p.k = k; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
     p.sA = bt.sA; p.sB = bt.sB; p.sC = bt.sC;
-    p.A = A; p.B = B; p.C = C; p.Z = bt.Z; p.Zim = bt.Zim;
+    p.A = A; p.B = B; p.C = C; p.Z = bt.Z; p.Zim = bt.Zim; p.diag = bt.diag;
     p.idx = bt.idx; p.gate = bt.gate; p.imap = bt.imap;
     p.accumulate = accumulate ? 1 : 0;
     p.ktiles = (int)((k + GK - 1) / GK);
@@ -243,6 +383,23 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     }
     p.S = S;
     p.units = bt.n_inst;
+    if (bt.precision == EPSX_PREC_FP32 || bt.precision == EPSX_PREC_BF16) {
+        p.mt = (int)((m + 63) / 64);
+        p.nt = (int)((n + 63) / 64);
+        p.units = (int64_t)p.mt * p.nt * S * bt.n_inst;
+        const int64_t grid = (p.units + NXCD - 1) / NXCD * NXCD;
+        if (bt.precision == EPSX_PREC_BF16)
+            hipLaunchKernelGGL(lpgemm_kernel<true>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
+        else
+            hipLaunchKernelGGL(lpgemm_kernel<false>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
+        EPSX_HIP(ctx, hipGetLastError());
+        if (S > 1) {
+            hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
+                               dim3(256), 0, ctx->stream, p);
+            EPSX_HIP(ctx, hipGetLastError());
+        }
+        return EPSX_OK;
+    }
     // The tile shape: EPSX_GEMM_SHAPE forces one; otherwise the autotuner times every shape once
     // on the first product of this size that overwrites its output (outside graph capture) and
     // keeps the fastest for the context.  All shapes give the same bits.

@@ -28,7 +28,27 @@ __global__ void kk_build_kernel(int64_t n, const double *__restrict__ ws, double
     K[i + n * j] = v / 3.141592653589793;
 }
 
+// Ko = K with a zero diagonal, Kd = diag(K): the principal-value self-term kept apart so a
+// low-precision product can leave it in FP64.
+__global__ void kk_split_kernel(int64_t n, const double *__restrict__ K, double *__restrict__ Ko,
+                                double *__restrict__ Kd)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n * n) return;
+    const int64_t i = q % n, j = q / n;
+    Ko[q] = i == j ? 0.0 : K[q];
+    if (i == j) Kd[i] = K[q];
+}
+
 }  // namespace
+
+int kk_split(epsx_ctx *ctx, int64_t n, const double *d_K, double *d_Ko, double *d_Kd)
+{
+    hipLaunchKernelGGL(kk_split_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, ctx->stream, n, d_K,
+                       d_Ko, d_Kd);
+    EPSX_HIP(ctx, hipGetLastError());
+    return EPSX_OK;
+}
 
 int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K)
 {

@@ -492,6 +492,9 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     // KK matrix (src/scgd0.jl:183)
     EPSX_TRY(dev_alloc(ctx, s.KK, sizeof(double) * nw * nw));
     EPSX_TRY(kk_build(ctx, nw, dptr<double>(s.ws), dptr<double>(s.KK)));
+    EPSX_TRY(dev_alloc(ctx, s.KKo, sizeof(double) * nw * nw));
+    EPSX_TRY(dev_alloc(ctx, s.KKd, sizeof(double) * nw));
+    EPSX_TRY(kk_split(ctx, nw, dptr<double>(s.KK), dptr<double>(s.KKo), dptr<double>(s.KKd)));
 
     // star order of the inner states
     std::vector<int32_t> fsrc;
@@ -597,8 +600,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         return set_error(ctx, EPSX_ERR_ARG, "scgd0 run: null output array");
     if (par->fm_method < EPSX_FM_DIRECT || par->fm_method > EPSX_FM_AUTO)
         return set_error(ctx, EPSX_ERR_ARG, "unknown Fan-Migdal method %d", par->fm_method);
-    if (par->precision != EPSX_PREC_FP64 && par->precision != EPSX_PREC_FP32)
-        return set_error(ctx, EPSX_ERR_ARG, "precision %d is not available for this method", par->precision);
+    if (par->precision != EPSX_PREC_FP64 && par->precision != EPSX_PREC_FP32 && par->precision != EPSX_PREC_BF16)
+        return set_error(ctx, EPSX_ERR_ARG, "unknown precision %d", par->precision);
     // The Fan-Migdal method: the factorised forms only behind their preconditions.
     int method = par->fm_method;
     if (method == EPSX_FM_AUTO) method = (s.einstein && s.star_ok) ? EPSX_FM_STAR : EPSX_FM_DIRECT;
@@ -606,8 +609,9 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     if (method == EPSX_FM_STAR && !(s.einstein && s.star_ok))
         return set_error(ctx, EPSX_ERR_ARG, "the star Fan-Migdal method needs dispersionless modes (ωph[ν, :] "
                          "constant) and ε_f == ε_i of the representative for every inner state");
-    if (method == EPSX_FM_STAR && par->precision != EPSX_PREC_FP64)
-        return set_error(ctx, EPSX_ERR_ARG, "the star Fan-Migdal method runs in FP64 only");
+    if (method == EPSX_FM_DIRECT && par->precision == EPSX_PREC_BF16)
+        return set_error(ctx, EPSX_ERR_ARG, "the direct Fan-Migdal kernel runs in FP64 or FP32 (BF16 is for the "
+                         "star method's products)");
     EPSX_HIP(ctx, hipSetDevice(ctx->device));
     const int64_t nw = s.nw, ni = s.ni, nwa = s.nwa, nf = s.nf;
     const size_t nS = (size_t)nw * ni * nT;
@@ -707,7 +711,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // device mask (graph mode) every kernel skips converged temperatures, and the convergence
     // test writes the history and freezes temperatures on the device.
     auto enqueue = [&](int nA, const int32_t *d_tidx, const std::vector<int32_t> &h_tidx, const int32_t *d_act,
-                       const GraphBook &book, bool events) -> int {
+                       const GraphBook &book, bool events, int gp) -> int {
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[0], st));
         FmTemps tp;
         tp.T = B.T.ptr<double>(); tp.mu = B.mu.ptr<double>(); tp.eta_min = B.eta.ptr<double>();
@@ -727,19 +731,20 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             bt.idx = d_tidx;
             bt.imap = 4;
             bt.gate = d_act;
+            bt.precision = gp;
             EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
                            B.ImS.ptr<double>(), nw, false, bt));
         } else if (par->batched) {
             tp.nT = nA;
             tp.tidx = d_tidx;
-            EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile, par->precision,
+            EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile, gp,
                                          B.part.ptr<double>(), (int64_t)npart, B.ImS.ptr<double>(), false));
         } else {
             for (int z = 0; z < nA; z++) {
                 tp.nT = 1;
                 tp.tidx = d_tidx + z;
                 EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile,
-                                             par->precision, B.part.ptr<double>(), (int64_t)npart,
+                                             gp, B.part.ptr<double>(), (int64_t)npart,
                                              B.ImS.ptr<double>(), false));
             }
         }
@@ -754,7 +759,14 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             bt.gate = d_act;
             bt.Z = B.Snew.ptr<double2>();
             bt.Zim = B.ImS.ptr<double>();
-            EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
+            if (gp == EPSX_PREC_FP64) {
+                EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
+            } else {
+                // the off-diagonal part in low precision, the principal-value diagonal in FP64
+                bt.precision = gp;
+                bt.diag = dptr<double>(s.KKd);
+                EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KKo), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
+            }
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[2], st));
         // 5. multiplets
@@ -833,10 +845,24 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     };
 
     std::vector<int32_t> active(nT, 1), act_idx;
-    std::vector<double> h_err(nT), h_mu(nT), h_minA(nT);
+    std::vector<double> h_err(nT), h_mu(nT), h_minA(nT), prev_err(nT, INFINITY);
     std::vector<int32_t> h_status(nT);
     std::vector<double> mu_host(par->mu_start, par->mu_start + nT);
     int rc = EPSX_OK;
+    // Precision schedule: with FP32 / BF16 the Fan-Migdal products (and the off-diagonal of KK)
+    // run in low precision and no temperature may freeze, until every active temperature is
+    // within 2 tol or has stopped contracting (err > 0.98 err_prev, the low-precision floor);
+    // then FP64 iterations run to tol.
+    bool lp = par->precision != EPSX_PREC_FP64;
+    auto lp_done = [&](int64_t iter) {
+        for (int64_t t = 0; t < nT; t++) {
+            if (!active[t]) continue;
+            const bool near = h_err[t] <= 2 * par->tol;
+            const bool floor_reached = iter >= 3 && h_err[t] > 0.98 * prev_err[t];
+            if (!(near || floor_reached || !(h_err[t] == h_err[t]))) return false;
+        }
+        return true;
+    };
     if (!par->batched) {
         // The reference's loop: the active temperatures one after another, decisions on the host.
         GraphBook none{};
@@ -846,7 +872,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             for (int64_t t = 0; t < nT; t++) if (active[t]) act_idx.push_back((int32_t)t);
             const int nA = (int)act_idx.size();
             EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, act_idx.data(), 4 * nA, hipMemcpyHostToDevice, st));
-            EPSX_TRY(enqueue(nA, B.tidx.ptr<int32_t>(), act_idx, nullptr, none, true));
+            EPSX_TRY(enqueue(nA, B.tidx.ptr<int32_t>(), act_idx, nullptr, none, true,
+                             lp ? par->precision : EPSX_PREC_FP64));
             EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipMemcpyAsync(h_mu.data(), B.mu.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipMemcpyAsync(h_minA.data(), B.minA.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
@@ -860,7 +887,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 out->hist_minA[t + nT * (iter - 1)] = h_minA[t];
                 out->hist_err[t + nT * (iter - 1)] = h_err[t];
                 out->hist_mu[t + nT * (iter - 1)] = h_mu[t];
-                if (h_err[t] < par->tol) {
+                if (!lp && h_err[t] < par->tol) {
                     active[t] = 0;
                     out->iter_frozen[t] = iter;
                 }
@@ -868,6 +895,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             if (rc != EPSX_OK) break;
             out->wall[iter - 1] = now_s() - t0;
             out->niter = iter;
+            if (lp) {
+                out->n_low_precision = (int32_t)iter;
+                if (lp_done(iter)) lp = false;
+                prev_err = h_err;
+            }
             if (par->verbosity > 0) {
                 fprintf(stderr, "scGD0 iteration %lld: wall %.3f s\n", (long long)iter, out->wall[iter - 1]);
                 for (int z = 0; z < nA; z++)
@@ -879,8 +911,9 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         }
     } else {
         // Batched: every temperature in each launch, converged ones masked on the device.  The
-        // first iteration runs eagerly with timing events; the iteration is then captured in a
-        // hipGraph and replayed, the host reading back only the mask and the status words.
+        // first iteration of each precision phase runs eagerly (HIP events time the kernel
+        // classes, the GEMM autotuner sees the shapes); the iteration is then captured in a
+        // hipGraph and replayed, the host reading back only the mask, the status words and err.
         std::vector<int32_t> all(nT);
         for (int64_t t = 0; t < nT; t++) all[t] = (int32_t)t;
         Tmp d_act, d_frozen, d_it, d_he, d_hm, d_ha;
@@ -898,29 +931,42 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         book.hist_err = d_he.ptr<double>(); book.hist_mu = d_hm.ptr<double>(); book.hist_minA = d_ha.ptr<double>();
         book.mu = B.mu.ptr<double>(); book.minA = B.minA.ptr<double>();
         book.nT = nT; book.maxiter = maxiter; book.tol = par->tol;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        GraphBook book_lp = book;
+        book_lp.tol = -1.0;   // nothing freezes in the low-precision phase
+        hipGraph_t graph[2] = {nullptr, nullptr};
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
         struct GraphGuard {
             hipGraph_t *g; hipGraphExec_t *e;
-            ~GraphGuard() { if (*e) (void)hipGraphExecDestroy(*e); if (*g) (void)hipGraphDestroy(*g); }
-        } gg{&graph, &exec};
+            ~GraphGuard() {
+                for (int q = 0; q < 2; q++) {
+                    if (e[q]) (void)hipGraphExecDestroy(e[q]);
+                    if (g[q]) (void)hipGraphDestroy(g[q]);
+                }
+            }
+        } gg{graph, exec};
+        bool eager = true;
         for (int64_t iter = 1; iter <= maxiter; iter++) {
             const double t0 = now_s();
-            if (iter == 1) {
-                EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, true));
+            const int ph = lp ? 1 : 0;
+            const GraphBook &bk = lp ? book_lp : book;
+            const int gp = lp ? par->precision : EPSX_PREC_FP64;
+            if (eager) {
+                EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, bk, iter == 1, gp));
+                eager = false;
             } else {
-                if (!exec) {
+                if (!exec[ph]) {
                     EPSX_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                    int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, false);
-                    hipError_t ec = hipStreamEndCapture(st, &graph);
+                    int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, bk, false, gp);
+                    hipError_t ec = hipStreamEndCapture(st, &graph[ph]);
                     if (r != EPSX_OK) return r;
                     if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture", __FILE__, __LINE__);
-                    EPSX_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                    EPSX_HIP(ctx, hipGraphInstantiate(&exec[ph], graph[ph], nullptr, nullptr, 0));
                 }
-                EPSX_HIP(ctx, hipGraphLaunch(exec, st));
+                EPSX_HIP(ctx, hipGraphLaunch(exec[ph], st));
             }
             EPSX_HIP(ctx, hipMemcpyAsync(active.data(), book.act, 4 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipMemcpyAsync(h_status.data(), B.status.b.p, 4 * nT, hipMemcpyDeviceToHost, st));
+            if (lp) EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipStreamSynchronize(st));
             if (iter == 1) EPSX_TRY(add_event_times((int)nT));
             for (int64_t t = 0; t < nT; t++)
@@ -928,6 +974,14 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             if (rc != EPSX_OK) break;
             out->wall[iter - 1] = now_s() - t0;
             out->niter = iter;
+            if (lp) {
+                out->n_low_precision = (int32_t)iter;
+                if (lp_done(iter)) {
+                    lp = false;
+                    eager = true;
+                }
+                prev_err = h_err;
+            }
             bool any = false;
             for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
             if (par->verbosity > 0) fprintf(stderr, "scGD0 iteration %lld: wall %.4f s\n", (long long)iter, out->wall[iter - 1]);

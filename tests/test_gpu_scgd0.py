@@ -283,3 +283,27 @@ def test_star_method_matches_direct_and_oracle(ctx):
         epsx.loop_scgd0(s2, o, method="star", ctx=ctx, **kw)
     r2 = epsx.loop_scgd0(s2, o, method="auto", ctx=ctx, **kw)
     assert r2["info"]["fm_method"] == 0
+
+
+@pytest.mark.parametrize("prec,method", [("fp32", "star"), ("bf16", "star"), ("fp32", "direct")])
+def test_low_precision_phase_then_fp64_sweep(ctx, prec, method):
+    """FP32 / BF16 products (star) or FP32 terms (direct) until the low-precision floor, then FP64
+    iterations to the same tol: the result agrees with the all-FP64 run to the tolerance scale
+    (both are within tol of the fixed point), in the batched graph loop and the one-by-one loop."""
+    s, meta = t2g_setup(nk=8, Tlist_K=(100.0, 300.0), grid_windows=((0.3, 4e-3), (1.2, 16e-3), (3.0, 64e-3)))
+    o = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    kw = dict(η_min=0.0, η_init=5e-2, maxiter=200, tol=1e-7, ω_acoustic=1e-6, method=method)
+    dev = epsx.Scgd0Device(s, ctx)
+    ref = dev.loop(o, batched=True, **kw)
+    assert ref["converged"].all() and ref["info"]["n_low_precision"] == 0
+    for batched in (True, False):
+        r = dev.loop(o, batched=batched, precision=prec, **kw)
+        assert r["converged"].all()
+        nlp, n = r["info"]["n_low_precision"], r["history"]["err"].shape[1]
+        assert 1 <= nlp < n
+        # the last low-precision iterate is farther from the fixed point than tol allows, the FP64
+        # sweep brings it back: the final residual is below tol
+        assert np.all(r["history"]["err"][np.arange(len(o.Tlist)), r["history"]["iter_frozen"] - 1] < kw["tol"])
+        scale = np.max(np.abs(ref["Σ"]))
+        assert np.max(np.abs(r["Σ"] - ref["Σ"])) <= 50 * kw["tol"] + 1e-9 * scale
+        assert np.max(np.abs(r["μ"] - ref["μ"])) <= 1e-6
