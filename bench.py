@@ -158,6 +158,16 @@ def main():
         "roofline": roof,
     }
     line.update(extra)
+    if args.precision != "fp64":
+        # the error of this result against the all-FP64 solve of the same workload
+        r64 = dev.loop(occ, **dict(kw, precision="fp64"))
+        line["precision_error"] = {
+            "vs": "FP64 solve of the same workload",
+            "max_abs_dSigma_eV": float(np.max(np.abs(r["Σ"] - r64["Σ"]))),
+            "max_rel_dSigma": float(np.max(np.abs(r["Σ"] - r64["Σ"])) / np.max(np.abs(r64["Σ"]))),
+            "max_abs_dmu_eV": float(np.max(np.abs(r["μ"] - r64["μ"]))),
+            "iterations_low_precision": int(r["info"]["n_low_precision"]),
+            "iterations_total": iters[-1], "iterations_fp64_solve": int(r64["history"]["err"].shape[1])}
     if not args.no_plain:
         line["plain_fp64_gpu"] = plain_sample(dev, occ, kw, iters[0], args.plain_iters, nw * nm * ni * nf, g2_bytes)
         line["speedup_vs_plain"] = line["plain_fp64_gpu"]["value"] / s_per_T
