@@ -58,12 +58,13 @@ __device__ inline double occ_boson_d(double e, double T, bool step)
     return -1.0;
 }
 
-// e^{(ω±ωq-μ)/T} from the two split factors; the exponent is clamped where the Fermi factor is
-// already 0 or 1 to the working precision (and far from overflow in the products below).
+// e^{(ω±ωq-μ)/T} from the two split factors, the exponent clamped.
 template <typename FT>
 __device__ inline FT fermi_E(double Mw, int kw, double Mq, int kq)
 {
-    constexpr int KMAX = sizeof(FT) == 4 ? 100 : 900;
+    // 2^±400 (FP64) and 2^±50 (FP32) are far past where f is 0 or 1 to the working precision, and
+    // keep the product of the two denominators below the overflow threshold
+    constexpr int KMAX = sizeof(FT) == 4 ? 50 : 400;
     int k = kw + kq;
     k = k > KMAX ? KMAX : (k < -KMAX ? -KMAX : k);
     return (FT)ldexp(Mw * Mq, k);
@@ -100,8 +101,22 @@ struct Accum<R, float> {
     __device__ double get(int r) const { return (double)s[r] + (double)c[r]; }
 };
 
-template <int R, typename FT>
-__global__ __launch_bounds__(FM_THREADS) void fm_direct_kernel(FmGrid g, FmDirectArgs a,
+// Flat-extrapolated linear interpolation of the staged column at integer grid position pos plus
+// frac, without branches: the cell of the clamped position, and t forced to 0 below the grid and
+// 1 above it (so the end value comes back exactly).
+__device__ inline double2 lookup_ff(const double2 *Sc, const int32_t *ltab, const int32_t *off, const double *rgap,
+                                    int span, int pos, double frac)
+{
+    const int pc = pos < 0 ? 0 : (pos >= span ? span - 1 : pos);
+    const int k = ltab[pc];
+    double t = ((double)(pos - off[k]) + frac) * rgap[k];
+    t = pos < 0 ? 0.0 : (pos >= span ? 1.0 : t);
+    const double2 lo = Sc[k], hi = Sc[k + 1];
+    return make_double2((1 - t) * lo.x + t * hi.x, (1 - t) * lo.y + t * hi.y);
+}
+
+template <int R, typename FT, bool STEP>
+__device__ __forceinline__ void fm_direct_body(FmGrid g, FmDirectArgs a,
                                                                FmTemps tp, int64_t S,
                                                                int64_t seg_len, int64_t nwt,
                                                                const int64_t *__restrict__ star_ptr,
@@ -121,7 +136,7 @@ __global__ __launch_bounds__(FM_THREADS) void fm_direct_kernel(FmGrid g, FmDirec
     if (tp.act && !tp.act[tcol]) return;
     const double T = tp.T[tcol], mu = tp.mu[tcol];
     const double eta = tp.eta_min ? tp.eta_min[tcol] : -INFINITY;
-    const bool step = !(T > 1.4901161193847656e-08);
+    const bool step = STEP;   // T <= sqrt(eps): the step occupations
     const double h = g.dw_unit;
 
     // LDS carve-up
@@ -224,48 +239,31 @@ __global__ __launch_bounds__(FM_THREADS) void fm_direct_kernel(FmGrid g, FmDirec
                 const FT c = (FT)P.c, nq = (FT)P.nq;
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    // absorption: ω + ωq
-                    double2 Sa;
-                    {
-                        const int pos = ow[r] + P.ia;
-                        if (pos < 0) Sa = Sc[0];
-                        else if (pos >= span) Sa = Sc[nw - 1];
-                        else {
-                            const int k = ltab_g[pos];
-                            const double t = ((double)(pos - off[k]) + P.fa) * rgap[k];
-                            const double2 lo = Sc[k], hi = Sc[k + 1];
-                            Sa = make_double2((1 - t) * lo.x + t * hi.x, (1 - t) * lo.y + t * hi.y);
-                        }
-                    }
-                    double2 Se;
-                    {
-                        const int pos = ow[r] + P.ie;
-                        if (pos < 0) Se = Sc[0];
-                        else if (pos >= span) Se = Sc[nw - 1];
-                        else {
-                            const int k = ltab_g[pos];
-                            const double t = ((double)(pos - off[k]) + P.fe) * rgap[k];
-                            const double2 lo = Sc[k], hi = Sc[k + 1];
-                            Se = make_double2((1 - t) * lo.x + t * hi.x, (1 - t) * lo.y + t * hi.y);
-                        }
-                    }
+                    // absorption at ω + ωq, emission at ω - ωq
+                    const double2 Sa = lookup_ff(Sc, ltab_g, off, rgap, (int)span, ow[r] + P.ia, P.fa);
+                    const double2 Se = lookup_ff(Sc, ltab_g, off, rgap, (int)span, ow[r] + P.ie, P.fe);
                     const FT zar = (FT)(wv[r] + P.wq - P.ef - Sa.x), zai = (FT)(-Sa.y);
                     const FT zer = (FT)(wv[r] - P.wq - P.ef - Se.x), zei = (FT)(-Se.y);
                     FT Ea, Ee;
-                    if (!step) {
+                    if (!STEP) {
                         Ea = fermi_E<FT>(Mw[r], kw[r], P.Ma, P.ka);
                         Ee = fermi_E<FT>(Mw[r], kw[r], P.Me, P.ke);
                     } else {
                         // T below sqrt(eps): the step occupation of occ_fermion
-                        const FT BIG = sizeof(FT) == 4 ? (FT)1e30f : (FT)1e270;
+                        const FT BIG = sizeof(FT) == 4 ? (FT)1e15f : (FT)1e100;
                         const double ya = wv[r] + P.wq - mu, ye = wv[r] - P.wq - mu;
                         Ea = ya > 0 ? BIG : (ya < 0 ? (FT)0 : (FT)1);
                         Ee = ye > 0 ? BIG : (ye < 0 ? (FT)0 : (FT)1);
                     }
                     const FT one = (FT)1;
-                    const FT ta = -zai * (nq * (one + Ea) + one) / ((zar * zar + zai * zai) * (one + Ea));
-                    const FT te = -zei * (nq * (one + Ee) + Ee) / ((zer * zer + zei * zei) * (one + Ee));
-                    if (valid[r]) acc.add(r, c * (ta + te));
+                    // Im G(ω+ωq)(n + f) + Im G(ω-ωq)(n + 1 - f) with Im(1/z) = -Im z / |z|^2 and
+                    // f = 1 / (1 + E): one division for both terms
+                    const FT Da = (zar * zar + zai * zai) * (one + Ea);
+                    const FT De = (zer * zer + zei * zei) * (one + Ee);
+                    const FT Na = -zai * (nq * (one + Ea) + one);
+                    const FT Ne = -zei * (nq * (one + Ee) + Ee);
+                    // lanes past the grid end compute on the last node and are never written out
+                    acc.add(r, c * ((Na * De + Ne * Da) / (Da * De)));
                 }
             }
         }
@@ -278,6 +276,21 @@ __global__ __launch_bounds__(FM_THREADS) void fm_direct_kernel(FmGrid g, FmDirec
         const int64_t jw = w0 + tid + (int64_t)FM_THREADS * r;
         if (jw < nw) out[jw] = acc.get(r);
     }
+}
+
+// The block's temperature picks the occupation form once (T <= sqrt(eps): step functions).
+template <int R, typename FT>
+__global__ __launch_bounds__(FM_THREADS) void fm_direct_kernel(FmGrid g, FmDirectArgs a, FmTemps tp, int64_t S,
+                                                               int64_t seg_len, int64_t nwt,
+                                                               const int64_t *__restrict__ star_ptr,
+                                                               double *__restrict__ part)
+{
+    const int64_t tcol = tp.tidx[blockIdx.y];
+    if (tp.act && !tp.act[tcol]) return;
+    if (tp.T[tcol] > 1.4901161193847656e-08)
+        fm_direct_body<R, FT, false>(g, a, tp, S, seg_len, nwt, star_ptr, part);
+    else
+        fm_direct_body<R, FT, true>(g, a, tp, S, seg_len, nwt, star_ptr, part);
 }
 
 // ImS[ω, i, tidx[tz]] (+)= Σ_s part[tz, s, i, ω], s ascending
