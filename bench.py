@@ -208,29 +208,40 @@ def ladder_bench(args):
     # prime: B̃ of the first temperature and one solve (not timed)
     ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs[:1], rtol=1e-2, maxiter=30)
     t0 = time.perf_counter()
-    out = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs, rtol=1e-2, maxiter=30)
+    out = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs, rtol=1e-2, maxiter=30,
+                  precision=args.precision)
     el = time.perf_counter() - t0
     tm = ctx.ladder_timings()
+    # the same (Ω, T) by an FP64 GMRES at rtol 1e-8 (maxiter 60): the reference for the errors
+    ref = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs, rtol=1e-8, maxiter=60)
+
+    def relerr(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     ns = L.nshift
     flops = 2.0 * (16 * nw) * (3 * ni) * (6 * ni * ns)
     gemm_per_app = tm["shift_gemm"] / max(tm["applications"], 1)
     achieved = flops / gemm_per_app / 1e12
+    peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": 157.3, "bf16": 2500.0}[args.precision]
     line = {
         "metric": "ladder s per (Ω, T) solve at rtol 1e-2; shift-table GEMM TFLOP/s",
         "value": el / len(Ωs), "unit": "s/(Ω,T)", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": 1e3 * el, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+        "ms_per_step": 1e3 * el, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else f"{args.precision} shift GEMM, f64 GMRES", "data": "synthetic",
         "config": {"workload": f"t2g {nk}^3 ladder, GMRES rtol 1e-2, Ω = 0..{5 * (len(Ωs) - 1)} meV at 100 K",
                    "nk": nk, "ni": ni, "nf": nf, "nw": nw, "nshift": ns, "niter": out["niter"][:, 0].tolist(),
                    "scgd0_s_4T": t_scgd0, "applications": tm["applications"],
                    "extrapolated_s_41x4": el / len(Ωs) * 41 * 4},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "dgemm_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)",
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "dgemm_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)" if args.precision == "fp64"
+                     else f"lpgemm_kernel (shift-table GEMM W = J B̃, {args.precision} operands, FP32 MFMA accumulation)",
                      "algorithmic_flop_per_application": flops, "gemm_s_per_application": gemm_per_app,
                      "apply_s": tm["apply"] / max(tm["applications"], 1)},
         "sigma_ladder_xx": out["σ_ladder"][0, 0, :, 0].tolist(),
         "sigma_bubble_xx": out["σ_bubble"][0, 0, :, 0].tolist(),
+        "error_vs_fp64_rtol1e-8": {"sigma_ladder_max_rel": relerr(out["σ_ladder"], ref["σ_ladder"]),
+                                   "sigma_bubble_max_rel": relerr(out["σ_bubble"], ref["σ_bubble"]),
+                                   "reference_niter": ref["niter"][:, 0].tolist()},
     }
     print(json.dumps(line), flush=True)
 

@@ -435,7 +435,7 @@ static int ensure_Bt(epsx_ctx *ctx, double T, double w_ac)
 }
 
 // LJ = L(J) on device vectors at the temperature whose B̃ is loaded and the propagators in l.Gm/l.Gp
-static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ)
+static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int precision)
 {
     Scgd0Dev &s = ctx->s;
     LadderDev &l = ctx->l;
@@ -452,8 +452,10 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ)
     for (int64_t m0 = 0; m0 < l.ns; m0 += mb) {
         const int64_t mc = m0 + mb <= l.ns ? mb : l.ns - m0;
         ClassTimer t_gemm(ctx, 0);
-        EPSX_TRY(dgemm_nn(ctx, nrow, ncol * mc, K, dptr<double>(l.Jm), nrow, dptr<double>(l.Bt) + K * ncol * m0, K,
-                          dptr<double>(l.W), nrow, false));
+        GemmBatch bt;
+        bt.precision = precision;   // FP32 / BF16 operands with FP32 accumulation, or FP64
+        EPSX_TRY(dgemm(ctx, nrow, ncol * mc, K, dptr<double>(l.Jm), nrow, dptr<double>(l.Bt) + K * ncol * m0, K,
+                       dptr<double>(l.W), nrow, false, bt));
         EPSX_TRY(t_gemm.stop(1));
         hipLaunchKernelGGL(shift_add_chunk_kernel, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns, m0, mc,
                            dptr<double>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
@@ -587,7 +589,8 @@ extern "C" int epsx_apply_ladder_kernel(epsx_ctx *ctx, const double *J, const do
     if (!l.ready) return set_error(ctx, EPSX_ERR_STATE, "epsx_apply_ladder_kernel before epsx_ladder_set_problem");
     if (!J || !Gm || !Gp || !LJ) return set_error(ctx, EPSX_ERR_ARG, "apply_ladder_kernel!: null array");
     if (!(w_acoustic > 0)) return set_error(ctx, EPSX_ERR_ARG, "ω_acoustic must be positive, got %g", w_acoustic);
-    if (precision != EPSX_PREC_FP64) return set_error(ctx, EPSX_ERR_ARG, "ladder precision %d is not available", precision);
+    if (precision != EPSX_PREC_FP64 && precision != EPSX_PREC_FP32 && precision != EPSX_PREC_BF16)
+        return set_error(ctx, EPSX_ERR_ARG, "unknown ladder precision %d", precision);
     EPSX_HIP(ctx, hipSetDevice(ctx->device));
     for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
     const int64_t n = s.nw * 24 * s.ni, ng = s.nw * 4 * s.ni;
@@ -597,7 +600,7 @@ extern "C" int epsx_apply_ladder_kernel(epsx_ctx *ctx, const double *J, const do
     EPSX_HIP(ctx, hipMemcpyAsync(l.Gm.p, Gm, 16 * ng, hipMemcpyHostToDevice, ctx->stream));
     EPSX_HIP(ctx, hipMemcpyAsync(l.Gp.p, Gp, 16 * ng, hipMemcpyHostToDevice, ctx->stream));
     EPSX_TRY(ensure_Bt(ctx, T, w_acoustic));
-    EPSX_TRY(ladder_apply_dev(ctx, dJ.ptr<double2>(), dLJ.ptr<double2>()));
+    EPSX_TRY(ladder_apply_dev(ctx, dJ.ptr<double2>(), dLJ.ptr<double2>(), precision));
     EPSX_HIP(ctx, hipMemcpyAsync(LJ, dLJ.b.p, 16 * n, hipMemcpyDeviceToHost, ctx->stream));
     EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return EPSX_OK;
@@ -638,7 +641,8 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
                          "no row for a negative phonon frequency", par->w_acoustic);
     if (par->nT < 1 || !par->Tlist || !par->mu || !par->Sigma || !par->Omegas)
         return set_error(ctx, EPSX_ERR_ARG, "ladder run: null input");
-    if (par->precision != EPSX_PREC_FP64) return set_error(ctx, EPSX_ERR_ARG, "ladder precision %d is not available", par->precision);
+    if (par->precision != EPSX_PREC_FP64 && par->precision != EPSX_PREC_FP32 && par->precision != EPSX_PREC_BF16)
+        return set_error(ctx, EPSX_ERR_ARG, "unknown ladder precision %d", par->precision);
     if (!out->sigma_bubble || !out->sigma_ladder || !out->converged || !out->niter || !out->norms || !out->sigma_iter)
         return set_error(ctx, EPSX_ERR_ARG, "ladder run: null output");
     EPSX_HIP(ctx, hipSetDevice(ctx->device));
@@ -703,7 +707,7 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
                 const double atol = par->rtol * std::sqrt(nrm.x);
                 const bool warm = iO > 0;
                 if (warm) {
-                    EPSX_TRY(ladder_apply_dev(ctx, dJ, dLJ));
+                    EPSX_TRY(ladder_apply_dev(ctx, dJ, dLJ, par->precision));
                     // r = b - (x - Lx)
                     hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, dJ0, -1.0, dJ, 1.0, dLJ, dR);
                 } else {
@@ -729,7 +733,7 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
                     while (!(solved || niter >= it_max || breakdown)) {
                         const int64_t j = niter++;
                         double2 *Vj = Vb + n * j, *q = dR;
-                        EPSX_TRY(ladder_apply_dev(ctx, Vj, dLJ));
+                        EPSX_TRY(ladder_apply_dev(ctx, Vj, dLJ, par->precision));
                         hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, Vj, -1.0, dLJ, 0.0, dLJ, q);
                         for (int64_t i = 0; i <= j; i++) {
                             EPSX_TRY(ldot(ctx, n, Vb + n * i, q, dh + i));
@@ -775,7 +779,7 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
                 EPSX_HIP(ctx, hipMemcpyAsync(dJ, dJ0, 16 * n, hipMemcpyDeviceToDevice, st));
                 double *sit = out->sigma_iter + 9 * it_max * slot;
                 for (int64_t it = 1; it <= it_max; it++) {
-                    EPSX_TRY(ladder_apply_dev(ctx, dJ, dLJ));
+                    EPSX_TRY(ladder_apply_dev(ctx, dJ, dLJ, par->precision));
                     hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, dJ0, 1.0, dLJ, 0.0, dLJ, dJ);
                     EPSX_TRY(sigma_of(ctx, dJ, Om, T, sit + 9 * (it - 1)));
                     niter = it;

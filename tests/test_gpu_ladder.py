@@ -111,3 +111,24 @@ def test_holstein_ladder_equals_bubble(ctx):
         epsx.run_scgd0_ladder(dev, L, res, occ, ω_acoustic=-1.0)
     with pytest.raises(epsx.ArgumentError, match="solver"):
         epsx.run_scgd0_ladder(dev, L, res, occ, solver="newton")
+
+
+@pytest.mark.parametrize("prec,tol_apply", [("fp32", 1e-5), ("bf16", 2e-2)])
+def test_low_precision_shift_gemm(ctx, prec, tol_apply):
+    """The shift-table GEMMs in FP32 / BF16 (FP32 accumulation) against FP64: one application, and
+    σ of a GMRES solve at rtol 1e-2 against the FP64 solve at rtol 1e-8 (C4's check)."""
+    s, L, r = disordered_case()
+    ld = device(ctx, s, L)
+    nw, ni = len(s.ωs), s.g2.shape[1]
+    J = pseudorandom_complex(nw, 4, 3, 2, ni, seed=2.1)
+    Gm, Gp = propagators(L, r, 0, 0.2, 0.1)
+    ref = ld.apply(J, Gm, Gp, 0.1, 1e-6)
+    lp = ld.apply(J, Gm, Gp, 0.1, 1e-6, precision=prec)
+    assert rel(lp, ref) <= tol_apply
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        tight = ld.loop(r["Σ"], r["μ"], [0.1], [0.3], rtol=1e-8, maxiter=60)
+        loose = ld.loop(r["Σ"], r["μ"], [0.1], [0.3], rtol=1e-2, maxiter=30, precision=prec)
+    assert np.array_equal(loose["σ_bubble"], tight["σ_bubble"])
+    assert loose["converged"].all()
+    assert rel(loose["σ_ladder"], tight["σ_ladder"]) <= 1e-2
