@@ -128,7 +128,10 @@ def test_low_precision_shift_gemm(ctx, prec, tol_apply):
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         tight = ld.loop(r["Σ"], r["μ"], [0.1], [0.3], rtol=1e-8, maxiter=60)
+        loose64 = ld.loop(r["Σ"], r["μ"], [0.1], [0.3], rtol=1e-2, maxiter=30)
         loose = ld.loop(r["Σ"], r["μ"], [0.1], [0.3], rtol=1e-2, maxiter=30, precision=prec)
     assert np.array_equal(loose["σ_bubble"], tight["σ_bubble"])
     assert loose["converged"].all()
-    assert rel(loose["σ_ladder"], tight["σ_ladder"]) <= 1e-2
+    # the precision effect (same rtol) is far below the solver-tolerance effect (rtol 1e-2 vs 1e-8)
+    assert rel(loose["σ_ladder"], loose64["σ_ladder"]) <= (1e-6 if prec == "fp32" else 5e-3)
+    assert rel(loose["σ_ladder"], tight["σ_ladder"]) <= 3e-2
