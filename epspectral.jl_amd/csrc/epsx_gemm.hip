@@ -13,8 +13,9 @@
 //     per wave) is picked per problem size by an autotuner that times each once;
 //   * split-k partials go to a workspace and are summed in ascending split order by a second
 //     kernel, so results are bitwise reproducible;
-//   * workgroups are enumerated XCD-major (dispatch sends workgroup b to XCD b mod 8), so the
-//     tiles one XCD runs are neighbours that share B panels in that XCD's L2.
+//   * the units of the active (ungated) instances are dealt XCD-major (dispatch sends workgroup
+//     b to XCD b mod 8), so the tiles one XCD runs are neighbours that share B panels in that
+//     XCD's L2, and masked instances leave no XCD idle.
 #include <array>
 #include <cstdlib>
 #include <type_traits>
@@ -40,10 +41,36 @@ struct GemmArgs {
     const int32_t *gate;  // per operand index: 0 = skip
     int imap;             // bit 0: A, bit 1: B, bit 2: C/Z/Zim
     int S, mt, nt, ktiles, accumulate;
-    int64_t units;
+    int64_t n_inst;
 };
 
 __device__ inline int64_t opi(const GemmArgs &p, int64_t z, int bit) { return (p.idx && (p.imap & bit)) ? p.idx[z] : z; }
+
+// The work unit of this workgroup among the ACTIVE instances only (gated ones are skipped): the
+// units (tile m, tile n, split, active instance) are enumerated with the instance slowest and
+// dealt XCD-major over the 8 XCDs (dispatch sends workgroup b to XCD b mod 8), so each XCD runs
+// a contiguous run of neighbouring tiles however many instances are active.  Returns false for
+// a workgroup with no unit.
+__device__ inline bool unit_of_block(const GemmArgs &p, int &tm, int &tn, int &s, int64_t &z)
+{
+    int64_t nact = p.n_inst;
+    if (p.gate) {
+        nact = 0;
+        for (int64_t q = 0; q < p.n_inst; q++) nact += p.gate[p.idx ? p.idx[q] : q] != 0;
+    }
+    const int64_t U = (int64_t)p.mt * p.nt * p.S * nact;
+    const int64_t per_xcd = (U + NXCD - 1) / NXCD;
+    const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + blockIdx.x / NXCD;
+    if (v >= U) return false;
+    int64_t r = v;
+    tm = (int)(r % p.mt); r /= p.mt;
+    tn = (int)(r % p.nt); r /= p.nt;
+    s = (int)(r % p.S); r /= p.S;
+    if (!p.gate) { z = r; return true; }
+    for (int64_t q = 0; q < p.n_inst; q++)
+        if (p.gate[p.idx ? p.idx[q] : q] != 0 && r-- == 0) { z = q; return true; }
+    return false;
+}
 
 __device__ inline void store_out(const GemmArgs &p, int64_t zc, int64_t gi, int64_t gj, double v)
 {
@@ -66,15 +93,9 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
     constexpr int T = 64 * WMW * WNW;
     constexpr int TM = BM / WMW / 16, TN = BN / WNW / 16;
     constexpr int QA = BM * GKT / T, QB = GKT * BN / T;
-    const int64_t per_xcd = (int64_t)gridDim.x / NXCD;
-    const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + blockIdx.x / NXCD;
-    if (v >= p.units) return;
-    int64_t r = v;
-    const int tm = (int)(r % p.mt); r /= p.mt;
-    const int tn = (int)(r % p.nt); r /= p.nt;
-    const int s = (int)(r % p.S); r /= p.S;
-    const int64_t z = r;
-    if (p.gate && p.gate[p.idx ? p.idx[z] : z] == 0) return;  // a masked (converged) instance
+    int tm, tn, s;
+    int64_t z;
+    if (!unit_of_block(p, tm, tn, s, z)) return;
     const double *A = p.A + p.sA * opi(p, z, 1);
     const double *B = p.B + p.sB * opi(p, z, 2);
     // the split's k range in whole 32-wide tiles, whatever the staging width GKT
@@ -202,15 +223,9 @@ template <bool BF16>
 __global__ __launch_bounds__(256, 2) void lpgemm_kernel(GemmArgs p)
 {
     constexpr int BM = 64, BN = 64;
-    const int64_t per_xcd = (int64_t)gridDim.x / NXCD;
-    const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + blockIdx.x / NXCD;
-    if (v >= p.units) return;
-    int64_t r = v;
-    const int tm = (int)(r % p.mt); r /= p.mt;
-    const int tn = (int)(r % p.nt); r /= p.nt;
-    const int s = (int)(r % p.S); r /= p.S;
-    const int64_t z = r;
-    if (p.gate && p.gate[p.idx ? p.idx[z] : z] == 0) return;
+    int tm, tn, s;
+    int64_t z;
+    if (!unit_of_block(p, tm, tn, s, z)) return;
     const double *A = p.A + p.sA * opi(p, z, 1);
     const double *B = p.B + p.sB * opi(p, z, 2);
     const int kt0 = (int)((int64_t)p.ktiles * s / p.S), kt1 = (int)((int64_t)p.ktiles * (s + 1) / p.S);
@@ -329,8 +344,8 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
     const GemmShape &g = SHAPES[c];
     p.mt = (int)((p.m + g.bm - 1) / g.bm);
     p.nt = (int)((p.n + g.bn - 1) / g.bn);
-    p.units = (int64_t)p.mt * p.nt * p.S * (p.units);   // p.units holds the instance count here
-    const int64_t grid = (p.units + NXCD - 1) / NXCD * NXCD;
+    const int64_t units = (int64_t)p.mt * p.nt * p.S * p.n_inst;
+    const int64_t grid = (units + NXCD - 1) / NXCD * NXCD;
     const dim3 blk(64 * g.wmw * g.wnw);
     switch (c) {
     // the last parameter: waves per SIMD the register budget is cut for
@@ -382,12 +397,11 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
         p.part = static_cast<double *>(ctx->gemm_ws.p);
     }
     p.S = S;
-    p.units = bt.n_inst;
+    p.n_inst = bt.n_inst;
     if (bt.precision == EPSX_PREC_FP32 || bt.precision == EPSX_PREC_BF16) {
         p.mt = (int)((m + 63) / 64);
         p.nt = (int)((n + 63) / 64);
-        p.units = (int64_t)p.mt * p.nt * S * bt.n_inst;
-        const int64_t grid = (p.units + NXCD - 1) / NXCD * NXCD;
+        const int64_t grid = ((int64_t)p.mt * p.nt * S * bt.n_inst + NXCD - 1) / NXCD * NXCD;
         if (bt.precision == EPSX_PREC_BF16)
             hipLaunchKernelGGL(lpgemm_kernel<true>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
         else
