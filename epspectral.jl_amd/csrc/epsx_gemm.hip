@@ -27,7 +27,7 @@ namespace {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-constexpr int GK = 32, NXCD = 8;
+constexpr int GK = 32, NXCD = 8, NSPLIT_TAB = 16;
 
 struct GemmArgs {
     int64_t m, n, k, lda, ldb, ldc, sA, sB, sC;
@@ -42,7 +42,22 @@ struct GemmArgs {
     int imap;             // bit 0: A, bit 1: B, bit 2: C/Z/Zim
     int S, mt, nt, ktiles, accumulate;
     int64_t n_inst;
+    int S_of[NSPLIT_TAB + 1];   // the k split for c active instances (c <= NSPLIT_TAB; else S)
+    int64_t units_per_tile;     // the most (split, instance) units any active count gives a tile
 };
+
+// the number of active instances (gates) and the split that goes with it
+__device__ inline int64_t active_instances(const GemmArgs &p)
+{
+    if (!p.gate) return p.n_inst;
+    int64_t nact = 0;
+    for (int64_t q = 0; q < p.n_inst; q++) nact += p.gate[p.idx ? p.idx[q] : q] != 0;
+    return nact;
+}
+__device__ inline int split_for(const GemmArgs &p, int64_t nact)
+{
+    return (nact >= 1 && nact <= NSPLIT_TAB) ? p.S_of[nact] : p.S;
+}
 
 __device__ inline int64_t opi(const GemmArgs &p, int64_t z, int bit) { return (p.idx && (p.imap & bit)) ? p.idx[z] : z; }
 
@@ -51,21 +66,18 @@ __device__ inline int64_t opi(const GemmArgs &p, int64_t z, int bit) { return (p
 // dealt XCD-major over the 8 XCDs (dispatch sends workgroup b to XCD b mod 8), so each XCD runs
 // a contiguous run of neighbouring tiles however many instances are active.  Returns false for
 // a workgroup with no unit.
-__device__ inline bool unit_of_block(const GemmArgs &p, int &tm, int &tn, int &s, int64_t &z)
+__device__ inline bool unit_of_block(const GemmArgs &p, int &tm, int &tn, int &s, int64_t &z, int &S)
 {
-    int64_t nact = p.n_inst;
-    if (p.gate) {
-        nact = 0;
-        for (int64_t q = 0; q < p.n_inst; q++) nact += p.gate[p.idx ? p.idx[q] : q] != 0;
-    }
-    const int64_t U = (int64_t)p.mt * p.nt * p.S * nact;
+    const int64_t nact = active_instances(p);
+    S = split_for(p, nact);
+    const int64_t U = (int64_t)p.mt * p.nt * S * nact;
     const int64_t per_xcd = (U + NXCD - 1) / NXCD;
     const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + blockIdx.x / NXCD;
     if (v >= U) return false;
     int64_t r = v;
     tm = (int)(r % p.mt); r /= p.mt;
     tn = (int)(r % p.nt); r /= p.nt;
-    s = (int)(r % p.S); r /= p.S;
+    s = (int)(r % S); r /= S;
     if (!p.gate) { z = r; return true; }
     for (int64_t q = 0; q < p.n_inst; q++)
         if (p.gate[p.idx ? p.idx[q] : q] != 0 && r-- == 0) { z = q; return true; }
@@ -93,14 +105,14 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
     constexpr int T = 64 * WMW * WNW;
     constexpr int TM = BM / WMW / 16, TN = BN / WNW / 16;
     constexpr int QA = BM * GKT / T, QB = GKT * BN / T;
-    int tm, tn, s;
+    int tm, tn, s, S;
     int64_t z;
-    if (!unit_of_block(p, tm, tn, s, z)) return;
+    if (!unit_of_block(p, tm, tn, s, z, S)) return;
     const double *A = p.A + p.sA * opi(p, z, 1);
     const double *B = p.B + p.sB * opi(p, z, 2);
     // the split's k range in whole 32-wide tiles, whatever the staging width GKT
     constexpr int R = GK / GKT;
-    const int kt0 = R * (int)((int64_t)p.ktiles * s / p.S), kt1 = R * (int)((int64_t)p.ktiles * (s + 1) / p.S);
+    const int kt0 = R * (int)((int64_t)p.ktiles * s / S), kt1 = R * (int)((int64_t)p.ktiles * (s + 1) / S);
 
     __shared__ double As[GKT][BM + 1];
     __shared__ double Bs[GKT][BN + 1];
@@ -183,8 +195,8 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
                 const int64_t gi = row0 + mb + a * 16 + 4 * e + (lane >> 4);
                 const int64_t gj = col0 + nbase + b * 16 + (lane & 15);
                 if (gi < m && gj < n) {
-                    if (p.S == 1) store_out(p, zc, gi, gj, acc[a][b][e]);
-                    else p.part[gi + m * gj + m * n * (s + (int64_t)p.S * z)] = acc[a][b][e];
+                    if (S == 1) store_out(p, zc, gi, gj, acc[a][b][e]);
+                    else p.part[gi + m * gj + m * n * (s + (int64_t)S * z)] = acc[a][b][e];
                 }
             }
 }
@@ -197,9 +209,11 @@ __global__ __launch_bounds__(256) void dgemm_splitk_reduce_kernel(GemmArgs p)
     const int64_t z = blockIdx.y;
     if (q >= mn) return;
     if (p.gate && p.gate[p.idx ? p.idx[z] : z] == 0) return;
-    const double *src = p.part + mn * (int64_t)p.S * z + q;
+    const int S = split_for(p, active_instances(p));
+    if (S == 1) return;   // the product wrote its result directly
+    const double *src = p.part + mn * (int64_t)S * z + q;
     double acc = src[0];
-    for (int s = 1; s < p.S; s++) acc += src[mn * s];
+    for (int s = 1; s < S; s++) acc += src[mn * s];
     store_out(p, opi(p, z, 4), q % p.m, q / p.m, acc);
 }
 
@@ -223,12 +237,12 @@ template <bool BF16>
 __global__ __launch_bounds__(256, 2) void lpgemm_kernel(GemmArgs p)
 {
     constexpr int BM = 64, BN = 64;
-    int tm, tn, s;
+    int tm, tn, s, S;
     int64_t z;
-    if (!unit_of_block(p, tm, tn, s, z)) return;
+    if (!unit_of_block(p, tm, tn, s, z, S)) return;
     const double *A = p.A + p.sA * opi(p, z, 1);
     const double *B = p.B + p.sB * opi(p, z, 2);
-    const int kt0 = (int)((int64_t)p.ktiles * s / p.S), kt1 = (int)((int64_t)p.ktiles * (s + 1) / p.S);
+    const int kt0 = (int)((int64_t)p.ktiles * s / S), kt1 = (int)((int64_t)p.ktiles * (s + 1) / S);
     typedef typename std::conditional<BF16, unsigned short, float>::type st_t;
     constexpr int PAD = BF16 ? 8 : 1;
     __shared__ __align__(16) st_t As[BM][GK + PAD];
@@ -328,8 +342,8 @@ __global__ __launch_bounds__(256, 2) void lpgemm_kernel(GemmArgs p)
                 const int64_t gj = col0 + wn * 32 + b * 16 + (lane & 15);
                 if (gi < m && gj < n) {
                     const double val = (double)acc[a][b][e];
-                    if (p.S == 1) store_out(p, zc, gi, gj, val);
-                    else p.part[gi + m * gj + m * n * (s + (int64_t)p.S * z)] = val;
+                    if (S == 1) store_out(p, zc, gi, gj, val);
+                    else p.part[gi + m * gj + m * n * (s + (int64_t)S * z)] = val;
                 }
             }
 }
@@ -344,7 +358,7 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
     const GemmShape &g = SHAPES[c];
     p.mt = (int)((p.m + g.bm - 1) / g.bm);
     p.nt = (int)((p.n + g.bn - 1) / g.bn);
-    const int64_t units = (int64_t)p.mt * p.nt * p.S * p.n_inst;
+    const int64_t units = (int64_t)p.mt * p.nt * p.units_per_tile;
     const int64_t grid = (units + NXCD - 1) / NXCD * NXCD;
     const dim3 blk(64 * g.wmw * g.wnw);
     switch (c) {
@@ -375,21 +389,35 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     // Split k to fill the machine, by the 64x64 tile count whatever shape runs (so the split,
     // and with it every sum, does not depend on the tuned shape): 256 CUs hold 4 workgroups
     // each, so a launch runs in rounds of 1024 units; pick the split with the least rounds x
-    // k-tiles per unit (+2 for the prologue and epilogue), keeping >= 4 k-tiles per split.
+    // k-tiles per unit (+2 for the prologue and epilogue), keeping >= 4 k-tiles per split.  With
+    // gates the split follows the number of active instances (a table the kernels look up), so
+    // the last unconverged temperature of a batch still fills the chip.
     const int64_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
     constexpr int64_t SLOTS = 256 * 4;
-    int S = 1;
-    double best = 0;
-    for (int c = 1; c <= 16 && (c == 1 || p.ktiles / c >= 4); c++) {
-        const int64_t rounds = (tiles * c * bt.n_inst + SLOTS - 1) / SLOTS;
-        const double cost = (double)rounds * ((double)p.ktiles / c + 2.0);
-        if (c == 1 || cost < best) { best = cost; S = c; }
+    auto choose = [&](int64_t inst) {
+        int best_s = 1;
+        double best = 0;
+        for (int c = 1; c <= 16 && (c == 1 || p.ktiles / c >= 4); c++) {
+            const int64_t rounds = (tiles * c * inst + SLOTS - 1) / SLOTS;
+            const double cost = (double)rounds * ((double)p.ktiles / c + 2.0);
+            if (c == 1 || cost < best) { best = cost; best_s = c; }
+        }
+        return best_s;
+    };
+    const int S = choose(bt.n_inst);
+    int Smax = S;
+    p.S_of[0] = S;
+    p.units_per_tile = (int64_t)S * bt.n_inst;
+    for (int c = 1; c <= NSPLIT_TAB; c++) {
+        p.S_of[c] = (bt.gate && c <= bt.n_inst) ? choose(c) : S;
+        Smax = p.S_of[c] > Smax ? p.S_of[c] : Smax;
+        if (c <= bt.n_inst && (int64_t)p.S_of[c] * c > p.units_per_tile) p.units_per_tile = (int64_t)p.S_of[c] * c;
     }
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     EPSX_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;
-    if (S > 1) {
-        const size_t need = 8 * (size_t)m * n * S * bt.n_inst;
+    if (Smax > 1) {
+        const size_t need = 8 * (size_t)m * n * Smax * bt.n_inst;
         if (ctx->gemm_ws.bytes < need) {
             if (capturing) return set_error(ctx, EPSX_ERR_STATE, "GEMM workspace must be sized before graph capture");
             EPSX_TRY(dev_alloc(ctx, ctx->gemm_ws, need));
@@ -401,13 +429,13 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     if (bt.precision == EPSX_PREC_FP32 || bt.precision == EPSX_PREC_BF16) {
         p.mt = (int)((m + 63) / 64);
         p.nt = (int)((n + 63) / 64);
-        const int64_t grid = ((int64_t)p.mt * p.nt * S * bt.n_inst + NXCD - 1) / NXCD * NXCD;
+        const int64_t grid = ((int64_t)p.mt * p.nt * p.units_per_tile + NXCD - 1) / NXCD * NXCD;
         if (bt.precision == EPSX_PREC_BF16)
             hipLaunchKernelGGL(lpgemm_kernel<true>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
         else
             hipLaunchKernelGGL(lpgemm_kernel<false>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
         EPSX_HIP(ctx, hipGetLastError());
-        if (S > 1) {
+        if (Smax > 1) {
             hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
                                dim3(256), 0, ctx->stream, p);
             EPSX_HIP(ctx, hipGetLastError());
@@ -445,8 +473,7 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
         ctx->gemm_tune[key] = c;
     }
     EPSX_TRY(launch_shape(ctx, c, p));
-    if (S > 1) {
-        p.mt = p.nt = 0;
+    if (Smax > 1) {
         hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
                            dim3(256), 0, ctx->stream, p);
         EPSX_HIP(ctx, hipGetLastError());
