@@ -72,7 +72,9 @@ __device__ inline bool unit_of_block(const GemmArgs &p, int &tm, int &tn, int &s
     S = split_for(p, nact);
     const int64_t U = (int64_t)p.mt * p.nt * S * nact;
     const int64_t per_xcd = (U + NXCD - 1) / NXCD;
-    const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + blockIdx.x / NXCD;
+    const int64_t slot = blockIdx.x / NXCD;   // the grid may hold more workgroups than units
+    if (slot >= per_xcd) return false;
+    const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + slot;
     if (v >= U) return false;
     int64_t r = v;
     tm = (int)(r % p.mt); r /= p.mt;
