@@ -455,6 +455,8 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
         c = atoi(force) % NSHAPES;
     } else if (it != ctx->gemm_tune.end()) {
         c = it->second;
+    } else if (!bt.tune) {
+        c = 1;   // 128x64, the usual winner, until a tunable product of this size has run
     } else if (!accumulate && !capturing) {
         hipEvent_t e0, e1;
         EPSX_HIP(ctx, hipEventCreate(&e0));

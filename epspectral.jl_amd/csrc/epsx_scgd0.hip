@@ -357,9 +357,22 @@ __global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t 
     if (threadIdx.x == 0) bmax[blockIdx.x + gridDim.x * z] = red[0];
 }
 
+// The product gates of a mixed-precision run: lpg[t] = active and in the low-precision phase,
+// hpg[t] = active and in the FP64 phase (act null: every temperature active).
+__global__ void prec_gates_kernel(int64_t nT, const int32_t *__restrict__ act, const int32_t *__restrict__ hp,
+                                  int32_t *__restrict__ lpg, int32_t *__restrict__ hpg)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nT) return;
+    const int a = act ? act[t] != 0 : 1;
+    lpg[t] = a && !hp[t];
+    hpg[t] = a && hp[t];
+}
+
 // Device-side bookkeeping of the graph-replayed loop: the history row of this iteration, the
 // freeze of a converged temperature (src/scgd0.jl:326-333).
 struct GraphBook {
+    const int32_t *hp = nullptr;   // [nT] 1 = FP64 phase (may freeze); null: every temperature
     int32_t *act;            // [nT] convergence mask
     int64_t *frozen;         // [nT] iteration at which T froze
     const int32_t *it;       // 0-based iteration counter
@@ -395,7 +408,7 @@ __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__r
             g.hist_mu[t + g.nT * it] = g.mu[t];
             g.hist_minA[t + g.nT * it] = g.minA[t];
         }
-        if (m < g.tol) {
+        if (m < g.tol && (!g.hp || g.hp[t])) {
             g.act[t] = 0;
             g.frozen[t] = it + 1;
         }
@@ -710,9 +723,33 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // One iteration of src/scgd0.jl:252-333 for the nA temperature slots in d_tidx.  With a
     // device mask (graph mode) every kernel skips converged temperatures, and the convergence
     // test writes the history and freezes temperatures on the device.
+    // With a low-precision mode every temperature carries its own phase (hp): the products run
+    // twice per iteration, gated by lpg (low precision) and hpg (FP64).
+    const bool split = par->precision != EPSX_PREC_FP64;
+    std::vector<int32_t> hp(nT, split ? 0 : 1);
+    Tmp d_hp, d_lpg, d_hpg;
+    if (split) {
+        EPSX_TRY(tmp_upload(ctx, d_hp, hp.data(), 4 * nT));
+        EPSX_TRY(tmp_alloc(ctx, d_lpg.b, 4 * (size_t)nT));
+        EPSX_TRY(tmp_alloc(ctx, d_hpg.b, 4 * (size_t)nT));
+    }
     auto enqueue = [&](int nA, const int32_t *d_tidx, const std::vector<int32_t> &h_tidx, const int32_t *d_act,
-                       const GraphBook &book, bool events, int gp) -> int {
+                       const GraphBook &book, bool events) -> int {
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[0], st));
+        if (split) {
+            hipLaunchKernelGGL(prec_gates_kernel, dim3(nblk(nT)), dim3(256), 0, st, nT, d_act, d_hp.ptr<int32_t>(),
+                               d_lpg.ptr<int32_t>(), d_hpg.ptr<int32_t>());
+            EPSX_HIP(ctx, hipGetLastError());
+        }
+        // the (gate, precision) passes of the products: one FP64 pass, or the two phases
+        struct Pass { const int32_t *gate; int prec; bool tune; };
+        Pass passes[2] = {{d_act, EPSX_PREC_FP64, true}, {nullptr, 0, false}};
+        int npass = 1;
+        if (split) {
+            passes[0] = {d_lpg.ptr<int32_t>(), par->precision, false};
+            passes[1] = {d_hpg.ptr<int32_t>(), EPSX_PREC_FP64, false};
+            npass = 2;
+        }
         FmTemps tp;
         tp.T = B.T.ptr<double>(); tp.mu = B.mu.ptr<double>(); tp.eta_min = B.eta.ptr<double>();
         tp.act = d_act;
@@ -724,46 +761,57 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                                     hocc.ptr<double>(), hbuf.ptr<double>(), d_act));
             if (events) EPSX_HIP(ctx, hipEventRecord(ev[5], st));
             // ImΣ[:, :, t] = h_z G̃ for every slot z in one batched launch (G̃ shared)
-            GemmBatch bt;
-            bt.n_inst = nA;
-            bt.sA = nw * s.nm * ni;
-            bt.sC = nw * ni;
-            bt.idx = d_tidx;
-            bt.imap = 4;
-            bt.gate = d_act;
-            bt.precision = gp;
-            EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
-                           B.ImS.ptr<double>(), nw, false, bt));
-        } else if (par->batched) {
-            tp.nT = nA;
-            tp.tidx = d_tidx;
-            EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile, gp,
-                                         B.part.ptr<double>(), (int64_t)npart, B.ImS.ptr<double>(), false));
-        } else {
-            for (int z = 0; z < nA; z++) {
-                tp.nT = 1;
-                tp.tidx = d_tidx + z;
-                EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile,
-                                             gp, B.part.ptr<double>(), (int64_t)npart,
-                                             B.ImS.ptr<double>(), false));
+            for (int q = 0; q < npass; q++) {
+                GemmBatch bt;
+                bt.n_inst = nA;
+                bt.sA = nw * s.nm * ni;
+                bt.sC = nw * ni;
+                bt.idx = d_tidx;
+                bt.imap = 4;
+                bt.gate = passes[q].gate;
+                bt.precision = passes[q].prec;
+                bt.tune = passes[q].tune;
+                EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
+                               B.ImS.ptr<double>(), nw, false, bt));
             }
+        } else {
+            for (int q = 0; q < npass; q++) {
+                tp.act = passes[q].gate;
+                if (par->batched) {
+                    tp.nT = nA;
+                    tp.tidx = d_tidx;
+                    EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile,
+                                                 passes[q].prec, B.part.ptr<double>(), (int64_t)npart,
+                                                 B.ImS.ptr<double>(), false));
+                } else {
+                    for (int z = 0; z < nA; z++) {
+                        tp.nT = 1;
+                        tp.tidx = d_tidx + z;
+                        EPSX_TRY(fm_direct_launch_sp(ctx, g, fa, tp, dptr<int64_t>(s.star_ptr), par->f_tile,
+                                                     passes[q].prec, B.part.ptr<double>(), (int64_t)npart,
+                                                     B.ImS.ptr<double>(), false));
+                    }
+                }
+            }
+            tp.act = d_act;
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[1], st));
         // 4. Re Σ by Kramers-Kronig, Σ_new = complex(KK ImΣ, ImΣ) in the GEMM's epilogue
-        {
+        for (int q = 0; q < npass; q++) {
             GemmBatch bt;
             bt.n_inst = nA;
             bt.sB = bt.sC = nw * ni;
             bt.idx = d_tidx;
             bt.imap = 2 | 4;
-            bt.gate = d_act;
+            bt.gate = passes[q].gate;
+            bt.tune = passes[q].tune;
             bt.Z = B.Snew.ptr<double2>();
             bt.Zim = B.ImS.ptr<double>();
-            if (gp == EPSX_PREC_FP64) {
+            if (passes[q].prec == EPSX_PREC_FP64) {
                 EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
             } else {
                 // the off-diagonal part in low precision, the principal-value diagonal in FP64
-                bt.precision = gp;
+                bt.precision = passes[q].prec;
                 bt.diag = dptr<double>(s.KKd);
                 EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KKo), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
             }
@@ -849,19 +897,27 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     std::vector<int32_t> h_status(nT);
     std::vector<double> mu_host(par->mu_start, par->mu_start + nT);
     int rc = EPSX_OK;
-    // Precision schedule: with FP32 / BF16 the Fan-Migdal products (and the off-diagonal of KK)
-    // run in low precision and no temperature may freeze, until every active temperature is
-    // within 2 tol or has stopped contracting (err > 0.98 err_prev, the low-precision floor);
-    // then FP64 iterations run to tol.
-    bool lp = par->precision != EPSX_PREC_FP64;
-    auto lp_done = [&](int64_t iter) {
+    // Precision schedule (with FP32 / BF16): a temperature runs its Fan-Migdal products (and the
+    // off-diagonal of KK) in low precision, and may not freeze, until it is within 2 tol or stops
+    // contracting (err > 0.98 err_prev, the low-precision floor); from then on it runs in FP64 to tol.
+    std::vector<int64_t> lp_iters(nT, 0);
+    auto update_phases = [&]() -> int {
+        bool changed = false, any_lp = false;
         for (int64_t t = 0; t < nT; t++) {
-            if (!active[t]) continue;
+            if (hp[t]) continue;
+            any_lp = true;
+            lp_iters[t]++;
             const bool near = h_err[t] <= 2 * par->tol;
-            const bool floor_reached = iter >= 3 && h_err[t] > 0.98 * prev_err[t];
-            if (!(near || floor_reached || !(h_err[t] == h_err[t]))) return false;
+            const bool floor_reached = lp_iters[t] >= 3 && h_err[t] > 0.98 * prev_err[t];
+            if (near || floor_reached || !(h_err[t] == h_err[t])) {
+                hp[t] = 1;
+                changed = true;
+            }
+            prev_err[t] = h_err[t];
         }
-        return true;
+        if (any_lp) out->n_low_precision++;
+        if (changed) EPSX_HIP(ctx, hipMemcpyAsync(d_hp.b.p, hp.data(), 4 * nT, hipMemcpyHostToDevice, st));
+        return EPSX_OK;
     };
     if (!par->batched) {
         // The reference's loop: the active temperatures one after another, decisions on the host.
@@ -872,8 +928,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             for (int64_t t = 0; t < nT; t++) if (active[t]) act_idx.push_back((int32_t)t);
             const int nA = (int)act_idx.size();
             EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, act_idx.data(), 4 * nA, hipMemcpyHostToDevice, st));
-            EPSX_TRY(enqueue(nA, B.tidx.ptr<int32_t>(), act_idx, nullptr, none, true,
-                             lp ? par->precision : EPSX_PREC_FP64));
+            EPSX_TRY(enqueue(nA, B.tidx.ptr<int32_t>(), act_idx, nullptr, none, true));
             EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipMemcpyAsync(h_mu.data(), B.mu.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipMemcpyAsync(h_minA.data(), B.minA.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
@@ -887,7 +942,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 out->hist_minA[t + nT * (iter - 1)] = h_minA[t];
                 out->hist_err[t + nT * (iter - 1)] = h_err[t];
                 out->hist_mu[t + nT * (iter - 1)] = h_mu[t];
-                if (!lp && h_err[t] < par->tol) {
+                if (hp[t] && h_err[t] < par->tol) {
                     active[t] = 0;
                     out->iter_frozen[t] = iter;
                 }
@@ -895,11 +950,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             if (rc != EPSX_OK) break;
             out->wall[iter - 1] = now_s() - t0;
             out->niter = iter;
-            if (lp) {
-                out->n_low_precision = (int32_t)iter;
-                if (lp_done(iter)) lp = false;
-                prev_err = h_err;
-            }
+            if (split) EPSX_TRY(update_phases());
             if (par->verbosity > 0) {
                 fprintf(stderr, "scGD0 iteration %lld: wall %.3f s\n", (long long)iter, out->wall[iter - 1]);
                 for (int z = 0; z < nA; z++)
@@ -931,42 +982,31 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         book.hist_err = d_he.ptr<double>(); book.hist_mu = d_hm.ptr<double>(); book.hist_minA = d_ha.ptr<double>();
         book.mu = B.mu.ptr<double>(); book.minA = B.minA.ptr<double>();
         book.nT = nT; book.maxiter = maxiter; book.tol = par->tol;
-        GraphBook book_lp = book;
-        book_lp.tol = -1.0;   // nothing freezes in the low-precision phase
-        hipGraph_t graph[2] = {nullptr, nullptr};
-        hipGraphExec_t exec[2] = {nullptr, nullptr};
+        if (split) book.hp = d_hp.ptr<int32_t>();
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
         struct GraphGuard {
             hipGraph_t *g; hipGraphExec_t *e;
-            ~GraphGuard() {
-                for (int q = 0; q < 2; q++) {
-                    if (e[q]) (void)hipGraphExecDestroy(e[q]);
-                    if (g[q]) (void)hipGraphDestroy(g[q]);
-                }
-            }
-        } gg{graph, exec};
-        bool eager = true;
+            ~GraphGuard() { if (*e) (void)hipGraphExecDestroy(*e); if (*g) (void)hipGraphDestroy(*g); }
+        } gg{&graph, &exec};
         for (int64_t iter = 1; iter <= maxiter; iter++) {
             const double t0 = now_s();
-            const int ph = lp ? 1 : 0;
-            const GraphBook &bk = lp ? book_lp : book;
-            const int gp = lp ? par->precision : EPSX_PREC_FP64;
-            if (eager) {
-                EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, bk, iter == 1, gp));
-                eager = false;
+            if (iter == 1) {
+                EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, true));
             } else {
-                if (!exec[ph]) {
+                if (!exec) {
                     EPSX_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                    int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, bk, false, gp);
-                    hipError_t ec = hipStreamEndCapture(st, &graph[ph]);
+                    int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, false);
+                    hipError_t ec = hipStreamEndCapture(st, &graph);
                     if (r != EPSX_OK) return r;
                     if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture", __FILE__, __LINE__);
-                    EPSX_HIP(ctx, hipGraphInstantiate(&exec[ph], graph[ph], nullptr, nullptr, 0));
+                    EPSX_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
                 }
-                EPSX_HIP(ctx, hipGraphLaunch(exec[ph], st));
+                EPSX_HIP(ctx, hipGraphLaunch(exec, st));
             }
             EPSX_HIP(ctx, hipMemcpyAsync(active.data(), book.act, 4 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipMemcpyAsync(h_status.data(), B.status.b.p, 4 * nT, hipMemcpyDeviceToHost, st));
-            if (lp) EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+            if (split) EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipStreamSynchronize(st));
             if (iter == 1) EPSX_TRY(add_event_times((int)nT));
             for (int64_t t = 0; t < nT; t++)
@@ -974,13 +1014,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             if (rc != EPSX_OK) break;
             out->wall[iter - 1] = now_s() - t0;
             out->niter = iter;
-            if (lp) {
-                out->n_low_precision = (int32_t)iter;
-                if (lp_done(iter)) {
-                    lp = false;
-                    eager = true;
-                }
-                prev_err = h_err;
+            if (split) {
+                // frozen temperatures are not in a phase any more
+                for (int64_t t = 0; t < nT; t++)
+                    if (!active[t]) hp[t] = 1;
+                EPSX_TRY(update_phases());
             }
             bool any = false;
             for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
