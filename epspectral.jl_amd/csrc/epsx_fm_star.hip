@@ -24,31 +24,36 @@ constexpr int SR_THREADS = 256;
 constexpr int SR_CAP = 4096;   // doubles of g2 staged per block (whole stars)
 
 constexpr int SR_PER = SR_CAP / SR_THREADS;   // staged doubles per thread
+constexpr int SR_CPB = 8;                     // chunks per block, streamed through two LDS stages
 
-// One block per (chunk of whole stars, outer state i).  The chunk's slab of g2p[:, p, i]
-// (contiguous) is loaded with all of a thread's loads in flight at once (a fixed, unrolled
-// count) and staged in LDS times w_f, then one thread per (ν, i') sums its star in ascending p
-// from LDS.  A star too large for the stage is read from global memory.
+// One block per (run of SR_CPB chunks of whole stars, outer state i), streaming the chunks of
+// g2p[:, p, i] (contiguous) through two LDS stages: the next chunk's loads (all of a thread's in
+// flight at once, a fixed unrolled count, times w_f) are issued before the current chunk is
+// reduced, so HBM stays busy while one thread per (ν, i') sums its star in ascending p from LDS.
+// A star too large for a stage is read from global memory.
 __global__ __launch_bounds__(SR_THREADS) void star_reduce_kernel(
-    int64_t nm, int64_t ni, int64_t nfs, const double *__restrict__ g2p, const double *__restrict__ wf,
+    int64_t nm, int64_t ni, int64_t nfs, int64_t nchunks, const double *__restrict__ g2p, const double *__restrict__ wf,
     const int64_t *__restrict__ star_ptr, const int32_t *__restrict__ chunk_star, const uint8_t *__restrict__ skip_mode,
     double *__restrict__ Gt)
 {
-    __shared__ double buf[SR_CAP];
+    __shared__ double buf[2][SR_CAP];
     const int tid = threadIdx.x;
-    const int64_t c = blockIdx.x, i = blockIdx.y;
-    const int64_t s0 = chunk_star[c], s1 = chunk_star[c + 1];
-    const int64_t p0 = star_ptr[s0], p1 = star_ptr[s1];
-    const int n = (int)((p1 - p0) * nm);
+    const int64_t i = blockIdx.y;
+    const int64_t c0 = (int64_t)blockIdx.x * SR_CPB;
+    const int64_t c1 = c0 + SR_CPB < nchunks ? c0 + SR_CPB : nchunks;
     const int nm32 = (int)nm;
-    const double *src = g2p + nm * (p0 + nfs * i);
-    const bool staged = (p1 - p0) * nm <= SR_CAP;
-    if (staged) {
-        double v[SR_PER], w[SR_PER];
-        int nus[SR_PER];
-        // element e = tid + 256 q sits at mode e % nm of member e / nm, advanced without division
-        const int step_p = SR_THREADS / nm32, step_nu = SR_THREADS % nm32;
+    const double *slab = g2p + nm * nfs * i;
+    const int step_p = SR_THREADS / nm32, step_nu = SR_THREADS % nm32;
+    double v[SR_PER];
+    int nus[SR_PER];
+    // loads of chunk c into registers (times w_f, acoustic modes zeroed), if it fits a stage
+    auto fetch = [&](int64_t c) {
+        const int64_t p0 = star_ptr[chunk_star[c]], p1 = star_ptr[chunk_star[c + 1]];
+        const int n = (int)((p1 - p0) * nm);
+        if (n > SR_CAP) return;
+        const double *src = slab + nm * p0;
         int nu = tid % nm32, pl = tid / nm32;
+        double w[SR_PER];
 #pragma unroll
         for (int q = 0; q < SR_PER; q++) {
             const int e = tid + SR_THREADS * q;
@@ -60,27 +65,48 @@ __global__ __launch_bounds__(SR_THREADS) void star_reduce_kernel(
             if (nu >= nm32) { nu -= nm32; pl++; }
         }
 #pragma unroll
+        for (int q = 0; q < SR_PER; q++) v[q] = skip_mode[nus[q]] ? 0.0 : w[q] * v[q];
+    };
+    auto stage = [&](int64_t c, double *dst) {
+        const int64_t p0 = star_ptr[chunk_star[c]], p1 = star_ptr[chunk_star[c + 1]];
+        const int n = (int)((p1 - p0) * nm);
+        if (n > SR_CAP) return;
+#pragma unroll
         for (int q = 0; q < SR_PER; q++) {
             const int e = tid + SR_THREADS * q;
-            if (e < n) buf[e] = skip_mode[nus[q]] ? 0.0 : w[q] * v[q];
+            if (e < n) dst[e] = v[q];
         }
-        __syncthreads();
+    };
+    if (c0 < c1) {
+        fetch(c0);
+        stage(c0, buf[0]);
     }
-    const int nout = (int)((s1 - s0) * nm);
-    for (int o = tid; o < nout; o += SR_THREADS) {
-        const int nu = o % nm32;
-        const int64_t ip = s0 + o / nm32;
-        double acc = 0.0;
-        if (!skip_mode[nu]) {
-            const int a = (int)(star_ptr[ip] - p0), b = (int)(star_ptr[ip + 1] - p0);
-            if (staged) {
+    __syncthreads();
+    for (int64_t c = c0; c < c1; c++) {
+        const int cur = (int)((c - c0) & 1);
+        if (c + 1 < c1) fetch(c + 1);   // in flight during the reduction below
+        const int64_t s0 = chunk_star[c], s1 = chunk_star[c + 1];
+        const int64_t p0 = star_ptr[s0], p1 = star_ptr[s1];
+        const bool staged = (p1 - p0) * nm <= SR_CAP;
+        const double *src = slab + nm * p0;
+        const int nout = (int)((s1 - s0) * nm);
+        for (int o = tid; o < nout; o += SR_THREADS) {
+            const int nu = o % nm32;
+            const int64_t ip = s0 + o / nm32;
+            double acc = 0.0;
+            if (!skip_mode[nu]) {
+                const int a = (int)(star_ptr[ip] - p0), b = (int)(star_ptr[ip + 1] - p0);
+                if (staged) {
 #pragma unroll 4
-                for (int pl = a; pl < b; pl++) acc += buf[pl * nm32 + nu];
-            } else {
-                for (int pl = a; pl < b; pl++) acc += wf[p0 + pl] * src[(int64_t)pl * nm + nu];
+                    for (int pl = a; pl < b; pl++) acc += buf[cur][pl * nm32 + nu];
+                } else {
+                    for (int pl = a; pl < b; pl++) acc += wf[p0 + pl] * src[(int64_t)pl * nm + nu];
+                }
             }
+            Gt[(nu + nm * ip) + nm * ni * i] = acc;
         }
-        Gt[(nu + nm * ip) + nm * ni * i] = acc;
+        if (c + 1 < c1) stage(c + 1, buf[cur ^ 1]);
+        __syncthreads();
     }
 }
 
@@ -184,8 +210,8 @@ int star_reduce_launch(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nfs, const
                        const int64_t *d_star_ptr, const int32_t *d_chunks, int64_t nchunks, const uint8_t *d_skip,
                        double *Gt)
 {
-    dim3 grid((unsigned)nchunks, (unsigned)ni);
-    hipLaunchKernelGGL(star_reduce_kernel, grid, dim3(SR_THREADS), 0, ctx->stream, nm, ni, nfs, g2p, wf,
+    dim3 grid((unsigned)((nchunks + SR_CPB - 1) / SR_CPB), (unsigned)ni);
+    hipLaunchKernelGGL(star_reduce_kernel, grid, dim3(SR_THREADS), 0, ctx->stream, nm, ni, nfs, nchunks, g2p, wf,
                        d_star_ptr, d_chunks, d_skip, Gt);
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
