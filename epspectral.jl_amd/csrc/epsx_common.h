@@ -55,6 +55,7 @@ struct Scgd0Dev {
     DevBuf chunks, skip, wnu;         // star chunks of the reduction, acoustic flags, ω_ν
     DevBuf hk, ht;                    // cells and weights of ω ± ω_ν (Einstein modes)
     int64_t nchunks = 0;
+    int sr_cap = 0;                   // stage size of the reduction (doubles; 0: none)
     int64_t ltab_len = 0;
 };
 
@@ -246,10 +247,11 @@ int fm_direct_launch_sp(epsx_ctx *ctx, const FmGrid &g, const FmDirectArgs &a, c
                         const int64_t *d_star_ptr, int64_t f_tile, int precision, double *d_part,
                         int64_t part_cap, double *d_ImS, bool accumulate);
 // factorised (star) Fan-Migdal method, epsx_fm_star.hip
-void star_chunks(int64_t nm, const std::vector<int64_t> &star_ptr, std::vector<int32_t> &chunk);
-int star_reduce_launch(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nfs, const double *g2p, const double *wf,
-                       const int64_t *d_star_ptr, const int32_t *d_chunks, int64_t nchunks, const uint8_t *d_skip,
-                       double *Gt);
+int star_stage_cap(int64_t nm, const std::vector<int64_t> &star_ptr);
+void star_chunks(int64_t nm, int cap, const std::vector<int64_t> &star_ptr, std::vector<int32_t> &chunk);
+int star_reduce_launch(epsx_ctx *ctx, int cap, int64_t nm, int64_t ni, int64_t nfs, const double *g2p,
+                       const double *wf, const int64_t *d_star_ptr, const int32_t *d_chunks, int64_t nchunks,
+                       const uint8_t *d_skip, double *Gt);
 void star_lookup_host(int64_t nw, int64_t nm, const double *wnu, double h, const std::vector<int32_t> &off,
                       const std::vector<int32_t> &ltab, const std::vector<double> &rgap, int64_t span,
                       std::vector<int32_t> &hk, std::vector<double> &ht);

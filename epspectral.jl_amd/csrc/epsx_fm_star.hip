@@ -12,7 +12,10 @@
 // kernel that reads the whole coupling), after which every iteration is an h build
 // (nω nm ni terms per T) and an FP64 MFMA GEMM of nω x (nm ni) x ni.  Modes below ω_acoustic
 // are zeroed in G̃ without reading g2 (which may hold Inf there).
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "epsx_common.h"
@@ -21,92 +24,70 @@ namespace epsx {
 namespace {
 
 constexpr int SR_THREADS = 256;
-constexpr int SR_CAP = 4096;   // doubles of g2 staged per block (whole stars)
 
-constexpr int SR_PER = SR_CAP / SR_THREADS;   // staged doubles per thread
-constexpr int SR_CPB = 8;                     // chunks per block, streamed through two LDS stages
-
-// One block per (run of SR_CPB chunks of whole stars, outer state i), streaming the chunks of
-// g2p[:, p, i] (contiguous) through two LDS stages: the next chunk's loads (all of a thread's in
-// flight at once, a fixed unrolled count, times w_f) are issued before the current chunk is
-// reduced, so HBM stays busy while one thread per (ν, i') sums its star in ascending p from LDS.
-// A star too large for a stage is read from global memory.
+// One block per (chunk of whole stars, outer state i).  The chunk's slab of g2p[:, p, i]
+// (contiguous, n ≤ cap - 2 doubles, cap ≤ CAP) is read with 16-byte non-temporal loads from the 16-byte
+// boundary at or below its start (sh = 0 or 1 leading doubles belong to the previous chunk), all of
+// a thread's loads unconditional and in flight at once (indices clamped to the last pair), and
+// staged in LDS together with w_f of its rows; then one thread per (ν, i') sums its star in
+// ascending p.  A star too large for the stage is a chunk of its own, read from global memory.
+// The last pair may read one double past the slab: the g2p allocation carries two doubles of
+// padding.
+template <int CAP>
 __global__ __launch_bounds__(SR_THREADS) void star_reduce_kernel(
-    int64_t nm, int64_t ni, int64_t nfs, int64_t nchunks, const double *__restrict__ g2p, const double *__restrict__ wf,
+    int cap, int64_t nm, int64_t ni, int64_t nfs, const double *__restrict__ g2p, const double *__restrict__ wf,
     const int64_t *__restrict__ star_ptr, const int32_t *__restrict__ chunk_star, const uint8_t *__restrict__ skip_mode,
     double *__restrict__ Gt)
 {
-    __shared__ double buf[2][SR_CAP];
+    constexpr int PER = CAP / (2 * SR_THREADS);   // 16-byte loads per thread
+    __shared__ double2 buf2[CAP / 2];
+    __shared__ double wl[CAP / 4];
+    const double *buf = reinterpret_cast<const double *>(buf2);
     const int tid = threadIdx.x;
-    const int64_t i = blockIdx.y;
-    const int64_t c0 = (int64_t)blockIdx.x * SR_CPB;
-    const int64_t c1 = c0 + SR_CPB < nchunks ? c0 + SR_CPB : nchunks;
+    const int64_t c = blockIdx.x, i = blockIdx.y;
+    const int64_t s0 = chunk_star[c], s1 = chunk_star[c + 1];
+    const int64_t p0 = star_ptr[s0], p1 = star_ptr[s1];
+    const int64_t rows = p1 - p0;
     const int nm32 = (int)nm;
-    const double *slab = g2p + nm * nfs * i;
-    const int step_p = SR_THREADS / nm32, step_nu = SR_THREADS % nm32;
-    double v[SR_PER];
-    int nus[SR_PER];
-    // loads of chunk c into registers (times w_f, acoustic modes zeroed), if it fits a stage
-    auto fetch = [&](int64_t c) {
-        const int64_t p0 = star_ptr[chunk_star[c]], p1 = star_ptr[chunk_star[c + 1]];
-        const int n = (int)((p1 - p0) * nm);
-        if (n > SR_CAP) return;
-        const double *src = slab + nm * p0;
-        int nu = tid % nm32, pl = tid / nm32;
-        double w[SR_PER];
+    const int64_t g0 = nm * (p0 + nfs * i);
+    const int sh = (int)(g0 & 1);
+    const bool staged = rows * nm <= cap - 2 && rows <= cap / 4;   // cap ≤ CAP (0: nothing staged)
+    if (staged) {
+        const int n = (int)(rows * nm);
+        const int np = (n + sh + 1) >> 1;
+        const double2 *src = reinterpret_cast<const double2 *>(g2p + (g0 - sh));
+        double2 v[PER];
 #pragma unroll
-        for (int q = 0; q < SR_PER; q++) {
-            const int e = tid + SR_THREADS * q;
-            v[q] = e < n ? src[e] : 0.0;
-            w[q] = e < n ? wf[p0 + pl] : 0.0;
-            nus[q] = nu;
-            nu += step_nu;
-            pl += step_p;
-            if (nu >= nm32) { nu -= nm32; pl++; }
+        for (int q = 0; q < PER; q++) {
+            const int j = tid + SR_THREADS * q;
+            const double2 *ptr = src + (j < np ? j : np - 1);
+            v[q].x = __builtin_nontemporal_load(&ptr->x);
+            v[q].y = __builtin_nontemporal_load(&ptr->y);
         }
+        for (int r = tid; r < (int)rows; r += SR_THREADS) wl[r] = wf[p0 + r];
 #pragma unroll
-        for (int q = 0; q < SR_PER; q++) v[q] = skip_mode[nus[q]] ? 0.0 : w[q] * v[q];
-    };
-    auto stage = [&](int64_t c, double *dst) {
-        const int64_t p0 = star_ptr[chunk_star[c]], p1 = star_ptr[chunk_star[c + 1]];
-        const int n = (int)((p1 - p0) * nm);
-        if (n > SR_CAP) return;
-#pragma unroll
-        for (int q = 0; q < SR_PER; q++) {
-            const int e = tid + SR_THREADS * q;
-            if (e < n) dst[e] = v[q];
+        for (int q = 0; q < PER; q++) {
+            const int j = tid + SR_THREADS * q;
+            if (j < np) buf2[j] = v[q];
         }
-    };
-    if (c0 < c1) {
-        fetch(c0);
-        stage(c0, buf[0]);
-    }
-    __syncthreads();
-    for (int64_t c = c0; c < c1; c++) {
-        const int cur = (int)((c - c0) & 1);
-        if (c + 1 < c1) fetch(c + 1);   // in flight during the reduction below
-        const int64_t s0 = chunk_star[c], s1 = chunk_star[c + 1];
-        const int64_t p0 = star_ptr[s0], p1 = star_ptr[s1];
-        const bool staged = (p1 - p0) * nm <= SR_CAP;
-        const double *src = slab + nm * p0;
-        const int nout = (int)((s1 - s0) * nm);
-        for (int o = tid; o < nout; o += SR_THREADS) {
-            const int nu = o % nm32;
-            const int64_t ip = s0 + o / nm32;
-            double acc = 0.0;
-            if (!skip_mode[nu]) {
-                const int a = (int)(star_ptr[ip] - p0), b = (int)(star_ptr[ip + 1] - p0);
-                if (staged) {
-#pragma unroll 4
-                    for (int pl = a; pl < b; pl++) acc += buf[cur][pl * nm32 + nu];
-                } else {
-                    for (int pl = a; pl < b; pl++) acc += wf[p0 + pl] * src[(int64_t)pl * nm + nu];
-                }
-            }
-            Gt[(nu + nm * ip) + nm * ni * i] = acc;
-        }
-        if (c + 1 < c1) stage(c + 1, buf[cur ^ 1]);
         __syncthreads();
+    }
+    const double *src = g2p + g0;
+    const int nout = (int)((s1 - s0) * nm);
+    for (int o = tid; o < nout; o += SR_THREADS) {
+        const int nu = o % nm32;
+        const int64_t ip = s0 + o / nm32;
+        double acc = 0.0;
+        if (!skip_mode[nu]) {
+            const int a = (int)(star_ptr[ip] - p0), b = (int)(star_ptr[ip + 1] - p0);
+            if (staged) {
+#pragma unroll 4
+                for (int pl = a; pl < b; pl++) acc += wl[pl] * buf[pl * nm32 + nu + sh];
+            } else {
+                for (int pl = a; pl < b; pl++) acc += wf[p0 + pl] * src[(int64_t)pl * nm + nu];
+            }
+        }
+        Gt[(nu + nm * ip) + nm * ni * i] = acc;
     }
 }
 
@@ -190,9 +171,23 @@ __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
 
 }  // namespace
 
-// Chunks of whole stars with at most SR_CAP staged doubles each (a larger single star is a chunk
-// of its own, read from global memory).
-void star_chunks(int64_t nm, const std::vector<int64_t> &star_ptr, std::vector<int32_t> &chunk)
+// The stage size of the reduction: the smallest of 1024, 2048, 4096 doubles that holds the largest
+// star (rows * nm ≤ cap - 2, rows ≤ cap / 4), else 4096 (larger stars are read from global memory).
+// EPSX_STAR_STAGE=global stages nothing (every star read from global memory; for tests).
+int star_stage_cap(int64_t nm, const std::vector<int64_t> &star_ptr)
+{
+    const char *force = getenv("EPSX_STAR_STAGE");
+    if (force && std::string(force) == "global") return 0;
+    int64_t rmax = 0;
+    for (size_t s = 0; s + 1 < star_ptr.size(); s++) rmax = std::max(rmax, star_ptr[s + 1] - star_ptr[s]);
+    for (int cap : {1024, 2048, 4096})
+        if (rmax * nm <= cap - 2 && rmax <= cap / 4) return cap;
+    return 4096;
+}
+
+// Chunks of whole stars that fit a stage of cap doubles (a larger single star is a chunk of its
+// own, read from global memory).
+void star_chunks(int64_t nm, int cap, const std::vector<int64_t> &star_ptr, std::vector<int32_t> &chunk)
 {
     const int64_t ns = (int64_t)star_ptr.size() - 1;
     chunk.clear();
@@ -200,19 +195,32 @@ void star_chunks(int64_t nm, const std::vector<int64_t> &star_ptr, std::vector<i
     int64_t s = 0;
     while (s < ns) {
         int64_t e = s + 1;
-        while (e < ns && (star_ptr[e + 1] - star_ptr[s]) * nm <= SR_CAP) e++;
+        while (e < ns && (star_ptr[e + 1] - star_ptr[s]) * nm <= cap - 2 && star_ptr[e + 1] - star_ptr[s] <= cap / 4)
+            e++;
         chunk.push_back((int32_t)e);
         s = e;
     }
 }
 
-int star_reduce_launch(epsx_ctx *ctx, int64_t nm, int64_t ni, int64_t nfs, const double *g2p, const double *wf,
-                       const int64_t *d_star_ptr, const int32_t *d_chunks, int64_t nchunks, const uint8_t *d_skip,
-                       double *Gt)
+int star_reduce_launch(epsx_ctx *ctx, int cap, int64_t nm, int64_t ni, int64_t nfs, const double *g2p,
+                       const double *wf, const int64_t *d_star_ptr, const int32_t *d_chunks, int64_t nchunks,
+                       const uint8_t *d_skip, double *Gt)
 {
-    dim3 grid((unsigned)((nchunks + SR_CPB - 1) / SR_CPB), (unsigned)ni);
-    hipLaunchKernelGGL(star_reduce_kernel, grid, dim3(SR_THREADS), 0, ctx->stream, nm, ni, nfs, nchunks, g2p, wf,
-                       d_star_ptr, d_chunks, d_skip, Gt);
+    const dim3 grid((unsigned)nchunks, (unsigned)ni), block(SR_THREADS);
+    switch (cap) {
+    case 1024:
+        hipLaunchKernelGGL(star_reduce_kernel<1024>, grid, block, 0, ctx->stream, cap, nm, ni, nfs, g2p, wf, d_star_ptr,
+                           d_chunks, d_skip, Gt);
+        break;
+    case 2048:
+        hipLaunchKernelGGL(star_reduce_kernel<2048>, grid, block, 0, ctx->stream, cap, nm, ni, nfs, g2p, wf, d_star_ptr,
+                           d_chunks, d_skip, Gt);
+        break;
+    default:   // 4096, and 0 (nothing staged: every chunk exceeds the stage)
+        hipLaunchKernelGGL(star_reduce_kernel<4096>, grid, block, 0, ctx->stream, cap, nm, ni, nfs, g2p, wf, d_star_ptr,
+                           d_chunks, d_skip, Gt);
+        break;
+    }
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
 }

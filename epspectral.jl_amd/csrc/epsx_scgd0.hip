@@ -549,7 +549,8 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     }
     {
         std::vector<int32_t> ch;
-        star_chunks(nm, sptr, ch);
+        s.sr_cap = star_stage_cap(nm, sptr);
+        star_chunks(nm, s.sr_cap, sptr, ch);
         s.nchunks = (int64_t)ch.size() - 1;
         EPSX_TRY(upload(ctx, s.chunks, ch.data(), sizeof(int32_t) * ch.size()));
     }
@@ -570,7 +571,7 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
         Tmp raw;
         const size_t n = (size_t)nm * ni * nf;
         EPSX_TRY(tmp_upload(ctx, raw, P->g2, sizeof(double) * n));
-        EPSX_TRY(dev_alloc(ctx, s.g2p, sizeof(double) * n));
+        EPSX_TRY(dev_alloc(ctx, s.g2p, sizeof(double) * (n + 2)));   // padding read by the reduction
         hipLaunchKernelGGL(permute_g2_kernel, dim3(nblk((int64_t)n)), dim3(256), 0, ctx->stream, nm, ni,
                            nf, raw.ptr<double>(), dptr<int32_t>(s.fsrc_s), dptr<double>(s.g2p));
         EPSX_HIP(ctx, hipGetLastError());
@@ -658,7 +659,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             EPSX_HIP(ctx, hipEventCreate(&e0));
             EPSX_HIP(ctx, hipEventCreate(&e1));
             EPSX_HIP(ctx, hipEventRecord(e0, st));
-            EPSX_TRY(star_reduce_launch(ctx, s.nm, ni, nf, dptr<double>(s.g2p), dptr<double>(s.wf_s),
+            EPSX_TRY(star_reduce_launch(ctx, s.sr_cap, s.nm, ni, nf, dptr<double>(s.g2p), dptr<double>(s.wf_s),
                                         dptr<int64_t>(s.star_ptr), dptr<int32_t>(s.chunks), s.nchunks,
                                         dptr<uint8_t>(s.skip), dptr<double>(s.Gt)));
             EPSX_HIP(ctx, hipEventRecord(e1, st));

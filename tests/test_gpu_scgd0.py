@@ -5,6 +5,7 @@ glibc's by an ulp), 1e-10 relative for whole scGD0 runs in FP64 (the reference's
 test/test_scgd0_gpu.jl:36-37), bitwise for run-to-run repeats and for integer tables.
 """
 import math
+import os
 import warnings
 
 import numpy as np
@@ -274,6 +275,15 @@ def test_star_method_matches_direct_and_oracle(ctx):
     assert rel(rs["Σ"], ref["Σ"]) <= 1e-10
     assert np.all(np.abs(rs["μ"] - ref["μ"]) <= 1e-10 * np.abs(ref["μ"]))
     assert np.array_equal(ra["Σ"], rs["Σ"])
+    # every star read from global memory (the path of stars too large for the LDS stage)
+    os.environ["EPSX_STAR_STAGE"] = "global"
+    try:
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            rg = epsx.Scgd0Device(s, ctx).loop(o, method="star", **kw)
+    finally:
+        del os.environ["EPSX_STAR_STAGE"]
+    assert rel(rg["Σ"], rs["Σ"]) <= 1e-14
     # the precondition is checked: a dispersive mode refuses the star method
     import dataclasses
     s2 = dataclasses.replace(s, ωph=np.asfortranarray(np.hstack([s.ωph, s.ωph * 1.01])),
