@@ -18,4 +18,14 @@ __device__ inline void split_exp(double y, double &M, int &k)
     k = (int)yk;
 }
 
+// FP64 -> BF16 (as FP32 first, then round to nearest even on the upper 16 bits; NaN stays NaN).
+__device__ inline unsigned short to_bf16(double x)
+{
+    const float f = (float)x;
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+
 }  // namespace epsx

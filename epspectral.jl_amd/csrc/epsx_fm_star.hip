@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "epsx_common.h"
+#include "epsx_device.h"
 
 namespace epsx {
 namespace {
@@ -140,19 +141,11 @@ __global__ __launch_bounds__(256) void h_occ_kernel(HArgs a)
 
 // h[ω, ν, i', slot] = Im G_{i'}(ω+ω_ν) occ_a + Im G_{i'}(ω-ω_ν) occ_e: one thread per (ω, ν, i').
 // Σin is the clamped Σ, read at ω ± ω_ν through the precomputed cells of the integer grid.
-__global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
+// h[ω_w, ν, i'] of slot z (temperature index tcol) for a mode at or above ω_acoustic.
+__device__ inline double h_elem(const HArgs &a, int64_t w, int64_t nu, int64_t ip, int tcol, int z)
 {
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int z = blockIdx.y;
-    const int64_t nw = a.nw, total = nw * a.nm * a.ni;
-    if (q >= total) return;
-    const int tcol = a.tidx[z];
-    if (a.act && !a.act[tcol]) return;
-    const int64_t wn = q % (nw * a.nm), ip = q / (nw * a.nm);
-    const int64_t w = wn % nw, nu = wn / nw;
-    double *out = a.hout + total * (int64_t)z;
+    const int64_t nw = a.nw, wn = w + nw * nu;
     const double wnu = a.wnu[nu];
-    if (wnu < a.w_acoustic) { out[q] = 0.0; return; }
     const double eta = a.eta ? a.eta[tcol] : -INFINITY;
     const double2 *col = reinterpret_cast<const double2 *>(a.S) + nw * a.ni * (int64_t)tcol + nw * ip;
     const int ka = a.hk[2 * wn], ke = a.hk[2 * wn + 1];
@@ -166,7 +159,65 @@ __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
     const double zar = x0 + wnu - ei - sar, zai = -sai;
     const double zer = x0 - wnu - ei - ser, zei = -sei;
     const double2 o = reinterpret_cast<const double2 *>(a.occ)[wn + nw * a.nm * (int64_t)z];
-    out[q] = -zai / (zar * zar + zai * zai) * o.x - zei / (zer * zer + zei * zei) * o.y;
+    return -zai / (zar * zar + zai * zai) * o.x - zei / (zer * zer + zei * zei) * o.y;
+}
+
+// h[w + nw (ν + nm i')] per slot, FP64.
+__global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    const int64_t nw = a.nw, total = nw * a.nm * a.ni;
+    if (q >= total) return;
+    const int tcol = a.tidx[z];
+    if (a.act && !a.act[tcol]) return;
+    const int64_t wn = q % (nw * a.nm), ip = q / (nw * a.nm);
+    const int64_t w = wn % nw, nu = wn / nw;
+    double *out = a.hout + total * (int64_t)z;
+    out[q] = a.wnu[nu] < a.w_acoustic ? 0.0 : h_elem(a, w, nu, ip, tcol, z);
+}
+
+// h per slot in BF16, transposed for the BF16 product: hb[k + kp w] with k = ν + nm i' (zero for
+// nm ni <= k < kp, kp a multiple of 8).  One thread per (ω, run of 8 k): lanes along ω (so the Σ
+// reads stay coalesced), the 8 waves of a block along k, so a block writes whole 128-byte rows.
+__global__ __launch_bounds__(512) void h_build_bf16_kernel(HArgs a, uint16_t *hb, int64_t kp)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t w = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t k0 = ((int64_t)blockIdx.y * 8 + wave) * 8;
+    const int z = blockIdx.z;
+    if (w >= a.nw || k0 >= kp) return;
+    const int tcol = a.tidx[z];
+    if (a.act && !a.act[tcol]) return;
+    const int64_t K = a.nm * a.ni;
+    unsigned v[4];
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+        unsigned short b2[2];
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const int64_t k = k0 + e + d;
+            double h = 0.0;
+            if (k < K) {
+                const int64_t nu = k % a.nm, ip = k / a.nm;
+                if (a.wnu[nu] >= a.w_acoustic) h = h_elem(a, w, nu, ip, tcol, z);
+            }
+            b2[d] = to_bf16(h);
+        }
+        v[e / 2] = (unsigned)b2[0] | ((unsigned)b2[1] << 16);
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(hb + kp * a.nw * (int64_t)z + k0 + kp * w);
+    *dst = make_uint4(v[0], v[1], v[2], v[3]);
+}
+
+// G̃ in BF16 with the row pitch of the BF16 product: gb[k + kp i] = G̃[k + K i] (zero for K <= k < kp).
+__global__ __launch_bounds__(256) void gt_bf16_kernel(int64_t K, int64_t kp, int64_t ni, const double *__restrict__ Gt,
+                                                      uint16_t *__restrict__ gb)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= kp * ni) return;
+    const int64_t k = q % kp, i = q / kp;
+    gb[q] = k < K ? to_bf16(Gt[k + K * i]) : (uint16_t)0;
 }
 
 }  // namespace
@@ -257,7 +308,8 @@ void star_lookup_host(int64_t nw, int64_t nm, const double *wnu, double h, const
 int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const double *ws, const int32_t *hk,
                    const double *ht, const double *wnu, const double *e_i, const double *S, const double *T,
                    const double *mu, const double *eta, const int32_t *tidx, int nslots, double w_acoustic,
-                   double *occ, double *hout, const int32_t *act)
+                   double *occ, double *hout, const int32_t *act, const int32_t *act_h, uint16_t *hb, int64_t kp,
+                   const int32_t *act_b)
 {
     HArgs a;
     a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht;
@@ -266,9 +318,28 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
     hipLaunchKernelGGL(h_occ_kernel, dim3((unsigned)((nw * nm + 255) / 256), (unsigned)nslots), dim3(256), 0,
                        ctx->stream, a);
     EPSX_HIP(ctx, hipGetLastError());
-    const int64_t total = nw * nm * ni;
-    hipLaunchKernelGGL(h_build_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)nslots), dim3(256), 0,
-                       ctx->stream, a);
+    if (hout) {
+        HArgs ah = a;
+        ah.act = act_h;
+        const int64_t total = nw * nm * ni;
+        hipLaunchKernelGGL(h_build_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)nslots), dim3(256), 0,
+                           ctx->stream, ah);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
+    if (hb) {
+        HArgs ab = a;
+        ab.act = act_b;
+        const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((kp + 63) / 64), (unsigned)nslots);
+        hipLaunchKernelGGL(h_build_bf16_kernel, grid, dim3(512), 0, ctx->stream, ab, hb, kp);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
+    return EPSX_OK;
+}
+
+int gt_bf16_launch(epsx_ctx *ctx, int64_t K, int64_t kp, int64_t ni, const double *Gt, uint16_t *gb)
+{
+    hipLaunchKernelGGL(gt_bf16_kernel, dim3((unsigned)((kp * ni + 255) / 256)), dim3(256), 0, ctx->stream, K, kp, ni,
+                       Gt, gb);
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
 }

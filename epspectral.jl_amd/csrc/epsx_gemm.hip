@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "epsx_common.h"
+#include "epsx_device.h"
 
 namespace epsx {
 namespace {
@@ -222,15 +223,6 @@ __global__ __launch_bounds__(256) void dgemm_splitk_reduce_kernel(GemmArgs p)
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-__device__ inline unsigned short to_bf16(double x)
-{
-    const float f = (float)x;
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (unsigned short)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
-    return (unsigned short)(u >> 16);
-}
-
 // The same product with the FP64 operands rounded to FP32 (v_mfma_f32_16x16x4f32) or BF16
 // (v_mfma_f32_16x16x32_bf16) as they are staged to LDS, FP32 accumulation, FP64 output.  64x64
 // tiles of 4 waves (32x32 each); operands staged k-contiguous (As[m][k], Bs[n][k]) so a lane
@@ -350,6 +342,112 @@ __global__ __launch_bounds__(256, 2) void lpgemm_kernel(GemmArgs p)
             }
 }
 
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+// The product with BF16 operands already in HBM, both k-contiguous: C[m, n] = Σ_k AT[k, m] B[k, n]
+// with AT and B of row pitch lda, ldb (multiples of 8, zero beyond k, k a multiple of 8), FP32
+// accumulation, FP64 output through the split-k partials, gates and epilogue of the FP64 kernel.
+// 128x128 tiles of 4 waves (64x64 each as 4x4 accumulators of v_mfma_f32_16x16x32_bf16); k-stages
+// of 64 arrive as 16-byte runs written unchanged into k-contiguous LDS rows (padded by 8 so the
+// fragment reads of 16 rows hit distinct banks), and the next stage's loads are in flight during
+// the current stage's MFMAs.
+constexpr int BGK = 64;
+__global__ __launch_bounds__(256, 2) void bgemm_kernel(GemmArgs p)
+{
+    constexpr int BM = 128, BN = 128, LDK = BGK + 8;
+    constexpr int QA = BM * (BGK / 8) / 256, QB = BN * (BGK / 8) / 256;   // 16-byte runs per thread
+    int tm, tn, s, S;
+    int64_t z;
+    if (!unit_of_block(p, tm, tn, s, z, S)) return;
+    const unsigned short *A = reinterpret_cast<const unsigned short *>(p.A) + p.sA * opi(p, z, 1);
+    const unsigned short *B = reinterpret_cast<const unsigned short *>(p.B) + p.sB * opi(p, z, 2);
+    const int kt0 = (int)((int64_t)p.ktiles * s / S), kt1 = (int)((int64_t)p.ktiles * (s + 1) / S);
+    __shared__ __align__(16) unsigned short As[BM][LDK];
+    __shared__ __align__(16) unsigned short Bs[BN][LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
+    const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
+    const u32x4_t zero = {0u, 0u, 0u, 0u};
+    u32x4_t ra[QA], rb[QB];
+    // run e = tid + 256 q: row e / 8 of the tile, k-run e % 8 of the stage
+    auto load = [&](int kt) {
+        const int64_t k0 = (int64_t)kt * BGK;
+#pragma unroll
+        for (int q = 0; q < QA; q++) {
+            const int e = tid + 256 * q;
+            const int64_t gi = row0 + (e >> 3), gk = k0 + 8 * (e & 7);
+            ra[q] = (gi < m && gk < k) ? *reinterpret_cast<const u32x4_t *>(A + gk + lda * gi) : zero;
+        }
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const int e = tid + 256 * q;
+            const int64_t gj = col0 + (e >> 3), gk = k0 + 8 * (e & 7);
+            rb[q] = (gj < n && gk < k) ? *reinterpret_cast<const u32x4_t *>(B + gk + ldb * gj) : zero;
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int q = 0; q < QA; q++) {
+            const int e = tid + 256 * q;
+            *reinterpret_cast<u32x4_t *>(&As[e >> 3][8 * (e & 7)]) = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const int e = tid + 256 * q;
+            *reinterpret_cast<u32x4_t *>(&Bs[e >> 3][8 * (e & 7)]) = rb[q];
+        }
+    };
+    f32x4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    if (kt0 < kt1) {
+        load(kt0);
+        stage();
+    }
+    __syncthreads();
+    for (int kt = kt0; kt < kt1; kt++) {
+        const bool more = kt + 1 < kt1;
+        if (more) load(kt + 1);
+#pragma unroll
+        for (int ks = 0; ks < BGK; ks += 32) {
+            // lane l holds A[l % 16][8 (l / 16) .. + 7] and B[8 (l / 16) .. + 7][l % 16] of a 16x16x32 step
+            const int kb = ks + 8 * (lane >> 4);
+            bf16x8_t af[4], bf[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) __builtin_memcpy(&af[a], &As[wm * 64 + a * 16 + (lane & 15)][kb], 16);
+#pragma unroll
+            for (int b = 0; b < 4; b++) __builtin_memcpy(&bf[b], &Bs[wn * 64 + b * 16 + (lane & 15)][kb], 16);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bf[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) stage();
+        __syncthreads();
+    }
+    // C[i][j] of a 16x16 FP32 tile: element e of lane l holds i = 4 (l / 16) + e, j = l % 16
+    const int64_t zc = opi(p, z, 4);
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t gi = row0 + wm * 64 + a * 16 + 4 * (lane >> 4) + e;
+                const int64_t gj = col0 + wn * 64 + b * 16 + (lane & 15);
+                if (gi < m && gj < n) {
+                    const double val = (double)acc[a][b][e];
+                    if (S == 1) store_out(p, zc, gi, gj, val);
+                    else p.part[gi + m * gj + m * n * (s + (int64_t)S * z)] = val;
+                }
+            }
+}
+
 // The tile shapes the autotuner chooses from: (BM, BN, waves along m, waves along n)
 struct GemmShape { int bm, bn, wmw, wnw; };
 constexpr GemmShape SHAPES[] = {{64, 64, 2, 2}, {128, 64, 4, 2}, {64, 128, 2, 2}, {128, 128, 4, 2}, {128, 128, 2, 2}};
@@ -375,6 +473,51 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
     return EPSX_OK;
 }
 
+// The k split of a launch: the split with the least rounds of `slots` resident workgroups x
+// k-tiles per unit (+2 for the prologue and epilogue), keeping >= 4 k-tiles per split.
+static int split_choice(int64_t tiles, int ktiles, int64_t inst, int64_t slots)
+{
+    int best_s = 1;
+    double best = 0;
+    for (int c = 1; c <= 16 && (c == 1 || ktiles / c >= 4); c++) {
+        const int64_t rounds = (tiles * c * inst + slots - 1) / slots;
+        const double cost = (double)rounds * ((double)ktiles / c + 2.0);
+        if (c == 1 || cost < best) { best = cost; best_s = c; }
+    }
+    return best_s;
+}
+
+// The split table of a batch (p.ktiles set): S for all instances and, with gates, S_of[c] for c
+// active instances (a table the kernels look up, so the last unconverged temperature of a batch
+// still fills the chip); sizes the partials workspace, which must exist before graph capture.
+static int plan_splits(epsx_ctx *ctx, GemmArgs &p, const GemmBatch &bt, int64_t tiles, int64_t slots, int &S,
+                       int &Smax)
+{
+    S = split_choice(tiles, p.ktiles, bt.n_inst, slots);
+    Smax = S;
+    p.S_of[0] = S;
+    p.units_per_tile = (int64_t)S * bt.n_inst;
+    for (int c = 1; c <= NSPLIT_TAB; c++) {
+        p.S_of[c] = (bt.gate && c <= bt.n_inst) ? split_choice(tiles, p.ktiles, c, slots) : S;
+        Smax = p.S_of[c] > Smax ? p.S_of[c] : Smax;
+        if (c <= bt.n_inst && (int64_t)p.S_of[c] * c > p.units_per_tile) p.units_per_tile = (int64_t)p.S_of[c] * c;
+    }
+    if (Smax > 1) {
+        const size_t need = 8 * (size_t)p.m * p.n * Smax * bt.n_inst;
+        if (ctx->gemm_ws.bytes < need) {
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            EPSX_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
+            if (cap != hipStreamCaptureStatusNone)
+                return set_error(ctx, EPSX_ERR_STATE, "GEMM workspace must be sized before graph capture");
+            EPSX_TRY(dev_alloc(ctx, ctx->gemm_ws, need));
+        }
+        p.part = static_cast<double *>(ctx->gemm_ws.p);
+    }
+    p.S = S;
+    p.n_inst = bt.n_inst;
+    return EPSX_OK;
+}
+
 }  // namespace
 
 int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *B,
@@ -390,44 +533,13 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     p.ktiles = (int)((k + GK - 1) / GK);
     // Split k to fill the machine, by the 64x64 tile count whatever shape runs (so the split,
     // and with it every sum, does not depend on the tuned shape): 256 CUs hold 4 workgroups
-    // each, so a launch runs in rounds of 1024 units; pick the split with the least rounds x
-    // k-tiles per unit (+2 for the prologue and epilogue), keeping >= 4 k-tiles per split.  With
-    // gates the split follows the number of active instances (a table the kernels look up), so
-    // the last unconverged temperature of a batch still fills the chip.
+    // each, so a launch runs in rounds of 1024 units.
+    int S = 1, Smax = 1;
     const int64_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
-    constexpr int64_t SLOTS = 256 * 4;
-    auto choose = [&](int64_t inst) {
-        int best_s = 1;
-        double best = 0;
-        for (int c = 1; c <= 16 && (c == 1 || p.ktiles / c >= 4); c++) {
-            const int64_t rounds = (tiles * c * inst + SLOTS - 1) / SLOTS;
-            const double cost = (double)rounds * ((double)p.ktiles / c + 2.0);
-            if (c == 1 || cost < best) { best = cost; best_s = c; }
-        }
-        return best_s;
-    };
-    const int S = choose(bt.n_inst);
-    int Smax = S;
-    p.S_of[0] = S;
-    p.units_per_tile = (int64_t)S * bt.n_inst;
-    for (int c = 1; c <= NSPLIT_TAB; c++) {
-        p.S_of[c] = (bt.gate && c <= bt.n_inst) ? choose(c) : S;
-        Smax = p.S_of[c] > Smax ? p.S_of[c] : Smax;
-        if (c <= bt.n_inst && (int64_t)p.S_of[c] * c > p.units_per_tile) p.units_per_tile = (int64_t)p.S_of[c] * c;
-    }
+    EPSX_TRY(plan_splits(ctx, p, bt, tiles, 256 * 4, S, Smax));
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     EPSX_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;
-    if (Smax > 1) {
-        const size_t need = 8 * (size_t)m * n * Smax * bt.n_inst;
-        if (ctx->gemm_ws.bytes < need) {
-            if (capturing) return set_error(ctx, EPSX_ERR_STATE, "GEMM workspace must be sized before graph capture");
-            EPSX_TRY(dev_alloc(ctx, ctx->gemm_ws, need));
-        }
-        p.part = static_cast<double *>(ctx->gemm_ws.p);
-    }
-    p.S = S;
-    p.n_inst = bt.n_inst;
     if (bt.precision == EPSX_PREC_FP32 || bt.precision == EPSX_PREC_BF16) {
         p.mt = (int)((m + 63) / 64);
         p.nt = (int)((n + 63) / 64);
@@ -491,6 +603,36 @@ int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A,
     GemmBatch bt;
     bt.gate = gate;
     return dgemm(ctx, m, ncol, k, A, lda, B, ldb, C, ldc, accumulate, bt);
+}
+
+int bgemm_tn(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const uint16_t *AT, int64_t lda, const uint16_t *B,
+             int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt)
+{
+    if (m <= 0 || n <= 0 || bt.n_inst <= 0) return EPSX_OK;
+    if (k % 8 || lda % 8 || ldb % 8 || bt.sA % 8 || bt.sB % 8)
+        return set_error(ctx, EPSX_ERR_ARG, "BF16 product: k, row pitches and strides must be multiples of 8");
+    GemmArgs p{};
+    p.m = m; p.n = n; p.k = k; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
+    p.sA = bt.sA; p.sB = bt.sB; p.sC = bt.sC;
+    p.A = reinterpret_cast<const double *>(AT); p.B = reinterpret_cast<const double *>(B);
+    p.C = C; p.Z = bt.Z; p.Zim = bt.Zim; p.diag = bt.diag;
+    p.idx = bt.idx; p.gate = bt.gate; p.imap = bt.imap;
+    p.accumulate = accumulate ? 1 : 0;
+    p.ktiles = (int)((k + BGK - 1) / BGK);
+    p.mt = (int)((m + 127) / 128);
+    p.nt = (int)((n + 127) / 128);
+    // 36 KB of LDS and 2 waves per SIMD: 2 workgroups per CU
+    int S = 1, Smax = 1;
+    EPSX_TRY(plan_splits(ctx, p, bt, (int64_t)p.mt * p.nt, 256 * 2, S, Smax));
+    const int64_t grid = ((int64_t)p.mt * p.nt * p.units_per_tile + NXCD - 1) / NXCD * NXCD;
+    hipLaunchKernelGGL(bgemm_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
+    EPSX_HIP(ctx, hipGetLastError());
+    if (Smax > 1) {
+        hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
+                           dim3(256), 0, ctx->stream, p);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
+    return EPSX_OK;
 }
 
 }  // namespace epsx

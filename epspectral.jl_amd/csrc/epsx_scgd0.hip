@@ -4,6 +4,7 @@
 // err, μ, min ∫A and a status word per temperature.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -357,22 +358,29 @@ __global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t 
     if (threadIdx.x == 0) bmax[blockIdx.x + gridDim.x * z] = red[0];
 }
 
-// The product gates of a mixed-precision run: lpg[t] = active and in the low-precision phase,
-// hpg[t] = active and in the FP64 phase (act null: every temperature active).
-__global__ void prec_gates_kernel(int64_t nT, const int32_t *__restrict__ act, const int32_t *__restrict__ hp,
-                                  int32_t *__restrict__ lpg, int32_t *__restrict__ hpg)
+// The precision levels of a mixed-precision run, lowest first (the last is FP64).
+struct Levels {
+    int n;
+    int prec[3];
+};
+
+// The product gates of a mixed-precision run: gates[l][t] = active and at level l; row L.n: active
+// and needing the FP64 h of the star method (every level but native BF16).  act null: every
+// temperature active.
+__global__ void phase_gates_kernel(int64_t nT, const int32_t *__restrict__ act, const int32_t *__restrict__ prec,
+                                   Levels L, int native_bf16, int32_t *__restrict__ gates)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= nT) return;
     const int a = act ? act[t] != 0 : 1;
-    lpg[t] = a && !hp[t];
-    hpg[t] = a && hp[t];
+    for (int l = 0; l < L.n; l++) gates[t + nT * l] = a && prec[t] == L.prec[l];
+    gates[t + nT * L.n] = a && !(native_bf16 && prec[t] == EPSX_PREC_BF16);
 }
 
 // Device-side bookkeeping of the graph-replayed loop: the history row of this iteration, the
 // freeze of a converged temperature (src/scgd0.jl:326-333).
 struct GraphBook {
-    const int32_t *hp = nullptr;   // [nT] 1 = FP64 phase (may freeze); null: every temperature
+    const int32_t *prec = nullptr;   // [nT] precision level (only FP64 may freeze); null: all FP64
     int32_t *act;            // [nT] convergence mask
     int64_t *frozen;         // [nT] iteration at which T froze
     const int32_t *it;       // 0-based iteration counter
@@ -408,7 +416,7 @@ __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__r
             g.hist_mu[t + g.nT * it] = g.mu[t];
             g.hist_minA[t + g.nT * it] = g.minA[t];
         }
-        if (m < g.tol && (!g.hp || g.hp[t])) {
+        if (m < g.tol && (!g.prec || g.prec[t] == EPSX_PREC_FP64)) {
             g.act[t] = 0;
             g.frozen[t] = it + 1;
         }
@@ -431,6 +439,7 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     Scgd0Dev &s = ctx->s;
     s.ready = false;
     s.gt_ready = false;
+    s.gb_ready = false;
     const int64_t nm = P->nm, ni = P->ni, nf = P->nf, nw = P->nw, nwa = P->nwa;
     if (nm < 1 || ni < 1 || nf < 1 || nw < 2 || nwa < 2 || P->nq < 1 || P->nki < 1 || P->nkf < 1)
         return set_error(ctx, EPSX_ERR_ARG, "scgd0 problem: empty dimension (nm=%lld ni=%lld nf=%lld nw=%lld nwa=%lld)",
@@ -644,8 +653,18 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         npart = e > npart ? e : npart;
     }
     EPSX_TRY(tmp_alloc(ctx, B.part.b, 8 * npart));
+    // the GEMM split-k workspace for every product of the run (the k split is at most 16), sized
+    // before any graph capture: a change of precision level captures new products
+    EPSX_TRY(dev_alloc(ctx, ctx->gemm_ws, 8 * (size_t)nw * ni * 16 * nT));
     for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
-    Tmp hbuf, hocc;
+    Tmp hbuf, hocc, hb;
+    // BF16 low-precision phase of the star method: h and G̃ stored in BF16, k-contiguous with the
+    // row pitch kp (nm ni rounded up to a multiple of 8), multiplied by the BF16 MFMA kernel.
+    // EPSX_BF16_STAGED=1 rounds the FP64 operands as they are staged instead (for tests).
+    const char *staged_env = getenv("EPSX_BF16_STAGED");
+    const bool native_bf16 = method == EPSX_FM_STAR && par->precision == EPSX_PREC_BF16 &&
+                             !(staged_env && staged_env[0] == '1');
+    const int64_t kp = (s.nm * ni + 7) / 8 * 8;
     if (method == EPSX_FM_STAR) {
         EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
         EPSX_TRY(tmp_alloc(ctx, hocc.b, 16 * (size_t)nw * s.nm * nT));
@@ -671,8 +690,15 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             ctx->t_class[4] = ms * 1e-3;
             ctx->n_class[4] = 1;
             s.gt_ready = true;
+            s.gb_ready = false;
             s.gt_wac = par->w_acoustic;
         }
+        if (native_bf16 && !s.gb_ready) {
+            EPSX_TRY(dev_alloc(ctx, s.Gb, 2 * (size_t)kp * ni));
+            EPSX_TRY(gt_bf16_launch(ctx, s.nm * ni, kp, ni, dptr<double>(s.Gt), dptr<uint16_t>(s.Gb)));
+            s.gb_ready = true;
+        }
+        if (native_bf16) EPSX_TRY(tmp_alloc(ctx, hb.b, 2 * (size_t)kp * nw * nT));
     }
     const int nbmax = 512;
     EPSX_TRY(tmp_alloc(ctx, B.bmax.b, 8 * (size_t)nbmax * nT));
@@ -724,42 +750,69 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // One iteration of src/scgd0.jl:252-333 for the nA temperature slots in d_tidx.  With a
     // device mask (graph mode) every kernel skips converged temperatures, and the convergence
     // test writes the history and freezes temperatures on the device.
-    // With a low-precision mode every temperature carries its own phase (hp): the products run
-    // twice per iteration, gated by lpg (low precision) and hpg (FP64).
+    // With a low-precision mode every temperature carries its own precision level (lvl): BF16, then
+    // FP32, then FP64 for precision BF16; FP32, then FP64 for precision FP32.  The products run once
+    // per level present among the active temperatures, gated on the device by level.
     const bool split = par->precision != EPSX_PREC_FP64;
-    std::vector<int32_t> hp(nT, split ? 0 : 1);
-    Tmp d_hp, d_lpg, d_hpg;
+    Levels L{};
+    if (par->precision == EPSX_PREC_BF16) L = {3, {EPSX_PREC_BF16, EPSX_PREC_FP32, EPSX_PREC_FP64}};
+    else if (par->precision == EPSX_PREC_FP32) L = {2, {EPSX_PREC_FP32, EPSX_PREC_FP64, 0}};
+    else L = {1, {EPSX_PREC_FP64, 0, 0}};
+    std::vector<int32_t> lvl(nT, 0), prec_t(nT, L.prec[0]);
+    Tmp d_prec, d_gates;
     if (split) {
-        EPSX_TRY(tmp_upload(ctx, d_hp, hp.data(), 4 * nT));
-        EPSX_TRY(tmp_alloc(ctx, d_lpg.b, 4 * (size_t)nT));
-        EPSX_TRY(tmp_alloc(ctx, d_hpg.b, 4 * (size_t)nT));
+        EPSX_TRY(tmp_upload(ctx, d_prec, prec_t.data(), 4 * nT));
+        EPSX_TRY(tmp_alloc(ctx, d_gates.b, 4 * (size_t)nT * (L.n + 1)));
     }
+    std::vector<int32_t> active(nT, 1), act_idx;
+    // the levels present among the active temperatures, as a bit mask
+    auto present_levels = [&]() {
+        int mask = 0;
+        for (int64_t t = 0; t < nT; t++)
+            if (active[t]) mask |= 1 << lvl[t];
+        return mask;
+    };
     auto enqueue = [&](int nA, const int32_t *d_tidx, const std::vector<int32_t> &h_tidx, const int32_t *d_act,
                        const GraphBook &book, bool events) -> int {
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[0], st));
-        if (split) {
-            hipLaunchKernelGGL(prec_gates_kernel, dim3(nblk(nT)), dim3(256), 0, st, nT, d_act, d_hp.ptr<int32_t>(),
-                               d_lpg.ptr<int32_t>(), d_hpg.ptr<int32_t>());
-            EPSX_HIP(ctx, hipGetLastError());
-        }
-        // the (gate, precision) passes of the products: one FP64 pass, or the two phases
+        // the (gate, precision) passes of the products: one FP64 pass, or one per level present
         struct Pass { const int32_t *gate; int prec; bool tune; };
-        Pass passes[2] = {{d_act, EPSX_PREC_FP64, true}, {nullptr, 0, false}};
+        Pass passes[3] = {{d_act, EPSX_PREC_FP64, true}, {nullptr, 0, false}, {nullptr, 0, false}};
         int npass = 1;
+        // FP64 h (every level but native BF16) and BF16 h (native BF16) of the star method
+        bool need_h64 = true, need_hbf = false;
+        const int32_t *gate_h64 = d_act, *gate_hbf = nullptr;
         if (split) {
-            passes[0] = {d_lpg.ptr<int32_t>(), par->precision, false};
-            passes[1] = {d_hpg.ptr<int32_t>(), EPSX_PREC_FP64, false};
-            npass = 2;
+            hipLaunchKernelGGL(phase_gates_kernel, dim3(nblk(nT)), dim3(256), 0, st, nT, d_act,
+                               d_prec.ptr<int32_t>(), L, native_bf16 ? 1 : 0, d_gates.ptr<int32_t>());
+            EPSX_HIP(ctx, hipGetLastError());
+            const int mask = present_levels();
+            npass = 0;
+            need_h64 = false;
+            gate_h64 = d_gates.ptr<int32_t>() + nT * L.n;
+            for (int l = 0; l < L.n; l++) {
+                if (!(mask & (1 << l))) continue;
+                const int32_t *g = d_gates.ptr<int32_t>() + nT * l;
+                passes[npass++] = {g, L.prec[l], false};
+                if (native_bf16 && L.prec[l] == EPSX_PREC_BF16) {
+                    need_hbf = true;
+                    gate_hbf = g;
+                } else {
+                    need_h64 = true;
+                }
+            }
         }
         FmTemps tp;
         tp.T = B.T.ptr<double>(); tp.mu = B.mu.ptr<double>(); tp.eta_min = B.eta.ptr<double>();
         tp.act = d_act;
         // 1-3. clamp + Fan-Migdal (the clamp is applied where Σin is read)
         if (method == EPSX_FM_STAR) {
+            // FP64 h for the FP64 pass; with native BF16, BF16 h for the low-precision pass
             EPSX_TRY(h_build_launch(ctx, nw, s.nm, ni, dptr<double>(s.ws), dptr<int32_t>(s.hk), dptr<double>(s.ht),
                                     dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(), B.T.ptr<double>(),
                                     B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA, par->w_acoustic,
-                                    hocc.ptr<double>(), hbuf.ptr<double>(), d_act));
+                                    hocc.ptr<double>(), need_h64 ? hbuf.ptr<double>() : nullptr, d_act, gate_h64,
+                                    need_hbf ? hb.ptr<uint16_t>() : nullptr, kp, gate_hbf));
             if (events) EPSX_HIP(ctx, hipEventRecord(ev[5], st));
             // ImΣ[:, :, t] = h_z G̃ for every slot z in one batched launch (G̃ shared)
             for (int q = 0; q < npass; q++) {
@@ -772,8 +825,14 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 bt.gate = passes[q].gate;
                 bt.precision = passes[q].prec;
                 bt.tune = passes[q].tune;
-                EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
-                               B.ImS.ptr<double>(), nw, false, bt));
+                if (native_bf16 && passes[q].prec == EPSX_PREC_BF16) {
+                    bt.sA = kp * nw;
+                    EPSX_TRY(bgemm_tn(ctx, nw, ni, kp, hb.ptr<uint16_t>(), kp, dptr<uint16_t>(s.Gb), kp,
+                                      B.ImS.ptr<double>(), nw, false, bt));
+                } else {
+                    EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
+                                   B.ImS.ptr<double>(), nw, false, bt));
+                }
             }
         } else {
             for (int q = 0; q < npass; q++) {
@@ -893,31 +952,40 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                          s.wsa_host[0] - 0.1, s.wsa_host[nwa - 1] + 0.1, n_target, par->Tlist[t]);
     };
 
-    std::vector<int32_t> active(nT, 1), act_idx;
     std::vector<double> h_err(nT), h_mu(nT), h_minA(nT), prev_err(nT, INFINITY);
     std::vector<int32_t> h_status(nT);
     std::vector<double> mu_host(par->mu_start, par->mu_start + nT);
     int rc = EPSX_OK;
     // Precision schedule (with FP32 / BF16): a temperature runs its Fan-Migdal products (and the
-    // off-diagonal of KK) in low precision, and may not freeze, until it is within 2 tol or stops
-    // contracting (err > 0.98 err_prev, the low-precision floor); from then on it runs in FP64 to tol.
-    std::vector<int64_t> lp_iters(nT, 0);
+    // off-diagonal of KK) at its level, and may not freeze, until it is within 2 tol (then FP64) or
+    // has reached the level's floor (then the next level).  A floor is two consecutive iterations
+    // that do not contract (err > 0.95 err_prev) once err is two decades below its first value: the
+    // early iterations of a strongly coupled solve may rise once without being near any floor.  At
+    // most half of maxiter runs below FP64.
+    std::vector<int64_t> lp_iters(nT, 0), lp_flat(nT, 0);
+    std::vector<double> first_err(nT, NAN);
     auto update_phases = [&]() -> int {
         bool changed = false, any_lp = false;
         for (int64_t t = 0; t < nT; t++) {
-            if (hp[t]) continue;
+            if (lvl[t] == L.n - 1) continue;
             any_lp = true;
-            lp_iters[t]++;
+            if (lp_iters[t]++ == 0) first_err[t] = h_err[t];
             const bool near = h_err[t] <= 2 * par->tol;
-            const bool floor_reached = lp_iters[t] >= 3 && h_err[t] > 0.98 * prev_err[t];
-            if (near || floor_reached || !(h_err[t] == h_err[t])) {
-                hp[t] = 1;
+            lp_flat[t] = (h_err[t] > 0.95 * prev_err[t] && h_err[t] < 1e-2 * first_err[t]) ? lp_flat[t] + 1 : 0;
+            const bool budget = 2 * lp_iters[t] >= par->maxiter;
+            int next = lvl[t];
+            if (near || budget || !(h_err[t] == h_err[t])) next = L.n - 1;
+            else if (lp_flat[t] >= 2) next = lvl[t] + 1;
+            if (next != lvl[t]) {
+                lvl[t] = next;
+                prec_t[t] = L.prec[next];
+                lp_flat[t] = 0;
                 changed = true;
             }
             prev_err[t] = h_err[t];
         }
         if (any_lp) out->n_low_precision++;
-        if (changed) EPSX_HIP(ctx, hipMemcpyAsync(d_hp.b.p, hp.data(), 4 * nT, hipMemcpyHostToDevice, st));
+        if (changed) EPSX_HIP(ctx, hipMemcpyAsync(d_prec.b.p, prec_t.data(), 4 * nT, hipMemcpyHostToDevice, st));
         return EPSX_OK;
     };
     if (!par->batched) {
@@ -943,7 +1011,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 out->hist_minA[t + nT * (iter - 1)] = h_minA[t];
                 out->hist_err[t + nT * (iter - 1)] = h_err[t];
                 out->hist_mu[t + nT * (iter - 1)] = h_mu[t];
-                if (hp[t] && h_err[t] < par->tol) {
+                if (lvl[t] == L.n - 1 && h_err[t] < par->tol) {
                     active[t] = 0;
                     out->iter_frozen[t] = iter;
                 }
@@ -983,9 +1051,10 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         book.hist_err = d_he.ptr<double>(); book.hist_mu = d_hm.ptr<double>(); book.hist_minA = d_ha.ptr<double>();
         book.mu = B.mu.ptr<double>(); book.minA = B.minA.ptr<double>();
         book.nT = nT; book.maxiter = maxiter; book.tol = par->tol;
-        if (split) book.hp = d_hp.ptr<int32_t>();
+        if (split) book.prec = d_prec.ptr<int32_t>();
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+        int graph_levels = 0;
         struct GraphGuard {
             hipGraph_t *g; hipGraphExec_t *e;
             ~GraphGuard() { if (*e) (void)hipGraphExecDestroy(*e); if (*g) (void)hipGraphDestroy(*g); }
@@ -995,7 +1064,15 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             if (iter == 1) {
                 EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, true));
             } else {
+                // a new set of precision levels present: the passes change, capture again
+                if (exec && present_levels() != graph_levels) {
+                    EPSX_HIP(ctx, hipGraphExecDestroy(exec));
+                    EPSX_HIP(ctx, hipGraphDestroy(graph));
+                    exec = nullptr;
+                    graph = nullptr;
+                }
                 if (!exec) {
+                    graph_levels = present_levels();
                     EPSX_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                     int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, false);
                     hipError_t ec = hipStreamEndCapture(st, &graph);
@@ -1018,7 +1095,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             if (split) {
                 // frozen temperatures are not in a phase any more
                 for (int64_t t = 0; t < nT; t++)
-                    if (!active[t]) hp[t] = 1;
+                    if (!active[t]) { lvl[t] = L.n - 1; prec_t[t] = EPSX_PREC_FP64; }
                 EPSX_TRY(update_phases());
             }
             bool any = false;

@@ -317,3 +317,28 @@ def test_low_precision_phase_then_fp64_sweep(ctx, prec, method):
         scale = np.max(np.abs(ref["Σ"]))
         assert np.max(np.abs(r["Σ"] - ref["Σ"])) <= 50 * kw["tol"] + 1e-9 * scale
         assert np.max(np.abs(r["μ"] - ref["μ"])) <= 1e-6
+
+
+def test_bf16_star_product_native_matches_staged(ctx):
+    """The BF16 star product from BF16 operands stored k-contiguous (h built in BF16, G̃ converted
+    once) against the product that rounds the FP64 operands as it stages them: the same rounded
+    operands, so the two agree to FP32 accumulation order (1e-5 relative), while both differ from
+    FP64 by the BF16 operand rounding.  One iteration, in the low-precision phase."""
+    s, meta = t2g_setup(nk=8, Tlist_K=(100.0, 300.0), grid_windows=((0.3, 4e-3), (1.2, 16e-3), (3.0, 64e-3)))
+    o = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    kw = dict(η_min=0.0, η_init=5e-2, maxiter=1, tol=0.0, ω_acoustic=1e-6, method="star", fix_μ=True)
+    dev = epsx.Scgd0Device(s, ctx)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r64 = dev.loop(o, batched=True, **kw)
+        rn = dev.loop(o, batched=True, precision="bf16", **kw)
+        os.environ["EPSX_BF16_STAGED"] = "1"
+        try:
+            rs = dev.loop(o, batched=True, precision="bf16", **kw)
+        finally:
+            del os.environ["EPSX_BF16_STAGED"]
+    assert rn["info"]["n_low_precision"] == 1 and rs["info"]["n_low_precision"] == 1
+    d_ns = rel(rn["Σ"].imag, rs["Σ"].imag)
+    d_64 = rel(rn["Σ"].imag, r64["Σ"].imag)
+    assert d_ns <= 1e-5, d_ns
+    assert 1e-7 < d_64 < 1e-2, d_64
