@@ -52,8 +52,8 @@ struct Scgd0Dev {
     DevBuf Gt;                        // star-collapsed coupling G̃ [(ν, i'), i] (built on demand)
     bool gt_ready = false;            // G̃ holds the current problem's reduction ...
     double gt_wac = 0;                // ... with modes below this ω_acoustic skipped
-    DevBuf Gb;                        // G̃ in BF16, k-contiguous rows of pitch kp (built on demand)
-    bool gb_ready = false;            // Gb holds the current G̃
+    DevBuf Gb, Gf;                    // G̃ in BF16 / FP32, k-contiguous rows of pitch kp (on demand)
+    bool gb_ready = false, gf_ready = false;   // they hold the current G̃
     DevBuf chunks, skip, wnu;         // star chunks of the reduction, acoustic flags, ω_ν
     DevBuf hk, ht;                    // cells and weights of ω ± ω_ν (Einstein modes)
     int64_t nchunks = 0;
@@ -200,10 +200,11 @@ struct GemmBatch {
 // C[m, n] = A[m, k] * B[k, n] (+ C if accumulate), FP64, column-major, on ctx->stream
 int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *B,
           int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt);
-// The same product from BF16 operands stored k-contiguous (AT[k, m] and B[k, n], row pitches,
-// strides and k multiples of 8, zero-padded beyond the true k), FP32 accumulation, FP64 output.
-int bgemm_tn(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const uint16_t *AT, int64_t lda, const uint16_t *B,
-             int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt);
+// The same product from low-precision operands stored k-contiguous (precision EPSX_PREC_BF16:
+// uint16 BF16 bits, EPSX_PREC_FP32: float; AT[k, m] and B[k, n], row pitches, strides and k
+// multiples of 8, zero-padded beyond the true k), FP32 accumulation, FP64 output.
+int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, const void *AT, int64_t lda,
+              const void *B, int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt);
 int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A, int64_t lda,
              const double *B, int64_t ldb, double *C, int64_t ldc, bool accumulate,
              const int32_t *gate = nullptr);   // gate: device flag, 0 = skip the product
@@ -265,8 +266,8 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
                    const double *ht, const double *wnu, const double *e_i, const double *S, const double *T,
                    const double *mu, const double *eta, const int32_t *tidx, int nslots, double w_acoustic,
                    double *occ, double *hout, const int32_t *act, const int32_t *act_h, uint16_t *hb,
-                   int64_t kp, const int32_t *act_b);
-int gt_bf16_launch(epsx_ctx *ctx, int64_t K, int64_t kp, int64_t ni, const double *Gt, uint16_t *gb);
+                   const int32_t *act_b, float *hf, const int32_t *act_f, int64_t kp);
+int gt_lp_launch(epsx_ctx *ctx, int precision, int64_t K, int64_t kp, int64_t ni, const double *Gt, void *gl);
 size_t fm_part_elems(int64_t nfs, int64_t ni, int64_t nw, int64_t nT, int64_t f_tile);
 int fm_grid_host(int64_t nw, const int64_t *offsets, std::vector<int32_t> &off,
                  std::vector<double> &rgap, std::vector<int32_t> &ltab, int64_t &span);

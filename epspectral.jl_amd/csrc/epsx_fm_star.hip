@@ -177,47 +177,51 @@ __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
     out[q] = a.wnu[nu] < a.w_acoustic ? 0.0 : h_elem(a, w, nu, ip, tcol, z);
 }
 
-// h per slot in BF16, transposed for the BF16 product: hb[k + kp w] with k = ν + nm i' (zero for
-// nm ni <= k < kp, kp a multiple of 8).  One thread per (ω, run of 8 k): lanes along ω (so the Σ
-// reads stay coalesced), the 8 waves of a block along k, so a block writes whole 128-byte rows.
-__global__ __launch_bounds__(512) void h_build_bf16_kernel(HArgs a, uint16_t *hb, int64_t kp)
+// h per slot in low precision (E = unsigned short: BF16 bits, or float), transposed for the
+// low-precision product: hl[k + kp w] with k = ν + nm i' (zero for nm ni <= k < kp, kp a multiple
+// of 8).  One thread per (ω, 16-byte run of k): lanes along ω (so the Σ reads stay coalesced), the
+// 8 waves of a block along k, so a block writes whole 128-byte rows.
+template <typename E>
+__global__ __launch_bounds__(512) void h_build_lp_kernel(HArgs a, E *hl, int64_t kp)
 {
+    constexpr int R = 16 / (int)sizeof(E);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t w = (int64_t)blockIdx.x * 64 + lane;
-    const int64_t k0 = ((int64_t)blockIdx.y * 8 + wave) * 8;
+    const int64_t k0 = ((int64_t)blockIdx.y * 8 + wave) * R;
     const int z = blockIdx.z;
     if (w >= a.nw || k0 >= kp) return;
     const int tcol = a.tidx[z];
     if (a.act && !a.act[tcol]) return;
     const int64_t K = a.nm * a.ni;
-    unsigned v[4];
+    E v[R];
 #pragma unroll
-    for (int e = 0; e < 8; e += 2) {
-        unsigned short b2[2];
-#pragma unroll
-        for (int d = 0; d < 2; d++) {
-            const int64_t k = k0 + e + d;
-            double h = 0.0;
-            if (k < K) {
-                const int64_t nu = k % a.nm, ip = k / a.nm;
-                if (a.wnu[nu] >= a.w_acoustic) h = h_elem(a, w, nu, ip, tcol, z);
-            }
-            b2[d] = to_bf16(h);
+    for (int e = 0; e < R; e++) {
+        const int64_t k = k0 + e;
+        double h = 0.0;
+        if (k < K) {
+            const int64_t nu = k % a.nm, ip = k / a.nm;
+            if (a.wnu[nu] >= a.w_acoustic) h = h_elem(a, w, nu, ip, tcol, z);
         }
-        v[e / 2] = (unsigned)b2[0] | ((unsigned)b2[1] << 16);
+        if constexpr (sizeof(E) == 2) v[e] = to_bf16(h);
+        else v[e] = (float)h;
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(hb + kp * a.nw * (int64_t)z + k0 + kp * w);
-    *dst = make_uint4(v[0], v[1], v[2], v[3]);
+    uint4 pk;
+    __builtin_memcpy(&pk, v, 16);
+    *reinterpret_cast<uint4 *>(hl + kp * a.nw * (int64_t)z + k0 + kp * w) = pk;
 }
 
-// G̃ in BF16 with the row pitch of the BF16 product: gb[k + kp i] = G̃[k + K i] (zero for K <= k < kp).
-__global__ __launch_bounds__(256) void gt_bf16_kernel(int64_t K, int64_t kp, int64_t ni, const double *__restrict__ Gt,
-                                                      uint16_t *__restrict__ gb)
+// G̃ in low precision with the row pitch of the low-precision product: gl[k + kp i] = G̃[k + K i]
+// (zero for K <= k < kp).
+template <typename E>
+__global__ __launch_bounds__(256) void gt_lp_kernel(int64_t K, int64_t kp, int64_t ni, const double *__restrict__ Gt,
+                                                    E *__restrict__ gl)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= kp * ni) return;
     const int64_t k = q % kp, i = q / kp;
-    gb[q] = k < K ? to_bf16(Gt[k + K * i]) : (uint16_t)0;
+    const double g = k < K ? Gt[k + K * i] : 0.0;
+    if constexpr (sizeof(E) == 2) gl[q] = to_bf16(g);
+    else gl[q] = (float)g;
 }
 
 }  // namespace
@@ -308,8 +312,8 @@ void star_lookup_host(int64_t nw, int64_t nm, const double *wnu, double h, const
 int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const double *ws, const int32_t *hk,
                    const double *ht, const double *wnu, const double *e_i, const double *S, const double *T,
                    const double *mu, const double *eta, const int32_t *tidx, int nslots, double w_acoustic,
-                   double *occ, double *hout, const int32_t *act, const int32_t *act_h, uint16_t *hb, int64_t kp,
-                   const int32_t *act_b)
+                   double *occ, double *hout, const int32_t *act, const int32_t *act_h, uint16_t *hb,
+                   const int32_t *act_b, float *hf, const int32_t *act_f, int64_t kp)
 {
     HArgs a;
     a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht;
@@ -326,20 +330,31 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
                            ctx->stream, ah);
         EPSX_HIP(ctx, hipGetLastError());
     }
+    const dim3 blk(512);
     if (hb) {
         HArgs ab = a;
         ab.act = act_b;
         const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((kp + 63) / 64), (unsigned)nslots);
-        hipLaunchKernelGGL(h_build_bf16_kernel, grid, dim3(512), 0, ctx->stream, ab, hb, kp);
+        hipLaunchKernelGGL(h_build_lp_kernel<uint16_t>, grid, blk, 0, ctx->stream, ab, hb, kp);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
+    if (hf) {
+        HArgs af = a;
+        af.act = act_f;
+        const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((kp + 31) / 32), (unsigned)nslots);
+        hipLaunchKernelGGL(h_build_lp_kernel<float>, grid, blk, 0, ctx->stream, af, hf, kp);
         EPSX_HIP(ctx, hipGetLastError());
     }
     return EPSX_OK;
 }
 
-int gt_bf16_launch(epsx_ctx *ctx, int64_t K, int64_t kp, int64_t ni, const double *Gt, uint16_t *gb)
+int gt_lp_launch(epsx_ctx *ctx, int precision, int64_t K, int64_t kp, int64_t ni, const double *Gt, void *gl)
 {
-    hipLaunchKernelGGL(gt_bf16_kernel, dim3((unsigned)((kp * ni + 255) / 256)), dim3(256), 0, ctx->stream, K, kp, ni,
-                       Gt, gb);
+    const dim3 grid((unsigned)((kp * ni + 255) / 256)), blk(256);
+    if (precision == EPSX_PREC_BF16)
+        hipLaunchKernelGGL(gt_lp_kernel<uint16_t>, grid, blk, 0, ctx->stream, K, kp, ni, Gt, static_cast<uint16_t *>(gl));
+    else
+        hipLaunchKernelGGL(gt_lp_kernel<float>, grid, blk, 0, ctx->stream, K, kp, ni, Gt, static_cast<float *>(gl));
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
 }

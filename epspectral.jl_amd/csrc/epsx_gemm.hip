@@ -344,45 +344,49 @@ __global__ __launch_bounds__(256, 2) void lpgemm_kernel(GemmArgs p)
 
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
-// The product with BF16 operands already in HBM, both k-contiguous: C[m, n] = Σ_k AT[k, m] B[k, n]
-// with AT and B of row pitch lda, ldb (multiples of 8, zero beyond k, k a multiple of 8), FP32
-// accumulation, FP64 output through the split-k partials, gates and epilogue of the FP64 kernel.
-// 128x128 tiles of 4 waves (64x64 each as 4x4 accumulators of v_mfma_f32_16x16x32_bf16); k-stages
-// of 64 arrive as 16-byte runs written unchanged into k-contiguous LDS rows (padded by 8 so the
-// fragment reads of 16 rows hit distinct banks), and the next stage's loads are in flight during
-// the current stage's MFMAs.
-constexpr int BGK = 64;
-__global__ __launch_bounds__(256, 2) void bgemm_kernel(GemmArgs p)
+// The product with low-precision operands already in HBM, both k-contiguous: C[m, n] =
+// Σ_k AT[k, m] B[k, n] with AT and B of row pitch lda, ldb (multiples of 8, zero beyond k, k a
+// multiple of 8), FP32 accumulation, FP64 output through the split-k partials, gates and epilogue
+// of the FP64 kernel.  E = unsigned short (BF16, v_mfma_f32_16x16x32_bf16) or float
+// (v_mfma_f32_16x16x4f32).  128x128 tiles of 4 waves (64x64 each as 4x4 accumulators); k-stages of
+// 128 bytes per row (64 BF16 or 32 FP32) arrive as 16-byte runs written unchanged into
+// k-contiguous LDS rows (padded by 16 bytes so the fragment reads of 16 rows hit distinct banks),
+// and the next stage's loads are in flight during the current stage's MFMAs.  FP32: a lane reads
+// 4 consecutive k of its row and feeds them to 4 steps of 16x16x4, lane group g taking
+// k = 4 g + j in step j (the same order for both operands, so each step is a valid product).
+template <typename E>
+__global__ __launch_bounds__(256, 2) void lpkgemm_kernel(GemmArgs p)
 {
-    constexpr int BM = 128, BN = 128, LDK = BGK + 8;
-    constexpr int QA = BM * (BGK / 8) / 256, QB = BN * (BGK / 8) / 256;   // 16-byte runs per thread
+    constexpr int BM = 128, BN = 128, EPR = 16 / (int)sizeof(E);   // elements per 16-byte run
+    constexpr int SK = 8 * EPR, LDK = SK + EPR;                     // stage k, padded LDS row
+    constexpr int QA = BM * 8 / 256, QB = BN * 8 / 256;             // 16-byte runs per thread
     int tm, tn, s, S;
     int64_t z;
     if (!unit_of_block(p, tm, tn, s, z, S)) return;
-    const unsigned short *A = reinterpret_cast<const unsigned short *>(p.A) + p.sA * opi(p, z, 1);
-    const unsigned short *B = reinterpret_cast<const unsigned short *>(p.B) + p.sB * opi(p, z, 2);
+    const E *A = reinterpret_cast<const E *>(p.A) + p.sA * opi(p, z, 1);
+    const E *B = reinterpret_cast<const E *>(p.B) + p.sB * opi(p, z, 2);
     const int kt0 = (int)((int64_t)p.ktiles * s / S), kt1 = (int)((int64_t)p.ktiles * (s + 1) / S);
-    __shared__ __align__(16) unsigned short As[BM][LDK];
-    __shared__ __align__(16) unsigned short Bs[BN][LDK];
+    __shared__ __align__(16) E As[BM][LDK];
+    __shared__ __align__(16) E Bs[BN][LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
     const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
     const u32x4_t zero = {0u, 0u, 0u, 0u};
     u32x4_t ra[QA], rb[QB];
-    // run e = tid + 256 q: row e / 8 of the tile, k-run e % 8 of the stage
+    // run e = tid + 256 q: row e / 8 of the tile, run e % 8 of the stage
     auto load = [&](int kt) {
-        const int64_t k0 = (int64_t)kt * BGK;
+        const int64_t k0 = (int64_t)kt * SK;
 #pragma unroll
         for (int q = 0; q < QA; q++) {
             const int e = tid + 256 * q;
-            const int64_t gi = row0 + (e >> 3), gk = k0 + 8 * (e & 7);
+            const int64_t gi = row0 + (e >> 3), gk = k0 + EPR * (e & 7);
             ra[q] = (gi < m && gk < k) ? *reinterpret_cast<const u32x4_t *>(A + gk + lda * gi) : zero;
         }
 #pragma unroll
         for (int q = 0; q < QB; q++) {
             const int e = tid + 256 * q;
-            const int64_t gj = col0 + (e >> 3), gk = k0 + 8 * (e & 7);
+            const int64_t gj = col0 + (e >> 3), gk = k0 + EPR * (e & 7);
             rb[q] = (gj < n && gk < k) ? *reinterpret_cast<const u32x4_t *>(B + gk + ldb * gj) : zero;
         }
     };
@@ -390,12 +394,12 @@ __global__ __launch_bounds__(256, 2) void bgemm_kernel(GemmArgs p)
 #pragma unroll
         for (int q = 0; q < QA; q++) {
             const int e = tid + 256 * q;
-            *reinterpret_cast<u32x4_t *>(&As[e >> 3][8 * (e & 7)]) = ra[q];
+            *reinterpret_cast<u32x4_t *>(&As[e >> 3][EPR * (e & 7)]) = ra[q];
         }
 #pragma unroll
         for (int q = 0; q < QB; q++) {
             const int e = tid + 256 * q;
-            *reinterpret_cast<u32x4_t *>(&Bs[e >> 3][8 * (e & 7)]) = rb[q];
+            *reinterpret_cast<u32x4_t *>(&Bs[e >> 3][EPR * (e & 7)]) = rb[q];
         }
     };
     f32x4_t acc[4][4];
@@ -411,20 +415,36 @@ __global__ __launch_bounds__(256, 2) void bgemm_kernel(GemmArgs p)
     for (int kt = kt0; kt < kt1; kt++) {
         const bool more = kt + 1 < kt1;
         if (more) load(kt + 1);
+        // a lane's 16-byte run: row l % 16 of the 16-row block, run l / 16 of the 4 runs a step reads
 #pragma unroll
-        for (int ks = 0; ks < BGK; ks += 32) {
-            // lane l holds A[l % 16][8 (l / 16) .. + 7] and B[8 (l / 16) .. + 7][l % 16] of a 16x16x32 step
-            const int kb = ks + 8 * (lane >> 4);
-            bf16x8_t af[4], bf[4];
+        for (int ks = 0; ks < SK; ks += 4 * EPR) {
+            const int kb = ks + EPR * (lane >> 4);
+            if constexpr (sizeof(E) == 2) {
+                // lane l holds A[l % 16][8 (l / 16) .. + 7] and B[8 (l / 16) .. + 7][l % 16]
+                bf16x8_t af[4], bf[4];
 #pragma unroll
-            for (int a = 0; a < 4; a++) __builtin_memcpy(&af[a], &As[wm * 64 + a * 16 + (lane & 15)][kb], 16);
+                for (int a = 0; a < 4; a++) __builtin_memcpy(&af[a], &As[wm * 64 + a * 16 + (lane & 15)][kb], 16);
 #pragma unroll
-            for (int b = 0; b < 4; b++) __builtin_memcpy(&bf[b], &Bs[wn * 64 + b * 16 + (lane & 15)][kb], 16);
+                for (int b = 0; b < 4; b++) __builtin_memcpy(&bf[b], &Bs[wn * 64 + b * 16 + (lane & 15)][kb], 16);
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+                for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < 4; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bf[b], acc[a][b], 0, 0, 0);
+                    for (int b = 0; b < 4; b++)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bf[b], acc[a][b], 0, 0, 0);
+            } else {
+                f32x4_t af[4], bf[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) __builtin_memcpy(&af[a], &As[wm * 64 + a * 16 + (lane & 15)][kb], 16);
+#pragma unroll
+                for (int b = 0; b < 4; b++) __builtin_memcpy(&bf[b], &Bs[wn * 64 + b * 16 + (lane & 15)][kb], 16);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int b = 0; b < 4; b++)
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a][j], bf[b][j], acc[a][b], 0, 0, 0);
+            }
         }
         __syncthreads();
         if (more) stage();
@@ -605,27 +625,33 @@ int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A,
     return dgemm(ctx, m, ncol, k, A, lda, B, ldb, C, ldc, accumulate, bt);
 }
 
-int bgemm_tn(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const uint16_t *AT, int64_t lda, const uint16_t *B,
-             int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt)
+int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, const void *AT, int64_t lda,
+              const void *B, int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt)
 {
     if (m <= 0 || n <= 0 || bt.n_inst <= 0) return EPSX_OK;
+    if (precision != EPSX_PREC_BF16 && precision != EPSX_PREC_FP32)
+        return set_error(ctx, EPSX_ERR_ARG, "low-precision product: precision must be BF16 or FP32");
     if (k % 8 || lda % 8 || ldb % 8 || bt.sA % 8 || bt.sB % 8)
-        return set_error(ctx, EPSX_ERR_ARG, "BF16 product: k, row pitches and strides must be multiples of 8");
+        return set_error(ctx, EPSX_ERR_ARG, "low-precision product: k, row pitches and strides must be multiples of 8");
     GemmArgs p{};
     p.m = m; p.n = n; p.k = k; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
     p.sA = bt.sA; p.sB = bt.sB; p.sC = bt.sC;
-    p.A = reinterpret_cast<const double *>(AT); p.B = reinterpret_cast<const double *>(B);
+    p.A = static_cast<const double *>(AT); p.B = static_cast<const double *>(B);
     p.C = C; p.Z = bt.Z; p.Zim = bt.Zim; p.diag = bt.diag;
     p.idx = bt.idx; p.gate = bt.gate; p.imap = bt.imap;
     p.accumulate = accumulate ? 1 : 0;
-    p.ktiles = (int)((k + BGK - 1) / BGK);
+    const int sk = precision == EPSX_PREC_BF16 ? 64 : 32;   // k per 128-byte stage row
+    p.ktiles = (int)((k + sk - 1) / sk);
     p.mt = (int)((m + 127) / 128);
     p.nt = (int)((n + 127) / 128);
     // 36 KB of LDS and 2 waves per SIMD: 2 workgroups per CU
     int S = 1, Smax = 1;
     EPSX_TRY(plan_splits(ctx, p, bt, (int64_t)p.mt * p.nt, 256 * 2, S, Smax));
     const int64_t grid = ((int64_t)p.mt * p.nt * p.units_per_tile + NXCD - 1) / NXCD * NXCD;
-    hipLaunchKernelGGL(bgemm_kernel, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
+    if (precision == EPSX_PREC_BF16)
+        hipLaunchKernelGGL(lpkgemm_kernel<unsigned short>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
+    else
+        hipLaunchKernelGGL(lpkgemm_kernel<float>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
     EPSX_HIP(ctx, hipGetLastError());
     if (Smax > 1) {
         hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),

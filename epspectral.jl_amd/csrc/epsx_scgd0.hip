@@ -365,16 +365,16 @@ struct Levels {
 };
 
 // The product gates of a mixed-precision run: gates[l][t] = active and at level l; row L.n: active
-// and needing the FP64 h of the star method (every level but native BF16).  act null: every
-// temperature active.
+// and needing the FP64 h of the star method (FP64, or every level if the low-precision products
+// stage FP64 operands).  act null: every temperature active.
 __global__ void phase_gates_kernel(int64_t nT, const int32_t *__restrict__ act, const int32_t *__restrict__ prec,
-                                   Levels L, int native_bf16, int32_t *__restrict__ gates)
+                                   Levels L, int lp_native, int32_t *__restrict__ gates)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= nT) return;
     const int a = act ? act[t] != 0 : 1;
     for (int l = 0; l < L.n; l++) gates[t + nT * l] = a && prec[t] == L.prec[l];
-    gates[t + nT * L.n] = a && !(native_bf16 && prec[t] == EPSX_PREC_BF16);
+    gates[t + nT * L.n] = a && !(lp_native && prec[t] != EPSX_PREC_FP64);
 }
 
 // Device-side bookkeeping of the graph-replayed loop: the history row of this iteration, the
@@ -439,7 +439,7 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     Scgd0Dev &s = ctx->s;
     s.ready = false;
     s.gt_ready = false;
-    s.gb_ready = false;
+    s.gb_ready = s.gf_ready = false;
     const int64_t nm = P->nm, ni = P->ni, nf = P->nf, nw = P->nw, nwa = P->nwa;
     if (nm < 1 || ni < 1 || nf < 1 || nw < 2 || nwa < 2 || P->nq < 1 || P->nki < 1 || P->nkf < 1)
         return set_error(ctx, EPSX_ERR_ARG, "scgd0 problem: empty dimension (nm=%lld ni=%lld nf=%lld nw=%lld nwa=%lld)",
@@ -657,13 +657,15 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // before any graph capture: a change of precision level captures new products
     EPSX_TRY(dev_alloc(ctx, ctx->gemm_ws, 8 * (size_t)nw * ni * 16 * nT));
     for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
-    Tmp hbuf, hocc, hb;
-    // BF16 low-precision phase of the star method: h and G̃ stored in BF16, k-contiguous with the
-    // row pitch kp (nm ni rounded up to a multiple of 8), multiplied by the BF16 MFMA kernel.
-    // EPSX_BF16_STAGED=1 rounds the FP64 operands as they are staged instead (for tests).
-    const char *staged_env = getenv("EPSX_BF16_STAGED");
-    const bool native_bf16 = method == EPSX_FM_STAR && par->precision == EPSX_PREC_BF16 &&
-                             !(staged_env && staged_env[0] == '1');
+    Tmp hbuf, hocc, hb, hf;
+    // Low-precision levels of the star method: h and G̃ stored in BF16 / FP32, k-contiguous with the
+    // row pitch kp (nm ni rounded up to a multiple of 8), multiplied by the low-precision MFMA
+    // kernel.  EPSX_LP_STAGED=1 rounds the FP64 operands as they are staged instead (for tests).
+    const char *staged_env = getenv("EPSX_LP_STAGED");
+    const bool lp_native = method == EPSX_FM_STAR && par->precision != EPSX_PREC_FP64 &&
+                           !(staged_env && staged_env[0] == '1');
+    const bool use_bf = lp_native && par->precision == EPSX_PREC_BF16;   // BF16 level present
+    const bool use_f32 = lp_native;                                       // FP32 level present
     const int64_t kp = (s.nm * ni + 7) / 8 * 8;
     if (method == EPSX_FM_STAR) {
         EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
@@ -690,15 +692,21 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             ctx->t_class[4] = ms * 1e-3;
             ctx->n_class[4] = 1;
             s.gt_ready = true;
-            s.gb_ready = false;
+            s.gb_ready = s.gf_ready = false;
             s.gt_wac = par->w_acoustic;
         }
-        if (native_bf16 && !s.gb_ready) {
+        if (use_bf && !s.gb_ready) {
             EPSX_TRY(dev_alloc(ctx, s.Gb, 2 * (size_t)kp * ni));
-            EPSX_TRY(gt_bf16_launch(ctx, s.nm * ni, kp, ni, dptr<double>(s.Gt), dptr<uint16_t>(s.Gb)));
+            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_BF16, s.nm * ni, kp, ni, dptr<double>(s.Gt), s.Gb.p));
             s.gb_ready = true;
         }
-        if (native_bf16) EPSX_TRY(tmp_alloc(ctx, hb.b, 2 * (size_t)kp * nw * nT));
+        if (use_f32 && !s.gf_ready) {
+            EPSX_TRY(dev_alloc(ctx, s.Gf, 4 * (size_t)kp * ni));
+            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_FP32, s.nm * ni, kp, ni, dptr<double>(s.Gt), s.Gf.p));
+            s.gf_ready = true;
+        }
+        if (use_bf) EPSX_TRY(tmp_alloc(ctx, hb.b, 2 * (size_t)kp * nw * nT));
+        if (use_f32) EPSX_TRY(tmp_alloc(ctx, hf.b, 4 * (size_t)kp * nw * nT));
     }
     const int nbmax = 512;
     EPSX_TRY(tmp_alloc(ctx, B.bmax.b, 8 * (size_t)nbmax * nT));
@@ -779,12 +787,13 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         struct Pass { const int32_t *gate; int prec; bool tune; };
         Pass passes[3] = {{d_act, EPSX_PREC_FP64, true}, {nullptr, 0, false}, {nullptr, 0, false}};
         int npass = 1;
-        // FP64 h (every level but native BF16) and BF16 h (native BF16) of the star method
-        bool need_h64 = true, need_hbf = false;
-        const int32_t *gate_h64 = d_act, *gate_hbf = nullptr;
+        // FP64 h (FP64 level, or staged low precision) and BF16 / FP32 h (native levels) of the star
+        // method
+        bool need_h64 = true;
+        const int32_t *gate_h64 = d_act, *gate_hbf = nullptr, *gate_hf = nullptr;
         if (split) {
             hipLaunchKernelGGL(phase_gates_kernel, dim3(nblk(nT)), dim3(256), 0, st, nT, d_act,
-                               d_prec.ptr<int32_t>(), L, native_bf16 ? 1 : 0, d_gates.ptr<int32_t>());
+                               d_prec.ptr<int32_t>(), L, lp_native ? 1 : 0, d_gates.ptr<int32_t>());
             EPSX_HIP(ctx, hipGetLastError());
             const int mask = present_levels();
             npass = 0;
@@ -794,12 +803,9 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 if (!(mask & (1 << l))) continue;
                 const int32_t *g = d_gates.ptr<int32_t>() + nT * l;
                 passes[npass++] = {g, L.prec[l], false};
-                if (native_bf16 && L.prec[l] == EPSX_PREC_BF16) {
-                    need_hbf = true;
-                    gate_hbf = g;
-                } else {
-                    need_h64 = true;
-                }
+                if (lp_native && L.prec[l] == EPSX_PREC_BF16) gate_hbf = g;
+                else if (lp_native && L.prec[l] == EPSX_PREC_FP32) gate_hf = g;
+                else need_h64 = true;
             }
         }
         FmTemps tp;
@@ -812,7 +818,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                                     dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(), B.T.ptr<double>(),
                                     B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA, par->w_acoustic,
                                     hocc.ptr<double>(), need_h64 ? hbuf.ptr<double>() : nullptr, d_act, gate_h64,
-                                    need_hbf ? hb.ptr<uint16_t>() : nullptr, kp, gate_hbf));
+                                    gate_hbf ? hb.ptr<uint16_t>() : nullptr, gate_hbf,
+                                    gate_hf ? hf.ptr<float>() : nullptr, gate_hf, kp));
             if (events) EPSX_HIP(ctx, hipEventRecord(ev[5], st));
             // ImΣ[:, :, t] = h_z G̃ for every slot z in one batched launch (G̃ shared)
             for (int q = 0; q < npass; q++) {
@@ -825,10 +832,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 bt.gate = passes[q].gate;
                 bt.precision = passes[q].prec;
                 bt.tune = passes[q].tune;
-                if (native_bf16 && passes[q].prec == EPSX_PREC_BF16) {
+                if (lp_native && passes[q].prec != EPSX_PREC_FP64) {
+                    const bool bf = passes[q].prec == EPSX_PREC_BF16;
                     bt.sA = kp * nw;
-                    EPSX_TRY(bgemm_tn(ctx, nw, ni, kp, hb.ptr<uint16_t>(), kp, dptr<uint16_t>(s.Gb), kp,
-                                      B.ImS.ptr<double>(), nw, false, bt));
+                    EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ni, kp, bf ? hb.b.p : hf.b.p, kp,
+                                       bf ? s.Gb.p : s.Gf.p, kp, B.ImS.ptr<double>(), nw, false, bt));
                 } else {
                     EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
                                    B.ImS.ptr<double>(), nw, false, bt));
