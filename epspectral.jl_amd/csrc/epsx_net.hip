@@ -56,28 +56,27 @@ __device__ inline unsigned short f2bf(float f)
 // B operand element (k, n):
 //   B_IM2COL:   k = c * KW + t, value X[c][n + dir (t - h)] (same sample)
 //   B_IM2COL_T: k = n', n = c * KW + t, value X[c][n' + t - h]
+// x is the grid point of the position (n, or n') within its sample, found by the caller.
 template <int BM>
-__device__ inline float b_elem(const NetGemm &g, int64_t k, int64_t n)
+__device__ inline float b_elem(const NetGemm &g, int64_t k, int64_t n, int x)
 {
     if (k >= g.K || n >= g.N) return 0.f;
     const int h = g.KW / 2;
-    int64_t c, pos;
-    int t;
-    if (BM == B_IM2COL) {
-        c = k / g.KW;
-        t = (int)(k - c * g.KW);
-        pos = n;
-        const int x = (int)(pos % g.L) + g.dir * (t - h);
-        if (x < 0 || x >= g.L) return 0.f;
-        return g.X[c * g.ldx + pos + g.dir * (t - h)];
-    } else {
-        c = n / g.KW;
-        t = (int)(n - c * g.KW);
-        pos = k;
-        const int x = (int)(pos % g.L) + (t - h);
-        if (x < 0 || x >= g.L) return 0.f;
-        return g.X[c * g.ldx + pos + (t - h)];
-    }
+    const int kn = BM == B_IM2COL ? (int)k : (int)n;   // c * KW + t (c * KW + t < 2^31)
+    const int c = kn / g.KW, t = kn - c * g.KW;
+    const int64_t pos = BM == B_IM2COL ? n : k;
+    const int d = (BM == B_IM2COL ? g.dir : 1) * (t - h);
+    if (x + d < 0 || x + d >= g.L) return 0.f;
+    return g.X[c * g.ldx + pos + d];
+}
+
+// The grid point (position mod L) of position base + i, 0 <= i < 64, from xb = base mod L: one
+// 64-bit remainder per tile instead of one per element.
+__device__ inline int grid_point(int xb, int i, int L)
+{
+    int x = xb + i;
+    if (L >= 64) return x >= L ? x - L : x;
+    return x % L;
 }
 
 // One 64x64 tile of C per workgroup and one split of k; 4 waves, each 32x32 as 2x2 MFMA tiles.
@@ -101,6 +100,7 @@ __global__ __launch_bounds__(NG_T) void net_gemm_kernel(NetGemm g)
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    const int xcol = (int)(col0 % g.L);   // grid point of the tile's first column (B_IM2COL)
     auto put = [](st_t &d, float x) {
         if constexpr (BF16) d = f2bf(x);
         else d = x;
@@ -119,14 +119,15 @@ __global__ __launch_bounds__(NG_T) void net_gemm_kernel(NetGemm g)
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int e = tid + NG_T * q, nn = e & 63, kk = e >> 6;
-                put(Bs[nn][kk], b_elem<BM>(g, k0 + kk, col0 + nn));
+                put(Bs[nn][kk], b_elem<BM>(g, k0 + kk, col0 + nn, grid_point(xcol, nn, g.L)));
             }
         } else {
             // consecutive threads along k (contiguous in X)
+            const int xk = (int)(k0 % g.L);
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int e = tid + NG_T * q, kk = e & 31, nn = e >> 5;
-                put(Bs[nn][kk], b_elem<BM>(g, k0 + kk, col0 + nn));
+                put(Bs[nn][kk], b_elem<BM>(g, k0 + kk, col0 + nn, grid_point(xk, kk, g.L)));
             }
         }
         __syncthreads();

@@ -785,7 +785,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[0], st));
         // the (gate, precision) passes of the products: one FP64 pass, or one per level present
         struct Pass { const int32_t *gate; int prec; bool tune; };
-        Pass passes[3] = {{d_act, EPSX_PREC_FP64, true}, {nullptr, 0, false}, {nullptr, 0, false}};
+        // (a run of one or two iterations does not pay for timing the GEMM tile shapes)
+        Pass passes[3] = {{d_act, EPSX_PREC_FP64, maxiter >= 3}, {nullptr, 0, false}, {nullptr, 0, false}};
         int npass = 1;
         // FP64 h (FP64 level, or staged low precision) and BF16 / FP32 h (native levels) of the star
         // method
