@@ -196,6 +196,7 @@ struct GemmBatch {
     int precision = EPSX_PREC_FP64;   // FP32 / BF16: operands rounded when staged, FP32 accumulation
     const double *diag = nullptr;     // with Z: add diag[i] Zim[i, j] in FP64
     bool tune = true;                 // the autotuner may time this product (its gate is not hiding work)
+    hipEvent_t mark = nullptr;        // recorded after the main kernel (before the split-k sum)
 };
 // C[m, n] = A[m, k] * B[k, n] (+ C if accumulate), FP64, column-major, on ctx->stream
 int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *B,

@@ -569,6 +569,7 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
         else
             hipLaunchKernelGGL(lpgemm_kernel<false>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
         EPSX_HIP(ctx, hipGetLastError());
+        if (bt.mark) EPSX_HIP(ctx, hipEventRecord(bt.mark, ctx->stream));
         if (Smax > 1) {
             hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
                                dim3(256), 0, ctx->stream, p);
@@ -609,6 +610,7 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
         ctx->gemm_tune[key] = c;
     }
     EPSX_TRY(launch_shape(ctx, c, p));
+    if (bt.mark) EPSX_HIP(ctx, hipEventRecord(bt.mark, ctx->stream));
     if (Smax > 1) {
         hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
                            dim3(256), 0, ctx->stream, p);
@@ -653,6 +655,7 @@ int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, con
     else
         hipLaunchKernelGGL(lpkgemm_kernel<float>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
     EPSX_HIP(ctx, hipGetLastError());
+    if (bt.mark) EPSX_HIP(ctx, hipEventRecord(bt.mark, ctx->stream));
     if (Smax > 1) {
         hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
                            dim3(256), 0, ctx->stream, p);
