@@ -40,6 +40,25 @@ def test_kk_matrix_and_apply(ctx):
     assert rel(epsx.kramers_kronig(g2, z2, ctx), oracle.kramers_kronig_matrix(np.asarray(g2)) @ z2) <= 1e-13
 
 
+def test_gemm_tile_shapes_give_the_same_bits(ctx):
+    """Every FP64 GEMM tile shape runs the same k-tiles in the same order, so each gives the same
+    bits: Kramers-Kronig on a 700-node grid applied to 300 columns (several
+    tiles and k splits), each shape forced in turn, against the oracle at 1e-13."""
+    g = FrequencyGrid(-1.0, 0.003, np.arange(700))
+    z = pseudorandom(700, 300, seed=3)
+    ref = oracle.kramers_kronig_matrix(np.asarray(g)) @ z
+    outs = []
+    for shape in range(5):
+        os.environ["EPSX_GEMM_SHAPE"] = str(shape)
+        try:
+            outs.append(epsx.kramers_kronig(g, z, ctx))
+        finally:
+            del os.environ["EPSX_GEMM_SHAPE"]
+    assert rel(outs[0], ref) <= 1e-13
+    for k, y in enumerate(outs[1:], 1):
+        assert np.array_equal(y, outs[0]), k
+
+
 def kernel_case():
     """The data of test/test_scgd0_kernel.jl:6-31."""
     nm, ni, nf = 3, 13, 17
