@@ -52,6 +52,7 @@ def main():
     ap.add_argument("--plain-iters", type=int, default=2, help="iterations of the plain method timed")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-levels", action="store_true", help="skip the FP32 / BF16 precision-level runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="scgd0", choices=("scgd0", "ladder", "warmstart"),
                     help="scgd0: the headline solve; ladder: the Kubo ladder on the 16³ model; "
@@ -180,6 +181,28 @@ def main():
             "max_abs_dmu_eV": float(np.max(np.abs(r["μ"] - r64["μ"]))),
             "iterations_low_precision": int(r["info"]["n_low_precision"]),
             "iterations_total": iters[-1], "iterations_fp64_solve": int(r64["history"]["err"].shape[1])}
+    if args.precision == "fp64" and not args.no_levels:
+        # the precision levels on the same resident problem: BF16 / FP32 products, then FP64 to the
+        # same tol (only the FP64 level freezes a temperature); errors against this FP64 result
+        levels = {}
+        for prec in ("fp32", "bf16"):
+            kl = dict(kw, precision=prec)
+            dev.loop(occ, **kl)   # warm (G̃ copies, shapes)
+            tl, rl = [], None
+            for k in range(2):
+                t0 = time.perf_counter()
+                rl = dev.loop(occ, **kl)
+                tl.append(time.perf_counter() - t0)
+            levels[prec] = {"value": sum(tl) / (2 * nT), "unit": "s/temperature",
+                            "speedup_vs_fp64": s_per_T / (sum(tl) / (2 * nT)),
+                            "iterations": int(rl["history"]["err"].shape[1]),
+                            "iterations_below_fp64": int(rl["info"]["n_low_precision"]),
+                            "converged": bool(rl["converged"].all()),
+                            "max_abs_dSigma_eV": float(np.max(np.abs(rl["Σ"] - r["Σ"]))),
+                            "max_rel_dSigma": float(np.max(np.abs(rl["Σ"] - r["Σ"])) / np.max(np.abs(r["Σ"]))),
+                            "max_abs_dmu_eV": float(np.max(np.abs(rl["μ"] - r["μ"])))}
+            print(f"[bench] precision {prec}: {levels[prec]}", file=sys.stderr, flush=True)
+        line["precision_levels"] = levels
     if not args.no_plain:
         line["plain_fp64_gpu"] = plain_sample(dev, occ, kw, iters[0], args.plain_iters, nw * nm * ni * nf, g2_bytes)
         line["speedup_vs_plain"] = line["plain_fp64_gpu"]["value"] / s_per_T
