@@ -311,7 +311,11 @@ def warmstart_bench(args):
         dev = epsx.Scgd0Device(setup, ctx)
         occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
         kw = dict(η_min=0.0, η_init=5e-3, ω_acoustic=1e-6, method="auto", batched=True)
-        dev.loop(occ, maxiter=3, tol=0.0, **kw)  # prime (kernels and the GEMM tile-shape choice)
+        try:
+            dev.loop(occ, maxiter=3, tol=0.0, **kw)  # prime (kernels and the GEMM tile-shape choice)
+        except epsx.ArgumentError:
+            print(f"[bench] held-out {h + 1}: no μ root for this random model, skipped", file=sys.stderr, flush=True)
+            continue
         t0 = time.perf_counter()
         rc = dev.loop(occ, maxiter=300, tol=1e-6, **kw)
         t_plain = time.perf_counter() - t0

@@ -67,61 +67,6 @@ __global__ void multiplet_kernel(int64_t nw, int64_t ni, int64_t n_mult, const i
     for (int64_t k = a; k < b; k++) St[w + nw * (int64_t)(idx[k] - 1)] = make_double2(sr, si);
 }
 
-// A[a, i, z] = -Im(inv(ωa - ε_i - Σ(ωa)))/π with Σ gathered by the spectral table
-// (src/scgd0.jl:304-305); the complex inverse is Julia's scaled (Smith) form.
-__global__ void dos_A_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *__restrict__ wsa,
-                             const double *__restrict__ e_i, const int32_t *__restrict__ jsp,
-                             const double *__restrict__ tsp, const int32_t *__restrict__ tidx,
-                             const double2 *__restrict__ Snew, double *__restrict__ A, const int32_t *__restrict__ act)
-{
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= nwa * ni) return;
-    const int z = blockIdx.y;
-    if (act && !act[tidx[z]]) return;
-    const int64_t a = q % nwa, i = q / nwa;
-    const double2 *St = Snew + nw * ni * (int64_t)tidx[z] + nw * i;
-    const int64_t j = jsp[a] - 1;
-    const double t = tsp[a];
-    const double2 lo = St[j], hi = St[j + 1];
-    const double sr = (1 - t) * lo.x + t * hi.x, si = (1 - t) * lo.y + t * hi.y;
-    const double c = wsa[a] - e_i[i] - sr, d = -si;
-    double im;
-    if (fabs(d) <= fabs(c)) {
-        const double r = d / c, zz = fma(d, r, c);
-        im = -r / zz;
-    } else {
-        const double r = c / d, zz = fma(-c, r, -d);
-        im = 1.0 / zz;
-    }
-    A[q + nwa * ni * (int64_t)z] = -im / PI_D;
-}
-
-// D[a, z] = (Σ_i A[a, i] w_i) Wω[a]  (src/scgd0.jl:306-307): a block per 64 spectral points, its
-// 16 waves over interleaved outer states, then the 16 partial sums in wave order.
-__global__ __launch_bounds__(1024) void dos_D_kernel(int64_t ni, int64_t nwa, const double *__restrict__ A,
-                                                     const double *__restrict__ w_i, const double *__restrict__ Wa,
-                                                     double *__restrict__ D, const int32_t *__restrict__ tidx,
-                                                     const int32_t *__restrict__ act)
-{
-    __shared__ double part[16][64];
-    const int z = blockIdx.y;
-    if (act && !act[tidx[z]]) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t a = blockIdx.x * 64 + lane;
-    const double *Az = A + nwa * ni * (int64_t)z;
-    double acc = 0;
-    if (a < nwa)
-        for (int64_t i = wave; i < ni; i += 16) acc = fma(Az[a + nwa * i], w_i[i], acc);
-    part[wave][lane] = acc;
-    __syncthreads();
-    if (wave == 0 && a < nwa) {
-        double s = 0;
-#pragma unroll
-        for (int w = 0; w < 16; w++) s += part[w][lane];
-        D[a + nwa * (int64_t)z] = s * Wa[a];
-    }
-}
-
 template <int NT>
 __device__ inline double block_sum(double v, double *red)
 {
@@ -138,20 +83,87 @@ __device__ inline double block_sum(double v, double *red)
     return r;
 }
 
-// ∫A_i = Σ_a A[a, i] Wω[a] for one outer state per block (src/scgd0.jl:308)
-__global__ __launch_bounds__(256) void dos_int_kernel(int64_t ni, int64_t nwa, const double *__restrict__ A,
-                                                      const double *__restrict__ Wa, double *__restrict__ integ,
-                                                      const int32_t *__restrict__ tidx, const int32_t *__restrict__ act)
+// The spectral DOS in one pass over (a, i) (src/scgd0.jl:304-308): A[a, i] = -Im(inv(ωa - ε_i -
+// Σ(ωa)))/π with Julia's scaled complex inverse, formed in registers and never stored.  One block
+// per (tile of 64 a, chunk of DOS_IC outer states, slot): lanes along a, the 4 waves over
+// i ≡ wave mod 4 of the chunk.  It writes the chunk's partial Dp[a, chunk] = Σ_i w_i A[a, i] (the
+// wave sums in order) and, per i, the tile's partial integral ip[i, tile] = Σ_a A[a, i] Wa[a] (a
+// wave butterfly); dos_finish_kernel sums both in order.
+constexpr int DOS_IC = 32;
+__global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *__restrict__ wsa,
+                                                  const double *__restrict__ e_i, const int32_t *__restrict__ jsp,
+                                                  const double *__restrict__ tsp, const int32_t *__restrict__ tidx,
+                                                  const double2 *__restrict__ Snew, const double *__restrict__ w_i,
+                                                  const double *__restrict__ Wa, double *__restrict__ Dp,
+                                                  double *__restrict__ ipart, const int32_t *__restrict__ act)
 {
-    __shared__ double red[256];
-    const int64_t i = blockIdx.x;
+    __shared__ double part[4][64];
+    const int z = blockIdx.z;
+    if (act && !act[tidx[z]]) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t a = blockIdx.x * 64 + lane, ntile = gridDim.x, nchunk = gridDim.y;
+    const int64_t i0 = (int64_t)blockIdx.y * DOS_IC, i1 = i0 + DOS_IC < ni ? i0 + DOS_IC : ni;
+    const bool on = a < nwa;
+    const double2 *S = Snew + nw * ni * (int64_t)tidx[z];
+    int64_t j = 0;
+    double t = 0, x = 0, wa = 0;
+    if (on) {
+        j = jsp[a] - 1;
+        t = tsp[a];
+        x = wsa[a];
+        wa = Wa[a];
+    }
+    double acc = 0;
+    for (int64_t i = i0 + wave; i < i1; i += 4) {
+        double A = 0;
+        if (on) {
+            const double2 *St = S + nw * i;
+            const double2 lo = St[j], hi = St[j + 1];
+            const double sr = (1 - t) * lo.x + t * hi.x, si = (1 - t) * lo.y + t * hi.y;
+            const double c = x - e_i[i] - sr, d = -si;
+            double im;
+            if (fabs(d) <= fabs(c)) {
+                const double r = d / c, zz = fma(d, r, c);
+                im = -r / zz;
+            } else {
+                const double r = c / d, zz = fma(-c, r, -d);
+                im = 1.0 / zz;
+            }
+            A = -im / PI_D;
+            acc = fma(A, w_i[i], acc);
+        }
+        double v = A * wa;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) ipart[i + ni * (blockIdx.x + ntile * (int64_t)z)] = v;
+    }
+    part[wave][lane] = acc;
+    __syncthreads();
+    if (wave == 0 && on)
+        Dp[a + nwa * (blockIdx.y + nchunk * (int64_t)z)] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+}
+
+// D[a] = Wa[a] Σ_chunk Dp[a, chunk] and ∫A dω of each outer state, integ[i] = Σ_tile ip[i, tile]
+// (both in order), for the active slots.
+__global__ __launch_bounds__(256) void dos_finish_kernel(int64_t nwa, int64_t nchunk, int64_t ni, int64_t ntile,
+                                                         const double *__restrict__ Dp, const double *__restrict__ ipart,
+                                                         const double *__restrict__ Wa, const int32_t *__restrict__ tidx,
+                                                         double *__restrict__ D, double *__restrict__ integ,
+                                                         const int32_t *__restrict__ act)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
     if (act && !act[tidx[z]]) return;
-    const double *Ai = A + nwa * ni * (int64_t)z + nwa * i;
-    double acc = 0;
-    for (int64_t a = threadIdx.x; a < nwa; a += 256) acc = fma(Ai[a], Wa[a], acc);
-    const double s = block_sum<256>(acc, red);
-    if (threadIdx.x == 0) integ[i + ni * (int64_t)z] = s;
+    if (q < nwa) {
+        double sum = 0;
+        for (int64_t c = 0; c < nchunk; c++) sum += Dp[q + nwa * (c + nchunk * (int64_t)z)];
+        D[q + nwa * (int64_t)z] = sum * Wa[q];
+    }
+    if (q < ni) {
+        double sum = 0;
+        for (int64_t k = 0; k < ntile; k++) sum += ipart[q + ni * (k + ntile * (int64_t)z)];
+        integ[q + ni * (int64_t)z] = sum;
+    }
 }
 
 __global__ __launch_bounds__(256) void min_kernel(int64_t ni, const double *__restrict__ integ,
@@ -593,7 +605,7 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
 namespace {
 
 struct RunBufs {
-    Tmp S, Snew, ImS, A, D, integ, part, bmax;
+    Tmp S, Snew, ImS, D, Dp, ipart, integ, part, bmax;
     Tmp T, mu, eta, nlist, err, minA, status, tidx;
 };
 
@@ -644,8 +656,9 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     EPSX_TRY(tmp_alloc(ctx, B.S.b, 16 * nS));
     EPSX_TRY(tmp_alloc(ctx, B.Snew.b, 16 * nS));
     EPSX_TRY(tmp_alloc(ctx, B.ImS.b, 8 * nS));
-    EPSX_TRY(tmp_alloc(ctx, B.A.b, 8 * (size_t)nwa * ni * nT));
     EPSX_TRY(tmp_alloc(ctx, B.D.b, 8 * (size_t)nwa * nT));
+    EPSX_TRY(tmp_alloc(ctx, B.Dp.b, 8 * (size_t)nwa * ((ni + DOS_IC - 1) / DOS_IC) * nT));   // per-chunk D partials
+    EPSX_TRY(tmp_alloc(ctx, B.ipart.b, 8 * (size_t)ni * ((nwa + 63) / 64) * nT));            // per-tile integrals
     EPSX_TRY(tmp_alloc(ctx, B.integ.b, 8 * (size_t)ni * nT));
     size_t npart = 0;  // the largest partial buffer any launch of this run needs
     for (int64_t a = 1; a <= (par->batched ? nT : 1); a++) {
@@ -895,15 +908,15 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             EPSX_HIP(ctx, hipGetLastError());
         }
         // 6. spectral DOS and min ∫A
-        hipLaunchKernelGGL(dos_A_kernel, dim3(nblk(nwa * ni), nA), dim3(256), 0, st, nw, ni, nwa,
-                           dptr<double>(s.wsa), dptr<double>(s.e_i), dptr<int32_t>(s.jsp), dptr<double>(s.tsp),
-                           d_tidx, B.Snew.ptr<double2>(), B.A.ptr<double>(), d_act);
+        const unsigned ntile = nblk(nwa, 64), nchunk = nblk(ni, DOS_IC);
+        hipLaunchKernelGGL(dos_kernel, dim3(ntile, nchunk, nA), dim3(256), 0, st, nw, ni, nwa, dptr<double>(s.wsa),
+                           dptr<double>(s.e_i), dptr<int32_t>(s.jsp), dptr<double>(s.tsp), d_tidx,
+                           B.Snew.ptr<double2>(), dptr<double>(s.w_i), dptr<double>(s.Wa), B.Dp.ptr<double>(),
+                           B.ipart.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(dos_D_kernel, dim3(nblk(nwa, 64), nA), dim3(1024), 0, st, ni, nwa, B.A.ptr<double>(),
-                           dptr<double>(s.w_i), dptr<double>(s.Wa), B.D.ptr<double>(), d_tidx, d_act);
-        EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(dos_int_kernel, dim3((unsigned)ni, nA), dim3(256), 0, st, ni, nwa, B.A.ptr<double>(),
-                           dptr<double>(s.Wa), B.integ.ptr<double>(), d_tidx, d_act);
+        hipLaunchKernelGGL(dos_finish_kernel, dim3(nblk(nwa > ni ? nwa : ni), nA), dim3(256), 0, st, nwa,
+                           (int64_t)nchunk, ni, (int64_t)ntile, B.Dp.ptr<double>(), B.ipart.ptr<double>(),
+                           dptr<double>(s.Wa), d_tidx, B.D.ptr<double>(), B.integ.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
                            B.minA.ptr<double>(), d_act);

@@ -164,7 +164,10 @@ def make_training_set(ctx, n_models, nk=12, length=1024, seed=0, states_per_mode
         setup, meta = random_t2g_model(rng, nk=nk, length=length)
         dev = Scgd0Device(setup, ctx)
         occ = Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
-        r1, rc = g0d0_and_converged(dev, occ)
+        try:
+            r1, rc = g0d0_and_converged(dev, occ)
+        except ArgumentError:
+            continue   # a model whose window holds no μ for the electron count: not a sample
         if not rc["converged"].all():
             continue
         Σ0, Σc = r1["Σ"][:, :, 0], rc["Σ"][:, :, 0]
