@@ -54,6 +54,8 @@ struct Scgd0Dev {
     double gt_wac = 0;                // ... with modes below this ω_acoustic skipped
     DevBuf Gb, Gf;                    // G̃ in BF16 / FP32, k-contiguous rows of pitch kp (on demand)
     bool gb_ready = false, gf_ready = false;   // they hold the current G̃
+    DevBuf KKb, KKf;                  // off-diagonal KK transposed in BF16 / FP32, pitch nw rounded to 8
+    bool kkb_ready = false, kkf_ready = false;
     DevBuf chunks, skip, wnu;         // star chunks of the reduction, acoustic flags, ω_ν
     DevBuf hk, ht;                    // cells and weights of ω ± ω_ν (Einstein modes)
     int64_t nchunks = 0;
@@ -153,6 +155,11 @@ void grid_nodes(int64_t n, double origin, double unit, const int64_t *offsets, d
 // Kernel launchers (epsx_kk.hip)
 int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K);
 int kk_split(epsx_ctx *ctx, int64_t n, const double *d_K, double *d_Ko, double *d_Kd);
+// the off-diagonal KK transposed into a BF16 / FP32 k-contiguous operand of row pitch np
+int kk_lp_build(epsx_ctx *ctx, int precision, int64_t n, int64_t np, const double *d_Ko, void *out);
+// columns of the gated slots in BF16 / FP32 with rows padded to np (zero beyond nrow)
+int lp_cols(epsx_ctx *ctx, int precision, int64_t nrow, int64_t np, int64_t ncol, const double *src, int64_t ld,
+            int64_t sstride, void *dst, int64_t dstride, int nslots, const int32_t *tidx, const int32_t *gate);
 
 // Kernel time of a stretch of the stream added to ctx->t_class[k] (HIP events; stop() waits for
 // the end event).  Not for use under graph capture.

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "epsx_common.h"
+#include "epsx_device.h"
 
 namespace epsx {
 namespace {
@@ -40,7 +41,63 @@ __global__ void kk_split_kernel(int64_t n, const double *__restrict__ K, double 
     if (i == j) Kd[i] = K[q];
 }
 
+// The off-diagonal KK in low precision, transposed for the k-contiguous product:
+// out[k + np m] = Ko[m + n k] (zero for n <= k < np), E = BF16 bits or float.
+template <typename E>
+__global__ void kk_lp_kernel(int64_t n, int64_t np, const double *__restrict__ Ko, E *__restrict__ out)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= np * n) return;
+    const int64_t k = q % np, m = q / np;
+    const double v = k < n ? Ko[m + n * k] : 0.0;
+    if constexpr (sizeof(E) == 2) out[q] = to_bf16(v);
+    else out[q] = (float)v;
+}
+
+// Columns of the slots at one precision level in low precision, rows padded to np:
+// dst[r + np c + dstride t] = src[r + ld c + sstride t] (zero for nrow <= r < np), t = tidx[z].
+template <typename E>
+__global__ void lp_cols_kernel(int64_t nrow, int64_t np, int64_t ncol, const double *__restrict__ src, int64_t ld,
+                               int64_t sstride, E *__restrict__ dst, int64_t dstride, const int32_t *__restrict__ tidx,
+                               const int32_t *__restrict__ gate)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (q >= np * ncol) return;
+    const int64_t t = tidx[z];
+    if (gate && !gate[t]) return;
+    const int64_t r = q % np, c = q / np;
+    const double v = r < nrow ? src[r + ld * c + sstride * t] : 0.0;
+    if constexpr (sizeof(E) == 2) dst[q + dstride * t] = to_bf16(v);
+    else dst[q + dstride * t] = (float)v;
+}
+
 }  // namespace
+
+int kk_lp_build(epsx_ctx *ctx, int precision, int64_t n, int64_t np, const double *d_Ko, void *out)
+{
+    const dim3 grid((unsigned)((np * n + 255) / 256)), blk(256);
+    if (precision == EPSX_PREC_BF16)
+        hipLaunchKernelGGL(kk_lp_kernel<uint16_t>, grid, blk, 0, ctx->stream, n, np, d_Ko, static_cast<uint16_t *>(out));
+    else
+        hipLaunchKernelGGL(kk_lp_kernel<float>, grid, blk, 0, ctx->stream, n, np, d_Ko, static_cast<float *>(out));
+    EPSX_HIP(ctx, hipGetLastError());
+    return EPSX_OK;
+}
+
+int lp_cols(epsx_ctx *ctx, int precision, int64_t nrow, int64_t np, int64_t ncol, const double *src, int64_t ld,
+            int64_t sstride, void *dst, int64_t dstride, int nslots, const int32_t *tidx, const int32_t *gate)
+{
+    const dim3 grid((unsigned)((np * ncol + 255) / 256), (unsigned)nslots), blk(256);
+    if (precision == EPSX_PREC_BF16)
+        hipLaunchKernelGGL(lp_cols_kernel<uint16_t>, grid, blk, 0, ctx->stream, nrow, np, ncol, src, ld, sstride,
+                           static_cast<uint16_t *>(dst), dstride, tidx, gate);
+    else
+        hipLaunchKernelGGL(lp_cols_kernel<float>, grid, blk, 0, ctx->stream, nrow, np, ncol, src, ld, sstride,
+                           static_cast<float *>(dst), dstride, tidx, gate);
+    EPSX_HIP(ctx, hipGetLastError());
+    return EPSX_OK;
+}
 
 int kk_split(epsx_ctx *ctx, int64_t n, const double *d_K, double *d_Ko, double *d_Kd)
 {

@@ -452,6 +452,7 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     s.ready = false;
     s.gt_ready = false;
     s.gb_ready = s.gf_ready = false;
+    s.kkb_ready = s.kkf_ready = false;
     const int64_t nm = P->nm, ni = P->ni, nf = P->nf, nw = P->nw, nwa = P->nwa;
     if (nm < 1 || ni < 1 || nf < 1 || nw < 2 || nwa < 2 || P->nq < 1 || P->nki < 1 || P->nkf < 1)
         return set_error(ctx, EPSX_ERR_ARG, "scgd0 problem: empty dimension (nm=%lld ni=%lld nf=%lld nw=%lld nwa=%lld)",
@@ -670,7 +671,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // before any graph capture: a change of precision level captures new products
     EPSX_TRY(dev_alloc(ctx, ctx->gemm_ws, 8 * (size_t)nw * ni * 16 * nT));
     for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
-    Tmp hbuf, hocc, hb, hf;
+    Tmp hbuf, hocc, hb, hf, imb, imf;
     // Low-precision levels of the star method: h and G̃ stored in BF16 / FP32, k-contiguous with the
     // row pitch kp (nm ni rounded up to a multiple of 8), multiplied by the low-precision MFMA
     // kernel.  EPSX_LP_STAGED=1 rounds the FP64 operands as they are staged instead (for tests).
@@ -680,6 +681,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     const bool use_bf = lp_native && par->precision == EPSX_PREC_BF16;   // BF16 level present
     const bool use_f32 = lp_native;                                       // FP32 level present
     const int64_t kp = (s.nm * ni + 7) / 8 * 8;
+    const int64_t kpw = (nw + 7) / 8 * 8;
     if (method == EPSX_FM_STAR) {
         EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
         EPSX_TRY(tmp_alloc(ctx, hocc.b, 16 * (size_t)nw * s.nm * nT));
@@ -720,6 +722,20 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         }
         if (use_bf) EPSX_TRY(tmp_alloc(ctx, hb.b, 2 * (size_t)kp * nw * nT));
         if (use_f32) EPSX_TRY(tmp_alloc(ctx, hf.b, 4 * (size_t)kp * nw * nT));
+        // Kramers-Kronig's off-diagonal at the low-precision levels: KKo transposed once, ImΣ
+        // converted per iteration (rows padded to kpw)
+        if (use_bf && !s.kkb_ready) {
+            EPSX_TRY(dev_alloc(ctx, s.KKb, 2 * (size_t)kpw * nw));
+            EPSX_TRY(kk_lp_build(ctx, EPSX_PREC_BF16, nw, kpw, dptr<double>(s.KKo), s.KKb.p));
+            s.kkb_ready = true;
+        }
+        if (use_f32 && !s.kkf_ready) {
+            EPSX_TRY(dev_alloc(ctx, s.KKf, 4 * (size_t)kpw * nw));
+            EPSX_TRY(kk_lp_build(ctx, EPSX_PREC_FP32, nw, kpw, dptr<double>(s.KKo), s.KKf.p));
+            s.kkf_ready = true;
+        }
+        if (use_bf) EPSX_TRY(tmp_alloc(ctx, imb.b, 2 * (size_t)kpw * ni * nT));
+        if (use_f32) EPSX_TRY(tmp_alloc(ctx, imf.b, 4 * (size_t)kpw * ni * nT));
     }
     const int nbmax = 512;
     EPSX_TRY(tmp_alloc(ctx, B.bmax.b, 8 * (size_t)nbmax * nT));
@@ -892,6 +908,17 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             bt.Zim = B.ImS.ptr<double>();
             if (passes[q].prec == EPSX_PREC_FP64) {
                 EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
+            } else if (lp_native) {
+                // the off-diagonal part from stored BF16 / FP32 operands, the principal-value diagonal
+                // in FP64 (the epilogue)
+                const bool bf = passes[q].prec == EPSX_PREC_BF16;
+                void *im = bf ? imb.b.p : imf.b.p;
+                EPSX_TRY(lp_cols(ctx, passes[q].prec, nw, kpw, ni, B.ImS.ptr<double>(), nw, nw * ni, im, kpw * ni, nA,
+                                 d_tidx, passes[q].gate));
+                bt.sB = kpw * ni;
+                bt.diag = dptr<double>(s.KKd);
+                EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ni, kpw, bf ? s.KKb.p : s.KKf.p, kpw, im, kpw, nullptr, nw,
+                                   false, bt));
             } else {
                 // the off-diagonal part in low precision, the principal-value diagonal in FP64
                 bt.precision = passes[q].prec;

@@ -340,10 +340,11 @@ def test_low_precision_phase_then_fp64_sweep(ctx, prec, method):
 
 @pytest.mark.parametrize("prec", ["bf16", "fp32"])
 def test_low_precision_star_product_native_matches_staged(ctx, prec):
-    """The BF16 / FP32 star product from operands stored k-contiguous in that precision (h built in
-    it, G̃ converted once) against the product that rounds the FP64 operands as it stages them: the
-    same rounded operands, so the two agree to FP32 accumulation order (1e-5 relative), while both
-    differ from FP64 by the operand rounding.  One iteration, in the low-precision phase."""
+    """The BF16 / FP32 star product and Kramers-Kronig off-diagonal from operands stored k-contiguous
+    in that precision (h built in it, G̃ and KK converted once, ImΣ per iteration) against the
+    products that round the FP64 operands as they stage them: the same rounded operands, so the two
+    agree to FP32 accumulation order (1e-5 relative), while both differ from FP64 by the operand
+    rounding.  One iteration, in the low-precision phase."""
     s, meta = t2g_setup(nk=8, Tlist_K=(100.0, 300.0), grid_windows=((0.3, 4e-3), (1.2, 16e-3), (3.0, 64e-3)))
     o = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
     kw = dict(η_min=0.0, η_init=5e-2, maxiter=1, tol=0.0, ω_acoustic=1e-6, method="star", fix_μ=True)
@@ -361,5 +362,8 @@ def test_low_precision_star_product_native_matches_staged(ctx, prec):
     d_ns = rel(rn["Σ"].imag, rs["Σ"].imag)
     d_64 = rel(rn["Σ"].imag, r64["Σ"].imag)
     assert d_ns <= (1e-5 if prec == "bf16" else 1e-6), d_ns
+    # Re Σ: Kramers-Kronig's off-diagonal from the stored low-precision operands as well
+    d_re = rel(rn["Σ"].real, rs["Σ"].real)
+    assert d_re <= (1e-5 if prec == "bf16" else 1e-6), d_re
     lo, hi = (1e-7, 1e-2) if prec == "bf16" else (1e-12, 1e-5)
     assert lo < d_64 < hi, d_64
