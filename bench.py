@@ -278,7 +278,45 @@ def ladder_bench(args):
                                    "sigma_bubble_max_rel": relerr(out["σ_bubble"], ref["σ_bubble"]),
                                    "reference_niter": ref["niter"][:, 0].tolist()},
     }
+    if not args.no_cpu:
+        apps_per_solve = tm["applications"] / len(Ωs)
+        line["cpu_baseline"] = ladder_cpu_baseline(nw, ni, ns, apps_per_solve, args.cpu_seconds)
     print(json.dumps(line), flush=True)
+
+
+def ladder_cpu_baseline(nw, ni, ns, apps_per_solve, budget_s):
+    """The oracle's apply_ladder_kernel! (the reference algorithm restated in C, OpenMP) timed once on
+    the same model at 4³, 6³ or 8³ (the same grid and shifts; the first whose application takes at
+    least budget_s / 4, or 8³), scaled by the multiply-adds of its dominant product C± += X B̃
+    (2 x 16 nω x 6 ni x 3 nshift ni) to the measured size, times the applications per (Ω, T) solve
+    of the GPU run."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from epsx.ladder import setup_ladder
+    from epsx.models import t2g_ladder_setup
+
+    threads = min(os.cpu_count() or 1, 16)
+    rng = np.random.default_rng(5)
+    for nk in (4, 6, 8):
+        setup, R, v, meta = t2g_ladder_setup(nk=nk, Tlist_K=(100.0,))
+        L = setup_ladder(setup, R, v)
+        nw_s, ni_s = len(setup.ωs), setup.dims[1]
+        shape = (nw_s, 4, 3, 2, ni_s)
+        J = np.asfortranarray(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+        G = np.asfortranarray(rng.standard_normal((nw_s, 4, ni_s)) + 1j * rng.standard_normal((nw_s, 4, ni_s)))
+        t0 = time.perf_counter()
+        oracle.apply_ladder_kernel(L, J, G, G, float(meta["Tlist"][0]), 1e-6, nthreads=threads)
+        el = time.perf_counter() - t0
+        if el >= budget_s / 4:
+            break
+    macs_s = 2.0 * 16 * nw_s * 6 * ni_s * 3 * L.nshift * ni_s
+    macs = 2.0 * 16 * nw * 6 * ni * 3 * ns * ni
+    per_app = el * macs / macs_s
+    return {"value": per_app * apps_per_solve, "unit": "s/(Ω,T)", "cores": threads, "kind": "port",
+            "sample": f"oracle apply_ladder_kernel (C, OpenMP) on the {nk}³ model (ni = {ni_s}, nω = {nw_s}, "
+                      f"nshift = {L.nshift}): {el:.1f} s; scaled by the multiply-adds of C± += X B̃ to ni = {ni} "
+                      f"({per_app:.0f} s per application) x {apps_per_solve:g} applications per solve",
+            "macs_per_s": macs_s / el}
 
 
 def warmstart_bench(args):

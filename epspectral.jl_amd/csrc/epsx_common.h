@@ -72,9 +72,10 @@ struct LadderDev {
     DevBuf Rs, v, Ww, klo, wlo, whi;  // rotations in star order, velocities, weights, shift tables
     DevBuf Bt;                        // B̃ of the current temperature
     double bt_T = -1, bt_wac = -1;
-    int32_t bt_prec = -1;
-    // work buffers
-    DevBuf Gm, Gp, Jm, W, Cp, Cm, Y, KY, S, jm, tm, jp, tp, Yc, red, Bt32, Jm32;
+    DevBuf Btl;                       // B̃ in BF16 / FP32, rows of pitch 3 ni rounded to 8 (on demand)
+    int32_t btl_prec = -1;            // the precision Btl holds for the current B̃ (-1: none)
+    // work buffers (JmL: the vertex as the transposed low-precision operand)
+    DevBuf Gm, Gp, Jm, JmL, W, Cp, Cm, Y, KY, S, jm, tm, jp, tp, Yc, red;
 };
 
 constexpr int NET_MAX_LAYERS = 16;

@@ -13,9 +13,11 @@
 // followed by a two-tap shift-add.  B̃ depends only on T and is built once per temperature.
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <vector>
 
 #include "epsx_common.h"
+#include "epsx_device.h"
 
 namespace epsx {
 namespace {
@@ -193,6 +195,51 @@ __global__ void jperm_kernel(int64_t nw, int64_t ni, const double2 *__restrict__
     const int64_t b = k % 3, ip = k / 3;
     const double2 z = J[JIX(j, c12, b, c0, ip)];
     Jm[q] = ri == 0 ? z.x : z.y;
+}
+
+// The vertex as the transposed low-precision A operand of the shift-table product:
+// JmL[k + kp r] = Jm[r, k] in BF16 / FP32 (zero for 3 ni <= k < kp), through a 64 x 64 LDS tile so
+// both the reads (along ω) and the writes (along k) are contiguous.
+template <typename E>
+__global__ __launch_bounds__(256) void jperm_lp_kernel(int64_t nw, int64_t ni, int64_t kp, const double2 *__restrict__ J,
+                                                       E *__restrict__ JmL)
+{
+    __shared__ double tile[64][65];
+    const int64_t nrow = 16 * nw, K = 3 * ni;
+    const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+    const int lx = threadIdx.x & 63;
+    for (int sk = threadIdx.x >> 6; sk < 64; sk += 4) {
+        const int64_t r = r0 + lx, k = k0 + sk;
+        double v = 0.0;
+        if (r < nrow && k < K) {
+            const int64_t j = r % nw, c12 = (r / nw) % 4, c0 = (r / (4 * nw)) % 2, ri = r / (8 * nw);
+            const double2 z = J[JIX(j, c12, k % 3, c0, k / 3)];
+            v = ri == 0 ? z.x : z.y;
+        }
+        tile[sk][lx] = v;
+    }
+    __syncthreads();
+    for (int sr = threadIdx.x >> 6; sr < 64; sr += 4) {
+        const int64_t r = r0 + sr, k = k0 + lx;
+        if (r < nrow && k < kp) {
+            const double v = tile[lx][sr];
+            if constexpr (sizeof(E) == 2) JmL[k + kp * r] = to_bf16(v);
+            else JmL[k + kp * r] = (float)v;
+        }
+    }
+}
+
+// B̃ in BF16 / FP32 with the row pitch of the low-precision product: out[k + kp c] = Bt[k + K c]
+// (zero for K <= k < kp).
+template <typename E>
+__global__ void bt_lp_kernel(int64_t K, int64_t kp, int64_t ncols, const double *__restrict__ Bt, E *__restrict__ out)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= kp * ncols) return;
+    const int64_t k = q % kp, c = q / kp;
+    const double v = k < K ? Bt[k + K * c] : 0.0;
+    if constexpr (sizeof(E) == 2) out[q] = to_bf16(v);
+    else out[q] = (float)v;
 }
 
 // C_s[(j, q), col] = Σ_m wlo W_m[(klo - 1, q), col] + whi W_m[(klo, q), col], both signs
@@ -431,6 +478,7 @@ static int ensure_Bt(epsx_ctx *ctx, double T, double w_ac)
     EPSX_HIP(ctx, hipGetLastError());
     l.bt_T = T;
     l.bt_wac = w_ac;
+    l.btl_prec = -1;
     return tm.stop(1);
 }
 
@@ -442,7 +490,36 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int pr
     const int64_t nw = s.nw, ni = s.ni, nrow = 16 * nw, ncol = 6 * ni, K = 3 * ni;
     const hipStream_t st = ctx->stream;
     ClassTimer t_apply(ctx, 1);
-    hipLaunchKernelGGL(jperm_kernel, dim3(nb(nrow * K)), dim3(256), 0, st, nw, ni, J, dptr<double>(l.Jm));
+    // Low precision: B̃ converted once per temperature and the vertex transposed per application into
+    // BF16 / FP32 k-contiguous operands (row pitch kp) for the low-precision MFMA kernel;
+    // EPSX_LP_STAGED=1 rounds the FP64 operands as they are staged instead (for tests).
+    const char *staged_env = getenv("EPSX_LP_STAGED");
+    const bool native = precision != EPSX_PREC_FP64 && !(staged_env && staged_env[0] == '1');
+    const bool bf = precision == EPSX_PREC_BF16;
+    const int64_t kp = (K + 7) / 8 * 8, esz = bf ? 2 : 4;
+    if (native) {
+        if (l.btl_prec != precision) {
+            const int64_t ncols = ncol * l.ns;
+            EPSX_TRY(dev_alloc(ctx, l.Btl, esz * (size_t)kp * ncols));
+            const dim3 grid((unsigned)nb(kp * ncols));
+            if (bf)
+                hipLaunchKernelGGL(bt_lp_kernel<uint16_t>, grid, dim3(256), 0, st, K, kp, ncols, dptr<double>(l.Bt),
+                                   dptr<uint16_t>(l.Btl));
+            else
+                hipLaunchKernelGGL(bt_lp_kernel<float>, grid, dim3(256), 0, st, K, kp, ncols, dptr<double>(l.Bt),
+                                   dptr<float>(l.Btl));
+            EPSX_HIP(ctx, hipGetLastError());
+            l.btl_prec = precision;
+        }
+        EPSX_TRY(dev_alloc(ctx, l.JmL, esz * (size_t)kp * nrow));
+        const dim3 grid((unsigned)((nrow + 63) / 64), (unsigned)((kp + 63) / 64));
+        if (bf)
+            hipLaunchKernelGGL(jperm_lp_kernel<uint16_t>, grid, dim3(256), 0, st, nw, ni, kp, J, dptr<uint16_t>(l.JmL));
+        else
+            hipLaunchKernelGGL(jperm_lp_kernel<float>, grid, dim3(256), 0, st, nw, ni, kp, J, dptr<float>(l.JmL));
+    } else {
+        hipLaunchKernelGGL(jperm_kernel, dim3(nb(nrow * K)), dim3(256), 0, st, nw, ni, J, dptr<double>(l.Jm));
+    }
     EPSX_HIP(ctx, hipGetLastError());
     // W for chunks of shifts, each followed by its shift-add
     const size_t per_m = 8 * (size_t)nrow * ncol;
@@ -453,9 +530,15 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int pr
         const int64_t mc = m0 + mb <= l.ns ? mb : l.ns - m0;
         ClassTimer t_gemm(ctx, 0);
         GemmBatch bt;
-        bt.precision = precision;   // FP32 / BF16 operands with FP32 accumulation, or FP64
-        EPSX_TRY(dgemm(ctx, nrow, ncol * mc, K, dptr<double>(l.Jm), nrow, dptr<double>(l.Bt) + K * ncol * m0, K,
-                       dptr<double>(l.W), nrow, false, bt));
+        if (native) {
+            const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
+            EPSX_TRY(lpgemm_tn(ctx, precision, nrow, ncol * mc, kp, l.JmL.p, kp, Bl, kp, dptr<double>(l.W), nrow,
+                               false, bt));
+        } else {
+            bt.precision = precision;   // FP32 / BF16 operands rounded as staged, FP32 accumulation, or FP64
+            EPSX_TRY(dgemm(ctx, nrow, ncol * mc, K, dptr<double>(l.Jm), nrow, dptr<double>(l.Bt) + K * ncol * m0, K,
+                           dptr<double>(l.W), nrow, false, bt));
+        }
         EPSX_TRY(t_gemm.stop(1));
         hipLaunchKernelGGL(shift_add_chunk_kernel, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns, m0, mc,
                            dptr<double>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),

@@ -5,6 +5,7 @@ epsx_ladder.hip, so the sum order differs); σ and J of full solves 1e-10 relati
 reference's CPU-vs-GPU gate, test/test_ladder_gpu.jl:46-48) with equal iteration counts;
 bitwise run to run.
 """
+import os
 import warnings
 
 import numpy as np
@@ -125,6 +126,14 @@ def test_low_precision_shift_gemm(ctx, prec, tol_apply):
     ref = ld.apply(J, Gm, Gp, 0.1, 1e-6)
     lp = ld.apply(J, Gm, Gp, 0.1, 1e-6, precision=prec)
     assert rel(lp, ref) <= tol_apply
+    # from B̃ and the vertex stored in BF16 / FP32 (k-contiguous) against the product that rounds
+    # the FP64 operands as it stages them: the same rounded operands, FP32 accumulation order apart
+    os.environ["EPSX_LP_STAGED"] = "1"
+    try:
+        staged = ld.apply(J, Gm, Gp, 0.1, 1e-6, precision=prec)
+    finally:
+        del os.environ["EPSX_LP_STAGED"]
+    assert rel(lp, staged) <= (1e-5 if prec == "bf16" else 1e-6)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         tight = ld.loop(r["Σ"], r["μ"], [0.1], [0.3], rtol=1e-8, maxiter=60)
