@@ -88,8 +88,8 @@ __device__ inline double block_sum(double v, double *red)
 // per (tile of 64 a, chunk of DOS_IC outer states, slot): lanes along a, the 4 waves over
 // i ≡ wave mod 4 of the chunk.  It writes the chunk's partial Dp[a, chunk] = Σ_i w_i A[a, i] (the
 // wave sums in order) and, per i, the tile's partial integral ip[i, tile] = Σ_a A[a, i] Wa[a] (a
-// wave butterfly); dos_finish_kernel sums both in order.
-constexpr int DOS_IC = 32;
+// wave butterfly); dos_finish_kernel and integ_kernel sum them in order.
+constexpr int DOS_IC = 128;
 __global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *__restrict__ wsa,
                                                   const double *__restrict__ e_i, const int32_t *__restrict__ jsp,
                                                   const double *__restrict__ tsp, const int32_t *__restrict__ tidx,
@@ -143,27 +143,34 @@ __global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_
         Dp[a + nwa * (blockIdx.y + nchunk * (int64_t)z)] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
-// D[a] = Wa[a] Σ_chunk Dp[a, chunk] and ∫A dω of each outer state, integ[i] = Σ_tile ip[i, tile]
-// (both in order), for the active slots.
-__global__ __launch_bounds__(256) void dos_finish_kernel(int64_t nwa, int64_t nchunk, int64_t ni, int64_t ntile,
-                                                         const double *__restrict__ Dp, const double *__restrict__ ipart,
+// D[a] = Wa[a] Σ_chunk Dp[a, chunk] (in chunk order) for the active slots.
+__global__ __launch_bounds__(256) void dos_finish_kernel(int64_t nwa, int64_t nchunk, const double *__restrict__ Dp,
                                                          const double *__restrict__ Wa, const int32_t *__restrict__ tidx,
-                                                         double *__restrict__ D, double *__restrict__ integ,
-                                                         const int32_t *__restrict__ act)
+                                                         double *__restrict__ D, const int32_t *__restrict__ act)
 {
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
-    if (act && !act[tidx[z]]) return;
-    if (q < nwa) {
-        double sum = 0;
-        for (int64_t c = 0; c < nchunk; c++) sum += Dp[q + nwa * (c + nchunk * (int64_t)z)];
-        D[q + nwa * (int64_t)z] = sum * Wa[q];
-    }
-    if (q < ni) {
-        double sum = 0;
-        for (int64_t k = 0; k < ntile; k++) sum += ipart[q + ni * (k + ntile * (int64_t)z)];
-        integ[q + ni * (int64_t)z] = sum;
-    }
+    if (a >= nwa || (act && !act[tidx[z]])) return;
+    double sum = 0;
+    for (int64_t c = 0; c < nchunk; c++) sum += Dp[a + nwa * (c + nchunk * (int64_t)z)];
+    D[a + nwa * (int64_t)z] = sum * Wa[a];
+}
+
+// ∫A dω of each outer state from the tile partials: one wave per (i, slot), lanes over the tiles
+// (lane-strided, then a butterfly: a fixed order).
+__global__ __launch_bounds__(256) void integ_kernel(int64_t ni, int64_t ntile, const double *__restrict__ ipart,
+                                                    const int32_t *__restrict__ tidx, double *__restrict__ integ,
+                                                    const int32_t *__restrict__ act)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.y;
+    if (i >= ni || (act && !act[tidx[z]])) return;
+    double v = 0;
+    for (int64_t k = lane; k < ntile; k += 64) v += ipart[i + ni * (k + ntile * (int64_t)z)];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) integ[i + ni * (int64_t)z] = v;
 }
 
 __global__ __launch_bounds__(256) void min_kernel(int64_t ni, const double *__restrict__ integ,
@@ -941,9 +948,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                            B.Snew.ptr<double2>(), dptr<double>(s.w_i), dptr<double>(s.Wa), B.Dp.ptr<double>(),
                            B.ipart.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(dos_finish_kernel, dim3(nblk(nwa > ni ? nwa : ni), nA), dim3(256), 0, st, nwa,
-                           (int64_t)nchunk, ni, (int64_t)ntile, B.Dp.ptr<double>(), B.ipart.ptr<double>(),
-                           dptr<double>(s.Wa), d_tidx, B.D.ptr<double>(), B.integ.ptr<double>(), d_act);
+        hipLaunchKernelGGL(dos_finish_kernel, dim3(nblk(nwa), nA), dim3(256), 0, st, nwa, (int64_t)nchunk,
+                           B.Dp.ptr<double>(), dptr<double>(s.Wa), d_tidx, B.D.ptr<double>(), d_act);
+        EPSX_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(integ_kernel, dim3(nblk(ni, 4), nA), dim3(256), 0, st, ni, (int64_t)ntile,
+                           B.ipart.ptr<double>(), d_tidx, B.integ.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
         hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
                            B.minA.ptr<double>(), d_act);

@@ -314,6 +314,22 @@ def test_star_method_matches_direct_and_oracle(ctx):
     assert r2["info"]["fm_method"] == 0
 
 
+def test_star_method_holstein_fine_grid(ctx):
+    """The star method on the Holstein chain (one Einstein mode ω_ph = 0.2, inner states k and
+    their representatives min(j, N - j)) on a 2 meV grid, where ω ± ω_ph lands 100 cells away:
+    against the direct kernel, 1e-12."""
+    s = holstein(nk=32, nodes=np.arange(-1250, 1251) * 0.002)
+    o = occ([0.1])
+    kw = dict(η_min=0.0, η_init=0.05, maxiter=3, tol=0.0, ω_acoustic=1e-6, fix_μ=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dev = epsx.Scgd0Device(s, ctx)
+        rs = dev.loop(o, method="star", **kw)
+        rd = dev.loop(o, method="direct", **kw)
+    assert rs["info"]["fm_method"] == 2
+    assert rel(rs["Σ"], rd["Σ"]) <= 1e-12
+
+
 @pytest.mark.parametrize("prec,method", [("fp32", "star"), ("bf16", "star"), ("fp32", "direct")])
 def test_low_precision_phase_then_fp64_sweep(ctx, prec, method):
     """FP32 / BF16 products (star) or FP32 terms (direct) until the low-precision floor, then FP64
