@@ -159,7 +159,10 @@ __device__ inline double h_elem(const HArgs &a, int64_t w, int64_t nu, int64_t i
     const double zar = x0 + wnu - ei - sar, zai = -sai;
     const double zer = x0 - wnu - ei - ser, zei = -sei;
     const double2 o = reinterpret_cast<const double2 *>(a.occ)[wn + nw * a.nm * (int64_t)z];
-    return -zai / (zar * zar + zai * zai) * o.x - zei / (zer * zer + zei * zei) * o.y;
+    // Im 1/z_a o.x + Im 1/z_e o.y with Im 1/z = -Im z / |z|², over one division (the product of the
+    // two squared moduli stays in FP64 range for |z| between 1e-75 and 1e75 eV)
+    const double da = zar * zar + zai * zai, de = zer * zer + zei * zei;
+    return -(zai * o.x * de + zei * o.y * da) / (da * de);
 }
 
 // h[w + nw (ν + nm i')] per slot, FP64.
