@@ -139,53 +139,141 @@ __global__ __launch_bounds__(256) void h_occ_kernel(HArgs a)
     o[1] = fe;
 }
 
-// h[ω, ν, i', slot] = Im G_{i'}(ω+ω_ν) occ_a + Im G_{i'}(ω-ω_ν) occ_e: one thread per (ω, ν, i').
-// Σin is the clamped Σ, read at ω ± ω_ν through the precomputed cells of the integer grid.
-// h[ω_w, ν, i'] of slot z (temperature index tcol) for a mode at or above ω_acoustic.
-__device__ inline double h_elem(const HArgs &a, int64_t w, int64_t nu, int64_t ip, int tcol, int z)
+// What h[ω_w, ν, ·] of slot z needs beyond the outer state: the cells and weights of ω ± ω_ν, the
+// occupations, ω_w, ω_ν, the clamp η and the slot's Σ.
+struct HConst {
+    int ka, ke;
+    double ta, te, x0, wnu, eta;
+    double2 o;
+    const double2 *S;
+};
+
+__device__ inline HConst h_const(const HArgs &a, int64_t w, int64_t nu, int tcol, int z)
 {
     const int64_t nw = a.nw, wn = w + nw * nu;
-    const double wnu = a.wnu[nu];
-    const double eta = a.eta ? a.eta[tcol] : -INFINITY;
-    const double2 *col = reinterpret_cast<const double2 *>(a.S) + nw * a.ni * (int64_t)tcol + nw * ip;
-    const int ka = a.hk[2 * wn], ke = a.hk[2 * wn + 1];
-    const double ta = a.ht[2 * wn], te = a.ht[2 * wn + 1];
-    const double2 la = col[ka], ha = col[ka + 1], le = col[ke], he = col[ke + 1];
-    const double sar = (1 - ta) * la.x + ta * ha.x;
-    const double sai = (1 - ta) * fmin(la.y, -eta) + ta * fmin(ha.y, -eta);
-    const double ser = (1 - te) * le.x + te * he.x;
-    const double sei = (1 - te) * fmin(le.y, -eta) + te * fmin(he.y, -eta);
-    const double x0 = a.ws[w], ei = a.e_i[ip];
-    const double zar = x0 + wnu - ei - sar, zai = -sai;
-    const double zer = x0 - wnu - ei - ser, zei = -sei;
-    const double2 o = reinterpret_cast<const double2 *>(a.occ)[wn + nw * a.nm * (int64_t)z];
+    HConst c;
+    c.ka = a.hk[2 * wn];
+    c.ke = a.hk[2 * wn + 1];
+    c.ta = a.ht[2 * wn];
+    c.te = a.ht[2 * wn + 1];
+    c.x0 = a.ws[w];
+    c.wnu = a.wnu[nu];
+    c.eta = a.eta ? a.eta[tcol] : -INFINITY;
+    c.o = reinterpret_cast<const double2 *>(a.occ)[wn + nw * a.nm * (int64_t)z];
+    c.S = reinterpret_cast<const double2 *>(a.S) + nw * a.ni * (int64_t)tcol;
+    return c;
+}
+
+// h[ω_w, ν, i'] for a mode at or above ω_acoustic.
+__device__ inline double h_from(const HArgs &a, const HConst &c, int64_t ip)
+{
+    const double2 *col = c.S + a.nw * ip;
+    const double2 la = col[c.ka], ha = col[c.ka + 1], le = col[c.ke], he = col[c.ke + 1];
+    const double eta = c.eta;
+    const double sar = (1 - c.ta) * la.x + c.ta * ha.x;
+    const double sai = (1 - c.ta) * fmin(la.y, -eta) + c.ta * fmin(ha.y, -eta);
+    const double ser = (1 - c.te) * le.x + c.te * he.x;
+    const double sei = (1 - c.te) * fmin(le.y, -eta) + c.te * fmin(he.y, -eta);
+    const double ei = a.e_i[ip];
+    const double zar = c.x0 + c.wnu - ei - sar, zai = -sai;
+    const double zer = c.x0 - c.wnu - ei - ser, zei = -sei;
     // Im 1/z_a o.x + Im 1/z_e o.y with Im 1/z = -Im z / |z|², over one division (the product of the
     // two squared moduli stays in FP64 range for |z| between 1e-75 and 1e75 eV)
     const double da = zar * zar + zai * zai, de = zer * zer + zei * zei;
-    return -(zai * o.x * de + zei * o.y * da) / (da * de);
+    return -(zai * c.o.x * de + zei * c.o.y * da) / (da * de);
 }
 
-// h[w + nw (ν + nm i')] per slot, FP64.
+__device__ inline double h_elem(const HArgs &a, int64_t w, int64_t nu, int64_t ip, int tcol, int z)
+{
+    return h_from(a, h_const(a, w, nu, tcol, z), ip);
+}
+
+// h[w + nw (ν + nm i')] per slot, FP64.  One thread per (ω, ν) and HB_R consecutive outer states:
+// the (ω, ν) constants are read once and the HB_R independent Σ reads are in flight together
+// (lanes along ω, so every read and write is contiguous across a wave).
+constexpr int HB_R = 4;
 __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
-    const int64_t nw = a.nw, total = nw * a.nm * a.ni;
-    if (q >= total) return;
+    const int64_t nw = a.nw, ni = a.ni, nwm = nw * a.nm, ngrp = (ni + HB_R - 1) / HB_R;
+    if (q >= nwm * ngrp) return;
     const int tcol = a.tidx[z];
     if (a.act && !a.act[tcol]) return;
-    const int64_t wn = q % (nw * a.nm), ip = q / (nw * a.nm);
+    const int64_t wn = q % nwm, ip0 = (q / nwm) * HB_R;
     const int64_t w = wn % nw, nu = wn / nw;
-    double *out = a.hout + total * (int64_t)z;
-    out[q] = a.wnu[nu] < a.w_acoustic ? 0.0 : h_elem(a, w, nu, ip, tcol, z);
+    double *out = a.hout + nwm * ni * (int64_t)z;
+    double v[HB_R];
+    if (a.wnu[nu] < a.w_acoustic) {
+#pragma unroll
+        for (int r = 0; r < HB_R; r++) v[r] = 0.0;
+    } else {
+        const HConst c = h_const(a, w, nu, tcol, z);
+#pragma unroll
+        for (int r = 0; r < HB_R; r++) v[r] = h_from(a, c, ip0 + r < ni ? ip0 + r : ni - 1);
+    }
+#pragma unroll
+    for (int r = 0; r < HB_R; r++)
+        if (ip0 + r < ni) out[wn + nwm * (ip0 + r)] = v[r];
 }
 
 // h per slot in low precision (E = unsigned short: BF16 bits, or float), transposed for the
 // low-precision product: hl[k + kp w] with k = ν + nm i' (zero for nm ni <= k < kp, kp a multiple
-// of 8).  One thread per (ω, 16-byte run of k): lanes along ω (so the Σ reads stay coalesced), the
-// 8 waves of a block along k, so a block writes whole 128-byte rows.
+// of 8).  One block per (64 frequencies, HL_IP outer states, slot): one thread per (ω, ν) computes
+// its HL_IP states with the (ω, ν) constants read once (HL_IP Σ reads in flight) into an LDS row
+// of the block's nm HL_IP consecutive k, then whole 16-byte runs of each row are written out.
+constexpr int HL_W = 64, HL_IP = 8, HL_NM_MAX = 32;
 template <typename E>
-__global__ __launch_bounds__(512) void h_build_lp_kernel(HArgs a, E *hl, int64_t kp)
+__global__ __launch_bounds__(256) void h_build_lp_kernel(HArgs a, E *hl, int64_t kp)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    E *tile = reinterpret_cast<E *>(lds);   // [HL_W][nm HL_IP]
+    constexpr int RN = 16 / (int)sizeof(E);
+    const int nm = (int)a.nm, nk = nm * HL_IP;
+    const int z = blockIdx.z;
+    const int tcol = a.tidx[z];
+    if (a.act && !a.act[tcol]) return;
+    const int64_t nw = a.nw, ni = a.ni, w0 = (int64_t)blockIdx.x * HL_W, ip0 = (int64_t)blockIdx.y * HL_IP;
+    for (int p = threadIdx.x; p < HL_W * nm; p += 256) {
+        const int wl = p % HL_W, nu = p / HL_W;
+        const int64_t w = w0 + wl;
+        if (w >= nw) continue;
+        E *row = tile + wl * nk;
+        double v[HL_IP];
+        if (a.wnu[nu] < a.w_acoustic) {
+#pragma unroll
+            for (int r = 0; r < HL_IP; r++) v[r] = 0.0;
+        } else {
+            const HConst c = h_const(a, w, nu, tcol, z);
+#pragma unroll
+            for (int r = 0; r < HL_IP; r++) {
+                const int64_t ip = ip0 + r;
+                const double h = h_from(a, c, ip < ni ? ip : ni - 1);
+                v[r] = ip < ni ? h : 0.0;   // k >= nm ni: zero
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < HL_IP; r++) {
+            if constexpr (sizeof(E) == 2) row[nu + nm * r] = to_bf16(v[r]);
+            else row[nu + nm * r] = (float)v[r];
+        }
+    }
+    __syncthreads();
+    const int nrun = nk / RN;   // HL_IP = 8: whole runs
+    for (int q = threadIdx.x; q < HL_W * nrun; q += 256) {
+        const int run = q % nrun, wl = q / nrun;
+        const int64_t w = w0 + wl, k0 = (int64_t)nm * ip0 + (int64_t)RN * run;
+        if (w >= nw || k0 >= kp) continue;
+        uint4 pk;
+        __builtin_memcpy(&pk, tile + wl * nk + RN * run, 16);
+        *reinterpret_cast<uint4 *>(hl + kp * nw * (int64_t)z + k0 + kp * w) = pk;
+    }
+}
+
+// The same for nm > HL_NM_MAX (the LDS row would not fit): one thread per (ω, 16-byte run of k),
+// lanes along ω, the 8 waves of a block along k.
+template <typename E>
+__global__ __launch_bounds__(512) void h_build_lp_run_kernel(HArgs a, E *hl, int64_t kp)
 {
     constexpr int R = 16 / (int)sizeof(E);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -328,24 +416,38 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
     if (hout) {
         HArgs ah = a;
         ah.act = act_h;
-        const int64_t total = nw * nm * ni;
-        hipLaunchKernelGGL(h_build_kernel, dim3((unsigned)((total + 255) / 256), (unsigned)nslots), dim3(256), 0,
+        const int64_t threads = nw * nm * ((ni + HB_R - 1) / HB_R);
+        hipLaunchKernelGGL(h_build_kernel, dim3((unsigned)((threads + 255) / 256), (unsigned)nslots), dim3(256), 0,
                            ctx->stream, ah);
         EPSX_HIP(ctx, hipGetLastError());
     }
-    const dim3 blk(512);
+    // the transposed low-precision layouts: blocks of HL_IP states until nm ip covers kp
+    const bool rows = nm <= HL_NM_MAX;
+    const int64_t ipk = (kp + nm - 1) / nm;
+    const dim3 grid_rows((unsigned)((nw + HL_W - 1) / HL_W), (unsigned)(((ipk > ni ? ipk : ni) + HL_IP - 1) / HL_IP),
+                         (unsigned)nslots);
     if (hb) {
         HArgs ab = a;
         ab.act = act_b;
-        const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((kp + 63) / 64), (unsigned)nslots);
-        hipLaunchKernelGGL(h_build_lp_kernel<uint16_t>, grid, blk, 0, ctx->stream, ab, hb, kp);
+        if (rows) {
+            hipLaunchKernelGGL(h_build_lp_kernel<uint16_t>, grid_rows, dim3(256), 2 * HL_W * nm * HL_IP, ctx->stream,
+                               ab, hb, kp);
+        } else {
+            const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((kp + 63) / 64), (unsigned)nslots);
+            hipLaunchKernelGGL(h_build_lp_run_kernel<uint16_t>, grid, dim3(512), 0, ctx->stream, ab, hb, kp);
+        }
         EPSX_HIP(ctx, hipGetLastError());
     }
     if (hf) {
         HArgs af = a;
         af.act = act_f;
-        const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((kp + 31) / 32), (unsigned)nslots);
-        hipLaunchKernelGGL(h_build_lp_kernel<float>, grid, blk, 0, ctx->stream, af, hf, kp);
+        if (rows) {
+            hipLaunchKernelGGL(h_build_lp_kernel<float>, grid_rows, dim3(256), 4 * HL_W * nm * HL_IP, ctx->stream, af,
+                               hf, kp);
+        } else {
+            const dim3 grid((unsigned)((nw + 63) / 64), (unsigned)((kp + 31) / 32), (unsigned)nslots);
+            hipLaunchKernelGGL(h_build_lp_run_kernel<float>, grid, dim3(512), 0, ctx->stream, af, hf, kp);
+        }
         EPSX_HIP(ctx, hipGetLastError());
     }
     return EPSX_OK;

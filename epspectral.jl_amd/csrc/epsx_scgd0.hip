@@ -214,16 +214,6 @@ __device__ inline double bits_middle(double a, double b)
     return a < 0 ? -r : r;
 }
 
-// Number of floats between a < b (as a double): the distance of their ordered bit patterns.
-__device__ inline double ordered_gap(double a, double b)
-{
-    auto key = [](double x) -> double {
-        const double m = (double)(unsigned long long)__double_as_longlong(fabs(x));
-        return x < 0 ? -m : m;
-    };
-    return key(b) - key(a);
-}
-
 struct MuArgs {
     int64_t nwa;
     const double *wsa, *D;   // D [nwa, nz]
@@ -319,17 +309,17 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
         if (tid == 0) m.mu[tcol] = fa == 0 ? a : b;
         return;
     }
-    // Illinois steps (regula falsi with the retained end's value halved) shrink the bracket while
-    // it is wider than 2^12 floats, then ordered-bit bisection closes it to two adjacent floats.
-    // n(μ) is a monotone fixed-order sum, so the adjacent pair holding the sign change, and thus
-    // the root returned, is the one plain bisection reaches.
+    // Illinois steps (regula falsi with the retained end's value halved) shrink the bracket until
+    // it holds two adjacent floats, with ordered-bit bisection whenever the secant point is not
+    // strictly inside (and after 40 steps).  n(μ) is a monotone fixed-order sum, so the adjacent
+    // pair holding the sign change, and thus the root returned, is the one plain bisection reaches.
     double ga = fa, gb = fb;  // Illinois-scaled end values
     int side = 0;
     for (int it = 0; it < 300; it++) {
         const double bis = bits_middle(a, b);
         if (bis == a || bis == b) break;
         double x = bis;
-        if (it < 40 && ordered_gap(a, b) > 4096.0) {
+        if (it < 40) {
             const double r = (a * gb - b * ga) / (gb - ga);
             if (r > a && r < b) x = r;
         }
