@@ -269,7 +269,7 @@ def ladder_bench(args):
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "kernel": "dgemm_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)" if args.precision == "fp64"
-                     else f"lpgemm_kernel (shift-table GEMM W = J B̃, {args.precision} operands, FP32 MFMA accumulation)",
+                     else f"lpkgemm_kernel (shift-table GEMM W = J B̃ from stored {args.precision} operands, FP32 accumulation)",
                      "algorithmic_flop_per_application": flops, "gemm_s_per_application": gemm_per_app,
                      "apply_s": tm["apply"] / max(tm["applications"], 1)},
         "sigma_ladder_xx": out["σ_ladder"][0, 0, :, 0].tolist(),
