@@ -205,6 +205,7 @@ struct GemmBatch {
     const double *diag = nullptr;     // with Z: add diag[i] Zim[i, j] in FP64
     bool tune = true;                 // the autotuner may time this product (its gate is not hiding work)
     hipEvent_t mark = nullptr;        // recorded after the main kernel (before the split-k sum)
+    float *Cf = nullptr;              // lpgemm_tn only: FP32 output in place of C (no accumulate, no Z)
 };
 // C[m, n] = A[m, k] * B[k, n] (+ C if accumulate), FP64, column-major, on ctx->stream
 int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, const double *B,

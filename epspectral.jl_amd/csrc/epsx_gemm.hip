@@ -34,6 +34,7 @@ struct GemmArgs {
     int64_t m, n, k, lda, ldb, ldc, sA, sB, sC;
     const double *A, *B;
     double *C;
+    float *Cf;            // FP32 output instead of C (low-precision products, no accumulation)
     double2 *Z;           // complex epilogue: Z = complex(result + diag[i] Zim, Zim), C unused
     const double *Zim;
     const double *diag;   // FP64 diagonal term of the complex epilogue, or null
@@ -93,6 +94,8 @@ __device__ inline void store_out(const GemmArgs &p, int64_t zc, int64_t gi, int6
     if (p.Z) {
         const double im = p.Zim[o];
         p.Z[o] = make_double2(p.diag ? v + p.diag[gi] * im : v, im);
+    } else if (p.Cf) {
+        p.Cf[o] = (float)v;
     } else {
         double *c = p.C + o;
         *c = p.accumulate ? *c + v : v;
@@ -635,11 +638,13 @@ int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, con
         return set_error(ctx, EPSX_ERR_ARG, "low-precision product: precision must be BF16 or FP32");
     if (k % 8 || lda % 8 || ldb % 8 || bt.sA % 8 || bt.sB % 8)
         return set_error(ctx, EPSX_ERR_ARG, "low-precision product: k, row pitches and strides must be multiples of 8");
+    if (bt.Cf && (accumulate || bt.Z))
+        return set_error(ctx, EPSX_ERR_ARG, "low-precision product: an FP32 output neither accumulates nor takes the complex epilogue");
     GemmArgs p{};
     p.m = m; p.n = n; p.k = k; p.lda = lda; p.ldb = ldb; p.ldc = ldc;
     p.sA = bt.sA; p.sB = bt.sB; p.sC = bt.sC;
     p.A = static_cast<const double *>(AT); p.B = static_cast<const double *>(B);
-    p.C = C; p.Z = bt.Z; p.Zim = bt.Zim; p.diag = bt.diag;
+    p.C = C; p.Cf = bt.Cf; p.Z = bt.Z; p.Zim = bt.Zim; p.diag = bt.diag;
     p.idx = bt.idx; p.gate = bt.gate; p.imap = bt.imap;
     p.accumulate = accumulate ? 1 : 0;
     const int sk = precision == EPSX_PREC_BF16 ? 64 : 32;   // k per 128-byte stage row

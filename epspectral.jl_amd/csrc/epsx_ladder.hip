@@ -242,35 +242,6 @@ __global__ void bt_lp_kernel(int64_t K, int64_t kp, int64_t ncols, const double 
     else out[q] = (float)v;
 }
 
-// C_s[(j, q), col] = Σ_m wlo W_m[(klo - 1, q), col] + whi W_m[(klo, q), col], both signs
-// (klo 1-based; W[(r, col), m] with r = j' + nω q).
-__global__ void shift_add_kernel(int64_t nw, int64_t ncol, int64_t ns, const double *__restrict__ W,
-                                 const int32_t *__restrict__ klo, const double *__restrict__ wlo,
-                                 const double *__restrict__ whi, double *__restrict__ Cp, double *__restrict__ Cm)
-{
-    const int64_t nrow = 16 * nw;
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= nrow * ncol) return;
-    const int64_t r = q % nrow, col = q / nrow;
-    const int64_t j = r % nw, qb = r - j;
-    double sp = 0, sm = 0;
-    for (int64_t m = 0; m < ns; m++) {
-        const double *Wm = W + nrow * (col + ncol * m);
-        const int64_t t0 = j + nw * m, t1 = j + nw * (m + ns);
-        const double lo0 = wlo[t0], hi0 = whi[t0], lo1 = wlo[t1], hi1 = whi[t1];
-        if (lo0 != 0 || hi0 != 0) {
-            const int64_t k = klo[t0] - 1;
-            sp += lo0 * Wm[qb + k] + hi0 * Wm[qb + k + 1];
-        }
-        if (lo1 != 0 || hi1 != 0) {
-            const int64_t k = klo[t1] - 1;
-            sm += lo1 * Wm[qb + k] + hi1 * Wm[qb + k + 1];
-        }
-    }
-    Cp[q] = sp;
-    Cm[q] = sm;
-}
-
 // Y = (U+ - U-)/2 for the KK product: Y[(j, q), (a, i)] from the type-1 columns
 __global__ void exch_Y_kernel(int64_t n, const double *__restrict__ Cp, const double *__restrict__ Cm, double *__restrict__ Y)
 {
@@ -417,9 +388,11 @@ __global__ void combine3_kernel(int64_t n, const double2 *__restrict__ a, double
     y[q] = make_double2(a[q].x + sb * b[q].x + sc * c[q].x, a[q].y + sb * b[q].y + sc * c[q].y);
 }
 
-// C_s (+)= shifted W for the shifts [m0, m0 + mb) of one chunk
+// C_s[(j, q), col] (+)= Σ_m wlo W_m[(klo - 1, q), col] + whi W_m[(klo, q), col], both signs, for the
+// shifts [m0, m0 + mb) of one chunk (klo 1-based; W[(r, col), m] with r = j' + nω q, FP64 or FP32)
+template <typename TW>
 __global__ void shift_add_chunk_kernel(int64_t nw, int64_t ncol, int64_t ns, int64_t m0, int64_t mb,
-                                       const double *__restrict__ W, const int32_t *__restrict__ klo,
+                                       const TW *__restrict__ W, const int32_t *__restrict__ klo,
                                        const double *__restrict__ wlo, const double *__restrict__ whi,
                                        double *__restrict__ Cp, double *__restrict__ Cm, int accumulate)
 {
@@ -431,16 +404,16 @@ __global__ void shift_add_chunk_kernel(int64_t nw, int64_t ncol, int64_t ns, int
     double sp = accumulate ? Cp[q] : 0.0, sm = accumulate ? Cm[q] : 0.0;
     for (int64_t mm = 0; mm < mb; mm++) {
         const int64_t m = m0 + mm;
-        const double *Wm = W + nrow * (col + ncol * mm);
+        const TW *Wm = W + nrow * (col + ncol * mm);
         const int64_t t0 = j + nw * m, t1 = j + nw * (m + ns);
         const double lo0 = wlo[t0], hi0 = whi[t0], lo1 = wlo[t1], hi1 = whi[t1];
         if (lo0 != 0 || hi0 != 0) {
             const int64_t k = klo[t0] - 1;
-            sp += lo0 * Wm[qb + k] + hi0 * Wm[qb + k + 1];
+            sp += lo0 * (double)Wm[qb + k] + hi0 * (double)Wm[qb + k + 1];
         }
         if (lo1 != 0 || hi1 != 0) {
             const int64_t k = klo[t1] - 1;
-            sm += lo1 * Wm[qb + k] + hi1 * Wm[qb + k + 1];
+            sm += lo1 * (double)Wm[qb + k] + hi1 * (double)Wm[qb + k + 1];
         }
     }
     Cp[q] = sp;
@@ -521,8 +494,9 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int pr
         hipLaunchKernelGGL(jperm_kernel, dim3(nb(nrow * K)), dim3(256), 0, st, nw, ni, J, dptr<double>(l.Jm));
     }
     EPSX_HIP(ctx, hipGetLastError());
-    // W for chunks of shifts, each followed by its shift-add
-    const size_t per_m = 8 * (size_t)nrow * ncol;
+    // W for chunks of shifts, each followed by its shift-add (W in FP32 from the low-precision
+    // products, which accumulate in FP32)
+    const size_t per_m = (native ? 4 : 8) * (size_t)nrow * ncol;
     int64_t mb = (int64_t)((size_t(8) << 30) / per_m);
     mb = mb < 1 ? 1 : (mb > l.ns ? l.ns : mb);
     EPSX_TRY(dev_alloc(ctx, l.W, per_m * mb));
@@ -532,17 +506,22 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int pr
         GemmBatch bt;
         if (native) {
             const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
-            EPSX_TRY(lpgemm_tn(ctx, precision, nrow, ncol * mc, kp, l.JmL.p, kp, Bl, kp, dptr<double>(l.W), nrow,
-                               false, bt));
+            bt.Cf = dptr<float>(l.W);
+            EPSX_TRY(lpgemm_tn(ctx, precision, nrow, ncol * mc, kp, l.JmL.p, kp, Bl, kp, nullptr, nrow, false, bt));
         } else {
             bt.precision = precision;   // FP32 / BF16 operands rounded as staged, FP32 accumulation, or FP64
             EPSX_TRY(dgemm(ctx, nrow, ncol * mc, K, dptr<double>(l.Jm), nrow, dptr<double>(l.Bt) + K * ncol * m0, K,
                            dptr<double>(l.W), nrow, false, bt));
         }
         EPSX_TRY(t_gemm.stop(1));
-        hipLaunchKernelGGL(shift_add_chunk_kernel, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns, m0, mc,
-                           dptr<double>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
-                           dptr<double>(l.Cp), dptr<double>(l.Cm), m0 > 0 ? 1 : 0);
+        if (native)
+            hipLaunchKernelGGL(shift_add_chunk_kernel<float>, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns,
+                               m0, mc, dptr<float>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
+                               dptr<double>(l.Cp), dptr<double>(l.Cm), m0 > 0 ? 1 : 0);
+        else
+            hipLaunchKernelGGL(shift_add_chunk_kernel<double>, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns,
+                               m0, mc, dptr<double>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo),
+                               dptr<double>(l.whi), dptr<double>(l.Cp), dptr<double>(l.Cm), m0 > 0 ? 1 : 0);
         EPSX_HIP(ctx, hipGetLastError());
     }
     // KY = KK (U+ - U-)/2, then the channels and the attached propagators
