@@ -121,15 +121,8 @@ __global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_
             const double2 lo = St[j], hi = St[j + 1];
             const double sr = (1 - t) * lo.x + t * hi.x, si = (1 - t) * lo.y + t * hi.y;
             const double c = x - e_i[i] - sr, d = -si;
-            double im;
-            if (fabs(d) <= fabs(c)) {
-                const double r = d / c, zz = fma(d, r, c);
-                im = -r / zz;
-            } else {
-                const double r = c / d, zz = fma(-c, r, -d);
-                im = 1.0 / zz;
-            }
-            A = -im / PI_D;
+            // -Im(1/z) / π = d / (π |z|²): one division
+            A = d / (PI_D * fma(c, c, d * d));
             acc = fma(A, w_i[i], acc);
         }
         double v = A * wa;
@@ -308,6 +301,32 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
     if (fa == 0 || fb == 0) {
         if (tid == 0) m.mu[tcol] = fa == 0 ? a : b;
         return;
+    }
+    // The previous iterate's μ is close to this root: two evaluations at μ_prev ∓ δ narrow the
+    // bracket to [μ_prev - δ, μ_prev + δ] when the sign change lies there, else to the side that
+    // holds it.  Every end kept still brackets the one sign change, so the root is unchanged.
+    {
+        const double mp = m.mu[tcol], dl = 1e-4 * (b - a);
+        const double lo = mp - dl, hi = mp + dl;
+        if (lo > a && hi < b) {
+            const double flo = excess(lo);
+            if (flo == 0) {
+                if (tid == 0) m.mu[tcol] = lo;
+                return;
+            }
+            if ((flo > 0) == (fa > 0)) {
+                a = lo; fa = flo;
+                const double fhi = excess(hi);
+                if (fhi == 0) {
+                    if (tid == 0) m.mu[tcol] = hi;
+                    return;
+                }
+                if ((fhi > 0) == (fa > 0)) { a = hi; fa = fhi; }
+                else { b = hi; fb = fhi; }
+            } else {
+                b = lo; fb = flo;
+            }
+        }
     }
     // Illinois steps (regula falsi with the retained end's value halved) shrink the bracket until
     // it holds two adjacent floats, with ordered-bit bisection whenever the secant point is not
