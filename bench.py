@@ -240,8 +240,10 @@ def ladder_bench(args):
     L = setup_ladder(setup, R, v)
     ld = LadderDevice(dev, L)
     Ωs = 0.005 * np.arange(args.omegas)
-    # prime: B̃ of the first temperature and one solve (not timed)
-    ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs[:1], rtol=1e-2, maxiter=30)
+    # prime: B̃ of the first temperature and one solve in the timed precision (not timed: loads the
+    # GEMM code objects)
+    ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs[:1], rtol=1e-2, maxiter=30,
+            precision=args.precision)
     t0 = time.perf_counter()
     out = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs, rtol=1e-2, maxiter=30,
                   precision=args.precision)
@@ -268,8 +270,7 @@ def ladder_bench(args):
                    "extrapolated_s_41x4": el / len(Ωs) * 41 * 4},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "dgemm_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)" if args.precision == "fp64"
-                     else f"lpkgemm_kernel (shift-table GEMM W = J B̃ from stored {args.precision} operands, FP32 accumulation)",
+                     "kernel": ladder_gemm_label(args.precision),
                      "algorithmic_flop_per_application": flops, "gemm_s_per_application": gemm_per_app,
                      "apply_s": tm["apply"] / max(tm["applications"], 1)},
         "sigma_ladder_xx": out["σ_ladder"][0, 0, :, 0].tolist(),
@@ -282,6 +283,17 @@ def ladder_bench(args):
         apps_per_solve = tm["applications"] / len(Ωs)
         line["cpu_baseline"] = ladder_cpu_baseline(nw, ni, ns, apps_per_solve, args.cpu_seconds)
     print(json.dumps(line), flush=True)
+
+
+def ladder_gemm_label(precision):
+    """The kernel that runs the shift-table GEMM W = J·B̃: rocBLAS (a plain GEMM), or the library's
+    own MFMA kernels with EPSX_LADDER_BLAS=0."""
+    if os.environ.get("EPSX_LADDER_BLAS", "1") == "0":
+        return ("dgemm_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)" if precision == "fp64"
+                else f"lpkgemm_kernel (shift-table GEMM W = J B̃ from stored {precision} operands, FP32 accumulation)")
+    return {"fp64": "rocBLAS dgemm (shift-table GEMM W = J B̃, FP64)",
+            "fp32": "rocBLAS sgemm (shift-table GEMM W = J B̃ from stored FP32 operands)",
+            "bf16": "rocBLAS gemm_ex (shift-table GEMM W = J B̃ from stored BF16 operands, FP32 accumulation)"}[precision]
 
 
 def ladder_cpu_baseline(nw, ni, ns, apps_per_solve, budget_s):

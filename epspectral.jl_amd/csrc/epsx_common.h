@@ -101,6 +101,7 @@ struct epsx_ctx {
     epsx::NetDev net;
     std::vector<epsx::DevBuf *> owned;
     epsx::DevBuf gemm_ws;   // split-k partial products
+    void *blas = nullptr;   // rocBLAS handle on `stream` (created on first use), for plain GEMMs
     std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, split) -> tile shape
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
@@ -111,6 +112,7 @@ struct epsx_ctx {
 namespace epsx {
 
 int set_error(epsx_ctx *ctx, int code, const char *fmt, ...);
+void blas_destroy(epsx_ctx *ctx);   // epsx_ladder.hip
 int hip_fail(epsx_ctx *ctx, hipError_t e, const char *what, const char *file, int line);
 int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes);   // context-owned, freed at close
 void dev_free(DevBuf &b);

@@ -16,6 +16,8 @@
 #include <cstdlib>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "epsx_common.h"
 #include "epsx_device.h"
 
@@ -431,6 +433,40 @@ static int ldot(epsx_ctx *ctx, int64_t n, const double2 *x, const double2 *y, do
     return EPSX_OK;
 }
 
+// The shift-table product W = J B̃ is a plain GEMM (no epilogue, no batching): it runs in rocBLAS
+// (deterministic: atomics off) on the context's stream; EPSX_LADDER_BLAS=0 runs the library's own
+// MFMA kernels instead (for comparisons).
+static bool use_blas()
+{
+    const char *e = getenv("EPSX_LADDER_BLAS");
+    return !(e && e[0] == '0');
+}
+
+static int blas_handle(epsx_ctx *ctx, rocblas_handle &h)
+{
+    if (!ctx->blas) {
+        rocblas_handle nh = nullptr;
+        if (rocblas_create_handle(&nh) != rocblas_status_success)
+            return set_error(ctx, EPSX_ERR_DEVICE, "rocblas_create_handle failed");
+        if (rocblas_set_atomics_mode(nh, rocblas_atomics_not_allowed) != rocblas_status_success) {
+            (void)rocblas_destroy_handle(nh);
+            return set_error(ctx, EPSX_ERR_DEVICE, "rocblas_set_atomics_mode failed");
+        }
+        ctx->blas = nh;
+    }
+    h = static_cast<rocblas_handle>(ctx->blas);
+    if (rocblas_set_stream(h, ctx->stream) != rocblas_status_success)
+        return set_error(ctx, EPSX_ERR_DEVICE, "rocblas_set_stream failed");
+    return EPSX_OK;
+}
+
+static int blas_check(epsx_ctx *ctx, rocblas_status st, const char *what)
+{
+    if (st != rocblas_status_success)
+        return set_error(ctx, EPSX_ERR_DEVICE, "%s failed: %s", what, rocblas_status_to_string(st));
+    return EPSX_OK;
+}
+
 // B̃ for temperature T (once per T and ω_acoustic)
 static int ensure_Bt(epsx_ctx *ctx, double T, double w_ac)
 {
@@ -496,6 +532,7 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int pr
     EPSX_HIP(ctx, hipGetLastError());
     // W for chunks of shifts, each followed by its shift-add (W in FP32 from the low-precision
     // products, which accumulate in FP32)
+    const bool blas = use_blas();
     const size_t per_m = (native ? 4 : 8) * (size_t)nrow * ncol;
     int64_t mb = (int64_t)((size_t(8) << 30) / per_m);
     mb = mb < 1 ? 1 : (mb > l.ns ? l.ns : mb);
@@ -504,7 +541,30 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int pr
         const int64_t mc = m0 + mb <= l.ns ? mb : l.ns - m0;
         ClassTimer t_gemm(ctx, 0);
         GemmBatch bt;
-        if (native) {
+        const bool staged = precision != EPSX_PREC_FP64 && !native;
+        if (blas && !staged) {
+            rocblas_handle h;
+            EPSX_TRY(blas_handle(ctx, h));
+            const int bm = (int)nrow, bn = (int)(ncol * mc);
+            if (native) {
+                // W (FP32) = JmLᵀ Bl from the k-contiguous BF16 / FP32 operands, FP32 accumulation
+                const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
+                const float one = 1.0f, zero = 0.0f;
+                const rocblas_datatype dt = bf ? rocblas_datatype_bf16_r : rocblas_datatype_f32_r;
+                EPSX_TRY(blas_check(ctx, rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, bm, bn,
+                                                         (int)kp, &one, l.JmL.p, dt, (int)kp, Bl, dt, (int)kp, &zero,
+                                                         l.W.p, rocblas_datatype_f32_r, bm, l.W.p,
+                                                         rocblas_datatype_f32_r, bm, rocblas_datatype_f32_r,
+                                                         rocblas_gemm_algo_standard, 0, 0),
+                                    "rocblas_gemm_ex (ladder product)"));
+            } else {
+                const double one = 1.0, zero = 0.0;
+                EPSX_TRY(blas_check(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, bm, bn, (int)K,
+                                                       &one, dptr<double>(l.Jm), bm, dptr<double>(l.Bt) + K * ncol * m0,
+                                                       (int)K, &zero, dptr<double>(l.W), bm),
+                                    "rocblas_dgemm (ladder product)"));
+            }
+        } else if (native) {
             const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
             bt.Cf = dptr<float>(l.W);
             EPSX_TRY(lpgemm_tn(ctx, precision, nrow, ncol * mc, kp, l.JmL.p, kp, Bl, kp, nullptr, nrow, false, bt));
@@ -870,3 +930,13 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
     EPSX_HIP(ctx, hipStreamSynchronize(st));
     return EPSX_OK;
 }
+
+namespace epsx {
+
+void blas_destroy(epsx_ctx *ctx)
+{
+    if (ctx->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
+    ctx->blas = nullptr;
+}
+
+}  // namespace epsx
