@@ -45,6 +45,28 @@ def test_apply_ladder_kernel(ctx, case):
         assert np.array_equal(ld.apply(J, Gm, Gp, T, 1e-6), LJ)
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6), ("bf16", 1e-5)])
+def test_shift_gemm_blas_matches_own_kernels(ctx, prec, tol):
+    """The shift-table product in rocBLAS (the default) against the library's own MFMA kernels
+    (EPSX_LADDER_BLAS=0): the same operands, so one application agrees to the accumulation order
+    (FP64, or FP32 for the stored FP32 / BF16 operands); the default path is the one the other
+    ladder tests and the oracle comparisons run."""
+    s, L, r = disordered_case()
+    ld = device(ctx, s, L)
+    nw, ni = len(s.ωs), s.g2.shape[1]
+    J = pseudorandom_complex(nw, 4, 3, 2, ni, seed=3.7)
+    Gm, Gp = propagators(L, r, 0, 0.2, 0.1)
+    blas = ld.apply(J, Gm, Gp, 0.1, 1e-6, precision=prec)
+    os.environ["EPSX_LADDER_BLAS"] = "0"
+    try:
+        own = ld.apply(J, Gm, Gp, 0.1, 1e-6, precision=prec)
+    finally:
+        del os.environ["EPSX_LADDER_BLAS"]
+    assert rel(blas, own) <= tol
+    if prec == "fp64":
+        assert rel(blas, oracle.apply_ladder_kernel(L, J, Gm, Gp, 0.1, 1e-6)) <= 1e-12
+
+
 def oracle_loop(L, r, Ωs, T_idx, **kw):
     out = []
     for iT in T_idx:
