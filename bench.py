@@ -127,8 +127,11 @@ def main():
         avg = tsum["star_gemm"] / max(tsum["star_gemm_launches"], 1)
         achieved = flops / avg / 1e12
         if args.precision == "fp64":
-            peak, kname = FP64_PEAK_TFLOPS, "dgemm_kernel (star Fan-Migdal GEMM, FP64 MFMA 16x16x4)"
-            tr = traffic("dgemm_nn_kernel", args.nk, nT)
+            own = os.environ.get("EPSX_STAR_BLAS", "1") == "0" or os.environ.get("EPSX_GEMM_SHAPE")
+            peak = FP64_PEAK_TFLOPS
+            kname = ("dgemm_kernel (star Fan-Migdal GEMM, FP64 MFMA 16x16x4)" if own
+                     else "hipBLASLt FP64 GEMM (star Fan-Migdal product, strided batch with G̃ shared)")
+            tr = traffic("dgemm_kernel" if own else "hipblaslt_star", args.nk, nT)
         else:
             peak = BF16_PEAK_TFLOPS if args.precision == "bf16" else FP32_PEAK_TFLOPS
             kname = f"lpkgemm_kernel (star Fan-Migdal GEMM, {args.precision.upper()} MFMA)"

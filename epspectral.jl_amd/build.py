@@ -54,9 +54,9 @@ def build(jobs=None):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    # rocBLAS serves the ladder's plain shift-table GEMM (the ROCm install's library, found at run
-    # time through the rpath)
-    libs = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas"]
+    # rocBLAS serves the ladder's plain shift-table GEMM and hipBLASLt the star method's FP64
+    # product (the ROCm install's libraries, found at run time through the rpath)
+    libs = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas", "-lhipblaslt"]
     subprocess.run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs + libs, check=True)
     os.replace(tmp, OUT)
     return OUT

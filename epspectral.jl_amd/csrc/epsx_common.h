@@ -102,6 +102,7 @@ struct epsx_ctx {
     std::vector<epsx::DevBuf *> owned;
     epsx::DevBuf gemm_ws;   // split-k partial products
     void *blas = nullptr;   // rocBLAS handle on `stream` (created on first use), for plain GEMMs
+    void *blaslt = nullptr; // hipBLASLt handle and its chosen algorithms (epsx_gemm.hip)
     std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, split) -> tile shape
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
@@ -113,6 +114,7 @@ namespace epsx {
 
 int set_error(epsx_ctx *ctx, int code, const char *fmt, ...);
 void blas_destroy(epsx_ctx *ctx);   // epsx_ladder.hip
+void blaslt_destroy(epsx_ctx *ctx); // epsx_gemm.hip
 int hip_fail(epsx_ctx *ctx, hipError_t e, const char *what, const char *file, int line);
 int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes);   // context-owned, freed at close
 void dev_free(DevBuf &b);
@@ -215,6 +217,16 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
 // The same product from low-precision operands stored k-contiguous (precision EPSX_PREC_BF16:
 // uint16 BF16 bits, EPSX_PREC_FP32: float; AT[k, m] and B[k, n], row pitches, strides and k
 // multiples of 8, zero-padded beyond the true k), FP32 accumulation, FP64 output.
+// C_z = A_z B_z for z < batch: column-major FP64 (strides in elements, sB may be 0) through
+// hipBLASLt with a workspace-free algorithm (no split-k workspace, no atomics: repeats give the
+// same bits).  blaslt_prepare, outside graph capture, times the heuristic's workspace-free
+// candidates on the given buffers (C overwritten) for batch counts 1..max_batch and keeps the
+// fastest per problem in the context; blaslt_dgemm then only launches.
+int blaslt_prepare(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, int64_t sA,
+                   const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc, int64_t sC, int64_t max_batch,
+                   bool *ok);
+int blaslt_dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, int64_t sA,
+                 const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc, int64_t sC, int64_t batch);
 int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, const void *AT, int64_t lda,
               const void *B, int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt);
 int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A, int64_t lda,

@@ -108,6 +108,7 @@ void epsx_close(epsx_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     epsx::blas_destroy(ctx);
+    epsx::blaslt_destroy(ctx);
     for (auto *b : ctx->owned) epsx::dev_free(*b);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -18,7 +18,11 @@
 //     XCD's L2, and masked instances leave no XCD idle.
 #include <array>
 #include <cstdlib>
+#include <map>
+#include <memory>
 #include <type_traits>
+
+#include <hipblaslt/hipblaslt.h>
 
 #include "epsx_common.h"
 #include "epsx_device.h"
@@ -667,6 +671,157 @@ int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, con
         EPSX_HIP(ctx, hipGetLastError());
     }
     return EPSX_OK;
+}
+
+}  // namespace epsx
+
+// ------------------------------------------------------------------ hipBLASLt (plain FP64 GEMMs)
+namespace epsx {
+namespace {
+
+struct LtDescs {
+    hipblasLtMatmulDesc_t md = nullptr;
+    hipblasLtMatrixLayout_t la = nullptr, lb = nullptr, lc = nullptr;
+    ~LtDescs()
+    {
+        if (lc) (void)hipblasLtMatrixLayoutDestroy(lc);
+        if (lb) (void)hipblasLtMatrixLayoutDestroy(lb);
+        if (la) (void)hipblasLtMatrixLayoutDestroy(la);
+        if (md) (void)hipblasLtMatmulDescDestroy(md);
+    }
+};
+
+// A prepared product: its descriptors and the algorithm chosen for them.
+struct LtProduct {
+    LtDescs d;
+    hipblasLtMatmulAlgo_t algo;
+};
+
+struct BlasLt {
+    hipblasLtHandle_t h = nullptr;
+    // (m, n, k, lda, sA, ldb, sB, ldc, sC, batch) -> product; absent: no workspace-free algorithm
+    std::map<std::array<int64_t, 10>, std::unique_ptr<LtProduct>> prod;
+};
+
+int lt_check(epsx_ctx *ctx, hipblasStatus_t st, const char *what)
+{
+    if (st != HIPBLAS_STATUS_SUCCESS) return set_error(ctx, EPSX_ERR_DEVICE, "%s failed (hipBLASLt status %d)", what, (int)st);
+    return EPSX_OK;
+}
+
+int lt_layout(epsx_ctx *ctx, hipblasLtMatrixLayout_t &l, int64_t rows, int64_t cols, int64_t ld, int64_t stride,
+              int64_t batch)
+{
+    EPSX_TRY(lt_check(ctx, hipblasLtMatrixLayoutCreate(&l, HIP_R_64F, (uint64_t)rows, (uint64_t)cols, ld), "layout"));
+    if (batch > 1) {
+        const int32_t bc = (int32_t)batch;
+        EPSX_TRY(lt_check(ctx, hipblasLtMatrixLayoutSetAttribute(l, HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof(bc)),
+                          "layout batch"));
+        EPSX_TRY(lt_check(ctx, hipblasLtMatrixLayoutSetAttribute(l, HIPBLASLT_MATRIX_LAYOUT_STRIDED_BATCH_OFFSET, &stride,
+                                                                 sizeof(stride)),
+                          "layout stride"));
+    }
+    return EPSX_OK;
+}
+
+int lt_descs(epsx_ctx *ctx, LtDescs &d, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t sA, int64_t ldb,
+             int64_t sB, int64_t ldc, int64_t sC, int64_t batch)
+{
+    EPSX_TRY(lt_check(ctx, hipblasLtMatmulDescCreate(&d.md, HIPBLAS_COMPUTE_64F, HIP_R_64F), "matmul descriptor"));
+    EPSX_TRY(lt_layout(ctx, d.la, m, k, lda, sA, batch));
+    EPSX_TRY(lt_layout(ctx, d.lb, k, n, ldb, sB, batch));
+    EPSX_TRY(lt_layout(ctx, d.lc, m, n, ldc, sC, batch));
+    return EPSX_OK;
+}
+
+}  // namespace
+
+int blaslt_prepare(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, int64_t sA,
+                   const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc, int64_t sC, int64_t max_batch,
+                   bool *ok)
+{
+    *ok = false;
+    if (!ctx->blaslt) {
+        auto *b = new BlasLt;
+        if (hipblasLtCreate(&b->h) != HIPBLAS_STATUS_SUCCESS) {
+            delete b;
+            return set_error(ctx, EPSX_ERR_DEVICE, "hipblasLtCreate failed");
+        }
+        ctx->blaslt = b;
+    }
+    auto *b = static_cast<BlasLt *>(ctx->blaslt);
+    bool all = true;
+    for (int64_t batch = 1; batch <= max_batch; batch++) {
+        const std::array<int64_t, 10> key{m, n, k, lda, sA, ldb, sB, ldc, sC, batch};
+        if (b->prod.count(key)) continue;
+        auto pr = std::make_unique<LtProduct>();
+        LtDescs &d = pr->d;
+        EPSX_TRY(lt_descs(ctx, d, m, n, k, lda, sA, ldb, sB, ldc, sC, batch));
+        hipblasLtMatmulPreference_t pref = nullptr;
+        EPSX_TRY(lt_check(ctx, hipblasLtMatmulPreferenceCreate(&pref), "preference"));
+        const uint64_t wsz = 0;
+        hipblasStatus_t st = hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz,
+                                                                   sizeof(wsz));
+        constexpr int NCAND = 32;
+        hipblasLtMatmulHeuristicResult_t res[NCAND];
+        int nres = 0;
+        if (st == HIPBLAS_STATUS_SUCCESS)
+            st = hipblasLtMatmulAlgoGetHeuristic(b->h, d.md, d.la, d.lb, d.lc, d.lc, pref, NCAND, res, &nres);
+        (void)hipblasLtMatmulPreferenceDestroy(pref);
+        // the heuristic's order puts split-k algorithms first, slow without their workspace: every
+        // workspace-free candidate is timed once (after a warm launch) and the fastest kept
+        hipEvent_t e0, e1;
+        EPSX_HIP(ctx, hipEventCreate(&e0));
+        EPSX_HIP(ctx, hipEventCreate(&e1));
+        const double one = 1.0, zero = 0.0;
+        int pick = -1;
+        float best = 0;
+        for (int r = 0; st == HIPBLAS_STATUS_SUCCESS && r < nres; r++) {
+            if (res[r].state != HIPBLAS_STATUS_SUCCESS || res[r].workspaceSize != 0) continue;
+            if (hipblasLtMatmul(b->h, d.md, &one, A, d.la, B, d.lb, &zero, C, d.lc, C, d.lc, &res[r].algo, nullptr, 0,
+                                ctx->stream) != HIPBLAS_STATUS_SUCCESS)
+                continue;
+            EPSX_HIP(ctx, hipEventRecord(e0, ctx->stream));
+            const hipblasStatus_t rs = hipblasLtMatmul(b->h, d.md, &one, A, d.la, B, d.lb, &zero, C, d.lc, C, d.lc,
+                                                       &res[r].algo, nullptr, 0, ctx->stream);
+            EPSX_HIP(ctx, hipEventRecord(e1, ctx->stream));
+            EPSX_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0;
+            EPSX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            if (rs == HIPBLAS_STATUS_SUCCESS && (pick < 0 || ms < best)) { pick = r; best = ms; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (pick < 0) { all = false; continue; }
+        pr->algo = res[pick].algo;
+        b->prod[key] = std::move(pr);
+    }
+    *ok = all;
+    return EPSX_OK;
+}
+
+int blaslt_dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, int64_t sA,
+                 const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc, int64_t sC, int64_t batch)
+{
+    auto *b = static_cast<BlasLt *>(ctx->blaslt);
+    const std::array<int64_t, 10> key{m, n, k, lda, sA, ldb, sB, ldc, sC, batch};
+    auto it = b ? b->prod.find(key) : decltype(b->prod.end()){};
+    if (!b || it == b->prod.end()) return set_error(ctx, EPSX_ERR_STATE, "hipBLASLt product not prepared");
+    LtProduct &pr = *it->second;
+    const double one = 1.0, zero = 0.0;
+    return lt_check(ctx, hipblasLtMatmul(b->h, pr.d.md, &one, A, pr.d.la, B, pr.d.lb, &zero, C, pr.d.lc, C, pr.d.lc,
+                                         &pr.algo, nullptr, 0, ctx->stream),
+                    "hipblasLtMatmul");
+}
+
+void blaslt_destroy(epsx_ctx *ctx)
+{
+    auto *b = static_cast<BlasLt *>(ctx->blaslt);
+    if (!b) return;
+    b->prod.clear();   // descriptors before the handle
+    if (b->h) (void)hipblasLtDestroy(b->h);
+    delete b;
+    ctx->blaslt = nullptr;
 }
 
 }  // namespace epsx

@@ -818,6 +818,22 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         EPSX_TRY(tmp_alloc(ctx, d_gates.b, 4 * (size_t)nT * (L.n + 1)));
     }
     std::vector<int32_t> active(nT, 1), act_idx;
+    // The star method's FP64 product h·G̃ is a plain strided-batched GEMM with a shared G̃: hipBLASLt
+    // runs it (85% of the FP64 peak on the t2g 24³ shape against 58-62% for the library's kernel,
+    // which EPSX_STAR_BLAS=0 or EPSX_GEMM_SHAPE selects, and which serves every product with masked
+    // or low-precision instances); the host knows which slots take the FP64 pass, since a graph is
+    // captured again whenever the active set changes.
+    bool star_lt = false;
+    if (method == EPSX_FM_STAR) {
+        const char *e = getenv("EPSX_STAR_BLAS"), *shape = getenv("EPSX_GEMM_SHAPE");
+        if (!(e && e[0] == '0') && !(shape && *shape)) {
+            // candidates timed on zeroed h (ImΣ overwritten before its first use)
+            const int64_t K = s.nm * ni;
+            EPSX_HIP(ctx, hipMemsetAsync(hbuf.b.p, 0, 8 * (size_t)nw * K * nT, st));
+            EPSX_TRY(blaslt_prepare(ctx, nw, ni, K, hbuf.ptr<double>(), nw, nw * K, dptr<double>(s.Gt), K, 0,
+                                    B.ImS.ptr<double>(), nw, nw * ni, nT, &star_lt));
+        }
+    }
     // the levels present among the active temperatures, as a bit mask
     auto present_levels = [&]() {
         int mask = 0;
@@ -884,6 +900,23 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                     bt.sA = kp * nw;
                     EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ni, kp, bf ? hb.b.p : hf.b.p, kp,
                                        bf ? s.Gb.p : s.Gf.p, kp, B.ImS.ptr<double>(), nw, false, bt));
+                } else if (star_lt && passes[q].prec == EPSX_PREC_FP64) {
+                    // hipBLASLt, one strided-batched call per run of consecutive slots that take this
+                    // pass (active, and at the FP64 level) with consecutive temperatures
+                    const int64_t K = s.nm * ni, sA = nw * K, sC = nw * ni;
+                    auto take = [&](int z) {
+                        const int32_t t = h_tidx[z];
+                        return active[t] && (!split || prec_t[t] == EPSX_PREC_FP64);
+                    };
+                    for (int z = 0; z < nA;) {
+                        if (!take(z)) { z++; continue; }
+                        int z1 = z + 1;
+                        while (z1 < nA && take(z1) && h_tidx[z1] == h_tidx[z1 - 1] + 1) z1++;
+                        EPSX_TRY(blaslt_dgemm(ctx, nw, ni, K, hbuf.ptr<double>() + sA * z, nw, sA, dptr<double>(s.Gt), K,
+                                              0, B.ImS.ptr<double>() + sC * h_tidx[z], nw, sC, z1 - z));
+                        z = z1;
+                    }
+                    if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
                 } else {
                     EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
                                    B.ImS.ptr<double>(), nw, false, bt));
@@ -1126,6 +1159,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int graph_levels = 0;
+        std::vector<int32_t> graph_active, graph_prec;   // the active set and levels captured for
         // the HIP events time the second iteration (warm caches, the GEMM tile shapes chosen in the
         // first), or the first of a one-iteration run
         auto timed = [&](int64_t iter) { return iter == 2 || (iter == 1 && maxiter == 1); };
@@ -1138,8 +1172,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             if (iter <= 2) {
                 EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, timed(iter)));
             } else {
-                // a new set of precision levels present: the passes change, capture again
-                if (exec && present_levels() != graph_levels) {
+                // a new set of precision levels present, or (hipBLASLt products, which have no
+                // device mask) a new active set or level of some temperature: the launches change,
+                // capture again
+                if (exec && (present_levels() != graph_levels ||
+                             (star_lt && (active != graph_active || prec_t != graph_prec)))) {
                     EPSX_HIP(ctx, hipGraphExecDestroy(exec));
                     EPSX_HIP(ctx, hipGraphDestroy(graph));
                     exec = nullptr;
@@ -1147,6 +1184,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 }
                 if (!exec) {
                     graph_levels = present_levels();
+                    graph_active = active;
+                    graph_prec = prec_t;
                     EPSX_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                     int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, false);
                     hipError_t ec = hipStreamEndCapture(st, &graph);
