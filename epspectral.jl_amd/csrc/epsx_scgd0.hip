@@ -34,6 +34,18 @@ __global__ void permute_g2_kernel(int64_t nm, int64_t ni, int64_t nf, const doub
     g2p[q] = g2[nu + nm * (i + ni * (int64_t)fsrc[p])];
 }
 
+// Σ_new = complex(ReΣ, ImΣ) for the slots of a pass (ReΣ = KK ImΣ from a library product).
+__global__ void complex_pack_kernel(int64_t n, const int32_t *__restrict__ tidx, const int32_t *__restrict__ gate,
+                                    const double *__restrict__ re, const double *__restrict__ im,
+                                    double2 *__restrict__ Z)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int tcol = tidx[blockIdx.y];
+    if (q >= n || (gate && !gate[tcol])) return;
+    const int64_t o = q + n * (int64_t)tcol;
+    Z[o] = make_double2(re[o], im[o]);
+}
+
 __global__ void fill_sigma_kernel(int64_t n, double2 *S, double eta_init)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -823,7 +835,18 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // which EPSX_STAR_BLAS=0 or EPSX_GEMM_SHAPE selects, and which serves every product with masked
     // or low-precision instances); the host knows which slots take the FP64 pass, since a graph is
     // captured again whenever the active set changes.
-    bool star_lt = false;
+    bool star_lt = false, kk_lt = false;
+    Tmp ReS;   // KK ImΣ of the library product (then packed into Σ_new)
+    {
+        // Kramers-Kronig's FP64 product KK·ImΣ likewise (KK shared, the complex result packed after)
+        const char *e = getenv("EPSX_STAR_BLAS"), *shape = getenv("EPSX_GEMM_SHAPE");
+        if (!(e && e[0] == '0') && !(shape && *shape)) {
+            EPSX_TRY(tmp_alloc(ctx, ReS.b, 8 * nS));
+            EPSX_HIP(ctx, hipMemsetAsync(B.ImS.b.p, 0, 8 * nS, st));
+            EPSX_TRY(blaslt_prepare(ctx, nw, ni, nw, dptr<double>(s.KK), nw, 0, B.ImS.ptr<double>(), nw, nw * ni,
+                                    ReS.ptr<double>(), nw, nw * ni, nT, &kk_lt));
+        }
+    }
     if (method == EPSX_FM_STAR) {
         const char *e = getenv("EPSX_STAR_BLAS"), *shape = getenv("EPSX_GEMM_SHAPE");
         if (!(e && e[0] == '0') && !(shape && *shape)) {
@@ -843,6 +866,22 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     };
     auto enqueue = [&](int nA, const int32_t *d_tidx, const std::vector<int32_t> &h_tidx, const int32_t *d_act,
                        const GraphBook &book, bool events) -> int {
+        // The library products have no device mask: one strided-batched call per run of consecutive
+        // slots that take the FP64 pass (active, at the FP64 level) with consecutive temperatures.
+        auto fp64_runs = [&](auto &&call) -> int {
+            auto take = [&](int z) {
+                const int32_t t = h_tidx[z];
+                return active[t] && (!split || prec_t[t] == EPSX_PREC_FP64);
+            };
+            for (int z = 0; z < nA;) {
+                if (!take(z)) { z++; continue; }
+                int z1 = z + 1;
+                while (z1 < nA && take(z1) && h_tidx[z1] == h_tidx[z1 - 1] + 1) z1++;
+                EPSX_TRY(call(z, z1 - z));
+                z = z1;
+            }
+            return EPSX_OK;
+        };
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[0], st));
         // the (gate, precision) passes of the products: one FP64 pass, or one per level present
         struct Pass { const int32_t *gate; int prec; bool tune; };
@@ -901,21 +940,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                     EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ni, kp, bf ? hb.b.p : hf.b.p, kp,
                                        bf ? s.Gb.p : s.Gf.p, kp, B.ImS.ptr<double>(), nw, false, bt));
                 } else if (star_lt && passes[q].prec == EPSX_PREC_FP64) {
-                    // hipBLASLt, one strided-batched call per run of consecutive slots that take this
-                    // pass (active, and at the FP64 level) with consecutive temperatures
                     const int64_t K = s.nm * ni, sA = nw * K, sC = nw * ni;
-                    auto take = [&](int z) {
-                        const int32_t t = h_tidx[z];
-                        return active[t] && (!split || prec_t[t] == EPSX_PREC_FP64);
-                    };
-                    for (int z = 0; z < nA;) {
-                        if (!take(z)) { z++; continue; }
-                        int z1 = z + 1;
-                        while (z1 < nA && take(z1) && h_tidx[z1] == h_tidx[z1 - 1] + 1) z1++;
-                        EPSX_TRY(blaslt_dgemm(ctx, nw, ni, K, hbuf.ptr<double>() + sA * z, nw, sA, dptr<double>(s.Gt), K,
-                                              0, B.ImS.ptr<double>() + sC * h_tidx[z], nw, sC, z1 - z));
-                        z = z1;
-                    }
+                    EPSX_TRY(fp64_runs([&](int z, int n) {
+                        return blaslt_dgemm(ctx, nw, ni, K, hbuf.ptr<double>() + sA * z, nw, sA, dptr<double>(s.Gt), K,
+                                            0, B.ImS.ptr<double>() + sC * h_tidx[z], nw, sC, n);
+                    }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
                 } else {
                     EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
@@ -955,7 +984,17 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             bt.tune = passes[q].tune;
             bt.Z = B.Snew.ptr<double2>();
             bt.Zim = B.ImS.ptr<double>();
-            if (passes[q].prec == EPSX_PREC_FP64) {
+            if (passes[q].prec == EPSX_PREC_FP64 && kk_lt) {
+                const int64_t sC = nw * ni;
+                EPSX_TRY(fp64_runs([&](int z, int n) {
+                    const int64_t t0 = h_tidx[z];
+                    return blaslt_dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, 0, B.ImS.ptr<double>() + sC * t0, nw,
+                                        sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
+                }));
+                hipLaunchKernelGGL(complex_pack_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw * ni, d_tidx,
+                                   passes[q].gate, ReS.ptr<double>(), B.ImS.ptr<double>(), B.Snew.ptr<double2>());
+                EPSX_HIP(ctx, hipGetLastError());
+            } else if (passes[q].prec == EPSX_PREC_FP64) {
                 EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
             } else if (lp_native) {
                 // the off-diagonal part from stored BF16 / FP32 operands, the principal-value diagonal
@@ -1176,7 +1215,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 // device mask) a new active set or level of some temperature: the launches change,
                 // capture again
                 if (exec && (present_levels() != graph_levels ||
-                             (star_lt && (active != graph_active || prec_t != graph_prec)))) {
+                             ((star_lt || kk_lt) && (active != graph_active || prec_t != graph_prec)))) {
                     EPSX_HIP(ctx, hipGraphExecDestroy(exec));
                     EPSX_HIP(ctx, hipGraphDestroy(graph));
                     exec = nullptr;
