@@ -217,16 +217,16 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
 // The same product from low-precision operands stored k-contiguous (precision EPSX_PREC_BF16:
 // uint16 BF16 bits, EPSX_PREC_FP32: float; AT[k, m] and B[k, n], row pitches, strides and k
 // multiples of 8, zero-padded beyond the true k), FP32 accumulation, FP64 output.
-// C_z = A_z B_z for z < batch: column-major FP64 (strides in elements, sB may be 0) through
-// hipBLASLt with a workspace-free algorithm (no split-k workspace, no atomics: repeats give the
-// same bits).  blaslt_prepare, outside graph capture, times the heuristic's workspace-free
-// candidates on the given buffers (C overwritten) for batch counts 1..max_batch and keeps the
-// fastest per problem in the context; blaslt_dgemm then only launches.
-int blaslt_prepare(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, int64_t sA,
-                   const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc, int64_t sC, int64_t max_batch,
-                   bool *ok);
-int blaslt_dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, int64_t sA,
-                 const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc, int64_t sC, int64_t batch);
+// C_z = op(A_z) B_z for z < batch through hipBLASLt with a workspace-free algorithm (no split-k
+// workspace, no atomics: repeats give the same bits).  FP64: C = A B, column-major (strides in
+// elements, sA or sB may be 0).  BF16 / FP32: C (FP32) = Aᵀ B from k-contiguous operands (A k x m,
+// B k x n), FP32 accumulation.  blaslt_prepare, outside graph capture, times the heuristic's
+// workspace-free candidates on the given buffers (C overwritten) for batch counts 1..max_batch and
+// keeps the fastest per problem in the context; blaslt_gemm then only launches.
+int blaslt_prepare(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, int64_t sA,
+                   const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC, int64_t max_batch, bool *ok);
+int blaslt_gemm(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, int64_t sA,
+                const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC, int64_t batch);
 int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, const void *AT, int64_t lda,
               const void *B, int64_t ldb, double *C, int64_t ldc, bool accumulate, const GemmBatch &bt);
 int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A, int64_t lda,

@@ -675,7 +675,7 @@ int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, con
 
 }  // namespace epsx
 
-// ------------------------------------------------------------------ hipBLASLt (plain FP64 GEMMs)
+// ------------------------------------------------------------------ hipBLASLt (plain GEMMs)
 namespace epsx {
 namespace {
 
@@ -697,10 +697,13 @@ struct LtProduct {
     hipblasLtMatmulAlgo_t algo;
 };
 
+// (precision, m, n, k, lda, sA, ldb, sB, ldc, sC, batch) -> product; absent: no workspace-free
+// algorithm
+using LtKey = std::array<int64_t, 11>;
+
 struct BlasLt {
     hipblasLtHandle_t h = nullptr;
-    // (m, n, k, lda, sA, ldb, sB, ldc, sC, batch) -> product; absent: no workspace-free algorithm
-    std::map<std::array<int64_t, 10>, std::unique_ptr<LtProduct>> prod;
+    std::map<LtKey, std::unique_ptr<LtProduct>> prod;
 };
 
 int lt_check(epsx_ctx *ctx, hipblasStatus_t st, const char *what)
@@ -709,10 +712,10 @@ int lt_check(epsx_ctx *ctx, hipblasStatus_t st, const char *what)
     return EPSX_OK;
 }
 
-int lt_layout(epsx_ctx *ctx, hipblasLtMatrixLayout_t &l, int64_t rows, int64_t cols, int64_t ld, int64_t stride,
-              int64_t batch)
+int lt_layout(epsx_ctx *ctx, hipblasLtMatrixLayout_t &l, hipDataType t, int64_t rows, int64_t cols, int64_t ld,
+              int64_t stride, int64_t batch)
 {
-    EPSX_TRY(lt_check(ctx, hipblasLtMatrixLayoutCreate(&l, HIP_R_64F, (uint64_t)rows, (uint64_t)cols, ld), "layout"));
+    EPSX_TRY(lt_check(ctx, hipblasLtMatrixLayoutCreate(&l, t, (uint64_t)rows, (uint64_t)cols, ld), "layout"));
     if (batch > 1) {
         const int32_t bc = (int32_t)batch;
         EPSX_TRY(lt_check(ctx, hipblasLtMatrixLayoutSetAttribute(l, HIPBLASLT_MATRIX_LAYOUT_BATCH_COUNT, &bc, sizeof(bc)),
@@ -724,21 +727,46 @@ int lt_layout(epsx_ctx *ctx, hipblasLtMatrixLayout_t &l, int64_t rows, int64_t c
     return EPSX_OK;
 }
 
-int lt_descs(epsx_ctx *ctx, LtDescs &d, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t sA, int64_t ldb,
+// FP64: C = A B (A m x k, lda). BF16 / FP32: C (FP32) = Aᵀ B with A stored k x m (k-contiguous,
+// lda), B k x n, FP32 accumulation.
+int lt_descs(epsx_ctx *ctx, LtDescs &d, int prec, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t sA, int64_t ldb,
              int64_t sB, int64_t ldc, int64_t sC, int64_t batch)
 {
-    EPSX_TRY(lt_check(ctx, hipblasLtMatmulDescCreate(&d.md, HIPBLAS_COMPUTE_64F, HIP_R_64F), "matmul descriptor"));
-    EPSX_TRY(lt_layout(ctx, d.la, m, k, lda, sA, batch));
-    EPSX_TRY(lt_layout(ctx, d.lb, k, n, ldb, sB, batch));
-    EPSX_TRY(lt_layout(ctx, d.lc, m, n, ldc, sC, batch));
+    if (prec == EPSX_PREC_FP64) {
+        EPSX_TRY(lt_check(ctx, hipblasLtMatmulDescCreate(&d.md, HIPBLAS_COMPUTE_64F, HIP_R_64F), "matmul descriptor"));
+        EPSX_TRY(lt_layout(ctx, d.la, HIP_R_64F, m, k, lda, sA, batch));
+        EPSX_TRY(lt_layout(ctx, d.lb, HIP_R_64F, k, n, ldb, sB, batch));
+        EPSX_TRY(lt_layout(ctx, d.lc, HIP_R_64F, m, n, ldc, sC, batch));
+        return EPSX_OK;
+    }
+    const hipDataType t = prec == EPSX_PREC_BF16 ? HIP_R_16BF : HIP_R_32F;
+    EPSX_TRY(lt_check(ctx, hipblasLtMatmulDescCreate(&d.md, HIPBLAS_COMPUTE_32F, HIP_R_32F), "matmul descriptor"));
+    const hipblasOperation_t opT = HIPBLAS_OP_T;
+    EPSX_TRY(lt_check(ctx, hipblasLtMatmulDescSetAttribute(d.md, HIPBLASLT_MATMUL_DESC_TRANSA, &opT, sizeof(opT)),
+                      "transpose A"));
+    EPSX_TRY(lt_layout(ctx, d.la, t, k, m, lda, sA, batch));
+    EPSX_TRY(lt_layout(ctx, d.lb, t, k, n, ldb, sB, batch));
+    EPSX_TRY(lt_layout(ctx, d.lc, HIP_R_32F, m, n, ldc, sC, batch));
     return EPSX_OK;
+}
+
+int lt_run(BlasLt *b, LtProduct &pr, int prec, const void *A, const void *B, void *C, hipStream_t st,
+           const hipblasLtMatmulAlgo_t *algo)
+{
+    const double one = 1.0, zero = 0.0;
+    const float onef = 1.0f, zerof = 0.0f;
+    const void *al = prec == EPSX_PREC_FP64 ? (const void *)&one : (const void *)&onef;
+    const void *be = prec == EPSX_PREC_FP64 ? (const void *)&zero : (const void *)&zerof;
+    return hipblasLtMatmul(b->h, pr.d.md, al, A, pr.d.la, B, pr.d.lb, be, C, pr.d.lc, C, pr.d.lc, algo, nullptr, 0, st) ==
+                   HIPBLAS_STATUS_SUCCESS
+               ? 0
+               : 1;
 }
 
 }  // namespace
 
-int blaslt_prepare(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, int64_t sA,
-                   const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc, int64_t sC, int64_t max_batch,
-                   bool *ok)
+int blaslt_prepare(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, int64_t sA,
+                   const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC, int64_t max_batch, bool *ok)
 {
     *ok = false;
     if (!ctx->blaslt) {
@@ -752,11 +780,10 @@ int blaslt_prepare(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double 
     auto *b = static_cast<BlasLt *>(ctx->blaslt);
     bool all = true;
     for (int64_t batch = 1; batch <= max_batch; batch++) {
-        const std::array<int64_t, 10> key{m, n, k, lda, sA, ldb, sB, ldc, sC, batch};
+        const LtKey key{prec, m, n, k, lda, sA, ldb, sB, ldc, sC, batch};
         if (b->prod.count(key)) continue;
         auto pr = std::make_unique<LtProduct>();
-        LtDescs &d = pr->d;
-        EPSX_TRY(lt_descs(ctx, d, m, n, k, lda, sA, ldb, sB, ldc, sC, batch));
+        EPSX_TRY(lt_descs(ctx, pr->d, prec, m, n, k, lda, sA, ldb, sB, ldc, sC, batch));
         hipblasLtMatmulPreference_t pref = nullptr;
         EPSX_TRY(lt_check(ctx, hipblasLtMatmulPreferenceCreate(&pref), "preference"));
         const uint64_t wsz = 0;
@@ -766,29 +793,26 @@ int blaslt_prepare(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double 
         hipblasLtMatmulHeuristicResult_t res[NCAND];
         int nres = 0;
         if (st == HIPBLAS_STATUS_SUCCESS)
-            st = hipblasLtMatmulAlgoGetHeuristic(b->h, d.md, d.la, d.lb, d.lc, d.lc, pref, NCAND, res, &nres);
+            st = hipblasLtMatmulAlgoGetHeuristic(b->h, pr->d.md, pr->d.la, pr->d.lb, pr->d.lc, pr->d.lc, pref, NCAND, res,
+                                                 &nres);
         (void)hipblasLtMatmulPreferenceDestroy(pref);
         // the heuristic's order puts split-k algorithms first, slow without their workspace: every
         // workspace-free candidate is timed once (after a warm launch) and the fastest kept
         hipEvent_t e0, e1;
         EPSX_HIP(ctx, hipEventCreate(&e0));
         EPSX_HIP(ctx, hipEventCreate(&e1));
-        const double one = 1.0, zero = 0.0;
         int pick = -1;
         float best = 0;
         for (int r = 0; st == HIPBLAS_STATUS_SUCCESS && r < nres; r++) {
             if (res[r].state != HIPBLAS_STATUS_SUCCESS || res[r].workspaceSize != 0) continue;
-            if (hipblasLtMatmul(b->h, d.md, &one, A, d.la, B, d.lb, &zero, C, d.lc, C, d.lc, &res[r].algo, nullptr, 0,
-                                ctx->stream) != HIPBLAS_STATUS_SUCCESS)
-                continue;
+            if (lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[r].algo)) continue;
             EPSX_HIP(ctx, hipEventRecord(e0, ctx->stream));
-            const hipblasStatus_t rs = hipblasLtMatmul(b->h, d.md, &one, A, d.la, B, d.lb, &zero, C, d.lc, C, d.lc,
-                                                       &res[r].algo, nullptr, 0, ctx->stream);
+            const int rs = lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[r].algo);
             EPSX_HIP(ctx, hipEventRecord(e1, ctx->stream));
             EPSX_HIP(ctx, hipEventSynchronize(e1));
             float ms = 0;
             EPSX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-            if (rs == HIPBLAS_STATUS_SUCCESS && (pick < 0 || ms < best)) { pick = r; best = ms; }
+            if (rs == 0 && (pick < 0 || ms < best)) { pick = r; best = ms; }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -800,18 +824,16 @@ int blaslt_prepare(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double 
     return EPSX_OK;
 }
 
-int blaslt_dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64_t lda, int64_t sA,
-                 const double *B, int64_t ldb, int64_t sB, double *C, int64_t ldc, int64_t sC, int64_t batch)
+int blaslt_gemm(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, int64_t sA,
+                const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC, int64_t batch)
 {
     auto *b = static_cast<BlasLt *>(ctx->blaslt);
-    const std::array<int64_t, 10> key{m, n, k, lda, sA, ldb, sB, ldc, sC, batch};
-    auto it = b ? b->prod.find(key) : decltype(b->prod.end()){};
-    if (!b || it == b->prod.end()) return set_error(ctx, EPSX_ERR_STATE, "hipBLASLt product not prepared");
-    LtProduct &pr = *it->second;
-    const double one = 1.0, zero = 0.0;
-    return lt_check(ctx, hipblasLtMatmul(b->h, pr.d.md, &one, A, pr.d.la, B, pr.d.lb, &zero, C, pr.d.lc, C, pr.d.lc,
-                                         &pr.algo, nullptr, 0, ctx->stream),
-                    "hipblasLtMatmul");
+    if (!b) return set_error(ctx, EPSX_ERR_STATE, "hipBLASLt product not prepared");
+    const auto it = b->prod.find(LtKey{prec, m, n, k, lda, sA, ldb, sB, ldc, sC, batch});
+    if (it == b->prod.end()) return set_error(ctx, EPSX_ERR_STATE, "hipBLASLt product not prepared");
+    if (lt_run(b, *it->second, prec, A, B, C, ctx->stream, &it->second->algo))
+        return set_error(ctx, EPSX_ERR_DEVICE, "hipblasLtMatmul failed");
+    return EPSX_OK;
 }
 
 void blaslt_destroy(epsx_ctx *ctx)

@@ -46,6 +46,17 @@ __global__ void complex_pack_kernel(int64_t n, const int32_t *__restrict__ tidx,
     Z[o] = make_double2(re[o], im[o]);
 }
 
+// ImΣ (FP64) from the FP32 result of a low-precision library product, for the slots of a pass.
+__global__ void widen_kernel(int64_t n, const int32_t *__restrict__ tidx, const int32_t *__restrict__ gate,
+                             const float *__restrict__ src, double *__restrict__ dst)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int tcol = tidx[blockIdx.y];
+    if (q >= n || (gate && !gate[tcol])) return;
+    const int64_t o = q + n * (int64_t)tcol;
+    dst[o] = (double)src[o];
+}
+
 __global__ void fill_sigma_kernel(int64_t n, double2 *S, double eta_init)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -835,16 +846,17 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // which EPSX_STAR_BLAS=0 or EPSX_GEMM_SHAPE selects, and which serves every product with masked
     // or low-precision instances); the host knows which slots take the FP64 pass, since a graph is
     // captured again whenever the active set changes.
-    bool star_lt = false, kk_lt = false;
-    Tmp ReS;   // KK ImΣ of the library product (then packed into Σ_new)
+    bool star_lt = false, kk_lt = false, lp_lt[3] = {false, false, false};
+    Tmp ReS;    // KK ImΣ of the library product (then packed into Σ_new)
+    Tmp ImSf;   // FP32 result of the low-precision library products
     {
         // Kramers-Kronig's FP64 product KK·ImΣ likewise (KK shared, the complex result packed after)
         const char *e = getenv("EPSX_STAR_BLAS"), *shape = getenv("EPSX_GEMM_SHAPE");
         if (!(e && e[0] == '0') && !(shape && *shape)) {
             EPSX_TRY(tmp_alloc(ctx, ReS.b, 8 * nS));
             EPSX_HIP(ctx, hipMemsetAsync(B.ImS.b.p, 0, 8 * nS, st));
-            EPSX_TRY(blaslt_prepare(ctx, nw, ni, nw, dptr<double>(s.KK), nw, 0, B.ImS.ptr<double>(), nw, nw * ni,
-                                    ReS.ptr<double>(), nw, nw * ni, nT, &kk_lt));
+            EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ni, nw, dptr<double>(s.KK), nw, 0, B.ImS.ptr<double>(), nw,
+                                    nw * ni, ReS.ptr<double>(), nw, nw * ni, nT, &kk_lt));
         }
     }
     if (method == EPSX_FM_STAR) {
@@ -853,8 +865,23 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             // candidates timed on zeroed h (ImΣ overwritten before its first use)
             const int64_t K = s.nm * ni;
             EPSX_HIP(ctx, hipMemsetAsync(hbuf.b.p, 0, 8 * (size_t)nw * K * nT, st));
-            EPSX_TRY(blaslt_prepare(ctx, nw, ni, K, hbuf.ptr<double>(), nw, nw * K, dptr<double>(s.Gt), K, 0,
-                                    B.ImS.ptr<double>(), nw, nw * ni, nT, &star_lt));
+            EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ni, K, hbuf.ptr<double>(), nw, nw * K, dptr<double>(s.Gt),
+                                    K, 0, B.ImS.ptr<double>(), nw, nw * ni, nT, &star_lt));
+            // the low-precision levels' products from the stored BF16 / FP32 h and G̃ (FP32 result,
+            // widened into ImΣ)
+            if (lp_native) {
+                EPSX_TRY(tmp_alloc(ctx, ImSf.b, 4 * nS));
+                if (use_bf) {
+                    EPSX_HIP(ctx, hipMemsetAsync(hb.b.p, 0, 2 * (size_t)kp * nw * nT, st));
+                    EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, nw, ni, kp, hb.b.p, kp, kp * nw, s.Gb.p, kp, 0,
+                                            ImSf.ptr<float>(), nw, nw * ni, nT, &lp_lt[EPSX_PREC_BF16]));
+                }
+                if (use_f32) {
+                    EPSX_HIP(ctx, hipMemsetAsync(hf.b.p, 0, 4 * (size_t)kp * nw * nT, st));
+                    EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP32, nw, ni, kp, hf.b.p, kp, kp * nw, s.Gf.p, kp, 0,
+                                            ImSf.ptr<float>(), nw, nw * ni, nT, &lp_lt[EPSX_PREC_FP32]));
+                }
+            }
         }
     }
     // the levels present among the active temperatures, as a bit mask
@@ -867,11 +894,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     auto enqueue = [&](int nA, const int32_t *d_tidx, const std::vector<int32_t> &h_tidx, const int32_t *d_act,
                        const GraphBook &book, bool events) -> int {
         // The library products have no device mask: one strided-batched call per run of consecutive
-        // slots that take the FP64 pass (active, at the FP64 level) with consecutive temperatures.
-        auto fp64_runs = [&](auto &&call) -> int {
+        // slots that take a pass (active, at that level) with consecutive temperatures.
+        auto prec_runs = [&](int prec, auto &&call) -> int {
             auto take = [&](int z) {
                 const int32_t t = h_tidx[z];
-                return active[t] && (!split || prec_t[t] == EPSX_PREC_FP64);
+                return active[t] && (prec == EPSX_PREC_FP64 ? (!split || prec_t[t] == EPSX_PREC_FP64) : prec_t[t] == prec);
             };
             for (int z = 0; z < nA;) {
                 if (!take(z)) { z++; continue; }
@@ -882,6 +909,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             }
             return EPSX_OK;
         };
+        auto fp64_runs = [&](auto &&call) { return prec_runs(EPSX_PREC_FP64, call); };
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[0], st));
         // the (gate, precision) passes of the products: one FP64 pass, or one per level present
         struct Pass { const int32_t *gate; int prec; bool tune; };
@@ -934,7 +962,19 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 bt.precision = passes[q].prec;
                 bt.tune = passes[q].tune;
                 if (events) bt.mark = ev[6];
-                if (lp_native && passes[q].prec != EPSX_PREC_FP64) {
+                if (lp_native && passes[q].prec != EPSX_PREC_FP64 && lp_lt[passes[q].prec]) {
+                    const bool bf = passes[q].prec == EPSX_PREC_BF16;
+                    const int64_t esz = bf ? 2 : 4, sC = nw * ni;
+                    const char *hA = static_cast<const char *>(bf ? hb.b.p : hf.b.p);
+                    EPSX_TRY(prec_runs(passes[q].prec, [&](int z, int n) {
+                        return blaslt_gemm(ctx, passes[q].prec, nw, ni, kp, hA + esz * kp * nw * z, kp, kp * nw,
+                                           bf ? s.Gb.p : s.Gf.p, kp, 0, ImSf.ptr<float>() + sC * h_tidx[z], nw, sC, n);
+                    }));
+                    if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
+                    hipLaunchKernelGGL(widen_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw * ni, d_tidx,
+                                       passes[q].gate, ImSf.ptr<float>(), B.ImS.ptr<double>());
+                    EPSX_HIP(ctx, hipGetLastError());
+                } else if (lp_native && passes[q].prec != EPSX_PREC_FP64) {
                     const bool bf = passes[q].prec == EPSX_PREC_BF16;
                     bt.sA = kp * nw;
                     EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ni, kp, bf ? hb.b.p : hf.b.p, kp,
@@ -942,8 +982,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 } else if (star_lt && passes[q].prec == EPSX_PREC_FP64) {
                     const int64_t K = s.nm * ni, sA = nw * K, sC = nw * ni;
                     EPSX_TRY(fp64_runs([&](int z, int n) {
-                        return blaslt_dgemm(ctx, nw, ni, K, hbuf.ptr<double>() + sA * z, nw, sA, dptr<double>(s.Gt), K,
-                                            0, B.ImS.ptr<double>() + sC * h_tidx[z], nw, sC, n);
+                        return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ni, K, hbuf.ptr<double>() + sA * z, nw, sA,
+                                           dptr<double>(s.Gt), K, 0, B.ImS.ptr<double>() + sC * h_tidx[z], nw, sC, n);
                     }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
                 } else {
@@ -988,8 +1028,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 const int64_t sC = nw * ni;
                 EPSX_TRY(fp64_runs([&](int z, int n) {
                     const int64_t t0 = h_tidx[z];
-                    return blaslt_dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, 0, B.ImS.ptr<double>() + sC * t0, nw,
-                                        sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
+                    return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ni, nw, dptr<double>(s.KK), nw, 0,
+                                       B.ImS.ptr<double>() + sC * t0, nw, sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
                 }));
                 hipLaunchKernelGGL(complex_pack_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw * ni, d_tidx,
                                    passes[q].gate, ReS.ptr<double>(), B.ImS.ptr<double>(), B.Snew.ptr<double2>());
@@ -1215,7 +1255,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 // device mask) a new active set or level of some temperature: the launches change,
                 // capture again
                 if (exec && (present_levels() != graph_levels ||
-                             ((star_lt || kk_lt) && (active != graph_active || prec_t != graph_prec)))) {
+                             ((star_lt || kk_lt || lp_lt[1] || lp_lt[2]) &&
+                              (active != graph_active || prec_t != graph_prec)))) {
                     EPSX_HIP(ctx, hipGraphExecDestroy(exec));
                     EPSX_HIP(ctx, hipGraphDestroy(graph));
                     exec = nullptr;
