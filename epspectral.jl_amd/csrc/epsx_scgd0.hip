@@ -849,10 +849,13 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     bool star_lt = false, kk_lt = false, lp_lt[3] = {false, false, false};
     Tmp ReS;    // KK ImΣ of the library product (then packed into Σ_new)
     Tmp ImSf;   // FP32 result of the low-precision library products
+    // (a run of one or two iterations does not pay for timing the library's candidates: it keeps
+    // the library's own kernels, as it keeps their default tile shape)
+    const char *blas_env = getenv("EPSX_STAR_BLAS"), *shape_env = getenv("EPSX_GEMM_SHAPE");
+    const bool use_lt = maxiter >= 3 && !(blas_env && blas_env[0] == '0') && !(shape_env && *shape_env);
     {
         // Kramers-Kronig's FP64 product KK·ImΣ likewise (KK shared, the complex result packed after)
-        const char *e = getenv("EPSX_STAR_BLAS"), *shape = getenv("EPSX_GEMM_SHAPE");
-        if (!(e && e[0] == '0') && !(shape && *shape)) {
+        if (use_lt) {
             EPSX_TRY(tmp_alloc(ctx, ReS.b, 8 * nS));
             EPSX_HIP(ctx, hipMemsetAsync(B.ImS.b.p, 0, 8 * nS, st));
             EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ni, nw, dptr<double>(s.KK), nw, 0, B.ImS.ptr<double>(), nw,
@@ -860,8 +863,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         }
     }
     if (method == EPSX_FM_STAR) {
-        const char *e = getenv("EPSX_STAR_BLAS"), *shape = getenv("EPSX_GEMM_SHAPE");
-        if (!(e && e[0] == '0') && !(shape && *shape)) {
+        if (use_lt) {
             // candidates timed on zeroed h (ImΣ overwritten before its first use)
             const int64_t K = s.nm * ni;
             EPSX_HIP(ctx, hipMemsetAsync(hbuf.b.p, 0, 8 * (size_t)nw * K * nT, st));
