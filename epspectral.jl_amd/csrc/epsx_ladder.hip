@@ -404,19 +404,18 @@ __global__ void shift_add_chunk_kernel(int64_t nw, int64_t ncol, int64_t ns, int
     const int64_t r = q % nrow, col = q / nrow;
     const int64_t j = r % nw, qb = r - j;
     double sp = accumulate ? Cp[q] : 0.0, sm = accumulate ? Cm[q] : 0.0;
+    // every tap is read (klo is clamped to a valid cell, its weights are zero outside the grid), so
+    // the loads of several shifts are in flight together instead of waiting behind branches
+#pragma unroll 4
     for (int64_t mm = 0; mm < mb; mm++) {
         const int64_t m = m0 + mm;
-        const TW *Wm = W + nrow * (col + ncol * mm);
+        const TW *Wm = W + nrow * (col + ncol * mm) + qb;
         const int64_t t0 = j + nw * m, t1 = j + nw * (m + ns);
+        const int64_t k0 = klo[t0] - 1, k1 = klo[t1] - 1;
         const double lo0 = wlo[t0], hi0 = whi[t0], lo1 = wlo[t1], hi1 = whi[t1];
-        if (lo0 != 0 || hi0 != 0) {
-            const int64_t k = klo[t0] - 1;
-            sp += lo0 * (double)Wm[qb + k] + hi0 * (double)Wm[qb + k + 1];
-        }
-        if (lo1 != 0 || hi1 != 0) {
-            const int64_t k = klo[t1] - 1;
-            sm += lo1 * (double)Wm[qb + k] + hi1 * (double)Wm[qb + k + 1];
-        }
+        const double a0 = (double)Wm[k0], b0 = (double)Wm[k0 + 1], a1 = (double)Wm[k1], b1 = (double)Wm[k1 + 1];
+        sp += lo0 * a0 + hi0 * b0;
+        sm += lo1 * a1 + hi1 * b1;
     }
     Cp[q] = sp;
     Cm[q] = sm;
