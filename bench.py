@@ -296,7 +296,7 @@ def ladder_gemm_label(precision):
                 else f"lpkgemm_kernel (shift-table GEMM W = J B̃ from stored {precision} operands, FP32 accumulation)")
     return {"fp64": "rocBLAS dgemm (shift-table GEMM W = J B̃, FP64)",
             "fp32": "rocBLAS sgemm (shift-table GEMM W = J B̃ from stored FP32 operands)",
-            "bf16": "rocBLAS gemm_ex (shift-table GEMM W = J B̃ from stored BF16 operands, FP32 accumulation)"}[precision]
+            "bf16": "hipBLASLt BF16 GEMM (shift-table GEMM W = J B̃ from stored BF16 operands, FP32 accumulation)"}[precision]
 
 
 def ladder_cpu_baseline(nw, ni, ns, apps_per_solve, budget_s):

@@ -433,8 +433,8 @@ static int ldot(epsx_ctx *ctx, int64_t n, const double2 *x, const double2 *y, do
 }
 
 // The shift-table product W = J B̃ is a plain GEMM (no epilogue, no batching): it runs in rocBLAS
-// (deterministic: atomics off) on the context's stream; EPSX_LADDER_BLAS=0 runs the library's own
-// MFMA kernels instead (for comparisons).
+// (deterministic: atomics off; BF16 in hipBLASLt) on the context's stream; EPSX_LADDER_BLAS=0 runs
+// the library's own MFMA kernels instead (for comparisons).
 static bool use_blas()
 {
     const char *e = getenv("EPSX_LADDER_BLAS");
@@ -545,7 +545,17 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int pr
             rocblas_handle h;
             EPSX_TRY(blas_handle(ctx, h));
             const int bm = (int)nrow, bn = (int)(ncol * mc);
-            if (native) {
+            // BF16: hipBLASLt's fastest workspace-free algorithm (897 TF/s on this shape against
+            // rocBLAS's 735), timed once per chunk shape on the first application
+            bool lt = false;
+            if (native && bf)
+                EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, nrow, ncol * mc, kp, l.JmL.p, kp, 0,
+                                        static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0, kp, 0, l.W.p, nrow, 0,
+                                        1, &lt));
+            if (lt) {
+                EPSX_TRY(blaslt_gemm(ctx, EPSX_PREC_BF16, nrow, ncol * mc, kp, l.JmL.p, kp, 0,
+                                     static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0, kp, 0, l.W.p, nrow, 0, 1));
+            } else if (native) {
                 // W (FP32) = JmLᵀ Bl from the k-contiguous BF16 / FP32 operands, FP32 accumulation
                 const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
                 const float one = 1.0f, zero = 0.0f;
