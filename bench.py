@@ -118,12 +118,14 @@ def main():
     g2_bytes = 8.0 * nm * ni * nf
 
     if method_used == 2:
-        # dominant kernel: the FP64 MFMA GEMM ImΣ = h·G̃, nω x (nm ni) x ni per temperature
-        # (the HIP events time the first iteration of each run: with a low-precision mode that is
-        # the BF16 / FP32 product, priced against its own MFMA peak)
-        # one launch covers every temperature of the batch (star_gemm_instances over the launches)
+        # dominant kernel: the GEMM ImΣ = h·G̃, nω x (nm ni) x ncol per temperature, ncol the
+        # product's columns: one per multiplet (its members' ImΣ is the average the loop takes)
+        # and one per other outer state (the HIP events time the second iteration of each run:
+        # with a low-precision mode that is the BF16 / FP32 product, priced against its own peak);
+        # one call covers every temperature of the batch (star_gemm_instances over the calls)
+        ncol = ni - sum(len(m) for m in setup.multiplets) + len(setup.multiplets)
         per_launch = tsum["star_gemm_instances"] / max(tsum["star_gemm_launches"], 1)
-        flops = 2.0 * nw * (nm * ni) * ni * per_launch
+        flops = 2.0 * nw * (nm * ni) * ncol * per_launch
         avg = tsum["star_gemm"] / max(tsum["star_gemm_launches"], 1)
         achieved = flops / avg / 1e12
         if args.precision == "fp64":
@@ -138,7 +140,8 @@ def main():
             tr = None
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": tr, "kernel": kname,
-                "algorithmic_flop_per_launch": flops, "avg_launch_s": avg, "temperatures_per_launch": per_launch,
+                "algorithmic_flop_per_launch": flops, "product_columns": ncol, "avg_launch_s": avg,
+                "temperatures_per_launch": per_launch,
                 "launches_timed": int(tsum["star_gemm_launches"])}
         sr = tsum["star_reduce"] / max(tsum["star_reduce_launches"], 1)
         sr_bytes = g2_bytes + 8.0 * nf + 8.0 * nm * ni * ni

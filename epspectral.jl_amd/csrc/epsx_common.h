@@ -52,7 +52,13 @@ struct Scgd0Dev {
     DevBuf Gt;                        // star-collapsed coupling G̃ [(ν, i'), i] (built on demand)
     bool gt_ready = false;            // G̃ holds the current problem's reduction ...
     double gt_wac = 0;                // ... with modes below this ω_acoustic skipped
-    DevBuf Gb, Gf;                    // G̃ in BF16 / FP32, k-contiguous rows of pitch kp (on demand)
+    // Column classes of the star product: one per multiplet, one per other outer state (nc of
+    // them; 0: none, the product runs over all ni columns).  G̃c holds G̃ with the columns of each
+    // multiplet averaged, so the product's columns are the multiplet averages themselves.
+    int64_t nc = 0;
+    DevBuf cls, cls_ptr, cls_idx;     // class of each state; members of each class (CSR, 0-based)
+    DevBuf Gtc;                       // [(ν, i'), class]
+    DevBuf Gb, Gf;                    // G̃ (G̃c) in BF16 / FP32, k-contiguous rows of pitch kp (on demand)
     bool gb_ready = false, gf_ready = false;   // they hold the current G̃
     DevBuf KKb, KKf;                  // off-diagonal KK transposed in BF16 / FP32, pitch nw rounded to 8
     bool kkb_ready = false, kkf_ready = false;

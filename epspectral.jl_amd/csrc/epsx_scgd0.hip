@@ -57,6 +57,34 @@ __global__ void widen_kernel(int64_t n, const int32_t *__restrict__ tidx, const 
     dst[o] = (double)src[o];
 }
 
+// G̃c[:, c] = Σ_{members m of class c, in order} G̃[:, m] / |c|: the multiplet average of
+// src/scgd0.jl:294-300 applied to the columns of the linear map ImΣ = h G̃.
+__global__ void gt_classes_kernel(int64_t K, int64_t nc, const int64_t *__restrict__ cptr,
+                                  const int32_t *__restrict__ cidx, const double *__restrict__ Gt,
+                                  double *__restrict__ Gtc)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= K * nc) return;
+    const int64_t k = q % K, c = q / K;
+    const int64_t a = cptr[c], b = cptr[c + 1];
+    double sum = 0;
+    for (int64_t j = a; j < b; j++) sum += Gt[k + K * (int64_t)cidx[j]];
+    Gtc[q] = b - a > 1 ? sum / (double)(b - a) : sum;
+}
+
+// ImΣ[:, i] = ImΣc[:, class(i)] for the active slots (the members of a multiplet bitwise equal).
+__global__ void cls_scatter_kernel(int64_t nw, int64_t ni, int64_t nc, const int32_t *__restrict__ cls,
+                                   const int32_t *__restrict__ tidx, const double *__restrict__ ImSc,
+                                   double *__restrict__ ImS, const int32_t *__restrict__ act)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nw * ni) return;
+    const int tcol = tidx[blockIdx.y];
+    if (act && !act[tcol]) return;
+    const int64_t w = q % nw, i = q / nw;
+    ImS[q + nw * ni * (int64_t)tcol] = ImSc[w + nw * (int64_t)cls[i] + nw * nc * (int64_t)tcol];
+}
+
 __global__ void fill_sigma_kernel(int64_t n, double2 *S, double eta_init)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -623,9 +651,43 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     s.w_i_host.assign(P->w_i, P->w_i + ni);
     ctx->l.ready = false;  // a ladder problem belongs to the scGD0 problem it was set on
     ctx->l.bt_T = -1;
+    s.nc = 0;
     if (P->n_mult > 0) {
         EPSX_TRY(upload(ctx, s.mult_ptr, P->mult_ptr, sizeof(int64_t) * (P->n_mult + 1)));
         EPSX_TRY(upload(ctx, s.mult_idx, P->mult_idx, sizeof(int32_t) * P->mult_ptr[P->n_mult]));
+        // column classes of the star product in order of each class's first state (disjoint
+        // multiplets only; otherwise the product keeps every column)
+        std::vector<int64_t> mult_of(ni, -1);
+        bool disjoint = true;
+        for (int64_t m = 0; m < P->n_mult && disjoint; m++)
+            for (int64_t k = P->mult_ptr[m]; k < P->mult_ptr[m + 1]; k++) {
+                const int64_t i = P->mult_idx[k] - 1;
+                if (mult_of[i] >= 0) { disjoint = false; break; }
+                mult_of[i] = m;
+            }
+        if (disjoint) {
+            std::vector<int32_t> cls(ni, -1), cidx;
+            std::vector<int64_t> cptr{0};
+            for (int64_t i = 0; i < ni; i++) {
+                if (cls[i] >= 0) continue;
+                const int32_t c = (int32_t)(cptr.size() - 1);
+                if (mult_of[i] < 0) {
+                    cls[i] = c;
+                    cidx.push_back((int32_t)i);
+                } else {
+                    const int64_t m = mult_of[i];
+                    for (int64_t k = P->mult_ptr[m]; k < P->mult_ptr[m + 1]; k++) {
+                        cls[P->mult_idx[k] - 1] = c;
+                        cidx.push_back(P->mult_idx[k] - 1);
+                    }
+                }
+                cptr.push_back((int64_t)cidx.size());
+            }
+            s.nc = (int64_t)cptr.size() - 1;
+            EPSX_TRY(upload(ctx, s.cls, cls.data(), sizeof(int32_t) * ni));
+            EPSX_TRY(upload(ctx, s.cls_ptr, cptr.data(), sizeof(int64_t) * cptr.size()));
+            EPSX_TRY(upload(ctx, s.cls_idx, cidx.data(), sizeof(int32_t) * cidx.size()));
+        }
     }
     // g2: raw upload, then the star-ordered layout [ν, p, i] on the device
     {
@@ -710,7 +772,10 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // before any graph capture: a change of precision level captures new products
     EPSX_TRY(dev_alloc(ctx, ctx->gemm_ws, 8 * (size_t)nw * ni * 16 * nT));
     for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
-    Tmp hbuf, hocc, hb, hf, imb, imf;
+    Tmp hbuf, hocc, hb, hf, imb, imf, ImSc;
+    // the star product's columns: the classes when multiplets are present (G̃c), else every state
+    const bool use_cls = method == EPSX_FM_STAR && s.nc > 0;
+    const int64_t ncol = use_cls ? s.nc : ni;
     // Low-precision levels of the star method: h and G̃ stored in BF16 / FP32, k-contiguous with the
     // row pitch kp (nm ni rounded up to a multiple of 8), multiplied by the low-precision MFMA
     // kernel.  EPSX_LP_STAGED=1 rounds the FP64 operands as they are staged instead (for tests).
@@ -745,20 +810,29 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             (void)hipEventDestroy(e1);
             ctx->t_class[4] = ms * 1e-3;
             ctx->n_class[4] = 1;
+            if (s.nc > 0) {
+                EPSX_TRY(dev_alloc(ctx, s.Gtc, 8 * (size_t)s.nm * ni * s.nc));
+                hipLaunchKernelGGL(gt_classes_kernel, dim3(nblk(s.nm * ni * s.nc)), dim3(256), 0, st, s.nm * ni, s.nc,
+                                   dptr<int64_t>(s.cls_ptr), dptr<int32_t>(s.cls_idx), dptr<double>(s.Gt),
+                                   dptr<double>(s.Gtc));
+                EPSX_HIP(ctx, hipGetLastError());
+            }
             s.gt_ready = true;
             s.gb_ready = s.gf_ready = false;
             s.gt_wac = par->w_acoustic;
         }
+        const double *Gprod = use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt);
         if (use_bf && !s.gb_ready) {
-            EPSX_TRY(dev_alloc(ctx, s.Gb, 2 * (size_t)kp * ni));
-            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_BF16, s.nm * ni, kp, ni, dptr<double>(s.Gt), s.Gb.p));
+            EPSX_TRY(dev_alloc(ctx, s.Gb, 2 * (size_t)kp * ncol));
+            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_BF16, s.nm * ni, kp, ncol, Gprod, s.Gb.p));
             s.gb_ready = true;
         }
         if (use_f32 && !s.gf_ready) {
-            EPSX_TRY(dev_alloc(ctx, s.Gf, 4 * (size_t)kp * ni));
-            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_FP32, s.nm * ni, kp, ni, dptr<double>(s.Gt), s.Gf.p));
+            EPSX_TRY(dev_alloc(ctx, s.Gf, 4 * (size_t)kp * ncol));
+            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_FP32, s.nm * ni, kp, ncol, Gprod, s.Gf.p));
             s.gf_ready = true;
         }
+        if (use_cls) EPSX_TRY(tmp_alloc(ctx, ImSc.b, 8 * (size_t)nw * ncol * nT));
         if (use_bf) EPSX_TRY(tmp_alloc(ctx, hb.b, 2 * (size_t)kp * nw * nT));
         if (use_f32) EPSX_TRY(tmp_alloc(ctx, hf.b, 4 * (size_t)kp * nw * nT));
         // Kramers-Kronig's off-diagonal at the low-precision levels: KKo transposed once, ImΣ
@@ -867,21 +941,22 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             // candidates timed on zeroed h (ImΣ overwritten before its first use)
             const int64_t K = s.nm * ni;
             EPSX_HIP(ctx, hipMemsetAsync(hbuf.b.p, 0, 8 * (size_t)nw * K * nT, st));
-            EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ni, K, hbuf.ptr<double>(), nw, nw * K, dptr<double>(s.Gt),
-                                    K, 0, B.ImS.ptr<double>(), nw, nw * ni, nT, &star_lt));
+            EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>(), nw, nw * K,
+                                    use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt), K, 0,
+                                    use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), nw, nw * ncol, nT, &star_lt));
             // the low-precision levels' products from the stored BF16 / FP32 h and G̃ (FP32 result,
             // widened into ImΣ)
             if (lp_native) {
                 EPSX_TRY(tmp_alloc(ctx, ImSf.b, 4 * nS));
                 if (use_bf) {
                     EPSX_HIP(ctx, hipMemsetAsync(hb.b.p, 0, 2 * (size_t)kp * nw * nT, st));
-                    EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, nw, ni, kp, hb.b.p, kp, kp * nw, s.Gb.p, kp, 0,
-                                            ImSf.ptr<float>(), nw, nw * ni, nT, &lp_lt[EPSX_PREC_BF16]));
+                    EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, nw, ncol, kp, hb.b.p, kp, kp * nw, s.Gb.p, kp, 0,
+                                            ImSf.ptr<float>(), nw, nw * ncol, nT, &lp_lt[EPSX_PREC_BF16]));
                 }
                 if (use_f32) {
                     EPSX_HIP(ctx, hipMemsetAsync(hf.b.p, 0, 4 * (size_t)kp * nw * nT, st));
-                    EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP32, nw, ni, kp, hf.b.p, kp, kp * nw, s.Gf.p, kp, 0,
-                                            ImSf.ptr<float>(), nw, nw * ni, nT, &lp_lt[EPSX_PREC_FP32]));
+                    EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP32, nw, ncol, kp, hf.b.p, kp, kp * nw, s.Gf.p, kp, 0,
+                                            ImSf.ptr<float>(), nw, nw * ncol, nT, &lp_lt[EPSX_PREC_FP32]));
                 }
             }
         }
@@ -952,12 +1027,15 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                                     gate_hbf ? hb.ptr<uint16_t>() : nullptr, gate_hbf,
                                     gate_hf ? hf.ptr<float>() : nullptr, gate_hf, kp));
             if (events) EPSX_HIP(ctx, hipEventRecord(ev[5], st));
-            // ImΣ[:, :, t] = h_z G̃ for every slot z in one batched launch (G̃ shared)
+            // ImΣ[:, :, t] = h_z G̃ for every slot z in one batched launch (G̃ shared); with multiplets
+            // the class columns (G̃c) into ImΣc, copied to every member afterwards
+            double *Cprod = use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>();
+            const double *Gprod = use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt);
             for (int q = 0; q < npass; q++) {
                 GemmBatch bt;
                 bt.n_inst = nA;
                 bt.sA = nw * s.nm * ni;
-                bt.sC = nw * ni;
+                bt.sC = nw * ncol;
                 bt.idx = d_tidx;
                 bt.imap = 4;
                 bt.gate = passes[q].gate;
@@ -966,32 +1044,37 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 if (events) bt.mark = ev[6];
                 if (lp_native && passes[q].prec != EPSX_PREC_FP64 && lp_lt[passes[q].prec]) {
                     const bool bf = passes[q].prec == EPSX_PREC_BF16;
-                    const int64_t esz = bf ? 2 : 4, sC = nw * ni;
+                    const int64_t esz = bf ? 2 : 4, sC = nw * ncol;
                     const char *hA = static_cast<const char *>(bf ? hb.b.p : hf.b.p);
                     EPSX_TRY(prec_runs(passes[q].prec, [&](int z, int n) {
-                        return blaslt_gemm(ctx, passes[q].prec, nw, ni, kp, hA + esz * kp * nw * z, kp, kp * nw,
+                        return blaslt_gemm(ctx, passes[q].prec, nw, ncol, kp, hA + esz * kp * nw * z, kp, kp * nw,
                                            bf ? s.Gb.p : s.Gf.p, kp, 0, ImSf.ptr<float>() + sC * h_tidx[z], nw, sC, n);
                     }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
-                    hipLaunchKernelGGL(widen_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw * ni, d_tidx,
-                                       passes[q].gate, ImSf.ptr<float>(), B.ImS.ptr<double>());
+                    hipLaunchKernelGGL(widen_kernel, dim3(nblk(nw * ncol), nA), dim3(256), 0, st, nw * ncol, d_tidx,
+                                       passes[q].gate, ImSf.ptr<float>(), Cprod);
                     EPSX_HIP(ctx, hipGetLastError());
                 } else if (lp_native && passes[q].prec != EPSX_PREC_FP64) {
                     const bool bf = passes[q].prec == EPSX_PREC_BF16;
                     bt.sA = kp * nw;
-                    EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ni, kp, bf ? hb.b.p : hf.b.p, kp,
-                                       bf ? s.Gb.p : s.Gf.p, kp, B.ImS.ptr<double>(), nw, false, bt));
+                    EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ncol, kp, bf ? hb.b.p : hf.b.p, kp,
+                                       bf ? s.Gb.p : s.Gf.p, kp, Cprod, nw, false, bt));
                 } else if (star_lt && passes[q].prec == EPSX_PREC_FP64) {
-                    const int64_t K = s.nm * ni, sA = nw * K, sC = nw * ni;
+                    const int64_t K = s.nm * ni, sA = nw * K, sC = nw * ncol;
                     EPSX_TRY(fp64_runs([&](int z, int n) {
-                        return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ni, K, hbuf.ptr<double>() + sA * z, nw, sA,
-                                           dptr<double>(s.Gt), K, 0, B.ImS.ptr<double>() + sC * h_tidx[z], nw, sC, n);
+                        return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>() + sA * z, nw, sA, Gprod,
+                                           K, 0, Cprod + sC * h_tidx[z], nw, sC, n);
                     }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
                 } else {
-                    EPSX_TRY(dgemm(ctx, nw, ni, s.nm * ni, hbuf.ptr<double>(), nw, dptr<double>(s.Gt), s.nm * ni,
-                                   B.ImS.ptr<double>(), nw, false, bt));
+                    EPSX_TRY(dgemm(ctx, nw, ncol, s.nm * ni, hbuf.ptr<double>(), nw, Gprod, s.nm * ni, Cprod, nw,
+                                   false, bt));
                 }
+            }
+            if (use_cls) {
+                hipLaunchKernelGGL(cls_scatter_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw, ni, s.nc,
+                                   dptr<int32_t>(s.cls), d_tidx, ImSc.ptr<double>(), B.ImS.ptr<double>(), d_act);
+                EPSX_HIP(ctx, hipGetLastError());
             }
         } else {
             for (int q = 0; q < npass; q++) {
@@ -1057,8 +1140,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             }
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[2], st));
-        // 5. multiplets
-        if (s.n_mult > 0) {
+        // 5. multiplets (the star product with classes already gave every member the average)
+        if (s.n_mult > 0 && !use_cls) {
             hipLaunchKernelGGL(multiplet_kernel, dim3(nblk(nw * s.n_mult), nA), dim3(256), 0, st, nw, ni, s.n_mult,
                                dptr<int64_t>(s.mult_ptr), dptr<int32_t>(s.mult_idx), d_tidx, B.Snew.ptr<double2>(),
                                d_act);
