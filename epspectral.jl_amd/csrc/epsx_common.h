@@ -109,6 +109,8 @@ struct epsx_ctx {
     epsx::DevBuf gemm_ws;   // split-k partial products
     void *blas = nullptr;   // rocBLAS handle on `stream` (created on first use), for plain GEMMs
     void *blaslt = nullptr; // hipBLASLt handle and its chosen algorithms (epsx_gemm.hip)
+    void *pin = nullptr;    // pinned host staging of read_back (kept for the next call)
+    size_t pin_bytes = 0;
     std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, split) -> tile shape
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
@@ -154,6 +156,12 @@ inline T *dptr(DevBuf &b) { return static_cast<T *>(b.p); }
 
 // Upload a host array into a (re)allocated device buffer.
 int upload(epsx_ctx *ctx, DevBuf &b, const void *host, size_t bytes);
+
+// Copy device memory into any host array after the work on the context's stream: through a pinned
+// staging buffer (at most 64 MiB per pass, kept in the context), then host threads copy out of it
+// (a pageable destination is copied at host memory speed, its first-touch page faults spread over
+// the threads).
+int read_back(epsx_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 // Host-side grid maps (epsx_grids.cpp), shared by the C ABI and the loop set-up.
 int64_t cell_search_table(int64_t n, const double *ws, double dw_min, double *x0,

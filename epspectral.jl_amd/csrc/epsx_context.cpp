@@ -1,6 +1,8 @@
 // Context, errors and device buffers of libepsx.
 #include <cstdarg>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "epsx_common.h"
 
@@ -71,6 +73,42 @@ int tmp_upload(epsx_ctx *ctx, Tmp &t, const void *host, size_t bytes)
     return EPSX_OK;
 }
 
+int read_back(epsx_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t STAGE = size_t(64) << 20;
+    const size_t stage = bytes < STAGE ? bytes : STAGE;
+    if (bytes == 0) return EPSX_OK;
+    if (ctx->pin_bytes < stage) {
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_bytes = 0;
+        EPSX_HIP(ctx, hipHostMalloc(&ctx->pin, stage, hipHostMallocDefault));
+        ctx->pin_bytes = stage;
+    }
+    unsigned hw = std::thread::hardware_concurrency();
+    hw = hw == 0 ? 1 : (hw > 8 ? 8 : hw);
+    for (size_t off = 0; off < bytes; off += stage) {
+        const size_t n = bytes - off < stage ? bytes - off : stage;
+        EPSX_HIP(ctx, hipMemcpyAsync(ctx->pin, static_cast<const char *>(src) + off, n, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // slices of at least 1 MiB, one per thread, the first on this thread
+        const size_t want = n >> 20;
+        const unsigned nt = want < 1 ? 1u : (want < hw ? (unsigned)want : hw);
+        const size_t per = (n + nt - 1) / nt;
+        char *d = static_cast<char *>(dst) + off;
+        const char *h = static_cast<const char *>(ctx->pin);
+        std::vector<std::thread> th;
+        for (unsigned k = 1; k < nt; k++) {
+            const size_t a = per * k, b = a + per < n ? a + per : n;
+            if (a < b) th.emplace_back([=] { std::memcpy(d + a, h + a, b - a); });
+        }
+        std::memcpy(d, h, per < n ? per : n);
+        for (auto &t : th) t.join();
+    }
+    return EPSX_OK;
+}
+
 int upload(epsx_ctx *ctx, DevBuf &b, const void *host, size_t bytes)
 {
     EPSX_TRY(dev_alloc(ctx, b, bytes));
@@ -109,6 +147,7 @@ void epsx_close(epsx_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     epsx::blas_destroy(ctx);
     epsx::blaslt_destroy(ctx);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     for (auto *b : ctx->owned) epsx::dev_free(*b);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

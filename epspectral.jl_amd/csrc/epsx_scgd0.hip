@@ -2,6 +2,7 @@
 // the loop of Fan-Migdal sum, Kramers-Kronig, multiplet average, spectral DOS, μ solve and the
 // convergence test.  Everything stays on the device; per iteration the host reads back only
 // err, μ, min ∫A and a status word per temperature.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -91,9 +92,6 @@ __global__ void fill_sigma_kernel(int64_t n, double2 *S, double eta_init)
     if (q < n) S[q] = make_double2(0.0, -eta_init);
 }
 
-// the device iteration counter of the graph-replayed loop
-__global__ void iter_inc_kernel(int32_t *it) { if (threadIdx.x == 0 && blockIdx.x == 0) *it += 1; }
-
 // Multiplet average (src/scgd0.jl:294-300): members summed in order, divided by the count,
 // written to every member (so members are bitwise equal).
 __global__ void multiplet_kernel(int64_t nw, int64_t ni, int64_t n_mult, const int64_t *__restrict__ ptr,
@@ -134,17 +132,17 @@ __device__ inline double block_sum(double v, double *red)
     return r;
 }
 
-// The spectral DOS in one pass over (a, i) (src/scgd0.jl:304-308): A[a, i] = -Im(inv(ωa - ε_i -
-// Σ(ωa)))/π with Julia's scaled complex inverse, formed in registers and never stored.  One block
-// per (tile of 64 a, chunk of DOS_IC outer states, slot): lanes along a, the 4 waves over
-// i ≡ wave mod 4 of the chunk.  It writes the chunk's partial Dp[a, chunk] = Σ_i w_i A[a, i] (the
-// wave sums in order) and, per i, the tile's partial integral ip[i, tile] = Σ_a A[a, i] Wa[a] (a
-// wave butterfly); dos_finish_kernel and integ_kernel sum them in order.
+// The spectral DOS of Σ_new (already copied into Σ) in one pass over (a, i) (src/scgd0.jl:304-308):
+// A[a, i] = -Im(1/z)/π = Im z / (π |z|²) with z = ωa - ε_i - Σ(ωa), formed in registers, never stored.
+// One block per (tile of 64 a, chunk of DOS_IC outer states, slot): lanes along a, the 4 waves
+// over i ≡ wave mod 4 of the chunk.  It writes the chunk's partial Dp[a, chunk] = Σ_i w_i A[a, i]
+// (the wave sums in order) and, per i, the tile's partial integral ip[i, tile] = Σ_a A[a, i] Wa[a]
+// (a wave butterfly); dos_finish_kernel sums them in order.
 constexpr int DOS_IC = 128;
 __global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *__restrict__ wsa,
                                                   const double *__restrict__ e_i, const int32_t *__restrict__ jsp,
                                                   const double *__restrict__ tsp, const int32_t *__restrict__ tidx,
-                                                  const double2 *__restrict__ Snew, const double *__restrict__ w_i,
+                                                  const double2 *__restrict__ Sg, const double *__restrict__ w_i,
                                                   const double *__restrict__ Wa, double *__restrict__ Dp,
                                                   double *__restrict__ ipart, const int32_t *__restrict__ act)
 {
@@ -155,7 +153,7 @@ __global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_
     const int64_t a = blockIdx.x * 64 + lane, ntile = gridDim.x, nchunk = gridDim.y;
     const int64_t i0 = (int64_t)blockIdx.y * DOS_IC, i1 = i0 + DOS_IC < ni ? i0 + DOS_IC : ni;
     const bool on = a < nwa;
-    const double2 *S = Snew + nw * ni * (int64_t)tidx[z];
+    const double2 *S = Sg + nw * ni * (int64_t)tidx[z];
     int64_t j = 0;
     double t = 0, x = 0, wa = 0;
     if (on) {
@@ -187,29 +185,32 @@ __global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_
         Dp[a + nwa * (blockIdx.y + nchunk * (int64_t)z)] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
-// D[a] = Wa[a] Σ_chunk Dp[a, chunk] (in chunk order) for the active slots.
-__global__ __launch_bounds__(256) void dos_finish_kernel(int64_t nwa, int64_t nchunk, const double *__restrict__ Dp,
-                                                         const double *__restrict__ Wa, const int32_t *__restrict__ tidx,
-                                                         double *__restrict__ D, const int32_t *__restrict__ act)
+// The sums after dos_kernel for the active slots, in one launch: D[a] = Wa[a] Σ_chunk Dp[a, chunk]
+// (in chunk order; the first nblk(nwa) blocks), and ∫A dω of each outer state from the tile
+// partials (the other blocks: one wave per (i, slot), lanes over the tiles, lane-strided then a
+// butterfly: a fixed order).  it (graph mode): the device iteration counter, bumped once.
+__global__ __launch_bounds__(256) void dos_finish_kernel(int64_t nwa, int64_t nchunk, int64_t ni, int64_t ntile,
+                                                         const double *__restrict__ Dp, const double *__restrict__ Wa,
+                                                         const double *__restrict__ ipart,
+                                                         const int32_t *__restrict__ tidx, double *__restrict__ D,
+                                                         double *__restrict__ integ, const int32_t *__restrict__ act,
+                                                         int32_t *__restrict__ it)
 {
-    const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
-    if (a >= nwa || (act && !act[tidx[z]])) return;
-    double sum = 0;
-    for (int64_t c = 0; c < nchunk; c++) sum += Dp[a + nwa * (c + nchunk * (int64_t)z)];
-    D[a + nwa * (int64_t)z] = sum * Wa[a];
-}
-
-// ∫A dω of each outer state from the tile partials: one wave per (i, slot), lanes over the tiles
-// (lane-strided, then a butterfly: a fixed order).
-__global__ __launch_bounds__(256) void integ_kernel(int64_t ni, int64_t ntile, const double *__restrict__ ipart,
-                                                    const int32_t *__restrict__ tidx, double *__restrict__ integ,
-                                                    const int32_t *__restrict__ act)
-{
+    if (it && blockIdx.x == 0 && z == 0 && threadIdx.x == 0) *it += 1;
+    if (act && !act[tidx[z]]) return;
+    const int64_t nbd = (nwa + 255) / 256;
+    if (blockIdx.x < nbd) {
+        const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+        if (a >= nwa) return;
+        double sum = 0;
+        for (int64_t c = 0; c < nchunk; c++) sum += Dp[a + nwa * (c + nchunk * (int64_t)z)];
+        D[a + nwa * (int64_t)z] = sum * Wa[a];
+        return;
+    }
     const int lane = threadIdx.x & 63;
-    const int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.y;
-    if (i >= ni || (act && !act[tidx[z]])) return;
+    const int64_t i = (blockIdx.x - nbd) * 4 + (threadIdx.x >> 6);
+    if (i >= ni) return;
     double v = 0;
     for (int64_t k = lane; k < ntile; k += 64) v += ipart[i + ni * (k + ntile * (int64_t)z)];
 #pragma unroll
@@ -217,6 +218,7 @@ __global__ __launch_bounds__(256) void integ_kernel(int64_t ni, int64_t ntile, c
     if (lane == 0) integ[i + ni * (int64_t)z] = v;
 }
 
+// min_i ∫A per slot (mu_kernel takes it with μ; this kernel serves a fixed μ)
 __global__ __launch_bounds__(256) void min_kernel(int64_t ni, const double *__restrict__ integ,
                                                   const int32_t *__restrict__ tidx, double *__restrict__ minA,
                                                   const int32_t *__restrict__ act)
@@ -240,6 +242,20 @@ __global__ __launch_bounds__(256) void min_kernel(int64_t ni, const double *__re
     }
     if (threadIdx.x == 0) minA[tidx[z]] = red[0];
 }
+
+// Device-side bookkeeping of the graph-replayed loop: the history row of this iteration, the
+// freeze of a converged temperature (src/scgd0.jl:326-333).
+struct GraphBook {
+    const int32_t *prec = nullptr;   // [nT] precision level (only FP64 may freeze); null: all FP64
+    int32_t *act;            // [nT] convergence mask
+    int64_t *frozen;         // [nT] iteration at which T froze
+    const int32_t *it;       // iterations begun (dos_finish_kernel bumps it)
+    int32_t *status = nullptr;   // [nT] μ status words (after act in one allocation), or null
+    double *hist_err, *hist_mu, *hist_minA;  // [nT, maxiter]
+    const double *mu, *minA;
+    int64_t nT, maxiter;
+    double tol;
+};
 
 // Ordered-bit midpoint of a bracket (Float64 bisection as in Roots.jl), 0 across a sign change.
 __device__ inline double bits_middle(double a, double b)
@@ -268,6 +284,9 @@ struct MuArgs {
     double *mu;              // per temperature index
     int32_t *status;         // per temperature index
     const int32_t *act;      // convergence mask or null
+    int64_t ni;
+    const double *integ;     // [ni, nz] ∫A dω of each outer state
+    double *minA;            // per temperature index: min_i ∫A
 };
 
 // Sum over a wave by an xor butterfly: every lane adds the same pairs in the same order, so all
@@ -288,13 +307,30 @@ constexpr int MU_REG = 8;  // spectral points per thread held in registers (nωa
 // pattern until two adjacent floats hold the sign change.  For T > sqrt(eps) each thread keeps
 // its D_a and e^{±ωa/T} (as mantissa and binary exponent) in registers, so a step costs one
 // exponential per thread: e^{(ωa-μ)/T} = e^{ωa/T} e^{-μ/T}.
-__global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
+__device__ __forceinline__ void mu_root(const MuArgs &m, int z, int tcol, double (*part)[16])
 {
-    __shared__ double part[2][16];
-    const int z = blockIdx.x;
-    const int tcol = m.tidx[z];
-    if (m.act && !m.act[tcol]) return;
     const int tid = threadIdx.x, wave = tid >> 6;
+    {
+        // min_i ∫A (NaN wins, as in min_kernel)
+        double v = INFINITY;
+        for (int64_t i = tid; i < m.ni; i += 1024) {
+            const double o = m.integ[i + m.ni * (int64_t)z];
+            v = (o < v || o != o) ? o : v;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double u = __shfl_xor(v, o);
+            v = (u < v || u != u) ? u : v;
+        }
+        if ((tid & 63) == 0) part[0][wave] = v;
+        __syncthreads();
+        if (tid == 0) {
+            double r = part[0][0];
+            for (int k = 1; k < 16; k++) r = (part[0][k] < r || part[0][k] != part[0][k]) ? part[0][k] : r;
+            m.minA[tcol] = r;
+        }
+        __syncthreads();
+    }
     const double T = m.T[tcol];
     const double n_target = m.nlist[tcol] + m.nelec;
     const int holes = m.is_semi && m.nlist[tcol] < 0;
@@ -408,8 +444,70 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m)
     if (tid == 0) m.mu[tcol] = fabs(fa) <= fabs(fb) ? a : b;
 }
 
+// The per-block err maxima of slot z at most ERR_NB per slot (64 lanes x 8).
+constexpr int ERR_NB = 512;
+
+// This lane's max of slot z's per-block err maxima (NaN wins): all its loads issued together.
+__device__ inline double err_lane_max(int nb, const double *__restrict__ bmax, int z)
+{
+    const int lane = threadIdx.x & 63;
+    double o[ERR_NB / 64];
+#pragma unroll
+    for (int r = 0; r < ERR_NB / 64; r++) {
+        const int b = lane + 64 * r;
+        o[r] = b < nb ? bmax[b + nb * z] : 0.0;
+    }
+    double m = 0;
+#pragma unroll
+    for (int r = 0; r < ERR_NB / 64; r++) m = (o[r] > m || o[r] != o[r]) ? o[r] : m;
+    return m;
+}
+
+// err of slot z = the max over wave 0's lane maxima m (NaN wins), then in graph mode the history
+// row and the freeze of a converged temperature (src/scgd0.jl:326-333).  Wave 0 of the block.
+__device__ inline void err_book(double m, int t, double *__restrict__ err, const GraphBook &g)
+{
+    const int lane = threadIdx.x;
+    if (lane >= 64) return;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v = __shfl_xor(m, o);
+        m = (v > m || v != v) ? v : m;
+    }
+    if (lane != 0) return;
+    err[t] = m;
+    if (g.act) {
+        const int it = *g.it - 1;   // bumped by this iteration's dos_finish_kernel
+        if (it < g.maxiter) {
+            g.hist_err[t + g.nT * it] = m;
+            g.hist_mu[t + g.nT * it] = g.mu[t];
+            g.hist_minA[t + g.nT * it] = g.minA[t];
+        }
+        if (m < g.tol && (!g.prec || g.prec[t] == EPSX_PREC_FP64)) {
+            g.act[t] = 0;
+            g.frozen[t] = it + 1;
+        }
+    }
+}
+
+// μ of every active slot (one block of 16 waves each), min ∫A, and the err bookkeeping of the
+// iteration after μ (its history row records the new μ).
+__global__ __launch_bounds__(1024) void mu_kernel(MuArgs m, int nb, const double *__restrict__ bmax,
+                                                  double *__restrict__ err, GraphBook g)
+{
+    __shared__ double part[2][16];
+    const int z = blockIdx.x;
+    const int tcol = m.tidx[z];
+    if (m.act && !m.act[tcol]) return;
+    // wave 0's err maxima first: the loads are in flight during the root search
+    const double em = threadIdx.x < 64 ? err_lane_max(nb, bmax, z) : 0.0;
+    mu_root(m, z, tcol, part);
+    // thread 0 wrote μ and min ∫A; the bookkeeping reads them on the same thread
+    err_book(em, tcol, err, g);
+}
+
 // err = max |Σ_new - Σ| over the slice, then Σ ← Σ_new (src/scgd0.jl:326-328): per-block maxima,
-// reduced by err_final_kernel.
+// reduced by err_book (in mu_kernel, or err_final_kernel with a fixed μ).
 __global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t *__restrict__ tidx,
                                                        const double2 *__restrict__ Snew, double2 *__restrict__ S,
                                                        double *__restrict__ bmax, const int32_t *__restrict__ act)
@@ -424,6 +522,44 @@ __global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t 
         const double e = hypot(a.x - b.x, a.y - b.y);
         m = (e > m || e != e) ? e : m;
         S[base + q] = a;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bmax[blockIdx.x + gridDim.x * z] = red[0];
+}
+
+// Σ_new = complex(ReΣ, ImΣ) from the columns of the library products (column cls[i] of outer
+// state i: its multiplet class, or i itself with cls null), err = max |Σ_new - Σ| over the slice
+// (per-block maxima as err_copy_kernel) and Σ ← Σ_new, for the slots of a pass: the pack, the
+// multiplet copy and src/scgd0.jl:326-328 in one pass over Σ.
+__global__ __launch_bounds__(256) void pack_err_kernel(int32_t nw, int32_t ni, int64_t nc,
+                                                       const int32_t *__restrict__ cls,
+                                                       const int32_t *__restrict__ tidx, const double *__restrict__ re,
+                                                       const double *__restrict__ im, double2 *__restrict__ S,
+                                                       double *__restrict__ bmax, const int32_t *__restrict__ gate)
+{
+    __shared__ double red[256];
+    const int z = blockIdx.y;
+    const int tcol = tidx[z];
+    if (gate && !gate[tcol]) return;
+    const int32_t n = nw * ni;
+    double2 *St = S + (int64_t)n * tcol;
+    const double *rt = re + nw * nc * (int64_t)tcol, *it = im + nw * nc * (int64_t)tcol;
+    double m = 0;
+    for (int32_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (int32_t)gridDim.x * 256) {
+        const int32_t i = q / nw, w = q - i * nw;
+        const int64_t o = w + (int64_t)nw * (cls ? cls[i] : i);
+        const double2 a = make_double2(rt[o], it[o]), b = St[q];
+        const double e = hypot(a.x - b.x, a.y - b.y);
+        m = (e > m || e != e) ? e : m;
+        St[q] = a;
     }
     red[threadIdx.x] = m;
     __syncthreads();
@@ -456,18 +592,6 @@ __global__ void phase_gates_kernel(int64_t nT, const int32_t *__restrict__ act, 
     gates[t + nT * L.n] = a && !(lp_native && prec[t] != EPSX_PREC_FP64);
 }
 
-// Device-side bookkeeping of the graph-replayed loop: the history row of this iteration, the
-// freeze of a converged temperature (src/scgd0.jl:326-333).
-struct GraphBook {
-    const int32_t *prec = nullptr;   // [nT] precision level (only FP64 may freeze); null: all FP64
-    int32_t *act;            // [nT] convergence mask
-    int64_t *frozen;         // [nT] iteration at which T froze
-    const int32_t *it;       // 0-based iteration counter
-    double *hist_err, *hist_mu, *hist_minA;  // [nT, maxiter]
-    const double *mu, *minA;
-    int64_t nT, maxiter;
-    double tol;
-};
 
 __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__restrict__ bmax,
                                                       const int32_t *__restrict__ tidx, double *__restrict__ err,
@@ -476,30 +600,7 @@ __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__r
     const int z = blockIdx.x;
     const int t = tidx[z];
     if (g.act && !g.act[t]) return;
-    double m = 0;
-    for (int b = threadIdx.x; b < nb; b += 64) {
-        const double o = bmax[b + nb * z];
-        m = (o > m || o != o) ? o : m;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double v = __shfl_xor(m, o);
-        m = (v > m || v != v) ? v : m;
-    }
-    if (threadIdx.x != 0) return;
-    err[t] = m;
-    if (g.act) {
-        const int it = *g.it;
-        if (it < g.maxiter) {
-            g.hist_err[t + g.nT * it] = m;
-            g.hist_mu[t + g.nT * it] = g.mu[t];
-            g.hist_minA[t + g.nT * it] = g.minA[t];
-        }
-        if (m < g.tol && (!g.prec || g.prec[t] == EPSX_PREC_FP64)) {
-            g.act[t] = 0;
-            g.frozen[t] = it + 1;
-        }
-    }
+    err_book(err_lane_max(nb, bmax, z), t, err, g);
 }
 
 inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
@@ -721,6 +822,7 @@ double now_s()
 extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd0_outputs *out)
 {
     if (!ctx) return EPSX_ERR_ARG;
+    const double t_entry = now_s();
     Scgd0Dev &s = ctx->s;
     if (!s.ready) return set_error(ctx, EPSX_ERR_STATE, "epsx_scgd0_run before epsx_scgd0_set_problem");
     if (!par || par->struct_size != (int64_t)sizeof(epsx_scgd0_params) || !out ||
@@ -850,7 +952,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         if (use_bf) EPSX_TRY(tmp_alloc(ctx, imb.b, 2 * (size_t)kpw * ni * nT));
         if (use_f32) EPSX_TRY(tmp_alloc(ctx, imf.b, 4 * (size_t)kpw * ni * nT));
     }
-    const int nbmax = 512;
+    const int nbmax = ERR_NB;
     EPSX_TRY(tmp_alloc(ctx, B.bmax.b, 8 * (size_t)nbmax * nT));
     EPSX_TRY(tmp_upload(ctx, B.T, par->Tlist, 8 * nT));
     EPSX_TRY(tmp_upload(ctx, B.mu, par->mu_start, 8 * nT));
@@ -921,19 +1023,22 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // or low-precision instances); the host knows which slots take the FP64 pass, since a graph is
     // captured again whenever the active set changes.
     bool star_lt = false, kk_lt = false, lp_lt[3] = {false, false, false};
-    Tmp ReS;    // KK ImΣ of the library product (then packed into Σ_new)
+    Tmp ReS;    // KK ImΣ of the library product, per column of the star product (then packed into Σ)
     Tmp ImSf;   // FP32 result of the low-precision library products
     // (a run of one or two iterations does not pay for timing the library's candidates: it keeps
     // the library's own kernels, as it keeps their default tile shape)
     const char *blas_env = getenv("EPSX_STAR_BLAS"), *shape_env = getenv("EPSX_GEMM_SHAPE");
     const bool use_lt = maxiter >= 3 && !(blas_env && blas_env[0] == '0') && !(shape_env && *shape_env);
     {
-        // Kramers-Kronig's FP64 product KK·ImΣ likewise (KK shared, the complex result packed after)
+        // Kramers-Kronig's FP64 product KK·ImΣ likewise (KK shared, over the star product's columns,
+        // the complex result packed after)
         if (use_lt) {
             EPSX_TRY(tmp_alloc(ctx, ReS.b, 8 * nS));
             EPSX_HIP(ctx, hipMemsetAsync(B.ImS.b.p, 0, 8 * nS, st));
-            EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ni, nw, dptr<double>(s.KK), nw, 0, B.ImS.ptr<double>(), nw,
-                                    nw * ni, ReS.ptr<double>(), nw, nw * ni, nT, &kk_lt));
+            if (use_cls) EPSX_HIP(ctx, hipMemsetAsync(ImSc.b.p, 0, 8 * (size_t)nw * ncol * nT, st));
+            EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, nw, dptr<double>(s.KK), nw, 0,
+                                    use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), nw, nw * ncol,
+                                    ReS.ptr<double>(), nw, nw * ncol, nT, &kk_lt));
         }
     }
     if (method == EPSX_FM_STAR) {
@@ -961,6 +1066,10 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             }
         }
     }
+    // The FP64 pass packs Σ_new from the library products' columns and writes Σ ← Σ_new with err
+    // in one kernel, unless multiplets without classes still need their average taken over Σ_new.
+    // (pack_err_kernel indexes a slice in 32 bits)
+    const bool kk_fuse = kk_lt && (use_cls || s.n_mult == 0) && nw * ni <= INT32_MAX - 65536 * 256;
     // the levels present among the active temperatures, as a bit mask
     auto present_levels = [&]() {
         int mask = 0;
@@ -1071,10 +1180,16 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                                    false, bt));
                 }
             }
+            // ImΣ per outer state for the products that take it (the fused FP64 pack reads the class
+            // columns)
             if (use_cls) {
-                hipLaunchKernelGGL(cls_scatter_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw, ni, s.nc,
-                                   dptr<int32_t>(s.cls), d_tidx, ImSc.ptr<double>(), B.ImS.ptr<double>(), d_act);
-                EPSX_HIP(ctx, hipGetLastError());
+                for (int q = 0; q < (kk_fuse ? npass : 1); q++) {
+                    if (kk_fuse && passes[q].prec == EPSX_PREC_FP64) continue;
+                    hipLaunchKernelGGL(cls_scatter_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw, ni, s.nc,
+                                       dptr<int32_t>(s.cls), d_tidx, ImSc.ptr<double>(), B.ImS.ptr<double>(),
+                                       kk_fuse ? passes[q].gate : d_act);
+                    EPSX_HIP(ctx, hipGetLastError());
+                }
             }
         } else {
             for (int q = 0; q < npass; q++) {
@@ -1098,7 +1213,9 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             tp.act = d_act;
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[1], st));
-        // 4. Re Σ by Kramers-Kronig, Σ_new = complex(KK ImΣ, ImΣ) in the GEMM's epilogue
+        // 4. Re Σ by Kramers-Kronig, Σ_new = complex(KK ImΣ, ImΣ) in the GEMM's epilogue; the FP64
+        // library product runs over the columns of the star product (one per multiplet class) and
+        // its pack writes Σ ← Σ_new with err directly
         for (int q = 0; q < npass; q++) {
             GemmBatch bt;
             bt.n_inst = nA;
@@ -1110,14 +1227,20 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             bt.Z = B.Snew.ptr<double2>();
             bt.Zim = B.ImS.ptr<double>();
             if (passes[q].prec == EPSX_PREC_FP64 && kk_lt) {
-                const int64_t sC = nw * ni;
+                const int64_t sC = nw * ncol;
+                const double *Cim = use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>();
                 EPSX_TRY(fp64_runs([&](int z, int n) {
                     const int64_t t0 = h_tidx[z];
-                    return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ni, nw, dptr<double>(s.KK), nw, 0,
-                                       B.ImS.ptr<double>() + sC * t0, nw, sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
+                    return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, nw, dptr<double>(s.KK), nw, 0, Cim + sC * t0, nw,
+                                       sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
                 }));
-                hipLaunchKernelGGL(complex_pack_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw * ni, d_tidx,
-                                   passes[q].gate, ReS.ptr<double>(), B.ImS.ptr<double>(), B.Snew.ptr<double2>());
+                if (kk_fuse)
+                    hipLaunchKernelGGL(pack_err_kernel, dim3(nb_err, nA), dim3(256), 0, st, (int32_t)nw, (int32_t)ni,
+                                       ncol, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx, ReS.ptr<double>(), Cim,
+                                       B.S.ptr<double2>(), B.bmax.ptr<double>(), passes[q].gate);
+                else
+                    hipLaunchKernelGGL(complex_pack_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw * ni, d_tidx,
+                                       passes[q].gate, ReS.ptr<double>(), B.ImS.ptr<double>(), B.Snew.ptr<double2>());
                 EPSX_HIP(ctx, hipGetLastError());
             } else if (passes[q].prec == EPSX_PREC_FP64) {
                 EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
@@ -1147,43 +1270,49 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                                d_act);
             EPSX_HIP(ctx, hipGetLastError());
         }
+        // 8-9. err and Σ ← Σ_new for the slots whose pack did not (Σ_new is read from Σ from here)
+        for (int q = 0; q < (kk_fuse ? npass : 1); q++) {
+            if (kk_fuse && passes[q].prec == EPSX_PREC_FP64) continue;
+            hipLaunchKernelGGL(err_copy_kernel, dim3(nb_err, nA), dim3(256), 0, st, nw * ni, d_tidx,
+                               B.Snew.ptr<double2>(), B.S.ptr<double2>(), B.bmax.ptr<double>(),
+                               kk_fuse ? passes[q].gate : d_act);
+            EPSX_HIP(ctx, hipGetLastError());
+        }
         // 6. spectral DOS and min ∫A
         const unsigned ntile = nblk(nwa, 64), nchunk = nblk(ni, DOS_IC);
         hipLaunchKernelGGL(dos_kernel, dim3(ntile, nchunk, nA), dim3(256), 0, st, nw, ni, nwa, dptr<double>(s.wsa),
                            dptr<double>(s.e_i), dptr<int32_t>(s.jsp), dptr<double>(s.tsp), d_tidx,
-                           B.Snew.ptr<double2>(), dptr<double>(s.w_i), dptr<double>(s.Wa), B.Dp.ptr<double>(),
+                           B.S.ptr<double2>(), dptr<double>(s.w_i), dptr<double>(s.Wa), B.Dp.ptr<double>(),
                            B.ipart.ptr<double>(), d_act);
         EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(dos_finish_kernel, dim3(nblk(nwa), nA), dim3(256), 0, st, nwa, (int64_t)nchunk,
-                           B.Dp.ptr<double>(), dptr<double>(s.Wa), d_tidx, B.D.ptr<double>(), d_act);
+        hipLaunchKernelGGL(dos_finish_kernel, dim3(nblk(nwa) + nblk(ni, 4), nA), dim3(256), 0, st, nwa,
+                           (int64_t)nchunk, ni, (int64_t)ntile, B.Dp.ptr<double>(), dptr<double>(s.Wa),
+                           B.ipart.ptr<double>(), d_tidx, B.D.ptr<double>(), B.integ.ptr<double>(), d_act,
+                           d_act ? const_cast<int32_t *>(book.it) : nullptr);
         EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(integ_kernel, dim3(nblk(ni, 4), nA), dim3(256), 0, st, ni, (int64_t)ntile,
-                           B.ipart.ptr<double>(), d_tidx, B.integ.ptr<double>(), d_act);
-        EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
-                           B.minA.ptr<double>(), d_act);
-        EPSX_HIP(ctx, hipGetLastError());
+        if (par->fix_mu) {
+            hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
+                               B.minA.ptr<double>(), d_act);
+            EPSX_HIP(ctx, hipGetLastError());
+        }
         // 7. μ
         if (!par->fix_mu) {
             MuArgs m;
             m.nwa = nwa; m.wsa = dptr<double>(s.wsa); m.D = B.D.ptr<double>(); m.tidx = d_tidx;
             m.T = B.T.ptr<double>(); m.nlist = B.nlist.ptr<double>();
             m.nelec = par->nelec; m.spin = par->spin_degeneracy; m.W = s.W; m.nsb = s.nstates_base;
-            m.is_semi = par->is_semiconductor; m.mu = B.mu.ptr<double>(); m.status = B.status.ptr<int32_t>();
+            m.is_semi = par->is_semiconductor; m.mu = B.mu.ptr<double>();
+            m.status = book.status ? book.status : B.status.ptr<int32_t>();
             m.act = d_act;
-            hipLaunchKernelGGL(mu_kernel, dim3(nA), dim3(1024), 0, st, m);
+            m.ni = ni; m.integ = B.integ.ptr<double>(); m.minA = B.minA.ptr<double>();
+            hipLaunchKernelGGL(mu_kernel, dim3(nA), dim3(1024), 0, st, m, nb_err, B.bmax.ptr<double>(),
+                               B.err.ptr<double>(), book);
             EPSX_HIP(ctx, hipGetLastError());
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[3], st));
-        // 8-9. err and Σ ← Σ_new
-        hipLaunchKernelGGL(err_copy_kernel, dim3(nb_err, nA), dim3(256), 0, st, nw * ni, d_tidx,
-                           B.Snew.ptr<double2>(), B.S.ptr<double2>(), B.bmax.ptr<double>(), d_act);
-        EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(err_final_kernel, dim3(nA), dim3(64), 0, st, nb_err, B.bmax.ptr<double>(), d_tidx,
-                           B.err.ptr<double>(), book);
-        EPSX_HIP(ctx, hipGetLastError());
-        if (d_act) {
-            hipLaunchKernelGGL(iter_inc_kernel, dim3(1), dim3(64), 0, st, const_cast<int32_t *>(book.it));
+        if (par->fix_mu) {
+            hipLaunchKernelGGL(err_final_kernel, dim3(nA), dim3(64), 0, st, nb_err, B.bmax.ptr<double>(), d_tidx,
+                               B.err.ptr<double>(), book);
             EPSX_HIP(ctx, hipGetLastError());
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[4], st));
@@ -1255,6 +1384,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         if (changed) EPSX_HIP(ctx, hipMemcpyAsync(d_prec.b.p, prec_t.data(), 4 * nT, hipMemcpyHostToDevice, st));
         return EPSX_OK;
     };
+    const double t_loop = now_s();
     if (!par->batched) {
         // The reference's loop: the active temperatures one after another, decisions on the host.
         GraphBook none{};
@@ -1305,7 +1435,18 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         std::vector<int32_t> all(nT);
         for (int64_t t = 0; t < nT; t++) all[t] = (int32_t)t;
         Tmp d_act, d_frozen, d_it, d_he, d_hm, d_ha;
-        EPSX_TRY(tmp_upload(ctx, d_act, active.data(), 4 * nT));
+        // the mask and the μ status words in one allocation, read back by one copy per iteration
+        // into pinned host memory
+        std::vector<int32_t> act_status(2 * nT, 0);
+        std::copy(active.begin(), active.end(), act_status.begin());
+        EPSX_TRY(tmp_upload(ctx, d_act, act_status.data(), 8 * nT));
+        // (two slots: an iteration may be read back while the next one runs)
+        int32_t *h_as = nullptr;
+        EPSX_HIP(ctx, hipHostMalloc((void **)&h_as, 16 * nT, hipHostMallocDefault));
+        struct PinGuard { int32_t *p; ~PinGuard() { (void)hipHostFree(p); } } pg{h_as};
+        hipEvent_t ev_read[2] = {nullptr, nullptr};
+        struct ReadGuard { hipEvent_t *e; ~ReadGuard() { for (int k = 0; k < 2; k++) if (e[k]) (void)hipEventDestroy(e[k]); } } rg{ev_read};
+        for (auto &e : ev_read) EPSX_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         std::vector<int64_t> zeros64(nT, 0);
         EPSX_TRY(tmp_upload(ctx, d_frozen, zeros64.data(), 8 * nT));
         const int32_t zero = 0;
@@ -1315,7 +1456,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         EPSX_TRY(tmp_upload(ctx, d_ha, out->hist_minA, 8 * nT * maxiter));
         EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, all.data(), 4 * nT, hipMemcpyHostToDevice, st));
         GraphBook book;
-        book.act = d_act.ptr<int32_t>(); book.frozen = d_frozen.ptr<int64_t>(); book.it = d_it.ptr<int32_t>();
+        book.act = d_act.ptr<int32_t>(); book.status = d_act.ptr<int32_t>() + nT; book.frozen = d_frozen.ptr<int64_t>(); book.it = d_it.ptr<int32_t>();
         book.hist_err = d_he.ptr<double>(); book.hist_mu = d_hm.ptr<double>(); book.hist_minA = d_ha.ptr<double>();
         book.mu = B.mu.ptr<double>(); book.minA = B.minA.ptr<double>();
         book.nT = nT; book.maxiter = maxiter; book.tol = par->tol;
@@ -1328,11 +1469,26 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         // first), or the first of a one-iteration run
         auto timed = [&](int64_t iter) { return iter == 2 || (iter == 1 && maxiter == 1); };
         struct GraphGuard {
-            hipGraph_t *g; hipGraphExec_t *e;
-            ~GraphGuard() { if (*e) (void)hipGraphExecDestroy(*e); if (*g) (void)hipGraphDestroy(*g); }
-        } gg{&graph, &exec};
-        for (int64_t iter = 1; iter <= maxiter; iter++) {
-            const double t0 = now_s();
+            hipGraph_t *g; hipGraphExec_t *e; hipStream_t st;
+            ~GraphGuard()
+            {
+                // an iteration launched ahead may still run the graph
+                (void)hipStreamSynchronize(st);
+                if (*e) (void)hipGraphExecDestroy(*e);
+                if (*g) (void)hipGraphDestroy(*g);
+            }
+        } gg{&graph, &exec, st};
+        // Replayed iterations of an FP64 run go to the GPU one ahead of the host: iteration k + 1 is
+        // launched before the host reads back iteration k's mask, so the GPU does not wait for the
+        // host between replays.  An iteration launched ahead of the host's view runs the device-masked
+        // kernels on the current mask; the library products (no device mask) keep the active set of
+        // the graph, and their output for a temperature frozen meanwhile is never read.  After the
+        // last iteration the one launched ahead finds every temperature masked.  (Mixed-precision runs
+        // set each temperature's level on the host from the last err, so they wait for it.)
+        const bool ahead = !split;
+        int64_t launched = 0;
+        double t_prev = now_s();
+        auto launch = [&](int64_t iter) -> int {
             if (iter <= 2) {
                 EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, timed(iter)));
             } else {
@@ -1342,6 +1498,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 if (exec && (present_levels() != graph_levels ||
                              ((star_lt || kk_lt || lp_lt[1] || lp_lt[2]) &&
                               (active != graph_active || prec_t != graph_prec)))) {
+                    EPSX_HIP(ctx, hipStreamSynchronize(st));
                     EPSX_HIP(ctx, hipGraphExecDestroy(exec));
                     EPSX_HIP(ctx, hipGraphDestroy(graph));
                     exec = nullptr;
@@ -1360,15 +1517,27 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 }
                 EPSX_HIP(ctx, hipGraphLaunch(exec, st));
             }
-            EPSX_HIP(ctx, hipMemcpyAsync(active.data(), book.act, 4 * nT, hipMemcpyDeviceToHost, st));
-            EPSX_HIP(ctx, hipMemcpyAsync(h_status.data(), B.status.b.p, 4 * nT, hipMemcpyDeviceToHost, st));
+            int32_t *slot = h_as + 2 * nT * (iter & 1);
+            EPSX_HIP(ctx, hipMemcpyAsync(slot, book.act, 8 * nT, hipMemcpyDeviceToHost, st));
             if (split) EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
-            EPSX_HIP(ctx, hipStreamSynchronize(st));
+            EPSX_HIP(ctx, hipEventRecord(ev_read[iter & 1], st));
+            launched = iter;
+            return EPSX_OK;
+        };
+        for (int64_t iter = 1; iter <= maxiter; iter++) {
+            if (launched < iter) EPSX_TRY(launch(iter));
+            if (ahead && iter >= 3 && iter < maxiter) EPSX_TRY(launch(iter + 1));
+            EPSX_HIP(ctx, hipEventSynchronize(ev_read[iter & 1]));
+            const int32_t *slot = h_as + 2 * nT * (iter & 1);
+            std::copy(slot, slot + nT, active.begin());
+            std::copy(slot + nT, slot + 2 * nT, h_status.begin());
             if (timed(iter)) EPSX_TRY(add_event_times((int)nT));
             for (int64_t t = 0; t < nT; t++)
                 if (!par->fix_mu && h_status[t] == 2) { rc = no_root(t); break; }
             if (rc != EPSX_OK) break;
-            out->wall[iter - 1] = now_s() - t0;
+            const double t1 = now_s();
+            out->wall[iter - 1] = t1 - t_prev;
+            t_prev = t1;
             out->niter = iter;
             if (split) {
                 // frozen temperatures are not in a phase any more
@@ -1394,8 +1563,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         }
     }
     if (rc != EPSX_OK) return rc;
-    EPSX_HIP(ctx, hipMemcpyAsync(out->Sigma, B.S.b.p, 16 * nS, hipMemcpyDeviceToHost, st));
-    EPSX_HIP(ctx, hipStreamSynchronize(st));
+    const double t_read = now_s();
+    EPSX_TRY(read_back(ctx, out->Sigma, B.S.b.p, 16 * nS));
+    if (par->verbosity > 0)
+        fprintf(stderr, "scGD0 run: set-up %.4f s, iterations %.4f s, Σ read back %.4f s\n", t_loop - t_entry,
+                t_read - t_loop, now_s() - t_read);
     for (int64_t t = 0; t < nT; t++) {
         out->mu[t] = mu_host[t];
         out->converged[t] = !active[t];
