@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void dos_finish_kernel(int64_t nwa, int64_t nc
     if (lane == 0) integ[i + ni * (int64_t)z] = v;
 }
 
-// min_i ∫A per slot (mu_kernel takes it with μ; this kernel serves a fixed μ)
+// min_i ∫A per slot (NaN wins)
 __global__ __launch_bounds__(256) void min_kernel(int64_t ni, const double *__restrict__ integ,
                                                   const int32_t *__restrict__ tidx, double *__restrict__ minA,
                                                   const int32_t *__restrict__ act)
@@ -284,9 +284,6 @@ struct MuArgs {
     double *mu;              // per temperature index
     int32_t *status;         // per temperature index
     const int32_t *act;      // convergence mask or null
-    int64_t ni;
-    const double *integ;     // [ni, nz] ∫A dω of each outer state
-    double *minA;            // per temperature index: min_i ∫A
 };
 
 // Sum over a wave by an xor butterfly: every lane adds the same pairs in the same order, so all
@@ -310,27 +307,6 @@ constexpr int MU_REG = 8;  // spectral points per thread held in registers (nωa
 __device__ __forceinline__ void mu_root(const MuArgs &m, int z, int tcol, double (*part)[16])
 {
     const int tid = threadIdx.x, wave = tid >> 6;
-    {
-        // min_i ∫A (NaN wins, as in min_kernel)
-        double v = INFINITY;
-        for (int64_t i = tid; i < m.ni; i += 1024) {
-            const double o = m.integ[i + m.ni * (int64_t)z];
-            v = (o < v || o != o) ? o : v;
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const double u = __shfl_xor(v, o);
-            v = (u < v || u != u) ? u : v;
-        }
-        if ((tid & 63) == 0) part[0][wave] = v;
-        __syncthreads();
-        if (tid == 0) {
-            double r = part[0][0];
-            for (int k = 1; k < 16; k++) r = (part[0][k] < r || part[0][k] != part[0][k]) ? part[0][k] : r;
-            m.minA[tcol] = r;
-        }
-        __syncthreads();
-    }
     const double T = m.T[tcol];
     const double n_target = m.nlist[tcol] + m.nelec;
     const int holes = m.is_semi && m.nlist[tcol] < 0;
@@ -463,18 +439,21 @@ __device__ inline double err_lane_max(int nb, const double *__restrict__ bmax, i
     return m;
 }
 
-// err of slot z = the max over wave 0's lane maxima m (NaN wins), then in graph mode the history
-// row and the freeze of a converged temperature (src/scgd0.jl:326-333).  Wave 0 of the block.
-__device__ inline void err_book(double m, int t, double *__restrict__ err, const GraphBook &g)
+// The max over a wave (NaN wins), in every lane.
+__device__ inline double err_wave_max(double m)
 {
-    const int lane = threadIdx.x;
-    if (lane >= 64) return;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const double v = __shfl_xor(m, o);
         m = (v > m || v != v) ? v : m;
     }
-    if (lane != 0) return;
+    return m;
+}
+
+// err of temperature t = m, then in graph mode the history row and the freeze of a converged
+// temperature (src/scgd0.jl:326-333).  One thread.
+__device__ inline void err_book(double m, int t, double *__restrict__ err, const GraphBook &g)
+{
     err[t] = m;
     if (g.act) {
         const int it = *g.it - 1;   // bumped by this iteration's dos_finish_kernel
@@ -490,20 +469,25 @@ __device__ inline void err_book(double m, int t, double *__restrict__ err, const
     }
 }
 
-// μ of every active slot (one block of 16 waves each), min ∫A, and the err bookkeeping of the
-// iteration after μ (its history row records the new μ).
+// μ of every active slot (one block of 16 waves each) and the err bookkeeping of the iteration
+// after μ (its history row records the new μ).
 __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m, int nb, const double *__restrict__ bmax,
                                                   double *__restrict__ err, GraphBook g)
 {
     __shared__ double part[2][16];
+    __shared__ double emax;
     const int z = blockIdx.x;
     const int tcol = m.tidx[z];
     if (m.act && !m.act[tcol]) return;
-    // wave 0's err maxima first: the loads are in flight during the root search
-    const double em = threadIdx.x < 64 ? err_lane_max(nb, bmax, z) : 0.0;
+    // wave 0 reduces the err maxima first (its loads in flight with those of D); the value
+    // waits in LDS, not in registers, during the root search
+    if (threadIdx.x < 64) {
+        const double e = err_wave_max(err_lane_max(nb, bmax, z));
+        if (threadIdx.x == 0) emax = e;
+    }
     mu_root(m, z, tcol, part);
-    // thread 0 wrote μ and min ∫A; the bookkeeping reads them on the same thread
-    err_book(em, tcol, err, g);
+    // thread 0 wrote μ (and emax); the bookkeeping reads them on the same thread
+    if (threadIdx.x == 0) err_book(emax, tcol, err, g);
 }
 
 // err = max |Σ_new - Σ| over the slice, then Σ ← Σ_new (src/scgd0.jl:326-328): per-block maxima,
@@ -600,7 +584,8 @@ __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__r
     const int z = blockIdx.x;
     const int t = tidx[z];
     if (g.act && !g.act[t]) return;
-    err_book(err_lane_max(nb, bmax, z), t, err, g);
+    const double e = err_wave_max(err_lane_max(nb, bmax, z));
+    if (threadIdx.x == 0) err_book(e, t, err, g);
 }
 
 inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
@@ -1290,11 +1275,9 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                            B.ipart.ptr<double>(), d_tidx, B.D.ptr<double>(), B.integ.ptr<double>(), d_act,
                            d_act ? const_cast<int32_t *>(book.it) : nullptr);
         EPSX_HIP(ctx, hipGetLastError());
-        if (par->fix_mu) {
-            hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
-                               B.minA.ptr<double>(), d_act);
-            EPSX_HIP(ctx, hipGetLastError());
-        }
+        hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
+                           B.minA.ptr<double>(), d_act);
+        EPSX_HIP(ctx, hipGetLastError());
         // 7. μ
         if (!par->fix_mu) {
             MuArgs m;
@@ -1304,7 +1287,6 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             m.is_semi = par->is_semiconductor; m.mu = B.mu.ptr<double>();
             m.status = book.status ? book.status : B.status.ptr<int32_t>();
             m.act = d_act;
-            m.ni = ni; m.integ = B.integ.ptr<double>(); m.minA = B.minA.ptr<double>();
             hipLaunchKernelGGL(mu_kernel, dim3(nA), dim3(1024), 0, st, m, nb_err, B.bmax.ptr<double>(),
                                B.err.ptr<double>(), book);
             EPSX_HIP(ctx, hipGetLastError());
