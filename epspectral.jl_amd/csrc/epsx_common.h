@@ -236,7 +236,8 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
 // elements, sA or sB may be 0).  BF16 / FP32: C (FP32) = Aᵀ B from k-contiguous operands (A k x m,
 // B k x n), FP32 accumulation.  blaslt_prepare, outside graph capture, times the heuristic's
 // workspace-free candidates on the given buffers (C overwritten) for batch counts 1..max_batch and
-// keeps the fastest per problem in the context; blaslt_gemm then only launches.
+// keeps the fastest per problem and batch count in the context; blaslt_gemm then only launches,
+// splitting its batch into the chunks those timings price cheapest.
 int blaslt_prepare(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, int64_t sA,
                    const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC, int64_t max_batch, bool *ok);
 int blaslt_gemm(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const void *A, int64_t lda, int64_t sA,

@@ -17,10 +17,12 @@
 //     b to XCD b mod 8), so the tiles one XCD runs are neighbours that share B panels in that
 //     XCD's L2, and masked instances leave no XCD idle.
 #include <array>
+#include <cmath>
 #include <cstdlib>
 #include <map>
 #include <memory>
 #include <type_traits>
+#include <vector>
 
 #include <hipblaslt/hipblaslt.h>
 
@@ -695,6 +697,7 @@ struct LtDescs {
 struct LtProduct {
     LtDescs d;
     hipblasLtMatmulAlgo_t algo;
+    float ms = 0;   // its timed launch
 };
 
 // (precision, m, n, k, lda, sA, ldb, sB, ldc, sC, batch) -> product; absent: no workspace-free
@@ -814,10 +817,22 @@ int blaslt_prepare(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, con
             EPSX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
             if (rs == 0 && (pick < 0 || ms < best)) { pick = r; best = ms; }
         }
+        // the pick's time, which prices the batch splits, from three more launches
+        if (pick >= 0) {
+            EPSX_HIP(ctx, hipEventRecord(e0, ctx->stream));
+            int rs = 0;
+            for (int rep = 0; rep < 3; rep++) rs |= lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[pick].algo);
+            EPSX_HIP(ctx, hipEventRecord(e1, ctx->stream));
+            EPSX_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0;
+            EPSX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            if (rs == 0) best = ms / 3;
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (pick < 0) { all = false; continue; }
         pr->algo = res[pick].algo;
+        pr->ms = best;
         b->prod[key] = std::move(pr);
     }
     *ok = all;
@@ -828,11 +843,39 @@ int blaslt_gemm(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const 
                 const void *B, int64_t ldb, int64_t sB, void *C, int64_t ldc, int64_t sC, int64_t batch)
 {
     auto *b = static_cast<BlasLt *>(ctx->blaslt);
-    if (!b) return set_error(ctx, EPSX_ERR_STATE, "hipBLASLt product not prepared");
-    const auto it = b->prod.find(LtKey{prec, m, n, k, lda, sA, ldb, sB, ldc, sC, batch});
-    if (it == b->prod.end()) return set_error(ctx, EPSX_ERR_STATE, "hipBLASLt product not prepared");
-    if (lt_run(b, *it->second, prec, A, B, C, ctx->stream, &it->second->algo))
-        return set_error(ctx, EPSX_ERR_DEVICE, "hipblasLtMatmul failed");
+    if (!b || batch < 1) return set_error(ctx, EPSX_ERR_STATE, "hipBLASLt product not prepared");
+    // The batch in the chunks the timed launches price cheapest (cost[j]: the first j instances;
+    // a split must save 3% over one call, so timing noise does not split): a wide batch can run
+    // below the efficiency of two narrower ones on the same CUs.
+    auto product = [&](int64_t bb) -> LtProduct * {
+        const auto it = b->prod.find(LtKey{prec, m, n, k, lda, sA, ldb, sB, ldc, sC, bb});
+        return it == b->prod.end() ? nullptr : it->second.get();
+    };
+    std::vector<double> cost(batch + 1, 0.0);
+    std::vector<int64_t> chunk(batch + 1, 0);
+    for (int64_t j = 1; j <= batch; j++) {
+        cost[j] = INFINITY;
+        for (int64_t bb = j; bb >= 1; bb--) {
+            const LtProduct *pr = product(bb);
+            if (!pr) continue;
+            const double c = cost[j - bb] + pr->ms;
+            if (chunk[j] == 0 || c < 0.97 * cost[j]) { cost[j] = c; chunk[j] = bb; }
+        }
+        if (chunk[j] == 0) return set_error(ctx, EPSX_ERR_STATE, "hipBLASLt product not prepared");
+    }
+    std::vector<int64_t> sizes;
+    for (int64_t j = batch; j > 0; j -= chunk[j]) sizes.push_back(chunk[j]);
+    const int64_t ea = prec == EPSX_PREC_FP64 ? 8 : (prec == EPSX_PREC_BF16 ? 2 : 4);
+    const int64_t ec = prec == EPSX_PREC_FP64 ? 8 : 4;
+    int64_t off = 0;
+    for (auto it = sizes.rbegin(); it != sizes.rend(); ++it) {
+        LtProduct *pr = product(*it);
+        if (lt_run(b, *pr, prec, static_cast<const char *>(A) + ea * sA * off,
+                   static_cast<const char *>(B) + ea * sB * off, static_cast<char *>(C) + ec * sC * off, ctx->stream,
+                   &pr->algo))
+            return set_error(ctx, EPSX_ERR_DEVICE, "hipblasLtMatmul failed");
+        off += *it;
+    }
     return EPSX_OK;
 }
 

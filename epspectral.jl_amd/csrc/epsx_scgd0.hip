@@ -86,6 +86,22 @@ __global__ void cls_scatter_kernel(int64_t nw, int64_t ni, int64_t nc, const int
     ImS[q + nw * ni * (int64_t)tcol] = ImSc[w + nw * (int64_t)cls[i] + nw * nc * (int64_t)tcol];
 }
 
+// Pseudo-random values in [-1, 1) (a hash of the index) for timing the library products'
+// candidates: zero operands draw less power than a solve's, and so run at higher clocks.
+template <typename E>
+__global__ void fill_rand_kernel(E *p, int64_t n)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    uint64_t x = (uint64_t)q * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+    x ^= x >> 31;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 29;
+    const double v = (double)(x >> 11) * 0x1.0p-52 - 1.0;
+    if constexpr (sizeof(E) == 2) p[q] = (E)(__float_as_uint((float)v) >> 16);
+    else p[q] = (E)v;
+}
+
 __global__ void fill_sigma_kernel(int64_t n, double2 *S, double eta_init)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1019,8 +1035,12 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         // the complex result packed after)
         if (use_lt) {
             EPSX_TRY(tmp_alloc(ctx, ReS.b, 8 * nS));
-            EPSX_HIP(ctx, hipMemsetAsync(B.ImS.b.p, 0, 8 * nS, st));
-            if (use_cls) EPSX_HIP(ctx, hipMemsetAsync(ImSc.b.p, 0, 8 * (size_t)nw * ncol * nT, st));
+            // candidates timed on pseudo-random operands (ImΣ is overwritten before its first use)
+            hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(nS)), dim3(256), 0, st, B.ImS.ptr<double>(), nS);
+            if (use_cls)
+                hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(nw * ncol * nT)), dim3(256), 0, st,
+                                   ImSc.ptr<double>(), nw * ncol * nT);
+            EPSX_HIP(ctx, hipGetLastError());
             EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, nw, dptr<double>(s.KK), nw, 0,
                                     use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), nw, nw * ncol,
                                     ReS.ptr<double>(), nw, nw * ncol, nT, &kk_lt));
@@ -1028,9 +1048,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     }
     if (method == EPSX_FM_STAR) {
         if (use_lt) {
-            // candidates timed on zeroed h (ImΣ overwritten before its first use)
+            // candidates timed on pseudo-random h (overwritten before its first use)
             const int64_t K = s.nm * ni;
-            EPSX_HIP(ctx, hipMemsetAsync(hbuf.b.p, 0, 8 * (size_t)nw * K * nT, st));
+            hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(nw * K * nT)), dim3(256), 0, st, hbuf.ptr<double>(),
+                               nw * K * nT);
+            EPSX_HIP(ctx, hipGetLastError());
             EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>(), nw, nw * K,
                                     use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt), K, 0,
                                     use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), nw, nw * ncol, nT, &star_lt));
@@ -1039,12 +1061,16 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             if (lp_native) {
                 EPSX_TRY(tmp_alloc(ctx, ImSf.b, 4 * nS));
                 if (use_bf) {
-                    EPSX_HIP(ctx, hipMemsetAsync(hb.b.p, 0, 2 * (size_t)kp * nw * nT, st));
+                    hipLaunchKernelGGL(fill_rand_kernel<uint16_t>, dim3(nblk(kp * nw * nT)), dim3(256), 0, st,
+                                       hb.ptr<uint16_t>(), kp * nw * nT);
+                    EPSX_HIP(ctx, hipGetLastError());
                     EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, nw, ncol, kp, hb.b.p, kp, kp * nw, s.Gb.p, kp, 0,
                                             ImSf.ptr<float>(), nw, nw * ncol, nT, &lp_lt[EPSX_PREC_BF16]));
                 }
                 if (use_f32) {
-                    EPSX_HIP(ctx, hipMemsetAsync(hf.b.p, 0, 4 * (size_t)kp * nw * nT, st));
+                    hipLaunchKernelGGL(fill_rand_kernel<float>, dim3(nblk(kp * nw * nT)), dim3(256), 0, st,
+                                       hf.ptr<float>(), kp * nw * nT);
+                    EPSX_HIP(ctx, hipGetLastError());
                     EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP32, nw, ncol, kp, hf.b.p, kp, kp * nw, s.Gf.p, kp, 0,
                                             ImSf.ptr<float>(), nw, nw * ncol, nT, &lp_lt[EPSX_PREC_FP32]));
                 }
