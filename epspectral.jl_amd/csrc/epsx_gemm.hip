@@ -851,11 +851,14 @@ int blaslt_gemm(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const 
         const auto it = b->prod.find(LtKey{prec, m, n, k, lda, sA, ldb, sB, ldc, sC, bb});
         return it == b->prod.end() ? nullptr : it->second.get();
     };
+    // (EPSX_LT_CHUNK=c caps the chunks at c instances, for tests of the split)
+    const char *cap_env = getenv("EPSX_LT_CHUNK");
+    const int64_t cap = cap_env && atoll(cap_env) > 0 ? atoll(cap_env) : batch;
     std::vector<double> cost(batch + 1, 0.0);
     std::vector<int64_t> chunk(batch + 1, 0);
     for (int64_t j = 1; j <= batch; j++) {
         cost[j] = INFINITY;
-        for (int64_t bb = j; bb >= 1; bb--) {
+        for (int64_t bb = j < cap ? j : cap; bb >= 1; bb--) {
             const LtProduct *pr = product(bb);
             if (!pr) continue;
             const double c = cost[j - bb] + pr->ms;

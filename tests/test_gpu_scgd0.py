@@ -294,6 +294,32 @@ def test_star_product_blaslt_matches_own_kernel(ctx):
     assert np.all(np.abs(lt["μ"] - own["μ"]) <= 1e-12)
 
 
+def test_star_product_blaslt_batch_split(ctx):
+    """A library product over five temperatures split into chunks of at most two instances
+    (EPSX_LT_CHUNK=2: calls of 2 + 2 + 1, each chunk's pointers advanced by its offset) against the
+    library's MFMA kernel: same iterations, Σ and μ to 1e-12."""
+    T = (0.08, 0.1, 0.13, 0.16, 0.2)
+    s = holstein(T=T)
+    o = occ(list(T))
+    kw = dict(η_min=0.0, η_init=0.05, maxiter=80, tol=1e-6, batched=True, method="star")
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        os.environ["EPSX_LT_CHUNK"] = "2"
+        try:
+            lt = epsx.loop_scgd0(s, o, **kw)
+        finally:
+            del os.environ["EPSX_LT_CHUNK"]
+        os.environ["EPSX_STAR_BLAS"] = "0"
+        try:
+            own = epsx.loop_scgd0(s, o, **kw)
+        finally:
+            del os.environ["EPSX_STAR_BLAS"]
+    assert lt["converged"].all() and own["converged"].all()
+    assert np.array_equal(lt["history"]["iter_frozen"], own["history"]["iter_frozen"])
+    assert rel(lt["Σ"], own["Σ"]) <= 1e-12
+    assert np.all(np.abs(lt["μ"] - own["μ"]) <= 1e-12)
+
+
 def test_star_method_matches_direct_and_oracle(ctx):
     """The factorised Fan-Migdal method (Einstein modes, ε_f == ε_{i_f}) against the direct kernel
     and the oracle on the t2g model at nk = 8 (multiplets, 9 modes, graded grid)."""
