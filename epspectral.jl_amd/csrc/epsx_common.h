@@ -18,6 +18,7 @@ namespace epsx {
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    epsx_ctx *pool = nullptr;   // a run's temporary: freeing returns it to this context's pool
 };
 
 struct Timer {
@@ -111,6 +112,11 @@ struct epsx_ctx {
     void *blaslt = nullptr; // hipBLASLt handle and its chosen algorithms (epsx_gemm.hip)
     void *pin = nullptr;    // pinned host staging of read_back (kept for the next call)
     size_t pin_bytes = 0;
+    // device buffers a run released, by size, for the next run's temporaries (work on the
+    // context's one stream is ordered, so a buffer may be handed out while a released user's
+    // kernels are still queued ahead)
+    std::multimap<size_t, void *> pool;
+    size_t pool_bytes = 0;
     std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, split) -> tile shape
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
@@ -125,7 +131,8 @@ void blas_destroy(epsx_ctx *ctx);   // epsx_ladder.hip
 void blaslt_destroy(epsx_ctx *ctx); // epsx_gemm.hip
 int hip_fail(epsx_ctx *ctx, hipError_t e, const char *what, const char *file, int line);
 int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes);   // context-owned, freed at close
-void dev_free(DevBuf &b);
+void dev_free(DevBuf &b);   // hipFree, or back to its context's pool (a run's temporary)
+void pool_clear(epsx_ctx *ctx);
 int tmp_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes);   // caller-owned (see Tmp)
 
 // A scoped device temporary.

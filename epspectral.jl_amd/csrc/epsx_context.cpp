@@ -27,11 +27,27 @@ int hip_fail(epsx_ctx *ctx, hipError_t e, const char *what, const char *file, in
                      what, file, line);
 }
 
+// A context keeps at most this much released temporary memory for reuse.
+constexpr size_t POOL_MAX = size_t(32) << 30;
+
 void dev_free(DevBuf &b)
 {
-    if (b.p) (void)hipFree(b.p);
+    if (b.p && b.pool && b.pool->pool_bytes + b.bytes <= POOL_MAX) {
+        b.pool->pool.emplace(b.bytes, b.p);
+        b.pool->pool_bytes += b.bytes;
+    } else if (b.p) {
+        (void)hipFree(b.p);
+    }
     b.p = nullptr;
     b.bytes = 0;
+    b.pool = nullptr;
+}
+
+void pool_clear(epsx_ctx *ctx)
+{
+    for (auto &kv : ctx->pool) (void)hipFree(kv.second);
+    ctx->pool.clear();
+    ctx->pool_bytes = 0;
 }
 
 int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes)
@@ -56,13 +72,30 @@ int tmp_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes)
 {
     dev_free(b);
     if (bytes == 0) bytes = 8;
+    const auto hit = ctx->pool.find(bytes);
+    if (hit != ctx->pool.end()) {
+        b.p = hit->second;
+        ctx->pool.erase(hit);
+        ctx->pool_bytes -= bytes;
+        b.bytes = bytes;
+        b.pool = ctx;
+        return EPSX_OK;
+    }
     hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess && !ctx->pool.empty()) {
+        // the pool's buffers of other sizes may be what is missing
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(ctx->stream);
+        pool_clear(ctx);
+        e = hipMalloc(&b.p, bytes);
+    }
     if (e != hipSuccess) {
         b.p = nullptr;
         return set_error(ctx, EPSX_ERR_OOM, "device allocation of %zu bytes failed: %s", bytes,
                          hipGetErrorString(e));
     }
     b.bytes = bytes;
+    b.pool = ctx;
     return EPSX_OK;
 }
 
@@ -149,6 +182,7 @@ void epsx_close(epsx_ctx *ctx)
     epsx::blaslt_destroy(ctx);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     for (auto *b : ctx->owned) epsx::dev_free(*b);
+    epsx::pool_clear(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
