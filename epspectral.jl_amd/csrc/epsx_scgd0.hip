@@ -267,6 +267,7 @@ struct GraphBook {
     int64_t *frozen;         // [nT] iteration at which T froze
     const int32_t *it;       // iterations begun (dos_finish_kernel bumps it)
     int32_t *status = nullptr;   // [nT] μ status words (after act in one allocation), or null
+    int32_t *host = nullptr;     // pinned host slots [2][act nT | status nT] (device address), or null
     double *hist_err, *hist_mu, *hist_minA;  // [nT, maxiter]
     const double *mu, *minA;
     int64_t nT, maxiter;
@@ -466,8 +467,18 @@ __device__ inline double err_wave_max(double m)
     return m;
 }
 
+// Temperature t's mask word and μ status into the host slot of this iteration (slot = 1-based
+// iteration mod 2: the host reads one slot while the next iteration writes the other).  One thread.
+__device__ inline void host_report(const GraphBook &g, int t, int32_t act)
+{
+    if (!g.host) return;
+    int32_t *slot = g.host + 2 * g.nT * (*g.it & 1);
+    slot[t] = act;
+    slot[g.nT + t] = g.status ? g.status[t] : 0;
+}
+
 // err of temperature t = m, then in graph mode the history row and the freeze of a converged
-// temperature (src/scgd0.jl:326-333).  One thread.
+// temperature (src/scgd0.jl:326-333), reported to the host.  One thread.
 __device__ inline void err_book(double m, int t, double *__restrict__ err, const GraphBook &g)
 {
     err[t] = m;
@@ -482,6 +493,7 @@ __device__ inline void err_book(double m, int t, double *__restrict__ err, const
             g.act[t] = 0;
             g.frozen[t] = it + 1;
         }
+        host_report(g, t, g.act[t]);
     }
 }
 
@@ -494,7 +506,10 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m, int nb, const double
     __shared__ double emax;
     const int z = blockIdx.x;
     const int tcol = m.tidx[z];
-    if (m.act && !m.act[tcol]) return;
+    if (m.act && !m.act[tcol]) {
+        if (threadIdx.x == 0) host_report(g, tcol, 0);
+        return;
+    }
     // wave 0 reduces the err maxima first (its loads in flight with those of D); the value
     // waits in LDS, not in registers, during the root search
     if (threadIdx.x < 64) {
@@ -599,7 +614,10 @@ __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__r
 {
     const int z = blockIdx.x;
     const int t = tidx[z];
-    if (g.act && !g.act[t]) return;
+    if (g.act && !g.act[t]) {
+        if (threadIdx.x == 0) host_report(g, t, 0);
+        return;
+    }
     const double e = err_wave_max(err_lane_max(nb, bmax, z));
     if (threadIdx.x == 0) err_book(e, t, err, g);
 }
@@ -1443,15 +1461,18 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         std::vector<int32_t> all(nT);
         for (int64_t t = 0; t < nT; t++) all[t] = (int32_t)t;
         Tmp d_act, d_frozen, d_it, d_he, d_hm, d_ha;
-        // the mask and the μ status words in one allocation, read back by one copy per iteration
-        // into pinned host memory
+        // the mask and the μ status words in one allocation; the convergence kernel reports both
+        // into pinned host slots every iteration
         std::vector<int32_t> act_status(2 * nT, 0);
         std::copy(active.begin(), active.end(), act_status.begin());
         EPSX_TRY(tmp_upload(ctx, d_act, act_status.data(), 8 * nT));
         // (two slots: an iteration may be read back while the next one runs)
-        int32_t *h_as = nullptr;
-        EPSX_HIP(ctx, hipHostMalloc((void **)&h_as, 16 * nT, hipHostMallocDefault));
+        // (fine-grained: the convergence kernel writes them, the host reads them after the
+        // iteration's event)
+        int32_t *h_as = nullptr, *d_as = nullptr;
+        EPSX_HIP(ctx, hipHostMalloc((void **)&h_as, 16 * nT, hipHostMallocMapped | hipHostMallocCoherent));
         struct PinGuard { int32_t *p; ~PinGuard() { (void)hipHostFree(p); } } pg{h_as};
+        EPSX_HIP(ctx, hipHostGetDevicePointer((void **)&d_as, h_as, 0));
         hipEvent_t ev_read[2] = {nullptr, nullptr};
         struct ReadGuard { hipEvent_t *e; ~ReadGuard() { for (int k = 0; k < 2; k++) if (e[k]) (void)hipEventDestroy(e[k]); } } rg{ev_read};
         for (auto &e : ev_read) EPSX_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1464,7 +1485,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         EPSX_TRY(tmp_upload(ctx, d_ha, out->hist_minA, 8 * nT * maxiter));
         EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, all.data(), 4 * nT, hipMemcpyHostToDevice, st));
         GraphBook book;
-        book.act = d_act.ptr<int32_t>(); book.status = d_act.ptr<int32_t>() + nT; book.frozen = d_frozen.ptr<int64_t>(); book.it = d_it.ptr<int32_t>();
+        book.act = d_act.ptr<int32_t>(); book.status = d_act.ptr<int32_t>() + nT; book.host = d_as; book.frozen = d_frozen.ptr<int64_t>(); book.it = d_it.ptr<int32_t>();
         book.hist_err = d_he.ptr<double>(); book.hist_mu = d_hm.ptr<double>(); book.hist_minA = d_ha.ptr<double>();
         book.mu = B.mu.ptr<double>(); book.minA = B.minA.ptr<double>();
         book.nT = nT; book.maxiter = maxiter; book.tol = par->tol;
@@ -1525,8 +1546,6 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                 }
                 EPSX_HIP(ctx, hipGraphLaunch(exec, st));
             }
-            int32_t *slot = h_as + 2 * nT * (iter & 1);
-            EPSX_HIP(ctx, hipMemcpyAsync(slot, book.act, 8 * nT, hipMemcpyDeviceToHost, st));
             if (split) EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipEventRecord(ev_read[iter & 1], st));
             launched = iter;
