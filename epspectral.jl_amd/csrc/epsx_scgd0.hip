@@ -268,6 +268,7 @@ struct GraphBook {
     const int32_t *it;       // iterations begun (dos_finish_kernel bumps it)
     int32_t *status = nullptr;   // [nT] μ status words (after act in one allocation), or null
     int32_t *host = nullptr;     // pinned host slots [2][act nT | status nT] (device address), or null
+    double *host_err = nullptr;  // pinned host slots [2][nT] of err (device address), or null
     double *hist_err, *hist_mu, *hist_minA;  // [nT, maxiter]
     const double *mu, *minA;
     int64_t nT, maxiter;
@@ -482,6 +483,7 @@ __device__ inline void host_report(const GraphBook &g, int t, int32_t act)
 __device__ inline void err_book(double m, int t, double *__restrict__ err, const GraphBook &g)
 {
     err[t] = m;
+    if (g.host_err) g.host_err[g.nT * (*g.it & 1) + t] = m;
     if (g.act) {
         const int it = *g.it - 1;   // bumped by this iteration's dos_finish_kernel
         if (it < g.maxiter) {
@@ -1471,6 +1473,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         // iteration's event)
         int32_t *h_as = nullptr, *d_as = nullptr;
         EPSX_HIP(ctx, hipHostMalloc((void **)&h_as, 16 * nT, hipHostMallocMapped | hipHostMallocCoherent));
+        double *h_es = nullptr, *d_es = nullptr;   // and err, for the decision to run ahead
+        EPSX_HIP(ctx, hipHostMalloc((void **)&h_es, 16 * nT, hipHostMallocMapped | hipHostMallocCoherent));
+        struct PinGuardE { double *p; ~PinGuardE() { (void)hipHostFree(p); } } pge{h_es};
+        EPSX_HIP(ctx, hipHostGetDevicePointer((void **)&d_es, h_es, 0));
+        std::fill(h_es, h_es + 2 * nT, INFINITY);
         struct PinGuard { int32_t *p; ~PinGuard() { (void)hipHostFree(p); } } pg{h_as};
         EPSX_HIP(ctx, hipHostGetDevicePointer((void **)&d_as, h_as, 0));
         hipEvent_t ev_read[2] = {nullptr, nullptr};
@@ -1485,7 +1492,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         EPSX_TRY(tmp_upload(ctx, d_ha, out->hist_minA, 8 * nT * maxiter));
         EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, all.data(), 4 * nT, hipMemcpyHostToDevice, st));
         GraphBook book;
-        book.act = d_act.ptr<int32_t>(); book.status = d_act.ptr<int32_t>() + nT; book.host = d_as; book.frozen = d_frozen.ptr<int64_t>(); book.it = d_it.ptr<int32_t>();
+        book.act = d_act.ptr<int32_t>(); book.status = d_act.ptr<int32_t>() + nT; book.host = d_as; book.host_err = d_es; book.frozen = d_frozen.ptr<int64_t>(); book.it = d_it.ptr<int32_t>();
         book.hist_err = d_he.ptr<double>(); book.hist_mu = d_hm.ptr<double>(); book.hist_minA = d_ha.ptr<double>();
         book.mu = B.mu.ptr<double>(); book.minA = B.minA.ptr<double>();
         book.nT = nT; book.maxiter = maxiter; book.tol = par->tol;
@@ -1511,10 +1518,19 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         // launched before the host reads back iteration k's mask, so the GPU does not wait for the
         // host between replays.  An iteration launched ahead of the host's view runs the device-masked
         // kernels on the current mask; the library products (no device mask) keep the active set of
-        // the graph, and their output for a temperature frozen meanwhile is never read.  After the
-        // last iteration the one launched ahead finds every temperature masked.  (Mixed-precision runs
-        // set each temperature's level on the host from the last err, so they wait for it.)
+        // the graph, and their output for a temperature frozen meanwhile is never read.  Once some
+        // active temperature's err is within 2 tol (it may freeze in the iteration on the GPU), the
+        // host waits instead, so that no replay runs products for a temperature just frozen or after
+        // the last iteration.  (Mixed-precision runs set each temperature's level on the host from
+        // the last err, so they always wait for it.)
         const bool ahead = !split;
+        // an active temperature within 2 tol after the last iteration the host has read
+        auto near_tol = [&](int64_t iter_read) {
+            const double *e = h_es + nT * (iter_read & 1);
+            for (int64_t t = 0; t < nT; t++)
+                if (active[t] && !(e[t] >= 2 * par->tol)) return true;
+            return false;
+        };
         int64_t launched = 0;
         double t_prev = now_s();
         auto launch = [&](int64_t iter) -> int {
@@ -1553,7 +1569,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         };
         for (int64_t iter = 1; iter <= maxiter; iter++) {
             if (launched < iter) EPSX_TRY(launch(iter));
-            if (ahead && iter >= 3 && iter < maxiter) EPSX_TRY(launch(iter + 1));
+            if (ahead && iter >= 3 && iter < maxiter && !near_tol(iter - 1)) EPSX_TRY(launch(iter + 1));
             EPSX_HIP(ctx, hipEventSynchronize(ev_read[iter & 1]));
             const int32_t *slot = h_as + 2 * nT * (iter & 1);
             std::copy(slot, slot + nT, active.begin());
