@@ -112,6 +112,9 @@ struct epsx_ctx {
     void *blaslt = nullptr; // hipBLASLt handle and its chosen algorithms (epsx_gemm.hip)
     void *pin = nullptr;    // pinned host staging of read_back (kept for the next call)
     size_t pin_bytes = 0;
+    hipStream_t side = nullptr;   // a second stream (default priority) for copies beside the work
+    void *pin_side = nullptr;     // pinned host memory the side stream copies into
+    size_t pin_side_bytes = 0;
     // device buffers a run released, by size, for the next run's temporaries (work on the
     // context's one stream is ordered, so a buffer may be handed out while a released user's
     // kernels are still queued ahead)
@@ -169,6 +172,8 @@ int upload(epsx_ctx *ctx, DevBuf &b, const void *host, size_t bytes);
 // (a pageable destination is copied at host memory speed, its first-touch page faults spread over
 // the threads).
 int read_back(epsx_ctx *ctx, void *dst, const void *src, size_t bytes);
+// memcpy by up to 8 host threads (slices of at least 1 MiB).
+void parallel_copy(void *dst, const void *src, size_t n);
 
 // Host-side grid maps (epsx_grids.cpp), shared by the C ABI and the loop set-up.
 int64_t cell_search_table(int64_t n, const double *ws, double dw_min, double *x0,

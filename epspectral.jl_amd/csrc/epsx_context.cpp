@@ -106,6 +106,24 @@ int tmp_upload(epsx_ctx *ctx, Tmp &t, const void *host, size_t bytes)
     return EPSX_OK;
 }
 
+void parallel_copy(void *dst, const void *src, size_t n)
+{
+    unsigned hw = std::thread::hardware_concurrency();
+    hw = hw == 0 ? 1 : (hw > 8 ? 8 : hw);
+    const size_t want = n >> 20;
+    const unsigned nt = want < 1 ? 1u : (want < hw ? (unsigned)want : hw);
+    const size_t per = (n + nt - 1) / nt;
+    char *d = static_cast<char *>(dst);
+    const char *h = static_cast<const char *>(src);
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < nt; k++) {
+        const size_t a = per * k, b = a + per < n ? a + per : n;
+        if (a < b) th.emplace_back([=] { std::memcpy(d + a, h + a, b - a); });
+    }
+    std::memcpy(d, h, per < n ? per : n);
+    for (auto &t : th) t.join();
+}
+
 int read_back(epsx_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
     constexpr size_t STAGE = size_t(64) << 20;
@@ -118,26 +136,12 @@ int read_back(epsx_ctx *ctx, void *dst, const void *src, size_t bytes)
         EPSX_HIP(ctx, hipHostMalloc(&ctx->pin, stage, hipHostMallocDefault));
         ctx->pin_bytes = stage;
     }
-    unsigned hw = std::thread::hardware_concurrency();
-    hw = hw == 0 ? 1 : (hw > 8 ? 8 : hw);
     for (size_t off = 0; off < bytes; off += stage) {
         const size_t n = bytes - off < stage ? bytes - off : stage;
         EPSX_HIP(ctx, hipMemcpyAsync(ctx->pin, static_cast<const char *>(src) + off, n, hipMemcpyDeviceToHost,
                                      ctx->stream));
         EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        // slices of at least 1 MiB, one per thread, the first on this thread
-        const size_t want = n >> 20;
-        const unsigned nt = want < 1 ? 1u : (want < hw ? (unsigned)want : hw);
-        const size_t per = (n + nt - 1) / nt;
-        char *d = static_cast<char *>(dst) + off;
-        const char *h = static_cast<const char *>(ctx->pin);
-        std::vector<std::thread> th;
-        for (unsigned k = 1; k < nt; k++) {
-            const size_t a = per * k, b = a + per < n ? a + per : n;
-            if (a < b) th.emplace_back([=] { std::memcpy(d + a, h + a, b - a); });
-        }
-        std::memcpy(d, h, per < n ? per : n);
-        for (auto &t : th) t.join();
+        parallel_copy(static_cast<char *>(dst) + off, ctx->pin, n);
     }
     return EPSX_OK;
 }
@@ -181,6 +185,11 @@ void epsx_close(epsx_ctx *ctx)
     epsx::blas_destroy(ctx);
     epsx::blaslt_destroy(ctx);
     if (ctx->pin) (void)hipHostFree(ctx->pin);
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
+    }
+    if (ctx->pin_side) (void)hipHostFree(ctx->pin_side);
     for (auto *b : ctx->owned) epsx::dev_free(*b);
     epsx::pool_clear(ctx);
     (void)hipStreamDestroy(ctx->stream);

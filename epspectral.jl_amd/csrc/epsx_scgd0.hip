@@ -1413,6 +1413,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         return EPSX_OK;
     };
     const double t_loop = now_s();
+    // Σ slices of temperatures frozen in the batched loop, copied to pinned host memory on the side
+    // stream while the others iterate (their Σ no longer changes)
+    const size_t slice = 16 * (size_t)nw * ni;
+    std::vector<char> sent(nT, 0);
+    bool any_sent = false;
     if (!par->batched) {
         // The reference's loop: the active temperatures one after another, decisions on the host.
         GraphBook none{};
@@ -1567,6 +1572,16 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             launched = iter;
             return EPSX_OK;
         };
+        bool side_ok = 16 * nS <= (size_t(4) << 30);
+        if (side_ok && !ctx->side) EPSX_HIP(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        if (side_ok && ctx->pin_side_bytes < 16 * nS) {
+            EPSX_HIP(ctx, hipStreamSynchronize(ctx->side));
+            if (ctx->pin_side) (void)hipHostFree(ctx->pin_side);
+            ctx->pin_side = nullptr;
+            ctx->pin_side_bytes = 0;
+            EPSX_HIP(ctx, hipHostMalloc(&ctx->pin_side, 16 * nS, hipHostMallocDefault));
+            ctx->pin_side_bytes = 16 * nS;
+        }
         for (int64_t iter = 1; iter <= maxiter; iter++) {
             if (launched < iter) EPSX_TRY(launch(iter));
             if (ahead && iter >= 3 && iter < maxiter && !near_tol(iter - 1)) EPSX_TRY(launch(iter + 1));
@@ -1578,6 +1593,14 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             for (int64_t t = 0; t < nT; t++)
                 if (!par->fix_mu && h_status[t] == 2) { rc = no_root(t); break; }
             if (rc != EPSX_OK) break;
+            for (int64_t t = 0; side_ok && t < nT; t++) {
+                if (active[t] || sent[t]) continue;
+                EPSX_HIP(ctx, hipStreamWaitEvent(ctx->side, ev_read[iter & 1], 0));
+                EPSX_HIP(ctx, hipMemcpyAsync(static_cast<char *>(ctx->pin_side) + slice * t, B.S.ptr<char>() + slice * t,
+                                             slice, hipMemcpyDeviceToHost, ctx->side));
+                sent[t] = 1;
+                any_sent = true;
+            }
             const double t1 = now_s();
             out->wall[iter - 1] = t1 - t_prev;
             t_prev = t1;
@@ -1607,7 +1630,16 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     }
     if (rc != EPSX_OK) return rc;
     const double t_read = now_s();
-    EPSX_TRY(read_back(ctx, out->Sigma, B.S.b.p, 16 * nS));
+    if (any_sent) {
+        EPSX_HIP(ctx, hipStreamSynchronize(ctx->side));
+        for (int64_t t = 0; t < nT; t++) {
+            char *dst = reinterpret_cast<char *>(out->Sigma) + slice * t;
+            if (sent[t]) parallel_copy(dst, static_cast<char *>(ctx->pin_side) + slice * t, slice);
+            else EPSX_TRY(read_back(ctx, dst, B.S.ptr<char>() + slice * t, slice));
+        }
+    } else {
+        EPSX_TRY(read_back(ctx, out->Sigma, B.S.b.p, 16 * nS));
+    }
     if (par->verbosity > 0)
         fprintf(stderr, "scGD0 run: set-up %.4f s, iterations %.4f s, Σ read back %.4f s\n", t_loop - t_entry,
                 t_read - t_loop, now_s() - t_read);
