@@ -172,6 +172,9 @@ int upload(epsx_ctx *ctx, DevBuf &b, const void *host, size_t bytes);
 // (a pageable destination is copied at host memory speed, its first-touch page faults spread over
 // the threads).
 int read_back(epsx_ctx *ctx, void *dst, const void *src, size_t bytes);
+// The reverse of read_back: any host array into device memory through the pinned staging buffer
+// (host threads copy into it), ordered after the work already on the context's stream.
+int write_from_host(epsx_ctx *ctx, void *dst, const void *src, size_t bytes);
 // memcpy by up to 8 host threads (slices of at least 1 MiB).
 void parallel_copy(void *dst, const void *src, size_t n);
 

@@ -146,6 +146,29 @@ int read_back(epsx_ctx *ctx, void *dst, const void *src, size_t bytes)
     return EPSX_OK;
 }
 
+int write_from_host(epsx_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t STAGE = size_t(64) << 20;
+    const size_t stage = bytes < STAGE ? bytes : STAGE;
+    if (bytes == 0) return EPSX_OK;
+    // the staging buffer may still feed an earlier copy on the stream
+    EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->pin_bytes < stage) {
+        if (ctx->pin) (void)hipHostFree(ctx->pin);
+        ctx->pin = nullptr;
+        ctx->pin_bytes = 0;
+        EPSX_HIP(ctx, hipHostMalloc(&ctx->pin, stage, hipHostMallocDefault));
+        ctx->pin_bytes = stage;
+    }
+    for (size_t off = 0; off < bytes; off += stage) {
+        const size_t n = bytes - off < stage ? bytes - off : stage;
+        parallel_copy(ctx->pin, static_cast<const char *>(src) + off, n);
+        EPSX_HIP(ctx, hipMemcpyAsync(static_cast<char *>(dst) + off, ctx->pin, n, hipMemcpyHostToDevice, ctx->stream));
+        EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return EPSX_OK;
+}
+
 int upload(epsx_ctx *ctx, DevBuf &b, const void *host, size_t bytes)
 {
     EPSX_TRY(dev_alloc(ctx, b, bytes));

@@ -986,7 +986,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     EPSX_HIP(ctx, hipMemsetAsync(B.status.b.p, 0, 4 * nT, st));
 
     if (par->Sigma_init) {
-        EPSX_HIP(ctx, hipMemcpyAsync(B.S.b.p, par->Sigma_init, 16 * nS, hipMemcpyHostToDevice, st));
+        EPSX_TRY(write_from_host(ctx, B.S.b.p, par->Sigma_init, 16 * nS));
     } else {
         hipLaunchKernelGGL(fill_sigma_kernel, dim3(nblk((int64_t)nS)), dim3(256), 0, st, (int64_t)nS,
                            B.S.ptr<double2>(), par->eta_init);
