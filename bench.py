@@ -214,6 +214,9 @@ def main():
         line["speedup_vs_plain"] = line["plain_fp64_gpu"]["value"] / s_per_T
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(setup, meta, iters[0], args.cpu_seconds)
+        print(f"[bench] cpu baseline: {line['cpu_baseline']}", file=sys.stderr, flush=True)
+        line["c1_holstein"] = c1_pair(ctx)
+        print(f"[bench] C1: {line['c1_holstein']}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 
@@ -313,7 +316,7 @@ def ladder_cpu_baseline(nw, ni, ns, apps_per_solve, budget_s):
     from epsx.ladder import setup_ladder
     from epsx.models import t2g_ladder_setup
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     rng = np.random.default_rng(5)
     for nk in (4, 6, 8):
         setup, R, v, meta = t2g_ladder_setup(nk=nk, Tlist_K=(100.0,))
@@ -449,42 +452,88 @@ def plain_sample(dev, occ, kw, iterations, n_iter, terms_per_T, g2_bytes):
                                  "valu_frac": tflops / FP64_PEAK_TFLOPS, "g2_GBs": g2_bytes / fm_avg / 1e9}}
 
 
+def cpu_threads():
+    """Host threads for the oracle: the box's CPU share (16 for one GPU), or fewer if fewer exist."""
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
 def cpu_baseline(setup, meta, iterations, budget_s):
-    """The oracle's Fan-Migdal sum (the reference algorithm restated in C, OpenMP over chunks of
-    outer states) on a bounded sample: the first outer states of one temperature's first
-    iteration, grown until the sample takes about budget_s; scaled to s/temperature to tol with
-    the GPU run's iteration count."""
+    """The oracle's Fan-Migdal sum (the reference algorithm restated in C; OpenMP over 2 x threads
+    chunks of outer states, as src/scgd0_kernel.jl:62-66) on a bounded sample: 2 x threads outer
+    states, so that every thread has a chunk, against the first nf_s inner states, nf_s grown until
+    the sample takes about budget_s / 4 or more; scaled by terms to s/temperature to tol with the GPU
+    run's iteration count (the cost of a term does not depend on which states it pairs)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     nm, ni, nf = setup.dims
     ws = np.asarray(setup.ωs)
     table = oracle.cell_search_table(ws, setup.ωs.dω_min)
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     T, mu = float(meta["Tlist"][0]), float(setup.μ_start[0])
     Sin = np.full((len(ws), ni), -5e-3j, order="F")
-    n_i = 1
-    elapsed = 0.0
+    n_i = min(2 * threads, ni)
+    nf_s = min(256, nf)
     while True:
-        g2s = np.asfortranarray(setup.g2[:, :n_i, :])
-        wq = np.asfortranarray(np.broadcast_to(setup.ωph[:, 0][:, None, None], (nm, nf, n_i)))
+        g2s = np.asfortranarray(setup.g2[:, :n_i, :nf_s])
+        wq = np.asfortranarray(np.broadcast_to(setup.ωph[:, 0][:, None, None], (nm, nf_s, n_i)))
         ImS = np.zeros((len(ws), n_i), order="F")
         t0 = time.perf_counter()
-        oracle.add_fan_migdal(ImS, g2s, wq, setup.ε_f, setup.f_to_i, setup.w_f, ws, table, Sin, mu, T, 1e-6,
-                              nthreads=threads)
+        oracle.add_fan_migdal(ImS, g2s, wq, setup.ε_f[:nf_s], setup.f_to_i[:nf_s], setup.w_f[:nf_s], ws, table, Sin,
+                              mu, T, 1e-6, nthreads=threads)
         elapsed = time.perf_counter() - t0
-        if elapsed > budget_s / 4 or n_i * 4 > ni:
+        if elapsed > budget_s / 4 or nf_s >= nf:
             break
-        n_i *= 4
-    terms = float(len(ws)) * nm * n_i * nf
+        nf_s = min(nf, 4 * nf_s)
+    terms = float(len(ws)) * nm * n_i * nf_s
     per_term = elapsed / terms
     s_per_T = per_term * len(ws) * nm * ni * nf * iterations
-    return {"value": s_per_T, "unit": "s/temperature", "cores": threads, "kind": "port",
-            "sample": f"oracle Fan-Migdal sum (C, OpenMP) over {n_i} of {ni} outer states x all {nf} inner states "
-                      f"x {nm} modes x {len(ws)} frequencies at T = {meta['Tlist_K'][0]:g} K: {elapsed:.1f} s; "
-                      f"scaled by terms to one temperature x {iterations} iterations (the Fan-Migdal sum is "
-                      f">99% of the CPU iteration)",
+    return {"value": s_per_T, "unit": "s/temperature", "cores": min(threads, n_i), "kind": "port",
+            "sample": f"oracle Fan-Migdal sum (C, OpenMP, {threads} threads over {2 * threads} chunks of outer states) "
+                      f"over {n_i} of {ni} outer states x {nf_s} of {nf} inner states x {nm} modes x {len(ws)} "
+                      f"frequencies at T = {meta['Tlist_K'][0]:g} K: {elapsed:.1f} s; scaled by terms to one "
+                      f"temperature x {iterations} iterations (the Fan-Migdal sum is >99% of the CPU iteration)",
             "terms_per_s": terms / elapsed}
+
+
+def c1_pair(ctx):
+    """The 1D Holstein chain of the reference's CPU-runnable case (nk = 128, ωs = -4:0.005:4, one
+    mode ω0 = 0.2 with g0 = 0.4, η_init = 0.01, T = 0.1 and 0.3 at half filling, tol 1e-8) run to tol
+    in full on the GPU (FP64, direct and star methods) and by the oracle on the host cores."""
+    import warnings
+    import epsx
+    from epsx.grids import FrequencyGrid
+    from epsx.models import bare_mu, holstein_1d_setup, holstein_energies
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    nk, T = 128, [0.1, 0.3]
+    ωs = FrequencyGrid(-4.0, 0.005, np.arange(1601))
+    εs = holstein_energies(nk, 1.0)
+    μ0 = [bare_mu(εs, np.full(nk, 1.0 / nk), Ti, 1.0) for Ti in T]
+    s = holstein_1d_setup(nk, 1.0, 0.2, 0.4 ** 2 / (2 * 0.2), ωs, μ0)
+    occ = epsx.Occupation(Tlist=T, nlist=0.0, nelec=1.0)
+    kw = dict(η_min=0.0, η_init=0.01, maxiter=100, tol=1e-8, ω_acoustic=1e-6)
+    dev = epsx.Scgd0Device(s, ctx)
+    out = {"workload": "Holstein 1D nk=128, 1601 nodes, T = 0.1, 0.3 t, tol 1e-8 t"}
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for method, batched in (("direct", False), ("auto", True)):
+            dev.loop(occ, **dict(kw, method=method, batched=batched, maxiter=2, tol=0.0))
+            t0 = time.perf_counter()
+            r = dev.loop(occ, **dict(kw, method=method, batched=batched))
+            el = time.perf_counter() - t0
+            out[f"gpu_{method}"] = {"s_per_T": el / 2, "iterations": r["history"]["iter_frozen"].tolist(),
+                                    "converged": bool(r["converged"].all())}
+    threads = cpu_threads()
+    t0 = time.perf_counter()
+    ref = oracle.loop_scgd0(s, T, 0.0, 1.0, nthreads=threads, **kw)
+    el = time.perf_counter() - t0
+    out["cpu_oracle"] = {"s_per_T": el / 2, "cores": threads, "kind": "port",
+                         "iterations": ref["history"]["iter_frozen"].tolist()}
+    out["max_rel_dSigma_gpu_vs_oracle"] = float(np.max(np.abs(r["Σ"] - ref["Σ"])) / np.max(np.abs(ref["Σ"])))
+    out["speedup_gpu_auto_vs_cpu"] = out["cpu_oracle"]["s_per_T"] / out["gpu_auto"]["s_per_T"]
+    return out
 
 
 if __name__ == "__main__":

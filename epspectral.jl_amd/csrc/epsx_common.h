@@ -18,7 +18,8 @@ namespace epsx {
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
-    epsx_ctx *pool = nullptr;   // a run's temporary: freeing returns it to this context's pool
+    epsx_ctx *owner = nullptr;  // the context whose device memory count holds it
+    bool tmp = false;           // a run's temporary: freeing returns it to the owner's pool
 };
 
 struct Timer {
@@ -120,6 +121,11 @@ struct epsx_ctx {
     // kernels are still queued ahead)
     std::multimap<size_t, void *> pool;
     size_t pool_bytes = 0;
+    // device memory the context holds (its buffers, temporaries and pool), and an optional cap on
+    // it (EPSX_ALLOC_CAP_MB, read at epsx_open; 0: none) under which an allocation fails as if
+    // the device were full, so that the out-of-memory path can be exercised on any device
+    size_t dev_bytes = 0;
+    size_t alloc_cap = 0;
     std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, split) -> tile shape
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
@@ -136,6 +142,10 @@ int hip_fail(epsx_ctx *ctx, hipError_t e, const char *what, const char *file, in
 int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes);   // context-owned, freed at close
 void dev_free(DevBuf &b);   // hipFree, or back to its context's pool (a run's temporary)
 void pool_clear(epsx_ctx *ctx);
+// hipMalloc for the context: when it fails (or the cap is reached) and the pool holds released
+// buffers, the work on both streams is waited for, the pool is emptied and the allocation tried
+// once more.
+int ctx_malloc(epsx_ctx *ctx, void **p, size_t bytes);
 int tmp_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes);   // caller-owned (see Tmp)
 
 // A scoped device temporary.

@@ -1,5 +1,6 @@
 // Context, errors and device buffers of libepsx.
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -30,24 +31,60 @@ int hip_fail(epsx_ctx *ctx, hipError_t e, const char *what, const char *file, in
 // A context keeps at most this much released temporary memory for reuse.
 constexpr size_t POOL_MAX = size_t(32) << 30;
 
+static void raw_free(epsx_ctx *ctx, void *p, size_t bytes)
+{
+    (void)hipFree(p);
+    if (ctx) ctx->dev_bytes -= bytes;
+}
+
 void dev_free(DevBuf &b)
 {
-    if (b.p && b.pool && b.pool->pool_bytes + b.bytes <= POOL_MAX) {
-        b.pool->pool.emplace(b.bytes, b.p);
-        b.pool->pool_bytes += b.bytes;
+    if (b.p && b.tmp && b.owner && b.owner->pool_bytes + b.bytes <= POOL_MAX) {
+        b.owner->pool.emplace(b.bytes, b.p);
+        b.owner->pool_bytes += b.bytes;
     } else if (b.p) {
-        (void)hipFree(b.p);
+        raw_free(b.owner, b.p, b.bytes);
     }
     b.p = nullptr;
     b.bytes = 0;
-    b.pool = nullptr;
+    b.owner = nullptr;
+    b.tmp = false;
 }
 
 void pool_clear(epsx_ctx *ctx)
 {
-    for (auto &kv : ctx->pool) (void)hipFree(kv.second);
+    for (auto &kv : ctx->pool) raw_free(ctx, kv.second, kv.first);
     ctx->pool.clear();
     ctx->pool_bytes = 0;
+}
+
+static hipError_t try_malloc(epsx_ctx *ctx, void **p, size_t bytes)
+{
+    if (ctx->alloc_cap && ctx->dev_bytes + bytes > ctx->alloc_cap) return hipErrorOutOfMemory;
+    return hipMalloc(p, bytes);
+}
+
+int ctx_malloc(epsx_ctx *ctx, void **p, size_t bytes)
+{
+    *p = nullptr;
+    hipError_t e = try_malloc(ctx, p, bytes);
+    if (e != hipSuccess && !ctx->pool.empty()) {
+        // the pool's released buffers may be what is missing: a released buffer may still be read
+        // by work queued on either stream, so wait for both before freeing
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+        pool_clear(ctx);
+        e = try_malloc(ctx, p, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return set_error(ctx, EPSX_ERR_OOM, "device allocation of %zu bytes failed: %s", bytes,
+                         hipGetErrorString(e));
+    }
+    ctx->dev_bytes += bytes;
+    return EPSX_OK;
 }
 
 int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes)
@@ -55,13 +92,10 @@ int dev_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes)
     if (b.p && b.bytes >= bytes) return EPSX_OK;
     dev_free(b);
     if (bytes == 0) bytes = 8;
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        return set_error(ctx, EPSX_ERR_OOM, "device allocation of %zu bytes failed: %s", bytes,
-                         hipGetErrorString(e));
-    }
+    EPSX_TRY(ctx_malloc(ctx, &b.p, bytes));
     b.bytes = bytes;
+    b.owner = ctx;
+    b.tmp = false;
     bool seen = false;
     for (auto *q : ctx->owned) seen |= (q == &b);
     if (!seen) ctx->owned.push_back(&b);
@@ -77,25 +111,12 @@ int tmp_alloc(epsx_ctx *ctx, DevBuf &b, size_t bytes)
         b.p = hit->second;
         ctx->pool.erase(hit);
         ctx->pool_bytes -= bytes;
-        b.bytes = bytes;
-        b.pool = ctx;
-        return EPSX_OK;
-    }
-    hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess && !ctx->pool.empty()) {
-        // the pool's buffers of other sizes may be what is missing
-        (void)hipGetLastError();
-        (void)hipStreamSynchronize(ctx->stream);
-        pool_clear(ctx);
-        e = hipMalloc(&b.p, bytes);
-    }
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        return set_error(ctx, EPSX_ERR_OOM, "device allocation of %zu bytes failed: %s", bytes,
-                         hipGetErrorString(e));
+    } else {
+        EPSX_TRY(ctx_malloc(ctx, &b.p, bytes));
     }
     b.bytes = bytes;
-    b.pool = ctx;
+    b.owner = ctx;
+    b.tmp = true;
     return EPSX_OK;
 }
 
@@ -192,6 +213,7 @@ int epsx_open(int device, epsx_ctx **out)
     if (hipSetDevice(device) != hipSuccess) return EPSX_ERR_DEVICE;
     auto *ctx = new epsx_ctx();
     ctx->device = device;
+    if (const char *cap = getenv("EPSX_ALLOC_CAP_MB")) ctx->alloc_cap = size_t(atoll(cap)) << 20;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return EPSX_ERR_DEVICE;
@@ -220,6 +242,14 @@ void epsx_close(epsx_ctx *ctx)
 }
 
 const char *epsx_last_error(const epsx_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int epsx_device_memory(const epsx_ctx *ctx, int64_t *held, int64_t *pooled)
+{
+    if (!ctx) return EPSX_ERR_ARG;
+    if (held) *held = (int64_t)ctx->dev_bytes;
+    if (pooled) *pooled = (int64_t)ctx->pool_bytes;
+    return EPSX_OK;
+}
 
 int epsx_last_timings(const epsx_ctx *ctx, double *seconds8, int64_t *launches8)
 {

@@ -1418,6 +1418,12 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     const size_t slice = 16 * (size_t)nw * ni;
     std::vector<char> sent(nT, 0);
     bool any_sent = false;
+    // Every exit waits for the side stream's copies out of Σ once any was issued: Σ is a temporary,
+    // and the pool may hand it to the next call's work on the main stream.
+    struct SideGuard {
+        epsx_ctx *c; const bool *sent;
+        ~SideGuard() { if (*sent) (void)hipStreamSynchronize(c->side); }
+    } side_guard{ctx, &any_sent};
     if (!par->batched) {
         // The reference's loop: the active temperatures one after another, decisions on the host.
         GraphBook none{};

@@ -38,6 +38,7 @@ EXPORTS = [
     ("epsx_close", None, [P]),
     ("epsx_last_error", C.c_char_p, [P]),
     ("epsx_last_timings", C.c_int, [P, P, P]),
+    ("epsx_device_memory", C.c_int, [P, P, P]),
     ("epsx_cell_search_table", C.c_int, [P, I64, P, D, P, I64, P, P, P]),
     ("epsx_linear_interpolation_table", C.c_int, [P, I64, P, I64, P, P, P]),
     ("epsx_trapezoid_weights", C.c_int, [P, I64, P, P]),
@@ -209,6 +210,12 @@ class Context:
         if rc in (EPSX_ERR_ARG, EPSX_ERR_NO_MU_ROOT):
             raise ArgumentError(msg)
         raise DeviceError(f"libepsx error {rc}: {msg}")
+
+    def device_memory(self):
+        """(bytes held, bytes of released temporaries kept for reuse)."""
+        held, pooled = C.c_int64(), C.c_int64()
+        self.check(lib().epsx_device_memory(self.handle, C.byref(held), C.byref(pooled)))
+        return held.value, pooled.value
 
     def ladder_timings(self):
         """Kernel seconds and launch counts of the last ladder run (classes of epsx_last_timings)."""

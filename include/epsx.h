@@ -54,6 +54,11 @@ const char *epsx_last_error(const epsx_ctx *ctx);
  * After epsx_ladder_run / epsx_apply_ladder_kernel: [0] the shift-table GEMMs W = J·B̃,
  * [1] whole kernel applications (count = applications), [2] B̃ builds, [3..7] unused. */
 int epsx_last_timings(const epsx_ctx *ctx, double *seconds8, int64_t *launches8);
+/* Device memory the context holds (its buffers, the temporaries of the call in progress and the
+ * released temporaries it keeps for reuse) and the part of it that is released and kept, in bytes.
+ * No counterpart in the reference (its arrays are garbage-collected); a caller sizing problems
+ * against HBM reads it. */
+int epsx_device_memory(const epsx_ctx *ctx, int64_t *held, int64_t *pooled);
 
 /* ---- grid maps (host, bit-exact with the reference) ---------------------------------- */
 /* cell_search_table, src/frequency_grids.jl:184-196.  ws: the nodes, dw_min = ωs.dω_min.

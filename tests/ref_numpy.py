@@ -10,6 +10,9 @@ They pin the C oracle (oracle/epsx_oracle.c) without sharing its implementation:
 * `simpson_pv_kk`: the composite-Simpson subtracted PV integral of
   test/test_frequency_grids.jl:84-107.
 * `n_old`: the per-state carrier count of test/test_scgd0_mu.jl:7-25.
+* `scgd0_v2_reference`: the v2 scFM iteration as plain loops over a full per-pair ωq[ν, i, f]
+  (test/test_scgd0_pb.jl:12-67): f -> i the identity, per-temperature η clamp, μ by a root
+  finder on the bracket of the spectral grid ± 0.1.
 """
 import math
 
@@ -230,3 +233,51 @@ def spectral_bubble(ws, Σ, e_i, w_i, v, mu, T, Ω):
         win = np.sum(Ww * A(ws - Ω / 2) * A(ws + Ω / 2) * fac)
         σ += math.pi * w_i[i] * win * np.outer(v[:, i], v[:, i])
     return σ
+
+
+def scgd0_v2_reference(es, weights, nstates_base, g2, ωq, ws, wsa, KK, Tlist, nlist, nelec, spin_degeneracy,
+                       μlist, ηlist, η_init, niter, ω_acoustic):
+    """test/test_scgd0_pb.jl:12-67.  The outer and inner states are the same n states (f -> i the
+    identity); ωq[ν, i, f] is the per-pair phonon frequency.  Returns (Σ [nω, n, nT], μ [nT])."""
+    ws, wsa = np.asarray(ws, dtype=float), np.asarray(wsa, dtype=float)
+    nω, n, nT, nm = len(ws), len(es), len(Tlist), g2.shape[0]
+    Σs = np.full((nω, n, nT), complex(0.0, -η_init))
+    μlist = [float(m) for m in μlist]
+    Ww = trapezoid(wsa)
+    for _ in range(niter):
+        Σs_new = np.zeros((nω, n, nT), dtype=complex)
+        for iT in range(nT):
+            T, μ = Tlist[iT], μlist[iT]
+            for f in range(n):
+                Σ_f = Σs[:, f, iT]
+                Σ_clamped = Σ_f.real + 1j * np.minimum(Σ_f.imag, -ηlist[iT])
+                ekq, wkq = es[f], weights[f]
+                for i in range(n):
+                    for nu in range(nm):
+                        q = ωq[nu, i, f]
+                        if q < ω_acoustic:
+                            continue
+                        nq = 1 / math.expm1(q / T)
+                        A1 = np.imag(1 / (ws + q - ekq - interp_flat(ws + q, ws, Σ_clamped))) * (nq + fermi(ws + q - μ, T))
+                        A2 = np.imag(1 / (ws - q - ekq - interp_flat(ws - q, ws, Σ_clamped))) * (nq + 1 - fermi(ws - q - μ, T))
+                        Σs_new[:, i, iT] += (A1 + A2) * g2[nu, i, f] * wkq * 1j
+        for iT in range(nT):
+            for i in range(n):
+                Σs_new[:, i, iT] += KK @ Σs_new[:, i, iT].imag
+        Σs = Σs_new
+        for iT in range(nT):
+            T = Tlist[iT]
+            Sp = np.stack([interp_flat(wsa, ws, Σs[:, i, iT]) for i in range(n)], axis=1)
+
+            def nocc(μ_):
+                fer = fermi(wsa - μ_, T)
+                tot = 0.0
+                for i in range(n):
+                    As = -np.imag(1 / (wsa - es[i] - Sp[:, i])) / math.pi
+                    tot += np.sum(As * fer * Ww) * weights[i]
+                return (tot + nstates_base) * spin_degeneracy
+
+            n_target = nlist[iT] + nelec
+            μlist[iT] = brentq(lambda m: nocc(m) - n_target, wsa[0] - 0.1, wsa[-1] + 0.1, xtol=1e-15,
+                               rtol=4 * np.finfo(float).eps, maxiter=500)
+    return Σs, np.array(μlist)

@@ -432,3 +432,80 @@ def test_low_precision_star_product_native_matches_staged(ctx, prec):
     assert d_re <= (1e-5 if prec == "bf16" else 1e-6), d_re
     lo, hi = (1e-7, 1e-2) if prec == "bf16" else (1e-12, 1e-5)
     assert lo < d_64 < hi, d_64
+
+
+V2_T, V2_ETA = [0.03, 0.06], [0.0, 0.03]
+
+
+@pytest.mark.parametrize("identity", [True, False])
+def test_loop_dispersive_direct_eta_min_vs_oracle(ctx, identity):
+    """The direct Fan-Migdal method on a dispersive toy: ωph[ν, q] varying with q (one entry below
+    ω_acoustic), a non-square, non-symmetric iq_kk and iks_i != iks_f, so the device's per-pair
+    gather ωph[ν, iq_kk[iks_i[i], iks_f[f]]] is what the result depends on; per-temperature η_min =
+    [0, 0.03] (test/pb_fixture.jl:16); three iterations with the μ update (test/test_scgd0_gpu.jl:
+    7-37).  Against the oracle at 1e-10 (Σ and μ), one by one and batched, and, for f -> i the
+    identity, against the v2 iteration over the plain-loop ωq (test/test_scgd0_pb.jl:70-104)."""
+    from toys import dispersive_setup, pair_frequencies
+    from ref_numpy import scgd0_v2_reference
+    s = dispersive_setup(identity=identity)
+    o = occ(V2_T)
+    mu0 = [s.μ_start[0]] * 2
+    kw = dict(η_min=V2_ETA, η_init=0.05, maxiter=3, tol=0.0, ω_acoustic=1e-6, μ_start=mu0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dev = epsx.Scgd0Device(s, ctx)
+        r = dev.loop(o, method="auto", **kw)
+        rb = dev.loop(o, method="direct", batched=True, **kw)
+        ra = dev.loop(o, method="direct", **kw)
+    ref = oracle.loop_scgd0(s, V2_T, 0.0, 1.0, **kw)
+    assert r["info"]["fm_method"] == 0 and rb["info"]["fm_method"] == 0
+    for x in (r, rb):
+        assert rel(x["Σ"], ref["Σ"]) <= 1e-10
+        assert np.all(np.abs(x["μ"] - ref["μ"]) <= 1e-10 * np.abs(ref["μ"]))
+    assert np.array_equal(ra["Σ"], r["Σ"])
+    if identity:
+        ws = np.asarray(s.ωs)
+        Σ2, μ2 = scgd0_v2_reference(s.ε_i, s.w_i, s.nstates_base, s.g2, pair_frequencies(s), ws,
+                                    np.asarray(s.ωs_spectral), oracle.kramers_kronig_matrix(ws), V2_T, [0.0, 0.0],
+                                    1.0, 2.0, mu0, V2_ETA, 0.05, 3, 1e-6)
+        assert rel(r["Σ"], Σ2) <= 1e-10
+        assert np.all(np.abs(r["μ"] - μ2) <= 1e-9)
+
+
+@pytest.mark.parametrize("nlist", [-0.3, 0.3])
+def test_loop_semiconductor_mu_branches(ctx, nlist):
+    """occ.type = :Semiconductor: the hole count for nlist < 0 and the electron count for
+    nlist > 0 (src/scgd0.jl:314, :361-366) in the device μ solve, three iterations, against the
+    oracle at 1e-10; and the hole branch gives another μ than the electron formula would."""
+    s = holstein(nlist=nlist)
+    o = occ([0.1], nlist, typ="Semiconductor")
+    r, ref = run_both(s, o, η_init=0.05, maxiter=3, tol=0.0)
+    assert rel(r["Σ"], ref["Σ"]) <= 1e-10
+    assert abs(r["μ"][0] - ref["μ"][0]) <= 1e-10 * max(1.0, abs(ref["μ"][0]))
+    rm, _ = run_both(s, occ([0.1], nlist, typ="Metal"), η_init=0.05, maxiter=3, tol=0.0)
+    if nlist < 0:
+        # the window's Σ w_i ∫A_i is not exactly 1, so holes and electrons give different μ
+        assert abs(rm["μ"][0] - r["μ"][0]) > 1e-9
+    else:
+        assert rel(rm["Σ"], r["Σ"]) <= 1e-13 and rm["μ"][0] == r["μ"][0]
+
+
+def test_star_method_eta_min_clamp(ctx):
+    """The star method's h build clamps the representative's Σ at -η_min[T]: the t2g model at
+    nk = 8 with η_min = [0, 0.03] eV, star against direct (1e-12) and the oracle (1e-10)."""
+    s, meta = t2g_setup(nk=8, Tlist_K=(100.0, 300.0), grid_windows=((0.3, 4e-3), (1.2, 16e-3), (3.0, 64e-3)))
+    o = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    kw = dict(η_min=[0.0, 0.03], η_init=5e-2, maxiter=3, tol=0.0, ω_acoustic=1e-6)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dev = epsx.Scgd0Device(s, ctx)
+        rs = dev.loop(o, method="star", batched=True, **kw)
+        rd = dev.loop(o, method="direct", **kw)
+        r0 = dev.loop(o, method="star", batched=True, **dict(kw, η_min=0.0))
+    ref = oracle.loop_scgd0(s, o.Tlist, o.nlist, 1.0, **kw)
+    assert rs["info"]["fm_method"] == 2
+    assert rel(rs["Σ"], rd["Σ"]) <= 1e-12
+    assert rel(rs["Σ"], ref["Σ"]) <= 1e-10
+    assert np.all(np.abs(rs["μ"] - ref["μ"]) <= 1e-10 * np.abs(ref["μ"]))
+    assert np.array_equal(rs["Σ"][:, :, 0], r0["Σ"][:, :, 0]) or rel(rs["Σ"][:, :, 0], r0["Σ"][:, :, 0]) <= 1e-13
+    assert rel(rs["Σ"][:, :, 1], r0["Σ"][:, :, 1]) > 1e-4

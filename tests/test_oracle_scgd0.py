@@ -16,7 +16,7 @@ from epsx.grids import FrequencyGrid
 from epsx.models import holstein_1d_setup
 from epsx.scgd0 import build_multiplets
 from ref_numpy import (holstein_bare_mu, holstein_local_scgd0, holstein_oneshot_imsigma, interp_flat,
-                       n_old, naive_fan_migdal, pseudorandom, trapezoid)
+                       n_old, naive_fan_migdal, pseudorandom, scgd0_v2_reference, trapezoid)
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")))
 NK, T0, OMEGA0, G2 = 64, 0.1, 0.2, 0.2
@@ -197,3 +197,62 @@ def test_run_record():
     assert np.all(np.abs(np.log(err / np.array(rr["err"]))) < math.log(1.3))
     assert abs(r["μ"][0] - rr["mu"]) < 1e-5
     assert abs(r["history"]["min_spectral_weight"][0, -1] - rr["min_spectral_weight"]) < 1e-4
+
+
+def _v2_case(identity=True, nki=5, nkf=6):
+    from toys import dispersive_setup, pair_frequencies
+    s = dispersive_setup(identity=identity, nki=nki, nkf=nkf)
+    return s, pair_frequencies(s)
+
+
+def _v2_reference(s, ωq, Tlist, η_min, niter=2):
+    ws = np.asarray(s.ωs)
+    return scgd0_v2_reference(s.ε_i, s.w_i, s.nstates_base, s.g2, ωq, ws, np.asarray(s.ωs_spectral),
+                              oracle.kramers_kronig_matrix(ws), Tlist, [0.0] * len(Tlist), 1.0, 2.0,
+                              [s.μ_start[0]] * len(Tlist), η_min, 0.05, niter, 1e-6)
+
+
+V2_T, V2_ETA = [0.03, 0.06], [0.0, 0.03]
+
+
+def _oracle_v2(s, niter=2):
+    return oracle.loop_scgd0(s, V2_T, 0.0, 1.0, η_min=V2_ETA, η_init=0.05, maxiter=niter, tol=0.0,
+                             ω_acoustic=1e-6, μ_start=[s.μ_start[0]] * 2)
+
+
+def test_dispersive_gather_and_eta_clamp_against_v2_reference():
+    """test/test_scgd0_pb.jl:70-104 restated on a toy: phonon frequencies varying with q (one below
+    ω_acoustic), a non-square, non-symmetric iq_kk, iks_i != iks_f, per-temperature η_min = [0, 0.03]
+    (test/pb_fixture.jl:16), two iterations with the μ update.  The oracle's loop against the v2
+    iteration over the per-pair ωq[ν, i, f] built by a plain loop: Σ at 1e-10, μ at 1e-9."""
+    s, ωq = _v2_case()
+    assert len(np.unique(ωq)) > 10 and np.any(ωq < 1e-6)
+    r = _oracle_v2(s)
+    Σ_ref, μ_ref = _v2_reference(s, ωq, V2_T, V2_ETA)
+    assert rel(r["Σ"], Σ_ref) <= 1e-10
+    assert np.max(np.abs(r["μ"] - μ_ref)) <= 1e-9
+    # the clamp acts: the η_min = 0.03 temperature differs from the same run without it
+    Σ_0, _ = _v2_reference(s, ωq, V2_T, [0.0, 0.0])
+    assert rel(Σ_0[:, :, 1], Σ_ref[:, :, 1]) > 1e-3
+
+
+@pytest.mark.parametrize("plant", ["transposed iq_kk", "swapped k maps", "no clamp"])
+def test_dispersive_check_catches_planted_mistakes(plant):
+    """Plausible mistakes planted in the oracle's inputs, each of which the v2 check must catch: the
+    q map read transposed (iq_kk[iks_f[f], iks_i[i]]), the outer and inner k maps swapped, and the
+    η_min clamp dropped.  Square iq_kk so that the transposed read stays in range."""
+    import dataclasses
+    s, ωq = _v2_case(nki=6, nkf=6)
+    assert not np.array_equal(s.iq_kk, s.iq_kk.T) and not np.array_equal(s.iks_i, s.iks_f)
+    Σ_ref, μ_ref = _v2_reference(s, ωq, V2_T, V2_ETA)
+    assert rel(_oracle_v2(s)["Σ"], Σ_ref) <= 1e-10
+    if plant == "transposed iq_kk":
+        bad = dataclasses.replace(s, iq_kk=np.asfortranarray(s.iq_kk.T))
+        r = _oracle_v2(bad)
+    elif plant == "swapped k maps":
+        bad = dataclasses.replace(s, iks_i=s.iks_f.copy(), iks_f=s.iks_i.copy())
+        r = _oracle_v2(bad)
+    else:
+        r = oracle.loop_scgd0(s, V2_T, 0.0, 1.0, η_min=[0.0, 0.0], η_init=0.05, maxiter=2, tol=0.0,
+                              ω_acoustic=1e-6, μ_start=[s.μ_start[0]] * 2)
+    assert rel(r["Σ"], Σ_ref) > 1e-4
