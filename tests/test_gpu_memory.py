@@ -57,8 +57,9 @@ def test_pool_is_emptied_when_an_allocation_needs_it():
         hB, _ = c.device_memory()       # B's problem and all its run's temporaries
     finally:
         c.close()
-    # the A run released its temporaries to the pool, and B's upload kept them
-    assert p1 > 0 and p2 == p1
+    # the A run released its temporaries to the pool, and B's upload kept them (and released its
+    # own set-up temporaries next to them)
+    assert p1 > 0 and p2 >= p1
     # a cap that holds A's run and B's run on their own, but not B's upload next to A's pool
     lo = max(h1, hB)
     assert lo < h2
@@ -74,7 +75,8 @@ def test_pool_is_emptied_when_an_allocation_needs_it():
         assert c.device_memory() == (h1, p1)
         dev = epsx.Scgd0Device(B, c)
         held, pooled = c.device_memory()
-        assert pooled == 0 and held == h2 - p1     # uploaded by emptying the pool
+        # uploaded by emptying the pool: what it keeps now is at most B's own set-up temporaries
+        assert held <= cap_mb << 20 and pooled <= p2 - p1 < p1
         with warnings.catch_warnings():
             warnings.simplefilter("ignore")
             r = dev.loop(OCC1, **dict(KW, μ_start=[0.0]))
