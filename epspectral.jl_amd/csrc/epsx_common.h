@@ -126,6 +126,9 @@ struct epsx_ctx {
     // the device were full, so that the out-of-memory path can be exercised on any device
     size_t dev_bytes = 0;
     size_t alloc_cap = 0;
+    // this device's copy of the fixed workspace of epsx_kramers_kronig_noalloc (epsx_kk.hip),
+    // looked up when the context opens
+    double *kk_scratch = nullptr;
     std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, split) -> tile shape
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
@@ -199,6 +202,8 @@ void grid_nodes(int64_t n, double origin, double unit, const int64_t *offsets, d
 // Kernel launchers (epsx_kk.hip)
 int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K);
 int kk_split(epsx_ctx *ctx, int64_t n, const double *d_K, double *d_Ko, double *d_Kd);
+// the device address of the no-alloc workspace on the context's device (called by epsx_open)
+int kk_scratch_init(epsx_ctx *ctx);
 // the off-diagonal KK transposed into a BF16 / FP32 k-contiguous operand of row pitch np
 int kk_lp_build(epsx_ctx *ctx, int precision, int64_t n, int64_t np, const double *d_Ko, void *out);
 // columns of the gated slots in BF16 / FP32 with rows padded to np (zero beyond nrow)
@@ -248,6 +253,9 @@ struct GemmBatch {
     const double *diag = nullptr;     // with Z: add diag[i] Zim[i, j] in FP64
     bool tune = true;                 // the autotuner may time this product (its gate is not hiding work)
     hipEvent_t mark = nullptr;        // recorded after the main kernel (before the split-k sum)
+    double *part = nullptr;           // split-k partials in this caller-provided device buffer of
+    int64_t part_cap = 0;             // part_cap doubles (none: the context's workspace); a split
+                                      // whose partials do not fit runs unsplit
     float *Cf = nullptr;              // lpgemm_tn only: FP32 output in place of C (no accumulate, no Z)
 };
 // C[m, n] = A[m, k] * B[k, n] (+ C if accumulate), FP64, column-major, on ctx->stream

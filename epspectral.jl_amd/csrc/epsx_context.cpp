@@ -218,6 +218,11 @@ int epsx_open(int device, epsx_ctx **out)
         delete ctx;
         return EPSX_ERR_DEVICE;
     }
+    if (epsx::kk_scratch_init(ctx) != EPSX_OK) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return EPSX_ERR_DEVICE;
+    }
     *out = ctx;
     return EPSX_OK;
 }

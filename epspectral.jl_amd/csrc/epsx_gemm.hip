@@ -531,7 +531,15 @@ static int plan_splits(epsx_ctx *ctx, GemmArgs &p, const GemmBatch &bt, int64_t 
         Smax = p.S_of[c] > Smax ? p.S_of[c] : Smax;
         if (c <= bt.n_inst && (int64_t)p.S_of[c] * c > p.units_per_tile) p.units_per_tile = (int64_t)p.S_of[c] * c;
     }
-    if (Smax > 1) {
+    if (Smax > 1 && bt.part && (int64_t)((size_t)p.m * p.n * Smax * bt.n_inst) > bt.part_cap) {
+        // the caller's workspace cannot hold the partials: no split
+        S = Smax = 1;
+        for (int c = 0; c <= NSPLIT_TAB; c++) p.S_of[c] = 1;
+        p.units_per_tile = bt.n_inst;
+    }
+    if (Smax > 1 && bt.part) {
+        p.part = bt.part;
+    } else if (Smax > 1) {
         const size_t need = 8 * (size_t)p.m * p.n * Smax * bt.n_inst;
         if (ctx->gemm_ws.bytes < need) {
             hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;

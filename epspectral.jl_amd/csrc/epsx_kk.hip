@@ -107,6 +107,23 @@ int kk_split(epsx_ctx *ctx, int64_t n, const double *d_K, double *d_Ko, double *
     return EPSX_OK;
 }
 
+// The fixed device workspace of epsx_kramers_kronig_noalloc: the KK matrix (n² doubles) and, behind
+// it, the product's split-k partials.  40 MiB per device, held until the process ends; every call on
+// one device shares it.  Host code never uses this name as an address: epsx_open looks up this
+// device's copy with hipGetSymbolAddress and keeps it in the context (kk_scratch).
+constexpr int64_t KK_SCRATCH_DOUBLES = int64_t(5) << 20;
+constexpr int64_t KK_NOALLOC_MAX_N = 2048;   // n² <= 4 Mi doubles, the rest for partials
+static_assert(KK_NOALLOC_MAX_N * KK_NOALLOC_MAX_N < KK_SCRATCH_DOUBLES, "workspace too small");
+static __device__ __align__(256) double scratch[KK_SCRATCH_DOUBLES];
+
+int kk_scratch_init(epsx_ctx *ctx)
+{
+    void *p = nullptr;
+    EPSX_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ctx->kk_scratch = static_cast<double *>(p);
+    return EPSX_OK;
+}
+
 int kk_build(epsx_ctx *ctx, int64_t n, const double *d_ws, double *d_K)
 {
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)n);
@@ -153,6 +170,34 @@ int epsx_kramers_kronig(epsx_ctx *ctx, int64_t n, const double *ws, int64_t ncol
     EPSX_HIP(ctx, hipMemcpyAsync(ys, dy.b.p, sizeof(double) * n * ncol, hipMemcpyDeviceToHost, ctx->stream));
     EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return EPSX_OK;
+}
+
+int epsx_kramers_kronig_noalloc(epsx_ctx *ctx, int64_t n, const double *d_ws, int64_t ncol,
+                                const double *d_zs, double *d_ys, void *stream)
+{
+    if (!ctx) return EPSX_ERR_ARG;
+    if (n < 2 || ncol < 0 || !d_ws || (ncol && (!d_zs || !d_ys)))
+        return epsx::set_error(ctx, EPSX_ERR_ARG, "kramers_kronig_noalloc: bad arguments");
+    if (n > epsx::KK_NOALLOC_MAX_N)
+        return epsx::set_error(ctx, EPSX_ERR_ARG,
+                               "kramers_kronig_noalloc: %lld nodes exceed the fixed workspace's limit of %lld",
+                               (long long)n, (long long)epsx::KK_NOALLOC_MAX_N);
+    if (!ctx->kk_scratch) return epsx::set_error(ctx, EPSX_ERR_STATE, "kramers_kronig_noalloc: no workspace");
+    if (ncol == 0) return EPSX_OK;
+    EPSX_HIP(ctx, hipSetDevice(ctx->device));
+    // the launches go to the caller's stream for this call (every launcher uses ctx->stream)
+    struct StreamSwap {
+        epsx_ctx *c; hipStream_t saved;
+        ~StreamSwap() { c->stream = saved; }
+    } swap{ctx, ctx->stream};
+    if (stream) ctx->stream = static_cast<hipStream_t>(stream);
+    double *K = ctx->kk_scratch;
+    EPSX_TRY(epsx::kk_build(ctx, n, d_ws, K));
+    epsx::GemmBatch bt;
+    bt.tune = false;   // no timing runs: the tuned tile shape if one is known, else the default
+    bt.part = K + n * n;
+    bt.part_cap = epsx::KK_SCRATCH_DOUBLES - n * n;
+    return epsx::dgemm(ctx, n, ncol, n, K, n, d_zs, n, d_ys, n, false, bt);
 }
 
 }  // extern "C"

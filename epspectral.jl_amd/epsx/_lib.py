@@ -45,6 +45,7 @@ EXPORTS = [
     ("epsx_vertex_shift_tables", C.c_int, [P, I64, P, I64, P, P, P]),
     ("epsx_kramers_kronig_matrix", C.c_int, [P, I64, P, P]),
     ("epsx_kramers_kronig", C.c_int, [P, I64, P, I64, P, P]),
+    ("epsx_kramers_kronig_noalloc", C.c_int, [P, I64, P, I64, P, P, P]),
     ("epsx_add_fan_migdal_imsigma", C.c_int,
      [P, I64, I64, I64, P, P, P, P, P, I64, D, D, P, I64, P, D, D, D, I32, P]),
     ("epsx_scgd0_set_problem", C.c_int, [P, P]),

@@ -85,6 +85,17 @@ int epsx_kramers_kronig_matrix(epsx_ctx *ctx, int64_t n, const double *ws, doubl
 int epsx_kramers_kronig(epsx_ctx *ctx, int64_t n, const double *ws, int64_t ncol,
                         const double *zs, double *ys);
 
+/* No counterpart in the reference: epsx_kramers_kronig for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * ys = K * zs with ws [n], zs [n, ncol] and ys [n, ncol] in device memory; K and the product's
+ * partial sums live in a fixed 40 MiB device workspace held per device until the process ends,
+ * so n is at most 2048 (a larger n is EPSX_ERR_ARG).  Asynchronous on `stream` (a hipStream_t;
+ * NULL: the context's stream).  No hipMalloc, hipFree or memset is issued.  All calls on one
+ * device share the workspace: calls that overlap on one device (other streams, other contexts)
+ * race. */
+int epsx_kramers_kronig_noalloc(epsx_ctx *ctx, int64_t n, const double *d_ws, int64_t ncol,
+                                const double *d_zs, double *d_ys, void *stream);
+
 /* add_fan_migdal_imsigma!, src/scgd0_kernel.jl:59-72, with the per-pair inputs of the
  * reference's [ν, i, f] arrays (the (νf, i) fusion of src/scgd0.jl:264-279 is a layout
  * change the kernel does itself):
