@@ -166,3 +166,64 @@ def test_low_precision_shift_gemm(ctx, prec, tol_apply):
     # the precision effect (same rtol) is far below the solver-tolerance effect (rtol 1e-2 vs 1e-8)
     assert rel(loose["σ_ladder"], loose64["σ_ladder"]) <= (1e-6 if prec == "fp32" else 5e-3)
     assert rel(loose["σ_ladder"], tight["σ_ladder"]) <= 3e-2
+
+
+def test_t2g_oh_gmres_against_oracle(ctx):
+    """The t2g model at nk = 4 with its Oh vertex rotations (C4's model reduced; the Σ of the
+    oracle's scGD0 run): GMRES at rtol 1e-8 for Ω = 0 and 20 meV at 100 K, warm-started from the
+    previous Ω, against the oracle: equal iteration counts, σ and J at 1e-10."""
+    from test_oracle_ladder import t2g_case
+    s, L, r, meta = t2g_case()
+    ld = device(ctx, s, L)
+    T = meta["Tlist"][0]
+    Ωs = [0.0, 0.02]
+    dv = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], [T], Ωs, rtol=1e-8, maxiter=60, store_vertex=True)
+    J = None
+    for iO, Ω in enumerate(Ωs):
+        o = oracle.ladder_solve(L, r["Σ"][:, :, 0], r["μ"][0], T, Ω, J=J, rtol=1e-8, maxiter=60)
+        J = o["J"]
+        assert o["converged"] and dv["converged"][iO, 0]
+        assert dv["niter"][iO, 0] == o["niter"]
+        assert rel(dv["σ_bubble"][:, :, iO, 0], o["σ_bubble"]) <= 1e-12
+        assert rel(dv["σ_ladder"][:, :, iO, 0], o["σ_ladder"]) <= 1e-10
+        assert rel(dv["J"][..., iO, 0], o["J"]) <= 1e-10
+    assert rel(dv["σ_ladder"], dv["σ_bubble"]) > 0.05
+
+
+def test_c4_t2g16_gmres_solves_the_equation(ctx):
+    """C4 at its size: the t2g model at 16³ (ni = 331, nf = 8448, Oh rotations) from its converged
+    FP64 scGD0 Σ at 100 K; GMRES at rtol 1e-2 in FP64, FP32 and BF16 shift-table GEMMs for Ω = 0 and
+    100 meV.  The FP64 solution satisfies J = J0 + L(J) to 10 rtol with the device's own L (J0 from
+    the oracle's propagators, test/test_ladder_pb.jl:48-57), and σ_ladder of each precision lies
+    within 3 rtol of the FP64 GMRES at rtol 1e-8."""
+    from epsx.models import t2g_ladder_setup
+    from test_oracle_ladder import propagators
+    s, R, v, meta = t2g_ladder_setup(nk=16, Tlist_K=(100.0,))
+    dev = epsx.Scgd0Device(s, ctx)
+    occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r = dev.loop(occ, η_min=0.0, η_init=5e-3, maxiter=200, tol=1e-6, ω_acoustic=1e-6, method="auto",
+                     batched=True)
+    assert r["converged"].all()
+    L = setup_ladder(s, R, v)
+    ld = LadderDevice(dev, L)
+    T = float(meta["Tlist"][0])
+    Ωs = [0.0, 0.1]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ref = ld.loop(r["Σ"], r["μ"], [T], Ωs, rtol=1e-8, maxiter=80)
+        runs = {p: ld.loop(r["Σ"], r["μ"], [T], Ωs, rtol=1e-2, maxiter=30, precision=p, store_vertex=(p == "fp64"))
+                for p in ("fp64", "fp32", "bf16")}
+    g = runs["fp64"]
+    assert g["converged"].all() and np.all(g["niter"] >= 1)
+    for iO, Ω in enumerate(Ωs):
+        Gm, Gp = propagators(L, r, 0, Ω, T)
+        J0 = oracle.attach_greens_bare(Gm, L.v, Gp)
+        J = g["J"][..., iO, 0]
+        res = J - J0 - ld.apply(J, Gm, Gp, T, 1e-6)
+        assert np.linalg.norm(res) <= 10 * 1e-2 * np.linalg.norm(J0)
+    for p, x in runs.items():
+        assert x["converged"].all(), p
+        assert np.array_equal(x["σ_bubble"], ref["σ_bubble"]), p
+        assert rel(x["σ_ladder"], ref["σ_ladder"]) <= 3e-2, p

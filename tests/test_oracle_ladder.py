@@ -182,3 +182,61 @@ def test_solvers_satisfy_the_equation():
     # warm start from the solution: converged at once
     w = oracle.ladder_solve(L, r["Σ"][:, :, 0], r["μ"][0], T, Ω, rtol=1e-8, maxiter=60, J=g["J"])
     assert w["niter"] <= 1 and w["converged"]
+
+
+_T2G = {}
+
+
+def t2g_case():
+    """The t2g model at nk = 4 with its Oh vertex rotations R_f and band velocities (the C4 model
+    reduced), 3 modes, a coarse graded grid; Σ from 200 oracle iterations (converged or not, it is
+    the input of the ladder)."""
+    if not _T2G:
+        from epsx.models import t2g_ladder_setup
+        s, R, v, meta = t2g_ladder_setup(nk=4, Tlist_K=(100.0, 300.0), nmodes=3,
+                                         grid_windows=((0.3, 8e-3), (1.2, 32e-3), (3.0, 128e-3)))
+        r = oracle.loop_scgd0(s, meta["Tlist"], 0.0, 1.0, η_min=0.0, η_init=5e-2, tol=1e-7, maxiter=200,
+                              ω_acoustic=1e-6)
+        _T2G.update(s=s, L=setup_ladder(s, R, v), r=r, meta=meta)
+    return _T2G["s"], _T2G["L"], _T2G["r"], _T2G["meta"]
+
+
+def test_apply_ladder_against_naive_t2g_oh():
+    """One application on the t2g model with the Oh rotations of the vertex (R_f != identity, 3x3
+    mixing of the Cartesian components) against the naive loop (test/test_ladder_kernel.jl:134-172)."""
+    s, L, r, meta = t2g_case()
+    assert np.count_nonzero(np.abs(L.R[0, 1, :]) + np.abs(L.R[0, 2, :])) > 0
+    nw, ni = len(s.ωs), s.g2.shape[1]
+    J = pseudorandom_complex(nw, 4, 3, 2, ni, seed=1.3)
+    J[0] = 0
+    J[-1] = 0
+    T = meta["Tlist"][0]
+    Gm, Gp = propagators(L, r, 0, 0.02, T)
+    KK = oracle.kramers_kronig_matrix(np.asarray(s.ωs))
+    ref = naive_apply_ladder(s.g2, s.ωph, s.iq_kk, s.iks_i, s.iks_f, s.w_f, s.f_to_i, L.R, s.ωs, KK, J, Gm, Gp, T,
+                             1e-6)
+    LJ = oracle.apply_ladder_kernel(L, J, Gm, Gp, T, 1e-6)
+    assert np.linalg.norm(ref) > 1e-2 * np.linalg.norm(J)
+    assert rel(LJ, ref) <= 1e-12
+
+
+def test_t2g_sigma_symmetrised_over_oh():
+    """run_scgd0_ladder symmetrises the irreducible-state sum over the point group
+    (src/ladder.jl:94-101).  Over the 48 operations of Oh a rank-2 tensor becomes tr(σ)/3 times the
+    identity: the t2g model's unsymmetrised σ_bubble and σ_ladder (from the oracle's GMRES, a vertex
+    correction of tens of percent) are anisotropic, their symmetrised forms isotropic with the same
+    trace, and a wrong group (the identity only, or a single mirror) leaves them anisotropic."""
+    from epsx.ladder import symmetrize
+    from epsx.models import oh_operations
+    s, L, r, meta = t2g_case()
+    T = meta["Tlist"][0]
+    g = oracle.ladder_solve(L, r["Σ"][:, :, 0], r["μ"][0], T, 0.02, rtol=1e-8, maxiter=60)
+    assert g["converged"] and rel(g["σ_ladder"], g["σ_bubble"]) > 0.05
+    ops = oh_operations()
+    assert len(ops) == 48 and all(np.allclose(o @ o.T, np.eye(3)) for o in ops)
+    for σ in (g["σ_bubble"], g["σ_ladder"]):
+        aniso = np.linalg.norm(σ - np.trace(σ) / 3 * np.eye(3))
+        assert aniso > 1e-2 * abs(np.trace(σ))
+        sym = symmetrize(σ, ops)
+        assert np.linalg.norm(sym - np.trace(σ) / 3 * np.eye(3)) <= 1e-14 * abs(np.trace(σ))
+        assert np.linalg.norm(symmetrize(σ, [np.diag([-1.0, 1, 1])]) - np.trace(σ) / 3 * np.eye(3)) > 1e-3 * abs(np.trace(σ))
