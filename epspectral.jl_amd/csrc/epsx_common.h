@@ -340,6 +340,14 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
                    double *occ, double *hout, const int32_t *act, const int32_t *act_h, uint16_t *hb,
                    const int32_t *act_b, float *hf, const int32_t *act_f, int64_t kp);
 int gt_lp_launch(epsx_ctx *ctx, int precision, int64_t K, int64_t kp, int64_t ni, const double *Gt, void *gl);
+// the Einstein factorisation's H'[ω, ν + nm (p - p0)] = w_f H_ν[ω, f] for the sorted inner states
+// [p0, p0 + np) of every slot (occ from h_build_launch), and the check that no mode below
+// ω_acoustic holds a non-finite coupling
+int einstein_h_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const double *ws, const int32_t *hk, const double *ht,
+                      const double *wnu, const double *S, const double *eta, const int32_t *tidx, int nslots,
+                      double w_acoustic, const double *occ, int64_t p0, int64_t np, const double *ef, const double *wf,
+                      const int32_t *fi, double *hout, const int32_t *act);
+int acoustic_finite_check(epsx_ctx *ctx, int64_t nm, int64_t n, const double *g2p, const uint8_t *skip, bool *ok);
 size_t fm_part_elems(int64_t nfs, int64_t ni, int64_t nw, int64_t nT, int64_t f_tile);
 int fm_grid_host(int64_t nw, const int64_t *offsets, std::vector<int32_t> &off,
                  std::vector<double> &rgap, std::vector<int32_t> &ltab, int64_t &span);

@@ -217,6 +217,65 @@ __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
         if (ip0 + r < ni) out[wn + nwm * (ip0 + r)] = v[r];
 }
 
+// The Einstein factorisation (dispersionless modes, any inner-state energies): with ωq = ω_ν
+// for every pair, the term of (ω, ν, i, f) is w_f g2[ν, i, f] H_ν[ω, f] with
+//   H_ν[ω, f] = Im G_f(ω+ω_ν) (n_ν + f(ω+ω_ν-μ)) + Im G_f(ω-ω_ν) (n_ν + 1 - f(ω-ω_ν-μ)),
+// G_f(x) = 1 / (x - ε_f - Σin_{i_f}(x)), so ImΣ = H' g2p with H'[ω, ν + nm p] = w_f H_ν[ω, f] over
+// the star-sorted inner states p and g2p[ν + nm p, i] the resident coupling: one GEMM of
+// nω x (nm nf) x ni per temperature, in chunks of inner states.  This kernel builds H' for the
+// sorted inner states [p0, p0 + np) of every slot: one thread per (ω, ν) and EH_R consecutive
+// inner states (the (ω, ν) constants read once), lanes along ω.  Modes below ω_acoustic are zero.
+constexpr int EH_R = 4;
+__global__ __launch_bounds__(256) void einstein_h_kernel(HArgs a, int64_t p0, int64_t np, const double *__restrict__ ef,
+                                                         const double *__restrict__ wf, const int32_t *__restrict__ fi)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    const int64_t nw = a.nw, nwm = nw * a.nm, ngrp = (np + EH_R - 1) / EH_R;
+    if (q >= nwm * ngrp) return;
+    const int tcol = a.tidx[z];
+    if (a.act && !a.act[tcol]) return;
+    const int64_t wn = q % nwm, pl0 = (q / nwm) * EH_R;
+    const int64_t w = wn % nw, nu = wn / nw;
+    double *out = a.hout + nwm * np * (int64_t)z;
+    double v[EH_R];
+    if (a.wnu[nu] < a.w_acoustic) {
+#pragma unroll
+        for (int r = 0; r < EH_R; r++) v[r] = 0.0;
+    } else {
+        const HConst c = h_const(a, w, nu, tcol, z);
+        const double eta = c.eta;
+#pragma unroll
+        for (int r = 0; r < EH_R; r++) {
+            const int64_t p = p0 + (pl0 + r < np ? pl0 + r : np - 1);
+            const double2 *col = c.S + a.nw * fi[p];
+            const double2 la = col[c.ka], ha = col[c.ka + 1], le = col[c.ke], he = col[c.ke + 1];
+            const double sar = (1 - c.ta) * la.x + c.ta * ha.x;
+            const double sai = (1 - c.ta) * fmin(la.y, -eta) + c.ta * fmin(ha.y, -eta);
+            const double ser = (1 - c.te) * le.x + c.te * he.x;
+            const double sei = (1 - c.te) * fmin(le.y, -eta) + c.te * fmin(he.y, -eta);
+            const double e = ef[p];
+            const double zar = c.x0 + c.wnu - e - sar, zai = -sai;
+            const double zer = c.x0 - c.wnu - e - ser, zei = -sei;
+            const double da = zar * zar + zai * zai, de = zer * zer + zei * zei;
+            v[r] = wf[p] * (-(zai * c.o.x * de + zei * c.o.y * da) / (da * de));
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < EH_R; r++)
+        if (pl0 + r < np) out[wn + nwm * (pl0 + r)] = v[r];
+}
+
+// Any non-finite coupling in a mode below ω_acoustic (skip[ν] = 1): the Einstein product would
+// multiply it by a zero H column (NaN), where the term-by-term sum skips the mode before reading g2.
+__global__ __launch_bounds__(256) void acoustic_finite_kernel(int64_t nm, int64_t n, const double *__restrict__ g2p,
+                                                              const uint8_t *__restrict__ skip, int32_t *bad)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (skip[q % nm] && !isfinite(g2p[q])) atomicOr(bad, 1);
+}
+
 // h per slot in low precision (E = unsigned short: BF16 bits, or float), transposed for the
 // low-precision product: hl[k + kp w] with k = ν + nm i' (zero for nm ni <= k < kp, kp a multiple
 // of 8).  One block per (64 frequencies, HL_IP outer states, slot): one thread per (ω, ν) computes
@@ -450,6 +509,36 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
         }
         EPSX_HIP(ctx, hipGetLastError());
     }
+    return EPSX_OK;
+}
+
+int einstein_h_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const double *ws, const int32_t *hk, const double *ht,
+                      const double *wnu, const double *S, const double *eta, const int32_t *tidx, int nslots,
+                      double w_acoustic, const double *occ, int64_t p0, int64_t np, const double *ef, const double *wf,
+                      const int32_t *fi, double *hout, const int32_t *act)
+{
+    HArgs a{};
+    a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht; a.wnu = wnu; a.S = S; a.eta = eta;
+    a.tidx = tidx; a.w_acoustic = w_acoustic; a.occ = const_cast<double *>(occ); a.hout = hout; a.act = act;
+    const int64_t threads = nw * nm * ((np + EH_R - 1) / EH_R);
+    hipLaunchKernelGGL(einstein_h_kernel, dim3((unsigned)((threads + 255) / 256), (unsigned)nslots), dim3(256), 0,
+                       ctx->stream, a, p0, np, ef, wf, fi);
+    EPSX_HIP(ctx, hipGetLastError());
+    return EPSX_OK;
+}
+
+int acoustic_finite_check(epsx_ctx *ctx, int64_t nm, int64_t n, const double *g2p, const uint8_t *skip, bool *ok)
+{
+    Tmp flag;
+    const int32_t zero = 0;
+    EPSX_TRY(tmp_upload(ctx, flag, &zero, 4));
+    hipLaunchKernelGGL(acoustic_finite_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, nm, n, g2p,
+                       skip, flag.ptr<int32_t>());
+    EPSX_HIP(ctx, hipGetLastError());
+    int32_t bad = 0;
+    EPSX_HIP(ctx, hipMemcpyAsync(&bad, flag.b.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *ok = bad == 0;
     return EPSX_OK;
 }
 

@@ -864,14 +864,35 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         return set_error(ctx, EPSX_ERR_ARG, "unknown precision %d", par->precision);
     // The Fan-Migdal method: the factorised forms only behind their preconditions.
     int method = par->fm_method;
-    if (method == EPSX_FM_AUTO) method = (s.einstein && s.star_ok) ? EPSX_FM_STAR : EPSX_FM_DIRECT;
-    if (method == EPSX_FM_EINSTEIN) method = EPSX_FM_DIRECT;  // served by the direct kernel
+    if (method == EPSX_FM_AUTO)
+        method = s.einstein ? (s.star_ok ? EPSX_FM_STAR : EPSX_FM_EINSTEIN) : EPSX_FM_DIRECT;
     if (method == EPSX_FM_STAR && !(s.einstein && s.star_ok))
         return set_error(ctx, EPSX_ERR_ARG, "the star Fan-Migdal method needs dispersionless modes (ωph[ν, :] "
                          "constant) and ε_f == ε_i of the representative for every inner state");
+    if (method == EPSX_FM_EINSTEIN && !s.einstein)
+        return set_error(ctx, EPSX_ERR_ARG, "the Einstein Fan-Migdal method needs dispersionless modes (ωph[ν, :] "
+                         "constant)");
+    if (method == EPSX_FM_EINSTEIN) {
+        // a mode below ω_acoustic contributes a zero column of H, which must meet finite couplings
+        // (the term-by-term sum skips such a mode before reading g2, which may hold Inf there)
+        bool any_ac = false;
+        std::vector<uint8_t> skip(s.nm);
+        for (int64_t nu = 0; nu < s.nm; nu++) any_ac |= (skip[nu] = s.wnu_host[nu] < par->w_acoustic) != 0;
+        if (any_ac) {
+            EPSX_TRY(upload(ctx, s.skip, skip.data(), s.nm));
+            bool ok = true;
+            EPSX_TRY(acoustic_finite_check(ctx, s.nm, s.nm * s.ni * s.nf, dptr<double>(s.g2p), dptr<uint8_t>(s.skip), &ok));
+            if (!ok) {
+                if (par->fm_method == EPSX_FM_EINSTEIN)
+                    return set_error(ctx, EPSX_ERR_ARG, "the Einstein Fan-Migdal method needs finite couplings in the "
+                                     "modes below ω_acoustic");
+                method = EPSX_FM_DIRECT;
+            }
+        }
+    }
     if (method == EPSX_FM_DIRECT && par->precision == EPSX_PREC_BF16)
         return set_error(ctx, EPSX_ERR_ARG, "the direct Fan-Migdal kernel runs in FP64 or FP32 (BF16 is for the "
-                         "star method's products)");
+                         "factorised methods' products)");
     EPSX_HIP(ctx, hipSetDevice(ctx->device));
     const int64_t nw = s.nw, ni = s.ni, nwa = s.nwa, nf = s.nf;
     const size_t nS = (size_t)nw * ni * nT;
@@ -909,6 +930,16 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     const bool use_f32 = lp_native;                                       // FP32 level present
     const int64_t kp = (s.nm * ni + 7) / 8 * 8;
     const int64_t kpw = (nw + 7) / 8 * 8;
+    // Einstein method: H' for a chunk of ein_fc sorted inner states per slot (gpu_tile if given, else
+    // chunks that keep H' of all temperatures within 2 GiB)
+    int64_t ein_fc = 0;
+    if (method == EPSX_FM_EINSTEIN) {
+        const int64_t per_f = 8 * nw * s.nm * nT;
+        ein_fc = par->f_tile > 0 ? par->f_tile : (int64_t(2) << 30) / per_f;
+        ein_fc = ein_fc < 1 ? 1 : (ein_fc > nf ? nf : ein_fc);
+        EPSX_TRY(tmp_alloc(ctx, hbuf.b, (size_t)per_f * ein_fc));
+        EPSX_TRY(tmp_alloc(ctx, hocc.b, 16 * (size_t)nw * s.nm * nT));
+    }
     if (method == EPSX_FM_STAR) {
         EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
         EPSX_TRY(tmp_alloc(ctx, hocc.b, 16 * (size_t)nw * s.nm * nT));
@@ -1220,6 +1251,34 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
                                        dptr<int32_t>(s.cls), d_tidx, ImSc.ptr<double>(), B.ImS.ptr<double>(),
                                        kk_fuse ? passes[q].gate : d_act);
                     EPSX_HIP(ctx, hipGetLastError());
+                }
+            }
+        } else if (method == EPSX_FM_EINSTEIN) {
+            // the occupations of every (ω, ν) per slot, then per chunk of sorted inner states H' and
+            // ImΣ (+)= H' g2p[chunk rows] for every slot in one batched product (g2p shared), once
+            // per precision pass (the staged BF16 / FP32 products round H' and g2 as they stage them)
+            EPSX_TRY(h_build_launch(ctx, nw, s.nm, ni, dptr<double>(s.ws), dptr<int32_t>(s.hk), dptr<double>(s.ht),
+                                    dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(), B.T.ptr<double>(),
+                                    B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA, par->w_acoustic,
+                                    hocc.ptr<double>(), nullptr, d_act, nullptr, nullptr, nullptr, nullptr, nullptr, 0));
+            for (int64_t p0 = 0; p0 < nf; p0 += ein_fc) {
+                const int64_t np = ein_fc < nf - p0 ? ein_fc : nf - p0;
+                EPSX_TRY(einstein_h_launch(ctx, nw, s.nm, ni, dptr<double>(s.ws), dptr<int32_t>(s.hk),
+                                           dptr<double>(s.ht), dptr<double>(s.wnu), B.S.ptr<double>(), B.eta.ptr<double>(),
+                                           d_tidx, nA, par->w_acoustic, hocc.ptr<double>(), p0, np, dptr<double>(s.ef_s),
+                                           dptr<double>(s.wf_s), dptr<int32_t>(s.fi_s), hbuf.ptr<double>(), d_act));
+                for (int q = 0; q < npass; q++) {
+                    GemmBatch bt;
+                    bt.n_inst = nA;
+                    bt.sA = nw * s.nm * np;
+                    bt.sC = nw * ni;
+                    bt.idx = d_tidx;
+                    bt.imap = 4;
+                    bt.gate = passes[q].gate;
+                    bt.precision = passes[q].prec;
+                    bt.tune = passes[q].tune;
+                    EPSX_TRY(dgemm(ctx, nw, ni, s.nm * np, hbuf.ptr<double>(), nw, dptr<double>(s.g2p) + s.nm * p0,
+                                   s.nm * nf, B.ImS.ptr<double>(), nw, p0 > 0, bt));
                 }
             }
         } else {

@@ -539,3 +539,89 @@ def test_c3_t2g24_eight_temperatures_mixed_precision(ctx):
         for m in s.multiplets:
             for k in m[1:]:
                 assert np.array_equal(r["Σ"][:, k - 1, :], r["Σ"][:, m[0] - 1, :])
+
+
+def einstein_toy():
+    """The dispersive toy with f -> i not the identity and ε_f != ε_{i_f}, its modes made
+    dispersionless (ωph[ν, :] = ωph[ν, 4]): the Einstein factorisation applies, the star method does
+    not."""
+    import dataclasses
+    from toys import dispersive_setup
+    s = dispersive_setup(identity=False)
+    nm, nq = s.ωph.shape
+    return dataclasses.replace(s, ωph=np.asfortranarray(np.repeat(s.ωph[:, 4:5], nq, axis=1)))
+
+
+def test_einstein_method_matches_direct_and_oracle(ctx):
+    """The Einstein factorisation ImΣ = H' g2 (dispersionless modes, inner-state energies that
+    differ from their representatives') against the direct kernel (1e-12) and the oracle (1e-10),
+    with per-temperature η_min and the μ update: picked by the automatic choice, the same one by one,
+    batched and in chunks of 3 inner states (gpu_tile, which only reorders the sum over f)."""
+    s = einstein_toy()
+    o = occ(V2_T)
+    kw = dict(η_min=V2_ETA, η_init=0.05, maxiter=3, tol=0.0, ω_acoustic=1e-6, μ_start=[s.μ_start[0]] * 2)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dev = epsx.Scgd0Device(s, ctx)
+        ra = dev.loop(o, method="auto", **kw)
+        rd = dev.loop(o, method="direct", **kw)
+        rb = dev.loop(o, method="einstein", batched=True, **kw)
+        rt = dev.loop(o, method="einstein", gpu_tile=3, **kw)
+        with pytest.raises(epsx.ArgumentError, match="star"):
+            dev.loop(o, method="star", **kw)
+    ref = oracle.loop_scgd0(s, V2_T, 0.0, 1.0, **kw)
+    assert ra["info"]["fm_method"] == 1 and rb["info"]["fm_method"] == 1 and rd["info"]["fm_method"] == 0
+    assert rel(ra["Σ"], rd["Σ"]) <= 1e-12
+    assert rel(ra["Σ"], ref["Σ"]) <= 1e-10
+    assert np.all(np.abs(ra["μ"] - ref["μ"]) <= 1e-10 * np.abs(ref["μ"]))
+    assert rel(rb["Σ"], ra["Σ"]) <= 1e-13 and rel(rt["Σ"], ra["Σ"]) <= 1e-12
+
+
+def test_einstein_method_acoustic_modes(ctx):
+    """A mode below ω_acoustic is a zero column of H': with finite couplings there the Einstein method
+    runs and equals the direct kernel; with Inf couplings there (allowed: the term-by-term sum skips
+    the mode before reading g2, src/scgd0_kernel.jl:22-25) the automatic choice falls back to the
+    direct kernel and an explicit request is refused."""
+    import dataclasses
+    s = einstein_toy()
+    wph = s.ωph.copy()
+    wph[0, :] = 0.0
+    s0 = dataclasses.replace(s, ωph=np.asfortranarray(wph))
+    o = occ(V2_T)
+    kw = dict(η_min=V2_ETA, η_init=0.05, maxiter=2, tol=0.0, ω_acoustic=1e-6, μ_start=[s.μ_start[0]] * 2,
+              fix_μ=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dev = epsx.Scgd0Device(s0, ctx)
+        re_ = dev.loop(o, method="einstein", **kw)
+        rd = dev.loop(o, method="direct", **kw)
+        assert re_["info"]["fm_method"] == 1 and rel(re_["Σ"], rd["Σ"]) <= 1e-12
+        g2 = s0.g2.copy()
+        g2[0] = np.inf
+        dev = epsx.Scgd0Device(dataclasses.replace(s0, g2=np.asfortranarray(g2)), ctx)
+        ra = dev.loop(o, method="auto", **kw)
+        assert ra["info"]["fm_method"] == 0 and np.all(np.isfinite(ra["Σ"]))
+        assert rel(ra["Σ"], rd["Σ"]) <= 1e-12
+        with pytest.raises(epsx.ArgumentError, match="finite couplings"):
+            dev.loop(o, method="einstein", **kw)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_einstein_low_precision_then_fp64(ctx, prec):
+    """The t2g model at nk = 8 with the inner-state energies moved off their representatives' by up
+    to 1e-6 eV (the star method's precondition fails, the Einstein one holds): the automatic choice
+    runs the Einstein factorisation; its FP32 / BF16 products until the low-precision floor, then FP64
+    to the same tol, reach the FP64 fixed point."""
+    import dataclasses
+    s, meta = t2g_setup(nk=8, Tlist_K=(100.0, 300.0), grid_windows=((0.3, 4e-3), (1.2, 16e-3), (3.0, 64e-3)))
+    s = dataclasses.replace(s, ε_f=s.ε_f + 1e-6 * np.cos(np.arange(len(s.ε_f))))
+    o = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    kw = dict(η_min=0.0, η_init=5e-2, maxiter=200, tol=1e-7, ω_acoustic=1e-6, method="auto")
+    dev = epsx.Scgd0Device(s, ctx)
+    ref = dev.loop(o, batched=True, **kw)
+    assert ref["converged"].all() and ref["info"]["fm_method"] == 1
+    r = dev.loop(o, batched=True, precision=prec, **kw)
+    assert r["converged"].all() and r["info"]["fm_method"] == 1 and r["info"]["n_low_precision"] >= 1
+    scale = np.max(np.abs(ref["Σ"]))
+    assert np.max(np.abs(r["Σ"] - ref["Σ"])) <= 50 * kw["tol"] + 1e-9 * scale
+    assert np.max(np.abs(r["μ"] - ref["μ"])) <= 1e-6
