@@ -149,6 +149,20 @@ def main():
                                "avg_launch_s": sr, "algorithmic_bytes": sr_bytes,
                                "achieved_GBs": sr_bytes / sr / 1e9, "frac_hbm": sr_bytes / sr / 1e9 / HBM_PEAK_GBS,
                                "traffic": traffic("star_reduce_kernel", args.nk, nT)}}
+    elif method_used == 1:
+        # the Einstein factorisation: per Fan-Migdal pass H' (nω x nm nf per temperature) is built and
+        # multiplied by the resident g2 on the library's FP64 MFMA GEMM, 2 nω ni nm nf flop per
+        # temperature; the pass time (HIP events) includes the H' builds
+        fm_avg = tsum["fan_migdal"] / max(tsum["fan_migdal_launches"], 1)
+        per_launch = tsum["star_gemm_instances"] / max(tsum["fan_migdal_launches"], 1)
+        flops = 2.0 * nw * ni * nm * nf * per_launch
+        achieved = flops / fm_avg / 1e12
+        peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16": BF16_PEAK_TFLOPS}[args.precision]
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": None, "kernel": "Einstein Fan-Migdal pass: einstein_h_kernel + dgemm_kernel (FP64 MFMA "
+                "16x16x4, H' g2 batched over temperatures)", "algorithmic_flop_per_launch": flops,
+                "avg_launch_s": fm_avg, "temperatures_per_launch": per_launch}
+        extra = {}
     else:
         fm_avg = tsum["fan_migdal"] / max(tsum["fan_migdal_launches"], 1)
         gbs = g2_bytes / fm_avg / 1e9

@@ -1419,7 +1419,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             ctx->n_class[5] += 1;
             ctx->n_class[6] += nA;
         }
-        ctx->n_class[0] += (par->batched || method == EPSX_FM_STAR) ? 1 : nA;
+        if (method == EPSX_FM_EINSTEIN) ctx->n_class[6] += nA;
+        ctx->n_class[0] += (par->batched || method != EPSX_FM_DIRECT) ? 1 : nA;
         ctx->n_class[1] += nA;
         ctx->n_class[2] += 1;
         ctx->n_class[3] += 1;

@@ -49,7 +49,8 @@ const char *epsx_last_error(const epsx_ctx *ctx);
  * After epsx_scgd0_run: [0] Fan-Migdal sum, [1] Kramers-Kronig, [2] spectral DOS and μ,
  * [3] convergence and copies, [4] the coupling reduction of the star method (once per run),
  * [5] the star method's main h·G̃ GEMM kernel (inside [0]; without its split-k sum; count =
- * launches), [6] count only: the temperature instances of those launches, [7] unused.  In the
+ * launches), [6] count only: the temperature instances of those launches (of the Fan-Migdal
+ * passes [0] with the Einstein factorisation), [7] unused.  In the
  * batched loop the second iteration is the one timed.
  * After epsx_ladder_run / epsx_apply_ladder_kernel: [0] the shift-table GEMMs W = J·B̃,
  * [1] whole kernel applications (count = applications), [2] B̃ builds, [3..7] unused. */
