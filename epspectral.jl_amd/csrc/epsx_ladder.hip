@@ -421,6 +421,278 @@ __global__ void shift_add_chunk_kernel(int64_t nw, int64_t ncol, int64_t ns, int
     Cm[q] = sm;
 }
 
+// ---- instances of the batched solve: one (Ω, T) each; per-slot vectors of n complex at stride
+// `vs`; a launch runs the active instances k < nA of `list` (list[k] = slot), with the stacked
+// operands of the shift-table product indexed by k ----
+
+// Jm[(r + nrow k), kk] = Re / Im of the vertex of instance k: the stacked A operand (leading dim M)
+__global__ void jperm_b_kernel(int64_t nw, int64_t ni, int64_t M, const int32_t *__restrict__ list,
+                               const double2 *__restrict__ Jb, int64_t vs, double *__restrict__ Jm)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t nrow = 16 * nw, k = blockIdx.y;
+    if (q >= nrow * 3 * ni) return;
+    const double2 *J = Jb + vs * list[k];
+    const int64_t r = q % nrow, kk = q / nrow;
+    const int64_t j = r % nw, c12 = (r / nw) % 4, c0 = (r / (4 * nw)) % 2, ri = r / (8 * nw);
+    const double2 z = J[JIX(j, c12, kk % 3, c0, kk / 3)];
+    Jm[(r + nrow * k) + M * kk] = ri == 0 ? z.x : z.y;
+}
+
+// the stacked vertex as the transposed low-precision operand: JmL[kk + kp (r + nrow k)]
+template <typename E>
+__global__ __launch_bounds__(256) void jperm_lp_b_kernel(int64_t nw, int64_t ni, int64_t kp, const int32_t *__restrict__ list,
+                                                         const double2 *__restrict__ Jb, int64_t vs, E *__restrict__ JmL)
+{
+    __shared__ double tile[64][65];
+    const int64_t nrow = 16 * nw, K = 3 * ni, k = blockIdx.z;
+    const double2 *J = Jb + vs * list[k];
+    const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+    const int lx = threadIdx.x & 63;
+    for (int sk = threadIdx.x >> 6; sk < 64; sk += 4) {
+        const int64_t r = r0 + lx, kk = k0 + sk;
+        double v = 0.0;
+        if (r < nrow && kk < K) {
+            const int64_t j = r % nw, c12 = (r / nw) % 4, c0 = (r / (4 * nw)) % 2, ri = r / (8 * nw);
+            const double2 z = J[JIX(j, c12, kk % 3, c0, kk / 3)];
+            v = ri == 0 ? z.x : z.y;
+        }
+        tile[sk][lx] = v;
+    }
+    __syncthreads();
+    for (int sr = threadIdx.x >> 6; sr < 64; sr += 4) {
+        const int64_t r = r0 + sr, kk = k0 + lx;
+        if (r < nrow && kk < kp) {
+            const double v = tile[lx][sr];
+            E *out = JmL + kp * (r + nrow * k);
+            if constexpr (sizeof(E) == 2) out[kk] = to_bf16(v);
+            else out[kk] = (float)v;
+        }
+    }
+}
+
+// the two-tap shift-add of instance k = blockIdx.y from its rows of the stacked W (leading dim M)
+template <typename TW>
+__global__ void shift_add_b_kernel(int64_t nw, int64_t ncol, int64_t ns, int64_t m0, int64_t mb, int64_t M,
+                                   const TW *__restrict__ W, const int32_t *__restrict__ klo,
+                                   const double *__restrict__ wlo, const double *__restrict__ whi,
+                                   double *__restrict__ Cp, double *__restrict__ Cm, int accumulate)
+{
+    const int64_t nrow = 16 * nw, k = blockIdx.y;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nrow * ncol) return;
+    const int64_t r = q % nrow, col = q / nrow;
+    const int64_t j = r % nw, qb = r - j + nrow * k;
+    const int64_t o = q + nrow * ncol * k;
+    double sp = accumulate ? Cp[o] : 0.0, sm = accumulate ? Cm[o] : 0.0;
+#pragma unroll 4
+    for (int64_t mm = 0; mm < mb; mm++) {
+        const int64_t m = m0 + mm;
+        const TW *Wm = W + M * (col + ncol * mm) + qb;
+        const int64_t t0 = j + nw * m, t1 = j + nw * (m + ns);
+        const int64_t k0 = klo[t0] - 1, k1 = klo[t1] - 1;
+        const double lo0 = wlo[t0], hi0 = whi[t0], lo1 = wlo[t1], hi1 = whi[t1];
+        const double a0 = (double)Wm[k0], b0 = (double)Wm[k0 + 1], a1 = (double)Wm[k1], b1 = (double)Wm[k1 + 1];
+        sp += lo0 * a0 + hi0 * b0;
+        sm += lo1 * a1 + hi1 * b1;
+    }
+    Cp[o] = sp;
+    Cm[o] = sm;
+}
+
+// Y of instance k = blockIdx.y: (U+ - U-)/2 from the type-1 columns of its C±
+__global__ void exch_Y_b_kernel(int64_t ny, int64_t ncs, const double *__restrict__ Cp, const double *__restrict__ Cm,
+                                double *__restrict__ Y)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (q < ny) Y[q + ny * k] = (Cp[q + ncs * k] - Cm[q + ncs * k]) / 2;
+}
+
+// per slot z = list[k]: Y_z = A_z + sb B_z (n complex per slot, each array at its own slot stride)
+__global__ void combine2_b_kernel(int64_t n, const int32_t *__restrict__ list, const double2 *__restrict__ A, int64_t as,
+                                  double sb, const double2 *__restrict__ B, int64_t bs, double2 *__restrict__ Y, int64_t ys)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t z = list[blockIdx.y];
+    const double2 a = A[as * z + q], b = B[bs * z + q];
+    Y[ys * z + q] = make_double2(a.x + sb * b.x, a.y + sb * b.y);
+}
+
+// conj(x_z) y_z partial sums per (block, k), then per k into out[z * ostride + oidx]
+__global__ __launch_bounds__(256) void cdot_b_partial_kernel(int64_t n, const int32_t *__restrict__ list, const double2 *__restrict__ X,
+                                                             int64_t xs, const double2 *__restrict__ Yv, int64_t ys,
+                                                             double2 *__restrict__ part)
+{
+    __shared__ double rr[256], ri[256];
+    const int64_t z = list[blockIdx.y];
+    const double2 *x = X + xs * z, *y = Yv + ys * z;
+    double sr = 0, si = 0;
+    for (int64_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (int64_t)gridDim.x * 256) {
+        const double2 a = x[q], b = y[q];
+        sr = fma(a.x, b.x, fma(a.y, b.y, sr));
+        si = fma(a.x, b.y, fma(-a.y, b.x, si));
+    }
+    rr[threadIdx.x] = sr;
+    ri[threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            rr[threadIdx.x] += rr[threadIdx.x + s];
+            ri[threadIdx.x] += ri[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x + (int64_t)gridDim.x * blockIdx.y] = make_double2(rr[0], ri[0]);
+}
+
+__global__ __launch_bounds__(256) void cdot_b_final_kernel(int nb, const int32_t *__restrict__ list,
+                                                           const double2 *__restrict__ part, double2 *__restrict__ out,
+                                                           int64_t ostride, int64_t oidx)
+{
+    __shared__ double rr[256], ri[256];
+    const int k = blockIdx.x;
+    double sr = 0, si = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        sr += part[b + (int64_t)nb * k].x;
+        si += part[b + (int64_t)nb * k].y;
+    }
+    rr[threadIdx.x] = sr;
+    ri[threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            rr[threadIdx.x] += rr[threadIdx.x + s];
+            ri[threadIdx.x] += ri[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[ostride * list[k] + oidx] = make_double2(rr[0], ri[0]);
+}
+
+// y_z -= c_z x_z with c_z = coef[z * cs + ci] (an Arnoldi coefficient just computed)
+__global__ void caxpy_b_kernel(int64_t n, const int32_t *__restrict__ list, const double2 *__restrict__ coef, int64_t cs,
+                               int64_t ci, const double2 *__restrict__ X, int64_t xs, double2 *__restrict__ Y, int64_t ys)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t z = list[blockIdx.y];
+    const double2 a = coef[cs * z + ci], b = X[xs * z + q];
+    double2 &y = Y[ys * z + q];
+    y = make_double2(y.x - (a.x * b.x - a.y * b.y), y.y - (a.x * b.y + a.y * b.x));
+}
+
+// y_z = s_z x_z with s_z = scale[z]
+__global__ void cscale_b_kernel(int64_t n, const int32_t *__restrict__ list, const double *__restrict__ scale,
+                                const double2 *__restrict__ X, int64_t xs, double2 *__restrict__ Y, int64_t ys)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t z = list[blockIdx.y];
+    const double a = scale[z];
+    const double2 b = X[xs * z + q];
+    Y[ys * z + q] = make_double2(a * b.x, a * b.y);
+}
+
+__device__ inline double2 cdiv(double2 a, double2 b)
+{
+    // a / b with Smith's scaling
+    if (fabs(b.y) <= fabs(b.x)) {
+        const double r = b.y / b.x, d = b.x + b.y * r;
+        return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+    }
+    const double r = b.x / b.y, d = b.y + b.x * r;
+    return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+}
+__device__ inline double cabs2(double2 a) { return hypot(a.x, a.y); }
+__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ inline double2 cscl(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// The Givens step of GMRES for instance k (Krylov.jl's MGS Arnoldi with sym_givens, no restart):
+// the new Hessenberg column h_z[0..j] and ||q||² in h_z[j+1].x rotate into R's column j, the
+// residual norm |z_{j+1}| is appended to norms, and the status says whether the instance is solved
+// (norm <= atol) or broke down (h_{j+1,j} <= btol).  inv_h = 1 / h_{j+1,j} scales the next basis
+// vector.  State per slot: R [(cap) x (cap)], cs, sn, zv (cap + 1), at stride gst.
+struct GivensState {
+    double2 *R, *sn, *zv;
+    double *cs, *atol, *inv_h, *norms;   // norms [(maxiter + 1) per slot]
+    const double2 *h;                    // [cap + 1] per slot
+    int32_t *status;                     // per slot: 0 running, 1 solved, 2 breakdown
+    int64_t cap, maxiter;
+    double btol;
+};
+__global__ void givens_b_kernel(int nA, const int32_t *__restrict__ list, GivensState g, int64_t j)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nA) return;
+    const int64_t z = list[k], cap = g.cap;
+    double2 *R = g.R + cap * cap * z, *sn = g.sn + cap * z, *zv = g.zv + (cap + 1) * z;
+    double *cs = g.cs + cap * z;
+    const double2 *h = g.h + (cap + 1) * z;
+    for (int64_t i = 0; i <= j; i++) R[i + cap * j] = h[i];
+    const double Hbis = sqrt(h[j + 1].x);
+    for (int64_t i = 0; i < j; i++) {
+        const double2 a = R[i + cap * j], b = R[i + 1 + cap * j];
+        R[i + cap * j] = cadd(cscl(cs[i], a), cmul(sn[i], b));
+        R[i + 1 + cap * j] = csub(cmul(cconj(sn[i]), a), cscl(cs[i], b));
+    }
+    // sym_givens(R[j, j], Hbis)
+    const double2 a = R[j + cap * j];
+    const double aa = cabs2(a);
+    double c;
+    double2 sv, rho;
+    if (aa == 0) { c = 0; sv = make_double2(1, 0); rho = make_double2(Hbis, 0); }
+    else if (Hbis == 0) { c = 1; sv = make_double2(0, 0); rho = a; }
+    else {
+        const double nrm = hypot(aa, Hbis);
+        c = aa / nrm;
+        const double2 ph = make_double2(a.x / aa, a.y / aa);
+        sv = cscl(Hbis / nrm, ph);
+        rho = cscl(nrm, ph);
+    }
+    cs[j] = c;
+    sn[j] = sv;
+    R[j + cap * j] = rho;
+    zv[j + 1] = cmul(cconj(sv), zv[j]);
+    zv[j] = cscl(c, zv[j]);
+    const double rn = cabs2(zv[j + 1]);
+    g.norms[(g.maxiter + 1) * z + j + 1] = rn;
+    const bool solved = rn <= g.atol[z], breakdown = Hbis <= g.btol;
+    g.status[z] = solved ? 1 : (breakdown ? 2 : 0);
+    g.inv_h[z] = 1.0 / Hbis;
+}
+
+// y = R \ z over the instance's niter columns, then x_z += Σ_i y_i V_z,i (one thread per element)
+__global__ void gmres_solve_b_kernel(int nA, const int32_t *__restrict__ list, GivensState g,
+                                     const int64_t *__restrict__ niter, double2 *__restrict__ y)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nA) return;
+    const int64_t z = list[k], cap = g.cap, m = niter[z];
+    const double2 *R = g.R + cap * cap * z, *zv = g.zv + (cap + 1) * z;
+    double2 *yz = y + cap * z;
+    for (int64_t i = m - 1; i >= 0; i--) {
+        double2 acc = zv[i];
+        for (int64_t q = i + 1; q < m; q++) acc = csub(acc, cmul(R[i + cap * q], yz[q]));
+        yz[i] = cdiv(acc, R[i + cap * i]);
+    }
+}
+__global__ void gmres_update_b_kernel(int64_t n, const int32_t *__restrict__ list, const int64_t *__restrict__ niter,
+                                      const double2 *__restrict__ y, int64_t cap, const double2 *__restrict__ V, int64_t vstride,
+                                      double2 *__restrict__ X, int64_t xs)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t z = list[blockIdx.y], m = niter[z];
+    double2 x = X[xs * z + q];
+    for (int64_t i = 0; i < m; i++) {
+        const double2 a = y[cap * z + i], b = V[vstride * z + n * i + q];
+        x = make_double2(x.x + (a.x * b.x - a.y * b.y), x.y + (a.x * b.y + a.y * b.x));
+    }
+    X[xs * z + q] = x;
+}
+
 }  // namespace
 
 static int ldot(epsx_ctx *ctx, int64_t n, const double2 *x, const double2 *y, double2 *d_out)
@@ -605,6 +877,147 @@ static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int pr
     return t_apply.stop(1);
 }
 
+// Work buffers of the batched application, sized for nb instances.
+struct BatchWork {
+    Tmp Jm, JmL, W, Cp, Cm, Y, KY;
+    int64_t nb = 0, w_bytes = 0;
+};
+
+static int batch_work(epsx_ctx *ctx, BatchWork &bw, int64_t nb, int precision)
+{
+    Scgd0Dev &s = ctx->s;
+    const int64_t nw = s.nw, ni = s.ni, nrow = 16 * nw, K = 3 * ni, kp = (K + 7) / 8 * 8;
+    bw.nb = nb;
+    EPSX_TRY(tmp_alloc(ctx, bw.Cp.b, 8 * (size_t)nrow * 6 * ni * nb));
+    EPSX_TRY(tmp_alloc(ctx, bw.Cm.b, 8 * (size_t)nrow * 6 * ni * nb));
+    EPSX_TRY(tmp_alloc(ctx, bw.Y.b, 8 * (size_t)nrow * 3 * ni * nb));
+    EPSX_TRY(tmp_alloc(ctx, bw.KY.b, 8 * (size_t)nrow * 3 * ni * nb));
+    if (precision == EPSX_PREC_FP64) EPSX_TRY(tmp_alloc(ctx, bw.Jm.b, 8 * (size_t)nrow * K * nb));
+    else EPSX_TRY(tmp_alloc(ctx, bw.JmL.b, (precision == EPSX_PREC_BF16 ? 2 : 4) * (size_t)kp * nrow * nb));
+    // W for a chunk of shifts of all instances: at most 8 GiB, at least one shift
+    const size_t per_m = (precision == EPSX_PREC_FP64 ? 8 : 4) * (size_t)nrow * nb * 6 * ni;
+    bw.w_bytes = (int64_t)(per_m > (size_t(8) << 30) ? per_m : (size_t(8) << 30) / per_m * per_m);
+    if ((size_t)bw.w_bytes > per_m * (size_t)ctx->l.ns) bw.w_bytes = (int64_t)(per_m * ctx->l.ns);
+    EPSX_TRY(tmp_alloc(ctx, bw.W.b, (size_t)bw.w_bytes));
+    return EPSX_OK;
+}
+
+// LJ_z = L(V_z) for the active instances k < nA (slot list[k]) at the temperature whose B̃ is loaded:
+// the vertices stacked into one shift-table product W = [J_1; ...; J_nA] B̃ (B̃ read once for all
+// instances), then per instance the shift-add, the exchange and the propagators Gm_z, Gp_z.
+static int ladder_apply_batch(epsx_ctx *ctx, BatchWork &bw, int nA, const std::vector<int32_t> &list,
+                              const int32_t *d_list, const double2 *V, int64_t vs, double2 *LJ, int64_t ljs,
+                              const double2 *Gm, const double2 *Gp, int precision)
+{
+    Scgd0Dev &s = ctx->s;
+    LadderDev &l = ctx->l;
+    const int64_t nw = s.nw, ni = s.ni, nrow = 16 * nw, ncol = 6 * ni, K = 3 * ni, M = nrow * nA;
+    const hipStream_t st = ctx->stream;
+    ClassTimer t_apply(ctx, 1);
+    const bool bf = precision == EPSX_PREC_BF16, native = precision != EPSX_PREC_FP64;
+    const int64_t kp = (K + 7) / 8 * 8, esz = bf ? 2 : 4;
+    if (native) {
+        if (l.btl_prec != precision) {
+            const int64_t ncols = ncol * l.ns;
+            EPSX_TRY(dev_alloc(ctx, l.Btl, esz * (size_t)kp * ncols));
+            const dim3 grid((unsigned)nb(kp * ncols));
+            if (bf)
+                hipLaunchKernelGGL(bt_lp_kernel<uint16_t>, grid, dim3(256), 0, st, K, kp, ncols, dptr<double>(l.Bt),
+                                   dptr<uint16_t>(l.Btl));
+            else
+                hipLaunchKernelGGL(bt_lp_kernel<float>, grid, dim3(256), 0, st, K, kp, ncols, dptr<double>(l.Bt),
+                                   dptr<float>(l.Btl));
+            EPSX_HIP(ctx, hipGetLastError());
+            l.btl_prec = precision;
+        }
+        const dim3 grid((unsigned)((nrow + 63) / 64), (unsigned)((kp + 63) / 64), (unsigned)nA);
+        if (bf)
+            hipLaunchKernelGGL(jperm_lp_b_kernel<uint16_t>, grid, dim3(256), 0, st, nw, ni, kp, d_list, V, vs,
+                               bw.JmL.ptr<uint16_t>());
+        else
+            hipLaunchKernelGGL(jperm_lp_b_kernel<float>, grid, dim3(256), 0, st, nw, ni, kp, d_list, V, vs,
+                               bw.JmL.ptr<float>());
+    } else {
+        hipLaunchKernelGGL(jperm_b_kernel, dim3(nb(nrow * K), (unsigned)nA), dim3(256), 0, st, nw, ni, M, d_list, V, vs,
+                           bw.Jm.ptr<double>());
+    }
+    EPSX_HIP(ctx, hipGetLastError());
+    const size_t per_m = (native ? 4 : 8) * (size_t)M * ncol;
+    int64_t mb = (int64_t)((size_t)bw.w_bytes / per_m);
+    mb = mb < 1 ? 1 : (mb > l.ns ? l.ns : mb);
+    const bool blas = use_blas();
+    for (int64_t m0 = 0; m0 < l.ns; m0 += mb) {
+        const int64_t mc = m0 + mb <= l.ns ? mb : l.ns - m0;
+        ClassTimer t_gemm(ctx, 0);
+        if (native) {
+            const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
+            bool lt = false;
+            if (blas && bf)
+                EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, M, ncol * mc, kp, bw.JmL.b.p, kp, 0, Bl, kp, 0, bw.W.b.p, M,
+                                        0, 1, &lt));
+            if (lt) {
+                EPSX_TRY(blaslt_gemm(ctx, EPSX_PREC_BF16, M, ncol * mc, kp, bw.JmL.b.p, kp, 0, Bl, kp, 0, bw.W.b.p, M, 0, 1));
+            } else if (blas) {
+                rocblas_handle h;
+                EPSX_TRY(blas_handle(ctx, h));
+                const float one = 1.0f, zero = 0.0f;
+                const rocblas_datatype dt = bf ? rocblas_datatype_bf16_r : rocblas_datatype_f32_r;
+                EPSX_TRY(blas_check(ctx, rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, (int)M,
+                                                         (int)(ncol * mc), (int)kp, &one, bw.JmL.b.p, dt, (int)kp, Bl, dt,
+                                                         (int)kp, &zero, bw.W.b.p, rocblas_datatype_f32_r, (int)M,
+                                                         bw.W.b.p, rocblas_datatype_f32_r, (int)M, rocblas_datatype_f32_r,
+                                                         rocblas_gemm_algo_standard, 0, 0),
+                                    "rocblas_gemm_ex (batched ladder product)"));
+            } else {
+                GemmBatch bt;
+                bt.Cf = bw.W.ptr<float>();
+                EPSX_TRY(lpgemm_tn(ctx, precision, M, ncol * mc, kp, bw.JmL.b.p, kp, Bl, kp, nullptr, M, false, bt));
+            }
+        } else if (blas) {
+            rocblas_handle h;
+            EPSX_TRY(blas_handle(ctx, h));
+            const double one = 1.0, zero = 0.0;
+            EPSX_TRY(blas_check(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)M, (int)(ncol * mc),
+                                                   (int)K, &one, bw.Jm.ptr<double>(), (int)M,
+                                                   dptr<double>(l.Bt) + K * ncol * m0, (int)K, &zero, bw.W.ptr<double>(),
+                                                   (int)M),
+                                "rocblas_dgemm (batched ladder product)"));
+        } else {
+            GemmBatch bt;
+            EPSX_TRY(dgemm(ctx, M, ncol * mc, K, bw.Jm.ptr<double>(), M, dptr<double>(l.Bt) + K * ncol * m0, K,
+                           bw.W.ptr<double>(), M, false, bt));
+        }
+        EPSX_TRY(t_gemm.stop(1));
+        const dim3 grid(nb(nrow * ncol), (unsigned)nA);
+        if (native)
+            hipLaunchKernelGGL(shift_add_b_kernel<float>, grid, dim3(256), 0, st, nw, ncol, l.ns, m0, mc, M,
+                               bw.W.ptr<float>(), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
+                               bw.Cp.ptr<double>(), bw.Cm.ptr<double>(), m0 > 0 ? 1 : 0);
+        else
+            hipLaunchKernelGGL(shift_add_b_kernel<double>, grid, dim3(256), 0, st, nw, ncol, l.ns, m0, mc, M,
+                               bw.W.ptr<double>(), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
+                               bw.Cp.ptr<double>(), bw.Cm.ptr<double>(), m0 > 0 ? 1 : 0);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
+    // KY = KK (U+ - U-)/2 for all instances in one product, then per instance the channels and
+    // the attached propagators
+    const int64_t ny = nrow * 3 * ni, ncs = nrow * ncol;
+    hipLaunchKernelGGL(exch_Y_b_kernel, dim3(nb(ny), (unsigned)nA), dim3(256), 0, st, ny, ncs, bw.Cp.ptr<double>(),
+                       bw.Cm.ptr<double>(), bw.Y.ptr<double>());
+    EPSX_HIP(ctx, hipGetLastError());
+    EPSX_TRY(dgemm_nn(ctx, nw, ny / nw * nA, nw, dptr<double>(s.KK), nw, bw.Y.ptr<double>(), nw, bw.KY.ptr<double>(), nw,
+                      false));
+    const int64_t ng = nw * 4 * ni;
+    for (int k = 0; k < nA; k++) {
+        const int64_t z = list[k];
+        hipLaunchKernelGGL(exch_attach_kernel, dim3(nb(nw * 6 * ni)), dim3(256), 0, st, nw, ni, bw.Cp.ptr<double>() + ncs * k,
+                           bw.Cm.ptr<double>() + ncs * k, bw.KY.ptr<double>() + ny * k, Gm + ng * z, Gp + ng * z,
+                           LJ + ljs * z);
+    }
+    EPSX_HIP(ctx, hipGetLastError());
+    return t_apply.stop(1);
+}
+
 static int ladder_work(epsx_ctx *ctx)
 {
     Scgd0Dev &s = ctx->s;
@@ -753,6 +1166,196 @@ void sym_givens(std::complex<double> a, double b, double &c, std::complex<double
 
 }  // namespace
 
+// The GMRES solves of the probe frequencies of each temperature as instances of one batched solve:
+// every instance starts from J = 0 (no warm start from the previous Ω, which would chain the
+// instances), the Arnoldi steps of all active instances run together (one stacked shift-table
+// product per step, batched dots and updates, the Givens rotations on the device), and an instance
+// drops out of the launches once it is solved or breaks down.  The host reads one status word per
+// instance per step.  Instances per batch: as many as fit a 96 GiB working set.
+static int ladder_run_batched(epsx_ctx *ctx, const epsx_ladder_params *par, epsx_ladder_outputs *out)
+{
+    LadderDev &l = ctx->l;
+    Scgd0Dev &s = ctx->s;
+    const int64_t nw = s.nw, ni = s.ni, n = nw * 24 * ni, nT = par->nT, nO = par->nOmega, it_max = par->maxiter;
+    const int64_t ng = nw * 4 * ni, cap = it_max + 1, nrow = 16 * nw;
+    const hipStream_t st = ctx->stream;
+    // memory per instance: J0, x, q, LJ, the basis, the propagators, C±, Y, KY and the stacked
+    // operand of the product
+    const size_t per_inst = 16 * (size_t)n * (4 + cap) + 32 * (size_t)ng + 8 * (size_t)nrow * ni * (12 + 3 + 3 + 3);
+    int64_t nbat = (int64_t)((size_t(96) << 30) / per_inst);
+    nbat = nbat < 1 ? 1 : (nbat > nO ? nO : nbat);
+    Tmp J0, X, Q, LJ, V, Gm, Gp, d_list, h, R, sn, zv, cs, atol, invh, norms, status, niter_d, y, part;
+    EPSX_TRY(tmp_alloc(ctx, J0.b, 16 * (size_t)n * nbat));
+    EPSX_TRY(tmp_alloc(ctx, X.b, 16 * (size_t)n * nbat));
+    EPSX_TRY(tmp_alloc(ctx, Q.b, 16 * (size_t)n * nbat));
+    EPSX_TRY(tmp_alloc(ctx, LJ.b, 16 * (size_t)n * nbat));
+    EPSX_TRY(tmp_alloc(ctx, V.b, 16 * (size_t)n * cap * nbat));
+    EPSX_TRY(tmp_alloc(ctx, Gm.b, 16 * (size_t)ng * nbat));
+    EPSX_TRY(tmp_alloc(ctx, Gp.b, 16 * (size_t)ng * nbat));
+    EPSX_TRY(tmp_alloc(ctx, d_list.b, 4 * (size_t)nbat));
+    EPSX_TRY(tmp_alloc(ctx, h.b, 16 * (size_t)(cap + 1) * nbat));
+    EPSX_TRY(tmp_alloc(ctx, R.b, 16 * (size_t)cap * cap * nbat));
+    EPSX_TRY(tmp_alloc(ctx, sn.b, 16 * (size_t)cap * nbat));
+    EPSX_TRY(tmp_alloc(ctx, zv.b, 16 * (size_t)(cap + 1) * nbat));
+    EPSX_TRY(tmp_alloc(ctx, cs.b, 8 * (size_t)cap * nbat));
+    EPSX_TRY(tmp_alloc(ctx, atol.b, 8 * (size_t)nbat));
+    EPSX_TRY(tmp_alloc(ctx, invh.b, 8 * (size_t)nbat));
+    EPSX_TRY(tmp_alloc(ctx, norms.b, 8 * (size_t)cap * nbat));
+    EPSX_TRY(tmp_alloc(ctx, status.b, 4 * (size_t)nbat));
+    EPSX_TRY(tmp_alloc(ctx, niter_d.b, 8 * (size_t)nbat));
+    EPSX_TRY(tmp_alloc(ctx, y.b, 16 * (size_t)cap * nbat));
+    EPSX_TRY(tmp_alloc(ctx, part.b, 16 * (size_t)RED_BLOCKS * nbat));
+    BatchWork bw;
+    EPSX_TRY(batch_work(ctx, bw, nbat, par->precision));
+    double2 *dJ0 = J0.ptr<double2>(), *dX = X.ptr<double2>(), *dQ = Q.ptr<double2>(), *dLJ = LJ.ptr<double2>();
+    double2 *dV = V.ptr<double2>(), *dh = h.ptr<double2>();
+    const int32_t *dl = d_list.ptr<int32_t>();
+    const int64_t vst = n * cap;   // basis stride per slot
+    GivensState gs;
+    gs.R = R.ptr<double2>(); gs.sn = sn.ptr<double2>(); gs.zv = zv.ptr<double2>(); gs.cs = cs.ptr<double>();
+    gs.atol = atol.ptr<double>(); gs.inv_h = invh.ptr<double>(); gs.norms = norms.ptr<double>(); gs.h = dh;
+    gs.status = status.ptr<int32_t>(); gs.cap = cap; gs.maxiter = it_max;
+    gs.btol = std::pow(2.220446049250313e-16, 0.75);
+    std::vector<int32_t> jv(nw);
+    std::vector<double> tv(nw), xs(nw);
+    auto dots = [&](int nA, const double2 *Xv, int64_t xstr, const double2 *Yv, int64_t ystr, int64_t oidx) -> int {
+        hipLaunchKernelGGL(cdot_b_partial_kernel, dim3(RED_BLOCKS, (unsigned)nA), dim3(256), 0, st, n, dl, Xv, xstr, Yv,
+                           ystr, part.ptr<double2>());
+        hipLaunchKernelGGL(cdot_b_final_kernel, dim3((unsigned)nA), dim3(256), 0, st, RED_BLOCKS, dl, part.ptr<double2>(),
+                           dh, cap + 1, oidx);
+        EPSX_HIP(ctx, hipGetLastError());
+        return EPSX_OK;
+    };
+    for (int64_t iT = 0; iT < nT; iT++) {
+        const double T = par->Tlist[iT], mu = par->mu[iT];
+        if (!(T >= 0)) return set_error(ctx, EPSX_ERR_ARG, "Temperature must be positive");
+        EPSX_TRY(ensure_Bt(ctx, T, par->w_acoustic));
+        EPSX_HIP(ctx, hipMemcpyAsync(l.S.p, par->Sigma + 2 * nw * ni * iT, 16 * nw * ni, hipMemcpyHostToDevice, st));
+        for (int64_t o0 = 0; o0 < nO; o0 += nbat) {
+            const int nb_ = (int)(o0 + nbat <= nO ? nbat : nO - o0);
+            // per instance: the propagators, the bare vertex and σ_bubble
+            for (int z = 0; z < nb_; z++) {
+                const double Om = par->Omegas[o0 + z];
+                for (int64_t w = 0; w < nw; w++) xs[w] = s.ws_host[w] - Om / 2;
+                linear_interpolation_table(nw, s.ws_host.data(), nw, xs.data(), jv.data(), tv.data());
+                EPSX_HIP(ctx, hipMemcpyAsync(l.jm.p, jv.data(), 4 * nw, hipMemcpyHostToDevice, st));
+                EPSX_HIP(ctx, hipMemcpyAsync(l.tm.p, tv.data(), 8 * nw, hipMemcpyHostToDevice, st));
+                for (int64_t w = 0; w < nw; w++) xs[w] = s.ws_host[w] + Om / 2;
+                linear_interpolation_table(nw, s.ws_host.data(), nw, xs.data(), jv.data(), tv.data());
+                EPSX_HIP(ctx, hipMemcpyAsync(l.jp.p, jv.data(), 4 * nw, hipMemcpyHostToDevice, st));
+                EPSX_HIP(ctx, hipMemcpyAsync(l.tp.p, tv.data(), 8 * nw, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(greens_kernel, dim3(nb(nw * ni)), dim3(256), 0, st, nw, ni, dptr<double>(s.ws),
+                                   dptr<double>(s.e_i), dptr<double2>(l.S), dptr<int32_t>(l.jm), dptr<double>(l.tm),
+                                   dptr<int32_t>(l.jp), dptr<double>(l.tp), Om, mu, T, Gm.ptr<double2>() + ng * z,
+                                   Gp.ptr<double2>() + ng * z);
+                hipLaunchKernelGGL(bare_vertex_kernel, dim3(nb(n)), dim3(256), 0, st, nw, ni, Gm.ptr<double2>() + ng * z,
+                                   dptr<double>(l.v), Gp.ptr<double2>() + ng * z, dJ0 + n * z);
+                EPSX_HIP(ctx, hipGetLastError());
+                EPSX_TRY(sigma_of(ctx, dJ0 + n * z, Om, T, out->sigma_bubble + 9 * (o0 + z + nO * iT)));
+                // the host tables are read by the next instance's copies only after this one's
+                // kernels (one stream)
+            }
+            // GMRES from x = 0: r = J0, β = ||J0||, atol = rtol β
+            std::vector<int32_t> list(nb_), st_h(nb_);
+            for (int z = 0; z < nb_; z++) list[z] = z;
+            EPSX_HIP(ctx, hipMemcpyAsync(d_list.b.p, list.data(), 4 * nb_, hipMemcpyHostToDevice, st));
+            EPSX_TRY(dots(nb_, dJ0, n, dJ0, n, 0));
+            std::vector<double2> hb(nb_ * (cap + 1));
+            EPSX_HIP(ctx, hipMemcpyAsync(hb.data(), dh, 16 * (size_t)(cap + 1) * nb_, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipStreamSynchronize(st));
+            std::vector<double> beta(nb_), at(nb_), normsh(cap * nb_, NAN), inv(nb_);
+            std::vector<double2> z0((cap + 1) * nb_, make_double2(0, 0));
+            std::vector<int64_t> niter(nb_, 0);
+            std::vector<int32_t> conv(nb_, 0);
+            std::vector<int32_t> active;
+            for (int z = 0; z < nb_; z++) {
+                beta[z] = std::sqrt(hb[(cap + 1) * z].x);
+                at[z] = par->rtol * beta[z];
+                normsh[cap * z] = beta[z];
+                z0[(cap + 1) * z] = make_double2(beta[z], 0);
+                inv[z] = beta[z] > 0 ? 1.0 / beta[z] : 0.0;
+                conv[z] = beta[z] <= at[z];
+                if (beta[z] > 0 && !conv[z]) active.push_back(z);
+            }
+            EPSX_HIP(ctx, hipMemsetAsync(X.b.p, 0, 16 * (size_t)n * nb_, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(atol.b.p, at.data(), 8 * nb_, hipMemcpyHostToDevice, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(invh.b.p, inv.data(), 8 * nb_, hipMemcpyHostToDevice, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(norms.b.p, normsh.data(), 8 * (size_t)cap * nb_, hipMemcpyHostToDevice, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(zv.b.p, z0.data(), 16 * (size_t)(cap + 1) * nb_, hipMemcpyHostToDevice, st));
+            // V_0 = r / β for every instance (the list still holds all of them)
+            hipLaunchKernelGGL(cscale_b_kernel, dim3(nb(n), (unsigned)nb_), dim3(256), 0, st, n, dl, invh.ptr<double>(),
+                               dJ0, n, dV, vst);
+            EPSX_HIP(ctx, hipGetLastError());
+            for (int64_t j = 0; j < it_max && !active.empty(); j++) {
+                const int nA = (int)active.size();
+                EPSX_HIP(ctx, hipMemcpyAsync(d_list.b.p, active.data(), 4 * nA, hipMemcpyHostToDevice, st));
+                // q = V_j - L(V_j)
+                EPSX_TRY(ladder_apply_batch(ctx, bw, nA, active, dl, dV + n * j, vst, dLJ, n, Gm.ptr<double2>(),
+                                            Gp.ptr<double2>(), par->precision));
+                hipLaunchKernelGGL(combine2_b_kernel, dim3(nb(n), (unsigned)nA), dim3(256), 0, st, n, dl, dV + n * j, vst,
+                                   -1.0, dLJ, n, dQ, n);
+                EPSX_HIP(ctx, hipGetLastError());
+                for (int64_t i = 0; i <= j; i++) {
+                    EPSX_TRY(dots(nA, dV + n * i, vst, dQ, n, i));
+                    hipLaunchKernelGGL(caxpy_b_kernel, dim3(nb(n), (unsigned)nA), dim3(256), 0, st, n, dl, dh, cap + 1, i,
+                                       dV + n * i, vst, dQ, n);
+                }
+                EPSX_TRY(dots(nA, dQ, n, dQ, n, j + 1));
+                hipLaunchKernelGGL(givens_b_kernel, dim3((unsigned)((nA + 63) / 64)), dim3(64), 0, st, nA, dl, gs, j);
+                EPSX_HIP(ctx, hipGetLastError());
+                EPSX_HIP(ctx, hipMemcpyAsync(st_h.data(), status.b.p, 4 * nb_, hipMemcpyDeviceToHost, st));
+                EPSX_HIP(ctx, hipStreamSynchronize(st));
+                std::vector<int32_t> next;
+                for (int32_t z : active) {
+                    niter[z] = j + 1;
+                    if (st_h[z] == 1) conv[z] = 1;
+                    else if (st_h[z] == 0 && j + 1 < it_max) next.push_back(z);
+                }
+                // V_{j+1} = q / h_{j+1,j} for the instances that go on
+                if (!next.empty()) {
+                    EPSX_HIP(ctx, hipMemcpyAsync(d_list.b.p, next.data(), 4 * next.size(), hipMemcpyHostToDevice, st));
+                    hipLaunchKernelGGL(cscale_b_kernel, dim3(nb(n), (unsigned)next.size()), dim3(256), 0, st, n, dl,
+                                       invh.ptr<double>(), dQ, n, dV + n * (j + 1), vst);
+                    EPSX_HIP(ctx, hipGetLastError());
+                }
+                active.swap(next);
+            }
+            // x = V (R \ z) for every instance that ran
+            std::vector<int32_t> ran;
+            for (int z = 0; z < nb_; z++)
+                if (niter[z] > 0) ran.push_back(z);
+            if (!ran.empty()) {
+                EPSX_HIP(ctx, hipMemcpyAsync(niter_d.b.p, niter.data(), 8 * nb_, hipMemcpyHostToDevice, st));
+                EPSX_HIP(ctx, hipMemcpyAsync(d_list.b.p, ran.data(), 4 * ran.size(), hipMemcpyHostToDevice, st));
+                const int nr = (int)ran.size();
+                hipLaunchKernelGGL(gmres_solve_b_kernel, dim3((unsigned)((nr + 63) / 64)), dim3(64), 0, st, nr, dl, gs,
+                                   niter_d.ptr<int64_t>(), y.ptr<double2>());
+                hipLaunchKernelGGL(gmres_update_b_kernel, dim3(nb(n), (unsigned)nr), dim3(256), 0, st, n, dl,
+                                   niter_d.ptr<int64_t>(), y.ptr<double2>(), cap, dV, vst, dX, n);
+                EPSX_HIP(ctx, hipGetLastError());
+            }
+            EPSX_HIP(ctx, hipMemcpyAsync(normsh.data(), norms.b.p, 8 * (size_t)cap * nb_, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipStreamSynchronize(st));
+            for (int z = 0; z < nb_; z++) {
+                const int64_t slot = o0 + z + nO * iT;
+                const double Om = par->Omegas[o0 + z];
+                out->converged[slot] = conv[z];
+                out->niter[slot] = niter[z];
+                double *nm_out = out->norms + (it_max + 1) * slot;
+                for (int64_t q = 0; q <= niter[z]; q++) nm_out[q] = normsh[cap * z + q];
+                EPSX_TRY(sigma_of(ctx, dX + n * z, Om, T, out->sigma_ladder + 9 * slot));
+                if (par->store_vertex && out->J)
+                    EPSX_TRY(read_back(ctx, out->J + 2 * n * slot, dX + n * z, 16 * (size_t)n));
+                if (par->verbosity > 0)
+                    fprintf(stderr, "ladder (batched) T = %g, Ω = %g: niter %lld converged %d\n", T, Om,
+                            (long long)niter[z], conv[z]);
+            }
+        }
+    }
+    EPSX_HIP(ctx, hipStreamSynchronize(st));
+    return EPSX_OK;
+}
+
 extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, epsx_ladder_outputs *out)
 {
     using cd = std::complex<double>;
@@ -785,6 +1388,10 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
     EPSX_TRY(dev_alloc(ctx, l.jp, 4 * nw));
     EPSX_TRY(dev_alloc(ctx, l.tp, 8 * nw));
     const int gm = par->solver == EPSX_SOLVER_GMRES;
+    for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
+    for (int64_t q = 0; q < (it_max + 1) * nO * nT; q++) out->norms[q] = NAN;
+    for (int64_t q = 0; q < 9 * it_max * nO * nT; q++) out->sigma_iter[q] = NAN;
+    if (gm && par->batched) return ladder_run_batched(ctx, par, out);
     Tmp J0, J, LJ, R, V, hcol;
     EPSX_TRY(tmp_alloc(ctx, J0.b, 16 * n));
     EPSX_TRY(tmp_alloc(ctx, J.b, 16 * n));
@@ -793,14 +1400,11 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
     if (gm) EPSX_TRY(tmp_alloc(ctx, V.b, 16 * (size_t)n * (it_max + 1)));
     EPSX_TRY(tmp_alloc(ctx, hcol.b, 16 * (size_t)(it_max + 2)));
     const double btol = std::pow(2.220446049250313e-16, 0.75);
-    for (int64_t q = 0; q < (it_max + 1) * nO * nT; q++) out->norms[q] = NAN;
-    for (int64_t q = 0; q < 9 * it_max * nO * nT; q++) out->sigma_iter[q] = NAN;
     std::vector<int32_t> jv(nw);
     std::vector<double> tv(nw), xs(nw);
     double2 *dJ0 = J0.ptr<double2>(), *dJ = J.ptr<double2>(), *dLJ = LJ.ptr<double2>(), *dR = R.ptr<double2>();
     double2 *dh = hcol.ptr<double2>();
     auto launch_n = [&](int64_t m) { return dim3(nb(m)); };
-    for (int k = 0; k < 8; k++) { ctx->t_class[k] = 0; ctx->n_class[k] = 0; }
     for (int64_t iT = 0; iT < nT; iT++) {
         const double T = par->Tlist[iT], mu = par->mu[iT];
         if (!(T >= 0)) return set_error(ctx, EPSX_ERR_ARG, "Temperature must be positive");

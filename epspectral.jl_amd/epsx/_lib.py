@@ -138,7 +138,7 @@ class LadderParams(C.Structure):
         ("struct_size", I64), ("nT", I64), ("nOmega", I64),
         ("Tlist", P), ("mu", P), ("Sigma", P), ("Omegas", P),
         ("solver", I32), ("precision", I32), ("rtol", D), ("maxiter", I64), ("w_acoustic", D),
-        ("store_vertex", I32), ("verbosity", I32),
+        ("store_vertex", I32), ("verbosity", I32), ("batched", I32), ("pad", I32),
     ]
 
 

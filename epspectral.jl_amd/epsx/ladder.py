@@ -101,8 +101,10 @@ class LadderDevice:
         return LJ
 
     def loop(self, Σ, μ, Tlist, Ωs, *, solver="gmres", rtol=1e-2, maxiter=30, store_vertex=False,
-             ω_acoustic=None, precision="fp64", verbosity=0):
-        """`_loop_ladder` (src/ladder.jl:303-379): unsymmetrised σ without spin / volume."""
+             ω_acoustic=None, precision="fp64", verbosity=0, batched=False):
+        """`_loop_ladder` (src/ladder.jl:303-379): unsymmetrised σ without spin / volume.  With
+        `batched` (GMRES only) the Ωs of each temperature are solved together as instances of one
+        batched GMRES, each from J = 0 instead of the previous Ω's solution."""
         import ctypes
         import warnings
         from . import _lib
@@ -125,7 +127,8 @@ class LadderDevice:
         J = np.empty((nw, 4, 3, 2, ni, nO, nT), dtype=np.complex128, order="F") if store_vertex else None
         par = _lib.LadderParams(ctypes.sizeof(_lib.LadderParams), nT, nO, _lib.ptr(Tl), _lib.ptr(mu), _lib.ptr(Σ),
                                 _lib.ptr(Om), 0 if solver == "gmres" else 1, _lib.PRECISIONS[precision], float(rtol),
-                                int(maxiter), float(ω_acoustic), int(store_vertex), int(verbosity))
+                                int(maxiter), float(ω_acoustic), int(store_vertex), int(verbosity), int(bool(batched)),
+                                0)
         out = _lib.LadderOutputs(ctypes.sizeof(_lib.LadderOutputs), _lib.ptr(sb), _lib.ptr(sl), _lib.ptr(conv),
                                  _lib.ptr(nit), _lib.ptr(norms), _lib.ptr(sit), _lib.ptr(J))
         self.ctx.check(_lib.lib().epsx_ladder_run(self.ctx.handle, ctypes.addressof(par), ctypes.addressof(out)))

@@ -119,6 +119,7 @@ struct LadderParams
     maxiter::Int64
     w_acoustic::Float64
     store_vertex::Int32; verbosity::Int32
+    batched::Int32; pad::Int32
 end
 
 struct LadderOutputs
@@ -285,7 +286,7 @@ end
 `gpu_tile` only reorders sums and is not needed on this backend.
 """
 function loop_ladder(b::MI355XBackend, ladder, Σ, μ, Tlist, Ωs; solver, rtol, maxiter, store_vertex,
-        ω_acoustic, verbosity = 0, gpu_tile = nothing, precision = :fp64)
+        ω_acoustic, verbosity = 0, gpu_tile = nothing, precision = :fp64, batched = false)
     solver ∈ (:gmres, :fixed_point) || throw(ArgumentError(
         "solver must be :gmres or :fixed_point, got $solver"))
     set_problem!(b, ladder)
@@ -307,7 +308,7 @@ function loop_ladder(b::MI355XBackend, ladder, Σ, μ, Tlist, Ωs; solver, rtol,
     GC.@preserve Σh μh Th Ωh σb σl conv nit norms σit J begin
         par = Ref(LadderParams(sizeof(LadderParams), nT, nΩ, pointer(Th), pointer(μh), pointer(Σh),
             pointer(Ωh), Int32(solver === :gmres ? 0 : 1), PREC[precision], rtol, maxiter, ω_acoustic,
-            Int32(store_vertex), Int32(verbosity)))
+            Int32(store_vertex), Int32(verbosity), Int32(batched), Int32(0)))
         out = Ref(LadderOutputs(sizeof(LadderOutputs), pointer(σb), pointer(σl), pointer(conv),
             pointer(nit), pointer(norms), pointer(σit),
             J === nothing ? Ptr{ComplexF64}(C_NULL) : pointer(J)))

@@ -233,6 +233,11 @@ typedef struct {
     double w_acoustic;        /* > 0 */
     int32_t store_vertex;
     int32_t verbosity;
+    int32_t batched;          /* GMRES only.  0: the (T, Ω) solves in turn, each warm-started from the
+                                 previous Ω at its T (the reference's loop).  1: the Ωs of each T as
+                                 instances of one batched solve, each from J = 0, one stacked
+                                 shift-table product per Arnoldi step, converged instances dropped */
+    int32_t pad;
 } epsx_ladder_params;
 
 typedef struct {

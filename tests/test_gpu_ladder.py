@@ -227,3 +227,49 @@ def test_c4_t2g16_gmres_solves_the_equation(ctx):
         assert x["converged"].all(), p
         assert np.array_equal(x["σ_bubble"], ref["σ_bubble"]), p
         assert rel(x["σ_ladder"], ref["σ_ladder"]) <= 3e-2, p
+
+
+def test_batched_gmres_against_oracle_cold_start(ctx):
+    """The Ωs of each temperature as instances of one batched GMRES (stacked shift-table product,
+    batched MGS, Givens rotations on the device), every instance from J = 0: against the oracle's
+    GMRES from J = 0 for each (Ω, T) of the disordered case at rtol 1e-8, equal iteration counts and
+    residual histories, σ and J at 1e-10, bitwise on repeat."""
+    s, L, r = disordered_case()
+    ld = device(ctx, s, L)
+    Ωs = [0.0, 0.1, 0.3]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dv = ld.loop(r["Σ"], r["μ"], [0.1, 0.2], Ωs, rtol=1e-8, maxiter=60, store_vertex=True, batched=True)
+        again = ld.loop(r["Σ"], r["μ"], [0.1, 0.2], Ωs, rtol=1e-8, maxiter=60, store_vertex=True, batched=True)
+    for iT, T in enumerate((0.1, 0.2)):
+        for iO, Ω in enumerate(Ωs):
+            o = oracle.ladder_solve(L, r["Σ"][:, :, iT], r["μ"][iT], T, Ω, J=None, rtol=1e-8, maxiter=60)
+            assert dv["niter"][iO, iT] == o["niter"] and dv["converged"][iO, iT] == o["converged"]
+            assert rel(dv["σ_bubble"][:, :, iO, iT], o["σ_bubble"]) <= 1e-12
+            assert rel(dv["σ_ladder"][:, :, iO, iT], o["σ_ladder"]) <= 1e-10
+            assert rel(dv["J"][..., iO, iT], o["J"]) <= 1e-10
+            h = dv["history"][iO][iT]["norms"]
+            assert len(h) == len(o["norms"]) and np.allclose(h, o["norms"], rtol=1e-6, atol=1e-12 * o["norms"][0])
+    assert len(set(dv["niter"].ravel().tolist())) > 1     # the instances stop at different steps
+    assert np.array_equal(again["J"], dv["J"]) and np.array_equal(again["σ_ladder"], dv["σ_ladder"])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16"])
+def test_batched_gmres_t2g_matches_sequential(ctx, prec):
+    """The t2g model with Oh rotations: the batched solve (cold starts) and the reference's sequential
+    loop (warm starts from the previous Ω) reach the same σ_ladder to the solver tolerance; in BF16
+    shift-table products as well."""
+    from test_oracle_ladder import t2g_case
+    s, L, r, meta = t2g_case()
+    ld = device(ctx, s, L)
+    T = meta["Tlist"][:1]
+    Ωs = [0.0, 0.01, 0.02, 0.04]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ref = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], T, Ωs, rtol=1e-8, maxiter=60)
+        b = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], T, Ωs, rtol=1e-3, maxiter=60, batched=True, precision=prec)
+        sq = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], T, Ωs, rtol=1e-3, maxiter=60, precision=prec)
+    assert b["converged"].all() and sq["converged"].all()
+    assert np.array_equal(b["σ_bubble"], ref["σ_bubble"])
+    tol = 3e-2 if prec == "bf16" else 3e-3   # BF16 products: their rounding, not rtol, sets the floor
+    assert rel(b["σ_ladder"], ref["σ_ladder"]) <= tol and rel(sq["σ_ladder"], ref["σ_ladder"]) <= tol
