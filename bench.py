@@ -46,7 +46,7 @@ def main():
     ap.add_argument("--temps", default="100,300", help="temperatures in K")
     ap.add_argument("--method", default="auto")
     ap.add_argument("--precision", default="fp64")
-    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="temperatures (scgd0) or Ωs (ladder) one by one")
     ap.add_argument("--maxiter", type=int, default=100)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--plain-iters", type=int, default=2, help="iterations of the plain method timed")
@@ -60,7 +60,9 @@ def main():
     ap.add_argument("--models", type=int, default=64, help="warmstart: training models (12³)")
     ap.add_argument("--heldout", type=int, default=4, help="warmstart: held-out models (24³)")
     ap.add_argument("--epochs", type=int, default=40, help="warmstart: training epochs")
-    ap.add_argument("--omegas", type=int, default=4, help="ladder: probe frequencies 0, 5, ... meV timed")
+    ap.add_argument("--omegas", type=int, default=0, help="ladder: probe frequencies 0, 5, ... meV (default 41)")
+    ap.add_argument("--coupling", default="iid", choices=("iid", "forward"),
+                    help="ladder: i.i.d. couplings (the C2 model) or weighted by the momentum transfer")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (the HBM counter calibration)")
     args = ap.parse_args()
@@ -235,11 +237,15 @@ def main():
 
 
 def ladder_bench(args):
-    """The Kubo ladder on the converged scGD0 result of the t2g model at 16³ (ni = 331,
-    nf = 8448, nshift = 82): scGD0 at 100, 200, 300, 400 K to 1e-6 eV by the star method, then
-    GMRES at rtol 1e-2 for the first `--omegas` probe frequencies of 0:5:200 meV at 100 K.  The
-    value is seconds per (Ω, T) solve; the roofline line is the shift-table GEMM W = J·B̃
-    (FP64 MFMA), 2·(16 nω)·(3 ni)·(6 ni·nshift) flop per application."""
+    """C4: the Kubo ladder on the converged scGD0 result of the t2g model at 16³ (ni = 331,
+    nf = 8448, nshift = 82, Oh vertex rotations): scGD0 at 100, 200, 300, 400 K to 1e-6 eV by the
+    star method, then GMRES at rtol 1e-2 for all 41 probe frequencies 0:5:200 meV at all four
+    temperatures, the Ωs of each temperature batched as instances of one solve (`--no-batched`: the
+    reference's loop, each Ω warm-started from the previous one).  The value is seconds per (Ω, T)
+    solve over the whole run; the roofline line is the shift-table GEMM W = J·B̃, 2·(16 nω)·(3 ni)·
+    (6 ni·nshift) flop per instance and application.  σ is checked against an FP64 GMRES at rtol
+    1e-8 at three probe frequencies.  `--coupling forward` weights the couplings by the momentum
+    transfer (forward scattering), which gives the ladder a vertex correction."""
     import warnings
     import epsx
     from epsx.ladder import LadderDevice, setup_ladder
@@ -248,10 +254,12 @@ def ladder_bench(args):
     nk = 16 if args.nk == 24 else args.nk
     temps = (100.0, 200.0, 300.0, 400.0)
     t0 = time.time()
-    setup, R, v, meta = t2g_ladder_setup(nk=nk, Tlist_K=temps)
+    fq = nk / 8.0 if args.coupling == "forward" else None
+    setup, R, v, meta = t2g_ladder_setup(nk=nk, Tlist_K=temps, forward_q=fq)
     nm, ni, nf = setup.dims
     nw = len(setup.ωs)
-    print(f"[bench] ladder setup nk={nk} ni={ni} nf={nf} nω={nw} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    print(f"[bench] ladder setup nk={nk} ni={ni} nf={nf} nω={nw} coupling={args.coupling} ({time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
     ctx = epsx.Context(0)
     dev = epsx.Scgd0Device(setup, ctx)
     occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0, spin_degeneracy=2.0)
@@ -262,48 +270,58 @@ def ladder_bench(args):
     print(f"[bench] scGD0 4 T: {t_scgd0:.2f}s, converged {r['converged'].tolist()}", file=sys.stderr, flush=True)
     L = setup_ladder(setup, R, v)
     ld = LadderDevice(dev, L)
-    Ωs = 0.005 * np.arange(args.omegas)
-    # prime: B̃ of the first temperature and one solve in the timed precision (not timed: loads the
-    # GEMM code objects)
-    ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs[:1], rtol=1e-2, maxiter=30,
-            precision=args.precision)
+    nO = args.omegas if args.omegas else 41
+    Ωs = 0.005 * np.arange(nO)
+    batched = not args.no_batched
+    Tl = meta["Tlist"]
+    # prime: B̃ of the first temperature and one solve in the timed precision (loads the GEMM code
+    # objects and times the library's candidates)
+    ld.loop(r["Σ"][:, :, :1], r["μ"][:1], Tl[:1], Ωs[:1], rtol=1e-2, maxiter=30, precision=args.precision,
+            batched=batched)
     t0 = time.perf_counter()
-    out = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs, rtol=1e-2, maxiter=30,
-                  precision=args.precision)
+    out = ld.loop(r["Σ"], r["μ"], Tl, Ωs, rtol=1e-2, maxiter=30, precision=args.precision, batched=batched)
     el = time.perf_counter() - t0
     tm = ctx.ladder_timings()
-    # the same (Ω, T) by an FP64 GMRES at rtol 1e-8 (maxiter 60): the reference for the errors
-    ref = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], meta["Tlist"][:1], Ωs, rtol=1e-8, maxiter=60)
+    print(f"[bench] ladder {nO} Ω x {len(Tl)} T: {el:.2f}s, niter {out['niter'].ravel().tolist()}",
+          file=sys.stderr, flush=True)
+    # the FP64 GMRES at rtol 1e-8 (maxiter 60) at three probe frequencies of the first temperature
+    sub = sorted(set([0, nO // 2, nO - 1]))
+    ref = ld.loop(r["Σ"][:, :, :1], r["μ"][:1], Tl[:1], Ωs[sub], rtol=1e-8, maxiter=60, batched=True)
 
     def relerr(a, b):
         return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
     ns = L.nshift
     flops = 2.0 * (16 * nw) * (3 * ni) * (6 * ni * ns)
-    gemm_per_app = tm["shift_gemm"] / max(tm["applications"], 1)
-    achieved = flops / gemm_per_app / 1e12
+    inst_apps = max(tm["instance_applications"], 1)
+    achieved = flops * inst_apps / tm["shift_gemm"] / 1e12 if tm["shift_gemm"] > 0 else 0.0
     peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16": BF16_PEAK_TFLOPS}[args.precision]
+    nsolve = nO * len(Tl)
     line = {
         "metric": "ladder s per (Ω, T) solve at rtol 1e-2; shift-table GEMM TFLOP/s",
-        "value": el / len(Ωs), "unit": "s/(Ω,T)", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "value": el / nsolve, "unit": "s/(Ω,T)", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": 1e3 * el, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else f"{args.precision} shift GEMM, f64 GMRES", "data": "synthetic",
-        "config": {"workload": f"t2g {nk}^3 ladder, GMRES rtol 1e-2, Ω = 0..{5 * (len(Ωs) - 1)} meV at 100 K",
-                   "nk": nk, "ni": ni, "nf": nf, "nw": nw, "nshift": ns, "niter": out["niter"][:, 0].tolist(),
-                   "scgd0_s_4T": t_scgd0, "applications": tm["applications"],
-                   "extrapolated_s_41x4": el / len(Ωs) * 41 * 4},
+        "config": {"workload": f"t2g {nk}^3 ladder, GMRES rtol 1e-2, Ω = 0..{5 * (nO - 1)} meV ({nO}) x 4 T "
+                               f"(100-400 K), {'batched over Ω' if batched else 'sequential, warm-started'}",
+                   "nk": nk, "ni": ni, "nf": nf, "nw": nw, "nshift": ns, "coupling": args.coupling,
+                   "niter_total": int(out["niter"].sum()), "niter_max": int(out["niter"].max()),
+                   "scgd0_s_4T": t_scgd0, "instance_applications": tm["instance_applications"],
+                   "batched_applications": tm["applications"]},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "kernel": ladder_gemm_label(args.precision),
-                     "algorithmic_flop_per_application": flops, "gemm_s_per_application": gemm_per_app,
-                     "apply_s": tm["apply"] / max(tm["applications"], 1)},
-        "sigma_ladder_xx": out["σ_ladder"][0, 0, :, 0].tolist(),
-        "sigma_bubble_xx": out["σ_bubble"][0, 0, :, 0].tolist(),
-        "error_vs_fp64_rtol1e-8": {"sigma_ladder_max_rel": relerr(out["σ_ladder"], ref["σ_ladder"]),
-                                   "sigma_bubble_max_rel": relerr(out["σ_bubble"], ref["σ_bubble"]),
+                     "algorithmic_flop_per_instance_application": flops,
+                     "gemm_s_total": tm["shift_gemm"], "apply_s_total": tm["apply"]},
+        "sigma_ladder_xx_T100": out["σ_ladder"][0, 0, :, 0].tolist(),
+        "sigma_bubble_xx_T100": out["σ_bubble"][0, 0, :, 0].tolist(),
+        "vertex_correction_max_rel": relerr(out["σ_ladder"], out["σ_bubble"]),
+        "error_vs_fp64_rtol1e-8": {"omegas_eV": Ωs[sub].tolist(),
+                                   "sigma_ladder_max_rel": relerr(out["σ_ladder"][:, :, sub, :1], ref["σ_ladder"]),
+                                   "sigma_bubble_max_rel": relerr(out["σ_bubble"][:, :, sub, :1], ref["σ_bubble"]),
                                    "reference_niter": ref["niter"][:, 0].tolist()},
     }
     if not args.no_cpu:
-        apps_per_solve = tm["applications"] / len(Ωs)
+        apps_per_solve = inst_apps / nsolve
         line["cpu_baseline"] = ladder_cpu_baseline(nw, ni, ns, apps_per_solve, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
@@ -314,9 +332,10 @@ def ladder_gemm_label(precision):
     if os.environ.get("EPSX_LADDER_BLAS", "1") == "0":
         return ("dgemm_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)" if precision == "fp64"
                 else f"lpkgemm_kernel (shift-table GEMM W = J B̃ from stored {precision} operands, FP32 accumulation)")
-    return {"fp64": "rocBLAS dgemm (shift-table GEMM W = J B̃, FP64)",
-            "fp32": "rocBLAS sgemm (shift-table GEMM W = J B̃ from stored FP32 operands)",
-            "bf16": "hipBLASLt BF16 GEMM (shift-table GEMM W = J B̃ from stored BF16 operands, FP32 accumulation)"}[precision]
+    return {"fp64": "rocBLAS dgemm_strided_batched (shift-table GEMM W = J B̃, every shift of a column block)",
+            "fp32": "rocBLAS gemm_strided_batched_ex FP32 (shift-table GEMM W = J B̃ from stored FP32 operands)",
+            "bf16": "rocBLAS gemm_strided_batched_ex BF16 (shift-table GEMM W = J B̃ from stored BF16 operands, FP32 "
+                    "accumulation)"}[precision]
 
 
 def ladder_cpu_baseline(nw, ni, ns, apps_per_solve, budget_s):

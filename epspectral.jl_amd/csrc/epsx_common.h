@@ -80,10 +80,12 @@ struct LadderDev {
     DevBuf Rs, v, Ww, klo, wlo, whi;  // rotations in star order, velocities, weights, shift tables
     DevBuf Bt;                        // B̃ of the current temperature
     double bt_T = -1, bt_wac = -1;
-    DevBuf Btl;                       // B̃ in BF16 / FP32, rows of pitch 3 ni rounded to 8 (on demand)
-    int32_t btl_prec = -1;            // the precision Btl holds for the current B̃ (-1: none)
-    // work buffers (JmL: the vertex as the transposed low-precision operand)
-    DevBuf Gm, Gp, Jm, JmL, W, Cp, Cm, Y, KY, S, jm, tm, jp, tp, Yc, red;
+    DevBuf Btl;                       // B̃ regrouped by column blocks (every shift of a block contiguous), in
+                                      // FP64 or BF16 / FP32 with rows of pitch 3 ni rounded to 8 (on demand)
+    int32_t btl_prec = -1;            // the precision Btl holds for the current B̃ (-1: none) ...
+    int64_t btl_cb = 0;               // ... and its block width
+    // propagators and small work buffers (the application's large buffers belong to each call)
+    DevBuf Gm, Gp, S, jm, tm, jp, tp, Yc, red, zero_idx;
 };
 
 constexpr int NET_MAX_LAYERS = 16;

@@ -186,69 +186,25 @@ __global__ __launch_bounds__(256) void bt_build_kernel(BtArgs a)
             }
 }
 
-// Jm[(j, c12, c0, ri), (b, ip)] = Re / Im of J[j, c12, b, c0, ip]: the vertex as the A operand.
-__global__ void jperm_kernel(int64_t nw, int64_t ni, const double2 *__restrict__ J, double *__restrict__ Jm)
-{
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t nrow = 16 * nw;
-    if (q >= nrow * 3 * ni) return;
-    const int64_t r = q % nrow, k = q / nrow;
-    const int64_t j = r % nw, c12 = (r / nw) % 4, c0 = (r / (4 * nw)) % 2, ri = r / (8 * nw);
-    const int64_t b = k % 3, ip = k / 3;
-    const double2 z = J[JIX(j, c12, b, c0, ip)];
-    Jm[q] = ri == 0 ? z.x : z.y;
-}
-
-// The vertex as the transposed low-precision A operand of the shift-table product:
-// JmL[k + kp r] = Jm[r, k] in BF16 / FP32 (zero for 3 ni <= k < kp), through a 64 x 64 LDS tile so
-// both the reads (along ω) and the writes (along k) are contiguous.
-template <typename E>
-__global__ __launch_bounds__(256) void jperm_lp_kernel(int64_t nw, int64_t ni, int64_t kp, const double2 *__restrict__ J,
-                                                       E *__restrict__ JmL)
-{
-    __shared__ double tile[64][65];
-    const int64_t nrow = 16 * nw, K = 3 * ni;
-    const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
-    const int lx = threadIdx.x & 63;
-    for (int sk = threadIdx.x >> 6; sk < 64; sk += 4) {
-        const int64_t r = r0 + lx, k = k0 + sk;
-        double v = 0.0;
-        if (r < nrow && k < K) {
-            const int64_t j = r % nw, c12 = (r / nw) % 4, c0 = (r / (4 * nw)) % 2, ri = r / (8 * nw);
-            const double2 z = J[JIX(j, c12, k % 3, c0, k / 3)];
-            v = ri == 0 ? z.x : z.y;
-        }
-        tile[sk][lx] = v;
-    }
-    __syncthreads();
-    for (int sr = threadIdx.x >> 6; sr < 64; sr += 4) {
-        const int64_t r = r0 + sr, k = k0 + lx;
-        if (r < nrow && k < kp) {
-            const double v = tile[lx][sr];
-            if constexpr (sizeof(E) == 2) JmL[k + kp * r] = to_bf16(v);
-            else JmL[k + kp * r] = (float)v;
-        }
-    }
-}
-
 // B̃ in BF16 / FP32 with the row pitch of the low-precision product: out[k + kp c] = Bt[k + K c]
 // (zero for K <= k < kp).
+// B̃ with its columns regrouped by blocks of cb columns, every shift of a block contiguous: the
+// column (c0 + cw, m) of B̃ (Bt[:, c + ncol m]) goes to c0 ns + cw + cbb m, where c0 = cb ⌊c / cb⌋
+// and cbb = min(cb, ncol - c0), so that one GEMM over a block's cbb ns columns gives W in the
+// layout the shift-add reads; in E = double, float or BF16 bits, rows of pitch kp (zero beyond K).
 template <typename E>
-__global__ void bt_lp_kernel(int64_t K, int64_t kp, int64_t ncols, const double *__restrict__ Bt, E *__restrict__ out)
+__global__ void bt_perm_kernel(int64_t K, int64_t kp, int64_t ncol, int64_t ns, int64_t cb, const double *__restrict__ Bt,
+                               E *__restrict__ out)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= kp * ncols) return;
-    const int64_t k = q % kp, c = q / kp;
-    const double v = k < K ? Bt[k + K * c] : 0.0;
-    if constexpr (sizeof(E) == 2) out[q] = to_bf16(v);
-    else out[q] = (float)v;
-}
-
-// Y = (U+ - U-)/2 for the KK product: Y[(j, q), (a, i)] from the type-1 columns
-__global__ void exch_Y_kernel(int64_t n, const double *__restrict__ Cp, const double *__restrict__ Cm, double *__restrict__ Y)
-{
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q < n) Y[q] = (Cp[q] - Cm[q]) / 2;
+    if (q >= kp * ncol * ns) return;
+    const int64_t k = q % kp, src = q / kp;
+    const int64_t c = src % ncol, m = src / ncol;
+    const int64_t c0 = c / cb * cb, cbb = cb < ncol - c0 ? cb : ncol - c0;
+    const double v = k < K ? Bt[k + K * src] : 0.0;
+    E *o = out + kp * (c0 * ns + (c - c0) + cbb * m) + k;
+    if constexpr (sizeof(E) == 2) *o = to_bf16(v);
+    else *o = (E)v;
 }
 
 // The channels K^{ab} with the contour signs (src/ladder_kernel.jl:296-307), then
@@ -390,37 +346,6 @@ __global__ void combine3_kernel(int64_t n, const double2 *__restrict__ a, double
     y[q] = make_double2(a[q].x + sb * b[q].x + sc * c[q].x, a[q].y + sb * b[q].y + sc * c[q].y);
 }
 
-// C_s[(j, q), col] (+)= Σ_m wlo W_m[(klo - 1, q), col] + whi W_m[(klo, q), col], both signs, for the
-// shifts [m0, m0 + mb) of one chunk (klo 1-based; W[(r, col), m] with r = j' + nω q, FP64 or FP32)
-template <typename TW>
-__global__ void shift_add_chunk_kernel(int64_t nw, int64_t ncol, int64_t ns, int64_t m0, int64_t mb,
-                                       const TW *__restrict__ W, const int32_t *__restrict__ klo,
-                                       const double *__restrict__ wlo, const double *__restrict__ whi,
-                                       double *__restrict__ Cp, double *__restrict__ Cm, int accumulate)
-{
-    const int64_t nrow = 16 * nw;
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= nrow * ncol) return;
-    const int64_t r = q % nrow, col = q / nrow;
-    const int64_t j = r % nw, qb = r - j;
-    double sp = accumulate ? Cp[q] : 0.0, sm = accumulate ? Cm[q] : 0.0;
-    // every tap is read (klo is clamped to a valid cell, its weights are zero outside the grid), so
-    // the loads of several shifts are in flight together instead of waiting behind branches
-#pragma unroll 4
-    for (int64_t mm = 0; mm < mb; mm++) {
-        const int64_t m = m0 + mm;
-        const TW *Wm = W + nrow * (col + ncol * mm) + qb;
-        const int64_t t0 = j + nw * m, t1 = j + nw * (m + ns);
-        const int64_t k0 = klo[t0] - 1, k1 = klo[t1] - 1;
-        const double lo0 = wlo[t0], hi0 = whi[t0], lo1 = wlo[t1], hi1 = whi[t1];
-        const double a0 = (double)Wm[k0], b0 = (double)Wm[k0 + 1], a1 = (double)Wm[k1], b1 = (double)Wm[k1 + 1];
-        sp += lo0 * a0 + hi0 * b0;
-        sm += lo1 * a1 + hi1 * b1;
-    }
-    Cp[q] = sp;
-    Cm[q] = sm;
-}
-
 // ---- instances of the batched solve: one (Ω, T) each; per-slot vectors of n complex at stride
 // `vs`; a launch runs the active instances k < nA of `list` (list[k] = slot), with the stacked
 // operands of the shift-table product indexed by k ----
@@ -471,33 +396,53 @@ __global__ __launch_bounds__(256) void jperm_lp_b_kernel(int64_t nw, int64_t ni,
     }
 }
 
-// the two-tap shift-add of instance k = blockIdx.y from its rows of the stacked W (leading dim M)
+// The shift-add over a block of columns [c0, c0 + cb) for every shift at once: W holds W_m[:, c] of
+// all stacked instances, W[row + M (c - c0 + cb m)] with row = (j', q) + nrow k, and
+//   C±[(j, q), c] = Σ_m wlo[j, m, ±] W_m[(klo - 1, q), c] + whi[j, m, ±] W_m[(klo, q), c]
+// is complete after one pass (ascending m, the order of the reference's sum over the shift grid).
+// One wave per (64 frequencies j, q, instance k, group of SA_C columns): the taps of (j, m) are read
+// once and serve the group's columns; lanes along j, so every W read of a wave is one contiguous run.
+constexpr int SA_C = 8;
 template <typename TW>
-__global__ void shift_add_b_kernel(int64_t nw, int64_t ncol, int64_t ns, int64_t m0, int64_t mb, int64_t M,
-                                   const TW *__restrict__ W, const int32_t *__restrict__ klo,
-                                   const double *__restrict__ wlo, const double *__restrict__ whi,
-                                   double *__restrict__ Cp, double *__restrict__ Cm, int accumulate)
+__global__ __launch_bounds__(64) void shift_add_cols_kernel(int64_t nw, int64_t ns, int64_t ncol, int64_t c0, int64_t cb,
+                                                            int64_t M, int ngrp, const TW *__restrict__ W,
+                                                            const int32_t *__restrict__ klo, const double *__restrict__ wlo,
+                                                            const double *__restrict__ whi, double *__restrict__ Cp,
+                                                            double *__restrict__ Cm)
 {
-    const int64_t nrow = 16 * nw, k = blockIdx.y;
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= nrow * ncol) return;
-    const int64_t r = q % nrow, col = q / nrow;
-    const int64_t j = r % nw, qb = r - j + nrow * k;
-    const int64_t o = q + nrow * ncol * k;
-    double sp = accumulate ? Cp[o] : 0.0, sm = accumulate ? Cm[o] : 0.0;
-#pragma unroll 4
-    for (int64_t mm = 0; mm < mb; mm++) {
-        const int64_t m = m0 + mm;
-        const TW *Wm = W + M * (col + ncol * mm) + qb;
+    const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (j >= nw) return;
+    const int64_t q = blockIdx.y;
+    const int64_t k = blockIdx.z / ngrp, cg = (int64_t)(blockIdx.z % ngrp) * SA_C;
+    const int64_t nrow = 16 * nw;
+    const int nc = (int)(cb - cg < SA_C ? cb - cg : SA_C);
+    const TW *Wb = W + nrow * k + nw * q + M * cg;
+    double sp[SA_C], sm[SA_C];
+#pragma unroll
+    for (int c = 0; c < SA_C; c++) sp[c] = sm[c] = 0.0;
+    for (int64_t m = 0; m < ns; m++) {
         const int64_t t0 = j + nw * m, t1 = j + nw * (m + ns);
         const int64_t k0 = klo[t0] - 1, k1 = klo[t1] - 1;
         const double lo0 = wlo[t0], hi0 = whi[t0], lo1 = wlo[t1], hi1 = whi[t1];
-        const double a0 = (double)Wm[k0], b0 = (double)Wm[k0 + 1], a1 = (double)Wm[k1], b1 = (double)Wm[k1 + 1];
-        sp += lo0 * a0 + hi0 * b0;
-        sm += lo1 * a1 + hi1 * b1;
+        const TW *Wm = Wb + M * cb * m;
+#pragma unroll
+        for (int c = 0; c < SA_C; c++) {
+            if (c < nc) {
+                const TW *col = Wm + M * c;
+                sp[c] += lo0 * (double)col[k0] + hi0 * (double)col[k0 + 1];
+                sm[c] += lo1 * (double)col[k1] + hi1 * (double)col[k1 + 1];
+            }
+        }
     }
-    Cp[o] = sp;
-    Cm[o] = sm;
+    const int64_t r = j + nw * q;
+#pragma unroll
+    for (int c = 0; c < SA_C; c++) {
+        if (c < nc) {
+            const int64_t o = r + nrow * (c0 + cg + c) + nrow * ncol * k;
+            Cp[o] = sp[c];
+            Cm[o] = sm[c];
+        }
+    }
 }
 
 // Y of instance k = blockIdx.y: (U+ - U-)/2 from the type-1 columns of its C±
@@ -762,125 +707,10 @@ static int ensure_Bt(epsx_ctx *ctx, double T, double w_ac)
     return tm.stop(1);
 }
 
-// LJ = L(J) on device vectors at the temperature whose B̃ is loaded and the propagators in l.Gm/l.Gp
-static int ladder_apply_dev(epsx_ctx *ctx, const double2 *J, double2 *LJ, int precision)
-{
-    Scgd0Dev &s = ctx->s;
-    LadderDev &l = ctx->l;
-    const int64_t nw = s.nw, ni = s.ni, nrow = 16 * nw, ncol = 6 * ni, K = 3 * ni;
-    const hipStream_t st = ctx->stream;
-    ClassTimer t_apply(ctx, 1);
-    // Low precision: B̃ converted once per temperature and the vertex transposed per application into
-    // BF16 / FP32 k-contiguous operands (row pitch kp) for the low-precision MFMA kernel;
-    // EPSX_LP_STAGED=1 rounds the FP64 operands as they are staged instead (for tests).
-    const char *staged_env = getenv("EPSX_LP_STAGED");
-    const bool native = precision != EPSX_PREC_FP64 && !(staged_env && staged_env[0] == '1');
-    const bool bf = precision == EPSX_PREC_BF16;
-    const int64_t kp = (K + 7) / 8 * 8, esz = bf ? 2 : 4;
-    if (native) {
-        if (l.btl_prec != precision) {
-            const int64_t ncols = ncol * l.ns;
-            EPSX_TRY(dev_alloc(ctx, l.Btl, esz * (size_t)kp * ncols));
-            const dim3 grid((unsigned)nb(kp * ncols));
-            if (bf)
-                hipLaunchKernelGGL(bt_lp_kernel<uint16_t>, grid, dim3(256), 0, st, K, kp, ncols, dptr<double>(l.Bt),
-                                   dptr<uint16_t>(l.Btl));
-            else
-                hipLaunchKernelGGL(bt_lp_kernel<float>, grid, dim3(256), 0, st, K, kp, ncols, dptr<double>(l.Bt),
-                                   dptr<float>(l.Btl));
-            EPSX_HIP(ctx, hipGetLastError());
-            l.btl_prec = precision;
-        }
-        EPSX_TRY(dev_alloc(ctx, l.JmL, esz * (size_t)kp * nrow));
-        const dim3 grid((unsigned)((nrow + 63) / 64), (unsigned)((kp + 63) / 64));
-        if (bf)
-            hipLaunchKernelGGL(jperm_lp_kernel<uint16_t>, grid, dim3(256), 0, st, nw, ni, kp, J, dptr<uint16_t>(l.JmL));
-        else
-            hipLaunchKernelGGL(jperm_lp_kernel<float>, grid, dim3(256), 0, st, nw, ni, kp, J, dptr<float>(l.JmL));
-    } else {
-        hipLaunchKernelGGL(jperm_kernel, dim3(nb(nrow * K)), dim3(256), 0, st, nw, ni, J, dptr<double>(l.Jm));
-    }
-    EPSX_HIP(ctx, hipGetLastError());
-    // W for chunks of shifts, each followed by its shift-add (W in FP32 from the low-precision
-    // products, which accumulate in FP32)
-    const bool blas = use_blas();
-    const size_t per_m = (native ? 4 : 8) * (size_t)nrow * ncol;
-    int64_t mb = (int64_t)((size_t(8) << 30) / per_m);
-    mb = mb < 1 ? 1 : (mb > l.ns ? l.ns : mb);
-    EPSX_TRY(dev_alloc(ctx, l.W, per_m * mb));
-    for (int64_t m0 = 0; m0 < l.ns; m0 += mb) {
-        const int64_t mc = m0 + mb <= l.ns ? mb : l.ns - m0;
-        ClassTimer t_gemm(ctx, 0);
-        GemmBatch bt;
-        const bool staged = precision != EPSX_PREC_FP64 && !native;
-        if (blas && !staged) {
-            rocblas_handle h;
-            EPSX_TRY(blas_handle(ctx, h));
-            const int bm = (int)nrow, bn = (int)(ncol * mc);
-            // BF16: hipBLASLt's fastest workspace-free algorithm (897 TF/s on this shape against
-            // rocBLAS's 735), timed once per chunk shape on the first application
-            bool lt = false;
-            if (native && bf)
-                EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, nrow, ncol * mc, kp, l.JmL.p, kp, 0,
-                                        static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0, kp, 0, l.W.p, nrow, 0,
-                                        1, &lt));
-            if (lt) {
-                EPSX_TRY(blaslt_gemm(ctx, EPSX_PREC_BF16, nrow, ncol * mc, kp, l.JmL.p, kp, 0,
-                                     static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0, kp, 0, l.W.p, nrow, 0, 1));
-            } else if (native) {
-                // W (FP32) = JmLᵀ Bl from the k-contiguous BF16 / FP32 operands, FP32 accumulation
-                const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
-                const float one = 1.0f, zero = 0.0f;
-                const rocblas_datatype dt = bf ? rocblas_datatype_bf16_r : rocblas_datatype_f32_r;
-                EPSX_TRY(blas_check(ctx, rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, bm, bn,
-                                                         (int)kp, &one, l.JmL.p, dt, (int)kp, Bl, dt, (int)kp, &zero,
-                                                         l.W.p, rocblas_datatype_f32_r, bm, l.W.p,
-                                                         rocblas_datatype_f32_r, bm, rocblas_datatype_f32_r,
-                                                         rocblas_gemm_algo_standard, 0, 0),
-                                    "rocblas_gemm_ex (ladder product)"));
-            } else {
-                const double one = 1.0, zero = 0.0;
-                EPSX_TRY(blas_check(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, bm, bn, (int)K,
-                                                       &one, dptr<double>(l.Jm), bm, dptr<double>(l.Bt) + K * ncol * m0,
-                                                       (int)K, &zero, dptr<double>(l.W), bm),
-                                    "rocblas_dgemm (ladder product)"));
-            }
-        } else if (native) {
-            const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
-            bt.Cf = dptr<float>(l.W);
-            EPSX_TRY(lpgemm_tn(ctx, precision, nrow, ncol * mc, kp, l.JmL.p, kp, Bl, kp, nullptr, nrow, false, bt));
-        } else {
-            bt.precision = precision;   // FP32 / BF16 operands rounded as staged, FP32 accumulation, or FP64
-            EPSX_TRY(dgemm(ctx, nrow, ncol * mc, K, dptr<double>(l.Jm), nrow, dptr<double>(l.Bt) + K * ncol * m0, K,
-                           dptr<double>(l.W), nrow, false, bt));
-        }
-        EPSX_TRY(t_gemm.stop(1));
-        if (native)
-            hipLaunchKernelGGL(shift_add_chunk_kernel<float>, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns,
-                               m0, mc, dptr<float>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
-                               dptr<double>(l.Cp), dptr<double>(l.Cm), m0 > 0 ? 1 : 0);
-        else
-            hipLaunchKernelGGL(shift_add_chunk_kernel<double>, dim3(nb(nrow * ncol)), dim3(256), 0, st, nw, ncol, l.ns,
-                               m0, mc, dptr<double>(l.W), dptr<int32_t>(l.klo), dptr<double>(l.wlo),
-                               dptr<double>(l.whi), dptr<double>(l.Cp), dptr<double>(l.Cm), m0 > 0 ? 1 : 0);
-        EPSX_HIP(ctx, hipGetLastError());
-    }
-    // KY = KK (U+ - U-)/2, then the channels and the attached propagators
-    const int64_t ny = nrow * 3 * ni;
-    hipLaunchKernelGGL(exch_Y_kernel, dim3(nb(ny)), dim3(256), 0, st, ny, dptr<double>(l.Cp), dptr<double>(l.Cm),
-                       dptr<double>(l.Y));
-    EPSX_HIP(ctx, hipGetLastError());
-    EPSX_TRY(dgemm_nn(ctx, nw, ny / nw, nw, dptr<double>(s.KK), nw, dptr<double>(l.Y), nw, dptr<double>(l.KY), nw, false));
-    hipLaunchKernelGGL(exch_attach_kernel, dim3(nb(nw * 6 * ni)), dim3(256), 0, st, nw, ni, dptr<double>(l.Cp),
-                       dptr<double>(l.Cm), dptr<double>(l.KY), dptr<double2>(l.Gm), dptr<double2>(l.Gp), LJ);
-    EPSX_HIP(ctx, hipGetLastError());
-    return t_apply.stop(1);
-}
-
 // Work buffers of the batched application, sized for nb instances.
 struct BatchWork {
     Tmp Jm, JmL, W, Cp, Cm, Y, KY;
-    int64_t nb = 0, w_bytes = 0;
+    int64_t nb = 0, w_bytes = 0, cb = 0;   // instances, W bytes, columns per block of the product
 };
 
 static int batch_work(epsx_ctx *ctx, BatchWork &bw, int64_t nb, int precision)
@@ -894,10 +724,12 @@ static int batch_work(epsx_ctx *ctx, BatchWork &bw, int64_t nb, int precision)
     EPSX_TRY(tmp_alloc(ctx, bw.KY.b, 8 * (size_t)nrow * 3 * ni * nb));
     if (precision == EPSX_PREC_FP64) EPSX_TRY(tmp_alloc(ctx, bw.Jm.b, 8 * (size_t)nrow * K * nb));
     else EPSX_TRY(tmp_alloc(ctx, bw.JmL.b, (precision == EPSX_PREC_BF16 ? 2 : 4) * (size_t)kp * nrow * nb));
-    // W for a chunk of shifts of all instances: at most 8 GiB, at least one shift
-    const size_t per_m = (precision == EPSX_PREC_FP64 ? 8 : 4) * (size_t)nrow * nb * 6 * ni;
-    bw.w_bytes = (int64_t)(per_m > (size_t(8) << 30) ? per_m : (size_t(8) << 30) / per_m * per_m);
-    if ((size_t)bw.w_bytes > per_m * (size_t)ctx->l.ns) bw.w_bytes = (int64_t)(per_m * ctx->l.ns);
+    // W for a block of columns of all instances and every shift: at most 16 GiB (at least one column)
+    const size_t per_col = (precision == EPSX_PREC_FP64 ? 8 : 4) * (size_t)nrow * nb * ctx->l.ns;
+    const size_t ncol = 6 * (size_t)ni, cap = size_t(16) << 30;
+    const size_t cols = per_col >= cap ? 1 : (cap / per_col < ncol ? cap / per_col : ncol);
+    bw.cb = (int64_t)(cols >= ncol ? ncol : (cols >= (size_t)SA_C ? cols / SA_C * SA_C : cols));
+    bw.w_bytes = (int64_t)(per_col * (size_t)bw.cb);
     EPSX_TRY(tmp_alloc(ctx, bw.W.b, (size_t)bw.w_bytes));
     return EPSX_OK;
 }
@@ -916,20 +748,26 @@ static int ladder_apply_batch(epsx_ctx *ctx, BatchWork &bw, int nA, const std::v
     ClassTimer t_apply(ctx, 1);
     const bool bf = precision == EPSX_PREC_BF16, native = precision != EPSX_PREC_FP64;
     const int64_t kp = (K + 7) / 8 * 8, esz = bf ? 2 : 4;
+    // the block width of this call's buffers (fixed per BatchWork, so one regrouped B̃ serves it)
+    const int64_t cb = bw.cb;
+    if (l.btl_prec != precision || l.btl_cb != cb) {
+        const int64_t ncols = ncol * l.ns, pitch = native ? kp : K, eb = native ? esz : 8;
+        EPSX_TRY(dev_alloc(ctx, l.Btl, eb * (size_t)pitch * ncols));
+        const dim3 grid((unsigned)nb(pitch * ncols));
+        if (!native)
+            hipLaunchKernelGGL(bt_perm_kernel<double>, grid, dim3(256), 0, st, K, K, ncol, l.ns, cb, dptr<double>(l.Bt),
+                               dptr<double>(l.Btl));
+        else if (bf)
+            hipLaunchKernelGGL(bt_perm_kernel<uint16_t>, grid, dim3(256), 0, st, K, kp, ncol, l.ns, cb, dptr<double>(l.Bt),
+                               dptr<uint16_t>(l.Btl));
+        else
+            hipLaunchKernelGGL(bt_perm_kernel<float>, grid, dim3(256), 0, st, K, kp, ncol, l.ns, cb, dptr<double>(l.Bt),
+                               dptr<float>(l.Btl));
+        EPSX_HIP(ctx, hipGetLastError());
+        l.btl_prec = precision;
+        l.btl_cb = cb;
+    }
     if (native) {
-        if (l.btl_prec != precision) {
-            const int64_t ncols = ncol * l.ns;
-            EPSX_TRY(dev_alloc(ctx, l.Btl, esz * (size_t)kp * ncols));
-            const dim3 grid((unsigned)nb(kp * ncols));
-            if (bf)
-                hipLaunchKernelGGL(bt_lp_kernel<uint16_t>, grid, dim3(256), 0, st, K, kp, ncols, dptr<double>(l.Bt),
-                                   dptr<uint16_t>(l.Btl));
-            else
-                hipLaunchKernelGGL(bt_lp_kernel<float>, grid, dim3(256), 0, st, K, kp, ncols, dptr<double>(l.Bt),
-                                   dptr<float>(l.Btl));
-            EPSX_HIP(ctx, hipGetLastError());
-            l.btl_prec = precision;
-        }
         const dim3 grid((unsigned)((nrow + 63) / 64), (unsigned)((kp + 63) / 64), (unsigned)nA);
         if (bf)
             hipLaunchKernelGGL(jperm_lp_b_kernel<uint16_t>, grid, dim3(256), 0, st, nw, ni, kp, d_list, V, vs,
@@ -942,61 +780,60 @@ static int ladder_apply_batch(epsx_ctx *ctx, BatchWork &bw, int nA, const std::v
                            bw.Jm.ptr<double>());
     }
     EPSX_HIP(ctx, hipGetLastError());
-    const size_t per_m = (native ? 4 : 8) * (size_t)M * ncol;
-    int64_t mb = (int64_t)((size_t)bw.w_bytes / per_m);
-    mb = mb < 1 ? 1 : (mb > l.ns ? l.ns : mb);
+    // W for a block of columns and every shift (one product over the block's cbb ns regrouped columns,
+    // J read once per column tile), then that block's shift-add, which completes its columns of C±
     const bool blas = use_blas();
-    for (int64_t m0 = 0; m0 < l.ns; m0 += mb) {
-        const int64_t mc = m0 + mb <= l.ns ? mb : l.ns - m0;
+    for (int64_t c0 = 0; c0 < ncol; c0 += cb) {
+        const int64_t cbb = c0 + cb <= ncol ? cb : ncol - c0, nn = cbb * l.ns;
         ClassTimer t_gemm(ctx, 0);
         if (native) {
-            const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * ncol * m0;
+            const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * c0 * l.ns;
             bool lt = false;
             if (blas && bf)
-                EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, M, ncol * mc, kp, bw.JmL.b.p, kp, 0, Bl, kp, 0, bw.W.b.p, M,
-                                        0, 1, &lt));
+                EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, M, nn, kp, bw.JmL.b.p, kp, 0, Bl, kp, 0, bw.W.b.p, M, 0, 1,
+                                        &lt));
             if (lt) {
-                EPSX_TRY(blaslt_gemm(ctx, EPSX_PREC_BF16, M, ncol * mc, kp, bw.JmL.b.p, kp, 0, Bl, kp, 0, bw.W.b.p, M, 0, 1));
+                EPSX_TRY(blaslt_gemm(ctx, EPSX_PREC_BF16, M, nn, kp, bw.JmL.b.p, kp, 0, Bl, kp, 0, bw.W.b.p, M, 0, 1));
             } else if (blas) {
                 rocblas_handle h;
                 EPSX_TRY(blas_handle(ctx, h));
                 const float one = 1.0f, zero = 0.0f;
                 const rocblas_datatype dt = bf ? rocblas_datatype_bf16_r : rocblas_datatype_f32_r;
                 EPSX_TRY(blas_check(ctx, rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, (int)M,
-                                                         (int)(ncol * mc), (int)kp, &one, bw.JmL.b.p, dt, (int)kp, Bl, dt,
-                                                         (int)kp, &zero, bw.W.b.p, rocblas_datatype_f32_r, (int)M,
-                                                         bw.W.b.p, rocblas_datatype_f32_r, (int)M, rocblas_datatype_f32_r,
+                                                         (int)nn, (int)kp, &one, bw.JmL.b.p, dt, (int)kp, Bl, dt, (int)kp,
+                                                         &zero, bw.W.b.p, rocblas_datatype_f32_r, (int)M, bw.W.b.p,
+                                                         rocblas_datatype_f32_r, (int)M, rocblas_datatype_f32_r,
                                                          rocblas_gemm_algo_standard, 0, 0),
-                                    "rocblas_gemm_ex (batched ladder product)"));
+                                    "rocblas_gemm_ex (ladder product)"));
             } else {
                 GemmBatch bt;
                 bt.Cf = bw.W.ptr<float>();
-                EPSX_TRY(lpgemm_tn(ctx, precision, M, ncol * mc, kp, bw.JmL.b.p, kp, Bl, kp, nullptr, M, false, bt));
+                EPSX_TRY(lpgemm_tn(ctx, precision, M, nn, kp, bw.JmL.b.p, kp, Bl, kp, nullptr, M, false, bt));
             }
         } else if (blas) {
             rocblas_handle h;
             EPSX_TRY(blas_handle(ctx, h));
             const double one = 1.0, zero = 0.0;
-            EPSX_TRY(blas_check(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)M, (int)(ncol * mc),
-                                                   (int)K, &one, bw.Jm.ptr<double>(), (int)M,
-                                                   dptr<double>(l.Bt) + K * ncol * m0, (int)K, &zero, bw.W.ptr<double>(),
-                                                   (int)M),
-                                "rocblas_dgemm (batched ladder product)"));
+            EPSX_TRY(blas_check(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)M, (int)nn, (int)K,
+                                                   &one, bw.Jm.ptr<double>(), (int)M, dptr<double>(l.Btl) + K * c0 * l.ns,
+                                                   (int)K, &zero, bw.W.ptr<double>(), (int)M),
+                                "rocblas_dgemm (ladder product)"));
         } else {
             GemmBatch bt;
-            EPSX_TRY(dgemm(ctx, M, ncol * mc, K, bw.Jm.ptr<double>(), M, dptr<double>(l.Bt) + K * ncol * m0, K,
+            EPSX_TRY(dgemm(ctx, M, nn, K, bw.Jm.ptr<double>(), M, dptr<double>(l.Btl) + K * c0 * l.ns, K,
                            bw.W.ptr<double>(), M, false, bt));
         }
         EPSX_TRY(t_gemm.stop(1));
-        const dim3 grid(nb(nrow * ncol), (unsigned)nA);
+        const int ngrp = (int)((cbb + SA_C - 1) / SA_C);
+        const dim3 grid((unsigned)((nw + 63) / 64), 16u, (unsigned)(nA * ngrp));
         if (native)
-            hipLaunchKernelGGL(shift_add_b_kernel<float>, grid, dim3(256), 0, st, nw, ncol, l.ns, m0, mc, M,
+            hipLaunchKernelGGL(shift_add_cols_kernel<float>, grid, dim3(64), 0, st, nw, l.ns, ncol, c0, cbb, M, ngrp,
                                bw.W.ptr<float>(), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
-                               bw.Cp.ptr<double>(), bw.Cm.ptr<double>(), m0 > 0 ? 1 : 0);
+                               bw.Cp.ptr<double>(), bw.Cm.ptr<double>());
         else
-            hipLaunchKernelGGL(shift_add_b_kernel<double>, grid, dim3(256), 0, st, nw, ncol, l.ns, m0, mc, M,
+            hipLaunchKernelGGL(shift_add_cols_kernel<double>, grid, dim3(64), 0, st, nw, l.ns, ncol, c0, cbb, M, ngrp,
                                bw.W.ptr<double>(), dptr<int32_t>(l.klo), dptr<double>(l.wlo), dptr<double>(l.whi),
-                               bw.Cp.ptr<double>(), bw.Cm.ptr<double>(), m0 > 0 ? 1 : 0);
+                               bw.Cp.ptr<double>(), bw.Cm.ptr<double>());
         EPSX_HIP(ctx, hipGetLastError());
     }
     // KY = KK (U+ - U-)/2 for all instances in one product, then per instance the channels and
@@ -1015,21 +852,28 @@ static int ladder_apply_batch(epsx_ctx *ctx, BatchWork &bw, int nA, const std::v
                            LJ + ljs * z);
     }
     EPSX_HIP(ctx, hipGetLastError());
+    ctx->n_class[3] += nA;
     return t_apply.stop(1);
+}
+
+// LJ = L(J) for one vertex at the temperature whose B̃ is loaded and the propagators in l.Gm/l.Gp
+// (apply_ladder_kernel!, src/ladder.jl:245-296): the batched application with one instance.
+static int ladder_apply_dev(epsx_ctx *ctx, BatchWork &bw, const double2 *J, double2 *LJ, int precision)
+{
+    static const std::vector<int32_t> one{0};
+    return ladder_apply_batch(ctx, bw, 1, one, dptr<int32_t>(ctx->l.zero_idx), J, 0, LJ, 0, dptr<double2>(ctx->l.Gm),
+                              dptr<double2>(ctx->l.Gp), precision);
 }
 
 static int ladder_work(epsx_ctx *ctx)
 {
     Scgd0Dev &s = ctx->s;
     LadderDev &l = ctx->l;
-    const int64_t nw = s.nw, ni = s.ni, nrow = 16 * nw;
+    const int64_t nw = s.nw, ni = s.ni;
     EPSX_TRY(dev_alloc(ctx, l.Gm, 16 * (size_t)nw * 4 * ni));
     EPSX_TRY(dev_alloc(ctx, l.Gp, 16 * (size_t)nw * 4 * ni));
-    EPSX_TRY(dev_alloc(ctx, l.Jm, 8 * (size_t)nrow * 3 * ni));
-    EPSX_TRY(dev_alloc(ctx, l.Cp, 8 * (size_t)nrow * 6 * ni));
-    EPSX_TRY(dev_alloc(ctx, l.Cm, 8 * (size_t)nrow * 6 * ni));
-    EPSX_TRY(dev_alloc(ctx, l.Y, 8 * (size_t)nrow * 3 * ni));
-    EPSX_TRY(dev_alloc(ctx, l.KY, 8 * (size_t)nrow * 3 * ni));
+    const int32_t zero = 0;
+    EPSX_TRY(upload(ctx, l.zero_idx, &zero, 4));
     EPSX_TRY(dev_alloc(ctx, l.red, 16 * (size_t)RED_BLOCKS));
     EPSX_TRY(dev_alloc(ctx, l.Yc, 16 * (size_t)3 * ni));
     return EPSX_OK;
@@ -1144,7 +988,9 @@ extern "C" int epsx_apply_ladder_kernel(epsx_ctx *ctx, const double *J, const do
     EPSX_HIP(ctx, hipMemcpyAsync(l.Gm.p, Gm, 16 * ng, hipMemcpyHostToDevice, ctx->stream));
     EPSX_HIP(ctx, hipMemcpyAsync(l.Gp.p, Gp, 16 * ng, hipMemcpyHostToDevice, ctx->stream));
     EPSX_TRY(ensure_Bt(ctx, T, w_acoustic));
-    EPSX_TRY(ladder_apply_dev(ctx, dJ.ptr<double2>(), dLJ.ptr<double2>(), precision));
+    BatchWork bw;
+    EPSX_TRY(batch_work(ctx, bw, 1, precision));
+    EPSX_TRY(ladder_apply_dev(ctx, bw, dJ.ptr<double2>(), dLJ.ptr<double2>(), precision));
     EPSX_HIP(ctx, hipMemcpyAsync(LJ, dLJ.b.p, 16 * n, hipMemcpyDeviceToHost, ctx->stream));
     EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return EPSX_OK;
@@ -1400,6 +1246,8 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
     if (gm) EPSX_TRY(tmp_alloc(ctx, V.b, 16 * (size_t)n * (it_max + 1)));
     EPSX_TRY(tmp_alloc(ctx, hcol.b, 16 * (size_t)(it_max + 2)));
     const double btol = std::pow(2.220446049250313e-16, 0.75);
+    BatchWork bw;
+    EPSX_TRY(batch_work(ctx, bw, 1, par->precision));
     std::vector<int32_t> jv(nw);
     std::vector<double> tv(nw), xs(nw);
     double2 *dJ0 = J0.ptr<double2>(), *dJ = J.ptr<double2>(), *dLJ = LJ.ptr<double2>(), *dR = R.ptr<double2>();
@@ -1442,7 +1290,7 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
                 const double atol = par->rtol * std::sqrt(nrm.x);
                 const bool warm = iO > 0;
                 if (warm) {
-                    EPSX_TRY(ladder_apply_dev(ctx, dJ, dLJ, par->precision));
+                    EPSX_TRY(ladder_apply_dev(ctx, bw, dJ, dLJ, par->precision));
                     // r = b - (x - Lx)
                     hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, dJ0, -1.0, dJ, 1.0, dLJ, dR);
                 } else {
@@ -1468,7 +1316,7 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
                     while (!(solved || niter >= it_max || breakdown)) {
                         const int64_t j = niter++;
                         double2 *Vj = Vb + n * j, *q = dR;
-                        EPSX_TRY(ladder_apply_dev(ctx, Vj, dLJ, par->precision));
+                        EPSX_TRY(ladder_apply_dev(ctx, bw, Vj, dLJ, par->precision));
                         hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, Vj, -1.0, dLJ, 0.0, dLJ, q);
                         for (int64_t i = 0; i <= j; i++) {
                             EPSX_TRY(ldot(ctx, n, Vb + n * i, q, dh + i));
@@ -1514,7 +1362,7 @@ extern "C" int epsx_ladder_run(epsx_ctx *ctx, const epsx_ladder_params *par, eps
                 EPSX_HIP(ctx, hipMemcpyAsync(dJ, dJ0, 16 * n, hipMemcpyDeviceToDevice, st));
                 double *sit = out->sigma_iter + 9 * it_max * slot;
                 for (int64_t it = 1; it <= it_max; it++) {
-                    EPSX_TRY(ladder_apply_dev(ctx, dJ, dLJ, par->precision));
+                    EPSX_TRY(ladder_apply_dev(ctx, bw, dJ, dLJ, par->precision));
                     hipLaunchKernelGGL(combine3_kernel, launch_n(n), dim3(256), 0, st, n, dJ0, 1.0, dLJ, 0.0, dLJ, dJ);
                     EPSX_TRY(sigma_of(ctx, dJ, Om, T, sit + 9 * (it - 1)));
                     niter = it;

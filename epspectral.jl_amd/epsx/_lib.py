@@ -224,7 +224,7 @@ class Context:
         n = np.zeros(8, dtype=np.int64)
         lib().epsx_last_timings(self.handle, ptr(s), ptr(n))
         return {"shift_gemm": s[0], "shift_gemm_launches": int(n[0]), "apply": s[1], "applications": int(n[1]),
-                "bt_build": s[2], "bt_builds": int(n[2])}
+                "bt_build": s[2], "bt_builds": int(n[2]), "instance_applications": int(n[3])}
 
     def timings(self):
         s = np.zeros(8)

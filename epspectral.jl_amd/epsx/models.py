@@ -104,7 +104,7 @@ def oh_ibz(nk):
 def t2g_setup(nk=24, t=0.26, tp=0.03, nmodes=9, wmin=0.010, wmax=0.080, g2_median=2e-3,
               g2_sigma_ln=1.0, window=0.8, nelec=1.0, Tlist_K=(100.0, 300.0), nlist=0.0, seed=1234,
               grid_windows=((0.3, 1e-3), (1.2, 4e-3), (3.0, 16e-3)), spectral_step=None,
-              degen_cutoff=ELECTRON_DEGEN_CUTOFF, make_g2=True, uniform_grid=None):
+              degen_cutoff=ELECTRON_DEGEN_CUTOFF, make_g2=True, uniform_grid=None, forward_q=None):
     """The synthetic cubic t2g model of the benchmark, in eV.
 
     * bands ε_xy = -2t(cos kx + cos ky) - 2t' cos kz and cyclic, sorted per k;
@@ -114,7 +114,11 @@ def t2g_setup(nk=24, t=0.26, tp=0.03, nmodes=9, wmin=0.010, wmax=0.080, g2_media
       |g|² ~ LogNormal(ln g2_median, g2_sigma_ln), i.i.d., seeded;
     * ωs graded around the bare μ: 1 meV to ±0.3 eV, 4 meV to ±1.2 eV, 16 meV to ±3 eV, or, with
       `uniform_grid = (L, half_width)`, L uniform nodes over bare μ ± half_width;
-      ωs_spectral uniform at ωs.dω_min over the span of ωs (the run_scgd0 default).
+      ωs_spectral uniform at ωs.dω_min over the span of ωs (the run_scgd0 default);
+    * with `forward_q = q0` (grid steps) the couplings are also weighted by the momentum transfer,
+      0.25 + 3 exp(-|k_f - k_i|² / (2 q0²)) with |k_f - k_i| the minimal-image distance between the
+      inner state's k and the outer state's representative k: forward scattering, which gives the
+      ladder a vertex correction (i.i.d. couplings average it out).
     """
     rng = np.random.default_rng(seed)
     inv, canon, counts = oh_ibz(nk)
@@ -144,6 +148,13 @@ def t2g_setup(nk=24, t=0.26, tp=0.03, nmodes=9, wmin=0.010, wmax=0.080, g2_media
         for f0 in range(0, nf, chunk):
             f1 = min(nf, f0 + chunk)
             absg2 = rng.lognormal(mean=math.log(g2_median), sigma=g2_sigma_ln, size=(nmodes, ni, f1 - f0))
+            if forward_q is not None:
+                ki = np.array([canon[k] for k, b in sel_i], dtype=np.int64)             # (ni, 3)
+                kf = np.array([k for k, b in sel_f[f0:f1]], dtype=np.int64)
+                kfv = np.stack([kf % nk, (kf // nk) % nk, kf // (nk * nk)], axis=1)    # (nc, 3)
+                d = np.abs(kfv[None, :, :] - ki[:, None, :]) % nk
+                d2 = np.sum(np.minimum(d, nk - d) ** 2, axis=2).astype(np.float64)    # (ni, nc)
+                absg2 *= (0.25 + 3.0 * np.exp(-d2 / (2.0 * forward_q ** 2)))[None, :, :]
             g2[:, :, f0:f1] = absg2 / (2.0 * ω_ν[:, None, None])
     else:
         g2 = np.empty((nmodes, ni, 0), order="F")

@@ -53,7 +53,8 @@ const char *epsx_last_error(const epsx_ctx *ctx);
  * passes [0] with the Einstein factorisation), [7] unused.  In the
  * batched loop the second iteration is the one timed.
  * After epsx_ladder_run / epsx_apply_ladder_kernel: [0] the shift-table GEMMs W = J·B̃,
- * [1] whole kernel applications (count = applications), [2] B̃ builds, [3..7] unused. */
+ * [1] whole kernel applications (count = applications; a batched application counts once),
+ * [2] B̃ builds, [3] count only: the (Ω, T) instances of those applications, [4..7] unused. */
 int epsx_last_timings(const epsx_ctx *ctx, double *seconds8, int64_t *launches8);
 /* Device memory the context holds (its buffers, the temporaries of the call in progress and the
  * released temporaries it keeps for reuse) and the part of it that is released and kept, in bytes.
