@@ -415,8 +415,7 @@ def warmstart_bench(args):
         dΣ = float(np.max(np.abs(w["Σ"] - rc["Σ"])))
         rows.append(dict(plain_iterations=int(rc["history"]["err"].shape[1]), warm_iterations=int(w["iterations"]),
                          fallback=bool(w["fallback"][0]), plain_s=t_plain, warm_s=w["wall"], max_dSigma=dΣ,
-                         T_K=float(meta["Tlist_K"][0]), converged=bool(w["converged"].all() and rc["converged"].all()),
-                         warm_phases_s=w["phases"]))
+                         T_K=float(meta["Tlist_K"][0]), converged=bool(w["converged"].all() and rc["converged"].all())))
         print(f"[bench] held-out {h + 1}/{args.heldout}: {rows[-1]}", file=sys.stderr, flush=True)
     it_p = sum(r["plain_iterations"] for r in rows)
     it_w = sum(r["warm_iterations"] for r in rows)

@@ -350,6 +350,9 @@ int einstein_h_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const d
                       double w_acoustic, const double *occ, int64_t p0, int64_t np, const double *ef, const double *wf,
                       const int32_t *fi, double *hout, const int32_t *act);
 int acoustic_finite_check(epsx_ctx *ctx, int64_t nm, int64_t n, const double *g2p, const uint8_t *skip, bool *ok);
+// the warm-start network's Σ[:, :, tdst[z]] predicted from the G0D0 Σ[:, :, tsrc[z]], z < nz (epsx_net.hip)
+int net_predict_sigma(epsx_ctx *ctx, int64_t nw, int64_t ni, int nz, const int32_t *tsrc, const int32_t *tdst,
+                      double2 *S, const double *ws, const double *e_i, const double *mu, const double *T, int precision);
 size_t fm_part_elems(int64_t nfs, int64_t ni, int64_t nw, int64_t nT, int64_t f_tile);
 int fm_grid_host(int64_t nw, const int64_t *offsets, std::vector<int32_t> &off,
                  std::vector<double> &rgap, std::vector<int32_t> &ltab, int64_t &span);

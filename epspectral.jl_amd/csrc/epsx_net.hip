@@ -450,6 +450,48 @@ static int net_backward_dev(epsx_ctx *ctx, int64_t nb, bool bf16, float *grad, d
     return EPSX_OK;
 }
 
+// ---- the warm start's prediction on the device ---------------------------------------------------
+// Inputs per (outer state i, slot z), over the L = nω grid points (the features of
+// epspectral.jl_amd/epsx/learn.py, which the network was trained on):
+//   0, 1: asinh(Re Σ / SIGMA_SCALE), asinh(Im Σ / SIGMA_SCALE) of the G0D0 self-energy;
+//   2: (ω - μ) / ENERGY_SCALE;  3: (ε_i - μ) / ENERGY_SCALE;  4: T / T_SCALE.
+// The output (Re, Im) times SIGMA_SCALE is the predicted Σ.
+constexpr double WS_SIGMA_SCALE = 0.05, WS_ENERGY_SCALE = 1.0, WS_T_SCALE = 0.04;
+
+// X0[c][b L + x] for the samples b = i + ni z' (z' = z - z0 < nbz) of the slots tidx[z]
+__global__ void ws_features_kernel(int64_t L, int64_t ni, int64_t b0, int64_t nb, const double2 *__restrict__ S,
+                                   const double *__restrict__ ws, const double *__restrict__ e_i,
+                                   const double *__restrict__ mu, const double *__restrict__ T,
+                                   const int32_t *__restrict__ tsrc, float *__restrict__ X)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nb * L) return;
+    const int64_t x = q % L, bl = q / L, b = b0 + bl;
+    const int64_t i = b % ni, t = tsrc[b / ni];
+    const double2 v = S[x + L * (i + ni * t)];
+    const double m = mu[t];
+    const int64_t NB = nb * L;
+    X[q] = (float)asinh(v.x / WS_SIGMA_SCALE);
+    X[q + NB] = (float)asinh(v.y / WS_SIGMA_SCALE);
+    X[q + 2 * NB] = (float)((ws[x] - m) / WS_ENERGY_SCALE);
+    X[q + 3 * NB] = (float)((e_i[i] - m) / WS_ENERGY_SCALE);
+    X[q + 4 * NB] = (float)(T[t] / WS_T_SCALE);
+}
+
+// Σ[x, i, tdst[b / ni]] = complex(Y0, Y1) * SIGMA_SCALE (the scale applied in FP32, as the host
+// prediction applies it)
+__global__ void ws_output_kernel(int64_t L, int64_t ni, int64_t b0, int64_t nb, const float *__restrict__ Y,
+                                 const int32_t *__restrict__ tdst, double2 *__restrict__ S)
+{
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nb * L) return;
+    const int64_t x = q % L, bl = q / L, b = b0 + bl;
+    const int64_t i = b % ni, t = tdst[b / ni];
+    const int64_t NB = nb * L;
+    const float sc = (float)WS_SIGMA_SCALE;
+    S[x + L * (i + ni * t)] = make_double2((double)(Y[q] * sc), (double)(Y[q + NB] * sc));
+}
+
 static int net_check(epsx_ctx *ctx, int32_t precision)
 {
     if (!ctx) return EPSX_ERR_ARG;
@@ -471,6 +513,31 @@ static int upload_gather(epsx_ctx *ctx, const float *x, int64_t first, int64_t n
                        n.length, dptr<int64_t>(n.idx), raw.ptr<float>(), dst);
     EPSX_HIP(ctx, hipGetLastError());
     EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return EPSX_OK;
+}
+
+// The network's prediction Σ[:, :, tdst[z]] from the G0D0 Σ[:, :, tsrc[z]] for the nz slot pairs
+// (device tables tsrc / tdst), all on the device; in chunks of at most 512 samples.
+int net_predict_sigma(epsx_ctx *ctx, int64_t nw, int64_t ni, int nz, const int32_t *tsrc, const int32_t *tdst,
+                      double2 *S, const double *ws, const double *e_i, const double *mu, const double *T, int precision)
+{
+    EPSX_TRY(net_check(ctx, precision));
+    NetDev &n = ctx->net;
+    if (n.c_in != 5 || n.c_out != 2 || n.length != nw)
+        return set_error(ctx, EPSX_ERR_ARG, "the warm-start network takes 5 input and 2 output channels over the %lld "
+                         "nodes of ωs, got %d, %d and %lld", (long long)nw, n.c_in, n.c_out, (long long)n.length);
+    const int64_t total = ni * nz, chunk = 512;
+    EPSX_TRY(net_buffers(ctx, std::min(chunk, total)));
+    for (int64_t b0 = 0; b0 < total; b0 += chunk) {
+        const int64_t nb = std::min(chunk, total - b0);
+        hipLaunchKernelGGL(ws_features_kernel, dim3(nblk(nb * nw)), dim3(256), 0, ctx->stream, nw, ni, b0, nb,
+                           static_cast<const double2 *>(S), ws, e_i, mu, T, tsrc, dptr<float>(n.X[0]));
+        EPSX_HIP(ctx, hipGetLastError());
+        EPSX_TRY(net_forward_dev(ctx, nb, precision == EPSX_PREC_BF16));
+        hipLaunchKernelGGL(ws_output_kernel, dim3(nblk(nb * nw)), dim3(256), 0, ctx->stream, nw, ni, b0, nb,
+                           dptr<float>(n.X[n.layers]), tdst, S);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
     return EPSX_OK;
 }
 

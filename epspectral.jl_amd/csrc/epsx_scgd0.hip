@@ -840,7 +840,23 @@ double now_s()
 
 }  // namespace
 
-extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd0_outputs *out)
+// The learned warm start inside one batched run: slots [0, nTu) carry the plain iteration from
+// -iη_init, slots [nTu, 2 nTu) the warm starts (inactive until the prediction is written after the
+// first iteration, from the G0D0 self-energy of the plain slot).  After the second iteration, the
+// warm start's first, each temperature keeps the slot with the smaller residual: under the common
+// contraction rate of the iteration that is the one fewer iterations away from tol (the plain
+// start whenever the prediction does not cut the remaining iterations, or has no μ root); the
+// other slot is dropped.
+struct WarmHooks {
+    int64_t nTu = 0;
+    int precision = EPSX_PREC_BF16;
+    bool judged = false;
+    std::vector<char> dropped;        // [2 nTu]
+    std::vector<int32_t> kept;        // [nTu] slot kept per temperature
+    std::vector<double> err_pred, err_plain;
+};
+
+static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd0_outputs *out, WarmHooks *wh)
 {
     if (!ctx) return EPSX_ERR_ARG;
     const double t_entry = now_s();
@@ -1069,6 +1085,8 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         EPSX_TRY(tmp_alloc(ctx, d_gates.b, 4 * (size_t)nT * (L.n + 1)));
     }
     std::vector<int32_t> active(nT, 1), act_idx;
+    if (wh)
+        for (int64_t t = wh->nTu; t < nT; t++) active[t] = 0;
     // The star method's FP64 product h·G̃ is a plain strided-batched GEMM with a shared G̃: hipBLASLt
     // runs it (85% of the FP64 peak on the t2g 24³ shape against 58-62% for the library's kernel,
     // which EPSX_STAR_BLAS=0 or EPSX_GEMM_SHAPE selects, and which serves every product with masked
@@ -1448,11 +1466,13 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     // most half of maxiter runs below FP64.
     std::vector<int64_t> lp_iters(nT, 0), lp_flat(nT, 0);
     std::vector<double> first_err(nT, NAN);
+    std::vector<char> joined(nT, 0);   // slots that join the loop after its first iteration
     auto update_phases = [&]() -> int {
         bool changed = false, any_lp = false;
         for (int64_t t = 0; t < nT; t++) {
             if (lvl[t] == L.n - 1) continue;
             any_lp = true;
+            if (joined[t]) { joined[t] = 0; continue; }   // no err of its own yet
             if (lp_iters[t]++ == 0) first_err[t] = h_err[t];
             const bool near = h_err[t] <= 2 * par->tol;
             lp_flat[t] = (h_err[t] > 0.95 * prev_err[t] && h_err[t] < 1e-2 * first_err[t]) ? lp_flat[t] + 1 : 0;
@@ -1656,11 +1676,50 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
             std::copy(slot, slot + nT, active.begin());
             std::copy(slot + nT, slot + 2 * nT, h_status.begin());
             if (timed(iter)) EPSX_TRY(add_event_times((int)nT));
+            if (wh && iter == 2) {
+                // the judge: per temperature, the slot whose residual is smaller goes on (a warm slot
+                // without a μ root is dropped whatever its residual)
+                const int64_t nu = wh->nTu;
+                const double *e = h_es + nT * (iter & 1);
+                std::vector<int32_t> zeros(nT, 0);
+                for (int64_t t = 0; t < nu; t++) {
+                    const int64_t w = nu + t;
+                    wh->err_plain[t] = e[t];
+                    wh->err_pred[t] = h_status[w] == 2 ? INFINITY : e[w];
+                    const bool warm = wh->err_pred[t] < wh->err_plain[t];
+                    const int64_t drop = warm ? t : w;
+                    wh->kept[t] = (int32_t)(warm ? w : t);
+                    wh->dropped[drop] = 1;
+                    active[drop] = 0;
+                    h_status[drop] = 0;
+                    EPSX_HIP(ctx, hipMemcpyAsync(book.act + drop, zeros.data(), 4, hipMemcpyHostToDevice, st));
+                    EPSX_HIP(ctx, hipMemcpyAsync(book.status + drop, zeros.data(), 4, hipMemcpyHostToDevice, st));
+                }
+                EPSX_HIP(ctx, hipStreamSynchronize(st));
+                wh->judged = true;
+            }
             for (int64_t t = 0; t < nT; t++)
                 if (!par->fix_mu && h_status[t] == 2) { rc = no_root(t); break; }
             if (rc != EPSX_OK) break;
+            if (wh && iter == 1) {
+                // the prediction from the G0D0 self-energy of every plain slot into its warm slot, μ
+                // with it; the warm slots join the second iteration
+                const int64_t nu = wh->nTu;
+                std::vector<int32_t> tab(2 * nu), ones(nu, 1);
+                for (int64_t t = 0; t < nu; t++) { tab[t] = (int32_t)t; tab[nu + t] = (int32_t)(nu + t); }
+                Tmp d_tab;
+                EPSX_TRY(tmp_upload(ctx, d_tab, tab.data(), 8 * nu));
+                EPSX_TRY(net_predict_sigma(ctx, nw, ni, (int)nu, d_tab.ptr<int32_t>(), d_tab.ptr<int32_t>() + nu,
+                                           B.S.ptr<double2>(), dptr<double>(s.ws), dptr<double>(s.e_i), B.mu.ptr<double>(),
+                                           B.T.ptr<double>(), wh->precision));
+                EPSX_HIP(ctx, hipMemcpyAsync(B.mu.ptr<double>() + nu, B.mu.ptr<double>(), 8 * nu, hipMemcpyDeviceToDevice, st));
+                EPSX_HIP(ctx, hipMemcpyAsync(book.act + nu, ones.data(), 4 * nu, hipMemcpyHostToDevice, st));
+                EPSX_HIP(ctx, hipStreamSynchronize(st));
+                for (int64_t t = nu; t < nT; t++) active[t] = joined[t] = 1;
+            }
             for (int64_t t = 0; side_ok && t < nT; t++) {
                 if (active[t] || sent[t]) continue;
+                if (wh && (!wh->judged || wh->dropped[t])) continue;
                 EPSX_HIP(ctx, hipStreamWaitEvent(ctx->side, ev_read[iter & 1], 0));
                 EPSX_HIP(ctx, hipMemcpyAsync(static_cast<char *>(ctx->pin_side) + slice * t, B.S.ptr<char>() + slice * t,
                                              slice, hipMemcpyDeviceToHost, ctx->side));
@@ -1712,6 +1771,82 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     for (int64_t t = 0; t < nT; t++) {
         out->mu[t] = mu_host[t];
         out->converged[t] = !active[t];
+    }
+    return EPSX_OK;
+}
+
+extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd0_outputs *out)
+{
+    if (!ctx) return EPSX_ERR_ARG;
+    if (!par || par->struct_size != (int64_t)sizeof(epsx_scgd0_params) || !out ||
+        out->struct_size != (int64_t)sizeof(epsx_scgd0_outputs))
+        return set_error(ctx, EPSX_ERR_ARG, "epsx_scgd0_run: struct size mismatch (ABI %d)", EPSX_ABI_VERSION);
+    if (!par->warm_start) {
+        if (out->warm_kept)
+            for (int64_t t = 0; t < par->nT; t++) out->warm_kept[t] = 0;
+        return scgd0_run_impl(ctx, par, out, nullptr);
+    }
+    // the warm start: the run over 2 nT slots (plain, then warm), its results composed per
+    // temperature from the slot kept
+    const int64_t nT = par->nT, maxiter = par->maxiter;
+    if (nT < 1 || !par->Tlist || !par->nlist || !par->mu_start || !par->eta_min)
+        return set_error(ctx, EPSX_ERR_ARG, "scgd0 run: nT must be positive with Tlist, nlist, μ_start and η_min given");
+    if (par->Sigma_init) return set_error(ctx, EPSX_ERR_ARG, "the warm start replaces Σ_init: pass one or the other");
+    if (!par->batched) return set_error(ctx, EPSX_ERR_ARG, "the warm start runs in the batched loop (batched = 1)");
+    if (maxiter < 3) return set_error(ctx, EPSX_ERR_ARG, "the warm start needs maxiter >= 3, got %lld", (long long)maxiter);
+    if (par->warm_start != EPSX_PREC_BF16 && par->warm_start != EPSX_PREC_FP32)
+        return set_error(ctx, EPSX_ERR_ARG, "warm_start must be 0, EPSX_PREC_FP32 or EPSX_PREC_BF16, got %d",
+                         par->warm_start);
+    if (!ctx->net.ready) return set_error(ctx, EPSX_ERR_STATE, "the warm start needs a network: call epsx_net_create");
+    if (!out->Sigma || !out->mu || !out->converged || !out->hist_err || !out->hist_mu || !out->hist_minA ||
+        !out->wall || !out->iter_frozen)
+        return set_error(ctx, EPSX_ERR_ARG, "scgd0 run: null output array");
+    const int64_t n2 = 2 * nT, nS = 16 * ctx->s.nw * ctx->s.ni;
+    auto dup = [&](const double *x) {
+        std::vector<double> v(n2);
+        for (int64_t t = 0; t < n2; t++) v[t] = x[t % nT];
+        return v;
+    };
+    std::vector<double> T2 = dup(par->Tlist), nl2 = dup(par->nlist), mu2 = dup(par->mu_start), eta2 = dup(par->eta_min);
+    epsx_scgd0_params p2 = *par;
+    p2.nT = n2;
+    p2.Tlist = T2.data(); p2.nlist = nl2.data(); p2.mu_start = mu2.data(); p2.eta_min = eta2.data();
+    p2.warm_start = 0;
+    std::vector<char> S2((size_t)nS * n2);
+    std::vector<double> m2(n2), he(n2 * maxiter), hm(n2 * maxiter), ha(n2 * maxiter);
+    std::vector<int32_t> c2(n2);
+    std::vector<int64_t> f2(n2);
+    epsx_scgd0_outputs o2 = *out;
+    o2.Sigma = reinterpret_cast<double *>(S2.data());
+    o2.mu = m2.data(); o2.converged = c2.data(); o2.hist_err = he.data(); o2.hist_mu = hm.data();
+    o2.hist_minA = ha.data(); o2.iter_frozen = f2.data(); o2.warm_kept = nullptr;
+    WarmHooks wh;
+    wh.nTu = nT;
+    wh.precision = par->warm_start;
+    wh.dropped.assign(n2, 0);
+    wh.kept.assign(nT, 0);
+    wh.err_pred.assign(nT, NAN);
+    wh.err_plain.assign(nT, NAN);
+    const int rc = scgd0_run_impl(ctx, &p2, &o2, &wh);
+    if (rc != EPSX_OK) return rc;
+    if (!wh.judged)   // the run ended before the judge (maxiter or all converged at once): plain
+        for (int64_t t = 0; t < nT; t++) wh.kept[t] = (int32_t)t;
+    out->niter = o2.niter;
+    out->fm_method_used = o2.fm_method_used;
+    out->n_low_precision = o2.n_low_precision;
+    for (int64_t t = 0; t < nT; t++) {
+        const int64_t k = wh.kept[t];
+        std::memcpy(reinterpret_cast<char *>(out->Sigma) + nS * t, S2.data() + nS * k, nS);
+        out->mu[t] = m2[k];
+        out->converged[t] = c2[k];
+        out->iter_frozen[t] = f2[k];
+        for (int64_t it = 0; it < maxiter; it++) {
+            const int64_t src = it == 0 ? t : k;   // iteration 1: the G0D0 of the plain slot
+            out->hist_err[t + nT * it] = he[src + n2 * it];
+            out->hist_mu[t + nT * it] = hm[src + n2 * it];
+            out->hist_minA[t + nT * it] = ha[src + n2 * it];
+        }
+        if (out->warm_kept) out->warm_kept[t] = k >= nT;
     }
     return EPSX_OK;
 }

@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libepsx.so")
 
-EPSX_ABI_VERSION = 1
+EPSX_ABI_VERSION = 2
 EPSX_OK, EPSX_ERR_ARG, EPSX_ERR_NO_MU_ROOT, EPSX_ERR_DEVICE, EPSX_ERR_OOM, EPSX_ERR_STATE = range(6)
 FM_METHODS = {"direct": 0, "einstein": 1, "star": 2, "auto": 3}
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2}
@@ -115,7 +115,7 @@ class Scgd0Params(C.Structure):
         ("maxiter", I64), ("f_tile", I64),
         ("fm_method", I32), ("precision", I32), ("batched", I32), ("verbosity", I32),
         ("Sigma_init", P),
-        ("reuse_star", I32), ("pad", I32),
+        ("reuse_star", I32), ("warm_start", I32),
     ]
 
 
@@ -126,6 +126,7 @@ class Scgd0Outputs(C.Structure):
         ("hist_err", P), ("hist_mu", P), ("hist_minA", P), ("wall", P), ("iter_frozen", P),
         ("niter", I64),
         ("fm_method_used", I32), ("n_low_precision", I32),
+        ("warm_kept", P),
     ]
 
 

@@ -216,51 +216,25 @@ def make_training_set(ctx, n_models, nk=12, length=1024, seed=0, states_per_mode
 
 def run_warm_started(dev, setup, occ, net, *, η_init=5e-3, tol=1e-6, maxiter=300, method="auto",
                      precision="bf16"):
-    """scGD0 to tol from the network's prediction, with the fallback to the plain start.
+    """scGD0 to tol from the network's prediction, with the fallback to the plain start, in one
+    batched run on the device (`Scgd0Device.loop(..., warm_start=precision)`):
 
-    1. one iteration from -iη_init gives Σ_G0D0 and μ (counted);
-    2. the prediction Σ_p; one iteration from Σ_p and one from Σ_G0D0, run together as a batch
-       of 2 nT instances (both counted), give the residuals err_p and err_0: per temperature the
-       start with the smaller residual is kept (the plain start whenever the prediction does not
+    1. the first iteration from -iη_init gives Σ_G0D0 and μ;
+    2. the network (in the same context as `dev`) predicts Σ_p from Σ_G0D0 into a second set of
+       slots; the second iteration runs both the plain iterate and Σ_p, and per temperature the
+       slot with the smaller residual goes on (the plain start whenever the prediction does not
        cut the residual, or when no μ satisfies the electron count for it);
-    3. the loop continues from the kept iterate to tol.
-    The runs after the first reuse the star method's coupling reduction.
+    3. the loop continues from the kept slot to tol.
     Returns dict(Σ, μ, converged, iterations, fallback[T], wall)."""
-    import dataclasses
     from .scgd0 import OMEGA_ACOUSTIC
+    if net.ctx is not dev.ctx:
+        raise ArgumentError("the network and the scGD0 setup must live in the same context")
     kw = dict(η_min=0.0, η_init=η_init, ω_acoustic=OMEGA_ACOUSTIC, method=method, batched=True)
     t0 = time.perf_counter()
-    nT = len(occ)
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
-        r1 = dev.loop(occ, maxiter=1, tol=0.0, **kw)
-        Σ0, μ0 = r1["Σ"], r1["μ"]
-        t1 = time.perf_counter()
-        Σp = np.empty_like(Σ0)
-        for iT in range(nT):
-            Σp[:, :, iT] = predict_sigma(net, Σ0[:, :, iT], setup.ωs, setup.ε_i, μ0[iT], occ.Tlist[iT], precision)
-        t2 = time.perf_counter()
-        occ2 = dataclasses.replace(occ, Tlist=np.concatenate([occ.Tlist, occ.Tlist]),
-                                   nlist=np.concatenate([occ.nlist, occ.nlist]),
-                                   μlist=None if occ.μlist is None else np.concatenate([occ.μlist, occ.μlist]))
-        try:
-            r2 = dev.loop(occ2, maxiter=1, tol=0.0, Σ_init=np.concatenate([Σp, Σ0], axis=2),
-                          μ_start=np.concatenate([μ0, μ0]), reuse_star=True, **kw)
-            ep, eq = r2["history"]["err"][:nT, 0], r2["history"]["err"][nT:, 0]
-            Σ2, μ2 = r2["Σ"], r2["μ"]
-        except ArgumentError:
-            # a prediction so far off that no μ holds the electron count: rejected
-            r2 = dev.loop(occ, maxiter=1, tol=0.0, Σ_init=Σ0, μ_start=μ0, reuse_star=True, **kw)
-            eq = r2["history"]["err"][:, 0]
-            ep = np.full(nT, np.inf)
-            Σ2, μ2 = np.concatenate([r2["Σ"], r2["Σ"]], axis=2), np.concatenate([r2["μ"], r2["μ"]])
-        keep = ep < eq
-        start = np.where(keep[None, None, :], Σ2[:, :, :nT], Σ2[:, :, nT:])
-        μs = np.where(keep, μ2[:nT], μ2[nT:])
-        t3 = time.perf_counter()
-        rf = dev.loop(occ, maxiter=maxiter, tol=tol, Σ_init=start, μ_start=μs, reuse_star=True, **kw)
-    t4 = time.perf_counter()
-    n_final = rf["history"]["err"].shape[1]
-    return dict(Σ=rf["Σ"], μ=rf["μ"], converged=rf["converged"], iterations=1 + 2 + n_final,
-                iterations_final=n_final, fallback=~keep, err_pred=ep, err_plain=eq, wall=t4 - t0,
-                phases=dict(g0d0=t1 - t0, predict=t2 - t1, judge=t3 - t2, final=t4 - t3))
+        rf = dev.loop(occ, maxiter=maxiter, tol=tol, warm_start=precision, **kw)
+    wall = time.perf_counter() - t0
+    n = rf["history"]["err"].shape[1]
+    return dict(Σ=rf["Σ"], μ=rf["μ"], converged=rf["converged"], iterations=n,
+                fallback=~rf["info"]["warm_kept"], history=rf["history"], wall=wall)

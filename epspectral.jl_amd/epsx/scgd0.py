@@ -169,8 +169,13 @@ class Scgd0Device:
 
     def loop(self, occ: Occupation, *, η_min, η_init, maxiter, tol, gpu_tile=None, Σ_init=None,
              fix_μ=False, ω_acoustic=OMEGA_ACOUSTIC, verbosity=0, method="direct",
-             precision="fp64", batched=False, μ_start=None, reuse_star=False):
-        """`_loop_scgd0` (src/scgd0.jl:189-346) on the device."""
+             precision="fp64", batched=False, μ_start=None, reuse_star=False, warm_start=None):
+        """`_loop_scgd0` (src/scgd0.jl:189-346) on the device.
+
+        warm_start: None, or the precision ("bf16" / "fp32") of the context's network (set up with
+        epsx.learn.WarmStartNet), which predicts Σ_init from the G0D0 self-energy of the first
+        iteration on the device; each temperature goes on from the predicted or the plain start,
+        whichever has the smaller residual after one more iteration (info["warm_kept"])."""
         s = self.setup
         nm, ni, nf = s.dims
         nω, nT = len(s.ωs), len(occ)
@@ -202,6 +207,7 @@ class Scgd0Device:
         par.batched = int(bool(batched))
         par.verbosity = int(verbosity)
         par.Sigma_init = ptr(Σ_init)
+        par.warm_start = 0 if warm_start is None else _lib.PRECISIONS[warm_start]
         Σ = np.empty((nω, ni, nT), dtype=np.complex128, order="F")
         μ = np.empty(nT)
         conv = np.empty(nT, dtype=np.int32)
@@ -215,6 +221,8 @@ class Scgd0Device:
         out.Sigma, out.mu, out.converged = ptr(Σ), ptr(μ), ptr(conv)
         out.hist_err, out.hist_mu, out.hist_minA = ptr(herr), ptr(hmu), ptr(hmin)
         out.wall, out.iter_frozen = ptr(wall), ptr(frozen)
+        kept = np.zeros(nT, dtype=np.int32)
+        out.warm_kept = ptr(kept)
         self.ctx.check(_lib.lib().epsx_scgd0_run(self.ctx.handle, ctypes_ref(par), ctypes_ref(out)))
         n = int(out.niter)
         converged = conv.astype(bool)
@@ -224,7 +232,7 @@ class Scgd0Device:
         history = dict(err=herr[:, :n].copy(), μ=hmu[:, :n].copy(), min_spectral_weight=hmin[:, :n].copy(),
                        wall=wall[:n].copy(), iter_frozen=frozen.copy())
         info = dict(fm_method=int(out.fm_method_used), n_low_precision=int(out.n_low_precision),
-                    timings=self.ctx.timings())
+                    warm_kept=kept.astype(bool), timings=self.ctx.timings())
         return dict(Σ=Σ, μ=μ, converged=converged, history=history, info=info)
 
 

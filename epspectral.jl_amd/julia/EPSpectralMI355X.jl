@@ -92,7 +92,7 @@ struct Scgd0Params
     maxiter::Int64; f_tile::Int64
     fm_method::Int32; precision::Int32; batched::Int32; verbosity::Int32
     Sigma_init::Ptr{ComplexF64}
-    reuse_star::Int32; pad::Int32
+    reuse_star::Int32; warm_start::Int32
 end
 
 mutable struct Scgd0Outputs
@@ -102,6 +102,7 @@ mutable struct Scgd0Outputs
     iter_frozen::Ptr{Int64}
     niter::Int64
     fm_method_used::Int32; n_low_precision::Int32
+    warm_kept::Ptr{Int32}
 end
 
 struct LadderProblem

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define EPSX_ABI_VERSION 1
+#define EPSX_ABI_VERSION 2
 
 #define EPSX_OK 0
 #define EPSX_ERR_ARG 1          /* invalid argument: ArgumentError in the reference */
@@ -180,7 +180,14 @@ typedef struct {
     const double *Sigma_init; /* NULL (start at -iη_init) or complex [nw, ni, nT] */
     int32_t reuse_star;       /* 1: keep the star method's G̃ from the previous run on this
                                  problem when it was built with the same w_acoustic (0: rebuild) */
-    int32_t pad;
+    int32_t warm_start;       /* 0, or the precision (EPSX_PREC_BF16 / _FP32) of the context's
+                                 network (epsx_net_*): no counterpart in the reference.  Σ_init is
+                                 predicted on the device from the G0D0 self-energy of the first
+                                 iteration; the second iteration runs the plain and the predicted
+                                 start side by side, and each temperature goes on from the one with
+                                 the smaller residual (fewer iterations from tol), so the loop always
+                                 runs to the same tol.  Needs batched = 1, Sigma_init = NULL,
+                                 maxiter >= 3 and a network over the nω nodes of ωs. */
 } epsx_scgd0_params;
 
 typedef struct {
@@ -196,6 +203,7 @@ typedef struct {
     int64_t niter;            /* out: iterations run */
     int32_t fm_method_used;   /* out: EPSX_FM_* actually run */
     int32_t n_low_precision;  /* out: iterations run below FP64 */
+    int32_t *warm_kept;       /* NULL, or [nT]: 1 where the warm start was kept */
 } epsx_scgd0_outputs;
 
 /* Upload the problem (replaces the previous one). */

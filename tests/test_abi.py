@@ -40,7 +40,7 @@ def test_struct_layouts():
     # int64 + 6 int64 + 10 pointers + 2 doubles + (int64, 2 doubles, ptr) x2 + int64 + 2 ptrs
     assert ctypes.sizeof(_lib.Scgd0Problem) == 8 * (1 + 6 + 10 + 2 + 4 + 4 + 3)
     assert ctypes.sizeof(_lib.Scgd0Params) == 8 * (1 + 1 + 4 + 2) + 4 * 2 + 8 * 3 + 8 * 2 + 4 * 4 + 8 + 4 * 2
-    assert ctypes.sizeof(_lib.Scgd0Outputs) == 8 * (1 + 8 + 1) + 4 * 2
+    assert ctypes.sizeof(_lib.Scgd0Outputs) == 8 * (1 + 8 + 1) + 4 * 2 + 8
 
 
 def test_host_entry_points_without_device():

@@ -4,6 +4,7 @@ oracle (oracle/net_oracle.py), and the warm start's gain on scGD0.
 Tolerances (relative, in the 2-norm over all outputs / all gradient entries): FP32 MFMA path
 1e-5 (forward) and 1e-4 (gradient); BF16 operands with FP32 accumulation 3e-2 and 6e-2.
 """
+import dataclasses
 import warnings
 
 import numpy as np
@@ -11,7 +12,9 @@ import pytest
 
 import epsx
 import net_oracle
-from epsx.learn import WarmStartNet, make_training_set, random_t2g_model, g0d0_and_converged, run_warm_started
+from epsx.learn import (WarmStartNet, g0d0_and_converged, make_training_set, predict_sigma, random_t2g_model,
+                        run_warm_started)
+from epsx.scgd0 import OMEGA_ACOUSTIC
 
 pytestmark = pytest.mark.gpu
 
@@ -101,6 +104,7 @@ def test_warm_start_cuts_iterations(ctx):
             n_plain = rc["history"]["err"].shape[1]
             w = run_warm_started(dev, setup, occ, net)
             assert w["converged"].all() and rc["converged"].all()
+            check_judge(dev, setup, occ, net, w)
             # the same fixed point to within the tolerance scale
             assert np.max(np.abs(w["Σ"] - rc["Σ"])) < 1e-4
             gains.append((n_plain, w["iterations"], bool(w["fallback"][0])))
@@ -115,3 +119,28 @@ def test_warm_start_cuts_iterations(ctx):
         occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
         w = run_warm_started(dev, setup, occ, net)
         assert w["fallback"].all() and w["converged"].all()
+        check_judge(dev, setup, occ, net, w)
+
+
+def check_judge(dev, setup, occ, net, w):
+    """The device warm start against the same steps run from the host: the first iteration from
+    -iη_init, the prediction of `predict_sigma` on the host features, the second iteration of the
+    predicted and the plain start as one batch of 2 nT; the device run keeps the slot with the
+    smaller residual, and its second history entry is that residual."""
+    kw = dict(η_min=0.0, η_init=5e-3, ω_acoustic=OMEGA_ACOUSTIC, method="auto", batched=True)
+    nT = len(occ)
+    r1 = dev.loop(occ, maxiter=1, tol=0.0, **kw)
+    Σp = np.stack([predict_sigma(net, r1["Σ"][:, :, t], setup.ωs, setup.ε_i, r1["μ"][t], occ.Tlist[t], "bf16")
+                   for t in range(nT)], axis=2)
+    occ2 = dataclasses.replace(occ, Tlist=np.concatenate([occ.Tlist, occ.Tlist]),
+                               nlist=np.concatenate([occ.nlist, occ.nlist]), μlist=None)
+    try:
+        r2 = dev.loop(occ2, maxiter=1, tol=0.0, Σ_init=np.concatenate([Σp, r1["Σ"]], axis=2),
+                      μ_start=np.concatenate([r1["μ"], r1["μ"]]), **kw)
+        ep, eq = r2["history"]["err"][:nT, 0], r2["history"]["err"][nT:, 0]
+    except epsx.ArgumentError:   # no μ root for the prediction
+        r2 = dev.loop(occ, maxiter=1, tol=0.0, Σ_init=r1["Σ"], μ_start=r1["μ"], **kw)
+        ep, eq = np.full(nT, np.inf), r2["history"]["err"][:, 0]
+    assert np.array_equal(~w["fallback"], ep < eq), (ep, eq)
+    np.testing.assert_allclose(w["history"]["err"][:, 0], r1["history"]["err"][:, 0], rtol=1e-12)
+    np.testing.assert_allclose(w["history"]["err"][:, 1], np.minimum(ep, eq), rtol=1e-5)
