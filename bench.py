@@ -133,7 +133,7 @@ def main():
         if args.precision == "fp64":
             own = os.environ.get("EPSX_STAR_BLAS", "1") == "0" or os.environ.get("EPSX_GEMM_SHAPE")
             peak = FP64_PEAK_TFLOPS
-            kname = ("dgemm_kernel (star Fan-Migdal GEMM, FP64 MFMA 16x16x4)" if own
+            kname = ("dgemm_buf_kernel (star Fan-Migdal GEMM, FP64 MFMA 16x16x4)" if own
                      else "hipBLASLt FP64 GEMM (star Fan-Migdal product, strided batch with G̃ shared)")
             tr = traffic("dgemm_kernel" if own else "hipblaslt_star", args.nk, nT)
         else:
@@ -161,7 +161,7 @@ def main():
         achieved = flops / fm_avg / 1e12
         peak = {"fp64": FP64_PEAK_TFLOPS, "fp32": FP32_PEAK_TFLOPS, "bf16": BF16_PEAK_TFLOPS}[args.precision]
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": None, "kernel": "Einstein Fan-Migdal pass: einstein_h_kernel + dgemm_kernel (FP64 MFMA "
+                "traffic": None, "kernel": "Einstein Fan-Migdal pass: einstein_h_kernel + dgemm_buf_kernel (FP64 MFMA "
                 "16x16x4, H' g2 batched over temperatures)", "algorithmic_flop_per_launch": flops,
                 "avg_launch_s": fm_avg, "temperatures_per_launch": per_launch}
         extra = {}
@@ -330,7 +330,7 @@ def ladder_gemm_label(precision):
     """The kernel that runs the shift-table GEMM W = J·B̃: rocBLAS (a plain GEMM), or the library's
     own MFMA kernels with EPSX_LADDER_BLAS=0."""
     if os.environ.get("EPSX_LADDER_BLAS", "1") == "0":
-        return ("dgemm_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)" if precision == "fp64"
+        return ("dgemm_buf_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)" if precision == "fp64"
                 else f"lpkgemm_kernel (shift-table GEMM W = J B̃ from stored {precision} operands, FP32 accumulation)")
     return {"fp64": "rocBLAS dgemm_strided_batched (shift-table GEMM W = J B̃, every shift of a column block)",
             "fp32": "rocBLAS gemm_strided_batched_ex FP32 (shift-table GEMM W = J B̃ from stored FP32 operands)",
@@ -387,12 +387,12 @@ def warmstart_bench(args):
     L = 1024
     warnings.simplefilter("ignore")
     t0 = time.perf_counter()
-    x, t = make_training_set(ctx, args.models, nk=12, length=L, seed=2024, states_per_model=96)
+    x, t = make_training_set(ctx, args.models, nk=12, length=L, seed=2024, states_per_model=96, verbosity=1)
     t_data = time.perf_counter() - t0
     print(f"[bench] training set: {len(x)} samples from {args.models} models ({t_data:.1f}s)", file=sys.stderr, flush=True)
     net = WarmStartNet(ctx, L, seed=0)
     t0 = time.perf_counter()
-    losses = net.train(x, t, epochs=args.epochs, batch=64, lr=1e-3, precision="bf16", seed=0)
+    losses = net.train(x, t, epochs=args.epochs, batch=64, lr=1e-3, precision="bf16", seed=0, verbosity=1)
     t_train = time.perf_counter() - t0
     print(f"[bench] trained {args.epochs} epochs in {t_train:.1f}s, loss {losses[0]:.3e} -> {losses[-1]:.3e}",
           file=sys.stderr, flush=True)
@@ -411,6 +411,7 @@ def warmstart_bench(args):
         t0 = time.perf_counter()
         rc = dev.loop(occ, maxiter=300, tol=1e-6, **kw)
         t_plain = time.perf_counter() - t0
+        run_warm_started(dev, setup, occ, net, maxiter=3, tol=0.0)   # prime the warm-start path
         w = run_warm_started(dev, setup, occ, net)
         dΣ = float(np.max(np.abs(w["Σ"] - rc["Σ"])))
         rows.append(dict(plain_iterations=int(rc["history"]["err"].shape[1]), warm_iterations=int(w["iterations"]),

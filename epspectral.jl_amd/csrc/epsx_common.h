@@ -131,7 +131,7 @@ struct epsx_ctx {
     // this device's copy of the fixed workspace of epsx_kramers_kronig_noalloc (epsx_kk.hip),
     // looked up when the context opens
     double *kk_scratch = nullptr;
-    std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, split) -> tile shape
+    std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, caller workspace or -1) -> tile shape
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
     double t_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};

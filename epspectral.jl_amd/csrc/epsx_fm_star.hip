@@ -188,13 +188,27 @@ __device__ inline double h_elem(const HArgs &a, int64_t w, int64_t nu, int64_t i
     return h_from(a, h_const(a, w, nu, tcol, z), ip);
 }
 
+// Dispatch sends workgroup b of a launch to XCD b mod 8, and each XCD has its own L2.  With the
+// x grid a multiple of 8, block b works as the logical block (b mod 8) gridDim.x / 8 + b / 8, so
+// each XCD runs one contiguous eighth of the logical blocks: the blocks that read the same Σ
+// columns (all (ω, ν) of a group of states) share them in one L2 instead of fetching them 8 times.
+__device__ inline int64_t xcd_major_block()
+{
+    return (int64_t)(blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+}
+
+static inline unsigned xcd_grid(int64_t threads)
+{
+    return (unsigned)(((threads + 255) / 256 + 7) / 8 * 8);
+}
+
 // h[w + nw (ν + nm i')] per slot, FP64.  One thread per (ω, ν) and HB_R consecutive outer states:
 // the (ω, ν) constants are read once and the HB_R independent Σ reads are in flight together
 // (lanes along ω, so every read and write is contiguous across a wave).
 constexpr int HB_R = 4;
 __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
 {
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t q = xcd_major_block() * blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
     const int64_t nw = a.nw, ni = a.ni, nwm = nw * a.nm, ngrp = (ni + HB_R - 1) / HB_R;
     if (q >= nwm * ngrp) return;
@@ -229,7 +243,7 @@ constexpr int EH_R = 4;
 __global__ __launch_bounds__(256) void einstein_h_kernel(HArgs a, int64_t p0, int64_t np, const double *__restrict__ ef,
                                                          const double *__restrict__ wf, const int32_t *__restrict__ fi)
 {
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t q = xcd_major_block() * blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
     const int64_t nw = a.nw, nwm = nw * a.nm, ngrp = (np + EH_R - 1) / EH_R;
     if (q >= nwm * ngrp) return;
@@ -476,8 +490,7 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
         HArgs ah = a;
         ah.act = act_h;
         const int64_t threads = nw * nm * ((ni + HB_R - 1) / HB_R);
-        hipLaunchKernelGGL(h_build_kernel, dim3((unsigned)((threads + 255) / 256), (unsigned)nslots), dim3(256), 0,
-                           ctx->stream, ah);
+        hipLaunchKernelGGL(h_build_kernel, dim3(xcd_grid(threads), (unsigned)nslots), dim3(256), 0, ctx->stream, ah);
         EPSX_HIP(ctx, hipGetLastError());
     }
     // the transposed low-precision layouts: blocks of HL_IP states until nm ip covers kp
@@ -521,8 +534,8 @@ int einstein_h_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const d
     a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht; a.wnu = wnu; a.S = S; a.eta = eta;
     a.tidx = tidx; a.w_acoustic = w_acoustic; a.occ = const_cast<double *>(occ); a.hout = hout; a.act = act;
     const int64_t threads = nw * nm * ((np + EH_R - 1) / EH_R);
-    hipLaunchKernelGGL(einstein_h_kernel, dim3((unsigned)((threads + 255) / 256), (unsigned)nslots), dim3(256), 0,
-                       ctx->stream, a, p0, np, ef, wf, fi);
+    hipLaunchKernelGGL(einstein_h_kernel, dim3(xcd_grid(threads), (unsigned)nslots), dim3(256), 0, ctx->stream, a, p0,
+                       np, ef, wf, fi);
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
 }

@@ -213,6 +213,136 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_kernel(GemmArgs p)
             }
 }
 
+// The same tiles with the operands read by buffer loads, LDS double-buffered (one barrier per
+// k-tile of GKT = 16) and the global loads of tile t + 2 issued while tile t multiplies:
+//   * each workgroup's buffer descriptors start at its unit's first row / column and first k, and
+//     end at the operand's last element (A) or at the unit's last column (B), so the hardware's
+//     range check returns 0 for k >= k (A) and for columns >= n (B); rows of A past m and k rows of
+//     B past k inside a column read other elements of the operand, which only feed discarded
+//     outputs (rows >= m) or are zeroed (the partial last k-tile masks B);
+//   * offsets are a per-thread voffset fixed for the kernel plus a scalar soffset per load and
+//     tile, so the loop carries no per-load address arithmetic or bounds checks;
+//   * pitches: A rows of BM + 16 doubles (the four k rows an MFMA reads fall on two bank halves),
+//     B rows of BN + 2 (the 16 k rows a wave writes fall on distinct banks).
+// The MFMA k slots take the rows ks .. ks + 3 of each tile in ascending order, as in
+// dgemm_kernel, and the split bounds are the same 32-wide tiles: the bits do not depend on the
+// kernel or the tile shape.  The host takes this kernel only when every unit's descriptor span
+// fits 31 bits (buffer_fits).
+template <int BM, int BN, int WMW, int WNW, int MINW>
+__global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_buf_kernel(GemmArgs p)
+{
+    constexpr int GKT = 16, PA = BM + 16, PB = BN + 2;
+    constexpr int T = 64 * WMW * WNW;
+    constexpr int TM = BM / WMW / 16, TN = BN / WNW / 16;
+    constexpr int QA = BM * GKT / T, QB = GKT * BN / T;
+    static_assert(T % BM == 0 && T % GKT == 0 && QA * T == BM * GKT && QB * T == GKT * BN, "thread layout");
+    constexpr int KA = T / BM, CB = T / GKT;   // k rows of A / columns of B per load round
+    int tm, tn, s, S;
+    int64_t z;
+    if (!unit_of_block(p, tm, tn, s, z, S)) return;
+    constexpr int R = GK / GKT;
+    const int kt0 = R * (int)((int64_t)p.ktiles * s / S), kt1 = R * (int)((int64_t)p.ktiles * (s + 1) / S);
+    extern __shared__ double gemm_lds[];
+    double *const As = gemm_lds;                 // [2][GKT][PA]
+    double *const Bs = gemm_lds + 2 * GKT * PA;  // [2][GKT][PB]
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WMW, wn = wave / WMW;
+    const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
+    const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
+    const int64_t ks0 = (int64_t)kt0 * GKT, ke = (int64_t)kt1 * GKT < k ? (int64_t)kt1 * GKT : k;
+    const int64_t cl = col0 + BN < n ? col0 + BN : n;   // end of the unit's columns
+    const double *A = p.A + p.sA * opi(p, z, 1) + row0 + lda * ks0;
+    const double *B = p.B + p.sB * opi(p, z, 2) + ks0 + ldb * col0;
+    const int64_t extA = ke > ks0 ? (m - row0) + lda * (ke - 1 - ks0) : 0;
+    const int64_t extB = ke > ks0 ? (ke - ks0) + ldb * (cl - 1 - col0) : 0;
+    const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc((void *)A, (short)0, (int)(8 * extA), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc((void *)B, (short)0, (int)(8 * extB), 0x00020000);
+    const int rA = tid % BM, kA = tid / BM;     // A element (row0 + rA, k0 + kA + KA q)
+    const int kB = tid % GKT, cB = tid / GKT;   // B element (k0 + kB, col0 + cB + CB q)
+    const uint32_t voA = (uint32_t)(8 * (rA + lda * kA)), voB = (uint32_t)(8 * (kB + ldb * cB));
+    const uint32_t stepA = (uint32_t)(8 * lda * KA), tileA = (uint32_t)(8 * lda * GKT), stepB = (uint32_t)(8 * ldb * CB);
+    double ra[QA], rb[QB];
+    auto load = [&](int kt) {
+        const uint32_t kr = (uint32_t)(kt - kt0);
+        const uint32_t soA = tileA * kr, soB = 8u * GKT * kr;
+#pragma unroll
+        for (int q = 0; q < QA; q++) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsA, voA, soA + stepA * q, 0);
+            __builtin_memcpy(&ra[q], &v, 8);
+        }
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsB, voB, soB + stepB * q, 0);
+            __builtin_memcpy(&rb[q], &v, 8);
+        }
+        if ((int64_t)(kt + 1) * GKT > k && (int64_t)kt * GKT + kB >= k) {   // B rows past k: not zero in memory
+#pragma unroll
+            for (int q = 0; q < QB; q++) rb[q] = 0.0;
+        }
+    };
+    double *const wA = As + kA * PA + rA, *const wB = Bs + kB * PB + cB;
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < QA; q++) wA[buf * GKT * PA + KA * q * PA] = ra[q];
+#pragma unroll
+        for (int q = 0; q < QB; q++) wB[buf * GKT * PB + CB * q] = rb[q];
+    };
+    double4_t acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++) acc[a][b] = (double4_t){0, 0, 0, 0};
+    if (kt0 < kt1) {
+        load(kt0);
+        stage(0);
+        if (kt0 + 1 < kt1) load(kt0 + 1);
+    }
+    __syncthreads();
+    const int mb = wm * (BM / WMW), nbase = wn * (BN / WNW);
+    // A[i][k] of a 16x4 block sits in lane i + 16 k, B[k][j] in lane j + 16 k
+    const double *const rdA = As + (lane >> 4) * PA + mb + (lane & 15);
+    const double *const rdB = Bs + (lane >> 4) * PB + nbase + (lane & 15);
+    for (int kt = kt0; kt < kt1; kt++) {
+        const int cur = (kt - kt0) & 1;
+        const bool more = kt + 1 < kt1;
+#pragma unroll
+        for (int ks = 0; ks < GKT; ks += 4) {
+            double af[TM], bf[TN];
+#pragma unroll
+            for (int a = 0; a < TM; a++) af[a] = rdA[cur * GKT * PA + ks * PA + a * 16];
+#pragma unroll
+            for (int b = 0; b < TN; b++) bf[b] = rdB[cur * GKT * PB + ks * PB + b * 16];
+#pragma unroll
+            for (int a = 0; a < TM; a++)
+#pragma unroll
+                for (int b = 0; b < TN; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+            if (ks == 0 && more) {
+                // tile kt + 1 (in registers since the last tile) into the other buffer, whose
+                // readers all passed the last barrier; then the loads of tile kt + 2
+                stage(cur ^ 1);
+                if (kt + 2 < kt1) load(kt + 2);
+            }
+        }
+        __syncthreads();
+    }
+    // D[i][j] of a 16x16 f64 tile: element e of lane l holds i = 4 e + l / 16, j = l % 16
+    const int64_t zc = opi(p, z, 4);
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t gi = row0 + mb + a * 16 + 4 * e + (lane >> 4);
+                const int64_t gj = col0 + nbase + b * 16 + (lane & 15);
+                if (gi < m && gj < n) {
+                    if (S == 1) store_out(p, zc, gi, gj, acc[a][b][e]);
+                    else p.part[gi + m * gj + m * n * (s + (int64_t)S * z)] = acc[a][b][e];
+                }
+            }
+}
+
 // C = Σ_s part_s in ascending s (then + C if accumulating), or the complex epilogue.
 __global__ __launch_bounds__(256) void dgemm_splitk_reduce_kernel(GemmArgs p)
 {
@@ -477,10 +607,33 @@ __global__ __launch_bounds__(256, 2) void lpkgemm_kernel(GemmArgs p)
             }
 }
 
-// The tile shapes the autotuner chooses from: (BM, BN, waves along m, waves along n)
-struct GemmShape { int bm, bn, wmw, wnw; };
-constexpr GemmShape SHAPES[] = {{64, 64, 2, 2}, {128, 64, 4, 2}, {64, 128, 2, 2}, {128, 128, 4, 2}, {128, 128, 2, 2}};
-constexpr int NSHAPES = 5;
+// The tile shapes the autotuner chooses from: (BM, BN, waves along m, waves along n, workgroups
+// one CU holds by LDS and registers).  The thin 256x32 and 128x32 tiles give the products whose
+// m is the ~1300-node frequency grid about one workgroup per CU without a k split.
+struct GemmShape { int bm, bn, wmw, wnw, per_cu; };
+constexpr GemmShape SHAPES[] = {{64, 64, 2, 2, 4},  {128, 64, 2, 2, 3}, {64, 128, 2, 2, 3}, {128, 128, 2, 2, 2},
+                                {256, 128, 4, 2, 1}, {256, 32, 4, 1, 2}, {128, 32, 4, 1, 3}};
+constexpr int NSHAPES = 7;
+
+// The descriptor spans of the buffer-load kernel fit 31 bits for every unit: A spans the unit's k
+// range (at most the largest split, in 32-wide tiles) times lda, B its BN columns times ldb.
+static bool buffer_fits(const GemmArgs &p, const GemmShape &g, int Smin)
+{
+    const int64_t kspan = (int64_t)((p.ktiles + Smin - 1) / Smin + 1) * GK;
+    const int64_t lim = (int64_t(1) << 31) / 8;
+    return p.lda * (kspan + 16) + g.bm < lim && p.ldb * (g.bn + 1) + kspan < lim;
+}
+
+template <int BM, int BN, int WMW, int WNW, int MINW>
+static void launch_buf(epsx_ctx *ctx, unsigned grid, const GemmArgs &p)
+{
+    constexpr size_t lds = 8 * 2 * 16 * ((BM + 16) + (BN + 2));
+    static_assert(lds <= 160 * 1024, "LDS");
+    if (lds > 64 * 1024)   // above the default dynamic LDS limit
+        (void)hipFuncSetAttribute((const void *)dgemm_buf_kernel<BM, BN, WMW, WNW, MINW>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((dgemm_buf_kernel<BM, BN, WMW, WNW, MINW>), dim3(grid), dim3(64 * WMW * WNW), lds, ctx->stream, p);
+}
 
 static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
 {
@@ -488,15 +641,32 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
     p.mt = (int)((p.m + g.bm - 1) / g.bm);
     p.nt = (int)((p.n + g.bn - 1) / g.bn);
     const int64_t units = (int64_t)p.mt * p.nt * p.units_per_tile;
-    const int64_t grid = (units + NXCD - 1) / NXCD * NXCD;
-    const dim3 blk(64 * g.wmw * g.wnw);
-    switch (c) {
-    // the last parameter: waves per SIMD the register budget is cut for
-    case 0: hipLaunchKernelGGL((dgemm_kernel<64, 64, 2, 2, 3>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
-    case 1: hipLaunchKernelGGL((dgemm_kernel<128, 64, 4, 2, 4>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
-    case 2: hipLaunchKernelGGL((dgemm_kernel<64, 128, 2, 2, 2>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
-    case 3: hipLaunchKernelGGL((dgemm_kernel<128, 128, 4, 2, 2>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
-    default: hipLaunchKernelGGL((dgemm_kernel<128, 128, 2, 2, 2, 16>), dim3((unsigned)grid), blk, 0, ctx->stream, p); break;
+    const unsigned grid = (unsigned)((units + NXCD - 1) / NXCD * NXCD);
+    int Smin = p.S;
+    for (int q = 1; q <= NSPLIT_TAB; q++) Smin = p.S_of[q] < Smin ? p.S_of[q] : Smin;
+    if (buffer_fits(p, g, Smin > 0 ? Smin : 1)) {
+        // the last parameter: waves per SIMD the register budget is cut for
+        switch (c) {
+        case 0: launch_buf<64, 64, 2, 2, 4>(ctx, grid, p); break;
+        case 1: launch_buf<128, 64, 2, 2, 3>(ctx, grid, p); break;
+        case 2: launch_buf<64, 128, 2, 2, 3>(ctx, grid, p); break;
+        case 3: launch_buf<128, 128, 2, 2, 2>(ctx, grid, p); break;
+        case 4: launch_buf<256, 128, 4, 2, 1>(ctx, grid, p); break;
+        case 5: launch_buf<256, 32, 4, 1, 2>(ctx, grid, p); break;
+        default: launch_buf<128, 32, 4, 1, 3>(ctx, grid, p); break;
+        }
+    } else {
+        // operands whose per-unit span passes 31 bits: the register-staged kernel with 64-bit
+        // addressing and per-element bounds checks
+        switch (c) {
+        case 0: hipLaunchKernelGGL((dgemm_kernel<64, 64, 2, 2, 3>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
+        case 1: hipLaunchKernelGGL((dgemm_kernel<128, 64, 4, 2, 4>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
+        case 2: hipLaunchKernelGGL((dgemm_kernel<64, 128, 2, 2, 2>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
+        case 3: hipLaunchKernelGGL((dgemm_kernel<128, 128, 4, 2, 2>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
+        case 4: hipLaunchKernelGGL((dgemm_kernel<256, 128, 4, 2, 1, 16>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
+        case 5: hipLaunchKernelGGL((dgemm_kernel<256, 32, 4, 1, 2>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
+        default: hipLaunchKernelGGL((dgemm_kernel<128, 32, 4, 1, 3>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
+        }
     }
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
@@ -568,16 +738,21 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     p.idx = bt.idx; p.gate = bt.gate; p.imap = bt.imap;
     p.accumulate = accumulate ? 1 : 0;
     p.ktiles = (int)((k + GK - 1) / GK);
-    // Split k to fill the machine, by the 64x64 tile count whatever shape runs (so the split,
-    // and with it every sum, does not depend on the tuned shape): 256 CUs hold 4 workgroups
-    // each, so a launch runs in rounds of 1024 units.
-    int S = 1, Smax = 1;
-    const int64_t tiles = ((m + 63) / 64) * ((n + 63) / 64);
-    EPSX_TRY(plan_splits(ctx, p, bt, tiles, 256 * 4, S, Smax));
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     EPSX_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
     const bool capturing = cap != hipStreamCaptureStatusNone;
+    auto reduce = [&](const GemmArgs &q, int Smax) -> int {
+        if (Smax > 1) {
+            hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
+                               dim3(256), 0, ctx->stream, q);
+            EPSX_HIP(ctx, hipGetLastError());
+        }
+        return EPSX_OK;
+    };
     if (bt.precision == EPSX_PREC_FP32 || bt.precision == EPSX_PREC_BF16) {
+        // 64x64 tiles, 4 workgroups per CU: rounds of 1024 units
+        int S = 1, Smax = 1;
+        EPSX_TRY(plan_splits(ctx, p, bt, ((m + 63) / 64) * ((n + 63) / 64), 256 * 4, S, Smax));
         p.mt = (int)((m + 63) / 64);
         p.nt = (int)((n + 63) / 64);
         const int64_t grid = ((int64_t)p.mt * p.nt * p.units_per_tile + NXCD - 1) / NXCD * NXCD;
@@ -587,35 +762,45 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
             hipLaunchKernelGGL(lpgemm_kernel<false>, dim3((unsigned)grid), dim3(256), 0, ctx->stream, p);
         EPSX_HIP(ctx, hipGetLastError());
         if (bt.mark) EPSX_HIP(ctx, hipEventRecord(bt.mark, ctx->stream));
-        if (Smax > 1) {
-            hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
-                               dim3(256), 0, ctx->stream, p);
-            EPSX_HIP(ctx, hipGetLastError());
-        }
-        return EPSX_OK;
+        return reduce(p, Smax);
     }
-    // The tile shape: EPSX_GEMM_SHAPE forces one; otherwise the autotuner times every shape once
-    // on the first product of this size that overwrites its output (outside graph capture) and
-    // keeps the fastest for the context.  All shapes give the same bits.
+    // The k split of a shape fills the machine with that shape's tiles (its workgroups per CU
+    // times 256 CUs a round); the bits depend on the split, so the shape and split are chosen
+    // together: EPSX_GEMM_SHAPE forces a shape; otherwise the autotuner times every shape with its
+    // split once, on the first product of this size that overwrites its output (outside graph
+    // capture), and the context keeps the fastest, so its later products of the size give the
+    // same bits.
+    auto plan = [&](int c, GemmArgs &q, int &Smax) -> int {
+        const GemmShape &g = SHAPES[c];
+        int S = 1;
+        q = p;
+        return plan_splits(ctx, q, bt, ((m + g.bm - 1) / g.bm) * ((n + g.bn - 1) / g.bn), 256 * (int64_t)g.per_cu, S,
+                           Smax);
+    };
     int c = 0;
     const char *force = getenv("EPSX_GEMM_SHAPE");
-    const std::array<int64_t, 5> key{m, n, k, bt.n_inst, S};
+    const std::array<int64_t, 5> key{m, n, k, bt.n_inst, bt.part ? bt.part_cap : -1};
     auto it = ctx->gemm_tune.find(key);
     if (force && *force) {
         c = atoi(force) % NSHAPES;
     } else if (it != ctx->gemm_tune.end()) {
         c = it->second;
     } else if (!bt.tune) {
-        c = 1;   // 128x64, the usual winner, until a tunable product of this size has run
+        c = 1;   // 128x64 until a tunable product of this size has run
     } else if (!accumulate && !capturing) {
         hipEvent_t e0, e1;
         EPSX_HIP(ctx, hipEventCreate(&e0));
         EPSX_HIP(ctx, hipEventCreate(&e1));
         float bestms = 0;
         for (int q = 0; q < NSHAPES; q++) {
-            EPSX_TRY(launch_shape(ctx, q, p));   // warm
+            GemmArgs pq;
+            int Smq = 1;
+            EPSX_TRY(plan(q, pq, Smq));
+            EPSX_TRY(launch_shape(ctx, q, pq));   // warm
+            EPSX_TRY(reduce(pq, Smq));
             EPSX_HIP(ctx, hipEventRecord(e0, ctx->stream));
-            EPSX_TRY(launch_shape(ctx, q, p));
+            EPSX_TRY(launch_shape(ctx, q, pq));
+            EPSX_TRY(reduce(pq, Smq));
             EPSX_HIP(ctx, hipEventRecord(e1, ctx->stream));
             EPSX_HIP(ctx, hipEventSynchronize(e1));
             float ms = 0;
@@ -626,14 +811,12 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
         (void)hipEventDestroy(e1);
         ctx->gemm_tune[key] = c;
     }
-    EPSX_TRY(launch_shape(ctx, c, p));
+    GemmArgs pc;
+    int Smax = 1;
+    EPSX_TRY(plan(c, pc, Smax));
+    EPSX_TRY(launch_shape(ctx, c, pc));
     if (bt.mark) EPSX_HIP(ctx, hipEventRecord(bt.mark, ctx->stream));
-    if (Smax > 1) {
-        hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
-                           dim3(256), 0, ctx->stream, p);
-        EPSX_HIP(ctx, hipGetLastError());
-    }
-    return EPSX_OK;
+    return reduce(pc, Smax);
 }
 
 int dgemm_nn(epsx_ctx *ctx, int64_t m, int64_t ncol, int64_t k, const double *A, int64_t lda, const double *B,

@@ -19,6 +19,7 @@ Targets: Σ_converged / SIGMA_SCALE, real and imaginary parts (the converged sel
 smooth and of order 1 eV, unlike the correction from the spiky G0D0 one).
 """
 import ctypes
+import sys
 import time
 import warnings
 
@@ -210,7 +211,7 @@ def make_training_set(ctx, n_models, nk=12, length=1024, seed=0, states_per_mode
         ts.append(t)
         if verbosity:
             print(f"[learn] model {m + 1}/{n_models}: ni {len(setup.ε_i)}, plain iterations "
-                  f"{rc['history']['err'].shape[1]}", flush=True)
+                  f"{rc['history']['err'].shape[1]}", file=sys.stderr, flush=True)
     return np.concatenate(xs), np.concatenate(ts)
 
 
