@@ -231,6 +231,78 @@ __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
         if (ip0 + r < ni) out[wn + nwm * (ip0 + r)] = v[r];
 }
 
+// The same h with the Σ columns of HS consecutive outer states staged in LDS (η-clamped as they
+// are staged), one workgroup per (state group, slot) sweeping every (ω, ν) of its states: Σ is
+// read from memory once per launch and the four interpolation reads of every term come from LDS;
+// the (ω, ν) constants are read once per HS states.  Same operations in the same order as
+// h_from, so the same bits.
+template <int HS>
+__global__ __launch_bounds__(512) void h_build_lds_kernel(HArgs a)
+{
+    extern __shared__ __align__(16) double2 h_lds[];   // [HS][nw]
+    const int z = blockIdx.y;
+    const int tcol = a.tidx[z];
+    if (a.act && !a.act[tcol]) return;
+    const int64_t nw = a.nw, ni = a.ni, nwm = nw * a.nm, i0 = (int64_t)blockIdx.x * HS;
+    const int nr = (int)(ni - i0 < HS ? ni - i0 : HS);
+    const double eta = a.eta ? a.eta[tcol] : -INFINITY;
+    const double2 *S = reinterpret_cast<const double2 *>(a.S) + nw * (ni * (int64_t)tcol + i0);
+    for (int64_t q = threadIdx.x; q < nw * nr; q += 512) {
+        const double2 v = S[q];
+        h_lds[q] = make_double2(v.x, fmin(v.y, -eta));
+    }
+    __syncthreads();
+    double *out = a.hout + nwm * ni * (int64_t)z + nwm * i0;
+    const double2 *occ = reinterpret_cast<const double2 *>(a.occ) + nwm * (int64_t)z;
+    // HU (ω, ν) per thread and pass, their constants loaded together
+    constexpr int HU = 4;
+    for (int64_t wb = threadIdx.x; wb < nwm; wb += 512 * HU) {
+        int ka[HU], ke[HU];
+        double ta[HU], te[HU], x0[HU], wnu[HU];
+        double2 o[HU];
+        bool live[HU], zero[HU];
+#pragma unroll
+        for (int u = 0; u < HU; u++) {
+            const int64_t wn = wb + 512 * u;
+            live[u] = wn < nwm;
+            const int64_t wc = live[u] ? wn : nwm - 1;
+            const int64_t w = wc % nw, nu = wc / nw;
+            wnu[u] = a.wnu[nu];
+            zero[u] = wnu[u] < a.w_acoustic;
+            ka[u] = a.hk[2 * wc];
+            ke[u] = a.hk[2 * wc + 1];
+            ta[u] = a.ht[2 * wc];
+            te[u] = a.ht[2 * wc + 1];
+            x0[u] = a.ws[w];
+            o[u] = occ[wc];
+        }
+#pragma unroll
+        for (int u = 0; u < HU; u++) {
+            if (!live[u]) continue;
+            const int64_t wn = wb + 512 * u;
+            double v[HS];
+#pragma unroll
+            for (int r = 0; r < HS; r++) {
+                const int rr = r < nr ? r : nr - 1;
+                const double2 *col = h_lds + nw * rr;
+                const double2 la = col[ka[u]], ha = col[ka[u] + 1], le = col[ke[u]], he = col[ke[u] + 1];
+                const double sar = (1 - ta[u]) * la.x + ta[u] * ha.x;
+                const double sai = (1 - ta[u]) * la.y + ta[u] * ha.y;
+                const double ser = (1 - te[u]) * le.x + te[u] * he.x;
+                const double sei = (1 - te[u]) * le.y + te[u] * he.y;
+                const double ei = a.e_i[i0 + rr];
+                const double zar = x0[u] + wnu[u] - ei - sar, zai = -sai;
+                const double zer = x0[u] - wnu[u] - ei - ser, zei = -sei;
+                const double da = zar * zar + zai * zai, de = zer * zer + zei * zei;
+                v[r] = zero[u] ? 0.0 : -(zai * o[u].x * de + zei * o[u].y * da) / (da * de);
+            }
+#pragma unroll
+            for (int r = 0; r < HS; r++)
+                if (r < nr) out[wn + nwm * r] = v[r];
+        }
+    }
+}
+
 // The Einstein factorisation (dispersionless modes, any inner-state energies): with ωq = ω_ν
 // for every pair, the term of (ω, ν, i, f) is w_f g2[ν, i, f] H_ν[ω, f] with
 //   H_ν[ω, f] = Im G_f(ω+ω_ν) (n_ν + f(ω+ω_ν-μ)) + Im G_f(ω-ω_ν) (n_ν + 1 - f(ω-ω_ν-μ)),
@@ -489,8 +561,32 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
     if (hout) {
         HArgs ah = a;
         ah.act = act_h;
-        const int64_t threads = nw * nm * ((ni + HB_R - 1) / HB_R);
-        hipLaunchKernelGGL(h_build_kernel, dim3(xcd_grid(threads), (unsigned)nslots), dim3(256), 0, ctx->stream, ah);
+        // the LDS-staged build while HS columns fit 80 KB (two workgroups per CU), else the
+        // gather from memory; a kernel's LDS limit is raised past the default 64 KB once per
+        // process (the first build runs eagerly, before any graph capture)
+        const size_t col = 16 * (size_t)nw, cap = 80 * 1024;
+        if (4 * col <= cap) {
+            static bool raised4 = false;
+            if (4 * col > 64 * 1024 && !raised4) {
+                EPSX_HIP(ctx, hipFuncSetAttribute((const void *)h_build_lds_kernel<4>,
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap));
+                raised4 = true;
+            }
+            hipLaunchKernelGGL(h_build_lds_kernel<4>, dim3((unsigned)((ni + 3) / 4), (unsigned)nslots), dim3(512),
+                               4 * col, ctx->stream, ah);
+        } else if (2 * col <= cap) {
+            static bool raised2 = false;
+            if (2 * col > 64 * 1024 && !raised2) {
+                EPSX_HIP(ctx, hipFuncSetAttribute((const void *)h_build_lds_kernel<2>,
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap));
+                raised2 = true;
+            }
+            hipLaunchKernelGGL(h_build_lds_kernel<2>, dim3((unsigned)((ni + 1) / 2), (unsigned)nslots), dim3(512),
+                               2 * col, ctx->stream, ah);
+        } else {
+            const int64_t threads = nw * nm * ((ni + HB_R - 1) / HB_R);
+            hipLaunchKernelGGL(h_build_kernel, dim3(xcd_grid(threads), (unsigned)nslots), dim3(256), 0, ctx->stream, ah);
+        }
         EPSX_HIP(ctx, hipGetLastError());
     }
     // the transposed low-precision layouts: blocks of HL_IP states until nm ip covers kp

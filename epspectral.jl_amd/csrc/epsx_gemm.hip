@@ -629,9 +629,14 @@ static void launch_buf(epsx_ctx *ctx, unsigned grid, const GemmArgs &p)
 {
     constexpr size_t lds = 8 * 2 * 16 * ((BM + 16) + (BN + 2));
     static_assert(lds <= 160 * 1024, "LDS");
-    if (lds > 64 * 1024)   // above the default dynamic LDS limit
+    // above the default dynamic LDS limit: raised once per process, by the first launch (the
+    // autotuner's or an eager one, before any graph capture)
+    static bool raised = false;
+    if (lds > 64 * 1024 && !raised) {
         (void)hipFuncSetAttribute((const void *)dgemm_buf_kernel<BM, BN, WMW, WNW, MINW>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        raised = true;
+    }
     hipLaunchKernelGGL((dgemm_buf_kernel<BM, BN, WMW, WNW, MINW>), dim3(grid), dim3(64 * WMW * WNW), lds, ctx->stream, p);
 }
 
