@@ -98,6 +98,7 @@ struct NetDev {
     DevBuf params, grad, m, v, Wt, loss, bsum, part;
     DevBuf X[NET_MAX_LAYERS + 1], Z[NET_MAX_LAYERS];
     DevBuf Y, dA, dB, T, idx;
+    DevBuf Wp;   // BF16 weights of the inference convolution, tap-major, per layer
 };
 
 }  // namespace epsx

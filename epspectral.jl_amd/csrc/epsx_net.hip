@@ -385,6 +385,95 @@ static int net_buffers(epsx_ctx *ctx, int64_t nb)
     return EPSX_OK;
 }
 
+// ---- inference: one fused convolution per layer ---------------------------------------------------
+// X_next[co][pos] = act(b[co] + Σ_{ci,t} W[co][ci][t] X[ci][pos + t - h]) for c_in, c_out <= 64, as a
+// BF16 MFMA product (v_mfma_f32_16x16x32_bf16, FP32 sums) with the reduction index tap-major,
+// k = t CI + ci (CI = c_in rounded up to 8): eight consecutive k are eight channels at one tap, so a
+// lane's B operand is one 16-byte LDS read of the staged window.  One workgroup per IC_NT positions
+// of one sample (L a multiple of IC_NT): the window X[:, n0 - h .. n0 + IC_NT + h) is staged in LDS
+// as BF16 [position][channel] (zeros past the sample's ends and past c_in); wave w owns the output
+// channels 16 w .. 16 w + 15 with its A operand (the packed weights) held in registers for the
+// whole k range.  Bias and SiLU in the epilogue.
+constexpr int IC_NT = 128, IC_MAXK = 18 * 32;   // k up to 9 taps x 64 channels
+
+// Wp[m][k] (IC_KP = Kp columns, 64 rows): BF16 of W[m][ci][t] at k = t CI + ci, zero elsewhere
+__global__ void net_pack_weights_kernel(int cout, int cin, int KW, int CI, int Kp, const float *__restrict__ W,
+                                        uint16_t *__restrict__ Wp)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 64 * Kp) return;
+    const int m = q / Kp, k = q - m * Kp, t = k / CI, ci = k - t * CI;
+    const float w = (m < cout && ci < cin && t < KW) ? W[((int64_t)m * cin + ci) * KW + t] : 0.f;
+    Wp[q] = f2bf(w);
+}
+
+template <int KS>   // k steps of 32 (Kp = 32 KS): the A operand stays in registers
+__global__ __launch_bounds__(256) void net_conv_fwd_kernel(const uint16_t *__restrict__ Wp, const float *__restrict__ bias,
+                                                           int cout, int cin, int CI, int KW, int L,
+                                                           const float *__restrict__ X, int64_t ldx,
+                                                           float *__restrict__ Y, int act)
+{
+    constexpr int Kp = 32 * KS;
+    __shared__ __align__(16) uint16_t xw[(IC_NT + 16) * (64 + 8)];
+    const int PW = CI + 8;                       // row pitch (elements): 16-byte reads on distinct banks
+    const int h = KW / 2, NW = IC_NT + KW - 1;   // window positions
+    const int64_t n0 = (int64_t)blockIdx.x * IC_NT;
+    const int xs = (int)(n0 % L);                // grid point of the first position in its sample
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int e = tid; e < NW * CI; e += 256) {
+        const int pw = e % NW, c = e / NW;
+        const int x = xs - h + pw;
+        const float v = (c < cin && x >= 0 && x < L) ? X[c * ldx + n0 - h + pw] : 0.f;
+        xw[pw * PW + c] = f2bf(v);
+    }
+    const int row = 16 * wave + (lane & 15), kb = 8 * (lane >> 4);
+    bf16x8_t af[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) __builtin_memcpy(&af[ks], Wp + (int64_t)row * Kp + ks * 32 + kb, 16);
+    __syncthreads();
+    if (16 * wave >= cout) return;
+    f32x4_t acc[IC_NT / 16];
+#pragma unroll
+    for (int j = 0; j < IC_NT / 16; j++) acc[j] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        const int k = ks * 32 + kb;              // eight channels ci .. ci + 7 at tap t
+        const int t0 = k / CI, ci = k - t0 * CI;
+        const int t = t0 < KW ? t0 : KW - 1;     // taps past KW: zero weights, any finite window value
+#pragma unroll
+        for (int j = 0; j < IC_NT / 16; j++) {
+            bf16x8_t bfr;
+            __builtin_memcpy(&bfr, xw + (j * 16 + (lane & 15) + t) * PW + ci, 16);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ks], bfr, acc[j], 0, 0, 0);
+        }
+    }
+    // C[i][j] of a 16x16 tile: element e of lane l holds i = 4 (l / 16) + e, j = l % 16
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int co = 16 * wave + 4 * (lane >> 4) + e;
+        if (co >= cout) continue;
+        const float b = bias[co];
+#pragma unroll
+        for (int j = 0; j < IC_NT / 16; j++) {
+            const float z = acc[j][e] + b;
+            Y[co * ldx + n0 + j * 16 + (lane & 15)] = act ? z * sigmoidf_(z) : z;
+        }
+    }
+}
+
+// Whether the fused inference convolution serves this network
+static bool net_infer_fits(const NetDev &n)
+{
+    if (n.length % IC_NT != 0 || n.kernel > 17) return false;
+    const int cm = std::max(std::max(n.c_in, n.c_hidden), n.c_out);
+    const int CI = (cm + 7) / 8 * 8;
+    return cm <= 64 && (int64_t)n.kernel * CI <= IC_MAXK;
+}
+
+// BF16 inference of nb samples in X[0] into X[layers] (no Z kept): the fused convolution when it
+// fits, else the training forward pass
+static int net_infer_dev(epsx_ctx *ctx, int64_t nb);
+
 // Forward pass of nb samples already gathered into X[0]; output in X[layers]
 static int net_forward_dev(epsx_ctx *ctx, int64_t nb, bool bf16)
 {
@@ -401,6 +490,44 @@ static int net_forward_dev(epsx_ctx *ctx, int64_t nb, bool bf16)
         hipLaunchKernelGGL(net_bias_act_kernel, dim3(nblk(NB * cout[l])), dim3(256), 0, ctx->stream, NB,
                            (int64_t)cout[l], dptr<float>(n.Y), P + boff[l], dptr<float>(n.Z[l]),
                            dptr<float>(n.X[l + 1]), l < n.layers - 1 ? 1 : 0);
+        EPSX_HIP(ctx, hipGetLastError());
+    }
+    return EPSX_OK;
+}
+
+static int net_infer_dev(epsx_ctx *ctx, int64_t nb)
+{
+    NetDev &n = ctx->net;
+    if (!net_infer_fits(n)) return net_forward_dev(ctx, nb, true);
+    std::vector<int> cin, cout;
+    std::vector<int64_t> woff, boff;
+    net_layout(n, cin, cout, woff, boff);
+    const int64_t NB = nb * n.length;
+    const float *P = static_cast<const float *>(n.params.p);
+    const int Kp = (n.kernel * 64 + 31) / 32 * 32;   // the widest layer's k, rounded to 32
+    EPSX_TRY(dev_alloc(ctx, n.Wp, 2 * (size_t)n.layers * 64 * Kp));
+    for (int l = 0; l < n.layers; l++) {
+        const int CI = (cin[l] + 7) / 8 * 8, Kl = (n.kernel * CI + 31) / 32 * 32;
+        uint16_t *W = static_cast<uint16_t *>(n.Wp.p) + (size_t)l * 64 * Kp;
+        hipLaunchKernelGGL(net_pack_weights_kernel, dim3(nblk(64 * Kl)), dim3(256), 0, ctx->stream, cout[l], cin[l],
+                           n.kernel, CI, Kl, P + woff[l], W);
+        const dim3 grid((unsigned)(NB / IC_NT));
+        const float *b = P + boff[l], *X = dptr<float>(n.X[l]);
+        float *Y = dptr<float>(n.X[l + 1]);
+        const int act = l < n.layers - 1 ? 1 : 0;
+        switch (Kl / 32) {
+#define EPSX_CONV_CASE(q)                                                                                         \
+    case q:                                                                                                        \
+        hipLaunchKernelGGL(net_conv_fwd_kernel<q>, grid, dim3(256), 0, ctx->stream, W, b, cout[l], cin[l], CI,    \
+                           n.kernel, (int)n.length, X, NB, Y, act);                                                \
+        break;
+            EPSX_CONV_CASE(1) EPSX_CONV_CASE(2) EPSX_CONV_CASE(3) EPSX_CONV_CASE(4) EPSX_CONV_CASE(5)
+            EPSX_CONV_CASE(6) EPSX_CONV_CASE(7) EPSX_CONV_CASE(8) EPSX_CONV_CASE(9) EPSX_CONV_CASE(10)
+            EPSX_CONV_CASE(11) EPSX_CONV_CASE(12) EPSX_CONV_CASE(13) EPSX_CONV_CASE(14) EPSX_CONV_CASE(15)
+            EPSX_CONV_CASE(16) EPSX_CONV_CASE(17) EPSX_CONV_CASE(18)
+#undef EPSX_CONV_CASE
+        default: return set_error(ctx, EPSX_ERR_STATE, "inference convolution: %d k steps", Kl / 32);
+        }
         EPSX_HIP(ctx, hipGetLastError());
     }
     return EPSX_OK;
@@ -533,7 +660,7 @@ int net_predict_sigma(epsx_ctx *ctx, int64_t nw, int64_t ni, int nz, const int32
         hipLaunchKernelGGL(ws_features_kernel, dim3(nblk(nb * nw)), dim3(256), 0, ctx->stream, nw, ni, b0, nb,
                            static_cast<const double2 *>(S), ws, e_i, mu, T, tsrc, dptr<float>(n.X[0]));
         EPSX_HIP(ctx, hipGetLastError());
-        EPSX_TRY(net_forward_dev(ctx, nb, precision == EPSX_PREC_BF16));
+        EPSX_TRY(precision == EPSX_PREC_BF16 ? net_infer_dev(ctx, nb) : net_forward_dev(ctx, nb, false));
         hipLaunchKernelGGL(ws_output_kernel, dim3(nblk(nb * nw)), dim3(256), 0, ctx->stream, nw, ni, b0, nb,
                            dptr<float>(n.X[n.layers]), tdst, S);
         EPSX_HIP(ctx, hipGetLastError());
@@ -635,7 +762,7 @@ int epsx_net_forward(epsx_ctx *ctx, const float *x, int64_t nsamples, int32_t pr
     for (int64_t b0 = 0; b0 < nsamples; b0 += chunk) {
         const int64_t nb = std::min(chunk, nsamples - b0);
         EPSX_TRY(upload_gather(ctx, x, b0, nb, n.c_in, dptr<float>(n.X[0])));
-        EPSX_TRY(net_forward_dev(ctx, nb, precision == EPSX_PREC_BF16));
+        EPSX_TRY(precision == EPSX_PREC_BF16 ? net_infer_dev(ctx, nb) : net_forward_dev(ctx, nb, false));
         hipLaunchKernelGGL(net_scatter_kernel, dim3(nblk(nb * n.c_out * n.length)), dim3(256), 0, ctx->stream, nb,
                            (int64_t)n.c_out, n.length, dptr<float>(n.X[n.layers]), out.ptr<float>());
         EPSX_HIP(ctx, hipGetLastError());
