@@ -186,7 +186,7 @@ its floor, then FP64 to the same `tol`.
 """
 function loop_scgd0(b::MI355XBackend, setup, occ; η_min, η_init, maxiter, tol, Σ_init = nothing,
         fix_μ = false, ω_acoustic, gpu_tile = nothing, verbosity = 0, precision = :fp64,
-        batched = true, upload = true)
+        batched = true, upload = true, warm_start = nothing)
     upload && set_problem!(b, setup)
     nω, ni = length(setup.ωs), size(setup.g2, 2)
     nT = length(occ.Tlist)
@@ -200,14 +200,18 @@ function loop_scgd0(b::MI355XBackend, setup, occ; η_min, η_init, maxiter, tol,
     conv = Vector{Int32}(undef, nT)
     herr, hμ, hA = (Matrix{Float64}(undef, nT, maxiter) for _ in 1:3)
     wall = Vector{Float64}(undef, maxiter); frozen = Vector{Int64}(undef, nT)
+    kept = zeros(Int32, nT)
+    # warm_start: nothing, or the precision (:bf16 / :fp32) of the context's network (net_create /
+    # net_train): Σ_init predicted on the device from the first iteration's G0D0 self-energy
+    ws = warm_start === nothing ? Int32(0) : PREC[warm_start]
     out = Scgd0Outputs(sizeof(Scgd0Outputs), pointer(Σ), pointer(μ), pointer(conv), pointer(herr),
-        pointer(hμ), pointer(hA), pointer(wall), pointer(frozen), 0, 0, 0)
-    GC.@preserve T nl μ0 ηm Σi Σ μ conv herr hμ hA wall frozen out begin
+        pointer(hμ), pointer(hA), pointer(wall), pointer(frozen), 0, 0, 0, pointer(kept))
+    GC.@preserve T nl μ0 ηm Σi Σ μ conv herr hμ hA wall frozen kept out begin
         par = Ref(Scgd0Params(sizeof(Scgd0Params), nT, pointer(T), pointer(nl), pointer(μ0), pointer(ηm),
             occ.nelec, occ.spin_degeneracy, Int32(occ.type === :Semiconductor), Int32(fix_μ),
             η_init, tol, ω_acoustic, maxiter, gpu_tile === nothing ? 0 : gpu_tile,
             FM_AUTO, PREC[precision], Int32(batched), Int32(verbosity),
-            Σi === nothing ? Ptr{ComplexF64}(C_NULL) : pointer(Σi), Int32(0), Int32(0)))
+            Σi === nothing ? Ptr{ComplexF64}(C_NULL) : pointer(Σi), Int32(0), ws))
         check(b, ccall((:epsx_scgd0_run, libepsx), Cint,
             (Ptr{Cvoid}, Ref{Scgd0Params}, Ref{Scgd0Outputs}), handle(b), par, out))
     end
@@ -217,7 +221,7 @@ function loop_scgd0(b::MI355XBackend, setup, occ; η_min, η_init, maxiter, tol,
         @warn "scGD0 not converged after maxiter = $maxiter iterations at T = $(T[.!converged])"
     history = (; err = herr[:, 1:n], μ = hμ[:, 1:n], min_spectral_weight = hA[:, 1:n],
         wall = wall[1:n], iter_frozen = frozen)
-    (; Σ, μ, converged, history)
+    (; Σ, μ, converged, history, warm_kept = kept .!= 0)
 end
 
 """
