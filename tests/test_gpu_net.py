@@ -144,3 +144,46 @@ def check_judge(dev, setup, occ, net, w):
     assert np.array_equal(~w["fallback"], ep < eq), (ep, eq)
     np.testing.assert_allclose(w["history"]["err"][:, 0], r1["history"]["err"][:, 0], rtol=1e-12)
     np.testing.assert_allclose(w["history"]["err"][:, 1], np.minimum(ep, eq), rtol=1e-5)
+
+
+def test_c5_config_network_held_out_24(ctx):
+    """The configuration network (6 layers, 64 channels, width 9, 1024 nodes) trained on random
+    12³ models; a held-out 24³ model run from the device warm start reaches the plain run's fixed
+    point, and a corrupted network falls back to the plain start with the same result."""
+    L = 1024
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        x, t = make_training_set(ctx, 16, nk=12, length=L, seed=31, states_per_model=96)
+        net = WarmStartNet(ctx, L, seed=1)
+        assert net.nparam > 1e5
+        losses = net.train(x, t, epochs=30, batch=64, lr=1e-3, precision="bf16", seed=0)
+        assert losses[-1] < 0.5 * losses[0]
+        rng = np.random.default_rng(4242)
+        done = 0
+        for _ in range(4):
+            setup, meta = random_t2g_model(rng, nk=24, length=L)
+            dev = epsx.Scgd0Device(setup, ctx)
+            occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+            try:
+                _, rc = g0d0_and_converged(dev, occ)
+            except epsx.ArgumentError:
+                continue   # no μ root for this random model
+            if not rc["converged"].all():
+                continue   # the plain iteration itself does not converge: no fixed point to compare
+            w = run_warm_started(dev, setup, occ, net)
+            assert w["converged"].all()
+            assert np.max(np.abs(w["Σ"] - rc["Σ"])) < 1e-5
+            check_judge(dev, setup, occ, net, w)
+            print("24³ held-out: plain", rc["history"]["err"].shape[1], "warm", w["iterations"],
+                  "kept", (~w["fallback"]).tolist())
+            p = net.params
+            bad = p.copy()
+            bad[-2:] = 50.0
+            net.params = bad
+            wb = run_warm_started(dev, setup, occ, net)
+            net.params = p
+            assert wb["fallback"].all() and wb["converged"].all()
+            assert np.max(np.abs(wb["Σ"] - rc["Σ"])) < 1e-5
+            done += 1
+            break
+        assert done == 1, "no held-out 24³ model with a converging plain run"
