@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-levels", action="store_true", help="skip the FP32 / BF16 precision-level runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", default="scgd0", choices=("scgd0", "ladder", "warmstart"),
+    ap.add_argument("--workload", default="scgd0", choices=("scgd0", "ladder", "warmstart", "northstar"),
                     help="scgd0: the headline solve; ladder: the Kubo ladder on the 16³ model; "
                          "warmstart: train the Σ_init network and evaluate it on held-out models")
     ap.add_argument("--models", type=int, default=64, help="warmstart: training models (12³)")
@@ -76,6 +76,8 @@ def main():
         return ladder_bench(args)
     if args.workload == "warmstart":
         return warmstart_bench(args)
+    if args.workload == "northstar":
+        return northstar_bench(args)
 
     import epsx
     from epsx.models import t2g_setup
@@ -433,6 +435,89 @@ def warmstart_bench(args):
                    "train_s": t_train, "data_s": t_data, "loss_first": float(losses[0]), "loss_last": float(losses[-1])},
         "iterations_plain": it_p, "iterations_warm": it_w, "iterations_saved_frac": 1 - it_w / it_p,
         "heldout": rows,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def northstar_bench(args):
+    """The combined configuration: the 24³ t2g model at 8 temperatures (50, 100, ..., 400 K) to
+    tol = 1e-6 eV in one batched run with the BF16 / FP32 precision levels and the learned warm
+    start (the 6-layer network trained on `--models` random 12³ models on the same graded grid),
+    against the plain FP64 GPU method (the direct kernel, temperatures one by one, `--plain-iters`
+    iterations timed and scaled to the FP64 run's iterations) and the FP64 batched star solve,
+    whose Σ and μ it is compared with.  The value is seconds per temperature to tol."""
+    import warnings
+    import epsx
+    from epsx.learn import WarmStartNet, make_training_set
+    from epsx.models import t2g_setup
+
+    warnings.simplefilter("ignore")
+    ctx = epsx.Context(0)
+    temps = tuple(50.0 * (k + 1) for k in range(8))
+    t0 = time.perf_counter()
+    setup, meta = t2g_setup(nk=args.nk, Tlist_K=temps)
+    nm, ni, nf = setup.dims
+    nw = len(setup.ωs)
+    print(f"[bench] setup nk={args.nk} ni={ni} nω={nw} nT={len(temps)} ({time.perf_counter() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    x, t = make_training_set(ctx, args.models, nk=12, length=None, seed=2024, states_per_model=96, verbosity=1)
+    t_data = time.perf_counter() - t0
+    net = WarmStartNet(ctx, nw, seed=0)
+    t0 = time.perf_counter()
+    losses = net.train(x, t, epochs=args.epochs, batch=64, lr=1e-3, precision="bf16", seed=0, verbosity=1)
+    t_train = time.perf_counter() - t0
+    print(f"[bench] network: {len(x)} samples in {t_data:.1f}s, {args.epochs} epochs in {t_train:.1f}s, "
+          f"loss {losses[0]:.3e} -> {losses[-1]:.3e}", file=sys.stderr, flush=True)
+    dev = epsx.Scgd0Device(setup, ctx)
+    occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0, spin_degeneracy=2.0)
+    nT = len(occ)
+    kw = dict(η_min=0.0, η_init=5e-3, maxiter=args.maxiter, tol=args.tol, ω_acoustic=1e-6, method="auto",
+              batched=True)
+    variants = {"fp64": dict(precision="fp64"), "fp64_warm": dict(precision="fp64", warm_start="bf16"),
+                "bf16_levels": dict(precision="bf16"), "bf16_levels_warm": dict(precision="bf16", warm_start="bf16")}
+    res, secs = {}, {}
+    for name, extra in variants.items():
+        dev.loop(occ, **dict(kw, maxiter=3, tol=0.0, **extra))   # prime (kernels, tile shapes, library candidates)
+        ts = []
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            r = dev.loop(occ, **kw, **extra)
+            ts.append(time.perf_counter() - t0)
+        res[name], secs[name] = r, sum(ts) / (len(ts) * nT)
+        wl = r["history"]["wall"]
+        print(f"[bench] {name}: {secs[name]:.4f} s/T, iterations {r['history']['err'].shape[1]}, "
+              f"converged {r['converged'].all()}, first iterations {np.round(1e3 * wl[:6], 2).tolist()} ms, "
+              f"the rest {1e3 * wl[6:].sum():.1f} ms", file=sys.stderr, flush=True)
+    ref = res["fp64"]
+    iters64 = int(ref["history"]["err"].shape[1])
+    plain = plain_sample(dev, occ, kw, iters64, args.plain_iters, nw * nm * ni * nf, 8.0 * nm * ni * nf)
+    print(f"[bench] plain FP64 GPU: {plain['value']:.2f} s/T", file=sys.stderr, flush=True)
+    rows = {}
+    for name in ("fp64_warm", "bf16_levels", "bf16_levels_warm"):
+        r = res[name]
+        dS = np.abs(r["Σ"] - ref["Σ"])
+        rows[name] = {"s_per_T": secs[name], "iterations": int(r["history"]["err"].shape[1]),
+                      "converged_all": bool(r["converged"].all()),
+                      "max_abs_dSigma_eV": float(dS.max()), "max_rel_dSigma": float(dS.max() / np.abs(ref["Σ"]).max()),
+                      "max_abs_dmu_eV": float(np.abs(r["μ"] - ref["μ"]).max()),
+                      "speedup_vs_plain_fp64_gpu": plain["value"] / secs[name],
+                      "speedup_vs_fp64_star": secs["fp64"] / secs[name]}
+        if "warm" in name:
+            rows[name]["warm_kept"] = r["info"]["warm_kept"].tolist()
+        rows[name]["iter_frozen"] = r["history"]["iter_frozen"].tolist()
+    v = rows["bf16_levels_warm"]
+    line = {
+        "metric": "scGD0 s/temperature to tol: 8 T batched, BF16 levels + learned warm start, vs plain FP64 GPU",
+        "value": v["s_per_T"], "unit": "s/temperature", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": 1e3 * v["s_per_T"] * nT, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16/fp32 levels then f64", "data": "synthetic",
+        "config": {"workload": f"t2g {args.nk}^3, 8 T (50-400 K), tol {args.tol} eV, batched, BF16 levels, "
+                               f"warm start trained on {args.models} random 12^3 models",
+                   "nomega": nw, "ni": ni, "nf": nf, "samples": int(len(x)), "epochs": args.epochs,
+                   "loss_first": float(losses[0]), "loss_last": float(losses[-1]), "train_s": t_train},
+        "fp64_star_s_per_T": secs["fp64"], "fp64_iterations": iters64, "plain_fp64_gpu": plain,
+        "variants": rows,
     }
     print(json.dumps(line), flush=True)
 

@@ -390,7 +390,7 @@ static int net_buffers(epsx_ctx *ctx, int64_t nb)
 // BF16 MFMA product (v_mfma_f32_16x16x32_bf16, FP32 sums) with the reduction index tap-major,
 // k = t CI + ci (CI = c_in rounded up to 8): eight consecutive k are eight channels at one tap, so a
 // lane's B operand is one 16-byte LDS read of the staged window.  One workgroup per IC_NT positions
-// of one sample (L a multiple of IC_NT): the window X[:, n0 - h .. n0 + IC_NT + h) is staged in LDS
+// of one sample (the last tile of a sample may be partial): the window X[:, n0 - h .. n0 + IC_NT + h) is staged in LDS
 // as BF16 [position][channel] (zeros past the sample's ends and past c_in); wave w owns the output
 // channels 16 w .. 16 w + 15 with its A operand (the packed weights) held in registers for the
 // whole k range.  Bias and SiLU in the epilogue.
@@ -417,8 +417,9 @@ __global__ __launch_bounds__(256) void net_conv_fwd_kernel(const uint16_t *__res
     __shared__ __align__(16) uint16_t xw[(IC_NT + 16) * (64 + 8)];
     const int PW = CI + 8;                       // row pitch (elements): 16-byte reads on distinct banks
     const int h = KW / 2, NW = IC_NT + KW - 1;   // window positions
-    const int64_t n0 = (int64_t)blockIdx.x * IC_NT;
-    const int xs = (int)(n0 % L);                // grid point of the first position in its sample
+    const int tiles = (L + IC_NT - 1) / IC_NT;   // tiles per sample
+    const int xs = (int)(blockIdx.x % tiles) * IC_NT;   // grid point of the tile's first position
+    const int64_t n0 = (int64_t)(blockIdx.x / tiles) * L + xs;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int e = tid; e < NW * CI; e += 256) {
         const int pw = e % NW, c = e / NW;
@@ -455,8 +456,10 @@ __global__ __launch_bounds__(256) void net_conv_fwd_kernel(const uint16_t *__res
         const float b = bias[co];
 #pragma unroll
         for (int j = 0; j < IC_NT / 16; j++) {
+            const int x = j * 16 + (lane & 15);
+            if (xs + x >= L) continue;   // past the sample's last position
             const float z = acc[j][e] + b;
-            Y[co * ldx + n0 + j * 16 + (lane & 15)] = act ? z * sigmoidf_(z) : z;
+            Y[co * ldx + n0 + x] = act ? z * sigmoidf_(z) : z;
         }
     }
 }
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(256) void net_conv_fwd_kernel(const uint16_t *__res
 // Whether the fused inference convolution serves this network
 static bool net_infer_fits(const NetDev &n)
 {
-    if (n.length % IC_NT != 0 || n.kernel > 17) return false;
+    if (n.kernel > 17) return false;
     const int cm = std::max(std::max(n.c_in, n.c_hidden), n.c_out);
     const int CI = (cm + 7) / 8 * 8;
     return cm <= 64 && (int64_t)n.kernel * CI <= IC_MAXK;
@@ -511,7 +514,7 @@ static int net_infer_dev(epsx_ctx *ctx, int64_t nb)
         uint16_t *W = static_cast<uint16_t *>(n.Wp.p) + (size_t)l * 64 * Kp;
         hipLaunchKernelGGL(net_pack_weights_kernel, dim3(nblk(64 * Kl)), dim3(256), 0, ctx->stream, cout[l], cin[l],
                            n.kernel, CI, Kl, P + woff[l], W);
-        const dim3 grid((unsigned)(NB / IC_NT));
+        const dim3 grid((unsigned)(nb * ((n.length + IC_NT - 1) / IC_NT)));
         const float *b = P + boff[l], *X = dptr<float>(n.X[l]);
         float *Y = dptr<float>(n.X[l + 1]);
         const int act = l < n.layers - 1 ? 1 : 0;

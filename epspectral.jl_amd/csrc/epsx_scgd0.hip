@@ -854,6 +854,7 @@ struct WarmHooks {
     std::vector<char> dropped;        // [2 nTu]
     std::vector<int32_t> kept;        // [nTu] slot kept per temperature
     std::vector<double> err_pred, err_plain;
+    double *Sigma_user = nullptr;     // the caller's Σ [nω, ni, nTu]: the kept slots are read into it
 };
 
 static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd0_outputs *out, WarmHooks *wh)
@@ -1466,14 +1467,17 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     // most half of maxiter runs below FP64.
     std::vector<int64_t> lp_iters(nT, 0), lp_flat(nT, 0);
     std::vector<double> first_err(nT, NAN);
-    std::vector<char> joined(nT, 0);   // slots that join the loop after its first iteration
+    std::vector<char> joined(nT, 0);        // slots that join the loop after its first iteration
+    std::vector<int64_t> partner(nT, -1);   // and the slot whose first err they take
     auto update_phases = [&]() -> int {
         bool changed = false, any_lp = false;
         for (int64_t t = 0; t < nT; t++) {
             if (lvl[t] == L.n - 1) continue;
             any_lp = true;
             if (joined[t]) { joined[t] = 0; continue; }   // no err of its own yet
-            if (lp_iters[t]++ == 0) first_err[t] = h_err[t];
+            // a slot that joined after the first iteration measures its floor against its
+            // partner's first err (the G0D0 residual), not against its own small first one
+            if (lp_iters[t]++ == 0) first_err[t] = partner[t] >= 0 ? first_err[partner[t]] : h_err[t];
             const bool near = h_err[t] <= 2 * par->tol;
             lp_flat[t] = (h_err[t] > 0.95 * prev_err[t] && h_err[t] < 1e-2 * first_err[t]) ? lp_flat[t] + 1 : 0;
             const bool budget = 2 * lp_iters[t] >= par->maxiter;
@@ -1715,7 +1719,10 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                 EPSX_HIP(ctx, hipMemcpyAsync(B.mu.ptr<double>() + nu, B.mu.ptr<double>(), 8 * nu, hipMemcpyDeviceToDevice, st));
                 EPSX_HIP(ctx, hipMemcpyAsync(book.act + nu, ones.data(), 4 * nu, hipMemcpyHostToDevice, st));
                 EPSX_HIP(ctx, hipStreamSynchronize(st));
-                for (int64_t t = nu; t < nT; t++) active[t] = joined[t] = 1;
+                for (int64_t t = nu; t < nT; t++) {
+                    active[t] = joined[t] = 1;
+                    partner[t] = t - nu;
+                }
             }
             for (int64_t t = 0; side_ok && t < nT; t++) {
                 if (active[t] || sent[t]) continue;
@@ -1755,7 +1762,18 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     }
     if (rc != EPSX_OK) return rc;
     const double t_read = now_s();
-    if (any_sent) {
+    if (wh) {
+        // the warm start: only the slot kept for each temperature comes back, into the caller's Σ
+        if (!wh->judged)   // the run ended before the judge (maxiter or every slot frozen): plain
+            for (int64_t t = 0; t < wh->nTu; t++) wh->kept[t] = (int32_t)t;
+        if (any_sent) EPSX_HIP(ctx, hipStreamSynchronize(ctx->side));
+        for (int64_t t = 0; t < wh->nTu; t++) {
+            const int64_t k = wh->kept[t];
+            char *dst = reinterpret_cast<char *>(wh->Sigma_user) + slice * t;
+            if (sent[k]) parallel_copy(dst, static_cast<char *>(ctx->pin_side) + slice * k, slice);
+            else EPSX_TRY(read_back(ctx, dst, B.S.ptr<char>() + slice * k, slice));
+        }
+    } else if (any_sent) {
         EPSX_HIP(ctx, hipStreamSynchronize(ctx->side));
         for (int64_t t = 0; t < nT; t++) {
             char *dst = reinterpret_cast<char *>(out->Sigma) + slice * t;
@@ -1801,7 +1819,7 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     if (!out->Sigma || !out->mu || !out->converged || !out->hist_err || !out->hist_mu || !out->hist_minA ||
         !out->wall || !out->iter_frozen)
         return set_error(ctx, EPSX_ERR_ARG, "scgd0 run: null output array");
-    const int64_t n2 = 2 * nT, nS = 16 * ctx->s.nw * ctx->s.ni;
+    const int64_t n2 = 2 * nT;
     auto dup = [&](const double *x) {
         std::vector<double> v(n2);
         for (int64_t t = 0; t < n2; t++) v[t] = x[t % nT];
@@ -1812,12 +1830,10 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     p2.nT = n2;
     p2.Tlist = T2.data(); p2.nlist = nl2.data(); p2.mu_start = mu2.data(); p2.eta_min = eta2.data();
     p2.warm_start = 0;
-    std::vector<char> S2((size_t)nS * n2);
     std::vector<double> m2(n2), he(n2 * maxiter), hm(n2 * maxiter), ha(n2 * maxiter);
     std::vector<int32_t> c2(n2);
     std::vector<int64_t> f2(n2);
-    epsx_scgd0_outputs o2 = *out;
-    o2.Sigma = reinterpret_cast<double *>(S2.data());
+    epsx_scgd0_outputs o2 = *out;   // Σ: the run reads the kept slots into out->Sigma itself
     o2.mu = m2.data(); o2.converged = c2.data(); o2.hist_err = he.data(); o2.hist_mu = hm.data();
     o2.hist_minA = ha.data(); o2.iter_frozen = f2.data(); o2.warm_kept = nullptr;
     WarmHooks wh;
@@ -1827,16 +1843,14 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
     wh.kept.assign(nT, 0);
     wh.err_pred.assign(nT, NAN);
     wh.err_plain.assign(nT, NAN);
+    wh.Sigma_user = out->Sigma;
     const int rc = scgd0_run_impl(ctx, &p2, &o2, &wh);
     if (rc != EPSX_OK) return rc;
-    if (!wh.judged)   // the run ended before the judge (maxiter or all converged at once): plain
-        for (int64_t t = 0; t < nT; t++) wh.kept[t] = (int32_t)t;
     out->niter = o2.niter;
     out->fm_method_used = o2.fm_method_used;
     out->n_low_precision = o2.n_low_precision;
     for (int64_t t = 0; t < nT; t++) {
         const int64_t k = wh.kept[t];
-        std::memcpy(reinterpret_cast<char *>(out->Sigma) + nS * t, S2.data() + nS * k, nS);
         out->mu[t] = m2[k];
         out->converged[t] = c2[k];
         out->iter_frozen[t] = f2[k];

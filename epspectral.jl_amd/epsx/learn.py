@@ -161,16 +161,17 @@ def predict_sigma(net, Σ0, ωs, ε_i, μ, T, precision="bf16"):
     return out.T
 
 
-def random_t2g_model(rng, nk=12, length=1024, half_width=3.0):
+def random_t2g_model(rng, nk=12, length=1024, half_width=3.0, Tlist_K=None):
     """A seeded random three-band model: t ~ U(0.2, 0.35) eV (t' = 0.03 t / 0.26), median |g|²
-    log-uniform in [5e-4, 5e-3] eV², T ~ U(50, 500) K, on `length` uniform ω nodes."""
+    log-uniform in [5e-4, 5e-3] eV², T ~ U(50, 500) K (or the given temperatures), on `length`
+    uniform ω nodes over bare μ ± half_width, or (length None) on the benchmark's graded grid."""
     from .models import t2g_setup
     t = rng.uniform(0.2, 0.35)
     g2m = float(np.exp(rng.uniform(np.log(5e-4), np.log(5e-3))))
     TK = rng.uniform(50.0, 500.0)
     seed = int(rng.integers(1 << 31))
-    return t2g_setup(nk=nk, t=t, tp=0.03 * t / 0.26, g2_median=g2m, Tlist_K=(TK,), seed=seed,
-                     uniform_grid=(length, half_width))
+    return t2g_setup(nk=nk, t=t, tp=0.03 * t / 0.26, g2_median=g2m, Tlist_K=(TK,) if Tlist_K is None else Tlist_K,
+                     seed=seed, uniform_grid=None if length is None else (length, half_width))
 
 
 def g0d0_and_converged(dev, occ, η_init=5e-3, tol=1e-6, maxiter=300, method="auto"):
