@@ -176,6 +176,17 @@ def test_c5_config_network_held_out_24(ctx):
             check_judge(dev, setup, occ, net, w)
             print("24³ held-out: plain", rc["history"]["err"].shape[1], "warm", w["iterations"],
                   "kept", (~w["fallback"]).tolist())
+            # with the BF16 / FP32 precision levels: the warm slot leaves the BF16 level at its
+            # floor as the plain one does, and the run ends at the FP64 fixed point
+            kw = dict(η_min=0.0, η_init=5e-3, ω_acoustic=OMEGA_ACOUSTIC, method="auto", batched=True)
+            wl = dev.loop(occ, maxiter=300, tol=1e-6, precision="bf16", warm_start="bf16", **kw)
+            pl = dev.loop(occ, maxiter=300, tol=1e-6, precision="bf16", **kw)
+            assert wl["converged"].all() and pl["converged"].all()
+            assert np.max(np.abs(wl["Σ"] - rc["Σ"])) < 1e-5
+            n_wl, n_pl = wl["history"]["err"].shape[1], pl["history"]["err"].shape[1]
+            print("24³ held-out, BF16 levels: plain", n_pl, "warm", n_wl, "kept", wl["info"]["warm_kept"].tolist())
+            if wl["info"]["warm_kept"].all():
+                assert n_wl <= n_pl + 5
             p = net.params
             bad = p.copy()
             bad[-2:] = 50.0
