@@ -431,6 +431,10 @@ __global__ __launch_bounds__(256) void net_conv_fwd_kernel(const uint16_t *__res
     bf16x8_t af[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) __builtin_memcpy(&af[ks], Wp + (int64_t)row * Kp + ks * 32 + kb, 16);
+    // the fragments stay in registers for the whole k range (the empty asm keeps the compiler from
+    // reloading each one from memory next to its MFMA)
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) asm volatile("" : "+v"(af[ks]));
     __syncthreads();
     if (16 * wave >= cout) return;
     f32x4_t acc[IC_NT / 16];
