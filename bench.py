@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--plain-iters", type=int, default=2, help="iterations of the plain method timed")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (C3 with the warm start, C4) measured beside the headline")
     ap.add_argument("--no-levels", action="store_true", help="skip the FP32 / BF16 precision-level runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", default="scgd0", choices=("scgd0", "ladder", "warmstart", "northstar"),
@@ -235,10 +237,48 @@ def main():
         print(f"[bench] cpu baseline: {line['cpu_baseline']}", file=sys.stderr, flush=True)
         line["c1_holstein"] = c1_pair(ctx)
         print(f"[bench] C1: {line['c1_holstein']}", file=sys.stderr, flush=True)
+    if not args.no_configs:
+        line["other_configs"] = other_configs(args, line, iters[0])
     print(json.dumps(line), flush=True)
 
 
-def ladder_bench(args):
+def other_configs(args, line, iters_c2):
+    """The other configurations, measured in the same run beside the headline (untimed by the
+    headline's clock): C3 with the learned warm start (8 T, batched, precision levels; a network
+    trained on 64 random 12³ models, 20 epochs) and C4 (the ladder, 41 Ω x 4 T, BF16 products,
+    batched).  Compact summaries; `--workload northstar` / `--workload ladder` give the full lines."""
+    import copy
+    out = {}
+    a = copy.copy(args)
+    a.models, a.epochs, a.steps, a.maxiter, a.nk = 64, 20, 1, 300, 24
+    plain_it = None
+    if "plain_fp64_gpu" in line:
+        plain_it = line["plain_fp64_gpu"]["value"] / iters_c2
+    ns = northstar_bench(a, emit=False, plain_s_per_iter=plain_it)
+    v = ns["variants"]
+    out["c3_8T_levels_warm"] = {
+        "workload": ns["config"]["workload"], "unit": "s/temperature",
+        "fp64_star": ns["fp64_star_s_per_T"], "bf16_levels": v["bf16_levels"]["s_per_T"],
+        "bf16_levels_warm": v["bf16_levels_warm"]["s_per_T"], "fp64_warm": v["fp64_warm"]["s_per_T"],
+        "iterations": {"fp64": ns["fp64_iterations"], "bf16_levels": v["bf16_levels"]["iterations"],
+                       "bf16_levels_warm": v["bf16_levels_warm"]["iterations"]},
+        "max_abs_dSigma_eV_vs_fp64": v["bf16_levels_warm"]["max_abs_dSigma_eV"],
+        "max_rel_dSigma_vs_fp64": v["bf16_levels_warm"]["max_rel_dSigma"],
+        "plain_fp64_gpu": ns["plain_fp64_gpu"]["value"],
+        "speedup_vs_plain_fp64_gpu": v["bf16_levels_warm"]["speedup_vs_plain_fp64_gpu"],
+        "warm_kept": v["bf16_levels_warm"].get("warm_kept")}
+    print(f"[bench] C3 + warm start: {out['c3_8T_levels_warm']}", file=sys.stderr, flush=True)
+    b = copy.copy(args)
+    b.nk, b.precision, b.omegas, b.coupling, b.no_cpu, b.no_batched = 24, "bf16", 0, "iid", True, False
+    ld = ladder_bench(b, emit=False)
+    out["c4_ladder"] = {"workload": ld["config"]["workload"], "value": ld["value"], "unit": ld["unit"],
+                        "total_s": ld["ms_per_step"] / 1e3, "gemm_tflops": ld["roofline"]["achieved"],
+                        "gemm_frac": ld["roofline"]["frac"], "error_vs_fp64_rtol1e-8": ld.get("error_vs_fp64_rtol1e-8")}
+    print(f"[bench] C4: {out['c4_ladder']}", file=sys.stderr, flush=True)
+    return out
+
+
+def ladder_bench(args, emit=True):
     """C4: the Kubo ladder on the converged scGD0 result of the t2g model at 16³ (ni = 331,
     nf = 8448, nshift = 82, Oh vertex rotations): scGD0 at 100, 200, 300, 400 K to 1e-6 eV by the
     star method, then GMRES at rtol 1e-2 for all 41 probe frequencies 0:5:200 meV at all four
@@ -325,7 +365,9 @@ def ladder_bench(args):
     if not args.no_cpu:
         apps_per_solve = inst_apps / nsolve
         line["cpu_baseline"] = ladder_cpu_baseline(nw, ni, ns, apps_per_solve, args.cpu_seconds)
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def ladder_gemm_label(precision):
@@ -439,7 +481,7 @@ def warmstart_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def northstar_bench(args):
+def northstar_bench(args, emit=True, plain_s_per_iter=None):
     """The combined configuration: the 24³ t2g model at 8 temperatures (50, 100, ..., 400 K) to
     tol = 1e-6 eV in one batched run with the BF16 / FP32 precision levels and the learned warm
     start (the 6-layer network trained on `--models` random 12³ models on the same graded grid),
@@ -491,7 +533,12 @@ def northstar_bench(args):
               f"the rest {1e3 * wl[6:].sum():.1f} ms", file=sys.stderr, flush=True)
     ref = res["fp64"]
     iters64 = int(ref["history"]["err"].shape[1])
-    plain = plain_sample(dev, occ, kw, iters64, args.plain_iters, nw * nm * ni * nf, 8.0 * nm * ni * nf)
+    if plain_s_per_iter is None:
+        plain = plain_sample(dev, occ, kw, iters64, args.plain_iters, nw * nm * ni * nf, 8.0 * nm * ni * nf)
+    else:   # the per-iteration time of a plain sample on the same model, scaled to this run's iterations
+        plain = {"value": plain_s_per_iter * iters64, "unit": "s/temperature",
+                 "sample": f"the headline run's plain-method sample ({plain_s_per_iter:.3f} s per iteration and "
+                           f"temperature), scaled to {iters64} iterations"}
     print(f"[bench] plain FP64 GPU: {plain['value']:.2f} s/T", file=sys.stderr, flush=True)
     rows = {}
     for name in ("fp64_warm", "bf16_levels", "bf16_levels_warm"):
@@ -519,7 +566,9 @@ def northstar_bench(args):
         "fp64_star_s_per_T": secs["fp64"], "fp64_iterations": iters64, "plain_fp64_gpu": plain,
         "variants": rows,
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def calibrate_copy():
