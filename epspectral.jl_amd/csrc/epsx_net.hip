@@ -421,12 +421,22 @@ __global__ __launch_bounds__(256) void net_conv_fwd_kernel(const uint16_t *__res
     const int xs = (int)(blockIdx.x % tiles) * IC_NT;   // grid point of the tile's first position
     const int64_t n0 = (int64_t)(blockIdx.x / tiles) * L + xs;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < NW * CI; e += 256) {
-        const int pw = e % NW, c = e / NW;
+    // the window, eight channels of one position per item (one 16-byte LDS store); q / NW as a
+    // multiply-high by ceil(2^32 / NW), exact while q NW < 2^32 (q < NW CI / 8 <= 144 x 8)
+    const unsigned rnw = 0xFFFFFFFFu / (unsigned)NW + 1u;
+    for (int q = tid; q < NW * (CI / 8); q += 256) {
+        const int cg = (int)__umulhi((unsigned)q, rnw), pw = q - cg * NW;
         const int x = xs - h + pw;
-        const float v = (c < cin && x >= 0 && x < L) ? X[c * ldx + n0 - h + pw] : 0.f;
-        xw[pw * PW + c] = f2bf(v);
+        const bool in = x >= 0 && x < L;
+        bf16x8_t v;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = 8 * cg + j;
+            v[j] = (__bf16)((in && c < cin) ? X[c * ldx + n0 - h + pw] : 0.f);
+        }
+        __builtin_memcpy(xw + pw * PW + 8 * cg, &v, 16);
     }
+    const unsigned rci = 0xFFFFFFFFu / (unsigned)CI + 1u;   // k / CI likewise (k < 18 x 32)
     const int row = 16 * wave + (lane & 15), kb = 8 * (lane >> 4);
     bf16x8_t af[KS];
 #pragma unroll
@@ -443,7 +453,7 @@ __global__ __launch_bounds__(256) void net_conv_fwd_kernel(const uint16_t *__res
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
         const int k = ks * 32 + kb;              // eight channels ci .. ci + 7 at tap t
-        const int t0 = k / CI, ci = k - t0 * CI;
+        const int t0 = (int)__umulhi((unsigned)k, rci), ci = k - t0 * CI;
         const int t = t0 < KW ? t0 : KW - 1;     // taps past KW: zero weights, any finite window value
 #pragma unroll
         for (int j = 0; j < IC_NT / 16; j++) {
