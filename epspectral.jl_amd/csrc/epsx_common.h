@@ -99,6 +99,7 @@ struct NetDev {
     DevBuf X[NET_MAX_LAYERS + 1], Z[NET_MAX_LAYERS];
     DevBuf Y, dA, dB, T, idx;
     DevBuf Wp;   // BF16 weights of the inference convolution, tap-major, per layer
+    bool wp_ok = false;   // Wp holds the current params (cleared wherever params change)
 };
 
 }  // namespace epsx

@@ -522,12 +522,19 @@ static int net_infer_dev(epsx_ctx *ctx, int64_t nb)
     const int64_t NB = nb * n.length;
     const float *P = static_cast<const float *>(n.params.p);
     const int Kp = (n.kernel * 64 + 31) / 32 * 32;   // the widest layer's k, rounded to 32
-    EPSX_TRY(dev_alloc(ctx, n.Wp, 2 * (size_t)n.layers * 64 * Kp));
+    if (!n.wp_ok) {   // the packed weights of every layer, once per set of parameters
+        EPSX_TRY(dev_alloc(ctx, n.Wp, 2 * (size_t)n.layers * 64 * Kp));
+        for (int l = 0; l < n.layers; l++) {
+            const int CI = (cin[l] + 7) / 8 * 8, Kl = (n.kernel * CI + 31) / 32 * 32;
+            hipLaunchKernelGGL(net_pack_weights_kernel, dim3(nblk(64 * Kl)), dim3(256), 0, ctx->stream, cout[l], cin[l],
+                               n.kernel, CI, Kl, P + woff[l], static_cast<uint16_t *>(n.Wp.p) + (size_t)l * 64 * Kp);
+            EPSX_HIP(ctx, hipGetLastError());
+        }
+        n.wp_ok = true;
+    }
     for (int l = 0; l < n.layers; l++) {
         const int CI = (cin[l] + 7) / 8 * 8, Kl = (n.kernel * CI + 31) / 32 * 32;
-        uint16_t *W = static_cast<uint16_t *>(n.Wp.p) + (size_t)l * 64 * Kp;
-        hipLaunchKernelGGL(net_pack_weights_kernel, dim3(nblk(64 * Kl)), dim3(256), 0, ctx->stream, cout[l], cin[l],
-                           n.kernel, CI, Kl, P + woff[l], W);
+        const uint16_t *W = static_cast<const uint16_t *>(n.Wp.p) + (size_t)l * 64 * Kp;
         const dim3 grid((unsigned)(nb * ((n.length + IC_NT - 1) / IC_NT)));
         const float *b = P + boff[l], *X = dptr<float>(n.X[l]);
         float *Y = dptr<float>(n.X[l + 1]);
@@ -722,6 +729,7 @@ int epsx_net_create(epsx_ctx *ctx, const epsx_net_config *cfg, uint64_t seed)
         }
     }
     EPSX_TRY(upload(ctx, n.params, p.data(), 4 * n.nparam));
+    n.wp_ok = false;
     EPSX_TRY(dev_alloc(ctx, n.grad, 4 * (size_t)n.nparam));
     EPSX_TRY(dev_alloc(ctx, n.m, 4 * (size_t)n.nparam));
     EPSX_TRY(dev_alloc(ctx, n.v, 4 * (size_t)n.nparam));
@@ -760,6 +768,7 @@ int epsx_net_set_params(epsx_ctx *ctx, const float *params)
     if (!ctx || !params) return EPSX_ERR_ARG;
     if (!ctx->net.ready) return set_error(ctx, EPSX_ERR_STATE, "no network: call epsx_net_create first");
     EPSX_HIP(ctx, hipMemcpyAsync(ctx->net.params.p, params, 4 * ctx->net.nparam, hipMemcpyHostToDevice, ctx->stream));
+    ctx->net.wp_ok = false;
     EPSX_HIP(ctx, hipMemsetAsync(ctx->net.m.p, 0, 4 * ctx->net.nparam, ctx->stream));
     EPSX_HIP(ctx, hipMemsetAsync(ctx->net.v.p, 0, 4 * ctx->net.nparam, ctx->stream));
     EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -845,6 +854,7 @@ int epsx_net_train(epsx_ctx *ctx, const float *x, const float *target, int64_t n
             n.step++;
             const float c1 = (float)(1.0 - std::pow(tp->beta1, (double)n.step));
             const float c2 = (float)(1.0 - std::pow(tp->beta2, (double)n.step));
+            n.wp_ok = false;
             hipLaunchKernelGGL(net_adam_kernel, dim3(nblk(n.nparam)), dim3(256), 0, ctx->stream, n.nparam,
                                dptr<float>(n.params), dptr<float>(n.grad), dptr<float>(n.m), dptr<float>(n.v),
                                (float)tp->lr, (float)tp->beta1, (float)tp->beta2, (float)tp->eps,
