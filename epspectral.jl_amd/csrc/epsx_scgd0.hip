@@ -150,11 +150,15 @@ __device__ inline double block_sum(double v, double *red)
 
 // The spectral DOS of Σ_new (already copied into Σ) in one pass over (a, i) (src/scgd0.jl:304-308):
 // A[a, i] = -Im(1/z)/π = Im z / (π |z|²) with z = ωa - ε_i - Σ(ωa), formed in registers, never stored.
-// One block per (tile of 64 a, chunk of DOS_IC outer states, slot): lanes along a, the 4 waves
-// over i ≡ wave mod 4 of the chunk.  It writes the chunk's partial Dp[a, chunk] = Σ_i w_i A[a, i]
-// (the wave sums in order) and, per i, the tile's partial integral ip[i, tile] = Σ_a A[a, i] Wa[a]
-// (a wave butterfly); dos_finish_kernel sums them in order.
-constexpr int DOS_IC = 128;
+// One block per (tile of DOS_TA = 256 a, chunk of DOS_IC = 32 outer states, slot): lane l holds the
+// DOS_RA = 4 consecutive a = 256 tile + 4 l + r, the 4 waves take the states i ≡ wave mod 4 of the
+// chunk (DOS_KI = 8 each).  It writes the chunk's partial Dp[a, chunk] = Σ_i w_i A[a, i] (each
+// wave's sum in i order, the wave sums in order) and, per i, the tile's partial integral
+// ip[i, tile] = Σ_a A[a, i] Wa[a]: every lane's four terms in order, then the 64 lane values
+// through LDS (lane 8 k + p of the wave sums the values 8 q + p of its k-th state in q order, then
+// a butterfly over p).  dos_finish_kernel sums the partials in order.
+constexpr int DOS_IC = 32, DOS_RA = 4, DOS_TA = 64 * DOS_RA, DOS_KI = DOS_IC / 4;
+static_assert(DOS_KI * 8 == 64, "one ipart group of 8 lanes per state of a wave");
 __global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_t nwa, const double *__restrict__ wsa,
                                                   const double *__restrict__ e_i, const int32_t *__restrict__ jsp,
                                                   const double *__restrict__ tsp, const int32_t *__restrict__ tidx,
@@ -162,43 +166,67 @@ __global__ __launch_bounds__(256) void dos_kernel(int64_t nw, int64_t ni, int64_
                                                   const double *__restrict__ Wa, double *__restrict__ Dp,
                                                   double *__restrict__ ipart, const int32_t *__restrict__ act)
 {
-    __shared__ double part[4][64];
+    __shared__ double vred[4][DOS_KI][64];
+    __shared__ double part[4][DOS_TA];
     const int z = blockIdx.z;
     if (act && !act[tidx[z]]) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t a = blockIdx.x * 64 + lane, ntile = gridDim.x, nchunk = gridDim.y;
-    const int64_t i0 = (int64_t)blockIdx.y * DOS_IC, i1 = i0 + DOS_IC < ni ? i0 + DOS_IC : ni;
-    const bool on = a < nwa;
+    const int64_t ntile = gridDim.x, nchunk = gridDim.y;
+    const int64_t abase = (int64_t)blockIdx.x * DOS_TA, a0 = abase + DOS_RA * lane;
+    const int64_t i0 = (int64_t)blockIdx.y * DOS_IC;
     const double2 *S = Sg + nw * ni * (int64_t)tidx[z];
-    int64_t j = 0;
-    double t = 0, x = 0, wa = 0;
-    if (on) {
-        j = jsp[a] - 1;
-        t = tsp[a];
-        x = wsa[a];
-        wa = Wa[a];
-    }
-    double acc = 0;
-    for (int64_t i = i0 + wave; i < i1; i += 4) {
-        double A = 0;
-        if (on) {
-            const double2 *St = S + nw * i;
-            const double2 lo = St[j], hi = St[j + 1];
-            const double sr = (1 - t) * lo.x + t * hi.x, si = (1 - t) * lo.y + t * hi.y;
-            const double c = x - e_i[i] - sr, d = -si;
-            // -Im(1/z) / π = d / (π |z|²): one division
-            A = d / (PI_D * fma(c, c, d * d));
-            acc = fma(A, w_i[i], acc);
-        }
-        double v = A * wa;
+    int j[DOS_RA];
+    double t[DOS_RA], x[DOS_RA], wa[DOS_RA], acc[DOS_RA];
+    bool on[DOS_RA];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if (lane == 0) ipart[i + ni * (blockIdx.x + ntile * (int64_t)z)] = v;
+    for (int r = 0; r < DOS_RA; r++) {
+        const int64_t a = a0 + r;
+        on[r] = a < nwa;
+        j[r] = on[r] ? jsp[a] - 1 : 0;
+        t[r] = on[r] ? tsp[a] : 0.0;
+        x[r] = on[r] ? wsa[a] : 0.0;
+        wa[r] = on[r] ? Wa[a] : 0.0;
+        acc[r] = 0.0;
     }
-    part[wave][lane] = acc;
+#pragma unroll
+    for (int k = 0; k < DOS_KI; k++) {
+        const int64_t i = i0 + wave + 4 * k;
+        double v = 0;
+        if (i < ni) {
+            const double2 *St = S + nw * i;
+            const double ei = e_i[i], wi = w_i[i];
+#pragma unroll
+            for (int r = 0; r < DOS_RA; r++) {
+                if (!on[r]) continue;
+                const double2 lo = St[j[r]], hi = St[j[r] + 1];
+                const double sr = (1 - t[r]) * lo.x + t[r] * hi.x, si = (1 - t[r]) * lo.y + t[r] * hi.y;
+                const double c = x[r] - ei - sr, d = -si;
+                // -Im(1/z) / π = d / (π |z|²): one division
+                const double A = d / (PI_D * fma(c, c, d * d));
+                acc[r] = fma(A, wi, acc[r]);
+                v = fma(A, wa[r], v);
+            }
+        }
+        vred[wave][k][lane] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < DOS_RA; r++) part[wave][DOS_RA * lane + r] = acc[r];
     __syncthreads();
-    if (wave == 0 && on)
-        Dp[a + nwa * (blockIdx.y + nchunk * (int64_t)z)] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    {
+        const int k = lane >> 3, p = lane & 7;
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += vred[wave][k][8 * q + p];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        const int64_t i = i0 + wave + 4 * k;
+        if (p == 0 && i < ni) ipart[i + ni * (blockIdx.x + ntile * (int64_t)z)] = s;
+    }
+    const int64_t a = abase + threadIdx.x;
+    if (a < nwa)
+        Dp[a + nwa * (blockIdx.y + nchunk * (int64_t)z)] =
+            ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) + part[3][threadIdx.x];
 }
 
 // The sums after dos_kernel for the active slots, in one launch: D[a] = Wa[a] Σ_chunk Dp[a, chunk]
@@ -232,31 +260,6 @@ __global__ __launch_bounds__(256) void dos_finish_kernel(int64_t nwa, int64_t nc
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     if (lane == 0) integ[i + ni * (int64_t)z] = v;
-}
-
-// min_i ∫A per slot (NaN wins)
-__global__ __launch_bounds__(256) void min_kernel(int64_t ni, const double *__restrict__ integ,
-                                                  const int32_t *__restrict__ tidx, double *__restrict__ minA,
-                                                  const int32_t *__restrict__ act)
-{
-    __shared__ double red[256];
-    const int z = blockIdx.x;
-    if (act && !act[tidx[z]]) return;
-    double m = INFINITY;
-    for (int64_t i = threadIdx.x; i < ni; i += 256) {
-        const double v = integ[i + ni * (int64_t)z];
-        m = (v < m || v != v) ? v : m;
-    }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            const double o = red[threadIdx.x + s];
-            if (o < red[threadIdx.x] || o != o) red[threadIdx.x] = o;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) minA[tidx[z]] = red[0];
 }
 
 // Device-side bookkeeping of the graph-replayed loop: the history row of this iteration, the
@@ -302,7 +305,27 @@ struct MuArgs {
     double *mu;              // per temperature index
     int32_t *status;         // per temperature index
     const int32_t *act;      // convergence mask or null
+    int64_t ni;
+    const double *integ;     // ∫A per outer state [ni, nz]
+    double *minA;            // per temperature index: min_i ∫A
 };
+
+// min_i ∫A of slot z over one wave (NaN wins), in every lane: lane-strided, then a butterfly.
+__device__ inline double integ_wave_min(int64_t ni, const double *__restrict__ integ, int z)
+{
+    const int lane = threadIdx.x & 63;
+    double m = INFINITY;
+    for (int64_t i = lane; i < ni; i += 64) {
+        const double v = integ[i + ni * (int64_t)z];
+        m = (v < m || v != v) ? v : m;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v = __shfl_xor(m, o);
+        m = (v < m || v != v) ? v : m;
+    }
+    return m;
+}
 
 // Sum over a wave by an xor butterfly: every lane adds the same pairs in the same order, so all
 // lanes hold the bitwise same total.
@@ -313,15 +336,14 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-constexpr int MU_REG = 8;  // spectral points per thread held in registers (nωa ≤ 8192)
-
 // The μ root of n(μ) = n_target on the bracket extrema(ωs_spectral) ± 0.1 (src/scgd0.jl:310-324,
 // :361-366), one block of 16 waves per temperature.  Each n(μ) is a fixed-order sum (per thread,
 // per wave, then the 16 wave totals in order), which every thread ends with, so every thread
 // takes the same bisection step without a broadcast.  The bracket is bisected on the ordered bit
 // pattern until two adjacent floats hold the sign change.  For T > sqrt(eps) each thread keeps
-// its D_a and e^{±ωa/T} (as mantissa and binary exponent) in registers, so a step costs one
-// exponential per thread: e^{(ωa-μ)/T} = e^{ωa/T} e^{-μ/T}.
+// its NREG values D_a and e^{±ωa/T} (as mantissa and binary exponent) in registers (nωa ≤ 1024
+// NREG), so a step costs one exponential per thread: e^{(ωa-μ)/T} = e^{ωa/T} e^{-μ/T}.
+template <int NREG>
 __device__ __forceinline__ void mu_root(const MuArgs &m, int z, int tcol, double (*part)[16])
 {
     const int tid = threadIdx.x, wave = tid >> 6;
@@ -329,12 +351,12 @@ __device__ __forceinline__ void mu_root(const MuArgs &m, int z, int tcol, double
     const double n_target = m.nlist[tcol] + m.nelec;
     const int holes = m.is_semi && m.nlist[tcol] < 0;
     const double *D = m.D + m.nwa * (int64_t)z;
-    const bool regs = T > SQRT_EPS && m.nwa <= 1024 * MU_REG;
-    double Dv[MU_REG], Mv[MU_REG];
-    int kv[MU_REG];
+    const bool regs = T > SQRT_EPS && m.nwa <= 1024 * NREG;
+    double Dv[NREG], Mv[NREG];
+    int kv[NREG];
     if (regs) {
 #pragma unroll
-        for (int r = 0; r < MU_REG; r++) {
+        for (int r = 0; r < NREG; r++) {
             const int64_t a = tid + 1024 * r;
             Dv[r] = 0.0; Mv[r] = 1.0; kv[r] = 0;
             if (a < m.nwa) {
@@ -351,10 +373,18 @@ __device__ __forceinline__ void mu_root(const MuArgs &m, int z, int tcol, double
             int km;
             split_exp((holes ? mu : -mu) / T, Mm, km);
 #pragma unroll
-            for (int r = 0; r < MU_REG; r++) {
+            for (int r = 0; r < NREG; r++) {
                 int k = kv[r] + km;
                 k = k > 900 ? 900 : (k < -900 ? -900 : k);
-                acc = fma(Dv[r], 1.0 / (ldexp(Mv[r] * Mm, k) + 1.0), acc);
+                // 1/y by the hardware reciprocal and two Newton steps (y in [1, 2^902]: the
+                // reciprocal and the residuals stay normal)
+                const double y = ldexp(Mv[r] * Mm, k) + 1.0;
+                double q = __builtin_amdgcn_rcp(y);
+                double e = fma(-y, q, 1.0);
+                q = fma(q, e, q);
+                e = fma(-y, q, 1.0);
+                q = fma(q, e, q);
+                acc = fma(Dv[r], q, acc);
             }
         } else {
             for (int64_t a = tid; a < m.nwa; a += 1024) {
@@ -372,32 +402,44 @@ __device__ __forceinline__ void mu_root(const MuArgs &m, int z, int tcol, double
         if (holes) s = m.W - s;
         return (s + m.nsb) * m.spin - n_target;
     };
-    double a = m.wsa[0] - 0.1, b = m.wsa[m.nwa - 1] + 0.1;
-    double fa = excess(a), fb = excess(b);
-    if (fa * fb > 0) {
-        if (tid == 0) m.status[tcol] = 2;
-        return;
+    // The previous iterate's μ is close to this root: two evaluations at μ_prev ∓ δ come first.
+    // When they differ in sign, [μ_prev - δ, μ_prev + δ] is the bracket (n is monotone, so the
+    // ends of the full bracket would differ in sign too); else the full bracket's ends are
+    // evaluated (no root when they agree in sign) and narrowed to the side of μ_prev ∓ δ that
+    // holds the sign change.  Every end kept still brackets the one sign change, so the root is
+    // unchanged.
+    double a = m.wsa[0] - 0.1, b = m.wsa[m.nwa - 1] + 0.1, fa, fb;
+    const double mp = m.mu[tcol], dl = 1e-4 * (b - a);
+    const double lo = mp - dl, hi = mp + dl;
+    const bool near = lo > a && hi < b;
+    double flo = 0, fhi = 0;
+    if (near) {
+        flo = excess(lo);
+        fhi = excess(hi);
     }
-    if (tid == 0) m.status[tcol] = 0;
-    if (fa == 0 || fb == 0) {
-        if (tid == 0) m.mu[tcol] = fa == 0 ? a : b;
-        return;
-    }
-    // The previous iterate's μ is close to this root: two evaluations at μ_prev ∓ δ narrow the
-    // bracket to [μ_prev - δ, μ_prev + δ] when the sign change lies there, else to the side that
-    // holds it.  Every end kept still brackets the one sign change, so the root is unchanged.
-    {
-        const double mp = m.mu[tcol], dl = 1e-4 * (b - a);
-        const double lo = mp - dl, hi = mp + dl;
-        if (lo > a && hi < b) {
-            const double flo = excess(lo);
+    if (near && ((flo < 0 && fhi > 0) || (flo > 0 && fhi < 0))) {
+        if (tid == 0) m.status[tcol] = 0;
+        a = lo; fa = flo;
+        b = hi; fb = fhi;
+    } else {
+        fa = excess(a);
+        fb = excess(b);
+        if (fa * fb > 0) {
+            if (tid == 0) m.status[tcol] = 2;
+            return;
+        }
+        if (tid == 0) m.status[tcol] = 0;
+        if (fa == 0 || fb == 0) {
+            if (tid == 0) m.mu[tcol] = fa == 0 ? a : b;
+            return;
+        }
+        if (near) {
             if (flo == 0) {
                 if (tid == 0) m.mu[tcol] = lo;
                 return;
             }
             if ((flo > 0) == (fa > 0)) {
                 a = lo; fa = flo;
-                const double fhi = excess(hi);
                 if (fhi == 0) {
                     if (tid == 0) m.mu[tcol] = hi;
                     return;
@@ -499,28 +541,36 @@ __device__ inline void err_book(double m, int t, double *__restrict__ err, const
     }
 }
 
-// μ of every active slot (one block of 16 waves each) and the err bookkeeping of the iteration
-// after μ (its history row records the new μ).
+// μ of every active slot (one block of 16 waves each), min_i ∫A, and the err bookkeeping of the
+// iteration after μ (its history row records the new μ).
+template <int NREG>
 __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m, int nb, const double *__restrict__ bmax,
                                                   double *__restrict__ err, GraphBook g)
 {
     __shared__ double part[2][16];
-    __shared__ double emax;
+    __shared__ double emax, amin;
     const int z = blockIdx.x;
     const int tcol = m.tidx[z];
     if (m.act && !m.act[tcol]) {
         if (threadIdx.x == 0) host_report(g, tcol, 0);
         return;
     }
-    // wave 0 reduces the err maxima first (its loads in flight with those of D); the value
-    // waits in LDS, not in registers, during the root search
+    // waves 0 and 1 reduce the err maxima and min ∫A first (their loads in flight with those of
+    // D); the values wait in LDS, not in registers, during the root search
     if (threadIdx.x < 64) {
         const double e = err_wave_max(err_lane_max(nb, bmax, z));
         if (threadIdx.x == 0) emax = e;
+    } else if (threadIdx.x < 128) {
+        const double a = integ_wave_min(m.ni, m.integ, z);
+        if (threadIdx.x == 64) amin = a;
     }
-    mu_root(m, z, tcol, part);
-    // thread 0 wrote μ (and emax); the bookkeeping reads them on the same thread
-    if (threadIdx.x == 0) err_book(emax, tcol, err, g);
+    mu_root<NREG>(m, z, tcol, part);
+    // the root search passed a barrier after emax and amin were written; thread 0 wrote μ, and
+    // the bookkeeping reads them on the same thread
+    if (threadIdx.x == 0) {
+        m.minA[tcol] = amin;
+        err_book(emax, tcol, err, g);
+    }
 }
 
 // err = max |Σ_new - Σ| over the slice, then Σ ← Σ_new (src/scgd0.jl:326-328): per-block maxima,
@@ -610,9 +660,11 @@ __global__ void phase_gates_kernel(int64_t nT, const int32_t *__restrict__ act, 
 }
 
 
+// err and min_i ∫A of every active slot and the err bookkeeping, for a run with μ fixed.
 __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__restrict__ bmax,
                                                       const int32_t *__restrict__ tidx, double *__restrict__ err,
-                                                      GraphBook g)
+                                                      int64_t ni, const double *__restrict__ integ,
+                                                      double *__restrict__ minA, GraphBook g)
 {
     const int z = blockIdx.x;
     const int t = tidx[z];
@@ -620,8 +672,12 @@ __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__r
         if (threadIdx.x == 0) host_report(g, t, 0);
         return;
     }
+    const double am = integ_wave_min(ni, integ, z);
     const double e = err_wave_max(err_lane_max(nb, bmax, z));
-    if (threadIdx.x == 0) err_book(e, t, err, g);
+    if (threadIdx.x == 0) {
+        minA[t] = am;
+        err_book(e, t, err, g);
+    }
 }
 
 inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
@@ -921,7 +977,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     EPSX_TRY(tmp_alloc(ctx, B.ImS.b, 8 * nS));
     EPSX_TRY(tmp_alloc(ctx, B.D.b, 8 * (size_t)nwa * nT));
     EPSX_TRY(tmp_alloc(ctx, B.Dp.b, 8 * (size_t)nwa * ((ni + DOS_IC - 1) / DOS_IC) * nT));   // per-chunk D partials
-    EPSX_TRY(tmp_alloc(ctx, B.ipart.b, 8 * (size_t)ni * ((nwa + 63) / 64) * nT));            // per-tile integrals
+    EPSX_TRY(tmp_alloc(ctx, B.ipart.b, 8 * (size_t)ni * ((nwa + DOS_TA - 1) / DOS_TA) * nT));            // per-tile integrals
     EPSX_TRY(tmp_alloc(ctx, B.integ.b, 8 * (size_t)ni * nT));
     size_t npart = 0;  // the largest partial buffer any launch of this run needs
     for (int64_t a = 1; a <= (par->batched ? nT : 1); a++) {
@@ -1388,7 +1444,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             EPSX_HIP(ctx, hipGetLastError());
         }
         // 6. spectral DOS and min ∫A
-        const unsigned ntile = nblk(nwa, 64), nchunk = nblk(ni, DOS_IC);
+        const unsigned ntile = nblk(nwa, DOS_TA), nchunk = nblk(ni, DOS_IC);
         hipLaunchKernelGGL(dos_kernel, dim3(ntile, nchunk, nA), dim3(256), 0, st, nw, ni, nwa, dptr<double>(s.wsa),
                            dptr<double>(s.e_i), dptr<int32_t>(s.jsp), dptr<double>(s.tsp), d_tidx,
                            B.S.ptr<double2>(), dptr<double>(s.w_i), dptr<double>(s.Wa), B.Dp.ptr<double>(),
@@ -1399,10 +1455,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                            B.ipart.ptr<double>(), d_tidx, B.D.ptr<double>(), B.integ.ptr<double>(), d_act,
                            d_act ? const_cast<int32_t *>(book.it) : nullptr);
         EPSX_HIP(ctx, hipGetLastError());
-        hipLaunchKernelGGL(min_kernel, dim3(nA), dim3(256), 0, st, ni, B.integ.ptr<double>(), d_tidx,
-                           B.minA.ptr<double>(), d_act);
-        EPSX_HIP(ctx, hipGetLastError());
-        // 7. μ
+        // 7. μ (and min ∫A)
         if (!par->fix_mu) {
             MuArgs m;
             m.nwa = nwa; m.wsa = dptr<double>(s.wsa); m.D = B.D.ptr<double>(); m.tidx = d_tidx;
@@ -1411,14 +1464,22 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             m.is_semi = par->is_semiconductor; m.mu = B.mu.ptr<double>();
             m.status = book.status ? book.status : B.status.ptr<int32_t>();
             m.act = d_act;
-            hipLaunchKernelGGL(mu_kernel, dim3(nA), dim3(1024), 0, st, m, nb_err, B.bmax.ptr<double>(),
-                               B.err.ptr<double>(), book);
+            m.ni = ni; m.integ = B.integ.ptr<double>(); m.minA = B.minA.ptr<double>();
+            // the spectral points each thread keeps in registers: the fewest that hold nωa
+            const auto mu_launch = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(nA), dim3(1024), 0, st, m, nb_err, B.bmax.ptr<double>(),
+                                   B.err.ptr<double>(), book);
+            };
+            if (nwa <= 1024 * 2) mu_launch(mu_kernel<2>);
+            else if (nwa <= 1024 * 4) mu_launch(mu_kernel<4>);
+            else if (nwa <= 1024 * 6) mu_launch(mu_kernel<6>);
+            else mu_launch(mu_kernel<8>);
             EPSX_HIP(ctx, hipGetLastError());
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[3], st));
         if (par->fix_mu) {
             hipLaunchKernelGGL(err_final_kernel, dim3(nA), dim3(64), 0, st, nb_err, B.bmax.ptr<double>(), d_tidx,
-                               B.err.ptr<double>(), book);
+                               B.err.ptr<double>(), ni, B.integ.ptr<double>(), B.minA.ptr<double>(), book);
             EPSX_HIP(ctx, hipGetLastError());
         }
         if (events) EPSX_HIP(ctx, hipEventRecord(ev[4], st));
@@ -1816,6 +1877,11 @@ extern "C" int epsx_scgd0_run(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_
         return set_error(ctx, EPSX_ERR_ARG, "warm_start must be 0, EPSX_PREC_FP32 or EPSX_PREC_BF16, got %d",
                          par->warm_start);
     if (!ctx->net.ready) return set_error(ctx, EPSX_ERR_STATE, "the warm start needs a network: call epsx_net_create");
+    if (!ctx->s.ready) return set_error(ctx, EPSX_ERR_STATE, "epsx_scgd0_run before epsx_scgd0_set_problem");
+    if (ctx->net.c_in != 5 || ctx->net.c_out != 2 || ctx->net.length != ctx->s.nw)
+        return set_error(ctx, EPSX_ERR_ARG, "the warm-start network takes 5 input and 2 output channels over the %lld "
+                         "nodes of ωs, got %d, %d and %lld", (long long)ctx->s.nw, ctx->net.c_in, ctx->net.c_out,
+                         (long long)ctx->net.length);
     if (!out->Sigma || !out->mu || !out->converged || !out->hist_err || !out->hist_mu || !out->hist_minA ||
         !out->wall || !out->iter_frozen)
         return set_error(ctx, EPSX_ERR_ARG, "scgd0 run: null output array");

@@ -198,3 +198,28 @@ def test_c5_config_network_held_out_24(ctx):
             done += 1
             break
         assert done == 1, "no held-out 24³ model with a converging plain run"
+
+
+def test_warm_start_argument_checks(ctx):
+    """The warm start's preconditions: a network over the nω nodes, batched runs, no Σ_init,
+    maxiter >= 3; each refused with an ArgumentError before anything runs."""
+    rng = np.random.default_rng(5)
+    setup, meta = random_t2g_model(rng, nk=8, length=128)
+    dev = epsx.Scgd0Device(setup, ctx)
+    occ = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    kw = dict(η_min=0.0, η_init=5e-3, ω_acoustic=OMEGA_ACOUSTIC, method="auto", tol=1e-6)
+    WarmStartNet(ctx, 64, seed=0)   # a network over another length
+    with pytest.raises(epsx.ArgumentError, match="nodes of ωs"):
+        dev.loop(occ, maxiter=10, batched=True, warm_start="bf16", **kw)
+    WarmStartNet(ctx, 128, seed=0)
+    with pytest.raises(epsx.ArgumentError, match="batched"):
+        dev.loop(occ, maxiter=10, batched=False, warm_start="bf16", **kw)
+    with pytest.raises(epsx.ArgumentError, match="maxiter >= 3"):
+        dev.loop(occ, maxiter=2, batched=True, warm_start="bf16", **kw)
+    Σ0 = np.full((len(setup.ωs), setup.dims[1], len(occ)), -5e-3j)
+    with pytest.raises(epsx.ArgumentError, match="Σ_init"):
+        dev.loop(occ, maxiter=10, batched=True, warm_start="bf16", Σ_init=Σ0, **kw)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r = dev.loop(occ, maxiter=10, batched=True, warm_start="fp32", **kw)
+    assert r["info"]["warm_kept"].shape == (len(occ),)
