@@ -247,8 +247,17 @@ __global__ __launch_bounds__(256) void dos_finish_kernel(int64_t nwa, int64_t nc
     if (blockIdx.x < nbd) {
         const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
         if (a >= nwa) return;
+        // in chunk order, eight loads in flight at a time
+        const double *p = Dp + a + nwa * nchunk * (int64_t)z;
         double sum = 0;
-        for (int64_t c = 0; c < nchunk; c++) sum += Dp[a + nwa * (c + nchunk * (int64_t)z)];
+        for (int64_t c0 = 0; c0 < nchunk; c0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = c0 + u < nchunk ? p[nwa * (c0 + u)] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (c0 + u < nchunk) sum += v[u];
+        }
         D[a + nwa * (int64_t)z] = sum * Wa[a];
         return;
     }
@@ -310,20 +319,24 @@ struct MuArgs {
     double *minA;            // per temperature index: min_i ∫A
 };
 
-// min_i ∫A of slot z over one wave (NaN wins), in every lane: lane-strided, then a butterfly.
-__device__ inline double integ_wave_min(int64_t ni, const double *__restrict__ integ, int z)
+// The smaller of two ∫A values, NaN winning: the min over a set is then the same in any order.
+__device__ inline double nan_min(double m, double v) { return (v < m || v != v) ? v : m; }
+
+// min of slot z's ∫A over the states i ≡ r (mod stride), r = the thread's rank among `nthr`
+// (stride = nthr), then over the wave (a butterfly), in every lane; four loads in flight.
+__device__ inline double integ_wave_min(int64_t ni, const double *__restrict__ integ, int z, int rank, int nthr)
 {
-    const int lane = threadIdx.x & 63;
+    const double *p = integ + ni * (int64_t)z;
     double m = INFINITY;
-    for (int64_t i = lane; i < ni; i += 64) {
-        const double v = integ[i + ni * (int64_t)z];
-        m = (v < m || v != v) ? v : m;
+    for (int64_t i0 = rank; i0 < ni; i0 += 4 * (int64_t)nthr) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = i0 + u * nthr < ni ? p[i0 + u * nthr] : INFINITY;
+#pragma unroll
+        for (int u = 0; u < 4; u++) m = nan_min(m, v[u]);
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double v = __shfl_xor(m, o);
-        m = (v < m || v != v) ? v : m;
-    }
+    for (int o = 32; o > 0; o >>= 1) m = nan_min(m, __shfl_xor(m, o));
     return m;
 }
 
@@ -548,27 +561,30 @@ __global__ __launch_bounds__(1024) void mu_kernel(MuArgs m, int nb, const double
                                                   double *__restrict__ err, GraphBook g)
 {
     __shared__ double part[2][16];
-    __shared__ double emax, amin;
+    __shared__ double emax, amin[15];
     const int z = blockIdx.x;
     const int tcol = m.tidx[z];
     if (m.act && !m.act[tcol]) {
         if (threadIdx.x == 0) host_report(g, tcol, 0);
         return;
     }
-    // waves 0 and 1 reduce the err maxima and min ∫A first (their loads in flight with those of
-    // D); the values wait in LDS, not in registers, during the root search
+    // wave 0 reduces the err maxima and waves 1-15 min ∫A first (their loads in flight with those
+    // of D); the values wait in LDS, not in registers, during the root search
     if (threadIdx.x < 64) {
         const double e = err_wave_max(err_lane_max(nb, bmax, z));
         if (threadIdx.x == 0) emax = e;
-    } else if (threadIdx.x < 128) {
-        const double a = integ_wave_min(m.ni, m.integ, z);
-        if (threadIdx.x == 64) amin = a;
+    } else {
+        const double a = integ_wave_min(m.ni, m.integ, z, threadIdx.x - 64, 960);
+        if ((threadIdx.x & 63) == 0) amin[(threadIdx.x >> 6) - 1] = a;
     }
     mu_root<NREG>(m, z, tcol, part);
     // the root search passed a barrier after emax and amin were written; thread 0 wrote μ, and
     // the bookkeeping reads them on the same thread
     if (threadIdx.x == 0) {
-        m.minA[tcol] = amin;
+        double a = amin[0];
+#pragma unroll
+        for (int w = 1; w < 15; w++) a = nan_min(a, amin[w]);
+        m.minA[tcol] = a;
         err_book(emax, tcol, err, g);
     }
 }
@@ -672,7 +688,7 @@ __global__ __launch_bounds__(64) void err_final_kernel(int nb, const double *__r
         if (threadIdx.x == 0) host_report(g, t, 0);
         return;
     }
-    const double am = integ_wave_min(ni, integ, z);
+    const double am = integ_wave_min(ni, integ, z, threadIdx.x, 64);
     const double e = err_wave_max(err_lane_max(nb, bmax, z));
     if (threadIdx.x == 0) {
         minA[t] = am;
