@@ -137,7 +137,8 @@ def main():
         if args.precision == "fp64":
             own = os.environ.get("EPSX_STAR_BLAS", "1") == "0" or os.environ.get("EPSX_GEMM_SHAPE")
             peak = FP64_PEAK_TFLOPS
-            kname = ("dgemm_buf_kernel (star Fan-Migdal GEMM, FP64 MFMA 16x16x4)" if own
+            kname = ("own FP64 MFMA GEMM, autotuned tile shape: dgemm_reg_kernel 256x32 or dgemm_buf_kernel "
+                     "(star Fan-Migdal GEMM, v_mfma_f64_16x16x4f64)" if own
                      else "hipBLASLt FP64 GEMM (star Fan-Migdal product, strided batch with G̃ shared)")
             tr = traffic("dgemm_kernel" if own else "hipblaslt_star", args.nk, nT)
         else:
@@ -374,7 +375,8 @@ def ladder_gemm_label(precision):
     """The kernel that runs the shift-table GEMM W = J·B̃: rocBLAS (a plain GEMM), or the library's
     own MFMA kernels with EPSX_LADDER_BLAS=0."""
     if os.environ.get("EPSX_LADDER_BLAS", "1") == "0":
-        return ("dgemm_buf_kernel (shift-table GEMM W = J B̃, FP64 MFMA 16x16x4)" if precision == "fp64"
+        return ("own FP64 MFMA GEMM, autotuned tile shape: dgemm_buf_kernel or dgemm_reg_kernel "
+                "(shift-table GEMM W = J B̃, v_mfma_f64_16x16x4f64)" if precision == "fp64"
                 else f"lpkgemm_kernel (shift-table GEMM W = J B̃ from stored {precision} operands, FP32 accumulation)")
     return {"fp64": "rocBLAS dgemm_strided_batched (shift-table GEMM W = J B̃, every shift of a column block)",
             "fp32": "rocBLAS gemm_strided_batched_ex FP32 (shift-table GEMM W = J B̃ from stored FP32 operands)",

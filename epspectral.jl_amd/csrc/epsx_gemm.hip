@@ -343,6 +343,110 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_buf_kernel(GemmArg
             }
 }
 
+// The thin-tile product without LDS: a 256 x 32 tile per workgroup of 4 waves, each wave owning
+// 64 rows as 4 x 2 accumulators of 16x16.  A wave's rows of A are its own, so it loads its MFMA
+// operand fragments of A straight from memory into registers (lane l: row l mod 16 of a 16-row
+// block at k = l / 16 of a 4-deep step: 16 consecutive doubles per 16 lanes); every wave loads
+// the 32 columns of B the same way (one wave from L2, the others mostly from L1).  No LDS and no
+// barrier: each wave runs its own software pipeline with the loads of KP - 1 k-tiles of KT in
+// flight ahead of the MFMAs (at most 63 loads in flight per wave: 6 KT / 4 loads per k-tile).
+// The k order per accumulator is that of the other shapes (4-deep steps in ascending k within
+// the unit's split), so for the same split it gives the same bits.
+template <int KT, int KP>
+__global__ __launch_bounds__(256, 1) void dgemm_reg_kernel(GemmArgs p)
+{
+    constexpr int BM = 256, BN = 32, TM = 4, TN = 2, KQ = KT / 4, R = GK / KT;
+    static_assert(GK % KT == 0 && KT % 4 == 0 && (KP - 1) * KQ * (TM + TN) <= 63, "pipeline");
+    int tm, tn, s, S;
+    int64_t z;
+    if (!unit_of_block(p, tm, tn, s, z, S)) return;
+    const int kt0 = R * (int)((int64_t)p.ktiles * s / S), kt1 = R * (int)((int64_t)p.ktiles * (s + 1) / S);
+    // the wave index as a wave-uniform value, so the descriptors built from it stay scalar
+    const int lane = (int)threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int64_t row0 = (int64_t)tm * BM + 64 * wave, col0 = (int64_t)tn * BN;
+    const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
+    const int64_t ks0 = (int64_t)kt0 * KT, ke = (int64_t)kt1 * KT < k ? (int64_t)kt1 * KT : k;
+    const int64_t cl = col0 + BN < n ? col0 + BN : n;   // end of the unit's columns
+    const double *A = p.A + p.sA * opi(p, z, 1) + row0 + lda * ks0;
+    const double *B = p.B + p.sB * opi(p, z, 2) + ks0 + ldb * col0;
+    // descriptor ranges: A up to row m - 1 of column ke - 1 (a row at or past m reads the next
+    // column or 0; its outputs are not stored), B up to row ke - 1 of column cl - 1
+    const int64_t extA = (ke > ks0 && row0 < m) ? (m - row0) + lda * (ke - 1 - ks0) : 0;
+    const int64_t extB = ke > ks0 ? (ke - ks0) + ldb * (cl - 1 - col0) : 0;
+    const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc((void *)A, (short)0, (int)(8 * extA), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc((void *)B, (short)0, (int)(8 * extB), 0x00020000);
+    const uint32_t voA = (uint32_t)(8 * ((lane & 15) + lda * (lane >> 4)));
+    const uint32_t voB = (uint32_t)(8 * ((lane >> 4) + ldb * (lane & 15)));
+    const int nkt = kt1 - kt0;
+    double fa[KP][KQ][TM], fb[KP][KQ][TN];
+    // k-tile r of the unit (k from ks0 + KT r) into pipeline slot u
+    auto load = [&](int u, int r) {
+#pragma unroll
+        for (int q = 0; q < KQ; q++) {
+            const uint32_t kofs = (uint32_t)(KT * r + 4 * q);
+#pragma unroll
+            for (int a = 0; a < TM; a++) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsA, voA, 8u * (16u * a + (uint32_t)lda * kofs), 0);
+                __builtin_memcpy(&fa[u][q][a], &v, 8);
+            }
+#pragma unroll
+            for (int b = 0; b < TN; b++) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsB, voB, 8u * (kofs + 16u * (uint32_t)ldb * b), 0);
+                __builtin_memcpy(&fb[u][q][b], &v, 8);
+            }
+        }
+    };
+    double4_t acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++) acc[a][b] = (double4_t){0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < KP - 1; u++)
+        if (u < nkt) load(u, u);
+    for (int r0 = 0; r0 < nkt; r0 += KP) {
+#pragma unroll
+        for (int u = 0; u < KP; u++) {
+            const int r = r0 + u;
+            // the slot computed last step is free: the loads of tile r + KP - 1 go there, also
+            // past the unit's last tile (zeros from the descriptor range, never used), so every
+            // step issues the same loads and the waits count them statically
+            load((u + KP - 1) % KP, r + KP - 1);
+            if (r < nkt) {
+                if (ks0 + KT * (int64_t)(r + 1) > k) {   // B rows past k: not zero in memory
+#pragma unroll
+                    for (int q = 0; q < KQ; q++)
+                        if (ks0 + KT * r + 4 * q + (lane >> 4) >= k)
+#pragma unroll
+                            for (int b = 0; b < TN; b++) fb[u][q][b] = 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < KQ; q++)
+#pragma unroll
+                    for (int a = 0; a < TM; a++)
+#pragma unroll
+                        for (int b = 0; b < TN; b++)
+                            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][q][a], fb[u][q][b], acc[a][b], 0, 0, 0);
+            }
+        }
+    }
+    // D[i][j] of a 16x16 f64 tile: element e of lane l holds i = 4 e + l / 16, j = l % 16
+    const int64_t zc = opi(p, z, 4);
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t gi = row0 + a * 16 + 4 * e + (lane >> 4);
+                const int64_t gj = col0 + b * 16 + (lane & 15);
+                if (gi < m && gj < n) {
+                    if (S == 1) store_out(p, zc, gi, gj, acc[a][b][e]);
+                    else p.part[gi + m * gj + m * n * (s + (int64_t)S * z)] = acc[a][b][e];
+                }
+            }
+}
+
 // C = Σ_s part_s in ascending s (then + C if accumulating), or the complex epilogue.
 __global__ __launch_bounds__(256) void dgemm_splitk_reduce_kernel(GemmArgs p)
 {
@@ -612,8 +716,8 @@ __global__ __launch_bounds__(256, 2) void lpkgemm_kernel(GemmArgs p)
 // m is the ~1300-node frequency grid about one workgroup per CU without a k split.
 struct GemmShape { int bm, bn, wmw, wnw, per_cu; };
 constexpr GemmShape SHAPES[] = {{64, 64, 2, 2, 4},  {128, 64, 2, 2, 3}, {64, 128, 2, 2, 3}, {128, 128, 2, 2, 2},
-                                {256, 128, 4, 2, 1}, {256, 32, 4, 1, 2}, {128, 32, 4, 1, 3}};
-constexpr int NSHAPES = 7;
+                                {256, 128, 4, 2, 1}, {256, 32, 4, 1, 2}, {128, 32, 4, 1, 3}, {256, 32, 4, 1, 1}};
+constexpr int NSHAPES = 8;   // the last: dgemm_reg_kernel (no LDS)
 
 // The descriptor spans of the buffer-load kernel fit 31 bits for every unit: A spans the unit's k
 // range (at most the largest split, in 32-wide tiles) times lda, B its BN columns times ldb.
@@ -658,7 +762,10 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
         case 3: launch_buf<128, 128, 2, 2, 2>(ctx, grid, p); break;
         case 4: launch_buf<256, 128, 4, 2, 1>(ctx, grid, p); break;
         case 5: launch_buf<256, 32, 4, 1, 2>(ctx, grid, p); break;
-        default: launch_buf<128, 32, 4, 1, 3>(ctx, grid, p); break;
+        case 6: launch_buf<128, 32, 4, 1, 3>(ctx, grid, p); break;
+        default:
+            hipLaunchKernelGGL((dgemm_reg_kernel<8, 4>), dim3(grid), dim3(256), 0, ctx->stream, p);
+            break;
         }
     } else {
         // operands whose per-unit span passes 31 bits: the register-staged kernel with 64-bit
@@ -669,7 +776,7 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
         case 2: hipLaunchKernelGGL((dgemm_kernel<64, 128, 2, 2, 2>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
         case 3: hipLaunchKernelGGL((dgemm_kernel<128, 128, 4, 2, 2>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
         case 4: hipLaunchKernelGGL((dgemm_kernel<256, 128, 4, 2, 1, 16>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
-        case 5: hipLaunchKernelGGL((dgemm_kernel<256, 32, 4, 1, 2>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
+        case 5: case 7: hipLaunchKernelGGL((dgemm_kernel<256, 32, 4, 1, 2>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
         default: hipLaunchKernelGGL((dgemm_kernel<128, 32, 4, 1, 3>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
         }
     }

@@ -59,3 +59,44 @@ def test_buffer_gemm_indices_in_bounds(bm, bn, wmw, wnw):
             assert (i, j) not in out
             out[(i, j)] = tid
     assert len(out) == bm * bn and all(0 <= i < bm and 0 <= j < bn for i, j in out)
+
+
+@pytest.mark.parametrize("kt", [8, 16])
+def test_register_gemm_fragments(kt):
+    """dgemm_reg_kernel (256 x 32 tiles, 4 waves of 64 rows, operands loaded straight into MFMA
+    fragments): for every wave, k-step and lane, the A and B elements its buffer offsets address
+    are those of the MFMA operand layout (A[i][k] of a 16x4 block in lane i + 16 k, B[k][j] in
+    lane j + 16 k); over a k-tile each wave reads its 64 rows and the tile's 32 columns exactly
+    once per k; the accumulators cover the tile exactly once."""
+    lda, ldb = 1275, 8199
+    TM, TN, KQ = 4, 2, kt // 4
+    for wave in range(4):
+        row0 = 64 * wave
+        seen_a, seen_b = set(), set()
+        for lane in range(64):
+            voA = (lane & 15) + lda * (lane >> 4)
+            voB = (lane >> 4) + ldb * (lane & 15)
+            for q in range(KQ):
+                kofs = 4 * q
+                for a in range(TM):
+                    off = voA + 16 * a + lda * kofs
+                    i, k = off % lda, off // lda
+                    assert (i, k) == (16 * a + (lane & 15), 4 * q + (lane >> 4))
+                    seen_a.add((row0 + i, k))
+                for b in range(TN):
+                    off = voB + kofs + 16 * ldb * b
+                    k, j = off % ldb, off // ldb
+                    assert (k, j) == (4 * q + (lane >> 4), 16 * b + (lane & 15))
+                    seen_b.add((k, j))
+        assert seen_a == {(row0 + i, k) for i in range(64) for k in range(kt)}
+        assert seen_b == {(k, j) for k in range(kt) for j in range(32)}
+    out = {}
+    for wave, lane in itertools.product(range(4), range(64)):
+        for a, b, e in itertools.product(range(TM), range(TN), range(4)):
+            i, j = 64 * wave + 16 * a + 4 * e + (lane >> 4), 16 * b + (lane & 15)
+            assert (i, j) not in out
+            out[(i, j)] = (wave, lane)
+    assert len(out) == 256 * 32
+    # the launched pipeline (8-deep k-tiles, 4 slots): at most 63 loads in flight per wave
+    if kt == 8:
+        assert (4 - 1) * KQ * (TM + TN) <= 63

@@ -57,6 +57,21 @@ def test_gemm_tile_shapes_give_the_same_bits(ctx):
     assert rel(outs[0], ref) <= 1e-13
     for k, y in enumerate(outs[1:], 1):
         assert np.array_equal(y, outs[0]), k
+    # k = 200 (7 k-tiles of 32): no shape splits k, so all eight shapes (the last the LDS-free
+    # register-pipelined 256x32 kernel) give the same bits; 300 columns leave a partial tile
+    g = FrequencyGrid(-1.0, 0.01, np.arange(200))
+    z = pseudorandom(200, 300, seed=4)
+    ref = oracle.kramers_kronig_matrix(np.asarray(g)) @ z
+    outs = []
+    for shape in range(8):
+        os.environ["EPSX_GEMM_SHAPE"] = str(shape)
+        try:
+            outs.append(epsx.kramers_kronig(g, z, ctx))
+        finally:
+            del os.environ["EPSX_GEMM_SHAPE"]
+    assert rel(outs[0], ref) <= 1e-13
+    for k, y in enumerate(outs[1:], 1):
+        assert np.array_equal(y, outs[0]), k
 
 
 def kernel_case():
