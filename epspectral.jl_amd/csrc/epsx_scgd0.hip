@@ -621,12 +621,16 @@ __global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t 
 // Σ_new = complex(ReΣ, ImΣ) from the columns of the library products (column cls[i] of outer
 // state i: its multiplet class, or i itself with cls null), err = max |Σ_new - Σ| over the slice
 // (per-block maxima as err_copy_kernel) and Σ ← Σ_new, for the slots of a pass: the pack, the
-// multiplet copy and src/scgd0.jl:326-328 in one pass over Σ.
+// multiplet copy and src/scgd0.jl:326-328 in one pass over Σ.  F: the products' type (FP64, or
+// FP32 for the FP32 level, whose Kramers-Kronig product is the off-diagonal only: diag, the FP64
+// principal-value diagonal, is added here as ReΣ = re + diag[ω] ImΣ).
+template <typename F>
 __global__ __launch_bounds__(256) void pack_err_kernel(int32_t nw, int32_t ni, int64_t nc,
                                                        const int32_t *__restrict__ cls,
-                                                       const int32_t *__restrict__ tidx, const double *__restrict__ re,
-                                                       const double *__restrict__ im, double2 *__restrict__ S,
-                                                       double *__restrict__ bmax, const int32_t *__restrict__ gate)
+                                                       const int32_t *__restrict__ tidx, const F *__restrict__ re,
+                                                       const F *__restrict__ im, const double *__restrict__ diag,
+                                                       double2 *__restrict__ S, double *__restrict__ bmax,
+                                                       const int32_t *__restrict__ gate)
 {
     __shared__ double red[256];
     const int z = blockIdx.y;
@@ -634,12 +638,13 @@ __global__ __launch_bounds__(256) void pack_err_kernel(int32_t nw, int32_t ni, i
     if (gate && !gate[tcol]) return;
     const int32_t n = nw * ni;
     double2 *St = S + (int64_t)n * tcol;
-    const double *rt = re + nw * nc * (int64_t)tcol, *it = im + nw * nc * (int64_t)tcol;
+    const F *rt = re + nw * nc * (int64_t)tcol, *it = im + nw * nc * (int64_t)tcol;
     double m = 0;
     for (int32_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (int32_t)gridDim.x * 256) {
         const int32_t i = q / nw, w = q - i * nw;
         const int64_t o = w + (int64_t)nw * (cls ? cls[i] : i);
-        const double2 a = make_double2(rt[o], it[o]), b = St[q];
+        const double ai = (double)it[o];
+        const double2 a = make_double2(diag ? (double)rt[o] + diag[w] * ai : (double)rt[o], ai), b = St[q];
         const double e = hypot(a.x - b.x, a.y - b.y);
         m = (e > m || e != e) ? e : m;
         St[q] = a;
@@ -1165,9 +1170,10 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     // which EPSX_STAR_BLAS=0 or EPSX_GEMM_SHAPE selects, and which serves every product with masked
     // or low-precision instances); the host knows which slots take the FP64 pass, since a graph is
     // captured again whenever the active set changes.
-    bool star_lt = false, kk_lt = false, lp_lt[3] = {false, false, false};
+    bool star_lt = false, kk_lt = false, lp_lt[3] = {false, false, false}, kklp_lt = false;
     Tmp ReS;    // KK ImΣ of the library product, per column of the star product (then packed into Σ)
     Tmp ImSf;   // FP32 result of the low-precision library products
+    Tmp ReSf;   // the FP32 level's Kramers-Kronig off-diagonal by the library, per star column
     // (a run of one or two iterations does not pay for timing the library's candidates: it keeps
     // the library's own kernels, as it keeps their default tile shape)
     const char *blas_env = getenv("EPSX_STAR_BLAS"), *shape_env = getenv("EPSX_GEMM_SHAPE");
@@ -1215,6 +1221,13 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                     EPSX_HIP(ctx, hipGetLastError());
                     EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP32, nw, ncol, kp, hf.b.p, kp, kp * nw, s.Gf.p, kp, 0,
                                             ImSf.ptr<float>(), nw, nw * ncol, nT, &lp_lt[EPSX_PREC_FP32]));
+                    // the FP32 level's Kramers-Kronig off-diagonal over the same FP32 columns (the
+                    // stored FP32 KK, k-contiguous rows of kpw), packed into Σ with the diagonal
+                    if (lp_lt[EPSX_PREC_FP32] && (use_cls || s.n_mult == 0)) {
+                        EPSX_TRY(tmp_alloc(ctx, ReSf.b, 4 * (size_t)nw * ncol * nT));
+                        EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP32, nw, ncol, nw, s.KKf.p, kpw, 0, ImSf.ptr<float>(),
+                                                nw, nw * ncol, ReSf.ptr<float>(), nw, nw * ncol, nT, &kklp_lt));
+                    }
                 }
             }
         }
@@ -1223,6 +1236,12 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     // in one kernel, unless multiplets without classes still need their average taken over Σ_new.
     // (pack_err_kernel indexes a slice in 32 bits)
     const bool kk_fuse = kk_lt && (use_cls || s.n_mult == 0) && nw * ni <= INT32_MAX - 65536 * 256;
+    // the FP32 level likewise: its star and Kramers-Kronig products by the library in FP32, one
+    // pack of both into Σ with err
+    if (!(nw * ni <= INT32_MAX - 65536 * 256)) kklp_lt = false;
+    auto lp_fused = [&](int prec) { return kklp_lt && prec == EPSX_PREC_FP32; };
+    // err and Σ ← Σ_new per pass when some pass packs its own
+    const bool per_pass = kk_fuse || kklp_lt;
     // the levels present among the active temperatures, as a bit mask
     auto present_levels = [&]() {
         int mask = 0;
@@ -1313,9 +1332,11 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                                            bf ? s.Gb.p : s.Gf.p, kp, 0, ImSf.ptr<float>() + sC * h_tidx[z], nw, sC, n);
                     }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
-                    hipLaunchKernelGGL(widen_kernel, dim3(nblk(nw * ncol), nA), dim3(256), 0, st, nw * ncol, d_tidx,
-                                       passes[q].gate, ImSf.ptr<float>(), Cprod);
-                    EPSX_HIP(ctx, hipGetLastError());
+                    if (!lp_fused(passes[q].prec)) {
+                        hipLaunchKernelGGL(widen_kernel, dim3(nblk(nw * ncol), nA), dim3(256), 0, st, nw * ncol, d_tidx,
+                                           passes[q].gate, ImSf.ptr<float>(), Cprod);
+                        EPSX_HIP(ctx, hipGetLastError());
+                    }
                 } else if (lp_native && passes[q].prec != EPSX_PREC_FP64) {
                     const bool bf = passes[q].prec == EPSX_PREC_BF16;
                     bt.sA = kp * nw;
@@ -1336,11 +1357,11 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             // ImΣ per outer state for the products that take it (the fused FP64 pack reads the class
             // columns)
             if (use_cls) {
-                for (int q = 0; q < (kk_fuse ? npass : 1); q++) {
-                    if (kk_fuse && passes[q].prec == EPSX_PREC_FP64) continue;
+                for (int q = 0; q < (per_pass ? npass : 1); q++) {
+                    if ((kk_fuse && passes[q].prec == EPSX_PREC_FP64) || lp_fused(passes[q].prec)) continue;
                     hipLaunchKernelGGL(cls_scatter_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw, ni, s.nc,
                                        dptr<int32_t>(s.cls), d_tidx, ImSc.ptr<double>(), B.ImS.ptr<double>(),
-                                       kk_fuse ? passes[q].gate : d_act);
+                                       per_pass ? passes[q].gate : d_act);
                     EPSX_HIP(ctx, hipGetLastError());
                 }
             }
@@ -1416,15 +1437,30 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                                        sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
                 }));
                 if (kk_fuse)
-                    hipLaunchKernelGGL(pack_err_kernel, dim3(nb_err, nA), dim3(256), 0, st, (int32_t)nw, (int32_t)ni,
-                                       ncol, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx, ReS.ptr<double>(), Cim,
-                                       B.S.ptr<double2>(), B.bmax.ptr<double>(), passes[q].gate);
+                    hipLaunchKernelGGL(pack_err_kernel<double>, dim3(nb_err, nA), dim3(256), 0, st, (int32_t)nw,
+                                       (int32_t)ni, ncol, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx,
+                                       ReS.ptr<double>(), Cim, (const double *)nullptr, B.S.ptr<double2>(),
+                                       B.bmax.ptr<double>(), passes[q].gate);
                 else
                     hipLaunchKernelGGL(complex_pack_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw * ni, d_tidx,
                                        passes[q].gate, ReS.ptr<double>(), B.ImS.ptr<double>(), B.Snew.ptr<double2>());
                 EPSX_HIP(ctx, hipGetLastError());
             } else if (passes[q].prec == EPSX_PREC_FP64) {
                 EPSX_TRY(dgemm(ctx, nw, ni, nw, dptr<double>(s.KK), nw, B.ImS.ptr<double>(), nw, nullptr, nw, false, bt));
+            } else if (lp_fused(passes[q].prec)) {
+                // the off-diagonal in FP32 by the library over the star product's FP32 columns; the
+                // pack adds the FP64 principal-value diagonal and writes Σ ← Σ_new with err
+                const int64_t sC = nw * ncol;
+                EPSX_TRY(prec_runs(EPSX_PREC_FP32, [&](int z, int n) {
+                    const int64_t t0 = h_tidx[z];
+                    return blaslt_gemm(ctx, EPSX_PREC_FP32, nw, ncol, nw, s.KKf.p, kpw, 0, ImSf.ptr<float>() + sC * t0,
+                                       nw, sC, ReSf.ptr<float>() + sC * t0, nw, sC, n);
+                }));
+                hipLaunchKernelGGL(pack_err_kernel<float>, dim3(nb_err, nA), dim3(256), 0, st, (int32_t)nw, (int32_t)ni,
+                                   ncol, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx, ReSf.ptr<float>(),
+                                   ImSf.ptr<float>(), dptr<double>(s.KKd), B.S.ptr<double2>(), B.bmax.ptr<double>(),
+                                   passes[q].gate);
+                EPSX_HIP(ctx, hipGetLastError());
             } else if (lp_native) {
                 // the off-diagonal part from stored BF16 / FP32 operands, the principal-value diagonal
                 // in FP64 (the epilogue)
@@ -1452,11 +1488,11 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             EPSX_HIP(ctx, hipGetLastError());
         }
         // 8-9. err and Σ ← Σ_new for the slots whose pack did not (Σ_new is read from Σ from here)
-        for (int q = 0; q < (kk_fuse ? npass : 1); q++) {
-            if (kk_fuse && passes[q].prec == EPSX_PREC_FP64) continue;
+        for (int q = 0; q < (per_pass ? npass : 1); q++) {
+            if ((kk_fuse && passes[q].prec == EPSX_PREC_FP64) || lp_fused(passes[q].prec)) continue;
             hipLaunchKernelGGL(err_copy_kernel, dim3(nb_err, nA), dim3(256), 0, st, nw * ni, d_tidx,
                                B.Snew.ptr<double2>(), B.S.ptr<double2>(), B.bmax.ptr<double>(),
-                               kk_fuse ? passes[q].gate : d_act);
+                               per_pass ? passes[q].gate : d_act);
             EPSX_HIP(ctx, hipGetLastError());
         }
         // 6. spectral DOS and min ∫A
@@ -1546,7 +1582,11 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     std::vector<double> first_err(nT, NAN);
     std::vector<char> joined(nT, 0);        // slots that join the loop after its first iteration
     std::vector<int64_t> partner(nT, -1);   // and the slot whose first err they take
-    auto update_phases = [&]() -> int {
+    // e: the err of every temperature in the iteration just read; stage: null when the stream is
+    // idle at the next call (the levels are copied from prec_t), else a pinned ring of two slots
+    // of nT levels, one per parity of the iteration read (the copy from one slot is ordered before
+    // the event the host waits for two iterations later, when the slot is written again)
+    auto update_phases = [&](const double *e, int32_t *stage, int64_t iter) -> int {
         bool changed = false, any_lp = false;
         for (int64_t t = 0; t < nT; t++) {
             if (lvl[t] == L.n - 1) continue;
@@ -1554,12 +1594,12 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             if (joined[t]) { joined[t] = 0; continue; }   // no err of its own yet
             // a slot that joined after the first iteration measures its floor against its
             // partner's first err (the G0D0 residual), not against its own small first one
-            if (lp_iters[t]++ == 0) first_err[t] = partner[t] >= 0 ? first_err[partner[t]] : h_err[t];
-            const bool near = h_err[t] <= 2 * par->tol;
-            lp_flat[t] = (h_err[t] > 0.95 * prev_err[t] && h_err[t] < 1e-2 * first_err[t]) ? lp_flat[t] + 1 : 0;
+            if (lp_iters[t]++ == 0) first_err[t] = partner[t] >= 0 ? first_err[partner[t]] : e[t];
+            const bool near = e[t] <= 2 * par->tol;
+            lp_flat[t] = (e[t] > 0.95 * prev_err[t] && e[t] < 1e-2 * first_err[t]) ? lp_flat[t] + 1 : 0;
             const bool budget = 2 * lp_iters[t] >= par->maxiter;
             int next = lvl[t];
-            if (near || budget || !(h_err[t] == h_err[t])) next = L.n - 1;
+            if (near || budget || !(e[t] == e[t])) next = L.n - 1;
             else if (lp_flat[t] >= 2) next = lvl[t] + 1;
             if (next != lvl[t]) {
                 lvl[t] = next;
@@ -1567,10 +1607,18 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                 lp_flat[t] = 0;
                 changed = true;
             }
-            prev_err[t] = h_err[t];
+            prev_err[t] = e[t];
         }
         if (any_lp) out->n_low_precision++;
-        if (changed) EPSX_HIP(ctx, hipMemcpyAsync(d_prec.b.p, prec_t.data(), 4 * nT, hipMemcpyHostToDevice, st));
+        if (changed) {
+            const int32_t *src = prec_t.data();
+            if (stage) {
+                int32_t *slot = stage + nT * (iter & 1);
+                std::copy(prec_t.begin(), prec_t.end(), slot);
+                src = slot;
+            }
+            EPSX_HIP(ctx, hipMemcpyAsync(d_prec.b.p, src, 4 * nT, hipMemcpyHostToDevice, st));
+        }
         return EPSX_OK;
     };
     const double t_loop = now_s();
@@ -1616,7 +1664,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             if (rc != EPSX_OK) break;
             out->wall[iter - 1] = now_s() - t0;
             out->niter = iter;
-            if (split) EPSX_TRY(update_phases());
+            if (split) EPSX_TRY(update_phases(h_err.data(), nullptr, iter));
             if (par->verbosity > 0) {
                 fprintf(stderr, "scGD0 iteration %lld: wall %.3f s\n", (long long)iter, out->wall[iter - 1]);
                 for (int z = 0; z < nA; z++)
@@ -1648,6 +1696,9 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         double *h_es = nullptr, *d_es = nullptr;   // and err, for the decision to run ahead
         EPSX_HIP(ctx, hipHostMalloc((void **)&h_es, 16 * nT, hipHostMallocMapped | hipHostMallocCoherent));
         struct PinGuardE { double *p; ~PinGuardE() { (void)hipHostFree(p); } } pge{h_es};
+        int32_t *h_pr = nullptr;   // the staged levels of a mixed-precision run
+        EPSX_HIP(ctx, hipHostMalloc((void **)&h_pr, 8 * nT, hipHostMallocDefault));
+        struct PinGuardP { int32_t *p; ~PinGuardP() { (void)hipHostFree(p); } } pgp{h_pr};
         EPSX_HIP(ctx, hipHostGetDevicePointer((void **)&d_es, h_es, 0));
         std::fill(h_es, h_es + 2 * nT, INFINITY);
         struct PinGuard { int32_t *p; ~PinGuard() { (void)hipHostFree(p); } } pg{h_as};
@@ -1686,16 +1737,16 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                 if (*g) (void)hipGraphDestroy(*g);
             }
         } gg{&graph, &exec, st};
-        // Replayed iterations of an FP64 run go to the GPU one ahead of the host: iteration k + 1 is
+        // Replayed iterations go to the GPU one ahead of the host: iteration k + 1 is
         // launched before the host reads back iteration k's mask, so the GPU does not wait for the
         // host between replays.  An iteration launched ahead of the host's view runs the device-masked
         // kernels on the current mask; the library products (no device mask) keep the active set of
         // the graph, and their output for a temperature frozen meanwhile is never read.  Once some
         // active temperature's err is within 2 tol (it may freeze in the iteration on the GPU), the
         // host waits instead, so that no replay runs products for a temperature just frozen or after
-        // the last iteration.  (Mixed-precision runs set each temperature's level on the host from
-        // the last err, so they always wait for it.)
-        const bool ahead = !split;
+        // the last iteration.  Mixed-precision runs set each temperature's level on the host from
+        // the err it has read: with iteration k + 1 already launched, a level chosen from iteration
+        // k's err takes effect from iteration k + 2 (from k + 1 while the host waits near tol).
         // an active temperature within 2 tol after the last iteration the host has read
         auto near_tol = [&](int64_t iter_read) {
             const double *e = h_es + nT * (iter_read & 1);
@@ -1734,7 +1785,6 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                 }
                 EPSX_HIP(ctx, hipGraphLaunch(exec, st));
             }
-            if (split) EPSX_HIP(ctx, hipMemcpyAsync(h_err.data(), B.err.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipEventRecord(ev_read[iter & 1], st));
             launched = iter;
             return EPSX_OK;
@@ -1751,7 +1801,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         }
         for (int64_t iter = 1; iter <= maxiter; iter++) {
             if (launched < iter) EPSX_TRY(launch(iter));
-            if (ahead && iter >= 3 && iter < maxiter && !near_tol(iter - 1)) EPSX_TRY(launch(iter + 1));
+            if (iter >= 3 && iter < maxiter && !near_tol(iter - 1)) EPSX_TRY(launch(iter + 1));
             EPSX_HIP(ctx, hipEventSynchronize(ev_read[iter & 1]));
             const int32_t *slot = h_as + 2 * nT * (iter & 1);
             std::copy(slot, slot + nT, active.begin());
@@ -1818,7 +1868,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                 // frozen temperatures are not in a phase any more
                 for (int64_t t = 0; t < nT; t++)
                     if (!active[t]) { lvl[t] = L.n - 1; prec_t[t] = EPSX_PREC_FP64; }
-                EPSX_TRY(update_phases());
+                EPSX_TRY(update_phases(h_es + nT * (iter & 1), h_pr, iter));
             }
             bool any = false;
             for (int64_t t = 0; t < nT; t++) any |= active[t] != 0;
