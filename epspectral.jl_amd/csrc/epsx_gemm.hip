@@ -1000,17 +1000,48 @@ struct LtDescs {
 struct LtProduct {
     LtDescs d;
     hipblasLtMatmulAlgo_t algo;
+    size_t ws = 0;  // the algorithm's workspace (in the handle's buffer)
     float ms = 0;   // its timed launch
 };
 
-// (precision, m, n, k, lda, sA, ldb, sB, ldc, sC, batch) -> product; absent: no workspace-free
-// algorithm
+// (precision, m, n, k, lda, sA, ldb, sB, ldc, sC, batch) -> product; absent: no algorithm within
+// the workspace cap that gives the same bits twice
 using LtKey = std::array<int64_t, 11>;
+
+// The workspace the library's split-k algorithms may use (one buffer per context, allocated
+// before any capture at its largest size; the products run one after another on one stream).
+constexpr size_t LT_WS_CAP = size_t(256) << 20;
 
 struct BlasLt {
     hipblasLtHandle_t h = nullptr;
     std::map<LtKey, std::unique_ptr<LtProduct>> prod;
+    void *ws = nullptr;     // workspace, ws_bytes
+    size_t ws_bytes = 0;
+    void *chk = nullptr;    // a copy of the first result of a candidate, chk_bytes
+    size_t chk_bytes = 0;
+    int *flag = nullptr;    // device word: the second result differed
 };
+
+// flag = 1 where the two results differ bitwise
+__global__ void lt_bitwise_diff_kernel(size_t nwords, const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                                       int *__restrict__ flag)
+{
+    const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (q < nwords && a[q] != b[q]) *flag = 1;
+}
+
+// grow a device buffer of the handle (outside graph capture)
+int lt_grow(epsx_ctx *ctx, void **p, size_t *have, size_t need)
+{
+    if (*have >= need) return EPSX_OK;
+    EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (*p) EPSX_HIP(ctx, hipFree(*p));
+    *p = nullptr;
+    *have = 0;
+    EPSX_HIP(ctx, hipMalloc(p, need));
+    *have = need;
+    return EPSX_OK;
+}
 
 int lt_check(epsx_ctx *ctx, hipblasStatus_t st, const char *what)
 {
@@ -1057,14 +1088,14 @@ int lt_descs(epsx_ctx *ctx, LtDescs &d, int prec, int64_t m, int64_t n, int64_t 
 }
 
 int lt_run(BlasLt *b, LtProduct &pr, int prec, const void *A, const void *B, void *C, hipStream_t st,
-           const hipblasLtMatmulAlgo_t *algo)
+           const hipblasLtMatmulAlgo_t *algo, size_t ws)
 {
     const double one = 1.0, zero = 0.0;
     const float onef = 1.0f, zerof = 0.0f;
     const void *al = prec == EPSX_PREC_FP64 ? (const void *)&one : (const void *)&onef;
     const void *be = prec == EPSX_PREC_FP64 ? (const void *)&zero : (const void *)&zerof;
-    return hipblasLtMatmul(b->h, pr.d.md, al, A, pr.d.la, B, pr.d.lb, be, C, pr.d.lc, C, pr.d.lc, algo, nullptr, 0, st) ==
-                   HIPBLAS_STATUS_SUCCESS
+    return hipblasLtMatmul(b->h, pr.d.md, al, A, pr.d.la, B, pr.d.lb, be, C, pr.d.lc, C, pr.d.lc, algo,
+                           ws ? b->ws : nullptr, ws, st) == HIPBLAS_STATUS_SUCCESS
                ? 0
                : 1;
 }
@@ -1092,7 +1123,7 @@ int blaslt_prepare(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, con
         EPSX_TRY(lt_descs(ctx, pr->d, prec, m, n, k, lda, sA, ldb, sB, ldc, sC, batch));
         hipblasLtMatmulPreference_t pref = nullptr;
         EPSX_TRY(lt_check(ctx, hipblasLtMatmulPreferenceCreate(&pref), "preference"));
-        const uint64_t wsz = 0;
+        const uint64_t wsz = LT_WS_CAP;
         hipblasStatus_t st = hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &wsz,
                                                                    sizeof(wsz));
         constexpr int NCAND = 32;
@@ -1102,29 +1133,42 @@ int blaslt_prepare(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, con
             st = hipblasLtMatmulAlgoGetHeuristic(b->h, pr->d.md, pr->d.la, pr->d.lb, pr->d.lc, pr->d.lc, pref, NCAND, res,
                                                  &nres);
         (void)hipblasLtMatmulPreferenceDestroy(pref);
-        // the heuristic's order puts split-k algorithms first, slow without their workspace: every
-        // workspace-free candidate is timed once (after a warm launch) and the fastest kept
+        // every candidate within the workspace cap (split-k ones among them) that gives the same
+        // bits on a second launch is timed once (after that warm launch) and the fastest kept
+        const size_t cbytes = (size_t)(prec == EPSX_PREC_FP64 ? 8 : 4) * (size_t)(ldc * (n - 1) + m + sC * (batch - 1));
+        EPSX_TRY(lt_grow(ctx, &b->chk, &b->chk_bytes, cbytes));
+        if (!b->flag) EPSX_HIP(ctx, hipMalloc((void **)&b->flag, sizeof(int)));
         hipEvent_t e0, e1;
         EPSX_HIP(ctx, hipEventCreate(&e0));
         EPSX_HIP(ctx, hipEventCreate(&e1));
         int pick = -1;
         float best = 0;
         for (int r = 0; st == HIPBLAS_STATUS_SUCCESS && r < nres; r++) {
-            if (res[r].state != HIPBLAS_STATUS_SUCCESS || res[r].workspaceSize != 0) continue;
-            if (lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[r].algo)) continue;
+            const size_t ws = res[r].workspaceSize;
+            if (res[r].state != HIPBLAS_STATUS_SUCCESS || ws > LT_WS_CAP) continue;
+            EPSX_TRY(lt_grow(ctx, &b->ws, &b->ws_bytes, ws));
+            if (lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[r].algo, ws)) continue;
+            EPSX_HIP(ctx, hipMemcpyAsync(b->chk, C, cbytes, hipMemcpyDeviceToDevice, ctx->stream));
+            EPSX_HIP(ctx, hipMemsetAsync(b->flag, 0, sizeof(int), ctx->stream));
             EPSX_HIP(ctx, hipEventRecord(e0, ctx->stream));
-            const int rs = lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[r].algo);
+            const int rs = lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[r].algo, ws);
             EPSX_HIP(ctx, hipEventRecord(e1, ctx->stream));
-            EPSX_HIP(ctx, hipEventSynchronize(e1));
+            hipLaunchKernelGGL(lt_bitwise_diff_kernel, dim3((unsigned)((cbytes / 4 + 255) / 256)), dim3(256), 0,
+                               ctx->stream, cbytes / 4, static_cast<const uint32_t *>(C),
+                               static_cast<const uint32_t *>(b->chk), b->flag);
+            int differ = 0;
+            EPSX_HIP(ctx, hipMemcpyAsync(&differ, b->flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            EPSX_HIP(ctx, hipStreamSynchronize(ctx->stream));
             float ms = 0;
             EPSX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-            if (rs == 0 && (pick < 0 || ms < best)) { pick = r; best = ms; }
+            if (rs == 0 && !differ && (pick < 0 || ms < best)) { pick = r; best = ms; }
         }
         // the pick's time, which prices the batch splits, from three more launches
         if (pick >= 0) {
+            const size_t ws = res[pick].workspaceSize;
             EPSX_HIP(ctx, hipEventRecord(e0, ctx->stream));
             int rs = 0;
-            for (int rep = 0; rep < 3; rep++) rs |= lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[pick].algo);
+            for (int rep = 0; rep < 3; rep++) rs |= lt_run(b, *pr, prec, A, B, C, ctx->stream, &res[pick].algo, ws);
             EPSX_HIP(ctx, hipEventRecord(e1, ctx->stream));
             EPSX_HIP(ctx, hipEventSynchronize(e1));
             float ms = 0;
@@ -1135,6 +1179,7 @@ int blaslt_prepare(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, con
         (void)hipEventDestroy(e1);
         if (pick < 0) { all = false; continue; }
         pr->algo = res[pick].algo;
+        pr->ws = res[pick].workspaceSize;
         pr->ms = best;
         b->prod[key] = std::move(pr);
     }
@@ -1178,7 +1223,7 @@ int blaslt_gemm(epsx_ctx *ctx, int prec, int64_t m, int64_t n, int64_t k, const 
         LtProduct *pr = product(*it);
         if (lt_run(b, *pr, prec, static_cast<const char *>(A) + ea * sA * off,
                    static_cast<const char *>(B) + ea * sB * off, static_cast<char *>(C) + ec * sC * off, ctx->stream,
-                   &pr->algo))
+                   &pr->algo, pr->ws))
             return set_error(ctx, EPSX_ERR_DEVICE, "hipblasLtMatmul failed");
         off += *it;
     }
@@ -1191,6 +1236,9 @@ void blaslt_destroy(epsx_ctx *ctx)
     if (!b) return;
     b->prod.clear();   // descriptors before the handle
     if (b->h) (void)hipblasLtDestroy(b->h);
+    if (b->ws) (void)hipFree(b->ws);
+    if (b->chk) (void)hipFree(b->chk);
+    if (b->flag) (void)hipFree(b->flag);
     delete b;
     ctx->blaslt = nullptr;
 }
