@@ -37,6 +37,14 @@ FM_FLOP_PER_TERM = 60.0   # flop-equivalents of one Fan-Migdal term (2 lookups, 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_t2g24.json")
 
 
+# the library's own FP64 GEMM tile shapes (epsx_gemm.hip SHAPES), by index
+GEMM_SHAPE_NAMES = {0: "dgemm_buf_kernel 64x64", 1: "dgemm_buf_kernel 128x64", 2: "dgemm_buf_kernel 64x128",
+                    3: "dgemm_buf_kernel 128x128", 4: "dgemm_buf_kernel 256x128", 5: "dgemm_buf_kernel 256x32",
+                    6: "dgemm_buf_kernel 128x32", 7: "dgemm_reg_kernel 256x32 (1 wave along k)",
+                    8: "dgemm_reg_kernel 256x32 (2 waves along k)",
+                    9: "dgemm_pair_kernel 256x32 (16-byte loads, 4 x 2 waves)",
+                    10: "dgemm_pair_kernel 256x32 (16-byte loads, 2 x 4 waves)"}
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +123,7 @@ def main():
         tm = r["info"]["timings"]
         for key, v in tm.items():
             tsum[key] = tsum.get(key, 0) + v
+        tsum["star_gemm_kernel"] = tm["star_gemm_kernel"]
         iters.append(int(r["history"]["err"].shape[1]))
         print(f"[bench] step {k + 1}/{args.steps}: {times[-1]:.3f}s, iterations {iters[-1]}, "
               f"converged {r['converged'].tolist()}, μ {r['μ'].tolist()}", file=sys.stderr, flush=True)
@@ -135,12 +144,16 @@ def main():
         avg = tsum["star_gemm"] / max(tsum["star_gemm_launches"], 1)
         achieved = flops / avg / 1e12
         if args.precision == "fp64":
-            own = os.environ.get("EPSX_STAR_BLAS", "1") == "0" or os.environ.get("EPSX_GEMM_SHAPE")
+            # the kernel the library reports for the FP64 product (timing slot 7)
+            kid = tsum.get("star_gemm_kernel", 0)
             peak = FP64_PEAK_TFLOPS
-            kname = ("own FP64 MFMA GEMM, autotuned tile shape: dgemm_reg_kernel 256x32 or dgemm_buf_kernel "
-                     "(star Fan-Migdal GEMM, v_mfma_f64_16x16x4f64)" if own
-                     else "hipBLASLt FP64 GEMM (star Fan-Migdal product, strided batch with G̃ shared)")
-            tr = traffic("dgemm_kernel" if own else "hipblaslt_star", args.nk, nT)
+            if kid == 1:
+                kname = "hipBLASLt FP64 GEMM (star Fan-Migdal product, strided batch with G̃ shared)"
+                tr = traffic("hipblaslt_star", args.nk, nT)
+            else:
+                kname = (f"own FP64 MFMA GEMM {GEMM_SHAPE_NAMES.get(kid - 2, '?')} (star Fan-Migdal product, "
+                         "v_mfma_f64_16x16x4f64, autotuned tile shape)")
+                tr = traffic(GEMM_SHAPE_NAMES.get(kid - 2, "?").split()[0], args.nk, nT)
         else:
             peak = BF16_PEAK_TFLOPS if args.precision == "bf16" else FP32_PEAK_TFLOPS
             kname = f"lpkgemm_kernel (star Fan-Migdal GEMM, {args.precision.upper()} MFMA)"

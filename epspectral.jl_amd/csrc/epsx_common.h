@@ -49,6 +49,7 @@ struct Scgd0Dev {
     DevBuf wsa, Wa, jsp, tsp;         // spectral grid: nodes, trapezoid weights, table from ωs
     DevBuf KK;                        // [nw, nw]
     DevBuf KKo, KKd;                  // KK without its diagonal, and the diagonal
+    DevBuf KKp;                       // KK with the even column pitch nw + (nw & 1), pad rows zero
     DevBuf mult_ptr, mult_idx;
     DevBuf star_ptr;                  // [ni + 1] sorted-position ranges of each star
     DevBuf Gt;                        // star-collapsed coupling G̃ [(ν, i'), i] (built on demand)
@@ -134,8 +135,10 @@ struct epsx_ctx {
     // looked up when the context opens
     double *kk_scratch = nullptr;
     std::map<std::array<int64_t, 5>, int> gemm_tune;   // (m, n, k, instances, caller workspace or -1) -> tile shape
+    int last_gemm_shape = -1;   // the tile shape the last FP64 dgemm launched (after any fallback)
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
-    // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6-7 reserved
+    // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6 their instances,
+    // 7 the kernel of the star method's FP64 product (count only)
     double t_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t n_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -342,15 +345,15 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
                    const double *ht, const double *wnu, const double *e_i, const double *S, const double *T,
                    const double *mu, const double *eta, const int32_t *tidx, int nslots, double w_acoustic,
                    double *occ, double *hout, const int32_t *act, const int32_t *act_h, uint16_t *hb,
-                   const int32_t *act_b, float *hf, const int32_t *act_f, int64_t kp);
-int gt_lp_launch(epsx_ctx *ctx, int precision, int64_t K, int64_t kp, int64_t ni, const double *Gt, void *gl);
+                   const int32_t *act_b, float *hf, const int32_t *act_f, int64_t kp, int64_t ldh);
+int gt_lp_launch(epsx_ctx *ctx, int precision, int64_t K, int64_t ldg, int64_t kp, int64_t ni, const double *Gt, void *gl);
 // the Einstein factorisation's H'[ω, ν + nm (p - p0)] = w_f H_ν[ω, f] for the sorted inner states
 // [p0, p0 + np) of every slot (occ from h_build_launch), and the check that no mode below
 // ω_acoustic holds a non-finite coupling
 int einstein_h_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const double *ws, const int32_t *hk, const double *ht,
                       const double *wnu, const double *S, const double *eta, const int32_t *tidx, int nslots,
                       double w_acoustic, const double *occ, int64_t p0, int64_t np, const double *ef, const double *wf,
-                      const int32_t *fi, double *hout, const int32_t *act);
+                      const int32_t *fi, double *hout, int64_t ldh, const int32_t *act);
 int acoustic_finite_check(epsx_ctx *ctx, int64_t nm, int64_t n, const double *g2p, const uint8_t *skip, bool *ok);
 // the warm-start network's Σ[:, :, tdst[z]] predicted from the G0D0 Σ[:, :, tsrc[z]], z < nz (epsx_net.hip)
 int net_predict_sigma(epsx_ctx *ctx, int64_t nw, int64_t ni, int nz, const int32_t *tsrc, const int32_t *tdst,

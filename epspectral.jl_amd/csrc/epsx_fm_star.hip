@@ -104,7 +104,8 @@ struct HArgs {
     const int32_t *tidx;      // slot -> temperature index
     double w_acoustic;
     double *occ;              // [2, nw, nm] per slot: n_ν + f(ω+ω_ν-μ), n_ν + 1 - f(ω-ω_ν-μ)
-    double *hout;             // [nw, nm, ni] per slot
+    double *hout;             // [nw, nm, ni] per slot, columns of pitch ldh
+    int64_t ldh;              // column pitch of hout (>= nw)
     const int32_t *act;       // convergence mask per temperature index, or null
 };
 
@@ -202,23 +203,25 @@ static inline unsigned xcd_grid(int64_t threads)
     return (unsigned)(((threads + 255) / 256 + 7) / 8 * 8);
 }
 
-// h[w + nw (ν + nm i')] per slot, FP64.  One thread per (ω, ν) and HB_R consecutive outer states:
+// h[w + ldh (ν + nm i')] per slot, FP64.  One thread per (ω, ν) and HB_R consecutive outer states:
 // the (ω, ν) constants are read once and the HB_R independent Σ reads are in flight together
-// (lanes along ω, so every read and write is contiguous across a wave).
+// (lanes along ω, so every read and write is contiguous across a wave).  The threads sweep the
+// padded columns (ldh rows, the rows w >= nω written as zeros), so with ldh a multiple of 64 each
+// wave stores whole aligned 512-byte runs.
 constexpr int HB_R = 4;
 __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
 {
     const int64_t q = xcd_major_block() * blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
-    const int64_t nw = a.nw, ni = a.ni, nwm = nw * a.nm, ngrp = (ni + HB_R - 1) / HB_R;
-    if (q >= nwm * ngrp) return;
+    const int64_t nw = a.nw, ni = a.ni, ldh = a.ldh, ldm = ldh * a.nm, ngrp = (ni + HB_R - 1) / HB_R;
+    if (q >= ldm * ngrp) return;
     const int tcol = a.tidx[z];
     if (a.act && !a.act[tcol]) return;
-    const int64_t wn = q % nwm, ip0 = (q / nwm) * HB_R;
-    const int64_t w = wn % nw, nu = wn / nw;
-    double *out = a.hout + nwm * ni * (int64_t)z;
+    const int64_t wp = q % ldm, ip0 = (q / ldm) * HB_R;
+    const int64_t w = wp % ldh, nu = wp / ldh;
+    double *out = a.hout + ldm * ni * (int64_t)z + wp;
     double v[HB_R];
-    if (a.wnu[nu] < a.w_acoustic) {
+    if (w >= nw || a.wnu[nu] < a.w_acoustic) {
 #pragma unroll
         for (int r = 0; r < HB_R; r++) v[r] = 0.0;
     } else {
@@ -228,14 +231,14 @@ __global__ __launch_bounds__(256) void h_build_kernel(HArgs a)
     }
 #pragma unroll
     for (int r = 0; r < HB_R; r++)
-        if (ip0 + r < ni) out[wn + nwm * (ip0 + r)] = v[r];
+        if (ip0 + r < ni) out[ldm * (ip0 + r)] = v[r];
 }
 
 // The same h with the Σ columns of HS consecutive outer states staged in LDS (η-clamped as they
-// are staged), one workgroup per (state group, slot) sweeping every (ω, ν) of its states: Σ is
-// read from memory once per launch and the four interpolation reads of every term come from LDS;
-// the (ω, ν) constants are read once per HS states.  Same operations in the same order as
-// h_from, so the same bits.
+// are staged), one workgroup per (state group, slot) sweeping every (ω, ν) of its states (the
+// padded columns, as h_build_kernel): Σ is read from memory once per launch and the four
+// interpolation reads of every term come from LDS; the (ω, ν) constants are read once per HS
+// states.  Same operations in the same order as h_from, so the same bits.
 template <int HS>
 __global__ __launch_bounds__(512) void h_build_lds_kernel(HArgs a)
 {
@@ -244,6 +247,7 @@ __global__ __launch_bounds__(512) void h_build_lds_kernel(HArgs a)
     const int tcol = a.tidx[z];
     if (a.act && !a.act[tcol]) return;
     const int64_t nw = a.nw, ni = a.ni, nwm = nw * a.nm, i0 = (int64_t)blockIdx.x * HS;
+    const int64_t ldh = a.ldh, ldm = ldh * a.nm;
     const int nr = (int)(ni - i0 < HS ? ni - i0 : HS);
     const double eta = a.eta ? a.eta[tcol] : -INFINITY;
     const double2 *S = reinterpret_cast<const double2 *>(a.S) + nw * (ni * (int64_t)tcol + i0);
@@ -252,23 +256,24 @@ __global__ __launch_bounds__(512) void h_build_lds_kernel(HArgs a)
         h_lds[q] = make_double2(v.x, fmin(v.y, -eta));
     }
     __syncthreads();
-    double *out = a.hout + nwm * ni * (int64_t)z + nwm * i0;
+    double *out = a.hout + ldm * (ni * (int64_t)z + i0);
     const double2 *occ = reinterpret_cast<const double2 *>(a.occ) + nwm * (int64_t)z;
     // HU (ω, ν) per thread and pass, their constants loaded together
     constexpr int HU = 4;
-    for (int64_t wb = threadIdx.x; wb < nwm; wb += 512 * HU) {
+    for (int64_t wb = threadIdx.x; wb < ldm; wb += 512 * HU) {
         int ka[HU], ke[HU];
         double ta[HU], te[HU], x0[HU], wnu[HU];
         double2 o[HU];
         bool live[HU], zero[HU];
 #pragma unroll
         for (int u = 0; u < HU; u++) {
-            const int64_t wn = wb + 512 * u;
-            live[u] = wn < nwm;
-            const int64_t wc = live[u] ? wn : nwm - 1;
-            const int64_t w = wc % nw, nu = wc / nw;
+            const int64_t wp = wb + 512 * u;   // w + ldh ν: the row of the padded output columns
+            live[u] = wp < ldm;
+            const int64_t wpc = live[u] ? wp : ldm - 1;
+            const int64_t wq = wpc % ldh, nu = wpc / ldh;
+            const int64_t w = wq < nw ? wq : nw - 1, wc = w + nw * nu;
             wnu[u] = a.wnu[nu];
-            zero[u] = wnu[u] < a.w_acoustic;
+            zero[u] = wq >= nw || wnu[u] < a.w_acoustic;
             ka[u] = a.hk[2 * wc];
             ke[u] = a.hk[2 * wc + 1];
             ta[u] = a.ht[2 * wc];
@@ -279,7 +284,7 @@ __global__ __launch_bounds__(512) void h_build_lds_kernel(HArgs a)
 #pragma unroll
         for (int u = 0; u < HU; u++) {
             if (!live[u]) continue;
-            const int64_t wn = wb + 512 * u;
+            const int64_t wp = wb + 512 * u;
             double v[HS];
 #pragma unroll
             for (int r = 0; r < HS; r++) {
@@ -298,7 +303,7 @@ __global__ __launch_bounds__(512) void h_build_lds_kernel(HArgs a)
             }
 #pragma unroll
             for (int r = 0; r < HS; r++)
-                if (r < nr) out[wn + nwm * r] = v[r];
+                if (r < nr) out[wp + ldm * r] = v[r];
         }
     }
 }
@@ -317,15 +322,15 @@ __global__ __launch_bounds__(256) void einstein_h_kernel(HArgs a, int64_t p0, in
 {
     const int64_t q = xcd_major_block() * blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
-    const int64_t nw = a.nw, nwm = nw * a.nm, ngrp = (np + EH_R - 1) / EH_R;
-    if (q >= nwm * ngrp) return;
+    const int64_t nw = a.nw, ldh = a.ldh, ldm = ldh * a.nm, ngrp = (np + EH_R - 1) / EH_R;
+    if (q >= ldm * ngrp) return;
     const int tcol = a.tidx[z];
     if (a.act && !a.act[tcol]) return;
-    const int64_t wn = q % nwm, pl0 = (q / nwm) * EH_R;
-    const int64_t w = wn % nw, nu = wn / nw;
-    double *out = a.hout + nwm * np * (int64_t)z;
+    const int64_t wp = q % ldm, pl0 = (q / ldm) * EH_R;   // the padded columns, as h_build_kernel
+    const int64_t w = wp % ldh, nu = wp / ldh;
+    double *out = a.hout + ldm * np * (int64_t)z + wp;
     double v[EH_R];
-    if (a.wnu[nu] < a.w_acoustic) {
+    if (w >= nw || a.wnu[nu] < a.w_acoustic) {
 #pragma unroll
         for (int r = 0; r < EH_R; r++) v[r] = 0.0;
     } else {
@@ -349,7 +354,7 @@ __global__ __launch_bounds__(256) void einstein_h_kernel(HArgs a, int64_t p0, in
     }
 #pragma unroll
     for (int r = 0; r < EH_R; r++)
-        if (pl0 + r < np) out[wn + nwm * (pl0 + r)] = v[r];
+        if (pl0 + r < np) out[ldm * (pl0 + r)] = v[r];
 }
 
 // Any non-finite coupling in a mode below ω_acoustic (skip[ν] = 1): the Einstein product would
@@ -449,13 +454,13 @@ __global__ __launch_bounds__(512) void h_build_lp_run_kernel(HArgs a, E *hl, int
 // G̃ in low precision with the row pitch of the low-precision product: gl[k + kp i] = G̃[k + K i]
 // (zero for K <= k < kp).
 template <typename E>
-__global__ __launch_bounds__(256) void gt_lp_kernel(int64_t K, int64_t kp, int64_t ni, const double *__restrict__ Gt,
-                                                    E *__restrict__ gl)
+__global__ __launch_bounds__(256) void gt_lp_kernel(int64_t K, int64_t ldg, int64_t kp, int64_t ni,
+                                                    const double *__restrict__ Gt, E *__restrict__ gl)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= kp * ni) return;
     const int64_t k = q % kp, i = q / kp;
-    const double g = k < K ? Gt[k + K * i] : 0.0;
+    const double g = k < K ? Gt[k + ldg * i] : 0.0;
     if constexpr (sizeof(E) == 2) gl[q] = to_bf16(g);
     else gl[q] = (float)g;
 }
@@ -549,10 +554,10 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
                    const double *ht, const double *wnu, const double *e_i, const double *S, const double *T,
                    const double *mu, const double *eta, const int32_t *tidx, int nslots, double w_acoustic,
                    double *occ, double *hout, const int32_t *act, const int32_t *act_h, uint16_t *hb,
-                   const int32_t *act_b, float *hf, const int32_t *act_f, int64_t kp)
+                   const int32_t *act_b, float *hf, const int32_t *act_f, int64_t kp, int64_t ldh)
 {
     HArgs a;
-    a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht;
+    a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht; a.ldh = ldh;
     a.wnu = wnu; a.e_i = e_i; a.S = S; a.T = T; a.mu = mu; a.eta = eta; a.tidx = tidx;
     a.w_acoustic = w_acoustic; a.occ = occ; a.hout = hout; a.act = act;
     hipLaunchKernelGGL(h_occ_kernel, dim3((unsigned)((nw * nm + 255) / 256), (unsigned)nslots), dim3(256), 0,
@@ -584,7 +589,7 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
             hipLaunchKernelGGL(h_build_lds_kernel<2>, dim3((unsigned)((ni + 1) / 2), (unsigned)nslots), dim3(512),
                                2 * col, ctx->stream, ah);
         } else {
-            const int64_t threads = nw * nm * ((ni + HB_R - 1) / HB_R);
+            const int64_t threads = ldh * nm * ((ni + HB_R - 1) / HB_R);
             hipLaunchKernelGGL(h_build_kernel, dim3(xcd_grid(threads), (unsigned)nslots), dim3(256), 0, ctx->stream, ah);
         }
         EPSX_HIP(ctx, hipGetLastError());
@@ -624,12 +629,12 @@ int h_build_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const doub
 int einstein_h_launch(epsx_ctx *ctx, int64_t nw, int64_t nm, int64_t ni, const double *ws, const int32_t *hk, const double *ht,
                       const double *wnu, const double *S, const double *eta, const int32_t *tidx, int nslots,
                       double w_acoustic, const double *occ, int64_t p0, int64_t np, const double *ef, const double *wf,
-                      const int32_t *fi, double *hout, const int32_t *act)
+                      const int32_t *fi, double *hout, int64_t ldh, const int32_t *act)
 {
     HArgs a{};
-    a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht; a.wnu = wnu; a.S = S; a.eta = eta;
+    a.nw = nw; a.nm = nm; a.ni = ni; a.ws = ws; a.hk = hk; a.ht = ht; a.wnu = wnu; a.S = S; a.eta = eta; a.ldh = ldh;
     a.tidx = tidx; a.w_acoustic = w_acoustic; a.occ = const_cast<double *>(occ); a.hout = hout; a.act = act;
-    const int64_t threads = nw * nm * ((np + EH_R - 1) / EH_R);
+    const int64_t threads = ldh * nm * ((np + EH_R - 1) / EH_R);
     hipLaunchKernelGGL(einstein_h_kernel, dim3(xcd_grid(threads), (unsigned)nslots), dim3(256), 0, ctx->stream, a, p0,
                        np, ef, wf, fi);
     EPSX_HIP(ctx, hipGetLastError());
@@ -651,13 +656,13 @@ int acoustic_finite_check(epsx_ctx *ctx, int64_t nm, int64_t n, const double *g2
     return EPSX_OK;
 }
 
-int gt_lp_launch(epsx_ctx *ctx, int precision, int64_t K, int64_t kp, int64_t ni, const double *Gt, void *gl)
+int gt_lp_launch(epsx_ctx *ctx, int precision, int64_t K, int64_t ldg, int64_t kp, int64_t ni, const double *Gt, void *gl)
 {
     const dim3 grid((unsigned)((kp * ni + 255) / 256)), blk(256);
     if (precision == EPSX_PREC_BF16)
-        hipLaunchKernelGGL(gt_lp_kernel<uint16_t>, grid, blk, 0, ctx->stream, K, kp, ni, Gt, static_cast<uint16_t *>(gl));
+        hipLaunchKernelGGL(gt_lp_kernel<uint16_t>, grid, blk, 0, ctx->stream, K, ldg, kp, ni, Gt, static_cast<uint16_t *>(gl));
     else
-        hipLaunchKernelGGL(gt_lp_kernel<float>, grid, blk, 0, ctx->stream, K, kp, ni, Gt, static_cast<float *>(gl));
+        hipLaunchKernelGGL(gt_lp_kernel<float>, grid, blk, 0, ctx->stream, K, ldg, kp, ni, Gt, static_cast<float *>(gl));
     EPSX_HIP(ctx, hipGetLastError());
     return EPSX_OK;
 }

@@ -343,27 +343,32 @@ __global__ __launch_bounds__(64 * WMW * WNW, MINW) void dgemm_buf_kernel(GemmArg
             }
 }
 
-// The thin-tile product without LDS: a 256 x 32 tile per workgroup of 4 waves, each wave owning
-// 64 rows as 4 x 2 accumulators of 16x16.  A wave's rows of A are its own, so it loads its MFMA
-// operand fragments of A straight from memory into registers (lane l: row l mod 16 of a 16-row
-// block at k = l / 16 of a 4-deep step: 16 consecutive doubles per 16 lanes); every wave loads
-// the 32 columns of B the same way (one wave from L2, the others mostly from L1).  No LDS and no
-// barrier: each wave runs its own software pipeline with the loads of KP - 1 k-tiles of KT in
-// flight ahead of the MFMAs (at most 63 loads in flight per wave: 6 KT / 4 loads per k-tile).
-// The k order per accumulator is that of the other shapes (4-deep steps in ascending k within
-// the unit's split), so for the same split it gives the same bits.
-template <int KT, int KP>
-__global__ __launch_bounds__(256, 1) void dgemm_reg_kernel(GemmArgs p)
+// The thin-tile product without operand staging: a 256 x 32 tile per workgroup of 4 WK waves, four
+// along m (64 rows each, as 4 x 2 accumulators of 16x16) times WK along k.  A wave's rows of A are
+// its own, so it loads its MFMA operand fragments of A straight from memory into registers (lane l:
+// row l mod 16 of a 16-row block at k = l / 16 of a 4-deep step: 16 consecutive doubles per 16
+// lanes); it loads the 32 columns of B the same way (one wave from L2, the others mostly from L1).
+// No barrier in the k loop: each wave runs its own software pipeline with the loads of KP - 1
+// k-tiles of KT in flight ahead of the MFMAs (at most 63 loads in flight per wave: 6 KT / 4 loads
+// per k-tile).  With WK = 2 the two waves of a row block take the even and the odd k-tiles of the
+// unit (two waves per SIMD, so one's MFMAs run while the other waits for its loads); the odd
+// wave's accumulators are added to the even wave's through LDS at the end.  Per accumulator the k
+// order is ascending 4-deep steps over the wave's tiles (WK = 1: the unit's whole k range, the
+// order of the other shapes, so for the same split the same bits; WK = 2: the even tiles' sum plus
+// the odd tiles' sum, a fixed order of its own).
+template <int KT, int KP, int WK>
+__global__ __launch_bounds__(256 * WK, 1) void dgemm_reg_kernel(GemmArgs p)
 {
     constexpr int BM = 256, BN = 32, TM = 4, TN = 2, KQ = KT / 4, R = GK / KT;
-    static_assert(GK % KT == 0 && KT % 4 == 0 && (KP - 1) * KQ * (TM + TN) <= 63, "pipeline");
+    static_assert(GK % KT == 0 && KT % 4 == 0 && (KP - 1) * KQ * (TM + TN) <= 63 && WK >= 1 && WK <= 2, "pipeline");
     int tm, tn, s, S;
     int64_t z;
-    if (!unit_of_block(p, tm, tn, s, z, S)) return;
+    if (!unit_of_block(p, tm, tn, s, z, S)) return;   // the whole workgroup: every wave reaches the barrier below
     const int kt0 = R * (int)((int64_t)p.ktiles * s / S), kt1 = R * (int)((int64_t)p.ktiles * (s + 1) / S);
     // the wave index as a wave-uniform value, so the descriptors built from it stay scalar
     const int lane = (int)threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int64_t row0 = (int64_t)tm * BM + 64 * wave, col0 = (int64_t)tn * BN;
+    const int wm = wave & 3, wk = wave >> 2;
+    const int64_t row0 = (int64_t)tm * BM + 64 * wm, col0 = (int64_t)tn * BN;
     const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
     const int64_t ks0 = (int64_t)kt0 * KT, ke = (int64_t)kt1 * KT < k ? (int64_t)kt1 * KT : k;
     const int64_t cl = col0 + BN < n ? col0 + BN : n;   // end of the unit's columns
@@ -378,9 +383,11 @@ __global__ __launch_bounds__(256, 1) void dgemm_reg_kernel(GemmArgs p)
     const uint32_t voA = (uint32_t)(8 * ((lane & 15) + lda * (lane >> 4)));
     const uint32_t voB = (uint32_t)(8 * ((lane >> 4) + ldb * (lane & 15)));
     const int nkt = kt1 - kt0;
+    const int nj = nkt > wk ? (nkt - wk + WK - 1) / WK : 0;   // this wave's k-tiles: r = wk + WK j
     double fa[KP][KQ][TM], fb[KP][KQ][TN];
-    // k-tile r of the unit (k from ks0 + KT r) into pipeline slot u
-    auto load = [&](int u, int r) {
+    // the wave's k-tile j (k from ks0 + KT (wk + WK j)) into pipeline slot u
+    auto load = [&](int u, int j) {
+        const int r = wk + WK * j;
 #pragma unroll
         for (int q = 0; q < KQ; q++) {
             const uint32_t kofs = (uint32_t)(KT * r + 4 * q);
@@ -403,16 +410,17 @@ __global__ __launch_bounds__(256, 1) void dgemm_reg_kernel(GemmArgs p)
         for (int b = 0; b < TN; b++) acc[a][b] = (double4_t){0, 0, 0, 0};
 #pragma unroll
     for (int u = 0; u < KP - 1; u++)
-        if (u < nkt) load(u, u);
-    for (int r0 = 0; r0 < nkt; r0 += KP) {
+        if (u < nj) load(u, u);
+    for (int j0 = 0; j0 < nj; j0 += KP) {
 #pragma unroll
         for (int u = 0; u < KP; u++) {
-            const int r = r0 + u;
-            // the slot computed last step is free: the loads of tile r + KP - 1 go there, also
-            // past the unit's last tile (zeros from the descriptor range, never used), so every
+            const int j = j0 + u;
+            // the slot computed last step is free: the loads of tile j + KP - 1 go there, also
+            // past the wave's last tile (zeros from the descriptor range, never used), so every
             // step issues the same loads and the waits count them statically
-            load((u + KP - 1) % KP, r + KP - 1);
-            if (r < nkt) {
+            load((u + KP - 1) % KP, j + KP - 1);
+            if (j < nj) {
+                const int r = wk + WK * j;
                 if (ks0 + KT * (int64_t)(r + 1) > k) {   // B rows past k: not zero in memory
 #pragma unroll
                     for (int q = 0; q < KQ; q++)
@@ -429,6 +437,28 @@ __global__ __launch_bounds__(256, 1) void dgemm_reg_kernel(GemmArgs p)
                             acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][q][a], fb[u][q][b], acc[a][b], 0, 0, 0);
             }
         }
+    }
+    if constexpr (WK == 2) {
+        // the odd wave's accumulators to LDS, added by the even wave of the same rows (lane-major:
+        // each wave's 32 values per lane as 32 rows of 64 consecutive doubles)
+        __shared__ double xs[4 * TM * TN * 4 * 64];
+        double *const xw = xs + wm * (TM * TN * 4 * 64) + lane;
+        if (wk == 1) {
+#pragma unroll
+            for (int a = 0; a < TM; a++)
+#pragma unroll
+                for (int b = 0; b < TN; b++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) xw[((a * TN + b) * 4 + e) * 64] = acc[a][b][e];
+        }
+        __syncthreads();
+        if (wk == 1) return;
+#pragma unroll
+        for (int a = 0; a < TM; a++)
+#pragma unroll
+            for (int b = 0; b < TN; b++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[a][b][e] += xw[((a * TN + b) * 4 + e) * 64];
     }
     // D[i][j] of a 16x16 f64 tile: element e of lane l holds i = 4 e + l / 16, j = l % 16
     const int64_t zc = opi(p, z, 4);
@@ -447,20 +477,182 @@ __global__ __launch_bounds__(256, 1) void dgemm_reg_kernel(GemmArgs p)
             }
 }
 
-// C = Σ_s part_s in ascending s (then + C if accumulating), or the complex epilogue.
+
+// The thin-tile product with 16-byte operand loads: a 256 x 32 tile per workgroup of WM x WK waves,
+// WM along m (each 256 / WM rows as TM = 16 / WM x 2 accumulators of 16x16) and WK along k.  Loads
+// fetch two doubles per lane (A needs an even lda and sA and a 16-byte aligned base; with B16 so
+// does B, else B's pair is two 8-byte loads):
+//   * A: rows are paired, so accumulators 2h and 2h + 1 of a wave take rows 32 h + 2 r and
+//     32 h + 2 r + 1 (r = lane mod 16) from one 16-byte load;
+//   * k runs in 8-deep blocks whose two MFMA steps take k = kb + 2 g and kb + 2 g + 1
+//     (g = lane / 16), so one 16-byte load of the k-contiguous B column serves both steps;
+// half the load instructions of dgemm_reg_kernel for the same MFMAs.  The WK waves of a row block
+// take the 8-deep blocks r = wk + WK j of the unit, each running its own software pipeline (KP
+// slots, KP - 1 blocks of loads in flight, no barrier in the k loop); their accumulators are
+// summed in LDS at the end by halving (WK = 4: (w0 + w2) + (w1 + w3)).  A fixed k order per
+// accumulator, its own: for the same split the same bits on every launch.
+template <int KP, int WM, int WK, bool B16>
+__global__ __launch_bounds__(64 * WM * WK, 1) void dgemm_pair_kernel(GemmArgs p)
+{
+    constexpr int KT = 8, BM = 256, BN = 32, TM = 16 / WM, TN = 2, NP = TM / 2, RW = 16 * TM, R = GK / KT;
+    static_assert(WM * TM == 16 && TM % 2 == 0 && (KP - 1) * (2 * NP + (B16 ? 1 : 2) * TN) <= 63 && (WK & (WK - 1)) == 0,
+                  "tiling");
+    int tm, tn, s, S;
+    int64_t z;
+    if (!unit_of_block(p, tm, tn, s, z, S)) return;   // the whole workgroup: every wave reaches the barriers below
+    const int kt0 = R * (int)((int64_t)p.ktiles * s / S), kt1 = R * (int)((int64_t)p.ktiles * (s + 1) / S);
+    const int lane = (int)threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int wm = wave % WM, wk = wave / WM;
+    const int64_t row0 = (int64_t)tm * BM + RW * wm, col0 = (int64_t)tn * BN;
+    const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
+    const int64_t ks0 = (int64_t)kt0 * KT, ke = (int64_t)kt1 * KT < k ? (int64_t)kt1 * KT : k;
+    const int64_t cl = col0 + BN < n ? col0 + BN : n;   // end of the unit's columns
+    const double *A = p.A + p.sA * opi(p, z, 1) + row0 + lda * ks0;
+    const double *B = p.B + p.sB * opi(p, z, 2) + ks0 + ldb * col0;
+    // descriptor ranges as in dgemm_reg_kernel: A up to row m - 1 of column ke - 1, B up to row
+    // ke - 1 of column cl - 1; past them the loads return 0
+    const int64_t extA = (ke > ks0 && row0 < m) ? (m - row0) + lda * (ke - 1 - ks0) : 0;
+    const int64_t extB = ke > ks0 ? (ke - ks0) + ldb * (cl - 1 - col0) : 0;
+    const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc((void *)A, (short)0, (int)(8 * extA), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc((void *)B, (short)0, (int)(8 * extB), 0x00020000);
+    const int r16 = lane & 15, g = lane >> 4;
+    const uint32_t voA = (uint32_t)(8 * (2 * r16 + lda * (2 * g)));   // rows 2 r, 2 r + 1 at k = 2 g
+    const uint32_t voB = (uint32_t)(8 * (2 * g + ldb * r16));         // k = 2 g, 2 g + 1 of column r
+    const int nkt = kt1 - kt0;
+    const int nj = nkt > wk ? (nkt - wk + WK - 1) / WK : 0;           // this wave's blocks: r = wk + WK j
+    double2 fa[KP][2][NP], fb[KP][TN];   // [slot][step][row pair], [slot][column block]
+    auto load = [&](int u, int j) {
+        const uint32_t kb = (uint32_t)(KT * (wk + WK * j));
+#pragma unroll
+        for (int st = 0; st < 2; st++)
+#pragma unroll
+            for (int pr = 0; pr < NP; pr++) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsA, voA, 8u * (32u * pr + (uint32_t)lda * (kb + st)), 0);
+                __builtin_memcpy(&fa[u][st][pr], &v, 16);
+            }
+#pragma unroll
+        for (int b = 0; b < TN; b++) {
+            if constexpr (B16) {
+                const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsB, voB, 8u * (kb + 16u * (uint32_t)ldb * b), 0);
+                __builtin_memcpy(&fb[u][b], &v, 16);
+            } else {
+                const auto v0 = __builtin_amdgcn_raw_buffer_load_b64(rsB, voB, 8u * (kb + 16u * (uint32_t)ldb * b), 0);
+                const auto v1 = __builtin_amdgcn_raw_buffer_load_b64(rsB, voB, 8u * (kb + 1u + 16u * (uint32_t)ldb * b), 0);
+                __builtin_memcpy(&fb[u][b].x, &v0, 8);
+                __builtin_memcpy(&fb[u][b].y, &v1, 8);
+            }
+        }
+    };
+    double4_t acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++) acc[a][b] = (double4_t){0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < KP - 1; u++)
+        if (u < nj) load(u, u);
+    for (int j0 = 0; j0 < nj; j0 += KP) {
+#pragma unroll
+        for (int u = 0; u < KP; u++) {
+            const int j = j0 + u;
+            // the slot computed last step is free: the loads of block j + KP - 1 go there, also
+            // past the wave's last block (zeros from the descriptor range, never used)
+            load((u + KP - 1) % KP, j + KP - 1);
+            if (j < nj) {
+                const int64_t kr = ks0 + KT * (int64_t)(wk + WK * j);
+                if (kr + KT > k) {   // B rows past k: not zero in memory
+                    const int64_t k0 = kr + 2 * g;
+#pragma unroll
+                    for (int b = 0; b < TN; b++) {
+                        if (k0 >= k) fb[u][b].x = 0.0;
+                        if (k0 + 1 >= k) fb[u][b].y = 0.0;
+                    }
+                }
+#pragma unroll
+                for (int st = 0; st < 2; st++)
+#pragma unroll
+                    for (int a = 0; a < TM; a++)
+#pragma unroll
+                        for (int b = 0; b < TN; b++) {
+                            const double av = (a & 1) ? fa[u][st][a >> 1].y : fa[u][st][a >> 1].x;
+                            const double bv = st ? fb[u][b].y : fb[u][b].x;
+                            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[a][b], 0, 0, 0);
+                        }
+            }
+        }
+    }
+    if constexpr (WK > 1) {
+        // waves h .. 2h - 1 hand their accumulators to waves 0 .. h - 1 of the same rows, h = WK / 2,
+        // WK / 4, ..., 1 (lane-major: a wave's values as rows of 64 consecutive doubles)
+        extern __shared__ double gemm_lds[];
+        constexpr int PER = TM * TN * 4 * 64;
+#pragma unroll
+        for (int h = WK / 2; h >= 1; h /= 2) {
+            if (wk >= h && wk < 2 * h) {
+                double *const xw = gemm_lds + ((wk - h) * WM + wm) * PER + lane;
+#pragma unroll
+                for (int a = 0; a < TM; a++)
+#pragma unroll
+                    for (int b = 0; b < TN; b++)
+#pragma unroll
+                        for (int e = 0; e < 4; e++) xw[((a * TN + b) * 4 + e) * 64] = acc[a][b][e];
+            }
+            __syncthreads();
+            if (wk < h) {
+                const double *const xr = gemm_lds + (wk * WM + wm) * PER + lane;
+#pragma unroll
+                for (int a = 0; a < TM; a++)
+#pragma unroll
+                    for (int b = 0; b < TN; b++)
+#pragma unroll
+                        for (int e = 0; e < 4; e++) acc[a][b][e] += xr[((a * TN + b) * 4 + e) * 64];
+            }
+            __syncthreads();
+        }
+        if (wk > 0) return;
+    }
+    // D[i][j] of a 16x16 f64 tile: element e of lane l holds i = 4 e + l / 16, j = l % 16; MFMA
+    // row i of accumulator a is row 32 (a / 2) + 2 i + a % 2 of the wave's block
+    const int64_t zc = opi(p, z, 4);
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t gi = row0 + 32 * (a >> 1) + 2 * (4 * e + (lane >> 4)) + (a & 1);
+                const int64_t gj = col0 + b * 16 + (lane & 15);
+                if (gi < m && gj < n) {
+                    if (S == 1) store_out(p, zc, gi, gj, acc[a][b][e]);
+                    else p.part[gi + m * gj + m * n * (s + (int64_t)S * z)] = acc[a][b][e];
+                }
+            }
+}
+
+// C = Σ_s part_s in ascending s (then + C if accumulating), or the complex epilogue.  A grid of
+// at most REDUCE_BLOCKS workgroups per instance strides over the m n outputs, so a launch whose
+// device-side split turns out to be 1 (gated batches: the host cannot know) costs little.
+constexpr int64_t REDUCE_BLOCKS = 1024;
 __global__ __launch_bounds__(256) void dgemm_splitk_reduce_kernel(GemmArgs p)
 {
     const int64_t mn = p.m * p.n;
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t z = blockIdx.y;
-    if (q >= mn) return;
     if (p.gate && p.gate[p.idx ? p.idx[z] : z] == 0) return;
     const int S = split_for(p, active_instances(p));
     if (S == 1) return;   // the product wrote its result directly
-    const double *src = p.part + mn * (int64_t)S * z + q;
-    double acc = src[0];
-    for (int s = 1; s < S; s++) acc += src[mn * s];
-    store_out(p, opi(p, z, 4), q % p.m, q / p.m, acc);
+    const int64_t zc = opi(p, z, 4);
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < mn; q += (int64_t)gridDim.x * blockDim.x) {
+        const double *src = p.part + mn * (int64_t)S * z + q;
+        double acc = src[0];
+        for (int s = 1; s < S; s++) acc += src[mn * s];
+        store_out(p, zc, q % p.m, q / p.m, acc);
+    }
+}
+
+static unsigned reduce_blocks(int64_t mn)
+{
+    const int64_t b = (mn + 255) / 256;
+    return (unsigned)(b < REDUCE_BLOCKS ? b : REDUCE_BLOCKS);
 }
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -716,8 +908,43 @@ __global__ __launch_bounds__(256, 2) void lpkgemm_kernel(GemmArgs p)
 // m is the ~1300-node frequency grid about one workgroup per CU without a k split.
 struct GemmShape { int bm, bn, wmw, wnw, per_cu; };
 constexpr GemmShape SHAPES[] = {{64, 64, 2, 2, 4},  {128, 64, 2, 2, 3}, {64, 128, 2, 2, 3}, {128, 128, 2, 2, 2},
-                                {256, 128, 4, 2, 1}, {256, 32, 4, 1, 2}, {128, 32, 4, 1, 3}, {256, 32, 4, 1, 1}};
-constexpr int NSHAPES = 8;   // the last: dgemm_reg_kernel (no LDS)
+                                {256, 128, 4, 2, 1}, {256, 32, 4, 1, 2}, {128, 32, 4, 1, 3}, {256, 32, 4, 1, 1},
+                                {256, 32, 4, 1, 1}, {256, 32, 4, 1, 1}, {256, 32, 2, 1, 1}};
+// 7, 8: dgemm_reg_kernel with one wave, or two waves, along k; 9, 10: dgemm_pair_kernel with 4 x 2
+// and 2 x 4 waves (m x k), for an A of even pitch and 16-byte aligned only
+constexpr int NSHAPES = 11;
+constexpr bool pair_shape(int c) { return c == 9 || c == 10; }
+
+// dgemm_pair_kernel's 16-byte loads: A with an even pitch and stride and a 16-byte aligned base
+// (pair_fits), and B likewise for its 16-byte form (pair_b16)
+static bool pair_fits(const GemmArgs &p)
+{
+    return p.lda % 2 == 0 && p.sA % 2 == 0 && (reinterpret_cast<uintptr_t>(p.A) & 15) == 0;
+}
+static bool pair_b16(const GemmArgs &p)
+{
+    return p.ldb % 2 == 0 && p.sB % 2 == 0 && (reinterpret_cast<uintptr_t>(p.B) & 15) == 0;
+}
+
+template <int KP, int WM, int WK, bool B16>
+static void launch_pair_b(epsx_ctx *ctx, unsigned grid, const GemmArgs &p)
+{
+    constexpr size_t lds = WK > 1 ? (size_t)(WK / 2) * WM * (16 / WM) * 2 * 4 * 64 * 8 : 0;
+    static_assert(lds <= 160 * 1024, "LDS");
+    static bool raised = false;   // once per process, by the first (eager) launch
+    if (lds > 64 * 1024 && !raised) {
+        (void)hipFuncSetAttribute((const void *)dgemm_pair_kernel<KP, WM, WK, B16>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        raised = true;
+    }
+    hipLaunchKernelGGL((dgemm_pair_kernel<KP, WM, WK, B16>), dim3(grid), dim3(64 * WM * WK), lds, ctx->stream, p);
+}
+template <int KP, int WM, int WK>
+static void launch_pair(epsx_ctx *ctx, unsigned grid, const GemmArgs &p)
+{
+    if (pair_b16(p)) launch_pair_b<KP, WM, WK, true>(ctx, grid, p);
+    else launch_pair_b<KP, WM, WK, false>(ctx, grid, p);
+}
 
 // The descriptor spans of the buffer-load kernel fit 31 bits for every unit: A spans the unit's k
 // range (at most the largest split, in 32-wide tiles) times lda, B its BN columns times ldb.
@@ -746,6 +973,8 @@ static void launch_buf(epsx_ctx *ctx, unsigned grid, const GemmArgs &p)
 
 static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
 {
+    if (pair_shape(c) && !pair_fits(p)) c = 7;   // odd pitches: the one-wave register kernel
+    ctx->last_gemm_shape = c;
     const GemmShape &g = SHAPES[c];
     p.mt = (int)((p.m + g.bm - 1) / g.bm);
     p.nt = (int)((p.n + g.bn - 1) / g.bn);
@@ -763,9 +992,10 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
         case 4: launch_buf<256, 128, 4, 2, 1>(ctx, grid, p); break;
         case 5: launch_buf<256, 32, 4, 1, 2>(ctx, grid, p); break;
         case 6: launch_buf<128, 32, 4, 1, 3>(ctx, grid, p); break;
-        default:
-            hipLaunchKernelGGL((dgemm_reg_kernel<8, 4>), dim3(grid), dim3(256), 0, ctx->stream, p);
-            break;
+        case 7: hipLaunchKernelGGL((dgemm_reg_kernel<8, 4, 1>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
+        case 8: hipLaunchKernelGGL((dgemm_reg_kernel<8, 4, 2>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
+        case 9: launch_pair<4, 4, 2>(ctx, grid, p); break;
+        default: launch_pair<2, 2, 4>(ctx, grid, p); break;
         }
     } else {
         // operands whose per-unit span passes 31 bits: the register-staged kernel with 64-bit
@@ -776,7 +1006,9 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
         case 2: hipLaunchKernelGGL((dgemm_kernel<64, 128, 2, 2, 2>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
         case 3: hipLaunchKernelGGL((dgemm_kernel<128, 128, 4, 2, 2>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
         case 4: hipLaunchKernelGGL((dgemm_kernel<256, 128, 4, 2, 1, 16>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
-        case 5: case 7: hipLaunchKernelGGL((dgemm_kernel<256, 32, 4, 1, 2>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
+        case 5: case 7: case 8: case 9: case 10:
+            hipLaunchKernelGGL((dgemm_kernel<256, 32, 4, 1, 2>), dim3(grid), dim3(256), 0, ctx->stream, p);
+            break;
         default: hipLaunchKernelGGL((dgemm_kernel<128, 32, 4, 1, 3>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
         }
     }
@@ -855,7 +1087,7 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
     const bool capturing = cap != hipStreamCaptureStatusNone;
     auto reduce = [&](const GemmArgs &q, int Smax) -> int {
         if (Smax > 1) {
-            hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
+            hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3(reduce_blocks(m * n), (unsigned)bt.n_inst),
                                dim3(256), 0, ctx->stream, q);
             EPSX_HIP(ctx, hipGetLastError());
         }
@@ -905,6 +1137,7 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
         EPSX_HIP(ctx, hipEventCreate(&e1));
         float bestms = 0;
         for (int q = 0; q < NSHAPES; q++) {
+            if (pair_shape(q) && !pair_fits(p)) continue;
             GemmArgs pq;
             int Smq = 1;
             EPSX_TRY(plan(q, pq, Smq));
@@ -971,7 +1204,7 @@ int lpgemm_tn(epsx_ctx *ctx, int precision, int64_t m, int64_t n, int64_t k, con
     EPSX_HIP(ctx, hipGetLastError());
     if (bt.mark) EPSX_HIP(ctx, hipEventRecord(bt.mark, ctx->stream));
     if (Smax > 1) {
-        hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((m * n + 255) / 256), (unsigned)bt.n_inst),
+        hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3(reduce_blocks(m * n), (unsigned)bt.n_inst),
                            dim3(256), 0, ctx->stream, p);
         EPSX_HIP(ctx, hipGetLastError());
     }

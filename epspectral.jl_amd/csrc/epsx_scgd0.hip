@@ -60,13 +60,16 @@ __global__ void widen_kernel(int64_t n, const int32_t *__restrict__ tidx, const 
 
 // G̃c[:, c] = Σ_{members m of class c, in order} G̃[:, m] / |c|: the multiplet average of
 // src/scgd0.jl:294-300 applied to the columns of the linear map ImΣ = h G̃.
-__global__ void gt_classes_kernel(int64_t K, int64_t nc, const int64_t *__restrict__ cptr,
+// Gtc has the column pitch Kp >= K (rows K .. Kp - 1 zero), so an even pitch can be chosen for the
+// product's 16-byte operand loads.
+__global__ void gt_classes_kernel(int64_t K, int64_t Kp, int64_t nc, const int64_t *__restrict__ cptr,
                                   const int32_t *__restrict__ cidx, const double *__restrict__ Gt,
                                   double *__restrict__ Gtc)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= K * nc) return;
-    const int64_t k = q % K, c = q / K;
+    if (q >= Kp * nc) return;
+    const int64_t k = q % Kp, c = q / Kp;
+    if (k >= K) { Gtc[q] = 0.0; return; }
     const int64_t a = cptr[c], b = cptr[c + 1];
     double sum = 0;
     for (int64_t j = a; j < b; j++) sum += Gt[k + K * (int64_t)cidx[j]];
@@ -796,6 +799,14 @@ extern "C" int epsx_scgd0_set_problem(epsx_ctx *ctx, const epsx_scgd0_problem *P
     EPSX_TRY(dev_alloc(ctx, s.KKo, sizeof(double) * nw * nw));
     EPSX_TRY(dev_alloc(ctx, s.KKd, sizeof(double) * nw));
     EPSX_TRY(kk_split(ctx, nw, dptr<double>(s.KK), dptr<double>(s.KKo), dptr<double>(s.KKd)));
+    {
+        // KK with an even column pitch, the A operand of the FP64 product's 16-byte loads
+        const int64_t ldk = nw + (nw & 1);
+        EPSX_TRY(dev_alloc(ctx, s.KKp, sizeof(double) * ldk * nw));
+        EPSX_HIP(ctx, hipMemsetAsync(s.KKp.p, 0, sizeof(double) * ldk * nw, ctx->stream));
+        EPSX_HIP(ctx, hipMemcpy2DAsync(s.KKp.p, sizeof(double) * ldk, s.KK.p, sizeof(double) * nw, sizeof(double) * nw,
+                                       nw, hipMemcpyDeviceToDevice, ctx->stream));
+    }
 
     // star order of the inner states
     std::vector<int32_t> fsrc;
@@ -1024,18 +1035,22 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     const bool use_f32 = lp_native;                                       // FP32 level present
     const int64_t kp = (s.nm * ni + 7) / 8 * 8;
     const int64_t kpw = (nw + 7) / 8 * 8;
+    // column pitches: h's a multiple of 64 (its builds store whole aligned 512-byte runs per wave,
+    // and the FP64 product's 16-byte loads need an even pitch), G̃c's even (Kp); G̃ without classes
+    // keeps the star reduction's pitch nm ni
+    const int64_t ldh = (nw + 63) / 64 * 64, K = s.nm * ni, Kp = K + (K & 1), ldG = use_cls ? Kp : K;
     // Einstein method: H' for a chunk of ein_fc sorted inner states per slot (gpu_tile if given, else
     // chunks that keep H' of all temperatures within 2 GiB)
     int64_t ein_fc = 0;
     if (method == EPSX_FM_EINSTEIN) {
-        const int64_t per_f = 8 * nw * s.nm * nT;
+        const int64_t per_f = 8 * ldh * s.nm * nT;
         ein_fc = par->f_tile > 0 ? par->f_tile : (int64_t(2) << 30) / per_f;
         ein_fc = ein_fc < 1 ? 1 : (ein_fc > nf ? nf : ein_fc);
         EPSX_TRY(tmp_alloc(ctx, hbuf.b, (size_t)per_f * ein_fc));
         EPSX_TRY(tmp_alloc(ctx, hocc.b, 16 * (size_t)nw * s.nm * nT));
     }
     if (method == EPSX_FM_STAR) {
-        EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)nw * s.nm * ni * nT));
+        EPSX_TRY(tmp_alloc(ctx, hbuf.b, 8 * (size_t)ldh * s.nm * ni * nT));
         EPSX_TRY(tmp_alloc(ctx, hocc.b, 16 * (size_t)nw * s.nm * nT));
         if (!(par->reuse_star && s.gt_ready && s.gt_wac == par->w_acoustic)) {
             // G̃[(ν, i'), i] = Σ_{f in star(i')} w_f g2[ν, i, f], acoustic modes zeroed
@@ -1059,8 +1074,8 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             ctx->t_class[4] = ms * 1e-3;
             ctx->n_class[4] = 1;
             if (s.nc > 0) {
-                EPSX_TRY(dev_alloc(ctx, s.Gtc, 8 * (size_t)s.nm * ni * s.nc));
-                hipLaunchKernelGGL(gt_classes_kernel, dim3(nblk(s.nm * ni * s.nc)), dim3(256), 0, st, s.nm * ni, s.nc,
+                EPSX_TRY(dev_alloc(ctx, s.Gtc, 8 * (size_t)Kp * s.nc));
+                hipLaunchKernelGGL(gt_classes_kernel, dim3(nblk(Kp * s.nc)), dim3(256), 0, st, K, Kp, s.nc,
                                    dptr<int64_t>(s.cls_ptr), dptr<int32_t>(s.cls_idx), dptr<double>(s.Gt),
                                    dptr<double>(s.Gtc));
                 EPSX_HIP(ctx, hipGetLastError());
@@ -1072,12 +1087,12 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         const double *Gprod = use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt);
         if (use_bf && !s.gb_ready) {
             EPSX_TRY(dev_alloc(ctx, s.Gb, 2 * (size_t)kp * ncol));
-            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_BF16, s.nm * ni, kp, ncol, Gprod, s.Gb.p));
+            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_BF16, K, ldG, kp, ncol, Gprod, s.Gb.p));
             s.gb_ready = true;
         }
         if (use_f32 && !s.gf_ready) {
             EPSX_TRY(dev_alloc(ctx, s.Gf, 4 * (size_t)kp * ncol));
-            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_FP32, s.nm * ni, kp, ncol, Gprod, s.Gf.p));
+            EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_FP32, K, ldG, kp, ncol, Gprod, s.Gf.p));
             s.gf_ready = true;
         }
         if (use_cls) EPSX_TRY(tmp_alloc(ctx, ImSc.b, 8 * (size_t)nw * ncol * nT));
@@ -1165,11 +1180,14 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     std::vector<int32_t> active(nT, 1), act_idx;
     if (wh)
         for (int64_t t = wh->nTu; t < nT; t++) active[t] = 0;
-    // The star method's FP64 product h·G̃ is a plain strided-batched GEMM with a shared G̃: hipBLASLt
-    // runs it (85% of the FP64 peak on the t2g 24³ shape against 58-62% for the library's kernel,
-    // which EPSX_STAR_BLAS=0 or EPSX_GEMM_SHAPE selects, and which serves every product with masked
-    // or low-precision instances); the host knows which slots take the FP64 pass, since a graph is
-    // captured again whenever the active set changes.
+    // The FP64 products run in the library's own FP64 MFMA GEMM (autotuned tile shape; on the t2g
+    // 24³ shapes the 16-byte-load kernel), gated per temperature on the device: the star method's
+    // h·G̃, and Kramers-Kronig's KK·ImΣ over the product's columns into ReS, packed into Σ with err
+    // by one kernel.  EPSX_STAR_BLAS=1 hands both to hipBLASLt as plain strided-batched GEMMs (no
+    // device mask: the host knows which slots take the FP64 pass, since a graph is captured again
+    // whenever the active set changes).  The low-precision levels' products run in hipBLASLt unless
+    // EPSX_STAR_BLAS=0 or EPSX_GEMM_SHAPE keeps every product in the library's kernels (and
+    // Kramers-Kronig in the GEMM's complex epilogue).
     bool star_lt = false, kk_lt = false, lp_lt[3] = {false, false, false}, kklp_lt = false;
     Tmp ReS;    // KK ImΣ of the library product, per column of the star product (then packed into Σ)
     Tmp ImSf;   // FP32 result of the low-precision library products
@@ -1178,10 +1196,14 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     // the library's own kernels, as it keeps their default tile shape)
     const char *blas_env = getenv("EPSX_STAR_BLAS"), *shape_env = getenv("EPSX_GEMM_SHAPE");
     const bool use_lt = maxiter >= 3 && !(blas_env && blas_env[0] == '0') && !(shape_env && *shape_env);
+    const bool star_blas = use_lt && blas_env && blas_env[0] == '1';
+    // Kramers-Kronig's FP64 product into ReS by the library's kernel, then packed with err
+    const bool kk_own = use_lt && !star_blas && (use_cls || s.n_mult == 0) && nw * ni <= INT32_MAX - 65536 * 256;
+    if (kk_own) EPSX_TRY(tmp_alloc(ctx, ReS.b, 8 * nS));
     {
         // Kramers-Kronig's FP64 product KK·ImΣ likewise (KK shared, over the star product's columns,
         // the complex result packed after)
-        if (use_lt) {
+        if (star_blas) {
             EPSX_TRY(tmp_alloc(ctx, ReS.b, 8 * nS));
             // candidates timed on pseudo-random operands (ImΣ is overwritten before its first use)
             hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(nS)), dim3(256), 0, st, B.ImS.ptr<double>(), nS);
@@ -1196,14 +1218,15 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     }
     if (method == EPSX_FM_STAR) {
         if (use_lt) {
-            // candidates timed on pseudo-random h (overwritten before its first use)
-            const int64_t K = s.nm * ni;
-            hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(nw * K * nT)), dim3(256), 0, st, hbuf.ptr<double>(),
-                               nw * K * nT);
+            // candidates timed on pseudo-random h (overwritten before its first use); the FP64
+            // product only when EPSX_STAR_BLAS=1 asks for it
+            hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(ldh * K * nT)), dim3(256), 0, st, hbuf.ptr<double>(),
+                               ldh * K * nT);
             EPSX_HIP(ctx, hipGetLastError());
-            EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>(), nw, nw * K,
-                                    use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt), K, 0,
-                                    use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), nw, nw * ncol, nT, &star_lt));
+            if (star_blas)
+                EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>(), ldh, ldh * K,
+                                        use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt), ldG, 0,
+                                        use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), nw, nw * ncol, nT, &star_lt));
             // the low-precision levels' products from the stored BF16 / FP32 h and G̃ (FP32 result,
             // widened into ImΣ)
             if (lp_native) {
@@ -1235,7 +1258,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     // The FP64 pass packs Σ_new from the library products' columns and writes Σ ← Σ_new with err
     // in one kernel, unless multiplets without classes still need their average taken over Σ_new.
     // (pack_err_kernel indexes a slice in 32 bits)
-    const bool kk_fuse = kk_lt && (use_cls || s.n_mult == 0) && nw * ni <= INT32_MAX - 65536 * 256;
+    const bool kk_fuse = (kk_lt || kk_own) && (use_cls || s.n_mult == 0) && nw * ni <= INT32_MAX - 65536 * 256;
     // the FP32 level likewise: its star and Kramers-Kronig products by the library in FP32, one
     // pack of both into Σ with err
     if (!(nw * ni <= INT32_MAX - 65536 * 256)) kklp_lt = false;
@@ -1306,7 +1329,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                                     B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA, par->w_acoustic,
                                     hocc.ptr<double>(), need_h64 ? hbuf.ptr<double>() : nullptr, d_act, gate_h64,
                                     gate_hbf ? hb.ptr<uint16_t>() : nullptr, gate_hbf,
-                                    gate_hf ? hf.ptr<float>() : nullptr, gate_hf, kp));
+                                    gate_hf ? hf.ptr<float>() : nullptr, gate_hf, kp, ldh));
             if (events) EPSX_HIP(ctx, hipEventRecord(ev[5], st));
             // ImΣ[:, :, t] = h_z G̃ for every slot z in one batched launch (G̃ shared); with multiplets
             // the class columns (G̃c) into ImΣc, copied to every member afterwards
@@ -1315,7 +1338,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             for (int q = 0; q < npass; q++) {
                 GemmBatch bt;
                 bt.n_inst = nA;
-                bt.sA = nw * s.nm * ni;
+                bt.sA = ldh * K;
                 bt.sC = nw * ncol;
                 bt.idx = d_tidx;
                 bt.imap = 4;
@@ -1343,15 +1366,16 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                     EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ncol, kp, bf ? hb.b.p : hf.b.p, kp,
                                        bf ? s.Gb.p : s.Gf.p, kp, Cprod, nw, false, bt));
                 } else if (star_lt && passes[q].prec == EPSX_PREC_FP64) {
-                    const int64_t K = s.nm * ni, sA = nw * K, sC = nw * ncol;
+                    const int64_t sA = ldh * K, sC = nw * ncol;
                     EPSX_TRY(fp64_runs([&](int z, int n) {
-                        return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>() + sA * z, nw, sA, Gprod,
-                                           K, 0, Cprod + sC * h_tidx[z], nw, sC, n);
+                        return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>() + sA * z, ldh, sA, Gprod,
+                                           ldG, 0, Cprod + sC * h_tidx[z], nw, sC, n);
                     }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
+                    ctx->n_class[7] = 1;
                 } else {
-                    EPSX_TRY(dgemm(ctx, nw, ncol, s.nm * ni, hbuf.ptr<double>(), nw, Gprod, s.nm * ni, Cprod, nw,
-                                   false, bt));
+                    EPSX_TRY(dgemm(ctx, nw, ncol, K, hbuf.ptr<double>(), ldh, Gprod, ldG, Cprod, nw, false, bt));
+                    if (passes[q].prec == EPSX_PREC_FP64) ctx->n_class[7] = 2 + ctx->last_gemm_shape;
                 }
             }
             // ImΣ per outer state for the products that take it (the fused FP64 pack reads the class
@@ -1372,24 +1396,26 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             EPSX_TRY(h_build_launch(ctx, nw, s.nm, ni, dptr<double>(s.ws), dptr<int32_t>(s.hk), dptr<double>(s.ht),
                                     dptr<double>(s.wnu), dptr<double>(s.e_i), B.S.ptr<double>(), B.T.ptr<double>(),
                                     B.mu.ptr<double>(), B.eta.ptr<double>(), d_tidx, nA, par->w_acoustic,
-                                    hocc.ptr<double>(), nullptr, d_act, nullptr, nullptr, nullptr, nullptr, nullptr, 0));
+                                    hocc.ptr<double>(), nullptr, d_act, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                    ldh));
             for (int64_t p0 = 0; p0 < nf; p0 += ein_fc) {
                 const int64_t np = ein_fc < nf - p0 ? ein_fc : nf - p0;
                 EPSX_TRY(einstein_h_launch(ctx, nw, s.nm, ni, dptr<double>(s.ws), dptr<int32_t>(s.hk),
                                            dptr<double>(s.ht), dptr<double>(s.wnu), B.S.ptr<double>(), B.eta.ptr<double>(),
                                            d_tidx, nA, par->w_acoustic, hocc.ptr<double>(), p0, np, dptr<double>(s.ef_s),
-                                           dptr<double>(s.wf_s), dptr<int32_t>(s.fi_s), hbuf.ptr<double>(), d_act));
+                                           dptr<double>(s.wf_s), dptr<int32_t>(s.fi_s), hbuf.ptr<double>(), ldh,
+                                           d_act));
                 for (int q = 0; q < npass; q++) {
                     GemmBatch bt;
                     bt.n_inst = nA;
-                    bt.sA = nw * s.nm * np;
+                    bt.sA = ldh * s.nm * np;
                     bt.sC = nw * ni;
                     bt.idx = d_tidx;
                     bt.imap = 4;
                     bt.gate = passes[q].gate;
                     bt.precision = passes[q].prec;
                     bt.tune = passes[q].tune;
-                    EPSX_TRY(dgemm(ctx, nw, ni, s.nm * np, hbuf.ptr<double>(), nw, dptr<double>(s.g2p) + s.nm * p0,
+                    EPSX_TRY(dgemm(ctx, nw, ni, s.nm * np, hbuf.ptr<double>(), ldh, dptr<double>(s.g2p) + s.nm * p0,
                                    s.nm * nf, B.ImS.ptr<double>(), nw, p0 > 0, bt));
                 }
             }
@@ -1428,14 +1454,26 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             bt.tune = passes[q].tune;
             bt.Z = B.Snew.ptr<double2>();
             bt.Zim = B.ImS.ptr<double>();
-            if (passes[q].prec == EPSX_PREC_FP64 && kk_lt) {
+            if (passes[q].prec == EPSX_PREC_FP64 && (kk_lt || kk_own)) {
                 const int64_t sC = nw * ncol;
                 const double *Cim = use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>();
-                EPSX_TRY(fp64_runs([&](int z, int n) {
-                    const int64_t t0 = h_tidx[z];
-                    return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, nw, dptr<double>(s.KK), nw, 0, Cim + sC * t0, nw,
-                                       sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
-                }));
+                if (kk_lt) {
+                    EPSX_TRY(fp64_runs([&](int z, int n) {
+                        const int64_t t0 = h_tidx[z];
+                        return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, nw, dptr<double>(s.KK), nw, 0, Cim + sC * t0,
+                                           nw, sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
+                    }));
+                } else {
+                    GemmBatch bk;
+                    bk.n_inst = nA;
+                    bk.sB = bk.sC = sC;
+                    bk.idx = d_tidx;
+                    bk.imap = 2 | 4;
+                    bk.gate = passes[q].gate;
+                    bk.tune = passes[q].tune;
+                    EPSX_TRY(dgemm(ctx, nw, ncol, nw, dptr<double>(s.KKp), nw + (nw & 1), Cim, nw, ReS.ptr<double>(), nw,
+                                   false, bk));
+                }
                 if (kk_fuse)
                     hipLaunchKernelGGL(pack_err_kernel<double>, dim3(nb_err, nA), dim3(256), 0, st, (int32_t)nw,
                                        (int32_t)ni, ncol, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx,

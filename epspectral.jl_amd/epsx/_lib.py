@@ -234,7 +234,7 @@ class Context:
         return {"fan_migdal": s[0], "kramers_kronig": s[1], "dos_mu": s[2], "other": s[3],
                 "star_reduce": s[4], "star_gemm": s[5], "fan_migdal_launches": int(n[0]),
                 "star_reduce_launches": int(n[4]), "star_gemm_launches": int(n[5]),
-                "star_gemm_instances": int(n[6])}
+                "star_gemm_instances": int(n[6]), "star_gemm_kernel": int(n[7])}
 
 
 _default = None

@@ -100,3 +100,89 @@ def test_register_gemm_fragments(kt):
     # the launched pipeline (8-deep k-tiles, 4 slots): at most 63 loads in flight per wave
     if kt == 8:
         assert (4 - 1) * KQ * (TM + TN) <= 63
+
+
+@pytest.mark.parametrize("nkt", [0, 1, 2, 3, 7, 8, 257, 1025])
+def test_register_gemm_two_waves_along_k(nkt):
+    """dgemm_reg_kernel with WK = 2 (8 waves: 4 along m, 2 along k): the two waves of a row block
+    take the k-tiles r = wk + 2 j, j < nj(wk), which cover the unit's tiles exactly once; the odd
+    wave's accumulators go to the static LDS array xs[4 * 4 * 2 * 4 * 64] at indices inside it,
+    each slot written once, and the even wave of the same rows reads back exactly the slots its
+    partner wrote for the same (accumulator, element, lane)."""
+    WK, TM, TN = 2, 4, 2
+    tiles = []
+    for wk in range(WK):
+        nj = (nkt - wk + WK - 1) // WK if nkt > wk else 0
+        tiles += [wk + WK * j for j in range(nj)]
+    assert sorted(tiles) == list(range(nkt))
+    size = 4 * TM * TN * 4 * 64
+    written = {}
+    for wm, lane, a, b, e in itertools.product(range(4), range(64), range(TM), range(TN), range(4)):
+        idx = wm * (TM * TN * 4 * 64) + lane + ((a * TN + b) * 4 + e) * 64
+        assert 0 <= idx < size and idx not in written
+        written[idx] = (wm, lane, a, b, e)
+    assert len(written) == size
+
+
+@pytest.mark.parametrize("wm_count,wk_count", [(4, 2), (2, 4)])
+def test_pair_gemm_fragments_and_reduction(wm_count, wk_count):
+    """dgemm_pair_kernel (256 x 32 tiles, WM waves along m, WK along k, 16-byte loads): for every
+    wave and lane the two doubles of each A load are the rows 32 pr + 2 r + {0, 1} at k = kb + 2 g +
+    step, which the MFMA takes as row r of accumulators 2 pr and 2 pr + 1 in lane slot g; the two
+    doubles of each B load are k = kb + 2 g + {0, 1} of column 16 b + r, one per step; over an 8-deep
+    block each wave reads its rows and the 32 columns exactly once per k; the 16-byte loads are
+    aligned for even pitches; the accumulators of the WM row blocks cover the tile exactly once; the
+    LDS halving rounds stay inside the allocation the host sizes and pair each writer with its
+    reader."""
+    lda, ldb = 1276, 8200
+    TM, TN = 16 // wm_count, 2
+    NP, RW = TM // 2, 16 * TM
+    for wm in range(wm_count):
+        seen_a, seen_b = set(), set()
+        for lane in range(64):
+            r, g = lane & 15, lane >> 4
+            voA = 2 * r + lda * (2 * g)
+            voB = 2 * g + ldb * r
+            assert voA % 2 == 0 and voB % 2 == 0        # 16-byte aligned (8-byte elements)
+            for st in range(2):
+                for pr in range(NP):
+                    off = voA + 32 * pr + lda * st
+                    assert off % 2 == 0
+                    for h in range(2):
+                        i, k = (off + h) % lda, (off + h) // lda
+                        assert (i, k) == (32 * pr + 2 * r + h, 2 * g + st)
+                        # accumulator a = 2 pr + h, MFMA row r, k slot g of step st
+                        seen_a.add((RW * wm + i, k))
+            for b in range(TN):
+                off = voB + 16 * ldb * b
+                for h in range(2):
+                    k, j = (off + h) % ldb, (off + h) // ldb
+                    assert (k, j) == (2 * g + h, 16 * b + r)
+                    seen_b.add((k, j))
+        assert seen_a == {(RW * wm + i, k) for i in range(RW) for k in range(8)}
+        assert seen_b == {(k, j) for k in range(8) for j in range(32)}
+    out = {}
+    for wm, lane in itertools.product(range(wm_count), range(64)):
+        for a, b, e in itertools.product(range(TM), range(TN), range(4)):
+            i = RW * wm + 32 * (a >> 1) + 2 * (4 * e + (lane >> 4)) + (a & 1)
+            j = 16 * b + (lane & 15)
+            assert (i, j) not in out
+            out[(i, j)] = (wm, lane)
+    assert len(out) == 256 * 32 and all(0 <= i < 256 and 0 <= j < 32 for i, j in out)
+    # the LDS halving rounds
+    PER = TM * TN * 4 * 64
+    lds = (wk_count // 2) * wm_count * PER
+    assert 8 * lds <= 160 * 1024
+    h = wk_count // 2
+    while h >= 1:
+        written = {}
+        for wk, wm, lane in itertools.product(range(h, 2 * h), range(wm_count), range(64)):
+            for a, b, e in itertools.product(range(TM), range(TN), range(4)):
+                idx = ((wk - h) * wm_count + wm) * PER + lane + ((a * TN + b) * 4 + e) * 64
+                assert 0 <= idx < lds and idx not in written
+                written[idx] = (wk, wm, lane, a, b, e)
+        for wk, wm, lane in itertools.product(range(h), range(wm_count), range(64)):
+            for a, b, e in itertools.product(range(TM), range(TN), range(4)):
+                idx = (wk * wm_count + wm) * PER + lane + ((a * TN + b) * 4 + e) * 64
+                assert written[idx] == (wk + h, wm, lane, a, b, e)
+        h //= 2
