@@ -43,7 +43,8 @@ GEMM_SHAPE_NAMES = {0: "dgemm_buf_kernel 64x64", 1: "dgemm_buf_kernel 128x64", 2
                     6: "dgemm_buf_kernel 128x32", 7: "dgemm_reg_kernel 256x32 (1 wave along k)",
                     8: "dgemm_reg_kernel 256x32 (2 waves along k)",
                     9: "dgemm_pair_kernel 256x32 (16-byte loads, 4 x 2 waves)",
-                    10: "dgemm_pair_kernel 256x32 (16-byte loads, 2 x 4 waves)"}
+                    10: "dgemm_pair_kernel 256x32 (16-byte loads, 2 x 4 waves)",
+                    11: "dgemm_pair_kernel 128x128 (16-byte loads, 2 x 2 waves)"}
 
 def main():
     ap = argparse.ArgumentParser()

@@ -478,41 +478,42 @@ __global__ __launch_bounds__(256 * WK, 1) void dgemm_reg_kernel(GemmArgs p)
 }
 
 
-// The thin-tile product with 16-byte operand loads: a 256 x 32 tile per workgroup of WM x WK waves,
-// WM along m (each 256 / WM rows as TM = 16 / WM x 2 accumulators of 16x16) and WK along k.  Loads
-// fetch two doubles per lane (A needs an even lda and sA and a 16-byte aligned base; with B16 so
-// does B, else B's pair is two 8-byte loads):
+// The product with 16-byte operand loads and no operand staging: a tile of (16 TM WM) x (16 TN WN)
+// per workgroup of WM x WN x WK waves, WM along m and WN along n (each wave a grid of TM x TN
+// accumulators of 16x16) and WK along k.  Loads fetch two doubles per lane (A needs an even lda and
+// sA and a 16-byte aligned base; with B16 so does B, else B's pair is two 8-byte loads):
 //   * A: rows are paired, so accumulators 2h and 2h + 1 of a wave take rows 32 h + 2 r and
 //     32 h + 2 r + 1 (r = lane mod 16) from one 16-byte load;
 //   * k runs in 8-deep blocks whose two MFMA steps take k = kb + 2 g and kb + 2 g + 1
 //     (g = lane / 16), so one 16-byte load of the k-contiguous B column serves both steps;
-// half the load instructions of dgemm_reg_kernel for the same MFMAs.  The WK waves of a row block
-// take the 8-deep blocks r = wk + WK j of the unit, each running its own software pipeline (KP
-// slots, KP - 1 blocks of loads in flight, no barrier in the k loop); their accumulators are
-// summed in LDS at the end by halving (WK = 4: (w0 + w2) + (w1 + w3)).  A fixed k order per
-// accumulator, its own: for the same split the same bits on every launch.
-template <int KP, int WM, int WK, bool B16>
-__global__ __launch_bounds__(64 * WM * WK, 1) void dgemm_pair_kernel(GemmArgs p)
+// half the load instructions of dgemm_reg_kernel for the same MFMAs.  Each wave loads its own
+// fragments straight into registers (waves that share rows or columns of the tile meet in L1); the
+// WK waves of a (row, column) block take the 8-deep blocks r = wk + WK j of the unit, each running
+// its own software pipeline (KP slots, KP - 1 blocks of loads in flight, no barrier in the k loop);
+// their accumulators are summed in LDS at the end by halving (WK = 4: (w0 + w2) + (w1 + w3)).  A
+// fixed k order per accumulator, its own: for the same split the same bits on every launch.
+template <int KP, int WM, int WN, int WK, int TM, int TN, bool B16, int MINW>
+__global__ __launch_bounds__(64 * WM * WN * WK, MINW) void dgemm_pair_kernel(GemmArgs p)
 {
-    constexpr int KT = 8, BM = 256, BN = 32, TM = 16 / WM, TN = 2, NP = TM / 2, RW = 16 * TM, R = GK / KT;
-    static_assert(WM * TM == 16 && TM % 2 == 0 && (KP - 1) * (2 * NP + (B16 ? 1 : 2) * TN) <= 63 && (WK & (WK - 1)) == 0,
-                  "tiling");
+    constexpr int KT = 8, BM = 16 * TM * WM, BN = 16 * TN * WN, NP = TM / 2, R = GK / KT;
+    static_assert(TM % 2 == 0 && (KP - 1) * (2 * NP + (B16 ? 1 : 2) * TN) <= 63 && (WK & (WK - 1)) == 0, "tiling");
     int tm, tn, s, S;
     int64_t z;
     if (!unit_of_block(p, tm, tn, s, z, S)) return;   // the whole workgroup: every wave reaches the barriers below
     const int kt0 = R * (int)((int64_t)p.ktiles * s / S), kt1 = R * (int)((int64_t)p.ktiles * (s + 1) / S);
     const int lane = (int)threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int wm = wave % WM, wk = wave / WM;
-    const int64_t row0 = (int64_t)tm * BM + RW * wm, col0 = (int64_t)tn * BN;
+    const int wm = wave % WM, wn = (wave / WM) % WN, wk = wave / (WM * WN);
+    const int64_t row0 = (int64_t)tm * BM + 16 * TM * wm, col0 = (int64_t)tn * BN + 16 * TN * wn;
     const int64_t m = p.m, n = p.n, k = p.k, lda = p.lda, ldb = p.ldb;
     const int64_t ks0 = (int64_t)kt0 * KT, ke = (int64_t)kt1 * KT < k ? (int64_t)kt1 * KT : k;
-    const int64_t cl = col0 + BN < n ? col0 + BN : n;   // end of the unit's columns
+    const int64_t cl = col0 + 16 * TN < n ? col0 + 16 * TN : n;   // end of the wave's columns
     const double *A = p.A + p.sA * opi(p, z, 1) + row0 + lda * ks0;
     const double *B = p.B + p.sB * opi(p, z, 2) + ks0 + ldb * col0;
     // descriptor ranges as in dgemm_reg_kernel: A up to row m - 1 of column ke - 1, B up to row
-    // ke - 1 of column cl - 1; past them the loads return 0
+    // ke - 1 of column cl - 1 (none for a wave whose columns all lie past n); past them the loads
+    // return 0
     const int64_t extA = (ke > ks0 && row0 < m) ? (m - row0) + lda * (ke - 1 - ks0) : 0;
-    const int64_t extB = ke > ks0 ? (ke - ks0) + ldb * (cl - 1 - col0) : 0;
+    const int64_t extB = (ke > ks0 && col0 < n) ? (ke - ks0) + ldb * (cl - 1 - col0) : 0;
     const __amdgpu_buffer_rsrc_t rsA = __builtin_amdgcn_make_buffer_rsrc((void *)A, (short)0, (int)(8 * extA), 0x00020000);
     const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc((void *)B, (short)0, (int)(8 * extB), 0x00020000);
     const int r16 = lane & 15, g = lane >> 4;
@@ -582,14 +583,15 @@ __global__ __launch_bounds__(64 * WM * WK, 1) void dgemm_pair_kernel(GemmArgs p)
         }
     }
     if constexpr (WK > 1) {
-        // waves h .. 2h - 1 hand their accumulators to waves 0 .. h - 1 of the same rows, h = WK / 2,
-        // WK / 4, ..., 1 (lane-major: a wave's values as rows of 64 consecutive doubles)
+        // waves h .. 2h - 1 hand their accumulators to waves 0 .. h - 1 of the same rows and
+        // columns, h = WK / 2, WK / 4, ..., 1 (lane-major: a wave's values as rows of 64 doubles)
         extern __shared__ double gemm_lds[];
         constexpr int PER = TM * TN * 4 * 64;
+        const int wmn = wm + WM * wn;
 #pragma unroll
         for (int h = WK / 2; h >= 1; h /= 2) {
             if (wk >= h && wk < 2 * h) {
-                double *const xw = gemm_lds + ((wk - h) * WM + wm) * PER + lane;
+                double *const xw = gemm_lds + ((wk - h) * WM * WN + wmn) * PER + lane;
 #pragma unroll
                 for (int a = 0; a < TM; a++)
 #pragma unroll
@@ -599,7 +601,7 @@ __global__ __launch_bounds__(64 * WM * WK, 1) void dgemm_pair_kernel(GemmArgs p)
             }
             __syncthreads();
             if (wk < h) {
-                const double *const xr = gemm_lds + (wk * WM + wm) * PER + lane;
+                const double *const xr = gemm_lds + (wk * WM * WN + wmn) * PER + lane;
 #pragma unroll
                 for (int a = 0; a < TM; a++)
 #pragma unroll
@@ -909,11 +911,12 @@ __global__ __launch_bounds__(256, 2) void lpkgemm_kernel(GemmArgs p)
 struct GemmShape { int bm, bn, wmw, wnw, per_cu; };
 constexpr GemmShape SHAPES[] = {{64, 64, 2, 2, 4},  {128, 64, 2, 2, 3}, {64, 128, 2, 2, 3}, {128, 128, 2, 2, 2},
                                 {256, 128, 4, 2, 1}, {256, 32, 4, 1, 2}, {128, 32, 4, 1, 3}, {256, 32, 4, 1, 1},
-                                {256, 32, 4, 1, 1}, {256, 32, 4, 1, 1}, {256, 32, 2, 1, 1}};
-// 7, 8: dgemm_reg_kernel with one wave, or two waves, along k; 9, 10: dgemm_pair_kernel with 4 x 2
-// and 2 x 4 waves (m x k), for an A of even pitch and 16-byte aligned only
-constexpr int NSHAPES = 11;
-constexpr bool pair_shape(int c) { return c == 9 || c == 10; }
+                                {256, 32, 4, 1, 1}, {256, 32, 4, 1, 1}, {256, 32, 2, 1, 1}, {128, 128, 2, 2, 2}};
+// 7, 8: dgemm_reg_kernel with one wave, or two waves, along k; 9, 10: dgemm_pair_kernel 256x32 with
+// 4 x 2 and 2 x 4 waves (m x k); 11: dgemm_pair_kernel 128x128 with 2 x 2 waves (m x n), two
+// workgroups per CU; the pair shapes for an A of even pitch and 16-byte aligned only
+constexpr int NSHAPES = 12;
+constexpr bool pair_shape(int c) { return c >= 9 && c <= 11; }
 
 // dgemm_pair_kernel's 16-byte loads: A with an even pitch and stride and a 16-byte aligned base
 // (pair_fits), and B likewise for its 16-byte form (pair_b16)
@@ -926,24 +929,25 @@ static bool pair_b16(const GemmArgs &p)
     return p.ldb % 2 == 0 && p.sB % 2 == 0 && (reinterpret_cast<uintptr_t>(p.B) & 15) == 0;
 }
 
-template <int KP, int WM, int WK, bool B16>
+template <int KP, int WM, int WN, int WK, int TM, int TN, int MINW, bool B16>
 static void launch_pair_b(epsx_ctx *ctx, unsigned grid, const GemmArgs &p)
 {
-    constexpr size_t lds = WK > 1 ? (size_t)(WK / 2) * WM * (16 / WM) * 2 * 4 * 64 * 8 : 0;
+    constexpr size_t lds = WK > 1 ? (size_t)(WK / 2) * WM * WN * TM * TN * 4 * 64 * 8 : 0;
     static_assert(lds <= 160 * 1024, "LDS");
     static bool raised = false;   // once per process, by the first (eager) launch
     if (lds > 64 * 1024 && !raised) {
-        (void)hipFuncSetAttribute((const void *)dgemm_pair_kernel<KP, WM, WK, B16>,
+        (void)hipFuncSetAttribute((const void *)dgemm_pair_kernel<KP, WM, WN, WK, TM, TN, B16, MINW>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         raised = true;
     }
-    hipLaunchKernelGGL((dgemm_pair_kernel<KP, WM, WK, B16>), dim3(grid), dim3(64 * WM * WK), lds, ctx->stream, p);
+    hipLaunchKernelGGL((dgemm_pair_kernel<KP, WM, WN, WK, TM, TN, B16, MINW>), dim3(grid), dim3(64 * WM * WN * WK), lds,
+                       ctx->stream, p);
 }
-template <int KP, int WM, int WK>
+template <int KP, int WM, int WN, int WK, int TM, int TN, int MINW>
 static void launch_pair(epsx_ctx *ctx, unsigned grid, const GemmArgs &p)
 {
-    if (pair_b16(p)) launch_pair_b<KP, WM, WK, true>(ctx, grid, p);
-    else launch_pair_b<KP, WM, WK, false>(ctx, grid, p);
+    if (pair_b16(p)) launch_pair_b<KP, WM, WN, WK, TM, TN, MINW, true>(ctx, grid, p);
+    else launch_pair_b<KP, WM, WN, WK, TM, TN, MINW, false>(ctx, grid, p);
 }
 
 // The descriptor spans of the buffer-load kernel fit 31 bits for every unit: A spans the unit's k
@@ -994,8 +998,9 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
         case 6: launch_buf<128, 32, 4, 1, 3>(ctx, grid, p); break;
         case 7: hipLaunchKernelGGL((dgemm_reg_kernel<8, 4, 1>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
         case 8: hipLaunchKernelGGL((dgemm_reg_kernel<8, 4, 2>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
-        case 9: launch_pair<4, 4, 2>(ctx, grid, p); break;
-        default: launch_pair<2, 2, 4>(ctx, grid, p); break;
+        case 9: launch_pair<4, 4, 1, 2, 4, 2, 1>(ctx, grid, p); break;
+        case 10: launch_pair<2, 2, 1, 4, 8, 2, 1>(ctx, grid, p); break;
+        default: launch_pair<3, 2, 2, 1, 4, 4, 2>(ctx, grid, p); break;
         }
     } else {
         // operands whose per-unit span passes 31 bits: the register-staged kernel with 64-bit
@@ -1009,6 +1014,7 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
         case 5: case 7: case 8: case 9: case 10:
             hipLaunchKernelGGL((dgemm_kernel<256, 32, 4, 1, 2>), dim3(grid), dim3(256), 0, ctx->stream, p);
             break;
+        case 11: hipLaunchKernelGGL((dgemm_kernel<128, 128, 4, 2, 2>), dim3(grid), dim3(512), 0, ctx->stream, p); break;
         default: hipLaunchKernelGGL((dgemm_kernel<128, 32, 4, 1, 3>), dim3(grid), dim3(256), 0, ctx->stream, p); break;
         }
     }

@@ -747,15 +747,17 @@ static int ladder_apply_batch(epsx_ctx *ctx, BatchWork &bw, int nA, const std::v
     const hipStream_t st = ctx->stream;
     ClassTimer t_apply(ctx, 1);
     const bool bf = precision == EPSX_PREC_BF16, native = precision != EPSX_PREC_FP64;
-    const int64_t kp = (K + 7) / 8 * 8, esz = bf ? 2 : 4;
+    // the regrouped B̃'s column pitch: kp (low precision, a multiple of 8) or K rounded up to even
+    // (FP64: the 16-byte operand loads of the library's GEMM)
+    const int64_t kp = (K + 7) / 8 * 8, esz = bf ? 2 : 4, Kd = K + (K & 1);
     // the block width of this call's buffers (fixed per BatchWork, so one regrouped B̃ serves it)
     const int64_t cb = bw.cb;
     if (l.btl_prec != precision || l.btl_cb != cb) {
-        const int64_t ncols = ncol * l.ns, pitch = native ? kp : K, eb = native ? esz : 8;
+        const int64_t ncols = ncol * l.ns, pitch = native ? kp : Kd, eb = native ? esz : 8;
         EPSX_TRY(dev_alloc(ctx, l.Btl, eb * (size_t)pitch * ncols));
         const dim3 grid((unsigned)nb(pitch * ncols));
         if (!native)
-            hipLaunchKernelGGL(bt_perm_kernel<double>, grid, dim3(256), 0, st, K, K, ncol, l.ns, cb, dptr<double>(l.Bt),
+            hipLaunchKernelGGL(bt_perm_kernel<double>, grid, dim3(256), 0, st, K, Kd, ncol, l.ns, cb, dptr<double>(l.Bt),
                                dptr<double>(l.Btl));
         else if (bf)
             hipLaunchKernelGGL(bt_perm_kernel<uint16_t>, grid, dim3(256), 0, st, K, kp, ncol, l.ns, cb, dptr<double>(l.Bt),
@@ -815,12 +817,12 @@ static int ladder_apply_batch(epsx_ctx *ctx, BatchWork &bw, int nA, const std::v
             EPSX_TRY(blas_handle(ctx, h));
             const double one = 1.0, zero = 0.0;
             EPSX_TRY(blas_check(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (int)M, (int)nn, (int)K,
-                                                   &one, bw.Jm.ptr<double>(), (int)M, dptr<double>(l.Btl) + K * c0 * l.ns,
-                                                   (int)K, &zero, bw.W.ptr<double>(), (int)M),
+                                                   &one, bw.Jm.ptr<double>(), (int)M, dptr<double>(l.Btl) + Kd * c0 * l.ns,
+                                                   (int)Kd, &zero, bw.W.ptr<double>(), (int)M),
                                 "rocblas_dgemm (ladder product)"));
         } else {
             GemmBatch bt;
-            EPSX_TRY(dgemm(ctx, M, nn, K, bw.Jm.ptr<double>(), M, dptr<double>(l.Btl) + K * c0 * l.ns, K,
+            EPSX_TRY(dgemm(ctx, M, nn, K, bw.Jm.ptr<double>(), M, dptr<double>(l.Btl) + Kd * c0 * l.ns, Kd,
                            bw.W.ptr<double>(), M, false, bt));
         }
         EPSX_TRY(t_gemm.stop(1));

@@ -124,20 +124,21 @@ def test_register_gemm_two_waves_along_k(nkt):
     assert len(written) == size
 
 
-@pytest.mark.parametrize("wm_count,wk_count", [(4, 2), (2, 4)])
-def test_pair_gemm_fragments_and_reduction(wm_count, wk_count):
-    """dgemm_pair_kernel (256 x 32 tiles, WM waves along m, WK along k, 16-byte loads): for every
-    wave and lane the two doubles of each A load are the rows 32 pr + 2 r + {0, 1} at k = kb + 2 g +
-    step, which the MFMA takes as row r of accumulators 2 pr and 2 pr + 1 in lane slot g; the two
-    doubles of each B load are k = kb + 2 g + {0, 1} of column 16 b + r, one per step; over an 8-deep
-    block each wave reads its rows and the 32 columns exactly once per k; the 16-byte loads are
-    aligned for even pitches; the accumulators of the WM row blocks cover the tile exactly once; the
-    LDS halving rounds stay inside the allocation the host sizes and pair each writer with its
-    reader."""
+@pytest.mark.parametrize("WM,WN,WK,TM,TN", [(4, 1, 2, 4, 2), (2, 1, 4, 8, 2), (2, 2, 1, 4, 4)])
+def test_pair_gemm_fragments_and_reduction(WM, WN, WK, TM, TN):
+    """dgemm_pair_kernel (tiles of 16 TM WM x 16 TN WN, WM x WN waves of TM x TN accumulators, WK
+    along k, 16-byte loads): for every wave and lane the two doubles of each A load are the rows
+    32 pr + 2 r + {0, 1} of the wave's block at k = kb + 2 g + step, which the MFMA takes as row r of
+    accumulators 2 pr and 2 pr + 1 in lane slot g; the two doubles of each B load (one 16-byte or two
+    8-byte loads at the same offsets) are k = kb + 2 g + {0, 1} of column 16 b + r of the wave's
+    columns, one per step; over an 8-deep block each wave reads its rows and columns exactly once
+    per k; the 16-byte loads are aligned for even pitches; the accumulators of the WM x WN waves
+    cover the tile exactly once; the LDS halving rounds stay inside the allocation the host sizes
+    and pair each writer with its reader."""
     lda, ldb = 1276, 8200
-    TM, TN = 16 // wm_count, 2
-    NP, RW = TM // 2, 16 * TM
-    for wm in range(wm_count):
+    NP, BM, BN = TM // 2, 16 * TM * WM, 16 * TN * WN
+    for wm, wn in itertools.product(range(WM), range(WN)):
+        row0, col0 = 16 * TM * wm, 16 * TN * wn
         seen_a, seen_b = set(), set()
         for lane in range(64):
             r, g = lane & 15, lane >> 4
@@ -151,38 +152,42 @@ def test_pair_gemm_fragments_and_reduction(wm_count, wk_count):
                     for h in range(2):
                         i, k = (off + h) % lda, (off + h) // lda
                         assert (i, k) == (32 * pr + 2 * r + h, 2 * g + st)
-                        # accumulator a = 2 pr + h, MFMA row r, k slot g of step st
-                        seen_a.add((RW * wm + i, k))
+                        seen_a.add((row0 + i, k))
             for b in range(TN):
                 off = voB + 16 * ldb * b
                 for h in range(2):
                     k, j = (off + h) % ldb, (off + h) // ldb
                     assert (k, j) == (2 * g + h, 16 * b + r)
-                    seen_b.add((k, j))
-        assert seen_a == {(RW * wm + i, k) for i in range(RW) for k in range(8)}
-        assert seen_b == {(k, j) for k in range(8) for j in range(32)}
+                    seen_b.add((k, col0 + j))
+        assert seen_a == {(row0 + i, k) for i in range(16 * TM) for k in range(8)}
+        assert seen_b == {(k, col0 + j) for k in range(8) for j in range(16 * TN)}
     out = {}
-    for wm, lane in itertools.product(range(wm_count), range(64)):
+    for wm, wn, lane in itertools.product(range(WM), range(WN), range(64)):
         for a, b, e in itertools.product(range(TM), range(TN), range(4)):
-            i = RW * wm + 32 * (a >> 1) + 2 * (4 * e + (lane >> 4)) + (a & 1)
-            j = 16 * b + (lane & 15)
+            i = 16 * TM * wm + 32 * (a >> 1) + 2 * (4 * e + (lane >> 4)) + (a & 1)
+            j = 16 * TN * wn + 16 * b + (lane & 15)
             assert (i, j) not in out
-            out[(i, j)] = (wm, lane)
-    assert len(out) == 256 * 32 and all(0 <= i < 256 and 0 <= j < 32 for i, j in out)
+            out[(i, j)] = (wm, wn, lane)
+    assert len(out) == BM * BN and all(0 <= i < BM and 0 <= j < BN for i, j in out)
+    # the launched pipelines: at most 63 loads in flight per wave (B as two 8-byte loads at most)
+    KP = {(4, 1, 2): 4, (2, 1, 4): 2, (2, 2, 1): 3}[(WM, WN, WK)]
+    assert (KP - 1) * (2 * NP + 2 * TN) <= 63
     # the LDS halving rounds
+    if WK == 1:
+        return
     PER = TM * TN * 4 * 64
-    lds = (wk_count // 2) * wm_count * PER
+    lds = (WK // 2) * WM * WN * PER
     assert 8 * lds <= 160 * 1024
-    h = wk_count // 2
+    h = WK // 2
     while h >= 1:
         written = {}
-        for wk, wm, lane in itertools.product(range(h, 2 * h), range(wm_count), range(64)):
+        for wk, wm, wn, lane in itertools.product(range(h, 2 * h), range(WM), range(WN), range(64)):
             for a, b, e in itertools.product(range(TM), range(TN), range(4)):
-                idx = ((wk - h) * wm_count + wm) * PER + lane + ((a * TN + b) * 4 + e) * 64
+                idx = ((wk - h) * WM * WN + wm + WM * wn) * PER + lane + ((a * TN + b) * 4 + e) * 64
                 assert 0 <= idx < lds and idx not in written
-                written[idx] = (wk, wm, lane, a, b, e)
-        for wk, wm, lane in itertools.product(range(h), range(wm_count), range(64)):
+                written[idx] = (wk, wm, wn, lane, a, b, e)
+        for wk, wm, wn, lane in itertools.product(range(h), range(WM), range(WN), range(64)):
             for a, b, e in itertools.product(range(TM), range(TN), range(4)):
-                idx = (wk * wm_count + wm) * PER + lane + ((a * TN + b) * 4 + e) * 64
-                assert written[idx] == (wk + h, wm, lane, a, b, e)
+                idx = (wk * WM * WN + wm + WM * wn) * PER + lane + ((a * TN + b) * 4 + e) * 64
+                assert written[idx] == (wk + h, wm, wn, lane, a, b, e)
         h //= 2

@@ -64,7 +64,7 @@ def test_gemm_tile_shapes_give_the_same_bits(ctx):
     z = pseudorandom(200, 300, seed=4)
     ref = oracle.kramers_kronig_matrix(np.asarray(g)) @ z
     outs = []
-    for shape in range(11):
+    for shape in range(12):
         os.environ["EPSX_GEMM_SHAPE"] = str(shape)
         try:
             outs.append(epsx.kramers_kronig(g, z, ctx))
@@ -74,13 +74,13 @@ def test_gemm_tile_shapes_give_the_same_bits(ctx):
     for k, y in enumerate(outs[1:8], 1):
         assert np.array_equal(y, outs[0]), k
     # shape 8 (two waves along k: the even and the odd k-tiles summed apart, then added) and the
-    # 16-byte-load shapes 9 and 10 (rows and k paired, 2 or 4 waves along k; the pitches here are
-    # even) have fixed orders of their own: the oracle at 1e-13, the same bits on a repeat, and
-    # on the 700-node grid (k splits, several row tiles)
+    # 16-byte-load shapes 9, 10 and 11 (rows and k paired; 2 or 4 waves along k, or 2 x 2 waves over
+    # a 128x128 tile; the pitches here are even) have fixed orders of their own: the oracle at 1e-13,
+    # the same bits on a repeat, and on the 700-node grid (k splits, several row tiles)
     g7 = FrequencyGrid(-1.0, 0.003, np.arange(700))
     z7 = pseudorandom(700, 300, seed=3)
     ref7 = oracle.kramers_kronig_matrix(np.asarray(g7)) @ z7
-    for shape in (8, 9, 10):
+    for shape in (8, 9, 10, 11):
         assert rel(outs[shape], ref) <= 1e-13, shape
         os.environ["EPSX_GEMM_SHAPE"] = str(shape)
         try:
@@ -92,7 +92,7 @@ def test_gemm_tile_shapes_give_the_same_bits(ctx):
     g3 = FrequencyGrid(0.0, 0.01, np.arange(77))
     z3 = pseudorandom(77, 3, seed=2)
     ref3 = oracle.kramers_kronig_matrix(np.asarray(g3)) @ z3
-    for shape in (9, 10):
+    for shape in (9, 10, 11):
         os.environ["EPSX_GEMM_SHAPE"] = str(shape)
         try:
             assert rel(epsx.kramers_kronig(g3, z3, ctx), ref3) <= 1e-13, shape
