@@ -52,6 +52,7 @@ struct GemmArgs {
     int64_t n_inst;
     int S_of[NSPLIT_TAB + 1];   // the k split for c active instances (c <= NSPLIT_TAB; else S)
     int64_t units_per_tile;     // the most (split, instance) units any active count gives a tile
+    int gm;                     // > 1: tiles enumerated in groups of gm tile rows (see unit_of_block)
 };
 
 // the number of active instances (gates) and the split that goes with it
@@ -85,8 +86,27 @@ __device__ inline bool unit_of_block(const GemmArgs &p, int &tm, int &tn, int &s
     const int64_t v = (int64_t)(blockIdx.x % NXCD) * per_xcd + slot;
     if (v >= U) return false;
     int64_t r = v;
-    tm = (int)(r % p.mt); r /= p.mt;
-    tn = (int)(r % p.nt); r /= p.nt;
+    if (p.gm > 1) {
+        // groups of gm tile rows: within a group the tile row fastest, then the tile column, so
+        // the contiguous run of units an XCD takes covers about gm rows of tiles and as many
+        // columns as its share allows (A and B panels shared in that XCD's L2)
+        const int64_t per = (int64_t)p.mt * p.nt, q = r % per;
+        r /= per;
+        const int gm = p.gm, full = p.mt / gm * gm;
+        if (q < (int64_t)full * p.nt) {
+            const int64_t gsz = (int64_t)gm * p.nt, gi = q / gsz, w = q % gsz;
+            tm = (int)(gi * gm + w % gm);
+            tn = (int)(w / gm);
+        } else {
+            const int rem = p.mt - full;
+            const int64_t w = q - (int64_t)full * p.nt;
+            tm = (int)(full + w % rem);
+            tn = (int)(w / rem);
+        }
+    } else {
+        tm = (int)(r % p.mt); r /= p.mt;
+        tn = (int)(r % p.nt); r /= p.nt;
+    }
     s = (int)(r % S); r /= S;
     if (!p.gate) { z = r; return true; }
     for (int64_t q = 0; q < p.n_inst; q++)
@@ -980,6 +1000,10 @@ static int launch_shape(epsx_ctx *ctx, int c, GemmArgs p)
     if (pair_shape(c) && !pair_fits(p)) c = 7;   // odd pitches: the one-wave register kernel
     ctx->last_gemm_shape = c;
     const GemmShape &g = SHAPES[c];
+    // the thin 256x32 pair tiles in groups of two tile rows: each XCD's run of units shares two A
+    // panels and about sixteen B panels in its L2 (on the t2g 24³ star product 0.72 GB of HBM
+    // reads per call instead of 0.86)
+    p.gm = (c == 9 || c == 10) ? 2 : 0;
     p.mt = (int)((p.m + g.bm - 1) / g.bm);
     p.nt = (int)((p.n + g.bn - 1) / g.bn);
     const int64_t units = (int64_t)p.mt * p.nt * p.units_per_tile;

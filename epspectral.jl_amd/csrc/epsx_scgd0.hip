@@ -1220,9 +1220,11 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         if (use_lt) {
             // candidates timed on pseudo-random h (overwritten before its first use); the FP64
             // product only when EPSX_STAR_BLAS=1 asks for it
-            hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(ldh * K * nT)), dim3(256), 0, st, hbuf.ptr<double>(),
-                               ldh * K * nT);
-            EPSX_HIP(ctx, hipGetLastError());
+            if (star_blas) {
+                hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(ldh * K * nT)), dim3(256), 0, st,
+                                   hbuf.ptr<double>(), ldh * K * nT);
+                EPSX_HIP(ctx, hipGetLastError());
+            }
             if (star_blas)
                 EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>(), ldh, ldh * K,
                                         use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt), ldG, 0,

@@ -191,3 +191,21 @@ def test_pair_gemm_fragments_and_reduction(WM, WN, WK, TM, TN):
                 idx = (wk * WM * WN + wm + WM * wn) * PER + lane + ((a * TN + b) * 4 + e) * 64
                 assert written[idx] == (wk + h, wm, wn, lane, a, b, e)
         h //= 2
+
+
+@pytest.mark.parametrize("gm", [2, 3])
+def test_grouped_tile_order_covers_every_tile_once(gm):
+    """unit_of_block with gm > 1 (the thin 16-byte-load tiles): the units of an instance, taken in
+    groups of gm tile rows (row fastest, then column; a last smaller group for the remaining rows),
+    cover every (tile row, tile column) exactly once, for tile grids smaller and larger than gm."""
+    def tile(q, mt, nt):
+        full = mt // gm * gm
+        if q < full * nt:
+            gi, w = divmod(q, gm * nt)
+            return gi * gm + w % gm, w // gm
+        rem = mt - full
+        w = q - full * nt
+        return full + w % rem, w // rem
+    for mt, nt in ((5, 25), (10, 25), (4, 7), (1, 3), (3, 3), (2, 5), (7, 1)):
+        seen = [tile(q, mt, nt) for q in range(mt * nt)]
+        assert sorted(seen) == [(a, b) for a in range(mt) for b in range(nt)]
