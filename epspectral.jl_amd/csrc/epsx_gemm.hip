@@ -1159,9 +1159,12 @@ int dgemm(epsx_ctx *ctx, int64_t m, int64_t n, int64_t k, const double *A, int64
         c = atoi(force) % NSHAPES;
     } else if (it != ctx->gemm_tune.end()) {
         c = it->second;
-    } else if (!bt.tune) {
-        c = 1;   // 128x64 until a tunable product of this size has run
-    } else if (!accumulate && !capturing) {
+    } else if (!bt.tune || accumulate || capturing) {
+        // until a tunable product of this size has run: the 16-byte-load kernel where its A fits
+        // (the thin 256x32 tile of 2 x 4 waves for up to 2048 columns, else the 128x128 tile),
+        // else the LDS-staged 128x64 tile
+        c = pair_fits(p) ? (n <= 2048 ? 10 : 11) : 1;
+    } else {
         hipEvent_t e0, e1;
         EPSX_HIP(ctx, hipEventCreate(&e0));
         EPSX_HIP(ctx, hipEventCreate(&e1));
