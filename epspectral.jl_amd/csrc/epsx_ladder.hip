@@ -650,8 +650,9 @@ static int ldot(epsx_ctx *ctx, int64_t n, const double2 *x, const double2 *y, do
 }
 
 // The shift-table product W = J B̃ is a plain GEMM (no epilogue, no batching): it runs in rocBLAS
-// (deterministic: atomics off; BF16 in hipBLASLt) on the context's stream; EPSX_LADDER_BLAS=0 runs
-// the library's own MFMA kernels instead (for comparisons).
+// (deterministic: atomics off; `rocblas_dgemm`, and `rocblas_gemm_ex` from the stored BF16 / FP32
+// operands with FP32 accumulation) on the context's stream; EPSX_LADDER_BLAS=0 runs the library's
+// own MFMA kernels instead (for comparisons), EPSX_LADDER_LT=1 the BF16 product in hipBLASLt.
 static bool use_blas()
 {
     const char *e = getenv("EPSX_LADDER_BLAS");
@@ -791,7 +792,12 @@ static int ladder_apply_batch(epsx_ctx *ctx, BatchWork &bw, int nA, const std::v
         if (native) {
             const char *Bl = static_cast<const char *>(l.Btl.p) + esz * kp * c0 * l.ns;
             bool lt = false;
-            if (blas && bf)
+            // hipBLASLt only when EPSX_LADDER_LT=1 asks for it: the batched solve's product changes
+            // its row count whenever instances converge, and each new size would time hipBLASLt's
+            // candidates again inside the solve (on the t2g 16³ forward-coupling C4 run 12.5 s
+            // against 8.2 s with rocBLAS's gemm_ex, which needs no timing)
+            const char *lte = getenv("EPSX_LADDER_LT");
+            if (blas && bf && lte && lte[0] == '1')
                 EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_BF16, M, nn, kp, bw.JmL.b.p, kp, 0, Bl, kp, 0, bw.W.b.p, M, 0, 1,
                                         &lt));
             if (lt) {
