@@ -156,8 +156,14 @@ def main():
                          "v_mfma_f64_16x16x4f64, autotuned tile shape)")
                 tr = traffic(GEMM_SHAPE_NAMES.get(kid - 2, "?").split()[0], args.nk, nT)
         else:
+            # the kernel the library reports for the timed iteration's product (timing slot 7)
+            kid = tsum.get("star_gemm_kernel", 0)
             peak = BF16_PEAK_TFLOPS if args.precision == "bf16" else FP32_PEAK_TFLOPS
-            kname = f"lpkgemm_kernel (star Fan-Migdal GEMM, {args.precision.upper()} MFMA)"
+            lp = args.precision.upper()
+            kname = {1: f"hipBLASLt {lp} GEMM (star Fan-Migdal product from stored {lp} h and G̃, FP32 accumulation)",
+                     30: f"lpkgemm_kernel (star Fan-Migdal product from stored {lp} operands, {lp} MFMA)",
+                     31: f"lpgemm_kernel (star Fan-Migdal product, FP64 operands rounded to {lp} as staged)"}.get(
+                         kid, f"own FP64 MFMA GEMM {GEMM_SHAPE_NAMES.get(kid - 2, '?')}")
             tr = None
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": tr, "kernel": kname,

@@ -138,7 +138,7 @@ struct epsx_ctx {
     int last_gemm_shape = -1;   // the tile shape the last FP64 dgemm launched (after any fallback)
     // kernel time of the last run by class: 0 Fan-Migdal, 1 Kramers-Kronig, 2 DOS + μ,
     // 3 convergence + copies, 4 coupling reduction (star method), 5 star GEMMs, 6 their instances,
-    // 7 the kernel of the star method's FP64 product (count only)
+    // 7 the kernel of the star method's product in the timed iteration (count only)
     double t_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t n_class[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };

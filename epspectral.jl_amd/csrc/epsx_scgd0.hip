@@ -1357,6 +1357,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                                            bf ? s.Gb.p : s.Gf.p, kp, 0, ImSf.ptr<float>() + sC * h_tidx[z], nw, sC, n);
                     }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
+                    if (events) ctx->n_class[7] = 1;
                     if (!lp_fused(passes[q].prec)) {
                         hipLaunchKernelGGL(widen_kernel, dim3(nblk(nw * ncol), nA), dim3(256), 0, st, nw * ncol, d_tidx,
                                            passes[q].gate, ImSf.ptr<float>(), Cprod);
@@ -1367,6 +1368,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                     bt.sA = kp * nw;
                     EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ncol, kp, bf ? hb.b.p : hf.b.p, kp,
                                        bf ? s.Gb.p : s.Gf.p, kp, Cprod, nw, false, bt));
+                    if (events) ctx->n_class[7] = 30;
                 } else if (star_lt && passes[q].prec == EPSX_PREC_FP64) {
                     const int64_t sA = ldh * K, sC = nw * ncol;
                     EPSX_TRY(fp64_runs([&](int z, int n) {
@@ -1374,10 +1376,11 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                                            ldG, 0, Cprod + sC * h_tidx[z], nw, sC, n);
                     }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
-                    ctx->n_class[7] = 1;
+                    if (events) ctx->n_class[7] = 1;
                 } else {
                     EPSX_TRY(dgemm(ctx, nw, ncol, K, hbuf.ptr<double>(), ldh, Gprod, ldG, Cprod, nw, false, bt));
-                    if (passes[q].prec == EPSX_PREC_FP64) ctx->n_class[7] = 2 + ctx->last_gemm_shape;
+                    // (the staged low-precision products of EPSX_LP_STAGED=1 run lpgemm_kernel)
+                    if (events) ctx->n_class[7] = passes[q].prec == EPSX_PREC_FP64 ? 2 + ctx->last_gemm_shape : 31;
                 }
             }
             // ImΣ per outer state for the products that take it (the fused FP64 pack reads the class

@@ -51,9 +51,10 @@ const char *epsx_last_error(const epsx_ctx *ctx);
  * [5] the star method's main h·G̃ GEMM kernel (inside [0]; without its split-k sum; count =
  * launches), [6] count only: the temperature instances of those launches (of the Fan-Migdal
  * passes [0] with the Einstein factorisation), [7] count only: the kernel that ran the star
- * method's FP64 product (0 none, 1 hipBLASLt, 2 + s the library's own FP64 GEMM with tile shape
- * s; 11 and 12 are the 16-byte-load kernel).  In the batched loop the second iteration is the
- * one timed.
+ * method's product in the timed iteration (0 none, 1 hipBLASLt, 2 + s the library's own FP64 GEMM
+ * with tile shape s (11-13: the 16-byte-load kernel), 30 its low-precision kernel from stored
+ * BF16 / FP32 operands, 31 the one that rounds FP64 operands as it stages them).  In the batched
+ * loop the second iteration is the one timed.
  * After epsx_ladder_run / epsx_apply_ladder_kernel: [0] the shift-table GEMMs W = J·B̃,
  * [1] whole kernel applications (count = applications; a batched application counts once),
  * [2] B̃ builds, [3] count only: the (Ω, T) instances of those applications, [4..7] unused. */
