@@ -47,15 +47,16 @@ __global__ void complex_pack_kernel(int64_t n, const int32_t *__restrict__ tidx,
     Z[o] = make_double2(re[o], im[o]);
 }
 
-// ImΣ (FP64) from the FP32 result of a low-precision library product, for the slots of a pass.
-__global__ void widen_kernel(int64_t n, const int32_t *__restrict__ tidx, const int32_t *__restrict__ gate,
-                             const float *__restrict__ src, double *__restrict__ dst)
+// ImΣ (FP64, column pitch ldd) from the FP32 result of a low-precision library product (column
+// pitch nw), for the slots of a pass: ncol columns per slot.
+__global__ void widen_kernel(int64_t nw, int64_t ncol, int64_t ldd, const int32_t *__restrict__ tidx,
+                             const int32_t *__restrict__ gate, const float *__restrict__ src, double *__restrict__ dst)
 {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int tcol = tidx[blockIdx.y];
-    if (q >= n || (gate && !gate[tcol])) return;
-    const int64_t o = q + n * (int64_t)tcol;
-    dst[o] = (double)src[o];
+    if (q >= nw * ncol || (gate && !gate[tcol])) return;
+    const int64_t w = q % nw, c = q / nw;
+    dst[w + ldd * c + ldd * ncol * (int64_t)tcol] = (double)src[q + nw * ncol * (int64_t)tcol];
 }
 
 // G̃c[:, c] = Σ_{members m of class c, in order} G̃[:, m] / |c|: the multiplet average of
@@ -76,8 +77,9 @@ __global__ void gt_classes_kernel(int64_t K, int64_t Kp, int64_t nc, const int64
     Gtc[q] = b - a > 1 ? sum / (double)(b - a) : sum;
 }
 
-// ImΣ[:, i] = ImΣc[:, class(i)] for the active slots (the members of a multiplet bitwise equal).
-__global__ void cls_scatter_kernel(int64_t nw, int64_t ni, int64_t nc, const int32_t *__restrict__ cls,
+// ImΣ[:, i] = ImΣc[:, class(i)] for the active slots (the members of a multiplet bitwise equal);
+// ImΣc's columns at pitch ldc.
+__global__ void cls_scatter_kernel(int64_t nw, int64_t ldc, int64_t ni, int64_t nc, const int32_t *__restrict__ cls,
                                    const int32_t *__restrict__ tidx, const double *__restrict__ ImSc,
                                    double *__restrict__ ImS, const int32_t *__restrict__ act)
 {
@@ -86,7 +88,7 @@ __global__ void cls_scatter_kernel(int64_t nw, int64_t ni, int64_t nc, const int
     const int tcol = tidx[blockIdx.y];
     if (act && !act[tcol]) return;
     const int64_t w = q % nw, i = q / nw;
-    ImS[q + nw * ni * (int64_t)tcol] = ImSc[w + nw * (int64_t)cls[i] + nw * nc * (int64_t)tcol];
+    ImS[q + nw * ni * (int64_t)tcol] = ImSc[w + ldc * (int64_t)cls[i] + ldc * nc * (int64_t)tcol];
 }
 
 // Pseudo-random values in [-1, 1) (a hash of the index) for timing the library products'
@@ -626,9 +628,10 @@ __global__ __launch_bounds__(256) void err_copy_kernel(int64_t n, const int32_t 
 // (per-block maxima as err_copy_kernel) and Σ ← Σ_new, for the slots of a pass: the pack, the
 // multiplet copy and src/scgd0.jl:326-328 in one pass over Σ.  F: the products' type (FP64, or
 // FP32 for the FP32 level, whose Kramers-Kronig product is the off-diagonal only: diag, the FP64
-// principal-value diagonal, is added here as ReΣ = re + diag[ω] ImΣ).
+// principal-value diagonal, is added here as ReΣ = re + diag[ω] ImΣ).  re's columns at pitch nw, im's
+// at pitch ldi.
 template <typename F>
-__global__ __launch_bounds__(256) void pack_err_kernel(int32_t nw, int32_t ni, int64_t nc,
+__global__ __launch_bounds__(256) void pack_err_kernel(int32_t nw, int32_t ni, int64_t nc, int64_t ldi,
                                                        const int32_t *__restrict__ cls,
                                                        const int32_t *__restrict__ tidx, const F *__restrict__ re,
                                                        const F *__restrict__ im, const double *__restrict__ diag,
@@ -641,12 +644,12 @@ __global__ __launch_bounds__(256) void pack_err_kernel(int32_t nw, int32_t ni, i
     if (gate && !gate[tcol]) return;
     const int32_t n = nw * ni;
     double2 *St = S + (int64_t)n * tcol;
-    const F *rt = re + nw * nc * (int64_t)tcol, *it = im + nw * nc * (int64_t)tcol;
+    const F *rt = re + nw * nc * (int64_t)tcol, *it = im + ldi * nc * (int64_t)tcol;
     double m = 0;
     for (int32_t q = blockIdx.x * 256 + threadIdx.x; q < n; q += (int32_t)gridDim.x * 256) {
         const int32_t i = q / nw, w = q - i * nw;
-        const int64_t o = w + (int64_t)nw * (cls ? cls[i] : i);
-        const double ai = (double)it[o];
+        const int64_t c = cls ? cls[i] : i, o = w + (int64_t)nw * c;
+        const double ai = (double)it[w + ldi * c];
         const double2 a = make_double2(diag ? (double)rt[o] + diag[w] * ai : (double)rt[o], ai), b = St[q];
         const double e = hypot(a.x - b.x, a.y - b.y);
         m = (e > m || e != e) ? e : m;
@@ -1025,6 +1028,8 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
     // the star product's columns: the classes when multiplets are present (G̃c), else every state
     const bool use_cls = method == EPSX_FM_STAR && s.nc > 0;
     const int64_t ncol = use_cls ? s.nc : ni;
+    // the star product's class columns at an even pitch (16-byte loads of Kramers-Kronig's B operand)
+    const int64_t ldi = use_cls ? nw + (nw & 1) : nw, sI = ldi * ncol;
     // Low-precision levels of the star method: h and G̃ stored in BF16 / FP32, k-contiguous with the
     // row pitch kp (nm ni rounded up to a multiple of 8), multiplied by the low-precision MFMA
     // kernel.  EPSX_LP_STAGED=1 rounds the FP64 operands as they are staged instead (for tests).
@@ -1095,7 +1100,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             EPSX_TRY(gt_lp_launch(ctx, EPSX_PREC_FP32, K, ldG, kp, ncol, Gprod, s.Gf.p));
             s.gf_ready = true;
         }
-        if (use_cls) EPSX_TRY(tmp_alloc(ctx, ImSc.b, 8 * (size_t)nw * ncol * nT));
+        if (use_cls) EPSX_TRY(tmp_alloc(ctx, ImSc.b, 8 * (size_t)ldi * ncol * nT));
         if (use_bf) EPSX_TRY(tmp_alloc(ctx, hb.b, 2 * (size_t)kp * nw * nT));
         if (use_f32) EPSX_TRY(tmp_alloc(ctx, hf.b, 4 * (size_t)kp * nw * nT));
         // Kramers-Kronig's off-diagonal at the low-precision levels: KKo transposed once, ImΣ
@@ -1208,11 +1213,11 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             // candidates timed on pseudo-random operands (ImΣ is overwritten before its first use)
             hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(nS)), dim3(256), 0, st, B.ImS.ptr<double>(), nS);
             if (use_cls)
-                hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(nw * ncol * nT)), dim3(256), 0, st,
-                                   ImSc.ptr<double>(), nw * ncol * nT);
+                hipLaunchKernelGGL(fill_rand_kernel<double>, dim3(nblk(ldi * ncol * nT)), dim3(256), 0, st,
+                                   ImSc.ptr<double>(), ldi * ncol * nT);
             EPSX_HIP(ctx, hipGetLastError());
             EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, nw, dptr<double>(s.KK), nw, 0,
-                                    use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), nw, nw * ncol,
+                                    use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), ldi, sI,
                                     ReS.ptr<double>(), nw, nw * ncol, nT, &kk_lt));
         }
     }
@@ -1228,7 +1233,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             if (star_blas)
                 EPSX_TRY(blaslt_prepare(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>(), ldh, ldh * K,
                                         use_cls ? dptr<double>(s.Gtc) : dptr<double>(s.Gt), ldG, 0,
-                                        use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), nw, nw * ncol, nT, &star_lt));
+                                        use_cls ? ImSc.ptr<double>() : B.ImS.ptr<double>(), ldi, sI, nT, &star_lt));
             // the low-precision levels' products from the stored BF16 / FP32 h and G̃ (FP32 result,
             // widened into ImΣ)
             if (lp_native) {
@@ -1341,7 +1346,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                 GemmBatch bt;
                 bt.n_inst = nA;
                 bt.sA = ldh * K;
-                bt.sC = nw * ncol;
+                bt.sC = sI;
                 bt.idx = d_tidx;
                 bt.imap = 4;
                 bt.gate = passes[q].gate;
@@ -1359,26 +1364,26 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
                     if (events) ctx->n_class[7] = 1;
                     if (!lp_fused(passes[q].prec)) {
-                        hipLaunchKernelGGL(widen_kernel, dim3(nblk(nw * ncol), nA), dim3(256), 0, st, nw * ncol, d_tidx,
-                                           passes[q].gate, ImSf.ptr<float>(), Cprod);
+                        hipLaunchKernelGGL(widen_kernel, dim3(nblk(nw * ncol), nA), dim3(256), 0, st, nw, ncol, ldi,
+                                           d_tidx, passes[q].gate, ImSf.ptr<float>(), Cprod);
                         EPSX_HIP(ctx, hipGetLastError());
                     }
                 } else if (lp_native && passes[q].prec != EPSX_PREC_FP64) {
                     const bool bf = passes[q].prec == EPSX_PREC_BF16;
                     bt.sA = kp * nw;
                     EPSX_TRY(lpgemm_tn(ctx, passes[q].prec, nw, ncol, kp, bf ? hb.b.p : hf.b.p, kp,
-                                       bf ? s.Gb.p : s.Gf.p, kp, Cprod, nw, false, bt));
+                                       bf ? s.Gb.p : s.Gf.p, kp, Cprod, ldi, false, bt));
                     if (events) ctx->n_class[7] = 30;
                 } else if (star_lt && passes[q].prec == EPSX_PREC_FP64) {
-                    const int64_t sA = ldh * K, sC = nw * ncol;
+                    const int64_t sA = ldh * K;
                     EPSX_TRY(fp64_runs([&](int z, int n) {
                         return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, K, hbuf.ptr<double>() + sA * z, ldh, sA, Gprod,
-                                           ldG, 0, Cprod + sC * h_tidx[z], nw, sC, n);
+                                           ldG, 0, Cprod + sI * h_tidx[z], ldi, sI, n);
                     }));
                     if (events) EPSX_HIP(ctx, hipEventRecord(ev[6], st));
                     if (events) ctx->n_class[7] = 1;
                 } else {
-                    EPSX_TRY(dgemm(ctx, nw, ncol, K, hbuf.ptr<double>(), ldh, Gprod, ldG, Cprod, nw, false, bt));
+                    EPSX_TRY(dgemm(ctx, nw, ncol, K, hbuf.ptr<double>(), ldh, Gprod, ldG, Cprod, ldi, false, bt));
                     // (the staged low-precision products of EPSX_LP_STAGED=1 run lpgemm_kernel)
                     if (events) ctx->n_class[7] = passes[q].prec == EPSX_PREC_FP64 ? 2 + ctx->last_gemm_shape : 31;
                 }
@@ -1388,7 +1393,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             if (use_cls) {
                 for (int q = 0; q < (per_pass ? npass : 1); q++) {
                     if ((kk_fuse && passes[q].prec == EPSX_PREC_FP64) || lp_fused(passes[q].prec)) continue;
-                    hipLaunchKernelGGL(cls_scatter_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw, ni, s.nc,
+                    hipLaunchKernelGGL(cls_scatter_kernel, dim3(nblk(nw * ni), nA), dim3(256), 0, st, nw, ldi, ni, s.nc,
                                        dptr<int32_t>(s.cls), d_tidx, ImSc.ptr<double>(), B.ImS.ptr<double>(),
                                        per_pass ? passes[q].gate : d_act);
                     EPSX_HIP(ctx, hipGetLastError());
@@ -1465,23 +1470,24 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                 if (kk_lt) {
                     EPSX_TRY(fp64_runs([&](int z, int n) {
                         const int64_t t0 = h_tidx[z];
-                        return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, nw, dptr<double>(s.KK), nw, 0, Cim + sC * t0,
-                                           nw, sC, ReS.ptr<double>() + sC * t0, nw, sC, n);
+                        return blaslt_gemm(ctx, EPSX_PREC_FP64, nw, ncol, nw, dptr<double>(s.KK), nw, 0, Cim + sI * t0,
+                                           ldi, sI, ReS.ptr<double>() + sC * t0, nw, sC, n);
                     }));
                 } else {
                     GemmBatch bk;
                     bk.n_inst = nA;
-                    bk.sB = bk.sC = sC;
+                    bk.sB = sI;
+                    bk.sC = sC;
                     bk.idx = d_tidx;
                     bk.imap = 2 | 4;
                     bk.gate = passes[q].gate;
                     bk.tune = passes[q].tune;
-                    EPSX_TRY(dgemm(ctx, nw, ncol, nw, dptr<double>(s.KKp), nw + (nw & 1), Cim, nw, ReS.ptr<double>(), nw,
+                    EPSX_TRY(dgemm(ctx, nw, ncol, nw, dptr<double>(s.KKp), nw + (nw & 1), Cim, ldi, ReS.ptr<double>(), nw,
                                    false, bk));
                 }
                 if (kk_fuse)
                     hipLaunchKernelGGL(pack_err_kernel<double>, dim3(nb_err, nA), dim3(256), 0, st, (int32_t)nw,
-                                       (int32_t)ni, ncol, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx,
+                                       (int32_t)ni, ncol, ldi, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx,
                                        ReS.ptr<double>(), Cim, (const double *)nullptr, B.S.ptr<double2>(),
                                        B.bmax.ptr<double>(), passes[q].gate);
                 else
@@ -1500,7 +1506,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                                        nw, sC, ReSf.ptr<float>() + sC * t0, nw, sC, n);
                 }));
                 hipLaunchKernelGGL(pack_err_kernel<float>, dim3(nb_err, nA), dim3(256), 0, st, (int32_t)nw, (int32_t)ni,
-                                   ncol, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx, ReSf.ptr<float>(),
+                                   ncol, nw, use_cls ? dptr<int32_t>(s.cls) : nullptr, d_tidx, ReSf.ptr<float>(),
                                    ImSf.ptr<float>(), dptr<double>(s.KKd), B.S.ptr<double2>(), B.bmax.ptr<double>(),
                                    passes[q].gate);
                 EPSX_HIP(ctx, hipGetLastError());
