@@ -313,32 +313,55 @@ def test_loop_t2g_small_vs_oracle(ctx):
 
 
 def test_star_product_blaslt_matches_own_kernel(ctx):
-    """The star method's FP64 product in hipBLASLt (the default) against the library's MFMA kernel
-    (EPSX_STAR_BLAS=0): batched runs of the Holstein chain at two temperatures that freeze at
+    """The star method's FP64 products in hipBLASLt (EPSX_STAR_BLAS=1) against the library's own
+    MFMA kernel (the default): batched runs of the Holstein chain at two temperatures that freeze at
     different iterations (the replayed graph captured again for one temperature), same iterations
-    and Σ to 1e-12."""
+    and Σ to 1e-12; and the t2g model at nk = 8 (multiplets: the product over the class columns,
+    stored at an even pitch) for three iterations, Σ to 1e-12."""
     s = holstein(T=(0.1, 0.2))
     o = occ([0.1, 0.2])
     kw = dict(η_min=0.0, η_init=0.05, maxiter=30, tol=1e-6, batched=True, method="star")
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
-        lt = epsx.loop_scgd0(s, o, **kw)
-        os.environ["EPSX_STAR_BLAS"] = "0"
+        own = epsx.loop_scgd0(s, o, **kw)
+        os.environ["EPSX_STAR_BLAS"] = "1"
         try:
-            own = epsx.loop_scgd0(s, o, **kw)
+            lt = epsx.loop_scgd0(s, o, **kw)
         finally:
             del os.environ["EPSX_STAR_BLAS"]
-    assert lt["converged"].all() and own["converged"].all()
+        # EPSX_STAR_BLAS=0: no library product at all (Kramers-Kronig in the GEMM's complex epilogue)
+        os.environ["EPSX_STAR_BLAS"] = "0"
+        try:
+            ep = epsx.loop_scgd0(s, o, **kw)
+        finally:
+            del os.environ["EPSX_STAR_BLAS"]
+    assert lt["converged"].all() and own["converged"].all() and ep["converged"].all()
     fr = lt["history"]["iter_frozen"]
     assert fr[0] != fr[1] and np.array_equal(fr, own["history"]["iter_frozen"])
-    assert rel(lt["Σ"], own["Σ"]) <= 1e-12
-    assert np.all(np.abs(lt["μ"] - own["μ"]) <= 1e-12)
+    assert np.array_equal(fr, ep["history"]["iter_frozen"])
+    for r in (lt, ep):
+        assert rel(r["Σ"], own["Σ"]) <= 1e-12
+        assert np.all(np.abs(r["μ"] - own["μ"]) <= 1e-12)
+    s2, meta = t2g_setup(nk=8, Tlist_K=(100.0, 300.0), grid_windows=((0.3, 4e-3), (1.2, 16e-3), (3.0, 64e-3)))
+    o2 = epsx.Occupation(Tlist=meta["Tlist"], nlist=0.0, nelec=1.0)
+    kw2 = dict(η_min=0.0, η_init=5e-2, maxiter=3, tol=0.0, ω_acoustic=1e-6, batched=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dev = epsx.Scgd0Device(s2, ctx)
+        own2 = dev.loop(o2, method="star", **kw2)
+        os.environ["EPSX_STAR_BLAS"] = "1"
+        try:
+            lt2 = dev.loop(o2, method="star", **kw2)
+        finally:
+            del os.environ["EPSX_STAR_BLAS"]
+    assert rel(lt2["Σ"], own2["Σ"]) <= 1e-12
+    assert np.all(np.abs(lt2["μ"] - own2["μ"]) <= 1e-12)
 
 
 def test_star_product_blaslt_batch_split(ctx):
-    """A library product over five temperatures split into chunks of at most two instances
-    (EPSX_LT_CHUNK=2: calls of 2 + 2 + 1, each chunk's pointers advanced by its offset) against the
-    library's MFMA kernel: same iterations, Σ and μ to 1e-12."""
+    """The hipBLASLt products (EPSX_STAR_BLAS=1) over five temperatures split into chunks of at most
+    two instances (EPSX_LT_CHUNK=2: calls of 2 + 2 + 1, each chunk's pointers advanced by its offset)
+    against the library's own MFMA kernel: same iterations, Σ and μ to 1e-12."""
     T = (0.08, 0.1, 0.13, 0.16, 0.2)
     s = holstein(T=T)
     o = occ(list(T))
@@ -346,15 +369,13 @@ def test_star_product_blaslt_batch_split(ctx):
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         os.environ["EPSX_LT_CHUNK"] = "2"
+        os.environ["EPSX_STAR_BLAS"] = "1"
         try:
             lt = epsx.loop_scgd0(s, o, **kw)
         finally:
             del os.environ["EPSX_LT_CHUNK"]
-        os.environ["EPSX_STAR_BLAS"] = "0"
-        try:
-            own = epsx.loop_scgd0(s, o, **kw)
-        finally:
             del os.environ["EPSX_STAR_BLAS"]
+        own = epsx.loop_scgd0(s, o, **kw)
     assert lt["converged"].all() and own["converged"].all()
     assert np.array_equal(lt["history"]["iter_frozen"], own["history"]["iter_frozen"])
     assert rel(lt["Σ"], own["Σ"]) <= 1e-12
