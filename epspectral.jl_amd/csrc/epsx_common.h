@@ -120,6 +120,12 @@ struct epsx_ctx {
     size_t pin_bytes = 0;
     hipStream_t side = nullptr;   // a second stream (default priority) for copies beside the work
     void *pin_side = nullptr;     // pinned host memory the side stream copies into
+    // the replayed loop's host slots, kept for the next run: mask words and err (mapped, coherent:
+    // the convergence kernel writes them) and the staged precision levels
+    void *pin_slots = nullptr;
+    size_t pin_slots_bytes = 0;
+    void *pin_prec = nullptr;
+    size_t pin_prec_bytes = 0;
     size_t pin_side_bytes = 0;
     // device buffers a run released, by size, for the next run's temporaries (work on the
     // context's one stream is ordered, so a buffer may be handed out while a released user's

@@ -240,6 +240,8 @@ void epsx_close(epsx_ctx *ctx)
         (void)hipStreamDestroy(ctx->side);
     }
     if (ctx->pin_side) (void)hipHostFree(ctx->pin_side);
+    if (ctx->pin_slots) (void)hipHostFree(ctx->pin_slots);
+    if (ctx->pin_prec) (void)hipHostFree(ctx->pin_prec);
     for (auto *b : ctx->owned) epsx::dev_free(*b);
     epsx::pool_clear(ctx);
     (void)hipStreamDestroy(ctx->stream);

@@ -1740,17 +1740,27 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         // (two slots: an iteration may be read back while the next one runs)
         // (fine-grained: the convergence kernel writes them, the host reads them after the
         // iteration's event)
-        int32_t *h_as = nullptr, *d_as = nullptr;
-        EPSX_HIP(ctx, hipHostMalloc((void **)&h_as, 16 * nT, hipHostMallocMapped | hipHostMallocCoherent));
-        double *h_es = nullptr, *d_es = nullptr;   // and err, for the decision to run ahead
-        EPSX_HIP(ctx, hipHostMalloc((void **)&h_es, 16 * nT, hipHostMallocMapped | hipHostMallocCoherent));
-        struct PinGuardE { double *p; ~PinGuardE() { (void)hipHostFree(p); } } pge{h_es};
-        int32_t *h_pr = nullptr;   // the staged levels of a mixed-precision run
-        EPSX_HIP(ctx, hipHostMalloc((void **)&h_pr, 8 * nT, hipHostMallocDefault));
-        struct PinGuardP { int32_t *p; ~PinGuardP() { (void)hipHostFree(p); } } pgp{h_pr};
+        // (kept in the context for the next run: pinned allocations cost tens of µs each; the
+        // previous run's work on the stream has ended, so no kernel still writes them)
+        if (ctx->pin_slots_bytes < 32 * (size_t)nT) {
+            if (ctx->pin_slots) (void)hipHostFree(ctx->pin_slots);
+            ctx->pin_slots = nullptr;
+            ctx->pin_slots_bytes = 0;
+            EPSX_HIP(ctx, hipHostMalloc(&ctx->pin_slots, 32 * nT, hipHostMallocMapped | hipHostMallocCoherent));
+            ctx->pin_slots_bytes = 32 * nT;
+        }
+        if (ctx->pin_prec_bytes < 8 * (size_t)nT) {
+            if (ctx->pin_prec) (void)hipHostFree(ctx->pin_prec);
+            ctx->pin_prec = nullptr;
+            ctx->pin_prec_bytes = 0;
+            EPSX_HIP(ctx, hipHostMalloc(&ctx->pin_prec, 8 * nT, hipHostMallocDefault));
+            ctx->pin_prec_bytes = 8 * nT;
+        }
+        int32_t *h_as = static_cast<int32_t *>(ctx->pin_slots), *d_as = nullptr;
+        double *h_es = reinterpret_cast<double *>(static_cast<char *>(ctx->pin_slots) + 16 * nT), *d_es = nullptr;   // and err, for the decision to run ahead
+        int32_t *h_pr = static_cast<int32_t *>(ctx->pin_prec);   // the staged levels of a mixed-precision run
         EPSX_HIP(ctx, hipHostGetDevicePointer((void **)&d_es, h_es, 0));
         std::fill(h_es, h_es + 2 * nT, INFINITY);
-        struct PinGuard { int32_t *p; ~PinGuard() { (void)hipHostFree(p); } } pg{h_as};
         EPSX_HIP(ctx, hipHostGetDevicePointer((void **)&d_as, h_as, 0));
         hipEvent_t ev_read[2] = {nullptr, nullptr};
         struct ReadGuard { hipEvent_t *e; ~ReadGuard() { for (int k = 0; k < 2; k++) if (e[k]) (void)hipEventDestroy(e[k]); } } rg{ev_read};
