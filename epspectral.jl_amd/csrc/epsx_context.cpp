@@ -242,6 +242,7 @@ void epsx_close(epsx_ctx *ctx)
     if (ctx->pin_side) (void)hipHostFree(ctx->pin_side);
     if (ctx->pin_slots) (void)hipHostFree(ctx->pin_slots);
     if (ctx->pin_prec) (void)hipHostFree(ctx->pin_prec);
+    if (ctx->pin_book) (void)hipHostFree(ctx->pin_book);
     for (auto *b : ctx->owned) epsx::dev_free(*b);
     epsx::pool_clear(ctx);
     (void)hipStreamDestroy(ctx->stream);

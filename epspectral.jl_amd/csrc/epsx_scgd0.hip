@@ -1731,7 +1731,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         // the host reading back only the mask, the status words and err.
         std::vector<int32_t> all(nT);
         for (int64_t t = 0; t < nT; t++) all[t] = (int32_t)t;
-        Tmp d_act, d_frozen, d_it, d_he, d_hm, d_ha;
+        Tmp d_act, d_book;
         // the mask and the μ status words in one allocation; the convergence kernel reports both
         // into pinned host slots every iteration
         std::vector<int32_t> act_status(2 * nT, 0);
@@ -1765,17 +1765,31 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         hipEvent_t ev_read[2] = {nullptr, nullptr};
         struct ReadGuard { hipEvent_t *e; ~ReadGuard() { for (int k = 0; k < 2; k++) if (e[k]) (void)hipEventDestroy(e[k]); } } rg{ev_read};
         for (auto &e : ev_read) EPSX_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        std::vector<int64_t> zeros64(nT, 0);
-        EPSX_TRY(tmp_upload(ctx, d_frozen, zeros64.data(), 8 * nT));
-        const int32_t zero = 0;
-        EPSX_TRY(tmp_upload(ctx, d_it, &zero, 4));
-        EPSX_TRY(tmp_upload(ctx, d_he, out->hist_err, 8 * nT * maxiter));
-        EPSX_TRY(tmp_upload(ctx, d_hm, out->hist_mu, 8 * nT * maxiter));
-        EPSX_TRY(tmp_upload(ctx, d_ha, out->hist_minA, 8 * nT * maxiter));
+        // the bookkeeping in one device buffer, uploaded (and at the end read back with μ) through
+        // one pinned staging buffer kept in the context: iter_frozen [nT] (zero), the iteration
+        // counter (zero, in 8 bytes), then the err, μ and min ∫A histories [nT, maxiter]
+        const size_t nh = (size_t)nT * maxiter, book_bytes = 8 * ((size_t)nT + 1 + 3 * nh);
+        if (ctx->pin_book_bytes < book_bytes + 8 * (size_t)nT) {
+            if (ctx->pin_book) (void)hipHostFree(ctx->pin_book);
+            ctx->pin_book = nullptr;
+            ctx->pin_book_bytes = 0;
+            // at least 256 KB, so a short first run (a prime call) leaves room for the long ones
+            const size_t want = std::max(book_bytes + 8 * (size_t)nT, (size_t)256 << 10);
+            EPSX_HIP(ctx, hipHostMalloc(&ctx->pin_book, want, hipHostMallocDefault));
+            ctx->pin_book_bytes = want;
+        }
+        char *const hbook = static_cast<char *>(ctx->pin_book);
+        std::memset(hbook, 0, 8 * ((size_t)nT + 1));
+        std::memcpy(hbook + 8 * (nT + 1), out->hist_err, 8 * nh);
+        std::memcpy(hbook + 8 * (nT + 1 + nh), out->hist_mu, 8 * nh);
+        std::memcpy(hbook + 8 * (nT + 1 + 2 * nh), out->hist_minA, 8 * nh);
+        EPSX_TRY(tmp_alloc(ctx, d_book.b, book_bytes));
+        EPSX_HIP(ctx, hipMemcpyAsync(d_book.b.p, hbook, book_bytes, hipMemcpyHostToDevice, st));
+        char *const dbook = d_book.ptr<char>();
         EPSX_HIP(ctx, hipMemcpyAsync(B.tidx.b.p, all.data(), 4 * nT, hipMemcpyHostToDevice, st));
         GraphBook book;
-        book.act = d_act.ptr<int32_t>(); book.status = d_act.ptr<int32_t>() + nT; book.host = d_as; book.host_err = d_es; book.frozen = d_frozen.ptr<int64_t>(); book.it = d_it.ptr<int32_t>();
-        book.hist_err = d_he.ptr<double>(); book.hist_mu = d_hm.ptr<double>(); book.hist_minA = d_ha.ptr<double>();
+        book.act = d_act.ptr<int32_t>(); book.status = d_act.ptr<int32_t>() + nT; book.host = d_as; book.host_err = d_es; book.frozen = reinterpret_cast<int64_t *>(dbook); book.it = reinterpret_cast<int32_t *>(dbook + 8 * nT);
+        book.hist_err = reinterpret_cast<double *>(dbook + 8 * (nT + 1)); book.hist_mu = book.hist_err + nh; book.hist_minA = book.hist_mu + nh;
         book.mu = B.mu.ptr<double>(); book.minA = B.minA.ptr<double>();
         book.nT = nT; book.maxiter = maxiter; book.tol = par->tol;
         if (split) book.prec = d_prec.ptr<int32_t>();
@@ -1935,12 +1949,14 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
             if (!any) break;
         }
         if (rc == EPSX_OK) {
-            EPSX_HIP(ctx, hipMemcpyAsync(out->hist_err, book.hist_err, 8 * nT * maxiter, hipMemcpyDeviceToHost, st));
-            EPSX_HIP(ctx, hipMemcpyAsync(out->hist_mu, book.hist_mu, 8 * nT * maxiter, hipMemcpyDeviceToHost, st));
-            EPSX_HIP(ctx, hipMemcpyAsync(out->hist_minA, book.hist_minA, 8 * nT * maxiter, hipMemcpyDeviceToHost, st));
-            EPSX_HIP(ctx, hipMemcpyAsync(out->iter_frozen, book.frozen, 8 * nT, hipMemcpyDeviceToHost, st));
-            EPSX_HIP(ctx, hipMemcpyAsync(mu_host.data(), B.mu.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(hbook, dbook, book_bytes, hipMemcpyDeviceToHost, st));
+            EPSX_HIP(ctx, hipMemcpyAsync(hbook + book_bytes, B.mu.b.p, 8 * nT, hipMemcpyDeviceToHost, st));
             EPSX_HIP(ctx, hipStreamSynchronize(st));
+            std::memcpy(out->iter_frozen, hbook, 8 * nT);
+            std::memcpy(out->hist_err, hbook + 8 * (nT + 1), 8 * nh);
+            std::memcpy(out->hist_mu, hbook + 8 * (nT + 1 + nh), 8 * nh);
+            std::memcpy(out->hist_minA, hbook + 8 * (nT + 1 + 2 * nh), 8 * nh);
+            std::memcpy(mu_host.data(), hbook + book_bytes, 8 * nT);
             // history beyond the last iteration is NaN, as in the reference
             for (int64_t q = nT * out->niter; q < nT * maxiter; q++)
                 out->hist_err[q] = out->hist_mu[q] = out->hist_minA[q] = NAN;
