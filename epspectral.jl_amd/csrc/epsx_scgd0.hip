@@ -1829,6 +1829,20 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         };
         int64_t launched = 0;
         double t_prev = now_s();
+        // the replayed iteration captured (and instantiated) into exec: nothing runs while the stream
+        // captures, and work enqueued before the capture runs on
+        auto capture = [&]() -> int {
+            graph_levels = present_levels();
+            graph_active = active;
+            graph_prec = prec_t;
+            EPSX_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, false);
+            hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (r != EPSX_OK) return r;
+            if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture", __FILE__, __LINE__);
+            EPSX_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            return EPSX_OK;
+        };
         auto launch = [&](int64_t iter) -> int {
             if (iter <= 2) {
                 EPSX_TRY(enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, timed(iter)));
@@ -1845,17 +1859,7 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
                     exec = nullptr;
                     graph = nullptr;
                 }
-                if (!exec) {
-                    graph_levels = present_levels();
-                    graph_active = active;
-                    graph_prec = prec_t;
-                    EPSX_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                    int r = enqueue((int)nT, B.tidx.ptr<int32_t>(), all, book.act, book, false);
-                    hipError_t ec = hipStreamEndCapture(st, &graph);
-                    if (r != EPSX_OK) return r;
-                    if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture", __FILE__, __LINE__);
-                    EPSX_HIP(ctx, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                }
+                if (!exec) EPSX_TRY(capture());
                 EPSX_HIP(ctx, hipGraphLaunch(exec, st));
             }
             EPSX_HIP(ctx, hipEventRecord(ev_read[iter & 1], st));
@@ -1875,6 +1879,10 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
         for (int64_t iter = 1; iter <= maxiter; iter++) {
             if (launched < iter) EPSX_TRY(launch(iter));
             if (iter >= 3 && iter < maxiter && !near_tol(iter - 1)) EPSX_TRY(launch(iter + 1));
+            // the replayed iteration captured while the second, eager one runs on the GPU (the host
+            // would otherwise capture with the GPU idle before the third); captured again before
+            // its first replay if the levels or, with library products, the active set change
+            if (iter == 2 && maxiter >= 3 && !exec) EPSX_TRY(capture());
             EPSX_HIP(ctx, hipEventSynchronize(ev_read[iter & 1]));
             const int32_t *slot = h_as + 2 * nT * (iter & 1);
             std::copy(slot, slot + nT, active.begin());
