@@ -128,6 +128,7 @@ struct epsx_ctx {
     size_t pin_prec_bytes = 0;
     void *pin_book = nullptr;     // the loop's bookkeeping (frozen, histories, μ) staged both ways
     size_t pin_book_bytes = 0;
+    hipEvent_t run_ev[8] = {};    // the scGD0 run's timing events (created once, kept)
     size_t pin_side_bytes = 0;
     // device buffers a run released, by size, for the next run's temporaries (work on the
     // context's one stream is ordered, so a buffer may be handed out while a released user's

@@ -243,6 +243,8 @@ void epsx_close(epsx_ctx *ctx)
     if (ctx->pin_slots) (void)hipHostFree(ctx->pin_slots);
     if (ctx->pin_prec) (void)hipHostFree(ctx->pin_prec);
     if (ctx->pin_book) (void)hipHostFree(ctx->pin_book);
+    for (auto &e : ctx->run_ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto *b : ctx->owned) epsx::dev_free(*b);
     epsx::pool_clear(ctx);
     (void)hipStreamDestroy(ctx->stream);

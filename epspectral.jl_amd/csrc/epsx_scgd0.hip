@@ -1156,9 +1156,10 @@ static int scgd0_run_impl(epsx_ctx *ctx, const epsx_scgd0_params *par, epsx_scgd
 
     for (int64_t q = 0; q < nT * maxiter; q++) out->hist_err[q] = out->hist_mu[q] = out->hist_minA[q] = NAN;
     for (int64_t t = 0; t < nT; t++) out->iter_frozen[t] = 0;
-    hipEvent_t ev[8];
-    for (auto &e : ev) EPSX_HIP(ctx, hipEventCreate(&e));
-    struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int k = 0; k < 8; k++) (void)hipEventDestroy(e[k]); } } eg{ev};
+    // (created once per context: each creation costs microseconds of host time per run)
+    for (auto &e : ctx->run_ev)
+        if (!e) EPSX_HIP(ctx, hipEventCreate(&e));
+    hipEvent_t *const ev = ctx->run_ev;
     out->niter = 0;
     out->fm_method_used = method;
     out->n_low_precision = 0;
